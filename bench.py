@@ -1,0 +1,176 @@
+"""Benchmark: Mpaths/s of the per-sample radiance path on one MI355X.
+
+Workload (one step = one full render): the 100,820-triangle displaced
+geodesic icosphere on a ground plane with 8 quad lights, 1920 x 1080 at
+64 spp, path depth 6, balance-heuristic MIS of one light and one BSDF sample
+per vertex, FP32 shading with FP64 per-pixel accumulation.  Scene, BVH and
+accumulator are resident in HBM before the timed region.
+
+Prints one JSON line (see the driver contract in the repository notes):
+metric/value/unit, roofline of the dominant kernel (closest-hit traversal)
+measured with HIP events on the render stream over the timed region, and a
+CPU baseline (the FP64 oracle on the host cores, a bounded row sample).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+NODE_BYTES = 128      # one 4-wide node (chk_bvh.h Node4)
+TRI_BYTES = 48        # one leaf triangle record (three float4)
+RAY_BYTES = 32 + 16   # chk_ray in + hit record out
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--spp", type=int, default=64)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--depth", type=int, default=6)
+    ap.add_argument("--freq", type=int, default=71, help="geodesic frequency (71 -> 100,820 triangles)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-row-step", type=int, default=17)
+    ap.add_argument("--cpu-spp", type=int, default=16)
+    ap.add_argument("--traffic", type=str, default=None,
+                    help="JSON file with measured HBM bytes per closest-hit launch")
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(json.dumps({"error": "this build measures one GPU; --gpus above 1 is not supported "
+                                   "(a render shards by pixel or sample range, see DESIGN.md)"}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from chakumrender_amd import abi, host, scenes
+
+    t0 = time.time()
+    sc = scenes.icosphere_scene(freq=args.freq, width=args.width, height=args.height)
+    ctx = abi.Context(0)
+    ds = abi.DeviceScene(ctx, sc)
+    info = ds.info()
+    params = dict(spp=args.spp, max_depth=args.depth, seed=0x5EED0002)
+    accum = torch.zeros((args.height, args.width, 3), dtype=torch.float64, device="cuda")
+    print(f"# scene {sc.n_tris} tris, {info['n_nodes']} nodes, stack {info['stack_size']}, "
+          f"build {info['build_ms']:.0f} ms, setup {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
+
+    # work model: node visits / triangle tests of this exact workload (untimed)
+    _, _, cnt = host.render(ds, params, accum=accum, count=True)
+    print(f"# counted: {cnt['rays_extend']} extend rays, {cnt['rays_shadow']} shadow rays, "
+          f"{cnt['node_visits']} node visits, {cnt['tri_tests']} tri tests", file=sys.stderr, flush=True)
+
+    prof = dict(params, flags=abi.RENDER_PROFILE)
+    for _ in range(args.warmup):
+        host.render(ds, params, accum=accum, stats=False)
+    torch.cuda.synchronize()
+    ext_ms = shade_ms = shadow_ms = other_ms = 0.0
+    launches = 0
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        _, _, st = host.render(ds, prof, accum=accum)
+        ext_ms += st["extend_ms"]
+        shade_ms += st["shade_ms"]
+        shadow_ms += st["shadow_ms"]
+        other_ms += st["other_ms"]
+        launches += st["extend_launches"]
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+
+    paths = args.width * args.height * args.spp
+    ms_per_step = elapsed * 1e3 / args.steps
+    value = paths * args.steps / elapsed / 1e6
+    rays_total = cnt["rays_extend"] + cnt["rays_shadow"]
+    mrays = rays_total * args.steps / elapsed / 1e6
+
+    # roofline of the closest-hit kernel: algorithmic bytes per launch over
+    # its average launch duration (HIP events on the render stream)
+    per_launch_launches = cnt and st["extend_launches"]
+    alg_bytes_total = (cnt["node_visits"] * NODE_BYTES + cnt["tri_tests"] * TRI_BYTES
+                       + cnt["rays_extend"] * RAY_BYTES)
+    alg_bytes_launch = alg_bytes_total / per_launch_launches
+    avg_launch_s = ext_ms / 1e3 / launches
+    achieved = alg_bytes_launch / avg_launch_s / 1e9
+    traffic = None
+    if args.traffic and Path(args.traffic).exists():
+        traffic = json.loads(Path(args.traffic).read_text()).get("bytes_per_launch")
+
+    line = {
+        "metric": "Mpaths/s",
+        "value": round(value, 3),
+        "unit": "Mpaths/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"icosphere {sc.n_tris} tris (freq {args.freq}, 3-octave noise 0.1), ground, "
+                        f"8 quad lights; {args.width}x{args.height} {args.spp} spp, depth {args.depth}, "
+                        "balance-heuristic MIS (1 light + 1 BSDF sample/vertex), FP64 accumulation",
+            "width": args.width, "height": args.height, "spp": args.spp, "max_depth": args.depth,
+            "n_tris": sc.n_tris, "paths_per_step": paths,
+        },
+        "mrays_per_s": round(mrays, 3),
+        "rays_per_path": round(rays_total / paths, 4),
+        "kernel_ms_per_step": {"closest_hit": round(ext_ms / args.steps, 3),
+                               "shade": round(shade_ms / args.steps, 3),
+                               "any_hit": round(shadow_ms / args.steps, 3),
+                               "camera_accumulate": round(other_ms / args.steps, 3)},
+        "roofline": {
+            "kernel": "k_trace<closest> (chakumrender_amd/csrc/chk_trace.hip)",
+            "bound": "hbm",
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5),
+            "traffic": traffic,
+            "bytes_per_launch": int(alg_bytes_launch),
+            "avg_launch_ms": round(avg_launch_s * 1e3, 4),
+            "work_model": f"{NODE_BYTES} B/node visit + {TRI_BYTES} B/triangle test + {RAY_BYTES} B/ray",
+        },
+    }
+
+    if not args.no_cpu_baseline:
+        import oracle
+
+        threads = min(16, os.cpu_count() or 1)
+        rows = (0, args.height)
+        t1 = time.perf_counter()
+        _, _, cpaths, crays = oracle.render(sc, dict(params, spp=args.cpu_spp), rows=rows,
+                                            row_step=args.cpu_row_step, threads=threads)
+        dt = time.perf_counter() - t1
+        n_rows = len(range(0, args.height, args.cpu_row_step))
+        line["cpu_baseline"] = {
+            "value": round(cpaths / dt / 1e6, 4),
+            "unit": "Mpaths/s",
+            "cores": threads,
+            "kind": "port",
+            "sample": f"FP64 oracle, every {args.cpu_row_step}th row ({n_rows} rows x {args.width}) "
+                      f"at {args.cpu_spp} spp = {cpaths} paths, {crays} rays, {dt:.1f} s",
+        }
+        line["speedup_vs_cpu"] = round(value / line["cpu_baseline"]["value"], 2)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

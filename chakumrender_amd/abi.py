@@ -1,0 +1,209 @@
+"""ctypes binding of libchakum.so (include/chakum_c.h).
+
+The library is the product: every call here goes straight to the HIP
+implementation.  Loading fails loudly when libchakum.so is missing; there is
+no CPU fallback.  Device buffers are passed as raw pointers (torch tensors
+on the GPU are used only as allocations: `tensor.data_ptr()`).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "libchakum.so"
+LTC_TABLE = PKG / "data" / "ltc_ggx_64.bin"
+
+
+class ChkError(RuntimeError):
+    pass
+
+
+class chk_material(C.Structure):
+    _fields_ = [("brdf", C.c_uint32), ("light", C.c_uint32), ("ks", C.c_float), ("alpha", C.c_float),
+                ("phong_exp", C.c_float), ("int_ior", C.c_float), ("ext_ior", C.c_float),
+                ("reflectance", C.c_float), ("color", C.c_float * 3), ("radiance", C.c_float * 3)]
+
+
+class chk_mesh_light(C.Structure):
+    _fields_ = [("first_tri", C.c_uint32), ("n_tris", C.c_uint32)]
+
+
+class chk_camera(C.Structure):
+    _fields_ = [("pos", C.c_float * 3), ("dir", C.c_float * 3), ("up", C.c_float * 3),
+                ("fovx_deg", C.c_float), ("fovy_deg", C.c_float), ("width", C.c_uint32),
+                ("height", C.c_uint32)]
+
+
+class chk_render_params(C.Structure):
+    _fields_ = [("spp", C.c_uint32), ("sample_offset", C.c_uint32), ("max_depth", C.c_uint32),
+                ("mis_mode", C.c_uint32), ("jitter_mode", C.c_uint32), ("seed_lo", C.c_uint32),
+                ("seed_hi", C.c_uint32), ("flags", C.c_uint32)]
+
+
+class chk_scene_info(C.Structure):
+    _fields_ = [("n_tris", C.c_uint32), ("n_tris_bvh", C.c_uint32), ("n_nodes", C.c_uint32),
+                ("n_leaves", C.c_uint32), ("max_depth", C.c_uint32), ("stack_bound", C.c_uint32),
+                ("stack_size", C.c_uint32), ("n_lights", C.c_uint32), ("device_bytes", C.c_uint64),
+                ("sah_cost", C.c_double), ("build_ms", C.c_double)]
+
+
+class chk_render_stats(C.Structure):
+    _fields_ = [("paths", C.c_uint64), ("rays_extend", C.c_uint64), ("rays_shadow", C.c_uint64),
+                ("node_visits", C.c_uint64), ("tri_tests", C.c_uint64),
+                ("shadow_node_visits", C.c_uint64), ("shadow_tri_tests", C.c_uint64),
+                ("chunks", C.c_uint32), ("extend_launches", C.c_uint32), ("extend_ms", C.c_double),
+                ("shadow_ms", C.c_double), ("shade_ms", C.c_double), ("other_ms", C.c_double)]
+
+
+assert C.sizeof(chk_material) == 56
+assert C.sizeof(chk_render_params) == 32
+
+# flags (chakum_types.h, chakum_c.h)
+RENDER_PHYSICAL_EMITTER = 0x1
+RENDER_SUMSQ = 0x2
+RENDER_PROFILE = 0x100
+JITTER_BOX, JITTER_REFERENCE = 0, 1
+MIS_BALANCE, MIS_LEARNED = 0, 1
+
+# Every exported symbol of include/chakum_c.h: name -> (restype, argtypes)
+_vp = C.c_void_p
+_u32 = C.c_uint32
+SIGNATURES = {
+    "chk_status_string": (C.c_char_p, [C.c_int]),
+    "chk_last_error": (C.c_char_p, []),
+    "chk_version": (C.c_char_p, []),
+    "chk_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
+    "chk_destroy": (C.c_int, [_vp]),
+    "chk_set_ltc_table": (C.c_int, [_vp, _vp, C.c_size_t]),
+    "chk_scene_create": (C.c_int, [_vp, _vp, _u32, _vp, _vp, _u32, _vp, _u32, C.POINTER(_vp)]),
+    "chk_scene_destroy": (C.c_int, [_vp]),
+    "chk_scene_get_info": (C.c_int, [_vp, C.POINTER(chk_scene_info)]),
+    "chk_trace_closest": (C.c_int, [_vp, _vp, _u32, _vp, _vp]),
+    "chk_trace_any": (C.c_int, [_vp, _vp, _u32, _vp, _vp]),
+    "chk_brdf_eval": (C.c_int, [_vp, _u32, _vp, _vp, _vp, _vp, _vp, _u32, _vp, _vp, _vp]),
+    "chk_brdf_sample": (C.c_int, [_vp, _u32, _vp, _vp, _vp, _vp, _vp, _u32, _vp, _vp, _vp]),
+    "chk_ltc_eval": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _u32, _vp, _vp, _u32, _vp, _vp]),
+    "chk_render": (C.c_int, [_vp, C.POINTER(chk_camera), C.POINTER(chk_render_params), _vp, _vp,
+                             C.POINTER(chk_render_stats), _vp]),
+    "chk_render_count": (C.c_int, [_vp, C.POINTER(chk_camera), C.POINTER(chk_render_params), _vp,
+                                   C.POINTER(chk_render_stats), _vp]),
+}
+
+_lib = None
+
+
+def load(path: str | os.PathLike | None = None):
+    """Load libchakum.so (raises ChkError when it is not built)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise ChkError(f"{p} is not built: run `make -C chakumrender_amd/csrc` "
+                       "(or __graft_entry__.build())")
+    L = C.CDLL(str(p))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = L
+    return L
+
+
+def check(status: int, what: str = ""):
+    if status != 0:
+        L = load()
+        raise ChkError(f"{what}: {L.chk_status_string(status).decode()}: "
+                       f"{L.chk_last_error().decode()}")
+
+
+def camera_struct(cam) -> chk_camera:
+    c = chk_camera()
+    c.pos[:] = [float(v) for v in cam.pos]
+    c.dir[:] = [float(v) for v in cam.dir]
+    c.up[:] = [float(v) for v in cam.up]
+    c.fovx_deg = cam.fovx_deg
+    c.fovy_deg = cam.fovy_deg
+    c.width = cam.width
+    c.height = cam.height
+    return c
+
+
+def params_struct(p) -> chk_render_params:
+    if isinstance(p, chk_render_params):
+        return p
+    d = dict(p)
+    r = chk_render_params()
+    r.spp = int(d.get("spp", 1))
+    r.sample_offset = int(d.get("sample_offset", 0))
+    r.max_depth = int(d.get("max_depth", 4))
+    r.mis_mode = int(d.get("mis_mode", MIS_BALANCE))
+    r.jitter_mode = int(d.get("jitter_mode", JITTER_BOX))
+    seed = int(d.get("seed", 0x5EED))
+    r.seed_lo = seed & 0xFFFFFFFF
+    r.seed_hi = (seed >> 32) & 0xFFFFFFFF
+    r.flags = int(d.get("flags", 0))
+    return r
+
+
+class Context:
+    """chk_ctx_t on one device (with the LTC table installed)."""
+
+    def __init__(self, device: int = 0, ltc_table: bool = True):
+        self.L = load()
+        h = C.c_void_p()
+        check(self.L.chk_create(device, C.byref(h)), "chk_create")
+        self.h = h
+        if ltc_table:
+            tab = np.ascontiguousarray(np.fromfile(LTC_TABLE, "<f8"))
+            check(self.L.chk_set_ltc_table(self.h, tab.ctypes.data, tab.size), "chk_set_ltc_table")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.chk_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceScene:
+    """chk_scene_t: triangles, materials, lights and the BVH resident in HBM."""
+
+    def __init__(self, ctx: Context, scene):
+        self.ctx = ctx
+        self.L = ctx.L
+        self.scene = scene
+        tris = np.ascontiguousarray(scene.tris, np.float32)
+        tri_mat = np.ascontiguousarray(scene.tri_mat, np.uint32)
+        mats = np.ascontiguousarray(scene.mats)
+        lights = np.ascontiguousarray(scene.lights)
+        h = C.c_void_p()
+        check(self.L.chk_scene_create(ctx.h, tris.ctypes.data, tris.shape[0], tri_mat.ctypes.data,
+                                      mats.ctypes.data, mats.shape[0], lights.ctypes.data,
+                                      lights.shape[0], C.byref(h)), "chk_scene_create")
+        self.h = h
+
+    def info(self) -> dict:
+        i = chk_scene_info()
+        check(self.L.chk_scene_get_info(self.h, C.byref(i)), "chk_scene_get_info")
+        return {k: getattr(i, k) for k, _ in chk_scene_info._fields_}
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.chk_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,440 @@
+// C ABI of libchakum (include/chakum_c.h): context, scene upload, trace,
+// BRDF and render entry points.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "chk_device.h"
+#include "chk_internal.h"
+
+namespace chk {
+
+thread_local std::string g_err;
+void set_error(const std::string &msg) { g_err = msg; }
+
+chk_status render_impl(chk_scene_s *sc, const chk_camera *cam, const chk_render_params *P,
+                       double *accum, double *sumsq, chk_render_stats *stats, bool count,
+                       hipStream_t st);
+
+namespace {
+
+__global__ void k_brdf_eval(const chk_material *__restrict__ mats, uint32_t n_mats,
+                            const uint32_t *__restrict__ mat_idx, const float *__restrict__ n,
+                            const float *__restrict__ wo_ref, const float *__restrict__ wo,
+                            const float *__restrict__ wi, uint32_t k, float *__restrict__ f,
+                            float *__restrict__ pdf) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= k) return;
+  const uint32_t mi = mat_idx[i];
+  if (mi >= n_mats) {
+    f[i] = NAN;
+    pdf[i] = NAN;
+    return;
+  }
+  const Mat m = load_mat(mats + mi);
+  const size_t o = 3 * (size_t)i;
+  float fv, pv;
+  brdf_pdf(m, mk(n[o], n[o + 1], n[o + 2]), mk(wo_ref[o], wo_ref[o + 1], wo_ref[o + 2]),
+           mk(wo[o], wo[o + 1], wo[o + 2]), mk(wi[o], wi[o + 1], wi[o + 2]), fv, pv);
+  f[i] = fv;
+  pdf[i] = pv;
+}
+
+__global__ void k_brdf_sample(const chk_material *__restrict__ mats, uint32_t n_mats,
+                              const uint32_t *__restrict__ mat_idx, const float *__restrict__ n,
+                              const float *__restrict__ wo_ref, const float *__restrict__ wo,
+                              const float *__restrict__ u, uint32_t k, float *__restrict__ wi,
+                              float *__restrict__ pdf) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= k) return;
+  const uint32_t mi = mat_idx[i];
+  const size_t o = 3 * (size_t)i;
+  if (mi >= n_mats) {
+    wi[o] = wi[o + 1] = wi[o + 2] = NAN;
+    pdf[i] = NAN;
+    return;
+  }
+  const Mat m = load_mat(mats + mi);
+  const f3 N = mk(n[o], n[o + 1], n[o + 2]), R = mk(wo_ref[o], wo_ref[o + 1], wo_ref[o + 2]),
+           O = mk(wo[o], wo[o + 1], wo[o + 2]);
+  const f3 w = brdf_sample(m, N, R, O, u[o], u[o + 1], u[o + 2]);
+  float fv, pv;
+  brdf_pdf(m, N, R, O, w, fv, pv);
+  wi[o] = w.x;
+  wi[o + 1] = w.y;
+  wi[o + 2] = w.z;
+  pdf[i] = pv;
+}
+
+inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+}  // namespace
+}  // namespace chk
+
+using namespace chk;
+
+#define CHK_TRY_BEGIN try {
+#define CHK_TRY_END                                   \
+  }                                                   \
+  catch (const std::bad_alloc &) {                    \
+    set_error("host allocation failed");              \
+    return CHK_ENOMEM;                                \
+  }                                                   \
+  catch (const std::exception &ex) {                  \
+    set_error(std::string("internal error: ") + ex.what()); \
+    return CHK_EINTERNAL;                             \
+  }                                                   \
+  catch (...) {                                       \
+    set_error("internal error");                      \
+    return CHK_EINTERNAL;                             \
+  }
+
+extern "C" {
+
+const char *chk_status_string(chk_status s) {
+  switch (s) {
+    case CHK_OK: return "ok";
+    case CHK_EINVAL: return "invalid argument";
+    case CHK_ENOMEM: return "out of memory";
+    case CHK_EDEVICE: return "device error";
+    case CHK_ECAPACITY: return "capacity exceeded";
+    case CHK_EINTERNAL: return "internal error";
+  }
+  return "unknown status";
+}
+
+const char *chk_last_error(void) { return g_err.c_str(); }
+
+const char *chk_version(void) { return "chakum-mi355x 0.1 (gfx950)"; }
+
+chk_status chk_create(int device, chk_ctx_t *out) {
+  CHK_TRY_BEGIN
+  if (!out) {
+    set_error("chk_create: null out");
+    return CHK_EINVAL;
+  }
+  *out = nullptr;
+  int n = 0;
+  CHK_HIP(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) {
+    set_error("chk_create: no such device");
+    return CHK_EINVAL;
+  }
+  CHK_HIP(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  CHK_HIP(hipGetDeviceProperties(&prop, device));
+  chk_ctx_s *c = new chk_ctx_s();
+  c->device = device;
+  c->n_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+  *out = c;
+  return CHK_OK;
+  CHK_TRY_END
+}
+
+chk_status chk_destroy(chk_ctx_t c) {
+  if (!c) return CHK_OK;
+  (void)hipSetDevice(c->device);
+  if (c->ltc_tab) (void)hipFree(c->ltc_tab);
+  if (c->ltc_dev) (void)hipFree(c->ltc_dev);
+  delete c;
+  return CHK_OK;
+}
+
+chk_status chk_scene_create(chk_ctx_t ctx, const float *tris, uint32_t n_tris,
+                            const uint32_t *tri_mat, const chk_material *mats, uint32_t n_mats,
+                            const chk_mesh_light *lights, uint32_t n_lights, chk_scene_t *out) {
+  CHK_TRY_BEGIN
+  if (!ctx || !out || (n_tris && (!tris || !tri_mat)) || (n_mats && !mats) ||
+      (n_lights && !lights)) {
+    set_error("chk_scene_create: null argument");
+    return CHK_EINVAL;
+  }
+  *out = nullptr;
+  for (uint32_t i = 0; i < n_tris; ++i)
+    if (tri_mat[i] >= n_mats) {
+      set_error("chk_scene_create: triangle " + std::to_string(i) + " has material index " +
+                std::to_string(tri_mat[i]) + " >= n_mats");
+      return CHK_EINVAL;
+    }
+  for (uint32_t j = 0; j < n_lights; ++j) {
+    if (lights[j].n_tris == 0 || (uint64_t)lights[j].first_tri + lights[j].n_tris > n_tris) {
+      set_error("chk_scene_create: mesh light " + std::to_string(j) + " is empty or out of range");
+      return CHK_EINVAL;
+    }
+  }
+  CHK_HIP(hipSetDevice(ctx->device));
+  auto t0 = std::chrono::steady_clock::now();
+  Bvh4 bvh;
+  std::string msg;
+  if (!build_bvh4(tris, n_tris, bvh, &msg)) {
+    set_error("chk_scene_create: " + msg);
+    return CHK_ECAPACITY;
+  }
+  const int stack = pick_stack_size(bvh.stats.stack_bound);
+  if (stack < 0) {
+    set_error("chk_scene_create: BVH needs a traversal stack of " +
+              std::to_string(bvh.stats.stack_bound) + " entries (> 96)");
+    return CHK_ECAPACITY;
+  }
+  auto t1 = std::chrono::steady_clock::now();
+  const uint32_t m = bvh.stats.n_tris;
+
+  // per-leaf (material, light)
+  std::vector<int32_t> tri_light(n_tris, -1);
+  for (uint32_t j = 0; j < n_lights; ++j)
+    for (uint32_t i = 0; i < lights[j].n_tris; ++i) tri_light[lights[j].first_tri + i] = (int32_t)j;
+  std::vector<uint32_t> info(2 * (size_t)std::max(m, 1u), 0xFFFFFFFFu);
+  for (uint32_t li = 0; li < m; ++li) {
+    uint32_t o = bvh.leaf_to_orig[li];
+    info[2 * li] = tri_mat[o];
+    info[2 * li + 1] = tri_light[o] >= 0 ? (uint32_t)tri_light[o] : 0xFFFFFFFFu;
+  }
+  // light tables: exact area CDF per light (the reference quantises it to
+  // 300 bins, lightSources.h:151, lightSources.cpp:784-807)
+  std::vector<LightTri> ltris;
+  std::vector<LightRec> lrecs;
+  uint32_t used_lights = 0;
+  for (uint32_t j = 0; j < n_lights; ++j) {
+    LightRec R;
+    R.first = (uint32_t)ltris.size();
+    R.count = 0;
+    double area = 0;
+    std::vector<double> areas;
+    for (uint32_t i = 0; i < lights[j].n_tris; ++i) {
+      const float *p = tris + 9 * (size_t)(lights[j].first_tri + i);
+      double e1[3], e2[3];
+      for (int k = 0; k < 3; ++k) {
+        e1[k] = (double)p[3 + k] - p[k];
+        e2[k] = (double)p[6 + k] - p[k];
+      }
+      double cx = e1[1] * e2[2] - e1[2] * e2[1], cy = e1[2] * e2[0] - e1[0] * e2[2],
+             cz = e1[0] * e2[1] - e1[1] * e2[0];
+      double a = 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);
+      areas.push_back(a);
+      area += a;
+    }
+    if (!(area > 0)) {
+      set_error("chk_scene_create: mesh light " + std::to_string(j) + " has zero area");
+      return CHK_EINVAL;
+    }
+    double cdf = 0;
+    for (uint32_t i = 0; i < lights[j].n_tris; ++i) {
+      const uint32_t o = lights[j].first_tri + i;
+      const float *p = tris + 9 * (size_t)o;
+      const chk_material &mt = mats[tri_mat[o]];
+      cdf += areas[i] / area;
+      LightTri T;
+      T.a_cdf = make_float4(p[0], p[1], p[2], i + 1 == lights[j].n_tris ? 2.0f : (float)cdf);
+      T.b_area = make_float4(p[3], p[4], p[5], (float)areas[i]);
+      uint32_t leaf = bvh.orig_to_leaf[o] >= 0 ? (uint32_t)bvh.orig_to_leaf[o] : 0xFFFFFFFFu;
+      float lf;
+      std::memcpy(&lf, &leaf, 4);
+      T.c_leaf = make_float4(p[6], p[7], p[8], lf);
+      T.rad = make_float4(mt.radiance[0], mt.radiance[1], mt.radiance[2], 0.0f);
+      ltris.push_back(T);
+      ++R.count;
+    }
+    R.area = (float)area;
+    R.pad = 0;
+    lrecs.push_back(R);
+    ++used_lights;
+  }
+
+  // one device blob
+  const size_t b_nodes = align256(bvh.nodes.size() * sizeof(Node4));
+  const size_t b_tris = align256(std::max<size_t>(bvh.tris.size(), 1) * sizeof(TriRec));
+  const size_t b_info = align256(info.size() * 4);
+  const size_t b_mats = align256(std::max<uint32_t>(n_mats, 1) * sizeof(chk_material));
+  const size_t b_lt = align256(std::max<size_t>(ltris.size(), 1) * sizeof(LightTri));
+  const size_t b_lr = align256(std::max<size_t>(lrecs.size(), 1) * sizeof(LightRec));
+  const size_t total = b_nodes + b_tris + b_info + b_mats + b_lt + b_lr;
+  void *blob = nullptr;
+  if (hipMalloc(&blob, total) != hipSuccess) {
+    set_error("chk_scene_create: cannot allocate " + std::to_string(total) + " device bytes");
+    return CHK_ENOMEM;
+  }
+  char *p = (char *)blob;
+  chk_scene_s *sc = new chk_scene_s();
+  sc->ctx = ctx;
+  sc->blob = blob;
+  sc->blob_bytes = total;
+  sc->d.nodes = (const Node4 *)p;
+  hipError_t e = hipMemcpy(p, bvh.nodes.data(), bvh.nodes.size() * sizeof(Node4),
+                           hipMemcpyHostToDevice);
+  p += b_nodes;
+  sc->d.tris = (const float4 *)p;
+  if (e == hipSuccess && !bvh.tris.empty())
+    e = hipMemcpy(p, bvh.tris.data(), bvh.tris.size() * sizeof(TriRec), hipMemcpyHostToDevice);
+  p += b_tris;
+  sc->d.leaf_info = (const uint2 *)p;
+  if (e == hipSuccess) e = hipMemcpy(p, info.data(), info.size() * 4, hipMemcpyHostToDevice);
+  p += b_info;
+  sc->d.mats = (const chk_material *)p;
+  if (e == hipSuccess && n_mats)
+    e = hipMemcpy(p, mats, n_mats * sizeof(chk_material), hipMemcpyHostToDevice);
+  p += b_mats;
+  sc->d.light_tris = (const LightTri *)p;
+  if (e == hipSuccess && !ltris.empty())
+    e = hipMemcpy(p, ltris.data(), ltris.size() * sizeof(LightTri), hipMemcpyHostToDevice);
+  p += b_lt;
+  sc->d.lights = (const LightRec *)p;
+  if (e == hipSuccess && !lrecs.empty())
+    e = hipMemcpy(p, lrecs.data(), lrecs.size() * sizeof(LightRec), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    set_error(std::string("chk_scene_create: upload: ") + hipGetErrorString(e));
+    (void)hipFree(blob);
+    delete sc;
+    return CHK_EDEVICE;
+  }
+  sc->d.n_lights = used_lights;
+  sc->d.n_leaf_tris = m;
+  sc->stack_size = stack;
+  chk_scene_info &I = sc->info;
+  I.n_tris = n_tris;
+  I.n_tris_bvh = m;
+  I.n_nodes = bvh.stats.n_nodes;
+  I.n_leaves = bvh.stats.n_leaves;
+  I.max_depth = bvh.stats.max_depth;
+  I.stack_bound = bvh.stats.stack_bound;
+  I.stack_size = (uint32_t)stack;
+  I.n_lights = used_lights;
+  I.device_bytes = total;
+  I.sah_cost = bvh.stats.sah_cost;
+  I.build_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  *out = sc;
+  return CHK_OK;
+  CHK_TRY_END
+}
+
+chk_status chk_scene_destroy(chk_scene_t sc) {
+  if (!sc) return CHK_OK;
+  (void)hipSetDevice(sc->ctx->device);
+  if (sc->work) (void)hipFree(sc->work);
+  if (sc->blob) (void)hipFree(sc->blob);
+  delete sc;
+  return CHK_OK;
+}
+
+chk_status chk_scene_get_info(chk_scene_t sc, chk_scene_info *info) {
+  if (!sc || !info) {
+    set_error("chk_scene_get_info: null argument");
+    return CHK_EINVAL;
+  }
+  *info = sc->info;
+  return CHK_OK;
+}
+
+static chk_status ensure_trace_scratch(chk_scene_s *sc) {
+  // 64 bytes of control words for the API trace calls live at the start of
+  // the render work buffer
+  if (sc->work_bytes >= 4096) return CHK_OK;
+  if (sc->work) (void)hipFree(sc->work);
+  sc->work = nullptr;
+  sc->work_bytes = 0;
+  if (hipMalloc(&sc->work, 4096) != hipSuccess) {
+    set_error("cannot allocate trace scratch");
+    return CHK_ENOMEM;
+  }
+  sc->work_bytes = 4096;
+  return CHK_OK;
+}
+
+chk_status chk_trace_closest(chk_scene_t sc, const chk_ray *rays, uint32_t n, chk_hit *hits,
+                             void *stream) {
+  CHK_TRY_BEGIN
+  if (!sc || (n && (!rays || !hits))) {
+    set_error("chk_trace_closest: null argument");
+    return CHK_EINVAL;
+  }
+  if (n == 0) return CHK_OK;
+  CHK_HIP(hipSetDevice(sc->ctx->device));
+  chk_status s = ensure_trace_scratch(sc);
+  if (s != CHK_OK) return s;
+  hipStream_t st = (hipStream_t)stream;
+  uint32_t *work = (uint32_t *)sc->work;
+  unsigned long long *cnt = (unsigned long long *)((char *)sc->work + 256);
+  CHK_HIP(launch_zero_u32(work, 64 + 16, st));
+  CHK_HIP(launch_closest(sc, (const float4 *)rays, n, nullptr, (float4 *)hits, false, work, false,
+                         cnt, trace_grid(sc), st));
+  return CHK_OK;
+  CHK_TRY_END
+}
+
+chk_status chk_trace_any(chk_scene_t sc, const chk_ray *rays, uint32_t n, uint8_t *occluded,
+                         void *stream) {
+  CHK_TRY_BEGIN
+  if (!sc || (n && (!rays || !occluded))) {
+    set_error("chk_trace_any: null argument");
+    return CHK_EINVAL;
+  }
+  if (n == 0) return CHK_OK;
+  CHK_HIP(hipSetDevice(sc->ctx->device));
+  chk_status s = ensure_trace_scratch(sc);
+  if (s != CHK_OK) return s;
+  hipStream_t st = (hipStream_t)stream;
+  uint32_t *work = (uint32_t *)sc->work;
+  unsigned long long *cnt = (unsigned long long *)((char *)sc->work + 256);
+  CHK_HIP(launch_zero_u32(work, 64 + 16, st));
+  CHK_HIP(launch_any(sc, (const float4 *)rays, n, nullptr, occluded, nullptr, nullptr, false, work,
+                     false, cnt, trace_grid(sc), st));
+  return CHK_OK;
+  CHK_TRY_END
+}
+
+chk_status chk_brdf_eval(const chk_material *mats, uint32_t n_mats, const uint32_t *mat_idx,
+                         const float *n, const float *wo_ref, const float *wo, const float *wi,
+                         uint32_t k, float *f, float *pdf, void *stream) {
+  if (k == 0) return CHK_OK;
+  if (!mats || !mat_idx || !n || !wo_ref || !wo || !wi || !f || !pdf || n_mats == 0) {
+    set_error("chk_brdf_eval: null argument");
+    return CHK_EINVAL;
+  }
+  hipLaunchKernelGGL(k_brdf_eval, dim3((k + 255) / 256), dim3(256), 0, (hipStream_t)stream, mats,
+                     n_mats, mat_idx, n, wo_ref, wo, wi, k, f, pdf);
+  CHK_HIP(hipGetLastError());
+  return CHK_OK;
+}
+
+chk_status chk_brdf_sample(const chk_material *mats, uint32_t n_mats, const uint32_t *mat_idx,
+                           const float *n, const float *wo_ref, const float *wo, const float *u,
+                           uint32_t k, float *wi, float *pdf, void *stream) {
+  if (k == 0) return CHK_OK;
+  if (!mats || !mat_idx || !n || !wo_ref || !wo || !u || !wi || !pdf || n_mats == 0) {
+    set_error("chk_brdf_sample: null argument");
+    return CHK_EINVAL;
+  }
+  hipLaunchKernelGGL(k_brdf_sample, dim3((k + 255) / 256), dim3(256), 0, (hipStream_t)stream,
+                     mats, n_mats, mat_idx, n, wo_ref, wo, u, k, wi, pdf);
+  CHK_HIP(hipGetLastError());
+  return CHK_OK;
+}
+
+chk_status chk_render(chk_scene_t sc, const chk_camera *camera, const chk_render_params *params,
+                      double *accum, double *sumsq, chk_render_stats *stats, void *stream) {
+  CHK_TRY_BEGIN
+  if (!sc) {
+    set_error("chk_render: null scene");
+    return CHK_EINVAL;
+  }
+  CHK_HIP(hipSetDevice(sc->ctx->device));
+  return render_impl(sc, camera, params, accum, sumsq, stats, false, (hipStream_t)stream);
+  CHK_TRY_END
+}
+
+chk_status chk_render_count(chk_scene_t sc, const chk_camera *camera,
+                            const chk_render_params *params, double *accum,
+                            chk_render_stats *stats, void *stream) {
+  CHK_TRY_BEGIN
+  if (!sc || !stats) {
+    set_error("chk_render_count: null scene or stats");
+    return CHK_EINVAL;
+  }
+  CHK_HIP(hipSetDevice(sc->ctx->device));
+  return render_impl(sc, camera, params, accum, nullptr, stats, true, (hipStream_t)stream);
+  CHK_TRY_END
+}
+
+}  // extern "C"

@@ -1,0 +1,307 @@
+// Analytic polygon-light shading: LightSource::getLightFromMeshSource_Analytic
+// (lightSources_CV.cpp:323-333) for many shading points.
+//
+// Per point: M_GGX (ltc.h:18-61) reads two rows of the 64 x 64 GGX fit
+// (only Minv entries a[0], a[2], a[4], a[6], a[8] are non-zero, so a row is
+// 5 floats + amplitude), lerps them in theta, rotates into the frame of
+// getBrdfApproxParam (materialTypes.cpp:52-74) and integrates every light
+// triangle in the cosine domain (getLightAnalytic, lightSources_CV.cpp:271-321)
+// and, for ks < 1, in world space (getLightAnalyticDiffuse, :221-269).
+// The reference's clipping rules are kept, including the x-only "between"
+// test on the CA edge (lightSources_CV.cpp:101-102,194-195).  The edge
+// integral uses theta = atan2(|v1 x v2|, v1.v2), equal to the reference's
+// acos(v1.v2) but well conditioned in FP32.
+#include "chk_device.h"
+#include "chk_internal.h"
+
+namespace chk {
+namespace {
+
+constexpr int kLtcBlock = 256;
+
+// integrateEdge (lightSources_CV.cpp:8-25): (v1 x v2).n * theta/sin(theta)
+__device__ inline float edge_factor(f3 v1, f3 v2, f3 c) {
+  const float s = sqrtf(dot(c, c));
+  const float th = atan2f(s, dot(v1, v2));
+  return th > 0.001f ? th / s : 1.0f;
+}
+__device__ inline float edge_z(f3 v1, f3 v2) {
+  const f3 c = cross(v1, v2);
+  return c.z * edge_factor(v1, v2, c);
+}
+__device__ inline float edge_n(f3 v1, f3 v2, f3 n) {
+  const f3 c = cross(v1, v2);
+  return dot(c, n) * edge_factor(v1, v2, c);
+}
+
+// clipTriangle against z = 0 (lightSources_CV.cpp:129-219)
+__device__ inline bool clip_z(f3 A, f3 B, f3 C, float nA, float nB, float nC, f3 *clip) {
+  const float eps = 1e-15f;
+  int cnt = 0, idx = 0;
+  const f3 dAB = A - B, dBC = B - C, dCA = C - A;
+  if (dAB.z * dAB.z > eps) {
+    const f3 I = dAB * (-A.z / dAB.z) + A;
+    if (dot(I - A, B - I) > 0.0f) {
+      ++cnt;
+      if (nA < 0.0f) { clip[idx++] = I; clip[idx++] = B; }
+      else { clip[idx++] = A; clip[idx++] = I; }
+    }
+  } else if (A.z * A.z < eps) {
+    return false;
+  }
+  if (dBC.z * dBC.z > eps) {
+    const f3 I = dBC * (-B.z / dBC.z) + B;
+    if (dot(I - B, C - I) > 0.0f) {
+      ++cnt;
+      if (idx < 4) {
+        if (nB < 0.0f) { clip[idx++] = I; clip[idx++] = C; }
+        else { clip[idx++] = B; clip[idx++] = I; }
+      }
+    }
+  } else if (B.z * B.z < eps) {
+    return false;
+  }
+  if (dCA.z * dCA.z > eps) {
+    const f3 I = dCA * (-C.z / dCA.z) + C;
+    if ((I.x - C.x) * (A.x - I.x) > 0.0f) {
+      ++cnt;
+      if (idx < 4) {
+        if (nC < 0.0f) { clip[idx++] = I; clip[idx++] = A; }
+        else { clip[idx++] = C; clip[idx++] = I; }
+      }
+    }
+  } else if (C.z * C.z < eps) {
+    return false;
+  }
+  return cnt > 1;
+}
+
+// clipTriangle against the plane through P with normal n
+// (lightSources_CV.cpp:28-126)
+__device__ inline bool clip_plane(f3 A, f3 B, f3 C, f3 n, f3 P, float nA, float nB, float nC,
+                                  f3 *clip) {
+  const float eps = 1e-15f;
+  int cnt = 0, idx = 0;
+  const f3 dAB = A - B, dBC = B - C, dCA = C - A;
+  const float nAB = dot(n, dAB), nBC = dot(n, dBC), nCA = dot(n, dCA);
+  const float pA = dot(n, P - A), pB = dot(n, P - B), pC = dot(n, P - C);
+  if (nAB * nAB > eps) {
+    const f3 I = dAB * (pA / nAB) + A;
+    if (dot(I - A, B - I) > 0.0f) {
+      ++cnt;
+      if (nA < 0.0f) { clip[idx++] = I; clip[idx++] = B; }
+      else { clip[idx++] = A; clip[idx++] = I; }
+    }
+  } else if (pA * pA < eps) {
+    return false;
+  }
+  if (nBC * nBC > eps) {
+    const f3 I = dBC * (pB / nBC) + B;
+    if (dot(I - B, C - I) > 0.0f) {
+      ++cnt;
+      if (idx < 4) {
+        if (nB < 0.0f) { clip[idx++] = I; clip[idx++] = C; }
+        else { clip[idx++] = B; clip[idx++] = I; }
+      }
+    }
+  } else if (pB * pB < eps) {
+    return false;
+  }
+  if (nCA * nCA > eps) {
+    const f3 I = dCA * (pC / nCA) + C;
+    if ((I.x - C.x) * (A.x - I.x) > 0.0f) {
+      ++cnt;
+      if (idx < 4) {
+        if (nC < 0.0f) { clip[idx++] = I; clip[idx++] = A; }
+        else { clip[idx++] = C; clip[idx++] = I; }
+      }
+    }
+  } else if (pC * pC < eps) {
+    return false;
+  }
+  return cnt > 1;
+}
+
+__device__ inline f3 row_mul(f3 r0, f3 r1, f3 r2, f3 v) { return mk(dot(r0, v), dot(r1, v), dot(r2, v)); }
+
+__global__ __launch_bounds__(kLtcBlock) void k_ltc(const float4 *__restrict__ tab,
+                                                   const float *__restrict__ X,
+                                                   const float *__restrict__ N,
+                                                   const float *__restrict__ V,
+                                                   const float *__restrict__ alpha,
+                                                   const float *__restrict__ ks, uint32_t k,
+                                                   const float *__restrict__ lt,
+                                                   const float *__restrict__ lrad, uint32_t nlt,
+                                                   float *__restrict__ rgb) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < k; i += stride) {
+    const size_t o = 3 * (size_t)i;
+    const f3 x = mk(X[o], X[o + 1], X[o + 2]);
+    const f3 n = mk(N[o], N[o + 1], N[o + 2]);
+    const f3 wo = mk(V[o], V[o + 1], V[o + 2]);
+    const float a = alpha[i];
+    const float kss = ks[i];
+    // ---- M_GGX(acos(n.wo), alpha) (ltc.h:18-61), bins in double like the reference
+    const float theta = acosf(dot(n, wo));
+    const double tq = (double)theta / (0.5 * 3.14159265359) * 64.0;
+    const int tb = max(0, min(63, (int)floor(tq)));
+    const int ab = max(0, min(63, (int)floor(sqrt((double)a) * 64.0)));
+    const int tp = min(63, tb + 1);
+    const float4 r0a = tab[2 * (ab + tb * 64)], r0b = tab[2 * (ab + tb * 64) + 1];
+    float m0 = r0a.x, m2 = r0a.y, m4 = r0a.z, m6 = r0a.w, m8 = r0b.x, amp = r0b.y;
+    if (tb != tp) {
+      const float mult = (float)(tq - (double)(float)tb);
+      const float4 r1a = tab[2 * (ab + tp * 64)], r1b = tab[2 * (ab + tp * 64) + 1];
+      m0 = fmaf(r1a.x - m0, mult, m0);
+      m2 = fmaf(r1a.y - m2, mult, m2);
+      m4 = fmaf(r1a.z - m4, mult, m4);
+      m6 = fmaf(r1a.w - m6, mult, m6);
+      m8 = fmaf(r1b.x - m8, mult, m8);
+      amp = fmaf(r1b.y - amp, mult, amp);
+    }
+    // ---- frame of getBrdfApproxParam (materialTypes.cpp:59-68)
+    const f3 T1 = normalize(wo - n * dot(n, wo));
+    const f3 T2 = normalize(cross(n, T1));
+    // Minv * R, R rows (T1, T2, n); Minv = [[m0,0,m2],[0,m4,0],[m6,0,m8]]
+    const f3 R0 = T1 * m0 + n * m2, R1 = T2 * m4, R2 = T1 * m6 + n * m8;
+    float sl0 = 0.0f, sl1 = 0.0f, sl2 = 0.0f, sd0 = 0.0f, sd1 = 0.0f, sd2 = 0.0f;
+    const bool do_ltc = kss != 0.0f, do_dif = kss != 1.0f;
+    for (uint32_t j = 0; j < nlt; ++j) {
+      const float *p = lt + 9 * (size_t)j;
+      const f3 A = mk(p[0], p[1], p[2]), B = mk(p[3], p[4], p[5]), C = mk(p[6], p[7], p[8]);
+      const f3 rad = mk(lrad[3 * j], lrad[3 * j + 1], lrad[3 * j + 2]);
+      if (do_ltc) {
+        f3 e1 = row_mul(R0, R1, R2, A - x), e2 = row_mul(R0, R1, R2, B - x),
+           e3 = row_mul(R0, R1, R2, C - x);
+        float sum = 0.0f;
+        bool use = true;
+        if (e1.z > 0.0f && e2.z > 0.0f && e3.z > 0.0f) {
+          e1 = normalize(e1);
+          e2 = normalize(e2);
+          e3 = normalize(e3);
+          sum = edge_z(e1, e2) + edge_z(e2, e3) + edge_z(e3, e1);
+        } else if (e1.z < 0.0f && e2.z < 0.0f && e3.z < 0.0f) {
+          use = false;
+        } else {
+          f3 cl[4] = {mk(0, 0, 0), mk(0, 0, 0), mk(0, 0, 0), mk(0, 0, 0)};
+          if (clip_z(e1, e2, e3, e1.z * rsqrtf(dot(e1, e1)), e2.z * rsqrtf(dot(e2, e2)),
+                     e3.z * rsqrtf(dot(e3, e3)), cl)) {
+            const f3 c1 = normalize(cl[0]), c2 = normalize(cl[1]), c3 = normalize(cl[2]),
+                     c4 = normalize(cl[3]);
+            sum = edge_z(c1, c2) + edge_z(c2, c3) + edge_z(c3, c4) + edge_z(c4, c1);
+          } else {
+            use = false;
+          }
+        }
+        if (use) {
+          sum = fabsf(sum);
+          sl0 = fmaf(rad.x, sum, sl0);
+          sl1 = fmaf(rad.y, sum, sl1);
+          sl2 = fmaf(rad.z, sum, sl2);
+        }
+      }
+      if (do_dif) {
+        const f3 e1 = normalize(A - x), e2 = normalize(B - x), e3 = normalize(C - x);
+        const float nA = dot(e1, n), nB = dot(e2, n), nC = dot(e3, n);
+        float sum = 0.0f;
+        bool use = true;
+        if (nA > 0.0f && nB > 0.0f && nC > 0.0f) {
+          sum = edge_n(e1, e2, n) + edge_n(e2, e3, n) + edge_n(e3, e1, n);
+        } else if (nA < 0.0f && nB < 0.0f && nC < 0.0f) {
+          use = false;
+        } else {
+          f3 cl[4] = {mk(0, 0, 0), mk(0, 0, 0), mk(0, 0, 0), mk(0, 0, 0)};
+          if (clip_plane(A, B, C, n, x, nA, nB, nC, cl)) {
+            const f3 c1 = normalize(cl[0] - x), c2 = normalize(cl[1] - x),
+                     c3 = normalize(cl[2] - x), c4 = normalize(cl[3] - x);
+            sum = edge_n(c1, c2, n) + edge_n(c2, c3, n) + edge_n(c3, c4, n) + edge_n(c4, c1, n);
+          } else {
+            use = false;
+          }
+        }
+        if (use) {
+          sum = fabsf(sum);
+          sd0 = fmaf(rad.x, sum, sd0);
+          sd1 = fmaf(rad.y, sum, sd1);
+          sd2 = fmaf(rad.z, sum, sd2);
+        }
+      }
+    }
+    // Sum / (2 pi), then Analytic = LTC * amp * ks + diffuse * (1 - ks)
+    const float inv2pi = 1.0f / (2.0f * kPi);
+    const float wl = inv2pi * amp * kss, wd = inv2pi * (1.0f - kss);
+    rgb[o] = (do_ltc ? sl0 * wl : 0.0f) + (do_dif ? sd0 * wd : 0.0f);
+    rgb[o + 1] = (do_ltc ? sl1 * wl : 0.0f) + (do_dif ? sd1 * wd : 0.0f);
+    rgb[o + 2] = (do_ltc ? sl2 * wl : 0.0f) + (do_dif ? sd2 * wd : 0.0f);
+  }
+}
+
+}  // namespace
+}  // namespace chk
+
+using namespace chk;
+
+extern "C" {
+
+chk_status chk_set_ltc_table(chk_ctx_t ctx, const double *table, size_t n_doubles) {
+  if (!ctx || !table || n_doubles != 4096u * 19u) {
+    set_error("chk_set_ltc_table: expected 4096 x 19 doubles");
+    return CHK_EINVAL;
+  }
+  CHK_HIP(hipSetDevice(ctx->device));
+  // packed row: Minv a[0], a[2], a[4], a[6], a[8] (= file entries 0, 6, 4,
+  // 2, 8 through mat3's column-major 9-argument ctor), amplitude, 0, 0
+  std::vector<float> packed(4096 * 8, 0.0f);
+  for (int r = 0; r < 4096; ++r) {
+    const double *f = table + 19 * (size_t)r + 9;
+    float *q = &packed[8 * (size_t)r];
+    q[0] = (float)f[0];
+    q[1] = (float)f[6];
+    q[2] = (float)f[4];
+    q[3] = (float)f[2];
+    q[4] = (float)f[8];
+    q[5] = (float)table[19 * (size_t)r + 18];
+    for (int z : {1, 3, 5, 7})
+      if (f[z] != 0.0) {
+        set_error("chk_set_ltc_table: Minv entry that should be zero is not");
+        return CHK_EINVAL;
+      }
+  }
+  if (!ctx->ltc_tab) {
+    if (hipMalloc(&ctx->ltc_tab, packed.size() * 4) != hipSuccess) {
+      ctx->ltc_tab = nullptr;
+      set_error("chk_set_ltc_table: cannot allocate the table");
+      return CHK_ENOMEM;
+    }
+  }
+  CHK_HIP(hipMemcpy(ctx->ltc_tab, packed.data(), packed.size() * 4, hipMemcpyHostToDevice));
+  ctx->ltc_host.assign(table, table + n_doubles);
+  return CHK_OK;
+}
+
+chk_status chk_ltc_eval(chk_ctx_t ctx, const float *x, const float *n, const float *v,
+                        const float *alpha, const float *ks, uint32_t k, const float *light_tris,
+                        const float *radiance, uint32_t n_light_tris, float *rgb, void *stream) {
+  if (!ctx) {
+    set_error("chk_ltc_eval: null context");
+    return CHK_EINVAL;
+  }
+  if (!ctx->ltc_tab) {
+    set_error("chk_ltc_eval: call chk_set_ltc_table first");
+    return CHK_EINVAL;
+  }
+  if (k == 0) return CHK_OK;
+  if (!x || !n || !v || !alpha || !ks || !rgb || (n_light_tris && (!light_tris || !radiance))) {
+    set_error("chk_ltc_eval: null argument");
+    return CHK_EINVAL;
+  }
+  CHK_HIP(hipSetDevice(ctx->device));
+  const uint32_t blocks = std::min<uint32_t>((k + kLtcBlock - 1) / kLtcBlock, (uint32_t)ctx->n_cu * 16);
+  hipLaunchKernelGGL(k_ltc, dim3(blocks), dim3(kLtcBlock), 0, (hipStream_t)stream,
+                     (const float4 *)ctx->ltc_tab, x, n, v, alpha, ks, k, light_tris, radiance,
+                     n_light_tris, rgb);
+  CHK_HIP(hipGetLastError());
+  return CHK_OK;
+}
+
+}  // extern "C"

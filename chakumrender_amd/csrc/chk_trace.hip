@@ -1,0 +1,254 @@
+// Closest-hit and any-hit BVH traversal for gfx950.
+//
+// Restates BvhAccel::intersectT (bvhAccel.cpp:200-249) and shadow()
+// (shader.cpp:76-97) over the 4-wide SoA tree of chk_bvh.cpp:
+//  * persistent waves fetch 64 rays at a time from a device work counter;
+//  * one ray per lane, the current node in VGPRs, pending nodes on a
+//    per-lane stack in LDS laid out [slot][lane] (bank-conflict free);
+//  * child boxes are tested with the robust slab test of Ize ("Robust BVH
+//    Ray Traversal", JCGT 2(2) 2013), children ordered by entry distance;
+//  * triangles use the watertight test (chk_device.h) with the reference's
+//    accept rules.
+#include "chk_device.h"
+#include "chk_internal.h"
+
+namespace chk {
+namespace {
+
+constexpr int kBlock = 128;
+
+__device__ inline float safe_rcp(float x) {
+  return 1.0f / (fabsf(x) > 1e-20f ? x : copysignf(1e-20f, x));
+}
+
+template <bool ANY, bool COUNT, int STACK>
+__device__ inline void traverse(const DeviceScene &S, uint32_t *__restrict__ stk, f3 o, f3 d,
+                                float tmax, uint32_t skip, bool leaf_ids, float &t_out,
+                                uint32_t &id_out, float &u_out, float &v_out, bool &hit_any,
+                                uint32_t &nv, uint32_t &nt, uint32_t &overflow) {
+  const RayPre pre = ray_pre(d);
+  const f3 inv = mk(safe_rcp(d.x), safe_rcp(d.y), safe_rcp(d.z));
+  float t = tmax;
+  uint32_t best = 0xFFFFFFFFu;
+  float bu = 0.0f, bv = 0.0f;
+  uint32_t entry = 0;  // root
+  int sp = 0;
+  hit_any = false;
+  const float kGrow = 1.0f + 2.0f * 3.5762786865234375e-07f;  // 1 + 2*gamma_3
+  while (true) {
+    if (!(entry & kLeafBit)) {
+      if (COUNT) ++nv;
+      const float4 *np = reinterpret_cast<const float4 *>(S.nodes + entry);
+      const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
+      const uint4 ch = *reinterpret_cast<const uint4 *>(np + 6);
+      float key[4];
+      uint32_t cid[4] = {ch.x, ch.y, ch.z, ch.w};
+      const float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, hxa[4] = {hx.x, hx.y, hx.z, hx.w};
+      const float lya[4] = {ly.x, ly.y, ly.z, ly.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w};
+      const float lza[4] = {lz.x, lz.y, lz.z, lz.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
+      int nh = 0;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        float tx0 = (lxa[c] - o.x) * inv.x, tx1 = (hxa[c] - o.x) * inv.x;
+        float ty0 = (lya[c] - o.y) * inv.y, ty1 = (hya[c] - o.y) * inv.y;
+        float tz0 = (lza[c] - o.z) * inv.z, tz1 = (hza[c] - o.z) * inv.z;
+        float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
+        float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), t));
+        bool h = tn <= tf * kGrow;
+        key[c] = h ? tn : kInf;
+        nh += h ? 1 : 0;
+      }
+      // sort (key, cid) ascending: 5 compare-exchanges
+#define CHK_CX(a, b)                                      \
+  {                                                       \
+    bool sw = key[b] < key[a];                            \
+    float tk = sw ? key[b] : key[a];                      \
+    key[b] = sw ? key[a] : key[b];                        \
+    key[a] = tk;                                          \
+    uint32_t tc = sw ? cid[b] : cid[a];                   \
+    cid[b] = sw ? cid[a] : cid[b];                        \
+    cid[a] = tc;                                          \
+  }
+      CHK_CX(0, 1) CHK_CX(2, 3) CHK_CX(0, 2) CHK_CX(1, 3) CHK_CX(1, 2)
+#undef CHK_CX
+      if (nh > 0) {
+        entry = cid[0];
+        // push far-to-near so the nearer ones pop first
+        for (int c = nh - 1; c >= 1; --c) {
+          if (sp < STACK) {
+            stk[sp * kBlock] = cid[c];
+            ++sp;
+          } else {
+            overflow = 1;
+          }
+        }
+        continue;
+      }
+    } else {
+      const uint32_t first = leaf_first(entry), cnt = leaf_count(entry);
+      for (uint32_t k = 0; k < cnt; ++k) {
+        const uint32_t li = first + k;
+        const float4 *tp = S.tris + 3 * (size_t)li;
+        const float4 a = tp[0], b = tp[1], c = tp[2];
+        const uint32_t tid = leaf_ids ? li : __float_as_uint(a.w);
+        if (COUNT) ++nt;
+        if (tid == skip) continue;
+        float tt = t, uu, vv;
+        if (tri_hit(pre, o, d, xyz(a), xyz(b), xyz(c), tt, uu, vv)) {
+          t = tt;
+          best = tid;
+          bu = uu;
+          bv = vv;
+          hit_any = true;
+          if (ANY) break;
+        }
+      }
+      if (ANY && hit_any) break;
+    }
+    if (sp == 0) break;
+    --sp;
+    entry = stk[sp * kBlock];
+  }
+  t_out = best == 0xFFFFFFFFu ? 1e30f : t;
+  id_out = best;
+  u_out = bu;
+  v_out = bv;
+}
+
+template <bool ANY, bool COUNT, int STACK>
+__global__ __launch_bounds__(kBlock) void k_trace(DeviceScene S, const float4 *__restrict__ rays,
+                                                  uint32_t n, const uint32_t *__restrict__ count_dev,
+                                                  float4 *__restrict__ hits,
+                                                  uint8_t *__restrict__ occluded,
+                                                  const float4 *__restrict__ shadow_L,
+                                                  float4 *__restrict__ path_L, int leaf_ids,
+                                                  uint32_t *__restrict__ work,
+                                                  unsigned long long *__restrict__ counters) {
+  __shared__ uint32_t stack_mem[STACK * kBlock];
+  uint32_t *stk = stack_mem + threadIdx.x;
+  const uint32_t lane = threadIdx.x & 63u;
+  uint32_t total = n;
+  if (count_dev) total = min(*count_dev, n);
+  uint32_t nv = 0, nt = 0, overflow = 0;
+  while (true) {
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(work, 64u);
+    base = __shfl(base, 0);
+    if (base >= total) break;
+    const uint32_t i = base + lane;
+    if (i < total) {
+      const float4 ot = rays[2 * (size_t)i], ds = rays[2 * (size_t)i + 1];
+      float t, u, v;
+      uint32_t id;
+      bool any;
+      traverse<ANY, COUNT, STACK>(S, stk, xyz(ot), xyz(ds), ot.w, __float_as_uint(ds.w),
+                                  leaf_ids != 0, t, id, u, v, any, nv, nt, overflow);
+      if (ANY) {
+        if (occluded) occluded[i] = any ? 1 : 0;
+        if (path_L && !any) {
+          const float4 c = shadow_L[i];
+          const uint32_t pid = __float_as_uint(c.w);
+          float4 L = path_L[pid];
+          L.x += c.x;
+          L.y += c.y;
+          L.z += c.z;
+          path_L[pid] = L;
+        }
+      } else {
+        hits[i] = make_float4(t, __uint_as_float(id), u, v);
+      }
+    }
+  }
+  // wave-uniform condition: every lane of the wave takes part in the shuffles
+  if (COUNT || __any(overflow != 0)) {
+    // wave reduction, one atomic per wave
+    unsigned long long a = nv, b = nt, c = overflow;
+    for (int off = 32; off > 0; off >>= 1) {
+      a += __shfl_xor(a, off);
+      b += __shfl_xor(b, off);
+      c += __shfl_xor(c, off);
+    }
+    if (lane == 0 && counters) {
+      if (COUNT) {
+        atomicAdd(&counters[0], a);
+        atomicAdd(&counters[1], b);
+      }
+      if (c) atomicAdd(&counters[2], c);
+    }
+  }
+}
+
+__global__ void k_zero_u32(uint32_t *p, uint32_t n) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+template <bool ANY, bool COUNT, int STACK>
+hipError_t launch_t(const chk_scene_s *s, const float4 *rays, uint32_t n, const uint32_t *count_dev,
+                    float4 *hits, uint8_t *occ, const float4 *sL, float4 *pL, bool leaf_ids,
+                    uint32_t *work, unsigned long long *counters, int grid, hipStream_t st) {
+  hipLaunchKernelGGL((k_trace<ANY, COUNT, STACK>), dim3(grid), dim3(kBlock), 0, st, s->d, rays, n,
+                     count_dev, hits, occ, sL, pL, leaf_ids ? 1 : 0, work, counters);
+  return hipGetLastError();
+}
+
+template <bool ANY, bool COUNT>
+hipError_t launch_s(const chk_scene_s *s, const float4 *rays, uint32_t n, const uint32_t *count_dev,
+                    float4 *hits, uint8_t *occ, const float4 *sL, float4 *pL, bool leaf_ids,
+                    uint32_t *work, unsigned long long *counters, int grid, hipStream_t st) {
+  switch (s->stack_size) {
+    case 32:
+      return launch_t<ANY, COUNT, 32>(s, rays, n, count_dev, hits, occ, sL, pL, leaf_ids, work,
+                                      counters, grid, st);
+    case 64:
+      return launch_t<ANY, COUNT, 64>(s, rays, n, count_dev, hits, occ, sL, pL, leaf_ids, work,
+                                      counters, grid, st);
+    default:
+      return launch_t<ANY, COUNT, 96>(s, rays, n, count_dev, hits, occ, sL, pL, leaf_ids, work,
+                                      counters, grid, st);
+  }
+}
+
+}  // namespace
+
+int pick_stack_size(uint32_t bound) {
+  if (bound <= 32) return 32;
+  if (bound <= 64) return 64;
+  if (bound <= 96) return 96;
+  return -1;
+}
+
+int trace_grid(const chk_scene_s *s) {
+  // enough persistent blocks for full occupancy; idle ones exit at once
+  int per_cu = s->stack_size <= 32 ? 10 : (s->stack_size <= 64 ? 5 : 3);
+  return s->ctx->n_cu * per_cu;
+}
+
+hipError_t launch_closest(const chk_scene_s *s, const float4 *rays, uint32_t n,
+                          const uint32_t *count_dev, float4 *hits, bool leaf_ids,
+                          uint32_t *work_counter, bool count, unsigned long long *counters,
+                          int grid, hipStream_t st) {
+  if (count)
+    return launch_s<false, true>(s, rays, n, count_dev, hits, nullptr, nullptr, nullptr, leaf_ids,
+                                 work_counter, counters, grid, st);
+  return launch_s<false, false>(s, rays, n, count_dev, hits, nullptr, nullptr, nullptr, leaf_ids,
+                                work_counter, counters, grid, st);
+}
+
+hipError_t launch_any(const chk_scene_s *s, const float4 *rays, uint32_t n,
+                      const uint32_t *count_dev, uint8_t *occluded, const float4 *shadow_L,
+                      float4 *path_L, bool leaf_ids, uint32_t *work_counter, bool count,
+                      unsigned long long *counters, int grid, hipStream_t st) {
+  if (count)
+    return launch_s<true, true>(s, rays, n, count_dev, nullptr, occluded, shadow_L, path_L,
+                                leaf_ids, work_counter, counters, grid, st);
+  return launch_s<true, false>(s, rays, n, count_dev, nullptr, occluded, shadow_L, path_L,
+                               leaf_ids, work_counter, counters, grid, st);
+}
+
+hipError_t launch_zero_u32(uint32_t *p, uint32_t n, hipStream_t st) {
+  hipLaunchKernelGGL(k_zero_u32, dim3((n + 255) / 256), dim3(256), 0, st, p, n);
+  return hipGetLastError();
+}
+
+}  // namespace chk

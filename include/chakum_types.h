@@ -1,0 +1,117 @@
+/*
+ * Plain-C scene records shared by the HIP library (libchakum.so) and its
+ * callers.  Every record is a fixed-layout POD: no torch types, no C++.
+ *
+ * Enumerations keep the reference's numeric values so scene data written
+ * for the reference maps one to one:
+ *   BRDFType  headers/materialTypes.h:7   {BRDF_NONE, CLASSIC_PHONG, GGX, BECKMANN, PHONG, GGXAPPROX}
+ *   LightType headers/materialTypes.h:6   {NONE, POINT, VOLUME, PLANAR, MESH}
+ */
+#ifndef CHAKUM_TYPES_H
+#define CHAKUM_TYPES_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum chk_brdf_type {
+  CHK_BRDF_NONE = 0,
+  CHK_BRDF_CLASSIC_PHONG = 1,
+  CHK_BRDF_GGX = 2,
+  CHK_BRDF_BECKMANN = 3,
+  CHK_BRDF_PHONG = 4,
+  CHK_BRDF_GGXAPPROX = 5
+};
+
+enum chk_light_type {
+  CHK_LIGHT_NONE = 0,
+  CHK_LIGHT_POINT = 1,
+  CHK_LIGHT_VOLUME = 2,
+  CHK_LIGHT_PLANAR = 3,
+  CHK_LIGHT_MESH = 4
+};
+
+/* One material.  Mirrors MaterialType (headers/materialTypes.h:20-120) plus
+ * the per-primitive colour `c` and `reflectance` of BasePrimitive
+ * (headers/geometryPrimitives.h:24-36), which the reference stores beside
+ * the material in every triangle.  56 bytes. */
+typedef struct chk_material {
+  uint32_t brdf;        /* chk_brdf_type */
+  uint32_t light;       /* chk_light_type; != NONE marks an emitter */
+  float ks;             /* specularCoef in [0,1] (materialTypes.h:88-89 clamps it) */
+  float alpha;          /* GGX/Beckmann roughness, or PHONG exponent */
+  float phong_exp;      /* CLASSIC_PHONG exponent */
+  float int_ior;        /* interior IOR */
+  float ext_ior;        /* exterior IOR */
+  float reflectance;    /* BasePrimitive::reflectance */
+  float color[3];       /* BasePrimitive::c */
+  float radiance[3];    /* emitter radiance = col.maxnorm()*intensity (lightSources.h:12-16) */
+} chk_material;
+
+/* A mesh light: a contiguous run of emitter triangles in the scene's
+ * triangle array (MeshLight, headers/lightSources.h:141-153; the reference
+ * appends light triangles to the scene list in LightSource::addMSource,
+ * lightSources.h:197-203). */
+typedef struct chk_mesh_light {
+  uint32_t first_tri;
+  uint32_t n_tris;
+} chk_mesh_light;
+
+/* Pinhole camera with the reference's conventions (prepareScene.cpp:12-36,
+ * main.cpp:72-79): cx = |dir x up| * (w/h) * tan(fovx/2),
+ * cy = |cx x dir| * tan(fovy/2); fov in degrees. */
+typedef struct chk_camera {
+  float pos[3];
+  float dir[3];
+  float up[3];
+  float fovx_deg;
+  float fovy_deg;
+  uint32_t width;
+  uint32_t height;
+} chk_camera;
+
+enum chk_mis_mode {
+  CHK_MIS_BALANCE = 0,      /* NEE + BSDF sampling, balance heuristic (mis.cpp:209-220 with N=1 each) */
+  CHK_MIS_LEARNED = 1       /* learned light selection + mixing weight, defensive mixture */
+};
+
+enum chk_jitter_mode {
+  CHK_JITTER_BOX = 0,       /* uniform sub-pixel jitter per sample */
+  CHK_JITTER_REFERENCE = 1  /* fixed (0.34, 0.20) offset of main.cpp:23-24,77 */
+};
+
+typedef struct chk_render_params {
+  uint32_t spp;             /* samples per pixel */
+  uint32_t sample_offset;   /* first sample index (progressive / resumable renders) */
+  uint32_t max_depth;       /* path vertices with next-event estimation; the BSDF ray
+                               leaving the last one is traced for emission only */
+  uint32_t mis_mode;        /* chk_mis_mode */
+  uint32_t jitter_mode;     /* chk_jitter_mode */
+  uint32_t seed_lo;         /* Philox key */
+  uint32_t seed_hi;
+  uint32_t flags;           /* CHK_RENDER_* bits */
+} chk_render_params;
+
+/* A directly seen emitter returns its radiance (physical) instead of
+ * radiance.maxnorm(), the reference's camera-ray value (materialTypes.h:34-38,
+ * shader.cpp:354-356). */
+#define CHK_RENDER_PHYSICAL_EMITTER 1u
+/* Also accumulate the per-pixel sum of squared per-sample values
+ * (the sumsq buffer receives W*H*3 doubles). */
+#define CHK_RENDER_SUMSQ            2u
+
+/* Philox counter layout shared by every implementation of the path:
+ *   ctr = { pixel index, sample index, (depth << 2) | stream, request id }
+ *   key = { seed_lo, seed_hi }
+ * stream 0: camera jitter (depth 0 only) / next-event estimation,
+ * stream 1: BSDF sampling.  A 32-bit output x maps to (x >> 8) * 2^-24. */
+#define CHK_STREAM_NEE  0u
+#define CHK_STREAM_BSDF 1u
+#define CHK_STREAM_CAM  2u
+
+#ifdef __cplusplus
+}
+#endif
+#endif

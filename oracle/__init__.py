@@ -1,0 +1,240 @@
+"""ORACLE -- test infrastructure only.
+
+ctypes bindings to the FP64 CPU restatement (oracle/liboracle.so, built from
+cpu_ref.cpp) and, where it was built, to the reference's own compiled code
+(oracle/_ref/libchkref.so, built from the reference sources by
+oracle/Makefile).  Only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg import this module, as the checker or the timed CPU
+baseline.  The product package never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+REF_PATH = HERE / "_ref" / "libchkref.so"
+LTC_TABLE = HERE.parent / "chakumrender_amd" / "data" / "ltc_ggx_64.bin"
+
+_dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_fp = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_up = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+_bp = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+_vp = C.c_void_p
+
+
+def build(ref: bool | None = None):
+    """Compile liboracle.so (and the reference library when the reference
+    checkout is present, or when ref=True)."""
+    if ref is None:
+        ref = Path(os.environ.get("CHK_REFERENCE", "/root/reference")).is_dir()
+    target = ["ref"] if ref else ["all"]
+    subprocess.run(["make", "-s", "-C", str(HERE), "-j8"] + target, check=True)
+
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build(ref=False)
+        L = C.CDLL(str(LIB_PATH))
+        L.orc_tri_intersect.restype = C.c_double
+        L.orc_tri_intersect.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp]
+        L.orc_aabb_intersect.restype = C.c_double
+        L.orc_aabb_intersect.argtypes = [_dp, _dp, _dp, _dp]
+        L.orc_closest_brute.argtypes = [_dp, C.c_uint32, _dp, C.c_uint32, _ip, _dp, _dp, _dp, _dp, _dp]
+        L.orc_occluded_brute.argtypes = [_dp, C.c_uint32, _dp, C.c_uint32, _bp]
+        L.orc_bvh_build.restype = _vp
+        L.orc_bvh_build.argtypes = [_dp, C.c_uint32]
+        L.orc_bvh_free.argtypes = [_vp]
+        L.orc_bvh_node_count.restype = C.c_uint32
+        L.orc_bvh_node_count.argtypes = [_vp]
+        L.orc_bvh_ordered.argtypes = [_vp, _up]
+        L.orc_bvh_closest.argtypes = [_vp, _dp, C.c_uint32, C.c_int, _ip, _ip, _dp, _dp, _up, _up]
+        L.orc_brdf_eval.argtypes = [_vp, _up, C.c_uint32, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.orc_brdf_sample.argtypes = [_vp, _up, C.c_uint32, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.orc_onb.argtypes = [_dp, _dp, _dp]
+        L.orc_ltc_set.argtypes = [_dp]
+        L.orc_M_GGX.argtypes = [C.c_float, C.c_float, _dp, _dp, _dp, _dp]
+        L.orc_ltc_shade.argtypes = [C.c_uint32, _dp, _dp, _dp, _dp, _dp, _dp, _dp, C.c_uint32,
+                                    _dp, _dp, _dp, _dp]
+        L.orc_ltc_brdf_cos.argtypes = [C.c_uint32, _dp, _dp, _dp, _dp, _dp]
+        L.orc_philox4x32_10.argtypes = [_up, _up, _up]
+        L.orc_render.restype = C.c_uint64
+        L.orc_render.argtypes = [_fp, C.c_uint32, _up, _vp, C.c_uint32, _vp, C.c_uint32, _vp, _vp,
+                                 C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, _dp, _vp,
+                                 C.POINTER(C.c_uint64)]
+        table = np.fromfile(LTC_TABLE, "<f8")
+        L.orc_ltc_set(np.ascontiguousarray(table))
+        _lib = L
+    return _lib
+
+
+def ref_lib():
+    """The compiled reference (None when oracle/_ref/libchkref.so is absent)."""
+    global _ref
+    if _ref is None and REF_PATH.exists():
+        R = C.CDLL(str(REF_PATH))
+        R.ref_bvh_closest.argtypes = [_dp, C.c_uint32, _dp, C.c_uint32, _ip, _dp, _dp]
+        R.ref_bvh_occluded.argtypes = [_dp, C.c_uint32, _dp, C.c_uint32, _bp]
+        R.ref_brdf.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                               C.c_double, C.c_uint32, _dp, _dp, _dp, _dp, _dp, _dp]
+        R.ref_M_GGX.argtypes = [C.c_float, C.c_float, _dp, _dp, _dp, _dp]
+        R.ref_ltc_shade.argtypes = [C.c_uint32, _dp, _dp, _dp, _dp, _dp, _dp, _dp, C.c_uint32, _dp]
+        _ref = R
+    return _ref
+
+
+def _d(a, shape=None):
+    a = np.ascontiguousarray(a, np.float64)
+    return a if shape is None else a.reshape(shape)
+
+
+# ---------------------------------------------------------------- geometry
+def closest_brute(tris, rays):
+    tris, rays = _d(tris, (-1, 9)), _d(rays, (-1, 6))
+    m = rays.shape[0]
+    out = dict(id=np.empty(m, np.int32), t=np.empty(m), N=np.empty((m, 3)), t2=np.empty(m),
+               margin=np.empty(m), cos2=np.empty(m))
+    lib().orc_closest_brute(tris, tris.shape[0], rays, m, out["id"], out["t"], out["N"],
+                            out["t2"], out["margin"], out["cos2"])
+    return out
+
+
+def occluded_brute(tris, rays7):
+    tris, rays7 = _d(tris, (-1, 9)), _d(rays7, (-1, 7))
+    occ = np.empty(rays7.shape[0], np.uint8)
+    lib().orc_occluded_brute(tris, tris.shape[0], rays7, rays7.shape[0], occ)
+    return occ
+
+
+class RefBvh:
+    """The reference's BVH (midpoint split, 1 primitive per leaf) restated."""
+
+    def __init__(self, tris):
+        self.tris = _d(tris, (-1, 9))
+        self.h = lib().orc_bvh_build(self.tris, self.tris.shape[0])
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_bvh_free(self.h)
+            self.h = None
+
+    @property
+    def node_count(self):
+        return lib().orc_bvh_node_count(self.h)
+
+    def ordered(self):
+        o = np.empty(self.tris.shape[0], np.uint32)
+        lib().orc_bvh_ordered(self.h, o)
+        return o
+
+    def closest(self, rays, prune=False):
+        rays = _d(rays, (-1, 6))
+        m = rays.shape[0]
+        out = dict(id=np.empty(m, np.int32), rid=np.empty(m, np.int32), t=np.empty(m),
+                   N=np.empty((m, 3)), nv=np.empty(m, np.uint32), nt=np.empty(m, np.uint32))
+        lib().orc_bvh_closest(self.h, rays, m, int(prune), out["id"], out["rid"], out["t"],
+                              out["N"], out["nv"], out["nt"])
+        return out
+
+
+# ---------------------------------------------------------------- materials
+def brdf_eval(mats, mat_idx, n, wo_ref, wo, wi):
+    mats = np.ascontiguousarray(mats)
+    mat_idx = np.ascontiguousarray(mat_idx, np.uint32)
+    k = mat_idx.shape[0]
+    f = np.empty(k)
+    pdf = np.empty(k)
+    lib().orc_brdf_eval(mats.ctypes.data, mat_idx, k, _d(n, (-1, 3)), _d(wo_ref, (-1, 3)),
+                        _d(wo, (-1, 3)), _d(wi, (-1, 3)), f, pdf)
+    return f, pdf
+
+
+def brdf_sample(mats, mat_idx, n, wo_ref, wo, u):
+    mats = np.ascontiguousarray(mats)
+    mat_idx = np.ascontiguousarray(mat_idx, np.uint32)
+    k = mat_idx.shape[0]
+    wi = np.empty((k, 3))
+    pdf = np.empty(k)
+    lib().orc_brdf_sample(mats.ctypes.data, mat_idx, k, _d(n, (-1, 3)), _d(wo_ref, (-1, 3)),
+                          _d(wo, (-1, 3)), _d(u, (-1, 3)), wi, pdf)
+    return wi, pdf
+
+
+def onb(n):
+    t = np.empty(3)
+    b = np.empty(3)
+    lib().orc_onb(_d(n), t, b)
+    return t, b
+
+
+# ---------------------------------------------------------------- LTC
+def M_GGX(theta, alpha):
+    M = np.empty(9)
+    Minv = np.empty(9)
+    amp = np.empty(1)
+    det = np.empty(1)
+    lib().orc_M_GGX(float(theta), float(alpha), M, Minv, amp, det)
+    return M, Minv, float(amp[0]), float(det[0])
+
+
+def ltc_shade(x, n, v, alpha, ks, tris, radiance):
+    x, n, v = _d(x, (-1, 3)), _d(n, (-1, 3)), _d(v, (-1, 3))
+    k = x.shape[0]
+    alpha = _d(alpha).reshape(-1)
+    ks = _d(np.broadcast_to(ks, (k,)).copy())
+    tris, radiance = _d(tris, (-1, 9)), _d(radiance, (-1, 3))
+    rgb, ltc, dif, amp = np.empty((k, 3)), np.empty((k, 3)), np.empty((k, 3)), np.empty(k)
+    lib().orc_ltc_shade(k, x, n, v, alpha, ks, tris, radiance, tris.shape[0], rgb, ltc, dif, amp)
+    return dict(rgb=rgb, ltc=ltc, diffuse=dif, amp=amp)
+
+
+def ltc_brdf_cos(n, wo, wi, alpha):
+    n, wo, wi = _d(n, (-1, 3)), _d(wo, (-1, 3)), _d(wi, (-1, 3))
+    k = n.shape[0]
+    out = np.empty(k)
+    lib().orc_ltc_brdf_cos(k, n, wo, wi, _d(alpha).reshape(-1), out)
+    return out
+
+
+# ---------------------------------------------------------------- RNG
+def philox(ctr, key):
+    out = np.empty(4, np.uint32)
+    lib().orc_philox4x32_10(np.ascontiguousarray(ctr, np.uint32), np.ascontiguousarray(key, np.uint32), out)
+    return out
+
+
+# ---------------------------------------------------------------- render
+def render(scene, params, rows=None, threads=0, sumsq=False, row_step=1):
+    """Render rows r0, r0 + row_step, ... < r1 of `scene` with the FP64 estimator.  params: a
+    chakumrender_amd.RenderParams or dict with spp, max_depth, seed ... .
+    Returns (accum (H, W, 3) float64 sums over samples, sumsq or None,
+    paths, rays)."""
+    from chakumrender_amd import abi
+
+    cam = abi.camera_struct(scene.camera)
+    P = abi.params_struct(params)
+    W, H = scene.camera.width, scene.camera.height
+    r0, r1 = (0, H) if rows is None else rows
+    acc = np.zeros((H, W, 3))
+    sq = np.zeros((H, W, 3)) if sumsq else None
+    nr = C.c_uint64(0)
+    mats = np.ascontiguousarray(scene.mats)
+    lights = np.ascontiguousarray(scene.lights)
+    paths = lib().orc_render(np.ascontiguousarray(scene.tris, np.float32), scene.n_tris,
+                             np.ascontiguousarray(scene.tri_mat, np.uint32), mats.ctypes.data,
+                             mats.shape[0], lights.ctypes.data, lights.shape[0], C.addressof(cam),
+                             C.addressof(P), r0, r1, int(row_step), int(threads), acc,
+                             sq.ctypes.data if sq is not None else None, C.byref(nr))
+    return acc, sq, int(paths), int(nr.value)

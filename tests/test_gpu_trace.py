@@ -1,0 +1,146 @@
+"""Closest-hit and any-hit traversal on the GPU (chk_trace_closest /
+chk_trace_any) against the reference's BvhAccel::intersectT and shadow().
+
+Bar: triangle ids bit-exact (original indices) outside the tie/grazing band
+of tests/parity.py, t within 1e-6 relative; occlusion bit-exact outside the
+band around tmax."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle
+from chakumrender_amd import abi, host, scenes
+from parity import classify_closest, oracle_closest_bvh, random_rays, trace_scene
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_closest(ctx, scene, rays):
+    t = host.torch()
+    acc = host.BvhAccel(ctx, scene)
+    r = host.make_rays(rays[:, :3], rays[:, 3:6])
+    n = r.shape[0]
+    rd = host.to_dev(r)
+    hd = host.empty_dev((n, 4), t.float32)
+    acc.trace_closest(C.c_void_p(rd.data_ptr()), n, C.c_void_p(hd.data_ptr()))
+    h = hd.cpu().numpy()
+    info = acc.ds.info()
+    return h[:, 1].view(np.int32).copy(), h[:, 0].astype(np.float64), h[:, 2:], info
+
+
+def check_closest(tris, rays, ref_id, ref_t, gid, gt, ref_t2=None, max_band=0.005):
+    band, mism = classify_closest(tris, rays, ref_id, ref_t, gid, gt, ref_t2)
+    print(f"rays {rays.shape[0]}, hits {np.mean(ref_id >= 0):.3f}, excluded {band.sum()} "
+          f"({band.mean():.2e}), mismatches {mism.sum()}")
+    assert mism.sum() == 0, np.nonzero(mism)[0][:10]
+    assert band.mean() < max_band
+    same = (~band) & (ref_id >= 0)
+    rel = np.abs(gt[same] - ref_t[same]) / ref_t[same]
+    assert rel.max() < 1e-6, rel.max()
+    miss = (~band) & (ref_id < 0)
+    assert np.all(gt[miss] >= 1e29)
+
+
+@pytest.mark.parametrize("name", ["cornell", "ico"])
+def test_closest_matches_reference_golden(ctx, golden, name):
+    G = golden("bvh_closest.npz")
+    tris, rays = G[f"{name}_tris"], G[f"{name}_rays"]
+    gid, gt, _, info = gpu_closest(ctx, trace_scene(tris), rays)
+    # the reference's own t / id; the band needs the second-nearest t (FP64 oracle)
+    b = oracle.closest_brute(tris.astype(np.float64), rays)
+    assert np.array_equal(b["id"], G[f"{name}_id"])
+    check_closest(tris, rays, G[f"{name}_id"], G[f"{name}_t"], gid, gt, b["t2"])
+    assert info["n_tris_bvh"] == tris.shape[0] - 1  # the degenerate dummy triangle is dropped
+
+
+def test_closest_100k_icosphere_vs_reference_bvh(ctx):
+    sc = scenes.icosphere_scene(freq=71, n_lights=8, width=64, height=64)
+    assert sc.n_tris == 100_820 + 2 + 1 + 16
+    rng = np.random.default_rng(11)
+    rays = random_rays(sc, rng, 1 << 16)
+    # camera rays too
+    pos, d, cx, cy = sc.camera.basis()
+    u = rng.uniform(-0.5, 0.5, (1 << 15, 2))
+    cd = cx * u[:, :1] + cy * u[:, 1:] + d
+    cd /= np.linalg.norm(cd, axis=1, keepdims=True)
+    cam = np.concatenate([np.repeat(pos[None], u.shape[0], 0), cd], 1).astype(np.float32).astype(np.float64)
+    rays = np.concatenate([rays, cam])
+    gid, gt, _, info = gpu_closest(ctx, sc, rays)
+    rid, rt = oracle_closest_bvh(sc.tris, rays)
+    check_closest(sc.tris, rays, rid, rt, gid, gt)
+    assert info["stack_bound"] <= info["stack_size"]
+
+
+def test_skip_and_tmax(ctx, golden):
+    """tmax bounds the reported hit; skip removes one original triangle."""
+    G = golden("bvh_closest.npz")
+    tris, rays = G["cornell_tris"], G["cornell_rays"][:2048]
+    t = host.torch()
+    acc = host.BvhAccel(ctx, trace_scene(tris))
+    ref = oracle.closest_brute(tris.astype(np.float64), rays)
+    hit = ref["id"] >= 0
+    skip = np.where(hit, ref["id"], -1)
+    r = host.make_rays(rays[:, :3], rays[:, 3:6], skip=skip)
+    rd = host.to_dev(r)
+    hd = host.empty_dev((r.shape[0], 4), t.float32)
+    acc.trace_closest(C.c_void_p(rd.data_ptr()), r.shape[0], C.c_void_p(hd.data_ptr()))
+    h = hd.cpu().numpy()
+    gid = h[:, 1].view(np.int32)
+    assert not np.any(gid[hit] == skip[hit])
+    # tmax just below the first hit: nothing is reported
+    r2 = host.make_rays(rays[:, :3], rays[:, 3:6], tmax=np.where(hit, ref["t"] * 0.999, 1e30))
+    rd2 = host.to_dev(r2)
+    acc.trace_closest(C.c_void_p(rd2.data_ptr()), r2.shape[0], C.c_void_p(hd.data_ptr()))
+    g2 = hd.cpu().numpy()[:, 1].view(np.int32)
+    assert np.all(g2[hit] == -1)
+
+
+def test_any_hit_matches_reference_shadow(ctx, golden):
+    S = golden("shadow.npz")
+    tris, r7 = S["tris"], S["rays7"]
+    acc = host.BvhAccel(ctx, trace_scene(tris))
+    vis = acc.shadow(r7[:, :3], r7[:, 3:6], r7[:, 6])
+    occ = (vis < 0.5).astype(np.uint8)
+    # band: the closest FP64 hit within 1e-5 of the shadow limit, or grazing
+    b = oracle.closest_brute(tris.astype(np.float64), r7[:, :6])
+    lim = r7[:, 6] - 1e-5
+    band = (b["id"] >= 0) & ((np.abs(b["t"] - lim) <= 1e-5 * np.maximum(1, lim)) | (b["margin"] < 1e-5)
+                             | (b["cos2"] < 1.001e-10))
+    mism = (~band) & (occ != S["occluded"])
+    print(f"shadow rays {r7.shape[0]}, occluded {S['occluded'].mean():.3f}, excluded {band.sum()}")
+    assert mism.sum() == 0, np.nonzero(mism)[0][:10]
+
+
+def test_any_hit_consistent_with_closest_100k(ctx):
+    """Occlusion with tmax just above / below the closest hit is 1 / 0."""
+    sc = scenes.icosphere_scene(freq=71, n_lights=8, width=64, height=64)
+    rays = random_rays(sc, np.random.default_rng(5), 1 << 15)
+    gid, gt, _, _ = gpu_closest(ctx, sc, rays)
+    acc = host.BvhAccel(ctx, sc)
+    hit = gid >= 0
+    above = acc.shadow(rays[:, :3], rays[:, 3:6], np.where(hit, gt * 1.0001 + 1e-5, 1e6))
+    below = acc.shadow(rays[:, :3], rays[:, 3:6], np.where(hit, gt * 0.9999 + 1e-5, 1e6))
+    assert np.all(above[hit] == 0.0)
+    assert np.all(below[hit] == 1.0)
+    assert np.all(above[~hit] == 1.0)
+
+
+def test_empty_and_degenerate_inputs(ctx):
+    t = host.torch()
+    # zero rays is a no-op
+    acc = host.BvhAccel(ctx, trace_scene(np.array([[0, 0, 0, 1, 0, 0, 0, 1, 0]], np.float32)))
+    abi.check(acc.L.chk_trace_closest(acc.ds.h, None, 0, None, None), "zero rays")
+    # a scene of only degenerate triangles: every ray misses
+    deg = np.tile(np.array([[1e6, 1e6, 1e6] * 3], np.float32), (3, 1))
+    acc2 = host.BvhAccel(ctx, trace_scene(deg))
+    hit, tt, ids = acc2.intersectT(np.zeros((4, 3)), np.tile([[0, 0, 1.0]], (4, 1)))
+    assert not hit.any() and np.all(ids == -1)
+    assert acc2.ds.info()["n_tris_bvh"] == 0
+    # a ray along a triangle's plane (d.n = 0) never hits it (geometryPrimitives.cpp:42)
+    tri = np.array([[0, 0, 0, 1, 0, 0, 0, 1, 0]], np.float32)
+    acc3 = host.BvhAccel(ctx, trace_scene(tri))
+    hit, _, _ = acc3.intersectT([[-1, 0.2, 0.0]], [[1, 0, 0]])
+    assert not hit.any()
+    hit, tt, ids = acc3.intersectT([[0.2, 0.2, 1.0]], [[0, 0, -1]])
+    assert hit.all() and ids[0] == 0 and abs(tt[0] - 1.0) < 1e-6
