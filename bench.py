@@ -40,8 +40,8 @@ def parse():
     ap.add_argument("--depth", type=int, default=6)
     ap.add_argument("--freq", type=int, default=71, help="geodesic frequency (71 -> 100,820 triangles)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-row-step", type=int, default=17)
-    ap.add_argument("--cpu-spp", type=int, default=16)
+    ap.add_argument("--cpu-row-step", type=int, default=8)
+    ap.add_argument("--cpu-spp", type=int, default=64)
     ap.add_argument("--traffic", type=str, default=None,
                     help="JSON file with measured HBM bytes per closest-hit launch")
     return ap.parse_args()
@@ -55,9 +55,6 @@ def main():
                                    "(a render shards by pixel or sample range, see DESIGN.md)"}))
         sys.exit(2)
 
-    import numpy as np
-    import torch
-
     from chakumrender_amd import abi, host, scenes
 
     t0 = time.time()
@@ -66,30 +63,30 @@ def main():
     ds = abi.DeviceScene(ctx, sc)
     info = ds.info()
     params = dict(spp=args.spp, max_depth=args.depth, seed=0x5EED0002)
-    accum = torch.zeros((args.height, args.width, 3), dtype=torch.float64, device="cuda")
+    image = host.Image(args.width, args.height)
     print(f"# scene {sc.n_tris} tris, {info['n_nodes']} nodes, stack {info['stack_size']}, "
           f"build {info['build_ms']:.0f} ms, setup {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
 
     # work model: node visits / triangle tests of this exact workload (untimed)
-    _, _, cnt = host.render(ds, params, accum=accum, count=True)
+    _, cnt = host.render(ds, params, image=image, count=True, download=False)
     print(f"# counted: {cnt['rays_extend']} extend rays, {cnt['rays_shadow']} shadow rays, "
           f"{cnt['node_visits']} node visits, {cnt['tri_tests']} tri tests", file=sys.stderr, flush=True)
 
     prof = dict(params, flags=abi.RENDER_PROFILE)
     for _ in range(args.warmup):
-        host.render(ds, params, accum=accum, stats=False)
-    torch.cuda.synchronize()
+        host.render(ds, params, image=image, stats=False, download=False)
+    abi.sync()
     ext_ms = shade_ms = shadow_ms = other_ms = 0.0
     launches = 0
     t_start = time.perf_counter()
     for _ in range(args.steps):
-        _, _, st = host.render(ds, prof, accum=accum)
+        _, st = host.render(ds, prof, image=image, download=False)
         ext_ms += st["extend_ms"]
         shade_ms += st["shade_ms"]
         shadow_ms += st["shadow_ms"]
         other_ms += st["other_ms"]
         launches += st["extend_launches"]
-    torch.cuda.synchronize()
+    abi.sync()
     elapsed = time.perf_counter() - t_start
 
     paths = args.width * args.height * args.spp
@@ -100,10 +97,9 @@ def main():
 
     # roofline of the closest-hit kernel: algorithmic bytes per launch over
     # its average launch duration (HIP events on the render stream)
-    per_launch_launches = cnt and st["extend_launches"]
     alg_bytes_total = (cnt["node_visits"] * NODE_BYTES + cnt["tri_tests"] * TRI_BYTES
                        + cnt["rays_extend"] * RAY_BYTES)
-    alg_bytes_launch = alg_bytes_total / per_launch_launches
+    alg_bytes_launch = alg_bytes_total / cnt["extend_launches"]
     avg_launch_s = ext_ms / 1e3 / launches
     achieved = alg_bytes_launch / avg_launch_s / 1e9
     traffic = None

@@ -76,6 +76,13 @@ SIGNATURES = {
     "chk_status_string": (C.c_char_p, [C.c_int]),
     "chk_last_error": (C.c_char_p, []),
     "chk_version": (C.c_char_p, []),
+    "chk_device_alloc": (C.c_int, [C.c_int, C.c_size_t, C.POINTER(_vp)]),
+    "chk_device_free": (C.c_int, [_vp]),
+    "chk_memcpy": (C.c_int, [_vp, _vp, C.c_size_t, C.c_int, _vp]),
+    "chk_memzero": (C.c_int, [_vp, C.c_size_t, _vp]),
+    "chk_stream_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
+    "chk_stream_destroy": (C.c_int, [_vp]),
+    "chk_stream_sync": (C.c_int, [_vp]),
     "chk_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
     "chk_destroy": (C.c_int, [_vp]),
     "chk_set_ltc_table": (C.c_int, [_vp, _vp, C.c_size_t]),
@@ -149,6 +156,64 @@ def params_struct(p) -> chk_render_params:
     r.seed_hi = (seed >> 32) & 0xFFFFFFFF
     r.flags = int(d.get("flags", 0))
     return r
+
+
+class DevBuf:
+    """Device allocation made by libchakum's own HIP runtime."""
+
+    def __init__(self, nbytes: int, device: int = 0):
+        self.L = load()
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        check(self.L.chk_device_alloc(device, self.nbytes, C.byref(p)), "chk_device_alloc")
+        self.p = p
+
+    @property
+    def ptr(self):
+        return self.p
+
+    @classmethod
+    def from_array(cls, a, device: int = 0):
+        a = np.ascontiguousarray(a)
+        b = cls(max(a.nbytes, 16), device)
+        b.upload(a)
+        return b
+
+    @classmethod
+    def zeros(cls, nbytes: int, device: int = 0):
+        b = cls(max(int(nbytes), 16), device)
+        b.zero()
+        return b
+
+    def upload(self, a, stream=None):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        check(self.L.chk_memcpy(self.p, a.ctypes.data, a.nbytes, 0, stream), "chk_memcpy h2d")
+        check(self.L.chk_stream_sync(stream), "chk_stream_sync")
+
+    def download(self, shape, dtype, stream=None):
+        out = np.empty(shape, dtype)
+        assert out.nbytes <= self.nbytes
+        check(self.L.chk_memcpy(out.ctypes.data, self.p, out.nbytes, 1, stream), "chk_memcpy d2h")
+        return out
+
+    def zero(self, stream=None):
+        check(self.L.chk_memzero(self.p, self.nbytes, stream), "chk_memzero")
+
+    def free(self):
+        if getattr(self, "p", None):
+            self.L.chk_device_free(self.p)
+            self.p = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def sync(stream=None):
+    check(load().chk_stream_sync(stream), "chk_stream_sync")
 
 
 class Context:

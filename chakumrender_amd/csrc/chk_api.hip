@@ -71,6 +71,13 @@ __global__ void k_brdf_sample(const chk_material *__restrict__ mats, uint32_t n_
 
 inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
+__global__ void k_fill_zero(uint4 *p, size_t n16, unsigned char *tail, size_t ntail) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t k = i; k < n16; k += stride) p[k] = make_uint4(0u, 0u, 0u, 0u);
+  if (i < ntail) tail[i] = 0;
+}
+
 }  // namespace
 }  // namespace chk
 
@@ -109,6 +116,80 @@ const char *chk_status_string(chk_status s) {
 const char *chk_last_error(void) { return g_err.c_str(); }
 
 const char *chk_version(void) { return "chakum-mi355x 0.1 (gfx950)"; }
+
+chk_status chk_device_alloc(int device, size_t bytes, void **dev) {
+  if (!dev) {
+    set_error("chk_device_alloc: null out");
+    return CHK_EINVAL;
+  }
+  *dev = nullptr;
+  CHK_HIP(hipSetDevice(device));
+  if (bytes == 0) return CHK_OK;
+  if (hipMalloc(dev, bytes) != hipSuccess) {
+    *dev = nullptr;
+    set_error("chk_device_alloc: cannot allocate " + std::to_string(bytes) + " bytes");
+    return CHK_ENOMEM;
+  }
+  return CHK_OK;
+}
+
+chk_status chk_device_free(void *dev) {
+  if (dev) CHK_HIP(hipFree(dev));
+  return CHK_OK;
+}
+
+chk_status chk_memcpy(void *dst, const void *src, size_t bytes, int kind, void *stream) {
+  if (bytes == 0) return CHK_OK;
+  if (!dst || !src || kind < 0 || kind > 2) {
+    set_error("chk_memcpy: null pointer or bad kind");
+    return CHK_EINVAL;
+  }
+  const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice
+                                    : (kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice);
+  CHK_HIP(hipMemcpyAsync(dst, src, bytes, k, (hipStream_t)stream));
+  if (kind == 1) CHK_HIP(hipStreamSynchronize((hipStream_t)stream));
+  return CHK_OK;
+}
+
+chk_status chk_memzero(void *dev, size_t bytes, void *stream) {
+  if (bytes == 0) return CHK_OK;
+  if (!dev) {
+    set_error("chk_memzero: null pointer");
+    return CHK_EINVAL;
+  }
+  if (((uintptr_t)dev & 15u) != 0) {
+    set_error("chk_memzero: pointer must be 16-byte aligned");
+    return CHK_EINVAL;
+  }
+  const size_t n16 = bytes / 16, nt = bytes % 16;
+  const unsigned blocks = (unsigned)std::min<size_t>(std::max<size_t>((n16 + 255) / 256, 1), 8192);
+  hipLaunchKernelGGL(k_fill_zero, dim3(blocks), dim3(256), 0, (hipStream_t)stream, (uint4 *)dev,
+                     n16, (unsigned char *)dev + 16 * n16, nt);
+  CHK_HIP(hipGetLastError());
+  return CHK_OK;
+}
+
+chk_status chk_stream_create(int device, void **stream) {
+  if (!stream) {
+    set_error("chk_stream_create: null out");
+    return CHK_EINVAL;
+  }
+  CHK_HIP(hipSetDevice(device));
+  hipStream_t s;
+  CHK_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  *stream = (void *)s;
+  return CHK_OK;
+}
+
+chk_status chk_stream_destroy(void *stream) {
+  if (stream) CHK_HIP(hipStreamDestroy((hipStream_t)stream));
+  return CHK_OK;
+}
+
+chk_status chk_stream_sync(void *stream) {
+  CHK_HIP(hipStreamSynchronize((hipStream_t)stream));
+  return CHK_OK;
+}
 
 chk_status chk_create(int device, chk_ctx_t *out) {
   CHK_TRY_BEGIN

@@ -2,9 +2,10 @@
 C ABI, batched.  Names and argument meanings follow the reference:
 
   BvhAccel.intersectT   bvhAccel.cpp:200-249   closest hit (t, N, id)
-  shadow                shader.cpp:76-97        1.0 if unoccluded else 0.0
-  MaterialType.brdf / pdfEval / getBrdfDirectionSampleRandomized
-                        materialTypes.cpp:9-153
+  BvhAccel.shadow       shader.cpp:76-97        1.0 if unoccluded else 0.0
+  MaterialType.brdf_pdf / sample
+                        materialTypes.cpp:9-153 (brdf, pdfEval,
+                        getBrdfDirectionSampleRandomized)
   LightSource.getLightFromMeshSource_Analytic
                         lightSources_CV.cpp:323-333
   render                main.cpp:62-94 around shade() (shader.cpp:362)
@@ -12,8 +13,9 @@ C ABI, batched.  Names and argument meanings follow the reference:
 
 Differences a caller sees: rays are batched numpy arrays; ids are ORIGINAL
 triangle indices (the reference returns indices into its reordered array);
-random numbers for sampling are explicit inputs.  Device memory comes from
-torch (allocation only); every computation runs in libchakum.so.
+random numbers for sampling are explicit inputs.  Device memory is
+allocated through libchakum itself (abi.DevBuf); every computation runs in
+libchakum.so.
 """
 from __future__ import annotations
 
@@ -23,33 +25,7 @@ import numpy as np
 
 from . import abi
 
-_torch = None
-
-
-def torch():
-    global _torch
-    if _torch is None:
-        import torch as t
-
-        if not t.cuda.is_available():
-            raise abi.ChkError("no GPU visible: libchakum needs a HIP device")
-        _torch = t
-    return _torch
-
-
-def to_dev(a, dtype=None):
-    t = torch()
-    a = np.ascontiguousarray(a if dtype is None else np.asarray(a, dtype))
-    return t.from_numpy(a).cuda()
-
-
-def empty_dev(shape, dtype):
-    t = torch()
-    return t.empty(shape, dtype=dtype, device="cuda")
-
-
-def stream_ptr():
-    return C.c_void_p(torch().cuda.current_stream().cuda_stream)
+DevBuf = abi.DevBuf
 
 
 def make_rays(o, d, tmax=None, skip=None):
@@ -74,37 +50,38 @@ class BvhAccel:
         self.L = ctx.L
 
     def trace_closest(self, rays_dev, n, hits_dev, stream=None):
-        abi.check(self.L.chk_trace_closest(self.ds.h, rays_dev, n, hits_dev,
-                                           stream if stream is not None else stream_ptr()),
+        abi.check(self.L.chk_trace_closest(self.ds.h, rays_dev, n, hits_dev, stream),
                   "chk_trace_closest")
 
     def trace_any(self, rays_dev, n, occ_dev, stream=None):
-        abi.check(self.L.chk_trace_any(self.ds.h, rays_dev, n, occ_dev,
-                                       stream if stream is not None else stream_ptr()),
-                  "chk_trace_any")
+        abi.check(self.L.chk_trace_any(self.ds.h, rays_dev, n, occ_dev, stream), "chk_trace_any")
+
+    def closest_raw(self, rays):
+        """rays: (n, 8) chk_ray records -> (n, 4) hit records (t, tri bits, u, v)."""
+        n = rays.shape[0]
+        rd = DevBuf.from_array(np.asarray(rays, np.float32))
+        hd = DevBuf(16 * n)
+        self.trace_closest(rd.ptr, n, hd.ptr)
+        return hd.download((n, 4), np.float32)
 
     def intersectT(self, o, d):
-        """Closest hit for rays (o, d): returns (hit bool, t, id) numpy arrays;
-        t = 1e30 and id = -1 on a miss (the reference: INF, 0xFFFFFFFF)."""
-        t = torch()
-        rays = make_rays(o, d)
+        """Closest hit for rays (o, d): (hit bool, t, id) arrays; t = 1e30 and
+        id = -1 on a miss (the reference: INF, 0xFFFFFFFF)."""
+        h = self.closest_raw(make_rays(o, d))
+        tri = h[:, 1].view(np.int32).copy()
+        return tri >= 0, h[:, 0].astype(np.float64), tri
+
+    def occluded_raw(self, rays):
         n = rays.shape[0]
-        rd = to_dev(rays)
-        hd = empty_dev((n, 4), t.float32)
-        self.trace_closest(C.c_void_p(rd.data_ptr()), n, C.c_void_p(hd.data_ptr()))
-        h = hd.cpu().numpy()
-        tri = h[:, 1].view(np.int32)
-        return tri >= 0, h[:, 0], tri
+        rd = DevBuf.from_array(np.asarray(rays, np.float32))
+        od = DevBuf(max(n, 16))
+        self.trace_any(rd.ptr, n, od.ptr)
+        return od.download((n,), np.uint8)
 
     def shadow(self, o, d, dist):
         """shader.cpp:76-97: 0.0 if some triangle is hit with t + 1e-5 < dist."""
-        t = torch()
-        rays = make_rays(o, d, tmax=np.asarray(dist, np.float32) - np.float32(1e-5))
-        n = rays.shape[0]
-        rd = to_dev(rays)
-        od = empty_dev((n,), t.uint8)
-        self.trace_any(C.c_void_p(rd.data_ptr()), n, C.c_void_p(od.data_ptr()))
-        return 1.0 - od.cpu().numpy().astype(np.float64)
+        occ = self.occluded_raw(make_rays(o, d, tmax=np.asarray(dist, np.float32) - np.float32(1e-5)))
+        return 1.0 - occ.astype(np.float64)
 
 
 class MaterialType:
@@ -114,36 +91,29 @@ class MaterialType:
         self.mats = np.ascontiguousarray(mats)
         self.L = abi.load()
 
-    def _args(self, mat_idx, *vecs):
-        t = torch()
-        md = to_dev(self.mats.view(np.uint8))
-        idx = to_dev(np.asarray(mat_idx, np.uint32).view(np.int32))
-        vs = [to_dev(np.asarray(v, np.float32).reshape(-1, 3)) for v in vecs]
-        return t, md, idx, vs
+    def _bufs(self, mat_idx, *vecs):
+        md = DevBuf.from_array(self.mats.view(np.uint8))
+        idx = DevBuf.from_array(np.asarray(mat_idx, np.uint32))
+        vs = [DevBuf.from_array(np.asarray(v, np.float32).reshape(-1, 3)) for v in vecs]
+        return md, idx, vs
 
     def brdf_pdf(self, mat_idx, n, wo_ref, wo, wi):
         """(BasePrimitive::brdf, MaterialType::pdfEval) per row."""
-        t, md, idx, (nd, rd, od, wd) = self._args(mat_idx, n, wo_ref, wo, wi)
-        k = idx.shape[0]
-        f = empty_dev((k,), t.float32)
-        p = empty_dev((k,), t.float32)
-        abi.check(self.L.chk_brdf_eval(md.data_ptr(), self.mats.shape[0], idx.data_ptr(),
-                                       nd.data_ptr(), rd.data_ptr(), od.data_ptr(), wd.data_ptr(),
-                                       k, f.data_ptr(), p.data_ptr(), stream_ptr()),
-                  "chk_brdf_eval")
-        return f.cpu().numpy(), p.cpu().numpy()
+        k = len(mat_idx)
+        md, idx, (nd, rd, od, wd) = self._bufs(mat_idx, n, wo_ref, wo, wi)
+        f, p = DevBuf(4 * k), DevBuf(4 * k)
+        abi.check(self.L.chk_brdf_eval(md.ptr, self.mats.shape[0], idx.ptr, nd.ptr, rd.ptr, od.ptr,
+                                       wd.ptr, k, f.ptr, p.ptr, None), "chk_brdf_eval")
+        return f.download((k,), np.float32), p.download((k,), np.float32)
 
     def sample(self, mat_idx, n, wo_ref, wo, u):
         """getBrdfDirectionSampleRandomized with explicit uniforms u (k, 3)."""
-        t, md, idx, (nd, rd, od, ud) = self._args(mat_idx, n, wo_ref, wo, u)
-        k = idx.shape[0]
-        wi = empty_dev((k, 3), t.float32)
-        p = empty_dev((k,), t.float32)
-        abi.check(self.L.chk_brdf_sample(md.data_ptr(), self.mats.shape[0], idx.data_ptr(),
-                                         nd.data_ptr(), rd.data_ptr(), od.data_ptr(), ud.data_ptr(),
-                                         k, wi.data_ptr(), p.data_ptr(), stream_ptr()),
-                  "chk_brdf_sample")
-        return wi.cpu().numpy(), p.cpu().numpy()
+        k = len(mat_idx)
+        md, idx, (nd, rd, od, ud) = self._bufs(mat_idx, n, wo_ref, wo, u)
+        wi, p = DevBuf(12 * k), DevBuf(4 * k)
+        abi.check(self.L.chk_brdf_sample(md.ptr, self.mats.shape[0], idx.ptr, nd.ptr, rd.ptr, od.ptr,
+                                         ud.ptr, k, wi.ptr, p.ptr, None), "chk_brdf_sample")
+        return wi.download((k, 3), np.float32), p.download((k,), np.float32)
 
 
 class LightSource:
@@ -152,44 +122,65 @@ class LightSource:
     def __init__(self, ctx: abi.Context, light_tris, radiance):
         self.ctx = ctx
         self.L = ctx.L
-        self.tris = to_dev(np.asarray(light_tris, np.float32).reshape(-1, 9))
-        self.rad = to_dev(np.asarray(radiance, np.float32).reshape(-1, 3))
+        self.n = int(np.asarray(light_tris).reshape(-1, 9).shape[0])
+        self.tris = DevBuf.from_array(np.asarray(light_tris, np.float32).reshape(-1, 9))
+        self.rad = DevBuf.from_array(np.asarray(radiance, np.float32).reshape(-1, 3))
 
-    def getLightFromMeshSource_Analytic(self, x, n, v, alpha, ks, out=None):
-        t = torch()
-        xs = [a if hasattr(a, "data_ptr") else to_dev(np.asarray(a, np.float32))
-              for a in (x, n, v, alpha, ks)]
-        k = xs[0].shape[0]
-        rgb = out if out is not None else empty_dev((k, 3), t.float32)
-        abi.check(self.L.chk_ltc_eval(self.ctx.h, *[a.data_ptr() for a in xs], k,
-                                      self.tris.data_ptr(), self.rad.data_ptr(),
-                                      self.tris.shape[0], rgb.data_ptr(), stream_ptr()),
-                  "chk_ltc_eval")
-        return rgb
+    def eval_dev(self, bufs, k, rgb, stream=None):
+        """bufs: DevBufs (x, n, v, alpha, ks); rgb: DevBuf (k, 3)."""
+        abi.check(self.L.chk_ltc_eval(self.ctx.h, *[b.ptr for b in bufs], k, self.tris.ptr,
+                                      self.rad.ptr, self.n, rgb.ptr, stream), "chk_ltc_eval")
+
+    def getLightFromMeshSource_Analytic(self, x, n, v, alpha, ks):
+        arrs = [np.asarray(a, np.float32) for a in (x, n, v, alpha, ks)]
+        k = arrs[0].reshape(-1, 3).shape[0]
+        bufs = [DevBuf.from_array(a) for a in arrs]
+        rgb = DevBuf(12 * k)
+        self.eval_dev(bufs, k, rgb)
+        return rgb.download((k, 3), np.float32)
 
 
-def render(ds: abi.DeviceScene, params, camera=None, accum=None, sumsq=None, stats=True,
-           count=False):
-    """Render into an FP64 accumulator (H, W, 3) on the GPU; returns
-    (accum tensor, sumsq tensor or None, stats dict)."""
-    t = torch()
+class Image:
+    """FP64 accumulator (H, W, 3) (+ optional sum of squares) in HBM."""
+
+    def __init__(self, width, height, sumsq=False):
+        self.W, self.H = int(width), int(height)
+        nb = 8 * 3 * self.W * self.H
+        self.acc = DevBuf.zeros(nb)
+        self.sq = DevBuf.zeros(nb) if sumsq else None
+
+    def zero(self):
+        self.acc.zero()
+        if self.sq is not None:
+            self.sq.zero()
+
+    def get(self):
+        a = self.acc.download((self.H, self.W, 3), np.float64)
+        s = self.sq.download((self.H, self.W, 3), np.float64) if self.sq is not None else None
+        return a, s
+
+
+def render(ds: abi.DeviceScene, params, camera=None, image: Image | None = None, stats=True,
+           count=False, stream=None, download=True):
+    """Render into an FP64 image accumulator; returns (accum, sumsq or None,
+    stats dict) as numpy arrays (or the Image itself when download=False)."""
     cam = abi.camera_struct(camera or ds.scene.camera)
     P = abi.params_struct(params)
-    H, W = cam.height, cam.width
-    if accum is None:
-        accum = t.zeros((H, W, 3), dtype=t.float64, device="cuda")
-    if (P.flags & abi.RENDER_SUMSQ) and sumsq is None:
-        sumsq = t.zeros((H, W, 3), dtype=t.float64, device="cuda")
+    if image is None:
+        image = Image(cam.width, cam.height, sumsq=bool(P.flags & abi.RENDER_SUMSQ))
     st = abi.chk_render_stats()
     if count:
-        abi.check(ds.L.chk_render_count(ds.h, C.byref(cam), C.byref(P), accum.data_ptr(),
-                                        C.byref(st), stream_ptr()), "chk_render_count")
+        abi.check(ds.L.chk_render_count(ds.h, C.byref(cam), C.byref(P), image.acc.ptr, C.byref(st),
+                                        stream), "chk_render_count")
     else:
-        abi.check(ds.L.chk_render(ds.h, C.byref(cam), C.byref(P), accum.data_ptr(),
-                                  sumsq.data_ptr() if sumsq is not None else None,
-                                  C.byref(st) if stats else None, stream_ptr()), "chk_render")
+        abi.check(ds.L.chk_render(ds.h, C.byref(cam), C.byref(P), image.acc.ptr,
+                                  image.sq.ptr if image.sq is not None else None,
+                                  C.byref(st) if stats else None, stream), "chk_render")
     sd = {k: getattr(st, k) for k, _ in abi.chk_render_stats._fields_} if (stats or count) else {}
-    return accum, sumsq, sd
+    if not download:
+        return image, sd
+    a, s = image.get()
+    return a, s, sd
 
 
 def to_display(x):

@@ -98,6 +98,19 @@ const char *chk_last_error(void);
 /* Library version string. */
 const char *chk_version(void);
 
+/* Device memory and streams through the library's own HIP runtime, so a
+ * caller needs no second runtime in its process (no reference counterpart:
+ * the reference is host-only).  kind: 0 host->device, 1 device->host,
+ * 2 device->device.  chk_memzero clears with a fill kernel (safe to record
+ * in a HIP graph).  A NULL stream is the default stream. */
+chk_status chk_device_alloc(int device, size_t bytes, void **dev);
+chk_status chk_device_free(void *dev);
+chk_status chk_memcpy(void *dst, const void *src, size_t bytes, int kind, void *stream);
+chk_status chk_memzero(void *dev, size_t bytes, void *stream);
+chk_status chk_stream_create(int device, void **stream);
+chk_status chk_stream_destroy(void *stream);
+chk_status chk_stream_sync(void *stream);
+
 /* Context: one per device.  Replaces the reference's process globals
  * (objects.h:7-17, bvhAccel.h:78, lightSources.h:251, mis.h:39). */
 chk_status chk_create(int device, chk_ctx_t *out);

@@ -17,14 +17,8 @@ pytestmark = pytest.mark.gpu
 
 
 def gpu_closest(ctx, scene, rays):
-    t = host.torch()
     acc = host.BvhAccel(ctx, scene)
-    r = host.make_rays(rays[:, :3], rays[:, 3:6])
-    n = r.shape[0]
-    rd = host.to_dev(r)
-    hd = host.empty_dev((n, 4), t.float32)
-    acc.trace_closest(C.c_void_p(rd.data_ptr()), n, C.c_void_p(hd.data_ptr()))
-    h = hd.cpu().numpy()
+    h = acc.closest_raw(host.make_rays(rays[:, :3], rays[:, 3:6]))
     info = acc.ds.info()
     return h[:, 1].view(np.int32).copy(), h[:, 0].astype(np.float64), h[:, 2:], info
 
@@ -76,23 +70,17 @@ def test_skip_and_tmax(ctx, golden):
     """tmax bounds the reported hit; skip removes one original triangle."""
     G = golden("bvh_closest.npz")
     tris, rays = G["cornell_tris"], G["cornell_rays"][:2048]
-    t = host.torch()
     acc = host.BvhAccel(ctx, trace_scene(tris))
     ref = oracle.closest_brute(tris.astype(np.float64), rays)
     hit = ref["id"] >= 0
     skip = np.where(hit, ref["id"], -1)
-    r = host.make_rays(rays[:, :3], rays[:, 3:6], skip=skip)
-    rd = host.to_dev(r)
-    hd = host.empty_dev((r.shape[0], 4), t.float32)
-    acc.trace_closest(C.c_void_p(rd.data_ptr()), r.shape[0], C.c_void_p(hd.data_ptr()))
-    h = hd.cpu().numpy()
+    h = acc.closest_raw(host.make_rays(rays[:, :3], rays[:, 3:6], skip=skip))
     gid = h[:, 1].view(np.int32)
     assert not np.any(gid[hit] == skip[hit])
     # tmax just below the first hit: nothing is reported
-    r2 = host.make_rays(rays[:, :3], rays[:, 3:6], tmax=np.where(hit, ref["t"] * 0.999, 1e30))
-    rd2 = host.to_dev(r2)
-    acc.trace_closest(C.c_void_p(rd2.data_ptr()), r2.shape[0], C.c_void_p(hd.data_ptr()))
-    g2 = hd.cpu().numpy()[:, 1].view(np.int32)
+    h2 = acc.closest_raw(host.make_rays(rays[:, :3], rays[:, 3:6],
+                                        tmax=np.where(hit, ref["t"] * 0.999, 1e30)))
+    g2 = h2[:, 1].view(np.int32)
     assert np.all(g2[hit] == -1)
 
 
@@ -127,7 +115,6 @@ def test_any_hit_consistent_with_closest_100k(ctx):
 
 
 def test_empty_and_degenerate_inputs(ctx):
-    t = host.torch()
     # zero rays is a no-op
     acc = host.BvhAccel(ctx, trace_scene(np.array([[0, 0, 0, 1, 0, 0, 0, 1, 0]], np.float32)))
     abi.check(acc.L.chk_trace_closest(acc.ds.h, None, 0, None, None), "zero rays")
