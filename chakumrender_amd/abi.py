@@ -98,7 +98,16 @@ SIGNATURES = {
                              C.POINTER(chk_render_stats), _vp]),
     "chk_render_count": (C.c_int, [_vp, C.POINTER(chk_camera), C.POINTER(chk_render_params), _vp,
                                    C.POINTER(chk_render_stats), _vp]),
+    "chk_ltc_fp64_count": (C.c_int, [_vp, C.POINTER(_u32), _vp]),
+    "chk_proposal_create": (C.c_int, [_vp, _u32, C.c_uint64, C.POINTER(_vp)]),
+    "chk_proposal_destroy": (C.c_int, [_vp]),
+    "chk_proposal_set_params": (C.c_int, [_vp, _vp, C.c_size_t]),
+    "chk_proposal_get_params": (C.c_int, [_vp, _vp, C.c_size_t]),
+    "chk_proposal_eval": (C.c_int, [_vp, _vp, _u32, _vp, _vp, _vp, _vp]),
+    "chk_proposal_grad": (C.c_int, [_vp, _vp, _vp, _u32, _vp, _vp]),
+    "chk_proposal_adam": (C.c_int, [_vp, _vp, C.c_float, _vp]),
 }
+N_PROPOSAL_PARAMS = 14625
 
 _lib = None
 

@@ -37,8 +37,12 @@ __global__ void k_brdf_eval(const chk_material *__restrict__ mats, uint32_t n_ma
   const Mat m = load_mat(mats + mi);
   const size_t o = 3 * (size_t)i;
   float fv, pv;
-  brdf_pdf(m, mk(n[o], n[o + 1], n[o + 2]), mk(wo_ref[o], wo_ref[o + 1], wo_ref[o + 2]),
-           mk(wo[o], wo[o + 1], wo[o + 2]), mk(wi[o], wi[o + 1], wi[o + 2]), fv, pv);
+  // the formulas assume unit vectors (materialTypes.cpp:8); FP32 inputs are
+  // unit only to rounding, so normalise them here
+  brdf_pdf(m, normalize(mk(n[o], n[o + 1], n[o + 2])),
+           normalize(mk(wo_ref[o], wo_ref[o + 1], wo_ref[o + 2])),
+           normalize(mk(wo[o], wo[o + 1], wo[o + 2])), normalize(mk(wi[o], wi[o + 1], wi[o + 2])),
+           fv, pv);
   f[i] = fv;
   pdf[i] = pv;
 }
@@ -58,8 +62,9 @@ __global__ void k_brdf_sample(const chk_material *__restrict__ mats, uint32_t n_
     return;
   }
   const Mat m = load_mat(mats + mi);
-  const f3 N = mk(n[o], n[o + 1], n[o + 2]), R = mk(wo_ref[o], wo_ref[o + 1], wo_ref[o + 2]),
-           O = mk(wo[o], wo[o + 1], wo[o + 2]);
+  const f3 N = normalize(mk(n[o], n[o + 1], n[o + 2])),
+           R = normalize(mk(wo_ref[o], wo_ref[o + 1], wo_ref[o + 2])),
+           O = normalize(mk(wo[o], wo[o + 1], wo[o + 2]));
   const f3 w = brdf_sample(m, N, R, O, u[o], u[o + 1], u[o + 2]);
   float fv, pv;
   brdf_pdf(m, N, R, O, w, fv, pv);
@@ -220,6 +225,7 @@ chk_status chk_destroy(chk_ctx_t c) {
   (void)hipSetDevice(c->device);
   if (c->ltc_tab) (void)hipFree(c->ltc_tab);
   if (c->ltc_dev) (void)hipFree(c->ltc_dev);
+  if (c->ltc_scratch) (void)hipFree(c->ltc_scratch);
   delete c;
   return CHK_OK;
 }

@@ -53,9 +53,11 @@ struct DeviceScene {
 struct chk_ctx_s {
   int device = 0;
   int n_cu = 256;
-  double *ltc_dev = nullptr;   // not used directly by kernels
+  double *ltc_dev = nullptr;   // packed FP64 table for the FP64 LTC pass (6 per record)
   float *ltc_tab = nullptr;    // packed FP32 table for the LTC kernel (see chk_ltc.hip)
   std::vector<double> ltc_host;
+  void *ltc_scratch = nullptr; // FP64-pass point list + counter
+  size_t ltc_scratch_bytes = 0;
 };
 
 struct chk_scene_s {

@@ -18,6 +18,10 @@ namespace chk {
 namespace {
 
 constexpr int kLtcBlock = 256;
+// A point whose edge terms cancel by more than this factor (sum of |terms|
+// over |sum|, radiance- and part-weighted) loses more than ~1e-6 relative in
+// FP32 and is recomputed by the FP64 pass.
+constexpr float kLtcKappa = 8.0f;
 
 // integrateEdge (lightSources_CV.cpp:8-25): (v1 x v2).n * theta/sin(theta)
 __device__ inline float edge_factor(f3 v1, f3 v2, f3 c) {
@@ -132,9 +136,15 @@ __global__ __launch_bounds__(kLtcBlock) void k_ltc(const float4 *__restrict__ ta
                                                    const float *__restrict__ ks, uint32_t k,
                                                    const float *__restrict__ lt,
                                                    const float *__restrict__ lrad, uint32_t nlt,
-                                                   float *__restrict__ rgb) {
+                                                   float *__restrict__ rgb,
+                                                   uint32_t *__restrict__ fb_list,
+                                                   uint32_t *__restrict__ fb_count, float kappa) {
   const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < k; i += stride) {
+  const uint32_t lane = __lane_id();
+  for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < k; base += stride) {
+    const uint32_t i = base + lane;
+    bool flag = false;
+    if (i < k) {
     const size_t o = 3 * (size_t)i;
     const f3 x = mk(X[o], X[o + 1], X[o + 2]);
     const f3 n = mk(N[o], N[o + 1], N[o + 2]);
@@ -165,21 +175,26 @@ __global__ __launch_bounds__(kLtcBlock) void k_ltc(const float4 *__restrict__ ta
     // Minv * R, R rows (T1, T2, n); Minv = [[m0,0,m2],[0,m4,0],[m6,0,m8]]
     const f3 R0 = T1 * m0 + n * m2, R1 = T2 * m4, R2 = T1 * m6 + n * m8;
     float sl0 = 0.0f, sl1 = 0.0f, sl2 = 0.0f, sd0 = 0.0f, sd1 = 0.0f, sd2 = 0.0f;
+    // conditioning: sum of |edge terms| against |edge sum| per part
+    float al = 0.0f, bl = 0.0f, ad = 0.0f, bd = 0.0f;
     const bool do_ltc = kss != 0.0f, do_dif = kss != 1.0f;
     for (uint32_t j = 0; j < nlt; ++j) {
       const float *p = lt + 9 * (size_t)j;
       const f3 A = mk(p[0], p[1], p[2]), B = mk(p[3], p[4], p[5]), C = mk(p[6], p[7], p[8]);
       const f3 rad = mk(lrad[3 * j], lrad[3 * j + 1], lrad[3 * j + 2]);
+      const float rmax = fmaxf(fmaxf(rad.x, rad.y), rad.z);
       if (do_ltc) {
         f3 e1 = row_mul(R0, R1, R2, A - x), e2 = row_mul(R0, R1, R2, B - x),
            e3 = row_mul(R0, R1, R2, C - x);
-        float sum = 0.0f;
+        float sum = 0.0f, mag = 0.0f;
         bool use = true;
         if (e1.z > 0.0f && e2.z > 0.0f && e3.z > 0.0f) {
           e1 = normalize(e1);
           e2 = normalize(e2);
           e3 = normalize(e3);
-          sum = edge_z(e1, e2) + edge_z(e2, e3) + edge_z(e3, e1);
+          const float t1 = edge_z(e1, e2), t2 = edge_z(e2, e3), t3 = edge_z(e3, e1);
+          sum = t1 + t2 + t3;
+          mag = fabsf(t1) + fabsf(t2) + fabsf(t3);
         } else if (e1.z < 0.0f && e2.z < 0.0f && e3.z < 0.0f) {
           use = false;
         } else {
@@ -188,7 +203,10 @@ __global__ __launch_bounds__(kLtcBlock) void k_ltc(const float4 *__restrict__ ta
                      e3.z * rsqrtf(dot(e3, e3)), cl)) {
             const f3 c1 = normalize(cl[0]), c2 = normalize(cl[1]), c3 = normalize(cl[2]),
                      c4 = normalize(cl[3]);
-            sum = edge_z(c1, c2) + edge_z(c2, c3) + edge_z(c3, c4) + edge_z(c4, c1);
+            const float t1 = edge_z(c1, c2), t2 = edge_z(c2, c3), t3 = edge_z(c3, c4),
+                        t4 = edge_z(c4, c1);
+            sum = t1 + t2 + t3 + t4;
+            mag = fabsf(t1) + fabsf(t2) + fabsf(t3) + fabsf(t4);
           } else {
             use = false;
           }
@@ -198,15 +216,19 @@ __global__ __launch_bounds__(kLtcBlock) void k_ltc(const float4 *__restrict__ ta
           sl0 = fmaf(rad.x, sum, sl0);
           sl1 = fmaf(rad.y, sum, sl1);
           sl2 = fmaf(rad.z, sum, sl2);
+          al = fmaf(rmax, mag, al);
+          bl = fmaf(rmax, sum, bl);
         }
       }
       if (do_dif) {
         const f3 e1 = normalize(A - x), e2 = normalize(B - x), e3 = normalize(C - x);
         const float nA = dot(e1, n), nB = dot(e2, n), nC = dot(e3, n);
-        float sum = 0.0f;
+        float sum = 0.0f, mag = 0.0f;
         bool use = true;
         if (nA > 0.0f && nB > 0.0f && nC > 0.0f) {
-          sum = edge_n(e1, e2, n) + edge_n(e2, e3, n) + edge_n(e3, e1, n);
+          const float t1 = edge_n(e1, e2, n), t2 = edge_n(e2, e3, n), t3 = edge_n(e3, e1, n);
+          sum = t1 + t2 + t3;
+          mag = fabsf(t1) + fabsf(t2) + fabsf(t3);
         } else if (nA < 0.0f && nB < 0.0f && nC < 0.0f) {
           use = false;
         } else {
@@ -214,7 +236,10 @@ __global__ __launch_bounds__(kLtcBlock) void k_ltc(const float4 *__restrict__ ta
           if (clip_plane(A, B, C, n, x, nA, nB, nC, cl)) {
             const f3 c1 = normalize(cl[0] - x), c2 = normalize(cl[1] - x),
                      c3 = normalize(cl[2] - x), c4 = normalize(cl[3] - x);
-            sum = edge_n(c1, c2, n) + edge_n(c2, c3, n) + edge_n(c3, c4, n) + edge_n(c4, c1, n);
+            const float t1 = edge_n(c1, c2, n), t2 = edge_n(c2, c3, n), t3 = edge_n(c3, c4, n),
+                        t4 = edge_n(c4, c1, n);
+            sum = t1 + t2 + t3 + t4;
+            mag = fabsf(t1) + fabsf(t2) + fabsf(t3) + fabsf(t4);
           } else {
             use = false;
           }
@@ -224,6 +249,8 @@ __global__ __launch_bounds__(kLtcBlock) void k_ltc(const float4 *__restrict__ ta
           sd0 = fmaf(rad.x, sum, sd0);
           sd1 = fmaf(rad.y, sum, sd1);
           sd2 = fmaf(rad.z, sum, sd2);
+          ad = fmaf(rmax, mag, ad);
+          bd = fmaf(rmax, sum, bd);
         }
       }
     }
@@ -233,6 +260,174 @@ __global__ __launch_bounds__(kLtcBlock) void k_ltc(const float4 *__restrict__ ta
     rgb[o] = (do_ltc ? sl0 * wl : 0.0f) + (do_dif ? sd0 * wd : 0.0f);
     rgb[o + 1] = (do_ltc ? sl1 * wl : 0.0f) + (do_dif ? sd1 * wd : 0.0f);
     rgb[o + 2] = (do_ltc ? sl2 * wl : 0.0f) + (do_dif ? sd2 * wd : 0.0f);
+    // an ill-conditioned point (edge terms cancel) is recomputed in FP64
+    const float num = (do_ltc ? wl * al : 0.0f) + (do_dif ? wd * ad : 0.0f);
+    const float den = (do_ltc ? wl * bl : 0.0f) + (do_dif ? wd * bd : 0.0f);
+    flag = fb_list != nullptr && num > kappa * den;
+    }
+    // compacted append of flagged points (wave ballot, one atomic per wave)
+    const unsigned long long mask = __ballot(flag);
+    if (mask) {
+      const uint32_t cnt = (uint32_t)__popcll(mask);
+      const int leader = __ffsll((long long)mask) - 1;
+      uint32_t b = 0;
+      if ((int)lane == leader) b = atomicAdd(fb_count, cnt);
+      b = __shfl(b, leader);
+      const unsigned long long below = lane ? (mask & ((~0ull) >> (64 - lane))) : 0ull;
+      if (flag) fb_list[b + (uint32_t)__popcll(below)] = i;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- FP64 pass
+// The same computation as getLightFromMeshSource_Analytic in FP64, from the
+// FP64 table, for the points the FP32 pass flagged.
+struct d3 {
+  double x, y, z;
+};
+__device__ inline d3 dk(double x, double y, double z) { return d3{x, y, z}; }
+__device__ inline d3 operator+(d3 a, d3 b) { return dk(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ inline d3 operator-(d3 a, d3 b) { return dk(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ inline d3 operator*(d3 a, double s) { return dk(a.x * s, a.y * s, a.z * s); }
+__device__ inline double ddot(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ inline d3 dcross(d3 a, d3 b) {
+  return dk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ inline d3 dnorm(d3 a) { return a * (1.0 / sqrt(ddot(a, a))); }
+__device__ inline double dedge(d3 v1, d3 v2, d3 n) {  // lightSources_CV.cpp:8-15
+  const double th = acos(ddot(v1, v2));
+  return ddot(dcross(v1, v2), n) * ((th > 0.001) ? th / sin(th) : 1.0);
+}
+
+__device__ bool dclip(d3 A, d3 B, d3 C, d3 n, d3 P, double nA, double nB, double nC, d3 *clip) {
+  const double eps = 1e-15;
+  int cnt = 0, idx = 0;
+  const d3 dAB = A - B, dBC = B - C, dCA = C - A;
+  const double nAB = ddot(n, dAB), nBC = ddot(n, dBC), nCA = ddot(n, dCA);
+  const double pA = ddot(n, P - A), pB = ddot(n, P - B), pC = ddot(n, P - C);
+  if (nAB * nAB > eps) {
+    const d3 I = dAB * (pA / nAB) + A;
+    if (ddot(I - A, B - I) > 0) {
+      ++cnt;
+      if (nA < 0) { clip[idx++] = I; clip[idx++] = B; }
+      else { clip[idx++] = A; clip[idx++] = I; }
+    }
+  } else if (pA * pA < eps) {
+    return false;
+  }
+  if (nBC * nBC > eps) {
+    const d3 I = dBC * (pB / nBC) + B;
+    if (ddot(I - B, C - I) > 0) {
+      ++cnt;
+      if (idx < 4) {
+        if (nB < 0) { clip[idx++] = I; clip[idx++] = C; }
+        else { clip[idx++] = B; clip[idx++] = I; }
+      }
+    }
+  } else if (pB * pB < eps) {
+    return false;
+  }
+  if (nCA * nCA > eps) {
+    const d3 I = dCA * (pC / nCA) + C;
+    if ((I.x - C.x) * (A.x - I.x) > 0) {
+      ++cnt;
+      if (idx < 4) {
+        if (nC < 0) { clip[idx++] = I; clip[idx++] = A; }
+        else { clip[idx++] = C; clip[idx++] = I; }
+      }
+    }
+  } else if (pC * pC < eps) {
+    return false;
+  }
+  return cnt > 1;
+}
+
+// One triangle of either part: the LTC part is the plane z = 0 through the
+// origin of the transformed frame (n = (0,0,1), P = 0), the diffuse part the
+// plane through x with normal n; both use the same clip rules.
+__device__ double dtri(d3 A, d3 B, d3 C, d3 n, d3 P) {
+  d3 e1 = dnorm(A - P), e2 = dnorm(B - P), e3 = dnorm(C - P);
+  const double nA = ddot(e1, n), nB = ddot(e2, n), nC = ddot(e3, n);
+  double sum;
+  if (nA > 0 && nB > 0 && nC > 0) {
+    sum = dedge(e1, e2, n);
+    sum += dedge(e2, e3, n);
+    sum += dedge(e3, e1, n);
+  } else if (nA < 0 && nB < 0 && nC < 0) {
+    return 0.0;
+  } else {
+    d3 cl[4] = {dk(0, 0, 0), dk(0, 0, 0), dk(0, 0, 0), dk(0, 0, 0)};
+    if (!dclip(A, B, C, n, P, nA, nB, nC, cl)) return 0.0;
+    const d3 c1 = dnorm(cl[0] - P), c2 = dnorm(cl[1] - P), c3 = dnorm(cl[2] - P),
+             c4 = dnorm(cl[3] - P);
+    sum = dedge(c1, c2, n);
+    sum += dedge(c2, c3, n);
+    sum += dedge(c3, c4, n);
+    sum += dedge(c4, c1, n);
+  }
+  return fabs(sum);
+}
+
+__global__ __launch_bounds__(128) void k_ltc64(const double *__restrict__ tab64,
+                                              const float *__restrict__ X,
+                                              const float *__restrict__ N,
+                                              const float *__restrict__ V,
+                                              const float *__restrict__ alpha,
+                                              const float *__restrict__ ks,
+                                              const float *__restrict__ lt,
+                                              const float *__restrict__ lrad, uint32_t nlt,
+                                              float *__restrict__ rgb,
+                                              const uint32_t *__restrict__ fb_list,
+                                              const uint32_t *__restrict__ fb_count) {
+  const uint32_t cnt = *fb_count;
+  const double PI = 3.14159265359;
+  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < cnt; q += gridDim.x * blockDim.x) {
+    const uint32_t i = fb_list[q];
+    const size_t o = 3 * (size_t)i;
+    const d3 x = dk(X[o], X[o + 1], X[o + 2]), n = dk(N[o], N[o + 1], N[o + 2]),
+             wo = dk(V[o], V[o + 1], V[o + 2]);
+    const double kss = ks[i];
+    // M_GGX((float)acos(n.wo), (float)alpha) (ltc.h:18-61, materialTypes.cpp:54)
+    const float theta = (float)acos(ddot(n, wo));
+    const double tq = (double)theta / (0.5 * PI) * 64.0;
+    const int tb = max(0, min(63, (int)floor(tq)));
+    const int ab = max(0, min(63, (int)floor(sqrt((double)alpha[i]) * 64.0)));
+    const int tp = min(63, tb + 1);
+    const double *r0 = tab64 + 6 * (ab + tb * 64);
+    double m[6] = {r0[0], r0[1], r0[2], r0[3], r0[4], r0[5]};
+    if (tb != tp) {
+      const float mult = (float)(tq - (double)(float)tb);
+      const double *r1 = tab64 + 6 * (ab + tp * 64);
+      for (int c = 0; c < 6; ++c) m[c] = m[c] + (r1[c] - m[c]) * (double)mult;
+    }
+    const d3 T1 = dnorm(wo - n * ddot(n, wo));
+    const d3 T2 = dnorm(dcross(n, T1));
+    const d3 R0 = T1 * m[0] + n * m[1], R1 = T2 * m[2], R2 = T1 * m[3] + n * m[4];
+    double sl[3] = {0, 0, 0}, sd[3] = {0, 0, 0};
+    const d3 zero = dk(0, 0, 0), zax = dk(0, 0, 1);
+    for (uint32_t j = 0; j < nlt; ++j) {
+      const float *p = lt + 9 * (size_t)j;
+      const d3 A = dk(p[0], p[1], p[2]), B = dk(p[3], p[4], p[5]), C = dk(p[6], p[7], p[8]);
+      const double rad[3] = {lrad[3 * j], lrad[3 * j + 1], lrad[3 * j + 2]};
+      if (kss != 0.0) {
+        const d3 a = A - x, b = B - x, c = C - x;
+        const d3 e1 = dk(ddot(R0, a), ddot(R1, a), ddot(R2, a));
+        const d3 e2 = dk(ddot(R0, b), ddot(R1, b), ddot(R2, b));
+        const d3 e3 = dk(ddot(R0, c), ddot(R1, c), ddot(R2, c));
+        // clipTriangle(e1, e2, e3, e.z/|e|, ...) against z = 0 (lightSources_CV.cpp:301)
+        const double s = dtri(e1, e2, e3, zax, zero);
+        for (int ch = 0; ch < 3; ++ch) sl[ch] += rad[ch] * s;
+      }
+      if (kss != 1.0) {
+        const double s = dtri(A, B, C, n, x);
+        for (int ch = 0; ch < 3; ++ch) sd[ch] += rad[ch] * s;
+      }
+    }
+    for (int ch = 0; ch < 3; ++ch) {
+      double G = sl[ch] / (2.0 * PI) * m[5] * kss;
+      G = G + sd[ch] / (2.0 * PI) * (1 - kss);
+      rgb[o + ch] = (float)G;
+    }
   }
 }
 
@@ -275,6 +470,26 @@ chk_status chk_set_ltc_table(chk_ctx_t ctx, const double *table, size_t n_double
     }
   }
   CHK_HIP(hipMemcpy(ctx->ltc_tab, packed.data(), packed.size() * 4, hipMemcpyHostToDevice));
+  // FP64 copy for the FP64 pass: same five Minv entries and the amplitude
+  std::vector<double> p64(4096 * 6);
+  for (int r = 0; r < 4096; ++r) {
+    const double *f = table + 19 * (size_t)r + 9;
+    double *q = &p64[6 * (size_t)r];
+    q[0] = f[0];
+    q[1] = f[6];
+    q[2] = f[4];
+    q[3] = f[2];
+    q[4] = f[8];
+    q[5] = table[19 * (size_t)r + 18];
+  }
+  if (!ctx->ltc_dev) {
+    if (hipMalloc(&ctx->ltc_dev, p64.size() * 8) != hipSuccess) {
+      ctx->ltc_dev = nullptr;
+      set_error("chk_set_ltc_table: cannot allocate the FP64 table");
+      return CHK_ENOMEM;
+    }
+  }
+  CHK_HIP(hipMemcpy(ctx->ltc_dev, p64.data(), p64.size() * 8, hipMemcpyHostToDevice));
   ctx->ltc_host.assign(table, table + n_doubles);
   return CHK_OK;
 }
@@ -296,11 +511,42 @@ chk_status chk_ltc_eval(chk_ctx_t ctx, const float *x, const float *n, const flo
     return CHK_EINVAL;
   }
   CHK_HIP(hipSetDevice(ctx->device));
+  const size_t need = 256 + 4 * (size_t)k;
+  if (ctx->ltc_scratch_bytes < need) {
+    if (ctx->ltc_scratch) (void)hipFree(ctx->ltc_scratch);
+    ctx->ltc_scratch = nullptr;
+    ctx->ltc_scratch_bytes = 0;
+    if (hipMalloc(&ctx->ltc_scratch, need) != hipSuccess) {
+      set_error("chk_ltc_eval: cannot allocate the FP64-pass list");
+      return CHK_ENOMEM;
+    }
+    ctx->ltc_scratch_bytes = need;
+  }
+  uint32_t *fb_count = (uint32_t *)ctx->ltc_scratch;
+  uint32_t *fb_list = (uint32_t *)((char *)ctx->ltc_scratch + 256);
+  hipStream_t st = (hipStream_t)stream;
+  CHK_HIP(launch_zero_u32(fb_count, 1, st));
   const uint32_t blocks = std::min<uint32_t>((k + kLtcBlock - 1) / kLtcBlock, (uint32_t)ctx->n_cu * 16);
-  hipLaunchKernelGGL(k_ltc, dim3(blocks), dim3(kLtcBlock), 0, (hipStream_t)stream,
-                     (const float4 *)ctx->ltc_tab, x, n, v, alpha, ks, k, light_tris, radiance,
-                     n_light_tris, rgb);
+  hipLaunchKernelGGL(k_ltc, dim3(blocks), dim3(kLtcBlock), 0, st, (const float4 *)ctx->ltc_tab, x, n,
+                     v, alpha, ks, k, light_tris, radiance, n_light_tris, rgb, fb_list, fb_count,
+                     kLtcKappa);
   CHK_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_ltc64, dim3(ctx->n_cu * 2), dim3(128), 0, st, (const double *)ctx->ltc_dev, x,
+                     n, v, alpha, ks, light_tris, radiance, n_light_tris, rgb, fb_list, fb_count);
+  CHK_HIP(hipGetLastError());
+  return CHK_OK;
+}
+
+chk_status chk_ltc_fp64_count(chk_ctx_t ctx, uint32_t *count, void *stream) {
+  if (!ctx || !count) {
+    set_error("chk_ltc_fp64_count: null argument");
+    return CHK_EINVAL;
+  }
+  *count = 0;
+  if (!ctx->ltc_scratch) return CHK_OK;
+  CHK_HIP(hipSetDevice(ctx->device));
+  CHK_HIP(hipMemcpyAsync(count, ctx->ltc_scratch, 4, hipMemcpyDeviceToHost, (hipStream_t)stream));
+  CHK_HIP(hipStreamSynchronize((hipStream_t)stream));
   return CHK_OK;
 }
 

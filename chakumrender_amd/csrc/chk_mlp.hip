@@ -1,0 +1,607 @@
+// Learned proposal network on MFMA: inference, backward deltas, weight
+// gradients (split-K MFMA GEMM), deterministic reduction and Adam.  See
+// chk_mlp.h for the layouts.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "chk_device.h"
+#include "chk_internal.h"
+#include "chk_mlp.h"
+
+namespace chk {
+
+typedef short s4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+__device__ inline uint16_t f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7F800000u) == 0x7F800000u) return (uint16_t)((u >> 16) | ((u & 0xFFFFu) ? 0x40u : 0u));
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ inline s4 pack4(float a, float b, float c, float d) {
+  s4 r;
+  r.x = (short)f2bf(a);
+  r.y = (short)f2bf(b);
+  r.z = (short)f2bf(c);
+  r.w = (short)f2bf(d);
+  return r;
+}
+__device__ inline float bf2f(short b) { return __uint_as_float(((uint32_t)(uint16_t)b) << 16); }
+
+__device__ inline f4 mfma(s4 a, s4 b, f4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// Frequency encoding of the 7 inputs: for scalar i, features 9i + r are
+// s, sin(2^k pi s) (r = 1+k), cos(2^k pi s) (r = 5+k), k = 0..3; feature 63 = 1.
+__device__ inline float encode(int f, const float *s) {
+  if (f >= 63) return 1.0f;
+  const int i = f / 9, r = f - 9 * i;
+  const float x = s[i];
+  if (r == 0) return x;
+  const int k = r <= 4 ? r - 1 : r - 5;
+  float sn, cs;
+  sincospif((float)(1 << k) * x, &sn, &cs);
+  return r <= 4 ? sn : cs;
+}
+
+__device__ inline void encode_b(const float *s, int lane, s4 xin[4]) {
+  const int g = lane >> 4;
+#pragma unroll
+  for (int kt = 0; kt < 4; ++kt) {
+    const int f0 = kt * 16 + 4 * g;
+    xin[kt] = pack4(encode(f0, s), encode(f0 + 1, s), encode(f0 + 2, s), encode(f0 + 3, s));
+  }
+}
+
+__device__ inline const s4 *frag_f(const MlpDev &W, int l) {
+  return reinterpret_cast<const s4 *>(W.F + (l < 3 ? l * kFragLayerF : 3 * kFragLayerF));
+}
+__device__ inline const s4 *frag_g(const MlpDev &W, int l) {
+  return reinterpret_cast<const s4 *>(W.G + (l < 3 ? l * kFragLayerF : 3 * kFragLayerF));
+}
+
+// One hidden layer: out tiles (fp32 + bias), then ReLU -> bf16 B operands.
+__device__ inline void hidden(const MlpDev &W, int l, const s4 in[4], int lane, s4 out[4]) {
+  const s4 *F = frag_f(W, l);
+  const float *b = W.bias + 64 * l;
+  const int g = lane >> 4;
+#pragma unroll
+  for (int ot = 0; ot < 4; ++ot) {
+    f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int kt = 0; kt < 4; ++kt) acc = mfma(F[(ot * 4 + kt) * 64 + lane], in[kt], acc);
+    const int o = ot * 16 + 4 * g;
+    out[ot] = pack4(fmaxf(acc[0] + b[o], 0.0f), fmaxf(acc[1] + b[o + 1], 0.0f),
+                    fmaxf(acc[2] + b[o + 2], 0.0f), fmaxf(acc[3] + b[o + 3], 0.0f));
+  }
+}
+
+__device__ inline void output_layer(const MlpDev &W, const s4 in[4], int lane, f4 out[3]) {
+  const s4 *F = frag_f(W, 3);
+  const float *b = W.bias + 192;
+  const int g = lane >> 4;
+#pragma unroll
+  for (int ot = 0; ot < 3; ++ot) {
+    f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int kt = 0; kt < 4; ++kt) acc = mfma(F[(ot * 4 + kt) * 64 + lane], in[kt], acc);
+    const int o = ot * 16 + 4 * g;
+    for (int v = 0; v < 4; ++v) acc[v] += b[o + v];
+    out[ot] = acc;
+  }
+}
+
+// Softmax over the first n_lights logits of each point (column), defensive
+// mixture, and the mixing weight.  Lanes l, l^16, l^32, l^48 share a point.
+__device__ inline void proposal_head(const f4 out[3], int lane, uint32_t L, float q[8], float &mix,
+                                     float &sum_e, float &mx) {
+  const int g = lane >> 4;
+  float e[8];
+  mx = -INFINITY;
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int o = t * 16 + 4 * g + v;
+      e[t * 4 + v] = (uint32_t)o < L ? out[t][v] : -INFINITY;
+      mx = fmaxf(mx, e[t * 4 + v]);
+    }
+  mx = fmaxf(mx, __shfl_xor(mx, 16));
+  mx = fmaxf(mx, __shfl_xor(mx, 32));
+  float s = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    e[k] = e[k] == -INFINITY ? 0.0f : __expf(e[k] - mx);
+    s += e[k];
+  }
+  s += __shfl_xor(s, 16);
+  s += __shfl_xor(s, 32);
+  sum_e = s;
+  const float invL = 1.0f / (float)L;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int o = (k >> 2) * 16 + 4 * g + (k & 3);
+    q[k] = (uint32_t)o < L ? (1.0f - kPropEpsQ) * (e[k] / s) + kPropEpsQ * invL : 0.0f;
+  }
+  // mixing logit: output 32 = tile 2, row 0 -> lanes of group 0, v = 0
+  const float zc = __shfl(out[2][0], lane & 15);
+  mix = kPropEpsC + (1.0f - 2.0f * kPropEpsC) / (1.0f + __expf(-zc));
+}
+
+// ------------------------------------------------------------------ kernels
+__global__ __launch_bounds__(256) void k_mlp_eval(MlpDev W, const float *__restrict__ inputs,
+                                                  uint32_t k, const uint32_t *__restrict__ kdev,
+                                                  float *__restrict__ probs,
+                                                  float *__restrict__ mix_out,
+                                                  float *__restrict__ logits) {
+  if (kdev) k = min(k, *kdev);
+  const int lane = threadIdx.x & 63;
+  const uint32_t wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const uint32_t nw = gridDim.x * (blockDim.x >> 6);
+  for (uint32_t g0 = wave * 16; g0 < k; g0 += nw * 16) {
+    const uint32_t p = g0 + (lane & 15);
+    const bool valid = p < k;
+    float s[7];
+#pragma unroll
+    for (int i = 0; i < 7; ++i) s[i] = valid ? inputs[7 * (size_t)p + i] : 0.0f;
+    s4 x[4], h1[4], h2[4], h3[4];
+    encode_b(s, lane, x);
+    hidden(W, 0, x, lane, h1);
+    hidden(W, 1, h1, lane, h2);
+    hidden(W, 2, h2, lane, h3);
+    f4 out[3];
+    output_layer(W, h3, lane, out);
+    float q[8], mix, se, mx;
+    proposal_head(out, lane, W.n_lights, q, mix, se, mx);
+    if (valid) {
+      const int g = lane >> 4;
+#pragma unroll
+      for (int kk = 0; kk < 8; ++kk) probs[32 * (size_t)p + (kk >> 2) * 16 + 4 * g + (kk & 3)] = q[kk];
+      if (g == 0) mix_out[p] = mix;
+      if (logits) {
+#pragma unroll
+        for (int t = 0; t < 3; ++t)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int o = t * 16 + 4 * g + v;
+            if (o < kMlpOut) logits[33 * (size_t)p + o] = out[t][v];
+          }
+      }
+    }
+  }
+}
+
+// Forward + backward deltas for a batch; stores the layer inputs X_l and
+// output deltas D_l (bf16, feature-major [feature][kpad]) for the weight
+// gradient GEMM.  dlogits: k x 33 (upstream gradient of the loss).
+__global__ __launch_bounds__(256) void k_mlp_fwd_bwd(MlpDev W, const float *__restrict__ inputs,
+                                                     const float *__restrict__ dlogits, uint32_t k,
+                                                     uint32_t kpad, uint16_t *__restrict__ X,
+                                                     uint16_t *__restrict__ D) {
+  const int lane = threadIdx.x & 63;
+  const int g = lane >> 4;
+  const uint32_t wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const uint32_t nw = gridDim.x * (blockDim.x >> 6);
+  uint16_t *X0 = X, *X1 = X + 64 * (size_t)kpad, *X2 = X + 128 * (size_t)kpad,
+           *X3 = X + 192 * (size_t)kpad;
+  uint16_t *D1 = D, *D2 = D + 64 * (size_t)kpad, *D3 = D + 128 * (size_t)kpad,
+           *D4 = D + 192 * (size_t)kpad;
+  for (uint32_t g0 = wave * 16; g0 < kpad; g0 += nw * 16) {
+    const uint32_t p = g0 + (lane & 15);
+    const bool valid = p < k;
+    float s[7];
+#pragma unroll
+    for (int i = 0; i < 7; ++i) s[i] = valid ? inputs[7 * (size_t)p + i] : 0.0f;
+    s4 x[4], h1[4], h2[4], h3[4];
+    encode_b(s, lane, x);
+    hidden(W, 0, x, lane, h1);
+    hidden(W, 1, h1, lane, h2);
+    hidden(W, 2, h2, lane, h3);
+    // store layer inputs (bf16) feature-major
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const size_t o = (size_t)(t * 16 + 4 * g + v) * kpad + p;
+        X0[o] = (uint16_t)x[t][v];
+        X1[o] = (uint16_t)h1[t][v];
+        X2[o] = (uint16_t)h2[t][v];
+        X3[o] = (uint16_t)h3[t][v];
+      }
+    // upstream deltas of the 33 outputs (masked lights and padding: 0)
+    s4 d4[3];
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+      float dv[4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int o = t * 16 + 4 * g + v;
+        const bool live = valid && o < kMlpOut && (o == 32 || (uint32_t)o < W.n_lights);
+        dv[v] = live ? dlogits[33 * (size_t)p + o] : 0.0f;
+      }
+      d4[t] = pack4(dv[0], dv[1], dv[2], dv[3]);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) D4[(size_t)(t * 16 + 4 * g + v) * kpad + p] = (uint16_t)d4[t][v];
+    }
+    // dH3 = W4 . d4 (K = 48 outputs), ReLU mask of h3
+    s4 d3[4], d2[4], d1[4];
+    {
+      const s4 *G = frag_g(W, 3);
+#pragma unroll
+      for (int it = 0; it < 4; ++it) {
+        f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int ot = 0; ot < 3; ++ot) acc = mfma(G[(it * 3 + ot) * 64 + lane], d4[ot], acc);
+        float m[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) m[v] = bf2f(h3[it][v]) > 0.0f ? acc[v] : 0.0f;
+        d3[it] = pack4(m[0], m[1], m[2], m[3]);
+      }
+    }
+    // dH2 = W3 . d3, dH1 = W2 . d2
+    {
+      const s4 *G = frag_g(W, 2);
+#pragma unroll
+      for (int it = 0; it < 4; ++it) {
+        f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int ot = 0; ot < 4; ++ot) acc = mfma(G[(it * 4 + ot) * 64 + lane], d3[ot], acc);
+        float m[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) m[v] = bf2f(h2[it][v]) > 0.0f ? acc[v] : 0.0f;
+        d2[it] = pack4(m[0], m[1], m[2], m[3]);
+      }
+    }
+    {
+      const s4 *G = frag_g(W, 1);
+#pragma unroll
+      for (int it = 0; it < 4; ++it) {
+        f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int ot = 0; ot < 4; ++ot) acc = mfma(G[(it * 4 + ot) * 64 + lane], d2[ot], acc);
+        float m[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) m[v] = bf2f(h1[it][v]) > 0.0f ? acc[v] : 0.0f;
+        d1[it] = pack4(m[0], m[1], m[2], m[3]);
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const size_t o = (size_t)(t * 16 + 4 * g + v) * kpad + p;
+        D1[o] = (uint16_t)d1[t][v];
+        D2[o] = (uint16_t)d2[t][v];
+        D3[o] = (uint16_t)d3[t][v];
+      }
+  }
+}
+
+// Weight gradient tiles: dW_l^T (out x in) = D_l (out x pts) . X_l^T (pts x in),
+// plus a fifth input tile of ones whose column 0 is the bias gradient.
+// One wave per (tile, K chunk of 256 points); partial sums in D layout.
+constexpr int kChunk = 256;
+constexpr int kTiles = 3 * 4 * 5 + 3 * 5;  // 75
+
+__device__ inline void tile_decode(int tile, int &l, int &mt, int &nt) {
+  if (tile < 60) {
+    l = tile / 20;
+    mt = (tile % 20) / 5;
+    nt = tile % 5;
+  } else {
+    l = 3;
+    mt = (tile - 60) / 5;
+    nt = (tile - 60) % 5;
+  }
+}
+
+__global__ __launch_bounds__(64) void k_mlp_wgrad(const uint16_t *__restrict__ X,
+                                                  const uint16_t *__restrict__ D, uint32_t kpad,
+                                                  uint32_t nchunk, float *__restrict__ partial) {
+  const int lane = threadIdx.x;
+  const int tile = blockIdx.x / nchunk;
+  const uint32_t chunk = blockIdx.x % nchunk;
+  int l, mt, nt;
+  tile_decode(tile, l, mt, nt);
+  const uint16_t *Dl = D + (size_t)(64 * l) * kpad;
+  const uint16_t *Xl = X + (size_t)(64 * l) * kpad;
+  const int m = mt * 16 + (lane & 15), n = nt * 16 + (lane & 15);
+  f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+  const s4 ones = (lane & 15) == 0 ? s4{(short)0x3F80, (short)0x3F80, (short)0x3F80, (short)0x3F80}
+                                   : s4{0, 0, 0, 0};
+  const uint32_t p0 = chunk * kChunk, p1 = min(p0 + kChunk, kpad);
+  for (uint32_t pb = p0; pb < p1; pb += 16) {
+    const uint32_t pp = pb + 4 * (lane >> 4);
+    const s4 a = *reinterpret_cast<const s4 *>(Dl + (size_t)m * kpad + pp);
+    const s4 b = nt < 4 ? *reinterpret_cast<const s4 *>(Xl + (size_t)n * kpad + pp) : ones;
+    acc = mfma(a, b, acc);
+  }
+  float *o = partial + ((size_t)tile * nchunk + chunk) * 256 + lane * 4;
+  o[0] = acc[0];
+  o[1] = acc[1];
+  o[2] = acc[2];
+  o[3] = acc[3];
+}
+
+// Sum partials over chunks (fixed order) into the flat gradient vector.
+__global__ void k_mlp_reduce(const float *__restrict__ partial, uint32_t nchunk,
+                             float *__restrict__ grad) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;  // tile * 256 + element
+  if (idx >= kTiles * 256) return;
+  const int tile = idx >> 8, e = idx & 255;
+  const int lane = e >> 2, v = e & 3;
+  int l, mt, nt;
+  tile_decode(tile, l, mt, nt);
+  const int out = mt * 16 + 4 * (lane >> 4) + v;
+  const int in = nt * 16 + (lane & 15);
+  const int O = l < 3 ? 64 : 33;
+  if (out >= O) return;
+  if (nt == 4 && in != 64) return;
+  float s = 0.0f;
+  for (uint32_t c = 0; c < nchunk; ++c) s += partial[((size_t)tile * nchunk + c) * 256 + e];
+  const int wo = l * (64 * 64 + 64);
+  if (nt < 4) grad[wo + in * O + out] = s;
+  else grad[wo + 64 * O + out] = s;
+}
+
+__global__ void k_adam(float *__restrict__ p, float *__restrict__ m, float *__restrict__ v,
+                       const float *__restrict__ g, int n, float lr_t, float b1, float b2,
+                       float eps) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.0f - b1) * gi;
+  const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  p[i] -= lr_t * mi / (sqrtf(vi) + eps);
+}
+
+// Rebuild the bf16 fragments and the padded bias vector from the FP32 params.
+__global__ void k_mlp_pack(const float *__restrict__ P, uint16_t *__restrict__ F,
+                           uint16_t *__restrict__ G, float *__restrict__ bias) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx < 240) {
+    const int l = idx < 192 ? idx / 64 : 3, o = idx < 192 ? idx % 64 : idx - 192;
+    const int O = l < 3 ? 64 : 33;
+    bias[idx] = o < O ? P[l * (64 * 64 + 64) + 64 * O + o] : 0.0f;
+  }
+  if (idx >= kFragTotal) return;
+  int l, r;
+  if (idx < 3 * kFragLayerF) {
+    l = idx / kFragLayerF;
+    r = idx % kFragLayerF;
+  } else {
+    l = 3;
+    r = idx - 3 * kFragLayerF;
+  }
+  const int OT = l < 3 ? 4 : 3, O = l < 3 ? 64 : 33;
+  const int j = r & 3, lane = (r >> 2) & 63, t = r >> 8;
+  const float *W = P + l * (64 * 64 + 64);
+  {  // F: t = ot * 4 + kt
+    const int ot = t / 4, kt = t % 4;
+    const int in = kt * 16 + 4 * (lane >> 4) + j, out = ot * 16 + (lane & 15);
+    F[idx] = out < O ? f2bf(W[in * O + out]) : 0;
+  }
+  {  // G: t = it * OT + ot
+    const int it = t / OT, ot = t % OT;
+    const int in = it * 16 + (lane & 15), out = ot * 16 + 4 * (lane >> 4) + j;
+    G[idx] = out < O ? f2bf(W[in * O + out]) : 0;
+  }
+}
+
+}  // namespace chk
+
+// ====================================================================== API
+struct chk_proposal_s {
+  chk_ctx_s *ctx = nullptr;
+  uint32_t n_lights = 0;
+  float *params = nullptr;  // FP32 master
+  float *adam_m = nullptr, *adam_v = nullptr;
+  uint16_t *F = nullptr, *G = nullptr;
+  float *bias = nullptr;
+  uint64_t step = 0;
+  // training scratch
+  void *scratch = nullptr;
+  size_t scratch_bytes = 0;
+};
+
+namespace chk {
+
+MlpDev proposal_dev(const chk_proposal_s *p) {
+  MlpDev W;
+  W.F = p->F;
+  W.G = p->G;
+  W.bias = p->bias;
+  W.n_lights = p->n_lights;
+  return W;
+}
+
+hipError_t launch_mlp_eval(const chk_proposal_s *p, const float *inputs, uint32_t k,
+                           const uint32_t *kdev, float *probs, float *mix, float *logits,
+                           hipStream_t st) {
+  if (k == 0) return hipSuccess;
+  const uint32_t groups = (k + 15) / 16;
+  const uint32_t blocks = std::min<uint32_t>((groups + 3) / 4, (uint32_t)p->ctx->n_cu * 8);
+  hipLaunchKernelGGL(k_mlp_eval, dim3(blocks), dim3(256), 0, st, proposal_dev(p), inputs, k, kdev,
+                     probs, mix, logits);
+  return hipGetLastError();
+}
+
+static hipError_t pack(chk_proposal_s *p, hipStream_t st) {
+  hipLaunchKernelGGL(k_mlp_pack, dim3((kFragTotal + 255) / 256), dim3(256), 0, st, p->params, p->F,
+                     p->G, p->bias);
+  return hipGetLastError();
+}
+
+// grad (dev, kMlpParams floats) of the loss whose output gradients are dlogits.
+chk_status proposal_grad(chk_proposal_s *p, const float *inputs, const float *dlogits, uint32_t k,
+                         float *grad, hipStream_t st) {
+  const uint32_t kpad = std::max<uint32_t>(16, (k + 15) / 16 * 16);
+  const uint32_t nchunk = (kpad + kChunk - 1) / kChunk;
+  const size_t bx = (size_t)256 * kpad * 2, bd = (size_t)240 * kpad * 2;
+  const size_t bp = (size_t)kTiles * nchunk * 256 * 4;
+  const size_t need = bx + bd + bp + 1024;
+  if (p->scratch_bytes < need) {
+    if (p->scratch) (void)hipFree(p->scratch);
+    p->scratch = nullptr;
+    p->scratch_bytes = 0;
+    if (hipMalloc(&p->scratch, need) != hipSuccess) {
+      set_error("proposal: cannot allocate training scratch");
+      return CHK_ENOMEM;
+    }
+    p->scratch_bytes = need;
+  }
+  uint16_t *X = (uint16_t *)p->scratch;
+  uint16_t *D = (uint16_t *)((char *)p->scratch + bx);
+  float *partial = (float *)((char *)p->scratch + ((bx + bd + 255) & ~(size_t)255));
+  const uint32_t groups = kpad / 16;
+  const uint32_t blocks = std::min<uint32_t>((groups + 3) / 4, (uint32_t)p->ctx->n_cu * 8);
+  hipLaunchKernelGGL(k_mlp_fwd_bwd, dim3(blocks), dim3(256), 0, st, proposal_dev(p), inputs, dlogits,
+                     k, kpad, X, D);
+  hipLaunchKernelGGL(k_mlp_wgrad, dim3(kTiles * nchunk), dim3(64), 0, st, X, D, kpad, nchunk,
+                     partial);
+  hipLaunchKernelGGL(k_mlp_reduce, dim3((kTiles * 256 + 255) / 256), dim3(256), 0, st, partial,
+                     nchunk, grad);
+  CHK_HIP(hipGetLastError());
+  return CHK_OK;
+}
+
+chk_status proposal_adam(chk_proposal_s *p, const float *grad, float lr, hipStream_t st) {
+  p->step += 1;
+  const double b1 = 0.9, b2 = 0.999;
+  const double t = (double)p->step;
+  const float lr_t = (float)(lr * std::sqrt(1.0 - std::pow(b2, t)) / (1.0 - std::pow(b1, t)));
+  hipLaunchKernelGGL(k_adam, dim3((kMlpParams + 255) / 256), dim3(256), 0, st, p->params, p->adam_m,
+                     p->adam_v, grad, kMlpParams, lr_t, (float)b1, (float)b2, 1e-8f);
+  CHK_HIP(hipGetLastError());
+  CHK_HIP(pack(p, st));
+  return CHK_OK;
+}
+
+}  // namespace chk
+
+using namespace chk;
+
+extern "C" {
+
+chk_status chk_proposal_create(chk_ctx_t ctx, uint32_t n_lights, uint64_t seed,
+                               chk_proposal_t *out) {
+  if (!ctx || !out) {
+    set_error("chk_proposal_create: null argument");
+    return CHK_EINVAL;
+  }
+  *out = nullptr;
+  if (n_lights == 0 || n_lights > (uint32_t)kMaxProposalLights) {
+    set_error("chk_proposal_create: the proposal supports 1..32 lights");
+    return CHK_ECAPACITY;
+  }
+  CHK_HIP(hipSetDevice(ctx->device));
+  chk_proposal_s *p = new chk_proposal_s();
+  p->ctx = ctx;
+  p->n_lights = n_lights;
+  const size_t pb = (size_t)kMlpParams * 4;
+  bool ok = hipMalloc(&p->params, pb) == hipSuccess && hipMalloc(&p->adam_m, pb) == hipSuccess &&
+            hipMalloc(&p->adam_v, pb) == hipSuccess &&
+            hipMalloc(&p->F, kFragTotal * 2) == hipSuccess &&
+            hipMalloc(&p->G, kFragTotal * 2) == hipSuccess &&
+            hipMalloc(&p->bias, 256 * 4) == hipSuccess;
+  if (!ok) {
+    chk_proposal_destroy(p);
+    set_error("chk_proposal_create: cannot allocate the network");
+    return CHK_ENOMEM;
+  }
+  // He-uniform hidden layers from a seeded splitmix64; zero output layer so
+  // an untrained network is the uniform pmf with mixing weight 1/2 (the
+  // balance heuristic with one light and one BSDF sample).
+  std::vector<float> h(kMlpParams, 0.0f);
+  uint64_t st = seed ^ 0x9E3779B97F4A7C15ull;
+  auto next = [&]() {
+    uint64_t z = (st += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+  };
+  const double lim = std::sqrt(6.0 / 64.0);
+  for (int l = 0; l < 3; ++l)
+    for (int i = 0; i < 64 * 64; ++i) h[l * (64 * 64 + 64) + i] = (float)((2.0 * next() - 1.0) * lim);
+  chk_status s = chk_proposal_set_params(p, h.data(), h.size());
+  if (s != CHK_OK) {
+    chk_proposal_destroy(p);
+    return s;
+  }
+  *out = p;
+  return CHK_OK;
+}
+
+chk_status chk_proposal_destroy(chk_proposal_t p) {
+  if (!p) return CHK_OK;
+  (void)hipSetDevice(p->ctx->device);
+  for (void *q : {(void *)p->params, (void *)p->adam_m, (void *)p->adam_v, (void *)p->F,
+                  (void *)p->G, (void *)p->bias, p->scratch})
+    if (q) (void)hipFree(q);
+  delete p;
+  return CHK_OK;
+}
+
+chk_status chk_proposal_set_params(chk_proposal_t p, const float *host, size_t n) {
+  if (!p || !host || n != (size_t)kMlpParams) {
+    set_error("chk_proposal_set_params: expected 14625 floats");
+    return CHK_EINVAL;
+  }
+  CHK_HIP(hipSetDevice(p->ctx->device));
+  CHK_HIP(hipMemcpy(p->params, host, n * 4, hipMemcpyHostToDevice));
+  CHK_HIP(hipMemset(p->adam_m, 0, n * 4));
+  CHK_HIP(hipMemset(p->adam_v, 0, n * 4));
+  p->step = 0;
+  CHK_HIP(pack(p, nullptr));
+  CHK_HIP(hipDeviceSynchronize());
+  return CHK_OK;
+}
+
+chk_status chk_proposal_get_params(chk_proposal_t p, float *host, size_t n) {
+  if (!p || !host || n != (size_t)kMlpParams) {
+    set_error("chk_proposal_get_params: expected 14625 floats");
+    return CHK_EINVAL;
+  }
+  CHK_HIP(hipSetDevice(p->ctx->device));
+  CHK_HIP(hipDeviceSynchronize());
+  CHK_HIP(hipMemcpy(host, p->params, n * 4, hipMemcpyDeviceToHost));
+  return CHK_OK;
+}
+
+chk_status chk_proposal_eval(chk_proposal_t p, const float *inputs, uint32_t k, float *probs,
+                             float *mix, float *logits, void *stream) {
+  if (!p || (k && (!inputs || !probs || !mix))) {
+    set_error("chk_proposal_eval: null argument");
+    return CHK_EINVAL;
+  }
+  CHK_HIP(hipSetDevice(p->ctx->device));
+  CHK_HIP(launch_mlp_eval(p, inputs, k, nullptr, probs, mix, logits, (hipStream_t)stream));
+  return CHK_OK;
+}
+
+chk_status chk_proposal_grad(chk_proposal_t p, const float *inputs, const float *dlogits,
+                             uint32_t k, float *grad, void *stream) {
+  if (!p || !grad || (k && (!inputs || !dlogits))) {
+    set_error("chk_proposal_grad: null argument");
+    return CHK_EINVAL;
+  }
+  CHK_HIP(hipSetDevice(p->ctx->device));
+  return proposal_grad(p, inputs, dlogits, k, grad, (hipStream_t)stream);
+}
+
+chk_status chk_proposal_adam(chk_proposal_t p, const float *grad, float lr, void *stream) {
+  if (!p || !grad || !(lr > 0.0f)) {
+    set_error("chk_proposal_adam: null argument or lr <= 0");
+    return CHK_EINVAL;
+  }
+  CHK_HIP(hipSetDevice(p->ctx->device));
+  return proposal_adam(p, grad, lr, (hipStream_t)stream);
+}
+
+}  // extern "C"

@@ -25,11 +25,12 @@ template <bool ANY, bool COUNT, int STACK>
 __device__ inline void traverse(const DeviceScene &S, uint32_t *__restrict__ stk, f3 o, f3 d,
                                 float tmax, uint32_t skip, bool leaf_ids, float &t_out,
                                 uint32_t &id_out, float &u_out, float &v_out, bool &hit_any,
-                                uint32_t &nv, uint32_t &nt, uint32_t &overflow) {
+                                uint32_t &leaf_out, uint32_t &nv, uint32_t &nt,
+                                uint32_t &overflow) {
   const RayPre pre = ray_pre(d);
   const f3 inv = mk(safe_rcp(d.x), safe_rcp(d.y), safe_rcp(d.z));
   float t = tmax;
-  uint32_t best = 0xFFFFFFFFu;
+  uint32_t best = 0xFFFFFFFFu, best_leaf = 0u;
   float bu = 0.0f, bv = 0.0f;
   uint32_t entry = 0;  // root
   int sp = 0;
@@ -97,6 +98,7 @@ __device__ inline void traverse(const DeviceScene &S, uint32_t *__restrict__ stk
         if (tri_hit(pre, o, d, xyz(a), xyz(b), xyz(c), tt, uu, vv)) {
           t = tt;
           best = tid;
+          best_leaf = li;
           bu = uu;
           bv = vv;
           hit_any = true;
@@ -111,8 +113,22 @@ __device__ inline void traverse(const DeviceScene &S, uint32_t *__restrict__ stk
   }
   t_out = best == 0xFFFFFFFFu ? 1e30f : t;
   id_out = best;
+  leaf_out = best_leaf;
   u_out = bu;
   v_out = bv;
+}
+
+// Distance to the winning triangle's plane in FP64 (the reference's
+// t = ((A - o).N) / (d.N), geometryPrimitives.cpp:44): the watertight test
+// decides hit / miss, this gives t to FP64 accuracy for the caller.
+__device__ inline float refine_t(const float4 *tp, f3 o, f3 d) {
+  const double ax = tp[0].x, ay = tp[0].y, az = tp[0].z;
+  const double e1x = (double)tp[1].x - ax, e1y = (double)tp[1].y - ay, e1z = (double)tp[1].z - az;
+  const double e2x = (double)tp[2].x - ax, e2y = (double)tp[2].y - ay, e2z = (double)tp[2].z - az;
+  const double nx = e2y * e1z - e2z * e1y, ny = e2z * e1x - e2x * e1z, nz = e2x * e1y - e2y * e1x;
+  const double num = (ax - o.x) * nx + (ay - o.y) * ny + (az - o.z) * nz;
+  const double den = (double)d.x * nx + (double)d.y * ny + (double)d.z * nz;
+  return (float)(num / den);
 }
 
 template <bool ANY, bool COUNT, int STACK>
@@ -139,10 +155,13 @@ __global__ __launch_bounds__(kBlock) void k_trace(DeviceScene S, const float4 *_
     if (i < total) {
       const float4 ot = rays[2 * (size_t)i], ds = rays[2 * (size_t)i + 1];
       float t, u, v;
-      uint32_t id;
+      uint32_t id, leaf;
       bool any;
       traverse<ANY, COUNT, STACK>(S, stk, xyz(ot), xyz(ds), ot.w, __float_as_uint(ds.w),
-                                  leaf_ids != 0, t, id, u, v, any, nv, nt, overflow);
+                                  leaf_ids != 0, t, id, u, v, any, leaf, nv, nt, overflow);
+      // the caller-facing query reports t in FP64 accuracy
+      if (!ANY && !leaf_ids && id != 0xFFFFFFFFu)
+        t = refine_t(S.tris + 3 * (size_t)leaf, xyz(ot), xyz(ds));
       if (ANY) {
         if (occluded) occluded[i] = any ? 1 : 0;
         if (path_L && !any) {

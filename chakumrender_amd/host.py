@@ -140,6 +140,61 @@ class LightSource:
         return rgb.download((k, 3), np.float32)
 
 
+class Proposal:
+    """Learned light-selection proposal (chk_proposal_t)."""
+
+    def __init__(self, ctx: abi.Context, n_lights: int, seed: int = 1):
+        self.ctx = ctx
+        self.L = ctx.L
+        self.n_lights = int(n_lights)
+        h = C.c_void_p()
+        abi.check(self.L.chk_proposal_create(ctx.h, self.n_lights, int(seed), C.byref(h)),
+                  "chk_proposal_create")
+        self.h = h
+
+    def get_params(self):
+        p = np.empty(abi.N_PROPOSAL_PARAMS, np.float32)
+        abi.check(self.L.chk_proposal_get_params(self.h, p.ctypes.data, p.size), "get_params")
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        abi.check(self.L.chk_proposal_set_params(self.h, p.ctypes.data, p.size), "set_params")
+
+    def eval(self, inputs):
+        s = np.ascontiguousarray(inputs, np.float32).reshape(-1, 7)
+        k = s.shape[0]
+        sd = DevBuf.from_array(s)
+        pr, mx, lg = DevBuf(128 * k), DevBuf(4 * k), DevBuf(132 * k)
+        abi.check(self.L.chk_proposal_eval(self.h, sd.ptr, k, pr.ptr, mx.ptr, lg.ptr, None),
+                  "chk_proposal_eval")
+        return (pr.download((k, 32), np.float32), mx.download((k,), np.float32),
+                lg.download((k, 33), np.float32))
+
+    def grad(self, inputs, dlogits):
+        s = np.ascontiguousarray(inputs, np.float32).reshape(-1, 7)
+        k = s.shape[0]
+        sd = DevBuf.from_array(s)
+        dd = DevBuf.from_array(np.ascontiguousarray(dlogits, np.float32).reshape(k, 33))
+        g = DevBuf(4 * abi.N_PROPOSAL_PARAMS)
+        abi.check(self.L.chk_proposal_grad(self.h, sd.ptr, dd.ptr, k, g.ptr, None), "chk_proposal_grad")
+        return g.download((abi.N_PROPOSAL_PARAMS,), np.float32), g
+
+    def adam(self, grad_buf, lr=1e-3):
+        abi.check(self.L.chk_proposal_adam(self.h, grad_buf.ptr, float(lr), None), "chk_proposal_adam")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.chk_proposal_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Image:
     """FP64 accumulator (H, W, 3) (+ optional sum of squares) in HBM."""
 

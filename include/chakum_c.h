@@ -172,6 +172,10 @@ chk_status chk_ltc_eval(chk_ctx_t ctx, const float *x, const float *n, const flo
                         const float *alpha, const float *ks, uint32_t k,
                         const float *light_tris, const float *radiance, uint32_t n_light_tris,
                         float *rgb, void *stream);
+/* Points of the last chk_ltc_eval on this context that the FP32 pass
+ * flagged as ill-conditioned (edge terms cancelling) and recomputed in FP64.
+ * Synchronises the stream. */
+chk_status chk_ltc_fp64_count(chk_ctx_t ctx, uint32_t *count, void *stream);
 
 /* Render: the pixel loop of main.cpp:62-94 around shade() (shader.cpp:362),
  * with the configs' estimator: path tracing to params->max_depth vertices,
@@ -188,6 +192,35 @@ chk_status chk_render(chk_scene_t scene, const chk_camera *camera, const chk_ren
 chk_status chk_render_count(chk_scene_t scene, const chk_camera *camera,
                             const chk_render_params *params, double *accum,
                             chk_render_stats *stats, void *stream);
+
+/* ---- learned light-selection proposal -----------------------------------
+ * No reference counterpart (the reference gives every mesh light the same
+ * sample count, mis.h:9, and the balance heuristic, mis.cpp:209-220).  A
+ * 64-64-64-64-33 ReLU MLP (BF16 weights, FP32 accumulation on MFMA) maps 7
+ * inputs in [0, 1] -- position in the scene box, normal and view direction as
+ * (theta/pi, phi/2pi) -- through a 64-wide frequency encoding to 32
+ * light-selection logits and one mixing logit.  The pmf is the defensive
+ * mixture q' = 0.9 softmax + 0.1 uniform and the mixing weight
+ * c = 0.1 + 0.8 sigmoid, so the estimator stays unbiased for any weights.
+ * Parameters: 14625 floats, W1[64][64] b1[64] W2 b2 W3 b3 W4[64][33] b4[33]
+ * (W[in][out]).  A new proposal has zero output weights: uniform pmf and
+ * c = 1/2, i.e. plain balance-heuristic MIS. */
+typedef struct chk_proposal_s *chk_proposal_t;
+chk_status chk_proposal_create(chk_ctx_t ctx, uint32_t n_lights, uint64_t seed,
+                               chk_proposal_t *out);
+chk_status chk_proposal_destroy(chk_proposal_t p);
+/* host float[14625]; set resets the Adam state. */
+chk_status chk_proposal_set_params(chk_proposal_t p, const float *host, size_t n);
+chk_status chk_proposal_get_params(chk_proposal_t p, float *host, size_t n);
+/* inputs k x 7, probs k x 32 (q'), mix k (c), logits k x 33 (may be NULL): dev. */
+chk_status chk_proposal_eval(chk_proposal_t p, const float *inputs, uint32_t k, float *probs,
+                             float *mix, float *logits, void *stream);
+/* Parameter gradient (dev float[14625], overwritten) of a loss whose
+ * gradient with respect to the 33 logits is dlogits (k x 33, dev). */
+chk_status chk_proposal_grad(chk_proposal_t p, const float *inputs, const float *dlogits,
+                             uint32_t k, float *grad, void *stream);
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8) with learning rate lr. */
+chk_status chk_proposal_adam(chk_proposal_t p, const float *grad, float lr, void *stream);
 
 #ifdef __cplusplus
 }

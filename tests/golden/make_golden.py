@@ -118,8 +118,11 @@ def gen_brdf(R, rng):
         wi = np.where((wi * n).sum(1, keepdims=True) <= 0.02, wi_u, wi)
         keep = ((wi * n).sum(1) > 0.02) & ((wo * n).sum(1) > 0.02)
         n, wo, wo_ref, wi = n[keep], wo[keep], wo_ref[keep], wi[keep]
-        # float32-representable inputs, as the GPU receives them
-        n, wo, wo_ref, wi = (a.astype(np.float32).astype(np.float64) for a in (n, wo, wo_ref, wi))
+        # float32-representable directions, renormalised in FP64: the
+        # reference formulas assume unit vectors (materialTypes.cpp:8) and
+        # amplify |v| - 1 for small roughness; the GPU receives the float32
+        # rounding of these unit vectors
+        n, wo, wo_ref, wi = (unit(a.astype(np.float32).astype(np.float64)) for a in (n, wo, wo_ref, wi))
         m = n.shape[0]
         f = np.empty(m)
         pdf = np.empty(m)

@@ -17,16 +17,31 @@ def rel_err(a, b, floor=0.0):
     return np.abs(a - b) / np.maximum(np.abs(b), floor)
 
 
+def fp32_rel_err(a, b, scale):
+    """Relative error where the FP64 value is an FP32-representable
+    magnitude (above 1e-30 of the case's largest value); below that both
+    must be negligible (FP32 flushes such lobe tails to zero)."""
+    big = np.abs(b) > 1e-30 * scale
+    e = np.zeros_like(b)
+    e[big] = np.abs(a[big] - b[big]) / np.abs(b[big])
+    small_ok = np.all(np.abs(a[~big]) <= 1e-20 * scale)
+    return e, small_ok
+
+
 def test_brdf_eval_matches_reference(golden):
     B = golden("brdf.npz")
     mats = brdf_materials(B["cases"])
     mt = host.MaterialType(mats)
     f, p = mt.brdf_pdf(B["case"].astype(np.uint32), B["n"], B["wo_ref"], B["wo"], B["wi"])
-    ef, ep = rel_err(f, B["f"]), rel_err(p, B["pdf"])
+    worst = 0.0
     for ci in range(len(B["cases"])):
         m = B["case"] == ci
-        print(ci, B["cases"][ci][:3], f"f {ef[m].max():.2e} pdf {ep[m].max():.2e}")
-    assert ef.max() < REL and ep.max() < REL
+        ef, okf = fp32_rel_err(f[m].astype(np.float64), B["f"][m], np.abs(B["f"][m]).max())
+        ep, okp = fp32_rel_err(p[m].astype(np.float64), B["pdf"][m], np.abs(B["pdf"][m]).max())
+        print(ci, B["cases"][ci][:3], f"f {ef.max():.2e} pdf {ep.max():.2e}")
+        assert okf and okp
+        worst = max(worst, ef.max(), ep.max())
+    assert worst < REL
 
 
 def test_brdf_sample_matches_oracle():
