@@ -21,11 +21,21 @@ chk_status render_impl(chk_scene_s *sc, const chk_camera *cam, const chk_render_
 
 namespace {
 
+// Load a direction given in FP64 at precision T and normalise it there: the
+// formulas assume unit vectors (materialTypes.cpp:8).
+template <typename T>
+__device__ inline v3<T> load_dir(const double *__restrict__ p) {
+  return normalize(mk3<T>((T)p[0], (T)p[1], (T)p[2]));
+}
+
+// T = double: the reference's precision; T = float: the arithmetic of the
+// fused render kernels, on the FP32 rounding of the inputs.
+template <typename T>
 __global__ void k_brdf_eval(const chk_material *__restrict__ mats, uint32_t n_mats,
-                            const uint32_t *__restrict__ mat_idx, const float *__restrict__ n,
-                            const float *__restrict__ wo_ref, const float *__restrict__ wo,
-                            const float *__restrict__ wi, uint32_t k, float *__restrict__ f,
-                            float *__restrict__ pdf) {
+                            const uint32_t *__restrict__ mat_idx, const double *__restrict__ n,
+                            const double *__restrict__ wo_ref, const double *__restrict__ wo,
+                            const double *__restrict__ wi, uint32_t k, double *__restrict__ f,
+                            double *__restrict__ pdf) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= k) return;
   const uint32_t mi = mat_idx[i];
@@ -34,24 +44,21 @@ __global__ void k_brdf_eval(const chk_material *__restrict__ mats, uint32_t n_ma
     pdf[i] = NAN;
     return;
   }
-  const Mat m = load_mat(mats + mi);
+  const MatT<T> m = load_mat<T>(mats + mi);
   const size_t o = 3 * (size_t)i;
-  float fv, pv;
-  // the formulas assume unit vectors (materialTypes.cpp:8); FP32 inputs are
-  // unit only to rounding, so normalise them here
-  brdf_pdf(m, normalize(mk(n[o], n[o + 1], n[o + 2])),
-           normalize(mk(wo_ref[o], wo_ref[o + 1], wo_ref[o + 2])),
-           normalize(mk(wo[o], wo[o + 1], wo[o + 2])), normalize(mk(wi[o], wi[o + 1], wi[o + 2])),
-           fv, pv);
+  T fv, pv;
+  brdf_pdf(m, load_dir<T>(n + o), load_dir<T>(wo_ref + o), load_dir<T>(wo + o),
+           load_dir<T>(wi + o), fv, pv);
   f[i] = fv;
   pdf[i] = pv;
 }
 
+template <typename T>
 __global__ void k_brdf_sample(const chk_material *__restrict__ mats, uint32_t n_mats,
-                              const uint32_t *__restrict__ mat_idx, const float *__restrict__ n,
-                              const float *__restrict__ wo_ref, const float *__restrict__ wo,
-                              const float *__restrict__ u, uint32_t k, float *__restrict__ wi,
-                              float *__restrict__ pdf) {
+                              const uint32_t *__restrict__ mat_idx, const double *__restrict__ n,
+                              const double *__restrict__ wo_ref, const double *__restrict__ wo,
+                              const double *__restrict__ u, uint32_t k, double *__restrict__ wi,
+                              double *__restrict__ pdf) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= k) return;
   const uint32_t mi = mat_idx[i];
@@ -61,12 +68,10 @@ __global__ void k_brdf_sample(const chk_material *__restrict__ mats, uint32_t n_
     pdf[i] = NAN;
     return;
   }
-  const Mat m = load_mat(mats + mi);
-  const f3 N = normalize(mk(n[o], n[o + 1], n[o + 2])),
-           R = normalize(mk(wo_ref[o], wo_ref[o + 1], wo_ref[o + 2])),
-           O = normalize(mk(wo[o], wo[o + 1], wo[o + 2]));
-  const f3 w = brdf_sample(m, N, R, O, u[o], u[o + 1], u[o + 2]);
-  float fv, pv;
+  const MatT<T> m = load_mat<T>(mats + mi);
+  const v3<T> N = load_dir<T>(n + o), R = load_dir<T>(wo_ref + o), O = load_dir<T>(wo + o);
+  const v3<T> w = brdf_sample(m, N, R, O, (T)u[o], (T)u[o + 1], (T)u[o + 2]);
+  T fv, pv;
   brdf_pdf(m, N, R, O, w, fv, pv);
   wi[o] = w.x;
   wi[o + 1] = w.y;
@@ -472,29 +477,49 @@ chk_status chk_trace_any(chk_scene_t sc, const chk_ray *rays, uint32_t n, uint8_
 }
 
 chk_status chk_brdf_eval(const chk_material *mats, uint32_t n_mats, const uint32_t *mat_idx,
-                         const float *n, const float *wo_ref, const float *wo, const float *wi,
-                         uint32_t k, float *f, float *pdf, void *stream) {
+                         const double *n, const double *wo_ref, const double *wo,
+                         const double *wi, uint32_t k, uint32_t flags, double *f, double *pdf,
+                         void *stream) {
   if (k == 0) return CHK_OK;
   if (!mats || !mat_idx || !n || !wo_ref || !wo || !wi || !f || !pdf || n_mats == 0) {
     set_error("chk_brdf_eval: null argument");
     return CHK_EINVAL;
   }
-  hipLaunchKernelGGL(k_brdf_eval, dim3((k + 255) / 256), dim3(256), 0, (hipStream_t)stream, mats,
-                     n_mats, mat_idx, n, wo_ref, wo, wi, k, f, pdf);
+  if (flags & ~(uint32_t)CHK_BRDF_FP32) {
+    set_error("chk_brdf_eval: unknown flags");
+    return CHK_EINVAL;
+  }
+  const dim3 grid((k + 255) / 256), block(256);
+  if (flags & CHK_BRDF_FP32)
+    hipLaunchKernelGGL(k_brdf_eval<float>, grid, block, 0, (hipStream_t)stream, mats, n_mats,
+                       mat_idx, n, wo_ref, wo, wi, k, f, pdf);
+  else
+    hipLaunchKernelGGL(k_brdf_eval<double>, grid, block, 0, (hipStream_t)stream, mats, n_mats,
+                       mat_idx, n, wo_ref, wo, wi, k, f, pdf);
   CHK_HIP(hipGetLastError());
   return CHK_OK;
 }
 
 chk_status chk_brdf_sample(const chk_material *mats, uint32_t n_mats, const uint32_t *mat_idx,
-                           const float *n, const float *wo_ref, const float *wo, const float *u,
-                           uint32_t k, float *wi, float *pdf, void *stream) {
+                           const double *n, const double *wo_ref, const double *wo,
+                           const double *u, uint32_t k, uint32_t flags, double *wi, double *pdf,
+                           void *stream) {
   if (k == 0) return CHK_OK;
   if (!mats || !mat_idx || !n || !wo_ref || !wo || !u || !wi || !pdf || n_mats == 0) {
     set_error("chk_brdf_sample: null argument");
     return CHK_EINVAL;
   }
-  hipLaunchKernelGGL(k_brdf_sample, dim3((k + 255) / 256), dim3(256), 0, (hipStream_t)stream,
-                     mats, n_mats, mat_idx, n, wo_ref, wo, u, k, wi, pdf);
+  if (flags & ~(uint32_t)CHK_BRDF_FP32) {
+    set_error("chk_brdf_sample: unknown flags");
+    return CHK_EINVAL;
+  }
+  const dim3 grid((k + 255) / 256), block(256);
+  if (flags & CHK_BRDF_FP32)
+    hipLaunchKernelGGL(k_brdf_sample<float>, grid, block, 0, (hipStream_t)stream, mats, n_mats,
+                       mat_idx, n, wo_ref, wo, u, k, wi, pdf);
+  else
+    hipLaunchKernelGGL(k_brdf_sample<double>, grid, block, 0, (hipStream_t)stream, mats, n_mats,
+                       mat_idx, n, wo_ref, wo, u, k, wi, pdf);
   CHK_HIP(hipGetLastError());
   return CHK_OK;
 }

@@ -1,4 +1,6 @@
-// Device-side building blocks of the per-sample radiance path (gfx950, FP32).
+// Device-side building blocks of the per-sample radiance path (gfx950).
+// Material terms are templates: FP32 in the fused render kernels, FP64 in
+// the per-sample entry points (the reference's own precision).
 //
 // Reference formulas restated here (FP64 in the reference):
 //   Triangle::intersect          geometryPrimitives.cpp:29-71  (replaced by the
@@ -22,19 +24,76 @@ constexpr float kPi = 3.14159265359f;  // mathPrimitives.h:4
 constexpr float kInvPi = 1.0f / 3.14159265359f;
 constexpr float kInf = 1e30f;
 
-struct f3 {
-  float x, y, z;
+// 3-vectors in FP32 (the fused render kernels) and FP64 (the per-sample
+// entry points, which take the reference's double vectors).
+template <typename T>
+struct v3 {
+  T x, y, z;
 };
-__host__ __device__ inline f3 mk(float x, float y, float z) { return f3{x, y, z}; }
-__host__ __device__ inline f3 operator+(f3 a, f3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
-__host__ __device__ inline f3 operator-(f3 a, f3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
-__host__ __device__ inline f3 operator*(f3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
-__host__ __device__ inline f3 operator*(f3 a, f3 b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
-__host__ __device__ inline float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__host__ __device__ inline f3 cross(f3 a, f3 b) {
-  return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+using f3 = v3<float>;
+using d3 = v3<double>;
+template <typename T>
+struct same {
+  using type = T;
+};
+template <typename T>
+__host__ __device__ inline v3<T> mk3(T x, T y, T z) {
+  return v3<T>{x, y, z};
 }
-__device__ inline f3 normalize(f3 a) { return a * rsqrtf(dot(a, a)); }
+__host__ __device__ inline f3 mk(float x, float y, float z) { return f3{x, y, z}; }
+__host__ __device__ inline d3 dk(double x, double y, double z) { return d3{x, y, z}; }
+template <typename T>
+__host__ __device__ inline v3<T> operator+(v3<T> a, v3<T> b) {
+  return mk3(a.x + b.x, a.y + b.y, a.z + b.z);
+}
+template <typename T>
+__host__ __device__ inline v3<T> operator-(v3<T> a, v3<T> b) {
+  return mk3(a.x - b.x, a.y - b.y, a.z - b.z);
+}
+template <typename T>
+__host__ __device__ inline v3<T> operator*(v3<T> a, typename same<T>::type s) {
+  return mk3(a.x * s, a.y * s, a.z * s);
+}
+template <typename T>
+__host__ __device__ inline v3<T> operator*(v3<T> a, v3<T> b) {
+  return mk3(a.x * b.x, a.y * b.y, a.z * b.z);
+}
+template <typename T>
+__host__ __device__ inline T dot(v3<T> a, v3<T> b) {
+  return a.x * b.x + a.y * b.y + a.z * b.z;
+}
+template <typename T>
+__host__ __device__ inline v3<T> cross(v3<T> a, v3<T> b) {
+  return mk3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+
+// scalar math by precision (FP32: hardware approximations where the
+// render's error budget allows them)
+__device__ inline float rsq(float a) { return rsqrtf(a); }
+__device__ inline double rsq(double a) { return 1.0 / sqrt(a); }
+__device__ inline float sqr_t(float a) { return sqrtf(a); }
+__device__ inline double sqr_t(double a) { return sqrt(a); }
+__device__ inline float ex(float a) { return __expf(a); }
+__device__ inline double ex(double a) { return exp(a); }
+__device__ inline float lg1p(float a) { return log1pf(a); }
+__device__ inline double lg1p(double a) { return log1p(a); }
+__device__ inline float pw(float a, float b) { return __powf(a, b); }
+__device__ inline double pw(double a, double b) { return pow(a, b); }
+__device__ inline float fm(float a, float b, float c) { return fmaf(a, b, c); }
+__device__ inline double fm(double a, double b, double c) { return fma(a, b, c); }
+__device__ inline float fab(float a) { return fabsf(a); }
+__device__ inline double fab(double a) { return fabs(a); }
+__device__ inline float fmx(float a, float b) { return fmaxf(a, b); }
+__device__ inline double fmx(double a, double b) { return fmax(a, b); }
+__device__ inline float csign(float a, float b) { return copysignf(a, b); }
+__device__ inline double csign(double a, double b) { return copysign(a, b); }
+__device__ inline void sncs(float a, float *s, float *c) { __sincosf(a, s, c); }
+__device__ inline void sncs(double a, double *s, double *c) { sincos(a, s, c); }
+
+template <typename T>
+__device__ inline v3<T> normalize(v3<T> a) {
+  return a * rsq(dot(a, a));
+}
 __device__ inline f3 xyz(float4 v) { return mk(v.x, v.y, v.z); }
 __device__ inline float comp(f3 a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
 
@@ -58,22 +117,26 @@ __device__ inline float u01(uint32_t x) { return (float)(x >> 8) * (1.0f / 16777
 
 // ------------------------------------------------------------------ ONB
 // Duff et al., "Building an Orthonormal Basis, Revisited", JCGT 6(1) 2017.
-__device__ inline void onb(f3 n, f3 &t, f3 &b) {
-  float sign = copysignf(1.0f, n.z);
-  float a = -1.0f / (sign + n.z);
-  float bb = n.x * n.y * a;
-  t = mk(1.0f + sign * n.x * n.x * a, sign * bb, -sign * n.x);
-  b = mk(bb, sign + n.y * n.y * a, -n.y);
+template <typename T>
+__device__ inline void onb(v3<T> n, v3<T> &t, v3<T> &b) {
+  const T sign = csign(T(1), n.z);
+  const T a = T(-1) / (sign + n.z);
+  const T bb = n.x * n.y * a;
+  t = mk3(T(1) + sign * n.x * n.x * a, sign * bb, -sign * n.x);
+  b = mk3(bb, sign + n.y * n.y * a, -n.y);
 }
 
 // ------------------------------------------------------------------ materials
-struct Mat {
+template <typename T>
+struct MatT {
   uint32_t brdf, light;
-  float ks, alpha, pe, iior, eior, refl;
-  f3 c, rad;
+  T ks, alpha, pe, iior, eior, refl;
+  v3<T> c, rad;
 };
-__device__ inline Mat load_mat(const chk_material *__restrict__ m) {
-  Mat r;
+using Mat = MatT<float>;
+template <typename T = float>
+__device__ inline MatT<T> load_mat(const chk_material *__restrict__ m) {
+  MatT<T> r;
   r.brdf = m->brdf;
   r.light = m->light;
   r.ks = m->ks;
@@ -82,148 +145,170 @@ __device__ inline Mat load_mat(const chk_material *__restrict__ m) {
   r.iior = m->int_ior;
   r.eior = m->ext_ior;
   r.refl = m->reflectance;
-  r.c = mk(m->color[0], m->color[1], m->color[2]);
-  r.rad = mk(m->radiance[0], m->radiance[1], m->radiance[2]);
+  r.c = mk3<T>(m->color[0], m->color[1], m->color[2]);
+  r.rad = mk3<T>(m->radiance[0], m->radiance[1], m->radiance[2]);
   return r;
 }
 
+// pi and 1/pi with the reference's literal (mathPrimitives.h:4)
+template <typename T>
+__device__ inline T pi_v() {
+  return T(3.14159265359);
+}
+template <typename T>
+__device__ inline T inv_pi_v() {
+  return T(1) / T(3.14159265359);
+}
+
 // fresnel (materialTypes.cpp:156-178)
-__device__ inline float fresnel(const Mat &m, float c) {
-  float g = m.iior / m.eior;
-  g = sqrtf(fmaf(g, g, fmaf(c, c, -1.0f)));
-  float t1 = g - c, t2 = g + c;
-  float a = fmaf(c, t2, -1.0f), b = fmaf(c, t1, 1.0f);
-  float q = t1 / t2;
-  return 0.5f * q * q * (1.0f + (a * a) / (b * b));
+template <typename T>
+__device__ inline T fresnel(const MatT<T> &m, T c) {
+  T g = m.iior / m.eior;
+  g = sqr_t(fm(g, g, fm(c, c, T(-1))));
+  const T t1 = g - c, t2 = g + c;
+  const T a = fm(c, t2, T(-1)), b = fm(c, t1, T(1));
+  const T q = t1 / t2;
+  return T(0.5) * q * q * (T(1) + (a * a) / (b * b));
 }
 
 // Microfacet terms from cos = h.n and sin^2 = |h x n|^2.
 // ggxDist (materialTypes.cpp:180-191): a2 / (pi cos^4 (a2 + tan2)^2)
-__device__ inline float ggx_D(float a2, float c, float s2) {
-  float c2 = c * c;
-  float den = fmaf(a2, c2, s2);  // cos^2 (a2 + tan2)
-  return a2 / (kPi * den * den);
+template <typename T>
+__device__ inline T ggx_D(T a2, T c, T s2) {
+  const T den = fm(a2, c * c, s2);  // cos^2 (a2 + tan2)
+  return a2 / (pi_v<T>() * den * den);
 }
 // beckmannDist (materialTypes.cpp:212-222): exp(-tan2/a2) / (pi cos^4 a2)
-__device__ inline float beck_D(float a2, float c, float s2) {
-  float c2 = c * c;
-  return __expf(-s2 / (c2 * a2)) / (kPi * c2 * c2 * a2);
+template <typename T>
+__device__ inline T beck_D(T a2, T c, T s2) {
+  const T c2 = c * c;
+  return ex(-s2 / (c2 * a2)) / (pi_v<T>() * c2 * c2 * a2);
 }
 // Phong D (materialTypes.cpp:39,107): (e+2)/(2 pi) cos^e
-__device__ inline float phong_D(float e, float c, float s2) {
-  float omc = s2 / (1.0f + c);  // 1 - cos, accurate near 1
-  return (e + 2.0f) * (0.5f * kInvPi) * __expf(e * log1pf(-omc));
+template <typename T>
+__device__ inline T phong_D(T e, T c, T s2) {
+  const T omc = s2 / (T(1) + c);  // 1 - cos, accurate near 1
+  return (e + T(2)) * (T(0.5) * inv_pi_v<T>()) * ex(e * lg1p(-omc));
 }
 // ggxG1 (materialTypes.cpp:193-210); v.n = cv, tan^2 = s2v / cv^2
-__device__ inline float ggx_G1(float a2, float cv, float s2v, float hv) {
-  float c2 = cv * cv;
-  if (s2v <= 1e-15f * c2) return 1.0f;
-  if (hv <= 1e-15f) return 0.0f;
-  return 2.0f / (1.0f + sqrtf(1.0f + a2 * s2v / c2));
+template <typename T>
+__device__ inline T ggx_G1(T a2, T cv, T s2v, T hv) {
+  const T c2 = cv * cv;
+  if (s2v <= T(1e-15) * c2) return T(1);
+  if (hv <= T(1e-15)) return T(0);
+  return T(2) / (T(1) + sqr_t(T(1) + a2 * s2v / c2));
 }
 // rational G1 of beckmannG1/phongG1 (materialTypes.cpp:224-258)
-__device__ inline float rational_G1(float a) {
-  if (a < 1.6f) return (3.535f * a + 2.181f * a * a) / (1.0f + 2.276f * a + 2.577f * a * a);
-  return 1.0f;
+template <typename T>
+__device__ inline T rational_G1(T a) {
+  if (a < T(1.6)) return (T(3.535) * a + T(2.181) * a * a) / (T(1) + T(2.276) * a + T(2.577) * a * a);
+  return T(1);
 }
-__device__ inline float beck_G1(float alpha, float cv, float s2v, float hv) {
-  float c2 = cv * cv;
-  if (c2 >= 0.999999f) return 1.0f;
-  if (hv <= 1e-15f) return 0.0f;
+template <typename T>
+__device__ inline T beck_G1(T alpha, T cv, T s2v, T hv) {
+  const T c2 = cv * cv;
+  if (c2 >= T(0.999999)) return T(1);
+  if (hv <= T(1e-15)) return T(0);
   // a = 1 / (alpha tan) = |cv| / (alpha sqrt(s2v))
-  return rational_G1(fabsf(cv) / (alpha * sqrtf(s2v)));
+  return rational_G1(fab(cv) / (alpha * sqr_t(s2v)));
 }
-__device__ inline float phong_G1(float e, float cv, float s2v, float hv) {
-  float c2 = cv * cv;
-  if (c2 >= 0.999999f) return 1.0f;
-  if (hv <= 1e-15f) return 0.0f;
+template <typename T>
+__device__ inline T phong_G1(T e, T cv, T s2v, T hv) {
+  const T c2 = cv * cv;
+  if (c2 >= T(0.999999)) return T(1);
+  if (hv <= T(1e-15)) return T(0);
   // a = sqrt((0.5 e + 1) / tan^2)
-  return rational_G1(sqrtf((0.5f * e + 1.0f) * c2 / s2v));
+  return rational_G1(sqr_t((T(0.5) * e + T(1)) * c2 / s2v));
 }
 
 // |a x b|^2 for unit vectors: sin^2 of the angle between them.
-__device__ inline float sin2(f3 a, f3 b) {
-  f3 c = cross(a, b);
+template <typename T>
+__device__ inline T sin2(v3<T> a, v3<T> b) {
+  const v3<T> c = cross(a, b);
   return dot(c, c);
 }
 
 // BasePrimitive::brdf = MaterialType::brdf * reflectance (scalar) and
 // MaterialType::pdfEval, evaluated together (they share h and D).
-__device__ inline void brdf_pdf(const Mat &m, f3 n, f3 wo_ref, f3 wo, f3 wi, float &f, float &pdf) {
-  float cwi = dot(wi, n);
-  float diffuse_f = (1.0f - m.ks) * kInvPi;
-  float diffuse_p = (1.0f - m.ks) * cwi * kInvPi;
+template <typename T>
+__device__ inline void brdf_pdf(const MatT<T> &m, v3<T> n, v3<T> wo_ref, v3<T> wo, v3<T> wi,
+                                T &f, T &pdf) {
+  const T ip = inv_pi_v<T>();
+  const T cwi = dot(wi, n);
+  const T diffuse_f = (T(1) - m.ks) * ip;
+  const T diffuse_p = (T(1) - m.ks) * cwi * ip;
   if (m.brdf == CHK_BRDF_CLASSIC_PHONG) {
-    float cs = fmaxf(dot(wo_ref, wi), 0.0f);
-    float lobe = (m.pe + 1.0f) * __powf(cs, m.pe) * (0.5f * kInvPi);
-    if (m.pe == 0.0f) lobe = 0.5f * kInvPi;
+    const T cs = fmx(dot(wo_ref, wi), T(0));
+    T lobe = (m.pe + T(1)) * pw(cs, m.pe) * (T(0.5) * ip);
+    if (m.pe == T(0)) lobe = T(0.5) * ip;
     f = (m.ks * lobe + diffuse_f) * m.refl;
     pdf = m.ks * lobe + diffuse_p;
     return;
   }
   if (m.brdf == CHK_BRDF_GGX || m.brdf == CHK_BRDF_BECKMANN || m.brdf == CHK_BRDF_PHONG) {
-    f3 h = normalize(wo + wi);
-    float ch = dot(h, n), s2h = sin2(h, n);
-    float cwo = dot(wo, n);
-    float hwi = dot(h, wi), hwo = dot(h, wo);
-    float F = fresnel(m, hwi);
-    float D, G;
-    float s2o = sin2(wo, n), s2i = sin2(wi, n);
+    const v3<T> h = normalize(wo + wi);
+    const T ch = dot(h, n), s2h = sin2(h, n);
+    const T cwo = dot(wo, n);
+    const T hwi = dot(h, wi), hwo = dot(h, wo);
+    const T F = fresnel(m, hwi);
+    T D, G;
+    const T s2o = sin2(wo, n), s2i = sin2(wi, n);
     if (m.brdf == CHK_BRDF_GGX) {
-      float a2 = m.alpha * m.alpha;
+      const T a2 = m.alpha * m.alpha;
       D = ggx_D(a2, ch, s2h);
       G = ggx_G1(a2, cwo, s2o, hwo) * ggx_G1(a2, cwi, s2i, hwi);
     } else if (m.brdf == CHK_BRDF_BECKMANN) {
-      float a2 = m.alpha * m.alpha;
+      const T a2 = m.alpha * m.alpha;
       D = beck_D(a2, ch, s2h);
       G = beck_G1(m.alpha, cwo, s2o, hwo) * beck_G1(m.alpha, cwi, s2i, hwi);
     } else {
       D = phong_D(m.alpha, ch, s2h);
       G = phong_G1(m.alpha, cwo, s2o, hwo) * phong_G1(m.alpha, cwi, s2i, hwi);
     }
-    f = ((m.ks * F * D * G) / (4.0f * cwi * cwo) + diffuse_f) * m.refl;
-    pdf = m.ks * D * ch / (4.0f * hwo) + diffuse_p;
+    f = ((m.ks * F * D * G) / (T(4) * cwi * cwo) + diffuse_f) * m.refl;
+    pdf = m.ks * D * ch / (T(4) * hwo) + diffuse_p;
     return;
   }
-  f = 0.0f;
+  f = T(0);
   pdf = m.ks + diffuse_p;  // pdfEval's fall-through (materialTypes.cpp:111-114)
 }
 
 // getBrdfDirectionSampleRandomized (materialTypes.cpp:136-153).
-__device__ inline f3 brdf_sample(const Mat &m, f3 n, f3 wo_ref, f3 wo, float ulobe, float e1,
-                                 float e2) {
-  float sphi, cphi;
-  __sincosf(2.0f * kPi * e2, &sphi, &cphi);
-  f3 t, b;
+template <typename T>
+__device__ inline v3<T> brdf_sample(const MatT<T> &m, v3<T> n, v3<T> wo_ref, v3<T> wo, T ulobe,
+                                    T e1, T e2) {
+  T sphi, cphi;
+  sncs(T(2) * pi_v<T>() * e2, &sphi, &cphi);
+  v3<T> t, b;
   if (ulobe < m.ks) {
     if (m.brdf == CHK_BRDF_CLASSIC_PHONG) {
       // getClassicPhongDirectionSamples (domainSampler.cpp:264-314)
-      float ct = __powf(1.0f - e1, 1.0f / (m.pe + 1.0f));
-      float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+      const T ct = pw(T(1) - e1, T(1) / (m.pe + T(1)));
+      const T st = sqr_t(fmx(T(0), T(1) - ct * ct));
       onb(wo_ref, t, b);
       return t * (st * cphi) + b * (st * sphi) + wo_ref * ct;
     }
-    float ct = 1.0f, st = 0.0f;
+    T ct = T(1), st = T(0);
     if (m.brdf == CHK_BRDF_GGX) {
       // tan = alpha sqrt(e1 / (1 - e1)) (domainSampler.cpp:225)
-      float t2 = m.alpha * m.alpha * e1 / (1.0f - e1);
-      ct = rsqrtf(1.0f + t2);
-      st = sqrtf(t2) * ct;
+      const T t2 = m.alpha * m.alpha * e1 / (T(1) - e1);
+      ct = rsq(T(1) + t2);
+      st = sqr_t(t2) * ct;
     } else if (m.brdf == CHK_BRDF_BECKMANN) {
       // tan^2 = -alpha^2 ln(1 - e1) (domainSampler.cpp:230)
-      float t2 = -m.alpha * m.alpha * log1pf(-e1);
-      ct = rsqrtf(1.0f + t2);
-      st = sqrtf(t2) * ct;
+      const T t2 = -m.alpha * m.alpha * lg1p(-e1);
+      ct = rsq(T(1) + t2);
+      st = sqr_t(t2) * ct;
     } else if (m.brdf == CHK_BRDF_PHONG) {
-      ct = __powf(e1, 1.0f / (m.alpha + 2.0f));  // domainSampler.cpp:235
-      st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+      ct = pw(e1, T(1) / (m.alpha + T(2)));  // domainSampler.cpp:235
+      st = sqr_t(fmx(T(0), T(1) - ct * ct));
     }
     onb(n, t, b);
-    f3 h = t * (st * cphi) + b * (st * sphi) + n * ct;
-    return normalize(h * (2.0f * dot(h, wo)) - wo);
+    const v3<T> h = t * (st * cphi) + b * (st * sphi) + n * ct;
+    return normalize(h * (T(2) * dot(h, wo)) - wo);
   }
   // cosine lobe: sin(theta) = sqrt(e1) (domainSampler.cpp:39-49,154-189)
-  float st = sqrtf(e1), ct = sqrtf(1.0f - e1);
+  const T st = sqr_t(e1), ct = sqr_t(T(1) - e1);
   onb(n, t, b);
   return t * (st * cphi) + b * (st * sphi) + n * ct;
 }

@@ -58,6 +58,7 @@ struct chk_ctx_s {
   std::vector<double> ltc_host;
   void *ltc_scratch = nullptr; // FP64-pass point list + counter
   size_t ltc_scratch_bytes = 0;
+  float ltc_kappa = 8.0f;  // FP64 re-evaluation threshold of chk_ltc_eval
 };
 
 struct chk_scene_s {

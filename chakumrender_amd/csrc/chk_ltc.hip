@@ -18,24 +18,21 @@ namespace chk {
 namespace {
 
 constexpr int kLtcBlock = 256;
-// A point whose edge terms cancel by more than this factor (sum of |terms|
-// over |sum|, radiance- and part-weighted) loses more than ~1e-6 relative in
-// FP32 and is recomputed by the FP64 pass.
-constexpr float kLtcKappa = 8.0f;
+// A point whose edge terms cancel by more than the context's kappa (sum of
+// |terms| over |sum|, radiance- and part-weighted) is recomputed by the FP64
+// pass: each FP32 edge term carries a few ulps, so kappa bounds the relative
+// error of the point at about kappa * 3e-7.
 
 // integrateEdge (lightSources_CV.cpp:8-25): (v1 x v2).n * theta/sin(theta)
-__device__ inline float edge_factor(f3 v1, f3 v2, f3 c) {
+// for v_i = p_i/|p_i|, from the unnormalised p1, p2 and c = p1 x p2 with
+// cn = c.n.  Since (v1 x v2).n = cn/(|p1||p2|) and sin(theta) = |c|/(|p1||p2|),
+// the term is cn * theta/|c|, theta = atan2(|c|, p1.p2): no normalisation,
+// and the caller forms c from a light edge (p1 x (p2 - p1)) without the
+// cancellation of crossing two nearly parallel unit vectors.
+__device__ inline float edge_u(f3 p1, f3 p2, f3 c, float cn) {
   const float s = sqrtf(dot(c, c));
-  const float th = atan2f(s, dot(v1, v2));
-  return th > 0.001f ? th / s : 1.0f;
-}
-__device__ inline float edge_z(f3 v1, f3 v2) {
-  const f3 c = cross(v1, v2);
-  return c.z * edge_factor(v1, v2, c);
-}
-__device__ inline float edge_n(f3 v1, f3 v2, f3 n) {
-  const f3 c = cross(v1, v2);
-  return dot(c, n) * edge_factor(v1, v2, c);
+  const float th = atan2f(s, dot(p1, p2));
+  return th > 0.001f ? cn * (th / s) : cn * rsqrtf(dot(p1, p1) * dot(p2, p2));
 }
 
 // clipTriangle against z = 0 (lightSources_CV.cpp:129-219)
@@ -152,7 +149,10 @@ __global__ __launch_bounds__(kLtcBlock) void k_ltc(const float4 *__restrict__ ta
     const float a = alpha[i];
     const float kss = ks[i];
     // ---- M_GGX(acos(n.wo), alpha) (ltc.h:18-61), bins in double like the reference
-    const float theta = acosf(dot(n, wo));
+    // float(acos(double(n.v))) as M_GGX receives it (ltc.h:18-19); float
+    // products are exact in double, so the dot matches the reference's
+    const float theta =
+        (float)acos((double)n.x * wo.x + (double)n.y * wo.y + (double)n.z * wo.z);
     const double tq = (double)theta / (0.5 * 3.14159265359) * 64.0;
     const int tb = max(0, min(63, (int)floor(tq)));
     const int ab = max(0, min(63, (int)floor(sqrt((double)a) * 64.0)));
@@ -184,15 +184,16 @@ __global__ __launch_bounds__(kLtcBlock) void k_ltc(const float4 *__restrict__ ta
       const f3 rad = mk(lrad[3 * j], lrad[3 * j + 1], lrad[3 * j + 2]);
       const float rmax = fmaxf(fmaxf(rad.x, rad.y), rad.z);
       if (do_ltc) {
-        f3 e1 = row_mul(R0, R1, R2, A - x), e2 = row_mul(R0, R1, R2, B - x),
-           e3 = row_mul(R0, R1, R2, C - x);
+        const f3 e1 = row_mul(R0, R1, R2, A - x), e2 = row_mul(R0, R1, R2, B - x),
+                 e3 = row_mul(R0, R1, R2, C - x);
         float sum = 0.0f, mag = 0.0f;
         bool use = true;
         if (e1.z > 0.0f && e2.z > 0.0f && e3.z > 0.0f) {
-          e1 = normalize(e1);
-          e2 = normalize(e2);
-          e3 = normalize(e3);
-          const float t1 = edge_z(e1, e2), t2 = edge_z(e2, e3), t3 = edge_z(e3, e1);
+          const f3 c1 = cross(e1, row_mul(R0, R1, R2, B - A)),
+                   c2 = cross(e2, row_mul(R0, R1, R2, C - B)),
+                   c3 = cross(e3, row_mul(R0, R1, R2, A - C));
+          const float t1 = edge_u(e1, e2, c1, c1.z), t2 = edge_u(e2, e3, c2, c2.z),
+                      t3 = edge_u(e3, e1, c3, c3.z);
           sum = t1 + t2 + t3;
           mag = fabsf(t1) + fabsf(t2) + fabsf(t3);
         } else if (e1.z < 0.0f && e2.z < 0.0f && e3.z < 0.0f) {
@@ -201,10 +202,10 @@ __global__ __launch_bounds__(kLtcBlock) void k_ltc(const float4 *__restrict__ ta
           f3 cl[4] = {mk(0, 0, 0), mk(0, 0, 0), mk(0, 0, 0), mk(0, 0, 0)};
           if (clip_z(e1, e2, e3, e1.z * rsqrtf(dot(e1, e1)), e2.z * rsqrtf(dot(e2, e2)),
                      e3.z * rsqrtf(dot(e3, e3)), cl)) {
-            const f3 c1 = normalize(cl[0]), c2 = normalize(cl[1]), c3 = normalize(cl[2]),
-                     c4 = normalize(cl[3]);
-            const float t1 = edge_z(c1, c2), t2 = edge_z(c2, c3), t3 = edge_z(c3, c4),
-                        t4 = edge_z(c4, c1);
+            const f3 c1 = cross(cl[0], cl[1]), c2 = cross(cl[1], cl[2]), c3 = cross(cl[2], cl[3]),
+                     c4 = cross(cl[3], cl[0]);
+            const float t1 = edge_u(cl[0], cl[1], c1, c1.z), t2 = edge_u(cl[1], cl[2], c2, c2.z),
+                        t3 = edge_u(cl[2], cl[3], c3, c3.z), t4 = edge_u(cl[3], cl[0], c4, c4.z);
             sum = t1 + t2 + t3 + t4;
             mag = fabsf(t1) + fabsf(t2) + fabsf(t3) + fabsf(t4);
           } else {
@@ -221,12 +222,15 @@ __global__ __launch_bounds__(kLtcBlock) void k_ltc(const float4 *__restrict__ ta
         }
       }
       if (do_dif) {
-        const f3 e1 = normalize(A - x), e2 = normalize(B - x), e3 = normalize(C - x);
-        const float nA = dot(e1, n), nB = dot(e2, n), nC = dot(e3, n);
+        const f3 p1 = A - x, p2 = B - x, p3 = C - x;
+        // signs of n.(V - x) / |V - x| as the reference tests them
+        const float nA = dot(p1, n), nB = dot(p2, n), nC = dot(p3, n);
         float sum = 0.0f, mag = 0.0f;
         bool use = true;
         if (nA > 0.0f && nB > 0.0f && nC > 0.0f) {
-          const float t1 = edge_n(e1, e2, n), t2 = edge_n(e2, e3, n), t3 = edge_n(e3, e1, n);
+          const f3 c1 = cross(p1, B - A), c2 = cross(p2, C - B), c3 = cross(p3, A - C);
+          const float t1 = edge_u(p1, p2, c1, dot(c1, n)), t2 = edge_u(p2, p3, c2, dot(c2, n)),
+                      t3 = edge_u(p3, p1, c3, dot(c3, n));
           sum = t1 + t2 + t3;
           mag = fabsf(t1) + fabsf(t2) + fabsf(t3);
         } else if (nA < 0.0f && nB < 0.0f && nC < 0.0f) {
@@ -234,10 +238,11 @@ __global__ __launch_bounds__(kLtcBlock) void k_ltc(const float4 *__restrict__ ta
         } else {
           f3 cl[4] = {mk(0, 0, 0), mk(0, 0, 0), mk(0, 0, 0), mk(0, 0, 0)};
           if (clip_plane(A, B, C, n, x, nA, nB, nC, cl)) {
-            const f3 c1 = normalize(cl[0] - x), c2 = normalize(cl[1] - x),
-                     c3 = normalize(cl[2] - x), c4 = normalize(cl[3] - x);
-            const float t1 = edge_n(c1, c2, n), t2 = edge_n(c2, c3, n), t3 = edge_n(c3, c4, n),
-                        t4 = edge_n(c4, c1, n);
+            const f3 q1 = cl[0] - x, q2 = cl[1] - x, q3 = cl[2] - x, q4 = cl[3] - x;
+            const f3 c1 = cross(q1, q2), c2 = cross(q2, q3), c3 = cross(q3, q4),
+                     c4 = cross(q4, q1);
+            const float t1 = edge_u(q1, q2, c1, dot(c1, n)), t2 = edge_u(q2, q3, c2, dot(c2, n)),
+                        t3 = edge_u(q3, q4, c3, dot(c3, n)), t4 = edge_u(q4, q1, c4, dot(c4, n));
             sum = t1 + t2 + t3 + t4;
             mag = fabsf(t1) + fabsf(t2) + fabsf(t3) + fabsf(t4);
           } else {
@@ -282,18 +287,9 @@ __global__ __launch_bounds__(kLtcBlock) void k_ltc(const float4 *__restrict__ ta
 // ---------------------------------------------------------------- FP64 pass
 // The same computation as getLightFromMeshSource_Analytic in FP64, from the
 // FP64 table, for the points the FP32 pass flagged.
-struct d3 {
-  double x, y, z;
-};
-__device__ inline d3 dk(double x, double y, double z) { return d3{x, y, z}; }
-__device__ inline d3 operator+(d3 a, d3 b) { return dk(a.x + b.x, a.y + b.y, a.z + b.z); }
-__device__ inline d3 operator-(d3 a, d3 b) { return dk(a.x - b.x, a.y - b.y, a.z - b.z); }
-__device__ inline d3 operator*(d3 a, double s) { return dk(a.x * s, a.y * s, a.z * s); }
-__device__ inline double ddot(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ inline d3 dcross(d3 a, d3 b) {
-  return dk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
-}
-__device__ inline d3 dnorm(d3 a) { return a * (1.0 / sqrt(ddot(a, a))); }
+__device__ inline double ddot(d3 a, d3 b) { return dot(a, b); }
+__device__ inline d3 dcross(d3 a, d3 b) { return cross(a, b); }
+__device__ inline d3 dnorm(d3 a) { return normalize(a); }
 __device__ inline double dedge(d3 v1, d3 v2, d3 n) {  // lightSources_CV.cpp:8-15
   const double th = acos(ddot(v1, v2));
   return ddot(dcross(v1, v2), n) * ((th > 0.001) ? th / sin(th) : 1.0);
@@ -529,11 +525,20 @@ chk_status chk_ltc_eval(chk_ctx_t ctx, const float *x, const float *n, const flo
   const uint32_t blocks = std::min<uint32_t>((k + kLtcBlock - 1) / kLtcBlock, (uint32_t)ctx->n_cu * 16);
   hipLaunchKernelGGL(k_ltc, dim3(blocks), dim3(kLtcBlock), 0, st, (const float4 *)ctx->ltc_tab, x, n,
                      v, alpha, ks, k, light_tris, radiance, n_light_tris, rgb, fb_list, fb_count,
-                     kLtcKappa);
+                     ctx->ltc_kappa);
   CHK_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_ltc64, dim3(ctx->n_cu * 2), dim3(128), 0, st, (const double *)ctx->ltc_dev, x,
                      n, v, alpha, ks, light_tris, radiance, n_light_tris, rgb, fb_list, fb_count);
   CHK_HIP(hipGetLastError());
+  return CHK_OK;
+}
+
+chk_status chk_ltc_set_fp64_threshold(chk_ctx_t ctx, float kappa) {
+  if (!ctx || !(kappa >= 0.0f)) {
+    set_error("chk_ltc_set_fp64_threshold: null context or kappa not >= 0");
+    return CHK_EINVAL;
+  }
+  ctx->ltc_kappa = kappa;
   return CHK_OK;
 }
 

@@ -273,11 +273,14 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(
               const float pl = A.inv_nl * d2 / (L.area * cl);  // mis.cpp:55-66
               const f3 c = thr * m.c * xyz(lt.rad) * (f * cx / (pl + pb));
               if (finite3(c) && (c.x > 0.0f || c.y > 0.0f || c.z > 0.0f)) {
+                // aim from the offset origin at the sampled point, so the ray
+                // reaches the light exactly at t = sdist (shader.cpp:82-90)
                 const f3 sd = yp - org;
                 const float sdist = sqrtf(dot(sd, sd));
+                const f3 sdir = sd * (1.0f / sdist);
                 emit_sh = true;
-                sr0 = make_float4(org.x, org.y, org.z, sdist - 1e-5f);  // shader.cpp:82-90
-                sr1 = make_float4(wi.x, wi.y, wi.z, __uint_as_float(leaf));
+                sr0 = make_float4(org.x, org.y, org.z, sdist - 1e-5f);
+                sr1 = make_float4(sdir.x, sdir.y, sdir.z, __uint_as_float(leaf));
                 sl = make_float4(c.x, c.y, c.z, __uint_as_float(pid));
               }
             }

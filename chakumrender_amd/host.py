@@ -85,7 +85,11 @@ class BvhAccel:
 
 
 class MaterialType:
-    """Batched MaterialType (materialTypes.h:20-120) over a material table."""
+    """Batched MaterialType (materialTypes.h:20-120) over a material table.
+    FP64 directions in and FP64 results out, like the reference's Vec
+    interface; fp32=True evaluates with the render kernels' FP32 arithmetic."""
+
+    FP32 = 1  # CHK_BRDF_FP32
 
     def __init__(self, mats):
         self.mats = np.ascontiguousarray(mats)
@@ -94,26 +98,28 @@ class MaterialType:
     def _bufs(self, mat_idx, *vecs):
         md = DevBuf.from_array(self.mats.view(np.uint8))
         idx = DevBuf.from_array(np.asarray(mat_idx, np.uint32))
-        vs = [DevBuf.from_array(np.asarray(v, np.float32).reshape(-1, 3)) for v in vecs]
+        vs = [DevBuf.from_array(np.asarray(v, np.float64).reshape(-1, 3)) for v in vecs]
         return md, idx, vs
 
-    def brdf_pdf(self, mat_idx, n, wo_ref, wo, wi):
+    def brdf_pdf(self, mat_idx, n, wo_ref, wo, wi, fp32=False):
         """(BasePrimitive::brdf, MaterialType::pdfEval) per row."""
         k = len(mat_idx)
         md, idx, (nd, rd, od, wd) = self._bufs(mat_idx, n, wo_ref, wo, wi)
-        f, p = DevBuf(4 * k), DevBuf(4 * k)
+        f, p = DevBuf(8 * k), DevBuf(8 * k)
         abi.check(self.L.chk_brdf_eval(md.ptr, self.mats.shape[0], idx.ptr, nd.ptr, rd.ptr, od.ptr,
-                                       wd.ptr, k, f.ptr, p.ptr, None), "chk_brdf_eval")
-        return f.download((k,), np.float32), p.download((k,), np.float32)
+                                       wd.ptr, k, self.FP32 if fp32 else 0, f.ptr, p.ptr, None),
+                  "chk_brdf_eval")
+        return f.download((k,), np.float64), p.download((k,), np.float64)
 
-    def sample(self, mat_idx, n, wo_ref, wo, u):
+    def sample(self, mat_idx, n, wo_ref, wo, u, fp32=False):
         """getBrdfDirectionSampleRandomized with explicit uniforms u (k, 3)."""
         k = len(mat_idx)
         md, idx, (nd, rd, od, ud) = self._bufs(mat_idx, n, wo_ref, wo, u)
-        wi, p = DevBuf(12 * k), DevBuf(4 * k)
+        wi, p = DevBuf(24 * k), DevBuf(8 * k)
         abi.check(self.L.chk_brdf_sample(md.ptr, self.mats.shape[0], idx.ptr, nd.ptr, rd.ptr, od.ptr,
-                                         ud.ptr, k, wi.ptr, p.ptr, None), "chk_brdf_sample")
-        return wi.download((k, 3), np.float32), p.download((k,), np.float32)
+                                         ud.ptr, k, self.FP32 if fp32 else 0, wi.ptr, p.ptr, None),
+                  "chk_brdf_sample")
+        return wi.download((k, 3), np.float64), p.download((k,), np.float64)
 
 
 class LightSource:
@@ -138,6 +144,17 @@ class LightSource:
         rgb = DevBuf(12 * k)
         self.eval_dev(bufs, k, rgb)
         return rgb.download((k, 3), np.float32)
+
+    def set_fp64_threshold(self, kappa):
+        """Cancellation factor above which a point is recomputed in FP64."""
+        abi.check(self.L.chk_ltc_set_fp64_threshold(self.ctx.h, float(kappa)),
+                  "chk_ltc_set_fp64_threshold")
+
+    def fp64_count(self):
+        """Points of the last evaluation recomputed by the FP64 pass."""
+        c = C.c_uint32(0)
+        abi.check(self.L.chk_ltc_fp64_count(self.ctx.h, C.byref(c), None), "chk_ltc_fp64_count")
+        return int(c.value)
 
 
 class Proposal:

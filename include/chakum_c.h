@@ -147,19 +147,27 @@ chk_status chk_trace_any(chk_scene_t scene, const chk_ray *rays, uint32_t n, uin
 
 /* BRDF value and sampling pdf for k configurations.  Replaces
  * BasePrimitive::brdf (geometryPrimitives.cpp:7-9 -> MaterialType::brdf,
- * materialTypes.cpp:9-50) and MaterialType::pdfEval (materialTypes.cpp:89-116).
- * mats (n_mats), mat_idx (k), n, wo_ref, wo, wi (k x 3 floats), f, pdf (k):
- * all dev. */
+ * materialTypes.cpp:9-50; headers/materialTypes.h:100) and
+ * MaterialType::pdfEval (materialTypes.cpp:89-116; materialTypes.h:103).
+ * Like the reference's Vec arguments, the directions are FP64 and the
+ * evaluation is FP64.  mats (n_mats), mat_idx (k), n, wo_ref, wo, wi
+ * (k x 3), f, pdf (k): all dev.  flags: 0, or CHK_BRDF_FP32 to evaluate
+ * with the FP32 arithmetic of the fused render kernels on the FP32
+ * rounding of the inputs. */
+#define CHK_BRDF_FP32 1u
 chk_status chk_brdf_eval(const chk_material *mats, uint32_t n_mats, const uint32_t *mat_idx,
-                         const float *n, const float *wo_ref, const float *wo, const float *wi,
-                         uint32_t k, float *f, float *pdf, void *stream);
+                         const double *n, const double *wo_ref, const double *wo,
+                         const double *wi, uint32_t k, uint32_t flags, double *f, double *pdf,
+                         void *stream);
 /* One BSDF sample per configuration from three uniforms u (k x 3: lobe,
  * e1, e2).  Replaces MaterialType::getBrdfDirectionSampleRandomized
- * (materialTypes.cpp:136-153) with explicit random numbers instead of a
- * clock()-seeded generator.  wi (k x 3), pdf (k): dev. */
+ * (materialTypes.cpp:136-153; materialTypes.h:102) with explicit random
+ * numbers instead of a clock()-seeded generator.  wi (k x 3), pdf (k): dev.
+ * FP64 unless flags has CHK_BRDF_FP32. */
 chk_status chk_brdf_sample(const chk_material *mats, uint32_t n_mats, const uint32_t *mat_idx,
-                           const float *n, const float *wo_ref, const float *wo, const float *u,
-                           uint32_t k, float *wi, float *pdf, void *stream);
+                           const double *n, const double *wo_ref, const double *wo,
+                           const double *u, uint32_t k, uint32_t flags, double *wi, double *pdf,
+                           void *stream);
 
 /* Analytic polygon-light shading of k points of a GGX material (alpha[i],
  * ks[i]) by n_light_tris emitter triangles with per-triangle radiance.
@@ -176,6 +184,11 @@ chk_status chk_ltc_eval(chk_ctx_t ctx, const float *x, const float *n, const flo
  * flagged as ill-conditioned (edge terms cancelling) and recomputed in FP64.
  * Synchronises the stream. */
 chk_status chk_ltc_fp64_count(chk_ctx_t ctx, uint32_t *count, void *stream);
+/* Cancellation factor above which chk_ltc_eval recomputes a point in FP64
+ * (default 8: relative error of the FP32 result about kappa * 3e-7).
+ * 0 sends every lit point to the FP64 pass; INFINITY keeps all in FP32.
+ * No reference counterpart (the reference computes everything in FP64). */
+chk_status chk_ltc_set_fp64_threshold(chk_ctx_t ctx, float kappa);
 
 /* Render: the pixel loop of main.cpp:62-94 around shade() (shader.cpp:362),
  * with the configs' estimator: path tracing to params->max_depth vertices,

@@ -46,7 +46,7 @@ def test_null_arguments_rejected_without_device():
     assert L.chk_create(0, None) == 1
     assert L.chk_scene_get_info(None, None) == 1
     assert L.chk_ltc_eval(None, None, None, None, None, None, 0, None, None, 0, None, None) == 1
-    assert L.chk_brdf_eval(None, 0, None, None, None, None, None, 5, None, None, None) == 1
+    assert L.chk_brdf_eval(None, 0, None, None, None, None, None, 5, 0, None, None, None) == 1
     assert b"null" in L.chk_last_error()
 
 
