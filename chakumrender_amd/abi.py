@@ -98,8 +98,6 @@ SIGNATURES = {
                              C.POINTER(chk_render_stats), _vp]),
     "chk_render_count": (C.c_int, [_vp, C.POINTER(chk_camera), C.POINTER(chk_render_params), _vp,
                                    C.POINTER(chk_render_stats), _vp]),
-    "chk_ltc_fp64_count": (C.c_int, [_vp, C.POINTER(_u32), _vp]),
-    "chk_ltc_set_fp64_threshold": (C.c_int, [_vp, C.c_float]),
     "chk_proposal_create": (C.c_int, [_vp, _u32, C.c_uint64, C.POINTER(_vp)]),
     "chk_proposal_destroy": (C.c_int, [_vp]),
     "chk_proposal_set_params": (C.c_int, [_vp, _vp, C.c_size_t]),

@@ -228,9 +228,7 @@ chk_status chk_create(int device, chk_ctx_t *out) {
 chk_status chk_destroy(chk_ctx_t c) {
   if (!c) return CHK_OK;
   (void)hipSetDevice(c->device);
-  if (c->ltc_tab) (void)hipFree(c->ltc_tab);
   if (c->ltc_dev) (void)hipFree(c->ltc_dev);
-  if (c->ltc_scratch) (void)hipFree(c->ltc_scratch);
   delete c;
   return CHK_OK;
 }
@@ -419,21 +417,6 @@ chk_status chk_scene_get_info(chk_scene_t sc, chk_scene_info *info) {
   return CHK_OK;
 }
 
-static chk_status ensure_trace_scratch(chk_scene_s *sc) {
-  // 64 bytes of control words for the API trace calls live at the start of
-  // the render work buffer
-  if (sc->work_bytes >= 4096) return CHK_OK;
-  if (sc->work) (void)hipFree(sc->work);
-  sc->work = nullptr;
-  sc->work_bytes = 0;
-  if (hipMalloc(&sc->work, 4096) != hipSuccess) {
-    set_error("cannot allocate trace scratch");
-    return CHK_ENOMEM;
-  }
-  sc->work_bytes = 4096;
-  return CHK_OK;
-}
-
 chk_status chk_trace_closest(chk_scene_t sc, const chk_ray *rays, uint32_t n, chk_hit *hits,
                              void *stream) {
   CHK_TRY_BEGIN
@@ -442,15 +425,15 @@ chk_status chk_trace_closest(chk_scene_t sc, const chk_ray *rays, uint32_t n, ch
     return CHK_EINVAL;
   }
   if (n == 0) return CHK_OK;
+  if (n > kMaxTraceBatch) {
+    set_error("chk_trace_closest: more than 2^31 rays in one call");
+    return CHK_ECAPACITY;
+  }
   CHK_HIP(hipSetDevice(sc->ctx->device));
-  chk_status s = ensure_trace_scratch(sc);
-  if (s != CHK_OK) return s;
-  hipStream_t st = (hipStream_t)stream;
-  uint32_t *work = (uint32_t *)sc->work;
-  unsigned long long *cnt = (unsigned long long *)((char *)sc->work + 256);
-  CHK_HIP(launch_zero_u32(work, 64 + 16, st));
-  CHK_HIP(launch_closest(sc, (const float4 *)rays, n, nullptr, (float4 *)hits, false, work, false,
-                         cnt, trace_grid(sc), st));
+  // no scratch: the stack depth bound of the tree is exact, so calls on
+  // distinct streams share nothing but the read-only scene
+  CHK_HIP(launch_closest(sc, (const float4 *)rays, linear_range(n), (float4 *)hits, false, false,
+                         nullptr, (hipStream_t)stream));
   return CHK_OK;
   CHK_TRY_END
 }
@@ -463,15 +446,13 @@ chk_status chk_trace_any(chk_scene_t sc, const chk_ray *rays, uint32_t n, uint8_
     return CHK_EINVAL;
   }
   if (n == 0) return CHK_OK;
+  if (n > kMaxTraceBatch) {
+    set_error("chk_trace_any: more than 2^31 rays in one call");
+    return CHK_ECAPACITY;
+  }
   CHK_HIP(hipSetDevice(sc->ctx->device));
-  chk_status s = ensure_trace_scratch(sc);
-  if (s != CHK_OK) return s;
-  hipStream_t st = (hipStream_t)stream;
-  uint32_t *work = (uint32_t *)sc->work;
-  unsigned long long *cnt = (unsigned long long *)((char *)sc->work + 256);
-  CHK_HIP(launch_zero_u32(work, 64 + 16, st));
-  CHK_HIP(launch_any(sc, (const float4 *)rays, n, nullptr, occluded, nullptr, nullptr, false, work,
-                     false, cnt, trace_grid(sc), st));
+  CHK_HIP(launch_any(sc, (const float4 *)rays, linear_range(n), occluded, nullptr, nullptr, false,
+                     false, nullptr, (hipStream_t)stream));
   return CHK_OK;
   CHK_TRY_END
 }

@@ -53,12 +53,7 @@ struct DeviceScene {
 struct chk_ctx_s {
   int device = 0;
   int n_cu = 256;
-  double *ltc_dev = nullptr;   // packed FP64 table for the FP64 LTC pass (6 per record)
-  float *ltc_tab = nullptr;    // packed FP32 table for the LTC kernel (see chk_ltc.hip)
-  std::vector<double> ltc_host;
-  void *ltc_scratch = nullptr; // FP64-pass point list + counter
-  size_t ltc_scratch_bytes = 0;
-  float ltc_kappa = 8.0f;  // FP64 re-evaluation threshold of chk_ltc_eval
+  double *ltc_dev = nullptr;  // packed LTC table (6 doubles per record, chk_ltc.hip)
 };
 
 struct chk_scene_s {
@@ -75,25 +70,44 @@ struct chk_scene_s {
 
 namespace chk {
 
+// Work layout of a traversal launch: kSegs segments of segcap items each
+// (item (r, k) at index r * segcap + k).  Segment r holds count[r *
+// kSegStride] items when count is set, else the items of [r * segcap,
+// (r + 1) * segcap) below n (a plain array of n items).  segcap is a
+// multiple of 256 (the largest block size of the consumers).
+constexpr uint32_t kSegs = 8;
+constexpr uint32_t kSegStride = 64;  // counters 256 bytes apart
+struct SegRange {
+  const uint32_t *count;
+  uint32_t n, segcap;
+  __host__ __device__ uint32_t count_of(uint32_t r) const {
+    if (count) return count[r * kSegStride];
+    const uint32_t lo = r * segcap;
+    return n > lo ? (n - lo < segcap ? n - lo : segcap) : 0u;
+  }
+};
+inline uint32_t seg_capacity(uint32_t n) {
+  const uint32_t per = (uint32_t)(((uint64_t)n + kSegs - 1) / kSegs);
+  return (per + 255u) & ~255u;
+}
+constexpr uint32_t kMaxTraceBatch = 1u << 31;  // rays per trace call
+inline SegRange linear_range(uint32_t n) { return SegRange{nullptr, n, seg_capacity(n)}; }
+
 // Kernel launchers (chk_trace.hip).
-// Closest hit over rays (chk_ray layout).  count_dev != nullptr: ray count
-// read from device memory (at most n).  leaf_ids: report/skip leaf indices
-// instead of original ones.  work_counter must be zero at launch.
-// counters (dev, 3 x u64): [node visits, tri tests] accumulated when
-// `count`, [2] = lanes whose stack overflowed (always).
-hipError_t launch_closest(const chk_scene_s *s, const float4 *rays, uint32_t n,
-                          const uint32_t *count_dev, float4 *hits, bool leaf_ids,
-                          uint32_t *work_counter, bool count, unsigned long long *counters,
-                          int grid, hipStream_t st);
+// Closest hit over rays (chk_ray layout) laid out by W.  leaf_ids:
+// report/skip leaf indices instead of original ones.  counters (dev, 4 x
+// u64, may be null): [node visits, tri tests] accumulated when `count`,
+// [2] = lanes whose stack overflowed, [3] += rays traced.
+hipError_t launch_closest(const chk_scene_s *s, const float4 *rays, const SegRange &W,
+                          float4 *hits, bool leaf_ids, bool count, unsigned long long *counters,
+                          hipStream_t st);
 // Any hit.  occluded (may be null) receives 0/1; with path_L, unoccluded
 // rays add shadow_L[i].xyz to path_L[bits(shadow_L[i].w)].
-hipError_t launch_any(const chk_scene_s *s, const float4 *rays, uint32_t n,
-                      const uint32_t *count_dev, uint8_t *occluded, const float4 *shadow_L,
-                      float4 *path_L, bool leaf_ids, uint32_t *work_counter, bool count,
-                      unsigned long long *counters, int grid, hipStream_t st);
+hipError_t launch_any(const chk_scene_s *s, const float4 *rays, const SegRange &W,
+                      uint8_t *occluded, const float4 *shadow_L, float4 *path_L, bool leaf_ids,
+                      bool count, unsigned long long *counters, hipStream_t st);
 hipError_t launch_zero_u32(uint32_t *p, uint32_t n, hipStream_t st);
 
 int pick_stack_size(uint32_t bound);
-int trace_grid(const chk_scene_s *s);
 
 }  // namespace chk

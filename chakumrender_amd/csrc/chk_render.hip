@@ -11,7 +11,11 @@
 //     closest hit over the live-ray queue          (bvhAccel.cpp:200-249)
 //     k_shade: emission + MIS weight, one light sample, one BSDF sample;
 //              survivors and shadow rays are appended to compacted queues
-//              by a wave ballot + one atomic per wave
+//              by a block-wide ballot prefix + one atomic per block
+//   Every queue is split into kSegs segments with their own counters; the
+//   blocks of class r = blockIdx % kSegs read and append segment r only, so
+//   a path stays in one segment (one XCD under round-robin dispatch) for its
+//   whole life and appends never contend across XCDs.
 //     any hit over the shadow queue                (shader.cpp:76-97)
 //   k_accumulate: per-pixel FP64 sums in sample order (main.cpp:62,83)
 // Every path owns one FP32 radiance slot; queue order never changes a
@@ -29,25 +33,25 @@ namespace {
 constexpr uint32_t kChunkPaths = 1u << 22;  // paths in flight per chunk
 constexpr int kShadeBlock = 256;
 
-// control words in the work buffer
+// control words in the work buffer: per-segment queue counters, each
+// segment's counter kSegStride words (256 bytes) after the previous one
 enum Ctrl : uint32_t {
-  kQ0 = 0,       // live-ray count, queue 0
-  kQ1 = 1,       // live-ray count, queue 1
-  kQS = 2,       // shadow-ray count
-  kWorkExt = 3,  // traversal work counters
-  kWorkAny = 4,
-  kCtrlWords = 16
+  kQ0 = 0,                        // live-ray counts, queue 0
+  kQ1 = kSegs * kSegStride,       // live-ray counts, queue 1
+  kQS = 2 * kSegs * kSegStride,   // shadow-ray counts
+  kCtrlWords = 3 * kSegs * kSegStride
 };
-// 64-bit statistics words
+// 64-bit statistics words: closest-hit group, then any-hit group, in the
+// order of the traversal kernels' counters
 enum Stat : uint32_t {
   kNodeVisits = 0,
   kTriTests = 1,
   kOverflow = 2,
   kRaysExt = 3,
-  kRaysShadow = 4,
-  kShNodeVisits = 5,
-  kShTriTests = 6,
-  kShOverflow = 7,
+  kShNodeVisits = 4,
+  kShTriTests = 5,
+  kShOverflow = 6,
+  kRaysShadow = 7,
   kStatWords = 8
 };
 
@@ -76,14 +80,16 @@ struct Buffers {
   float4 *path_L;
 };
 
-size_t buffers_bytes(uint32_t cap) {
-  size_t b = 256 + 256;
-  b += 2 * (size_t)cap * (32 + 16 + 4);
-  b += (size_t)cap * (16 + 32 + 16 + 16);
+// qcap: queue slots (kSegs * segcap); cap: paths per chunk
+size_t buffers_bytes(uint32_t qcap, uint32_t cap) {
+  size_t b = (size_t)kCtrlWords * 4 + 256 + 256;
+  b += 2 * (size_t)qcap * (32 + 16 + 4);
+  b += (size_t)qcap * (16 + 32 + 16);
+  b += (size_t)cap * 16;
   return b + 16 * 256;
 }
 
-Buffers carve(void *base, uint32_t cap) {
+Buffers carve(void *base, uint32_t qcap, uint32_t cap) {
   Buffers B;
   char *p = (char *)base;
   auto take = [&](size_t bytes) {
@@ -94,69 +100,87 @@ Buffers carve(void *base, uint32_t cap) {
   B.ctrl = (uint32_t *)take(kCtrlWords * 4);
   B.stats = (unsigned long long *)take(kStatWords * 8);
   for (int k = 0; k < 2; ++k) {
-    B.q_ray[k] = (float4 *)take((size_t)cap * 32);
-    B.q_thr[k] = (float4 *)take((size_t)cap * 16);
-    B.q_pid[k] = (uint32_t *)take((size_t)cap * 4);
+    B.q_ray[k] = (float4 *)take((size_t)qcap * 32);
+    B.q_thr[k] = (float4 *)take((size_t)qcap * 16);
+    B.q_pid[k] = (uint32_t *)take((size_t)qcap * 4);
   }
-  B.hits = (float4 *)take((size_t)cap * 16);
-  B.s_ray = (float4 *)take((size_t)cap * 32);
-  B.s_L = (float4 *)take((size_t)cap * 16);
+  B.hits = (float4 *)take((size_t)qcap * 16);
+  B.s_ray = (float4 *)take((size_t)qcap * 32);
+  B.s_L = (float4 *)take((size_t)qcap * 16);
   B.path_L = (float4 *)take((size_t)cap * 16);
   return B;
 }
 
 __device__ inline uint32_t lane_id() { return __lane_id(); }
 
-// Compacting append: every lane of the wave calls it (converged); lanes
-// with pred get consecutive slots.  One atomic per wave.
-__device__ inline uint32_t wave_append(bool pred, uint32_t *counter, unsigned long long *stat,
-                                       uint32_t &n_out) {
-  const unsigned long long mask = __ballot(pred);
-  const uint32_t cnt = (uint32_t)__popcll(mask);
-  n_out = cnt;
-  uint32_t base = 0;
-  const int leader = cnt ? (__ffsll((long long)mask) - 1) : 0;
-  if (cnt && (int)lane_id() == leader) {
-    base = atomicAdd(counter, cnt);
-    atomicAdd(stat, (unsigned long long)cnt);
+// Block-wide compacting append into two queues: every thread of the block
+// calls it once; threads with pa (pb) get consecutive slots after the
+// block's base in queue a (b).  One atomic per queue per block.
+template <int NW>
+__device__ inline void block_append2(bool pa, bool pb, uint32_t *ca, uint32_t *cb, uint32_t &sa,
+                                     uint32_t &sb) {
+  __shared__ uint32_t cnt[2][NW];
+  __shared__ uint32_t base[2];
+  const uint32_t lane = __lane_id(), w = threadIdx.x >> 6;
+  const unsigned long long ma = __ballot(pa), mb = __ballot(pb);
+  if (lane == 0) {
+    cnt[0][w] = (uint32_t)__popcll(ma);
+    cnt[1][w] = (uint32_t)__popcll(mb);
   }
-  base = __shfl(base, leader);
-  const unsigned long long below = lane_id() ? (mask & ((~0ull) >> (64 - lane_id()))) : 0ull;
-  return base + (uint32_t)__popcll(below);
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    uint32_t tot = 0;
+    for (int k = 0; k < NW; ++k) tot += cnt[threadIdx.x][k];
+    base[threadIdx.x] = tot ? atomicAdd(threadIdx.x ? cb : ca, tot) : 0u;
+  }
+  __syncthreads();
+  uint32_t oa = base[0], ob = base[1];
+  for (uint32_t k = 0; k < w; ++k) {
+    oa += cnt[0][k];
+    ob += cnt[1][k];
+  }
+  const unsigned long long below = (1ull << lane) - 1ull;
+  sa = oa + (uint32_t)__popcll(ma & below);
+  sb = ob + (uint32_t)__popcll(mb & below);
 }
 
 __global__ void k_reset(uint32_t *ctrl, unsigned long long *stats, int clear_stats) {
-  const uint32_t i = threadIdx.x;
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < kCtrlWords) ctrl[i] = 0u;
   if (clear_stats && i < kStatWords) stats[i] = 0ull;
 }
 
+// zero the output queue's and the shadow queue's segment counters
 __global__ void k_reset_bounce(uint32_t *ctrl, uint32_t out_q) {
-  if (threadIdx.x == 0) {
-    ctrl[out_q] = 0u;
-    ctrl[kQS] = 0u;
-    ctrl[kWorkExt] = 0u;
-    ctrl[kWorkAny] = 0u;
-  }
+  const uint32_t t = threadIdx.x;
+  if (t < kSegs) ctrl[out_q + t * kSegStride] = 0u;
+  else if (t < 2 * kSegs) ctrl[kQS + (t - kSegs) * kSegStride] = 0u;
 }
 
 // Camera rays: main.cpp:72-79 with prepareScene.cpp:24-33's cx, cy.
-__global__ void k_camera(RenderArgs A, float4 *__restrict__ q_ray, float4 *__restrict__ q_thr,
-                         uint32_t *__restrict__ q_pid, float4 *__restrict__ path_L,
-                         uint32_t *__restrict__ ctrl) {
+// Path p of the chunk goes to segment p / per, per = ceil(n / kSegs).
+__global__ void k_camera(RenderArgs A, uint32_t segcap, float4 *__restrict__ q_ray,
+                         float4 *__restrict__ q_thr, uint32_t *__restrict__ q_pid,
+                         float4 *__restrict__ path_L, uint32_t *__restrict__ ctrl) {
   const uint32_t n = A.n_pix * A.spp;
+  const uint32_t per = (n + kSegs - 1) / kSegs;
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p == 0) ctrl[kQ0] = n;
+  if (p < kSegs) {
+    const uint32_t lo = p * per;
+    ctrl[kQ0 + p * kSegStride] = n > lo ? min(per, n - lo) : 0u;
+  }
   if (p >= n) return;
+  const uint32_t r = p / per;
+  const size_t slot = (size_t)r * segcap + (p - r * per);
   const uint32_t idx = A.pix0 + p / A.spp;
   const uint32_t s = A.sample_offset + p % A.spp;
   const uint32_t row = idx / A.cam.W, x = idx % A.cam.W;
   const uint32_t y = A.cam.H - 1u - row;
   float jx = 0.34f, jy = 0.20f;  // main.cpp:23-24, k = 0
   if (A.jitter_mode == CHK_JITTER_BOX) {
-    u4 r = philox(u4{idx, s, (0u << 2) | CHK_STREAM_CAM, A.req}, A.key0, A.key1);
-    jx = u01(r.x);
-    jy = u01(r.y);
+    u4 rr = philox(u4{idx, s, (0u << 2) | CHK_STREAM_CAM, A.req}, A.key0, A.key1);
+    jx = u01(rr.x);
+    jy = u01(rr.y);
   }
   const float fx = ((float)x + jx) / (float)A.cam.W - 0.5f;
   const float fy = ((float)y + jy) / (float)A.cam.H - 0.5f;
@@ -164,10 +188,10 @@ __global__ void k_camera(RenderArgs A, float4 *__restrict__ q_ray, float4 *__res
          mk(A.cam.cy[0], A.cam.cy[1], A.cam.cy[2]) * fy +
          mk(A.cam.dir[0], A.cam.dir[1], A.cam.dir[2]);
   d = normalize(d);
-  q_ray[2 * (size_t)p] = make_float4(A.cam.pos[0], A.cam.pos[1], A.cam.pos[2], kInf);
-  q_ray[2 * (size_t)p + 1] = make_float4(d.x, d.y, d.z, __uint_as_float(0xFFFFFFFFu));
-  q_thr[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-  q_pid[p] = p;
+  q_ray[2 * slot] = make_float4(A.cam.pos[0], A.cam.pos[1], A.cam.pos[2], kInf);
+  q_ray[2 * slot + 1] = make_float4(d.x, d.y, d.z, __uint_as_float(0xFFFFFFFFu));
+  q_thr[slot] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+  q_pid[slot] = p;
   path_L[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
@@ -182,25 +206,25 @@ __device__ inline bool finite3(f3 a) {
   return isfinite(a.x) && isfinite(a.y) && isfinite(a.z);
 }
 
-// One vertex of every live path.
+// One vertex of every live path.  Block b shades items of segment
+// r = b % kSegs and appends to segment r of the output queues.
 __global__ __launch_bounds__(kShadeBlock) void k_shade(
-    RenderArgs A, uint32_t depth, const float4 *__restrict__ in_ray,
+    RenderArgs A, uint32_t depth, uint32_t segcap, const float4 *__restrict__ in_ray,
     const float4 *__restrict__ in_thr, const uint32_t *__restrict__ in_pid,
     const float4 *__restrict__ hits, float4 *__restrict__ out_ray, float4 *__restrict__ out_thr,
     uint32_t *__restrict__ out_pid, float4 *__restrict__ s_ray, float4 *__restrict__ s_L,
-    float4 *__restrict__ path_L, uint32_t *__restrict__ ctrl, uint32_t in_q, uint32_t out_q,
-    unsigned long long *__restrict__ stats) {
+    float4 *__restrict__ path_L, uint32_t *__restrict__ ctrl, uint32_t in_q, uint32_t out_q) {
   const DeviceScene &S = A.S;
-  const uint32_t total = ctrl[in_q];
-  const uint32_t lane = lane_id();
-  const uint32_t wave = threadIdx.x >> 6;
-  const uint32_t stride = gridDim.x * kShadeBlock;
-  for (uint32_t base = blockIdx.x * kShadeBlock + wave * 64; base < total; base += stride) {
-    const uint32_t i = base + lane;
+  const uint32_t r = blockIdx.x % kSegs, qb = blockIdx.x / kSegs;
+  const uint32_t total = ctrl[in_q + r * kSegStride];
+  if (qb * kShadeBlock >= total) return;  // whole block idle (uniform)
+  const uint32_t off = qb * kShadeBlock + threadIdx.x;
+  const size_t i = (size_t)r * segcap + off;
+  {
     bool emit_ray = false, emit_sh = false;
     float4 nr0, nr1, nthr, sr0, sr1, sl;
     uint32_t pid = 0;
-    if (i < total) {
+    if (off < total) {
       const float4 h = hits[i];
       const uint32_t leaf = __float_as_uint(h.y);
       pid = in_pid[i];
@@ -307,19 +331,22 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(
         }
       }
     }
-    uint32_t nc;
-    const uint32_t slot = wave_append(emit_ray, ctrl + out_q, stats + kRaysExt, nc);
+    uint32_t slot, sslot;
+    block_append2<kShadeBlock / 64>(emit_ray, emit_sh, ctrl + out_q + r * kSegStride,
+                                    ctrl + kQS + r * kSegStride, slot, sslot);
+    const size_t seg0 = (size_t)r * segcap;
     if (emit_ray) {
-      out_ray[2 * (size_t)slot] = nr0;
-      out_ray[2 * (size_t)slot + 1] = nr1;
-      out_thr[slot] = nthr;
-      out_pid[slot] = pid;
+      const size_t o = seg0 + slot;
+      out_ray[2 * o] = nr0;
+      out_ray[2 * o + 1] = nr1;
+      out_thr[o] = nthr;
+      out_pid[o] = pid;
     }
-    const uint32_t sslot = wave_append(emit_sh, ctrl + kQS, stats + kRaysShadow, nc);
     if (emit_sh) {
-      s_ray[2 * (size_t)sslot] = sr0;
-      s_ray[2 * (size_t)sslot + 1] = sr1;
-      s_L[sslot] = sl;
+      const size_t o = seg0 + sslot;
+      s_ray[2 * o] = sr0;
+      s_ray[2 * o + 1] = sr1;
+      s_L[o] = sl;
     }
   }
 }
@@ -424,7 +451,9 @@ chk_status render_impl(chk_scene_s *sc, const chk_camera *cam, const chk_render_
   const uint32_t pix_per_chunk = std::max<uint32_t>(1u, kChunkPaths / P->spp);
   const uint32_t cap = (uint32_t)std::min<uint64_t>((uint64_t)pix_per_chunk * P->spp,
                                                    n_pix_total * P->spp);
-  const size_t need = buffers_bytes(cap);
+  const uint32_t segcap = seg_capacity(cap);
+  const uint32_t qcap = kSegs * segcap;
+  const size_t need = buffers_bytes(qcap, cap);
   if (sc->work_bytes < need) {
     if (sc->work) (void)hipFree(sc->work);
     sc->work = nullptr;
@@ -435,7 +464,7 @@ chk_status render_impl(chk_scene_s *sc, const chk_camera *cam, const chk_render_
     }
     sc->work_bytes = need;
   }
-  Buffers B = carve(sc->work, cap);
+  Buffers B = carve(sc->work, qcap, cap);
 
   RenderArgs A;
   std::memset(&A, 0, sizeof(A));
@@ -482,28 +511,29 @@ chk_status render_impl(chk_scene_s *sc, const chk_camera *cam, const chk_render_
   Profiler prof;
   prof.on = (P->flags & CHK_RENDER_PROFILE) != 0;
   prof.st = st;
-  const int tgrid = trace_grid(sc);
-  const int sgrid = sc->ctx->n_cu * 8;
+  const uint32_t sgrid = kSegs * (segcap / kShadeBlock);
   uint32_t chunks = 0, ext_launches = 0;
+  auto qbase = [](uint32_t q) { return q == 0 ? (uint32_t)kQ0 : (uint32_t)kQ1; };
 
-  hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, B.ctrl, B.stats, 1);
+  hipLaunchKernelGGL(k_reset, dim3((kCtrlWords + 255) / 256), dim3(256), 0, st, B.ctrl, B.stats,
+                     1);
   for (uint64_t pix0 = 0; pix0 < n_pix_total; pix0 += pix_per_chunk) {
     ++chunks;
     A.pix0 = (uint32_t)pix0;
     A.n_pix = (uint32_t)std::min<uint64_t>(pix_per_chunk, n_pix_total - pix0);
     const uint32_t n_paths = A.n_pix * A.spp;
     prof.begin(3);
-    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, B.ctrl, B.stats, 0);
-    hipLaunchKernelGGL(k_camera, dim3((n_paths + 255) / 256), dim3(256), 0, st, A, B.q_ray[0],
-                       B.q_thr[0], B.q_pid[0], B.path_L, B.ctrl);
+    hipLaunchKernelGGL(k_reset, dim3((kCtrlWords + 255) / 256), dim3(256), 0, st, B.ctrl, B.stats,
+                       0);
+    hipLaunchKernelGGL(k_camera, dim3((n_paths + 255) / 256), dim3(256), 0, st, A, segcap,
+                       B.q_ray[0], B.q_thr[0], B.q_pid[0], B.path_L, B.ctrl);
     prof.end();
     for (uint32_t depth = 0; depth <= P->max_depth; ++depth) {
       const uint32_t in = depth & 1u, out = in ^ 1u;
-      hipLaunchKernelGGL(k_reset_bounce, dim3(1), dim3(64), 0, st, B.ctrl, out == 0 ? kQ0 : kQ1);
+      hipLaunchKernelGGL(k_reset_bounce, dim3(1), dim3(64), 0, st, B.ctrl, qbase(out));
       prof.begin(0);
-      hipError_t e = launch_closest(sc, B.q_ray[in], cap, B.ctrl + (in == 0 ? kQ0 : kQ1), B.hits,
-                                    true, B.ctrl + kWorkExt, count, B.stats + kNodeVisits, tgrid,
-                                    st);
+      hipError_t e = launch_closest(sc, B.q_ray[in], SegRange{B.ctrl + qbase(in), 0u, segcap},
+                                    B.hits, true, count, B.stats + kNodeVisits, st);
       prof.end();
       ++ext_launches;
       if (e != hipSuccess) {
@@ -511,16 +541,15 @@ chk_status render_impl(chk_scene_s *sc, const chk_camera *cam, const chk_render_
         return CHK_EDEVICE;
       }
       prof.begin(1);
-      hipLaunchKernelGGL(k_shade, dim3(sgrid), dim3(kShadeBlock), 0, st, A, depth, B.q_ray[in],
-                         B.q_thr[in], B.q_pid[in], B.hits, B.q_ray[out], B.q_thr[out],
-                         B.q_pid[out], B.s_ray, B.s_L, B.path_L, B.ctrl,
-                         in == 0 ? (uint32_t)kQ0 : (uint32_t)kQ1,
-                         out == 0 ? (uint32_t)kQ0 : (uint32_t)kQ1, B.stats);
+      hipLaunchKernelGGL(k_shade, dim3(sgrid), dim3(kShadeBlock), 0, st, A, depth, segcap,
+                         B.q_ray[in], B.q_thr[in], B.q_pid[in], B.hits, B.q_ray[out],
+                         B.q_thr[out], B.q_pid[out], B.s_ray, B.s_L, B.path_L, B.ctrl, qbase(in),
+                         qbase(out));
       prof.end();
       if (depth < P->max_depth) {
         prof.begin(2);
-        e = launch_any(sc, B.s_ray, cap, B.ctrl + kQS, nullptr, B.s_L, B.path_L, true,
-                       B.ctrl + kWorkAny, count, B.stats + kShNodeVisits, tgrid, st);
+        e = launch_any(sc, B.s_ray, SegRange{B.ctrl + kQS, 0u, segcap}, nullptr, B.s_L, B.path_L,
+                       true, count, B.stats + kShNodeVisits, st);
         prof.end();
         if (e != hipSuccess) {
           set_error(std::string("chk_render: any-hit launch: ") + hipGetErrorString(e));
@@ -540,7 +569,7 @@ chk_status render_impl(chk_scene_s *sc, const chk_camera *cam, const chk_render_
     CHK_HIP(hipMemcpyAsync(h, B.stats, sizeof(h), hipMemcpyDeviceToHost, st));
     CHK_HIP(hipStreamSynchronize(st));
     stats->paths = n_pix_total * P->spp;
-    stats->rays_extend = stats->paths + h[kRaysExt];
+    stats->rays_extend = h[kRaysExt];
     stats->rays_shadow = h[kRaysShadow];
     stats->node_visits = h[kNodeVisits];
     stats->tri_tests = h[kTriTests];

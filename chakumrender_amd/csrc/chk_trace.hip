@@ -2,7 +2,9 @@
 //
 // Restates BvhAccel::intersectT (bvhAccel.cpp:200-249) and shadow()
 // (shader.cpp:76-97) over the 4-wide SoA tree of chk_bvh.cpp:
-//  * persistent waves fetch 64 rays at a time from a device work counter;
+//  * one ray per lane; the rays of a launch are split into kSegs segments
+//    and block b works on segment b % kSegs, so with round-robin dispatch
+//    each XCD (and its L2) serves one segment; no global atomics;
 //  * one ray per lane, the current node in VGPRs, pending nodes on a
 //    per-lane stack in LDS laid out [slot][lane] (bank-conflict free);
 //  * child boxes are tested with the robust slab test of Ize ("Robust BVH
@@ -133,59 +135,59 @@ __device__ inline float refine_t(const float4 *tp, f3 o, f3 d) {
 
 template <bool ANY, bool COUNT, int STACK>
 __global__ __launch_bounds__(kBlock) void k_trace(DeviceScene S, const float4 *__restrict__ rays,
-                                                  uint32_t n, const uint32_t *__restrict__ count_dev,
-                                                  float4 *__restrict__ hits,
+                                                  SegRange W, float4 *__restrict__ hits,
                                                   uint8_t *__restrict__ occluded,
                                                   const float4 *__restrict__ shadow_L,
                                                   float4 *__restrict__ path_L, int leaf_ids,
-                                                  uint32_t *__restrict__ work,
                                                   unsigned long long *__restrict__ counters) {
   __shared__ uint32_t stack_mem[STACK * kBlock];
   uint32_t *stk = stack_mem + threadIdx.x;
   const uint32_t lane = threadIdx.x & 63u;
-  uint32_t total = n;
-  if (count_dev) total = min(*count_dev, n);
+  // block -> (segment, block within the segment); consecutive blocks go to
+  // consecutive XCDs, so each segment's rays stay on one XCD's L2
+  const uint32_t r = blockIdx.x % kSegs, q = blockIdx.x / kSegs;
+  if (counters && blockIdx.x == 0 && threadIdx.x == 0) {
+    unsigned long long tot = 0;
+    for (uint32_t k = 0; k < kSegs; ++k) tot += W.count_of(k);
+    atomicAdd(&counters[3], tot);  // rays traced by this launch
+  }
+  const uint32_t off = q * kBlock + threadIdx.x;
+  const uint32_t cnt = W.count_of(r);
+  if (q * kBlock >= cnt) return;  // whole block idle (uniform)
   uint32_t nv = 0, nt = 0, overflow = 0;
-  while (true) {
-    uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(work, 64u);
-    base = __shfl(base, 0);
-    if (base >= total) break;
-    const uint32_t i = base + lane;
-    if (i < total) {
-      const float4 ot = rays[2 * (size_t)i], ds = rays[2 * (size_t)i + 1];
-      float t, u, v;
-      uint32_t id, leaf;
-      bool any;
-      traverse<ANY, COUNT, STACK>(S, stk, xyz(ot), xyz(ds), ot.w, __float_as_uint(ds.w),
-                                  leaf_ids != 0, t, id, u, v, any, leaf, nv, nt, overflow);
-      // the caller-facing query reports t in FP64 accuracy
-      if (!ANY && !leaf_ids && id != 0xFFFFFFFFu)
-        t = refine_t(S.tris + 3 * (size_t)leaf, xyz(ot), xyz(ds));
-      if (ANY) {
-        if (occluded) occluded[i] = any ? 1 : 0;
-        if (path_L && !any) {
-          const float4 c = shadow_L[i];
-          const uint32_t pid = __float_as_uint(c.w);
-          float4 L = path_L[pid];
-          L.x += c.x;
-          L.y += c.y;
-          L.z += c.z;
-          path_L[pid] = L;
-        }
-      } else {
-        hits[i] = make_float4(t, __uint_as_float(id), u, v);
+  if (off < cnt) {
+    const size_t i = (size_t)r * W.segcap + off;
+    const float4 ot = rays[2 * i], ds = rays[2 * i + 1];
+    float t, u, v;
+    uint32_t id, leaf;
+    bool any;
+    traverse<ANY, COUNT, STACK>(S, stk, xyz(ot), xyz(ds), ot.w, __float_as_uint(ds.w),
+                                leaf_ids != 0, t, id, u, v, any, leaf, nv, nt, overflow);
+    // the caller-facing query reports t in FP64 accuracy
+    if (!ANY && !leaf_ids && id != 0xFFFFFFFFu)
+      t = refine_t(S.tris + 3 * (size_t)leaf, xyz(ot), xyz(ds));
+    if (ANY) {
+      if (occluded) occluded[i] = any ? 1 : 0;
+      if (path_L && !any) {
+        const float4 c = shadow_L[i];
+        const uint32_t pid = __float_as_uint(c.w);
+        float4 L = path_L[pid];
+        L.x += c.x;
+        L.y += c.y;
+        L.z += c.z;
+        path_L[pid] = L;
       }
+    } else {
+      hits[i] = make_float4(t, __uint_as_float(id), u, v);
     }
   }
   // wave-uniform condition: every lane of the wave takes part in the shuffles
   if (COUNT || __any(overflow != 0)) {
-    // wave reduction, one atomic per wave
     unsigned long long a = nv, b = nt, c = overflow;
-    for (int off = 32; off > 0; off >>= 1) {
-      a += __shfl_xor(a, off);
-      b += __shfl_xor(b, off);
-      c += __shfl_xor(c, off);
+    for (int o = 32; o > 0; o >>= 1) {
+      a += __shfl_xor(a, o);
+      b += __shfl_xor(b, o);
+      c += __shfl_xor(c, o);
     }
     if (lane == 0 && counters) {
       if (COUNT) {
@@ -203,28 +205,27 @@ __global__ void k_zero_u32(uint32_t *p, uint32_t n) {
 }
 
 template <bool ANY, bool COUNT, int STACK>
-hipError_t launch_t(const chk_scene_s *s, const float4 *rays, uint32_t n, const uint32_t *count_dev,
-                    float4 *hits, uint8_t *occ, const float4 *sL, float4 *pL, bool leaf_ids,
-                    uint32_t *work, unsigned long long *counters, int grid, hipStream_t st) {
-  hipLaunchKernelGGL((k_trace<ANY, COUNT, STACK>), dim3(grid), dim3(kBlock), 0, st, s->d, rays, n,
-                     count_dev, hits, occ, sL, pL, leaf_ids ? 1 : 0, work, counters);
+hipError_t launch_t(const chk_scene_s *s, const float4 *rays, const SegRange &W, float4 *hits,
+                    uint8_t *occ, const float4 *sL, float4 *pL, bool leaf_ids,
+                    unsigned long long *counters, hipStream_t st) {
+  const uint32_t grid = kSegs * (W.segcap / kBlock);
+  if (grid == 0) return hipSuccess;
+  hipLaunchKernelGGL((k_trace<ANY, COUNT, STACK>), dim3(grid), dim3(kBlock), 0, st, s->d, rays, W,
+                     hits, occ, sL, pL, leaf_ids ? 1 : 0, counters);
   return hipGetLastError();
 }
 
 template <bool ANY, bool COUNT>
-hipError_t launch_s(const chk_scene_s *s, const float4 *rays, uint32_t n, const uint32_t *count_dev,
-                    float4 *hits, uint8_t *occ, const float4 *sL, float4 *pL, bool leaf_ids,
-                    uint32_t *work, unsigned long long *counters, int grid, hipStream_t st) {
+hipError_t launch_s(const chk_scene_s *s, const float4 *rays, const SegRange &W, float4 *hits,
+                    uint8_t *occ, const float4 *sL, float4 *pL, bool leaf_ids,
+                    unsigned long long *counters, hipStream_t st) {
   switch (s->stack_size) {
     case 32:
-      return launch_t<ANY, COUNT, 32>(s, rays, n, count_dev, hits, occ, sL, pL, leaf_ids, work,
-                                      counters, grid, st);
+      return launch_t<ANY, COUNT, 32>(s, rays, W, hits, occ, sL, pL, leaf_ids, counters, st);
     case 64:
-      return launch_t<ANY, COUNT, 64>(s, rays, n, count_dev, hits, occ, sL, pL, leaf_ids, work,
-                                      counters, grid, st);
+      return launch_t<ANY, COUNT, 64>(s, rays, W, hits, occ, sL, pL, leaf_ids, counters, st);
     default:
-      return launch_t<ANY, COUNT, 96>(s, rays, n, count_dev, hits, occ, sL, pL, leaf_ids, work,
-                                      counters, grid, st);
+      return launch_t<ANY, COUNT, 96>(s, rays, W, hits, occ, sL, pL, leaf_ids, counters, st);
   }
 }
 
@@ -237,32 +238,24 @@ int pick_stack_size(uint32_t bound) {
   return -1;
 }
 
-int trace_grid(const chk_scene_s *s) {
-  // enough persistent blocks for full occupancy; idle ones exit at once
-  int per_cu = s->stack_size <= 32 ? 10 : (s->stack_size <= 64 ? 5 : 3);
-  return s->ctx->n_cu * per_cu;
+hipError_t launch_closest(const chk_scene_s *s, const float4 *rays, const SegRange &W,
+                          float4 *hits, bool leaf_ids, bool count, unsigned long long *counters,
+                          hipStream_t st) {
+  if (count)
+    return launch_s<false, true>(s, rays, W, hits, nullptr, nullptr, nullptr, leaf_ids, counters,
+                                 st);
+  return launch_s<false, false>(s, rays, W, hits, nullptr, nullptr, nullptr, leaf_ids, counters,
+                                st);
 }
 
-hipError_t launch_closest(const chk_scene_s *s, const float4 *rays, uint32_t n,
-                          const uint32_t *count_dev, float4 *hits, bool leaf_ids,
-                          uint32_t *work_counter, bool count, unsigned long long *counters,
-                          int grid, hipStream_t st) {
+hipError_t launch_any(const chk_scene_s *s, const float4 *rays, const SegRange &W,
+                      uint8_t *occluded, const float4 *shadow_L, float4 *path_L, bool leaf_ids,
+                      bool count, unsigned long long *counters, hipStream_t st) {
   if (count)
-    return launch_s<false, true>(s, rays, n, count_dev, hits, nullptr, nullptr, nullptr, leaf_ids,
-                                 work_counter, counters, grid, st);
-  return launch_s<false, false>(s, rays, n, count_dev, hits, nullptr, nullptr, nullptr, leaf_ids,
-                                work_counter, counters, grid, st);
-}
-
-hipError_t launch_any(const chk_scene_s *s, const float4 *rays, uint32_t n,
-                      const uint32_t *count_dev, uint8_t *occluded, const float4 *shadow_L,
-                      float4 *path_L, bool leaf_ids, uint32_t *work_counter, bool count,
-                      unsigned long long *counters, int grid, hipStream_t st) {
-  if (count)
-    return launch_s<true, true>(s, rays, n, count_dev, nullptr, occluded, shadow_L, path_L,
-                                leaf_ids, work_counter, counters, grid, st);
-  return launch_s<true, false>(s, rays, n, count_dev, nullptr, occluded, shadow_L, path_L,
-                               leaf_ids, work_counter, counters, grid, st);
+    return launch_s<true, true>(s, rays, W, nullptr, occluded, shadow_L, path_L, leaf_ids,
+                                counters, st);
+  return launch_s<true, false>(s, rays, W, nullptr, occluded, shadow_L, path_L, leaf_ids,
+                               counters, st);
 }
 
 hipError_t launch_zero_u32(uint32_t *p, uint32_t n, hipStream_t st) {

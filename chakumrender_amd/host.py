@@ -145,17 +145,6 @@ class LightSource:
         self.eval_dev(bufs, k, rgb)
         return rgb.download((k, 3), np.float32)
 
-    def set_fp64_threshold(self, kappa):
-        """Cancellation factor above which a point is recomputed in FP64."""
-        abi.check(self.L.chk_ltc_set_fp64_threshold(self.ctx.h, float(kappa)),
-                  "chk_ltc_set_fp64_threshold")
-
-    def fp64_count(self):
-        """Points of the last evaluation recomputed by the FP64 pass."""
-        c = C.c_uint32(0)
-        abi.check(self.L.chk_ltc_fp64_count(self.ctx.h, C.byref(c), None), "chk_ltc_fp64_count")
-        return int(c.value)
-
 
 class Proposal:
     """Learned light-selection proposal (chk_proposal_t)."""

@@ -180,16 +180,6 @@ chk_status chk_ltc_eval(chk_ctx_t ctx, const float *x, const float *n, const flo
                         const float *alpha, const float *ks, uint32_t k,
                         const float *light_tris, const float *radiance, uint32_t n_light_tris,
                         float *rgb, void *stream);
-/* Points of the last chk_ltc_eval on this context that the FP32 pass
- * flagged as ill-conditioned (edge terms cancelling) and recomputed in FP64.
- * Synchronises the stream. */
-chk_status chk_ltc_fp64_count(chk_ctx_t ctx, uint32_t *count, void *stream);
-/* Cancellation factor above which chk_ltc_eval recomputes a point in FP64
- * (default 8: relative error of the FP32 result about kappa * 3e-7).
- * 0 sends every lit point to the FP64 pass; INFINITY keeps all in FP32.
- * No reference counterpart (the reference computes everything in FP64). */
-chk_status chk_ltc_set_fp64_threshold(chk_ctx_t ctx, float kappa);
-
 /* Render: the pixel loop of main.cpp:62-94 around shade() (shader.cpp:362),
  * with the configs' estimator: path tracing to params->max_depth vertices,
  * next-event estimation from the mesh lights and BSDF sampling combined by

@@ -131,7 +131,7 @@ def test_ltc_matches_reference(ctx, golden, suffix):
     err = np.abs(rgb - g).max(1) / np.maximum(scale, 1e-30)
     lit = scale > 1e-6 * scale.max()
     print(f"points {g.shape[0]}, lit {lit.sum()}, max rel {err[lit].max():.2e}, "
-          f"p99 {np.quantile(err[lit], 0.99):.2e}, FP64 pass {ls.fp64_count()}")
+          f"p99 {np.quantile(err[lit], 0.99):.2e}")
     assert err[lit].max() < REL
     if (~lit).any():
         assert np.abs(rgb[~lit]).max() <= 1e-6 * scale.max()
@@ -144,39 +144,22 @@ def _ltc_err(rgb, g):
     return err[lit]
 
 
-def test_ltc_fp64_threshold_sweep(ctx, golden):
-    """The FP64 pass alone (kappa 0) reproduces the reference to the FP32
-    rounding of the output; the FP32 pass alone (kappa inf) is reported; the
-    default must meet 1e-5.
-    Also a cfg3-like set (16 random quads, random points) against the oracle."""
-    L = golden("ltc.npz")
-    sets = []
-    for suf in ("", "2"):
-        sets.append((f"golden{suf}", L["tris" + suf], L["rad" + suf],
-                     [L[k + suf].astype(np.float32) for k in ("x", "n", "v", "alpha", "ks")], L["rgb" + suf]))
+def test_ltc_random_quads_match_oracle(ctx):
+    """A cfg3-like set (16 random quads, 20,000 random points, ks in
+    {0, 0.5, 1}) against the oracle's FP64 restatement: the kernel computes
+    in FP64, so the only difference is the FP32 rounding of its output."""
     P = scenes.ltc_points(20000, seed=0x5EED0003, n_quads=16)
     tris, rad = scenes.quads_to_tris(P["quads"], P["radiance"])
+    rad = rad.astype(np.float32).astype(np.float64)
     ks = np.array([0.0, 0.5, 1.0], np.float32)[np.arange(20000) % 3]
     ins = [P["x"].astype(np.float32), P["n"].astype(np.float32), P["v"].astype(np.float32),
            P["alpha"].astype(np.float32), ks]
-    rad = rad.astype(np.float32).astype(np.float64)
     o = oracle.ltc_shade(*ins[:4], ks.astype(np.float64), tris.astype(np.float32), rad)
-    sets.append(("random16", tris, rad, ins, o["rgb"]))
-    try:
-        for name, t, r, xs, g in sets:
-            ls = host.LightSource(ctx, t, r)
-            for kappa in (0.0, 8.0, 16.0, np.inf):
-                ls.set_fp64_threshold(kappa)
-                rgb = ls.getLightFromMeshSource_Analytic(*xs).astype(np.float64)
-                e = _ltc_err(rgb, g)
-                print(f"{name} kappa {kappa}: max rel {e.max():.2e}, p99.9 {np.quantile(e, 0.999):.2e}, "
-                      f"FP64 pass {ls.fp64_count() / len(xs[3]):.3f}")
-                if kappa == 0.0:  # FP64 throughout; the FP32 output rounds to 6e-8
-                    assert e.max() < 1.2e-7
-                if kappa == 8.0:
-                    assert e.max() < REL
-    finally:
-        host.LightSource(ctx, sets[0][1], sets[0][2]).set_fp64_threshold(8.0)
+    ls = host.LightSource(ctx, tris, rad)
+    rgb = ls.getLightFromMeshSource_Analytic(*ins).astype(np.float64)
+    e = _ltc_err(rgb, o["rgb"])
+    print(f"random quads: max rel {e.max():.2e}, p99.9 {np.quantile(e, 0.999):.2e}")
+    assert e.max() < 2e-7
 
 
 def test_ltc_closed_box_integrates_to_one(ctx):
