@@ -40,11 +40,148 @@ def parse():
     ap.add_argument("--depth", type=int, default=6)
     ap.add_argument("--freq", type=int, default=71, help="geodesic frequency (71 -> 100,820 triangles)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-row-step", type=int, default=8)
+    ap.add_argument("--cpu-row-step", type=int, default=1)
     ap.add_argument("--cpu-spp", type=int, default=64)
     ap.add_argument("--traffic", type=str, default=None,
                     help="JSON file with measured HBM bytes per closest-hit launch")
+    # time-to-RMSE leg: learned proposal vs plain MIS (config 5 scene)
+    ap.add_argument("--no-rmse", action="store_true")
+    ap.add_argument("--rmse-width", type=int, default=480)
+    ap.add_argument("--rmse-height", type=int, default=270)
+    ap.add_argument("--rmse-tris", type=int, default=2_000_000)
+    ap.add_argument("--rmse-lights", type=int, default=32)
+    ap.add_argument("--rmse-depth", type=int, default=4)
+    ap.add_argument("--rmse-ref-spp", type=int, default=65536)
+    ap.add_argument("--rmse-pass-spp", type=int, default=32)
+    ap.add_argument("--rmse-max-spp", type=int, default=8192)
+    ap.add_argument("--rmse-target", type=float, default=1e-2)
+    ap.add_argument("--train-spp", type=int, default=32)
+    ap.add_argument("--train-batch", type=int, default=16384)
+    ap.add_argument("--train-steps", type=int, default=4, help="Adam steps per bounce")
+    ap.add_argument("--train-lr", type=float, default=1e-3)
     return ap.parse_args()
+
+
+def progressive(render_pass, ref, pass_spp, max_spp, target, sync, W, H):
+    """Render passes of pass_spp until the RMSE against ref reaches target or
+    max_spp is reached; returns [(gpu seconds so far, spp, rmse)].  Only the
+    render calls are timed (the RMSE readback is not)."""
+    from chakumrender_amd import host
+
+    img = host.Image(W, H)
+    t = 0.0
+    curve = []
+    for k in range(max_spp // pass_spp):
+        sync()
+        t0 = time.perf_counter()
+        render_pass(k * pass_spp, img)
+        sync()
+        t += time.perf_counter() - t0
+        n = (k + 1) * pass_spp
+        a, _ = img.get()
+        err = float(((a / n - ref) ** 2).mean() ** 0.5)
+        curve.append((t, n, err))
+        if err <= target:
+            break
+    return curve
+
+
+def time_to(curve, target):
+    """Seconds at which the curve crosses target (log-linear interpolation
+    between passes), or None."""
+    import math
+
+    prev = None
+    for t, _, e in curve:
+        if e <= target:
+            if prev is None:
+                return t
+            t0, e0 = prev
+            f = (math.log(e0) - math.log(target)) / (math.log(e0) - math.log(e))
+            return t0 + f * (t - t0)
+        prev = (t, e)
+    return None
+
+
+def rmse_at(curve, t_query):
+    """RMSE of the curve at time t_query (log-linear interpolation)."""
+    import math
+
+    if not curve or t_query < curve[0][0]:
+        return None
+    prev = curve[0]
+    for p in curve[1:]:
+        if p[0] >= t_query:
+            f = (t_query - prev[0]) / max(p[0] - prev[0], 1e-12)
+            return math.exp(math.log(prev[2]) + f * (math.log(p[2]) - math.log(prev[2])))
+        prev = p
+    return None
+
+
+def rmse_leg(args, ctx):
+    """Time-to-RMSE of plain MIS and of the learned proposal against a
+    plain-MIS reference of rmse_ref_spp samples (FP64 accumulation)."""
+    from chakumrender_amd import abi, host, scenes
+
+    W, H = args.rmse_width, args.rmse_height
+    sc = scenes.cornell_large(total_tris=args.rmse_tris, n_lights=args.rmse_lights, width=W, height=H)
+    ds = abi.DeviceScene(ctx, sc)
+    nl = ds.info()["n_lights"]
+    base = dict(max_depth=args.rmse_depth)
+    t0 = time.perf_counter()
+    ref_img = host.Image(W, H)
+    chunk = max(1, min(args.rmse_ref_spp, (1 << 22) // (W * H) * 4))
+    done = 0
+    while done < args.rmse_ref_spp:
+        n = min(chunk, args.rmse_ref_spp - done)
+        host.render(ds, dict(base, spp=n, sample_offset=done, seed=0x5EED0F00), image=ref_img,
+                    stats=False, download=False)
+        done += n
+    ref = ref_img.get()[0] / args.rmse_ref_spp
+    ref_s = time.perf_counter() - t0
+    print(f"# rmse leg: {sc.n_tris} tris, {nl} lights, {W}x{H}, reference {args.rmse_ref_spp} spp "
+          f"in {ref_s:.1f} s, mean {ref.mean():.4f}", file=sys.stderr, flush=True)
+
+    def plain_pass(off, img):
+        host.render(ds, dict(base, spp=args.rmse_pass_spp, sample_offset=off, seed=0x5EED0005),
+                    image=img, stats=False, download=False)
+
+    prop = host.Proposal(ctx, nl, seed=0x5EED0005)
+    train = dict(train_spp=args.train_spp, batch=args.train_batch,
+                 steps_per_bounce=args.train_steps, lr=args.train_lr)
+
+    def learned_pass(off, img):
+        host.render_learned(ds, dict(base, spp=args.rmse_pass_spp, sample_offset=off,
+                                     seed=0x5EED0005), prop, train=train, image=img,
+                            stats=False, download=False)
+
+    cp = progressive(plain_pass, ref, args.rmse_pass_spp, args.rmse_max_spp, args.rmse_target,
+                     abi.sync, W, H)
+    cl = progressive(learned_pass, ref, args.rmse_pass_spp, args.rmse_max_spp, args.rmse_target,
+                     abi.sync, W, H)
+    tp, tl = time_to(cp, args.rmse_target), time_to(cl, args.rmse_target)
+    # equal time: the RMSE each reaches at the time the faster one stops
+    t_eq = min(cp[-1][0], cl[-1][0])
+    out = {
+        "workload": f"cornell-style {sc.n_tris} tris, {nl} quad lights, {W}x{H}, depth "
+                    f"{args.rmse_depth}; reference plain MIS {args.rmse_ref_spp} spp FP64",
+        "target": args.rmse_target,
+        "rmse": "per-channel linear radiance over all pixels",
+        "plain_s": None if tp is None else round(tp, 4),
+        "learned_s": None if tl is None else round(tl, 4),
+        "ratio_learned_over_plain": None if (tp is None or tl is None) else round(tl / tp, 4),
+        "plain_spp_at_target": cp[-1][1] if tp is not None else None,
+        "learned_spp_at_target": cl[-1][1] if tl is not None else None,
+        "equal_time": {"seconds": round(t_eq, 4), "plain_rmse": rmse_at(cp, t_eq),
+                       "learned_rmse": rmse_at(cl, t_eq)},
+        "plain_mpaths_per_s": round(W * H * cp[-1][1] / cp[-1][0] / 1e6, 2),
+        "learned_mpaths_per_s": round(W * H * cl[-1][1] / cl[-1][0] / 1e6, 2),
+        "train": dict(train, pass_spp=args.rmse_pass_spp),
+        "reference_seconds": round(ref_s, 2),
+    }
+    prop.close()
+    ds.close()
+    return out
 
 
 def main():
@@ -145,6 +282,9 @@ def main():
             "work_model": f"{NODE_BYTES} B/node visit + {TRI_BYTES} B/triangle test + {RAY_BYTES} B/ray",
         },
     }
+
+    if not args.no_rmse:
+        line["time_to_rmse"] = rmse_leg(args, ctx)
 
     if not args.no_cpu_baseline:
         import oracle

@@ -51,12 +51,34 @@ class chk_scene_info(C.Structure):
                 ("sah_cost", C.c_double), ("build_ms", C.c_double)]
 
 
+class chk_request(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("camera", chk_camera), ("params", chk_render_params),
+                ("accum", C.c_void_p), ("sumsq", C.c_void_p)]
+
+
+class chk_batch_tuning(C.Structure):
+    _fields_ = [("chunk_paths", C.c_uint32), ("trace_block", C.c_uint32),
+                ("shade_block", C.c_uint32), ("use_graph", C.c_uint32)]
+
+
+class chk_batch_stats(C.Structure):
+    _fields_ = [("paths", C.c_uint64), ("rays_extend", C.c_uint64), ("rays_shadow", C.c_uint64),
+                ("chunks", C.c_uint32), ("segments", C.c_uint32), ("batch_ms", C.c_double),
+                ("capture_ms", C.c_double), ("graph_cached", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class chk_train_params(C.Structure):
+    _fields_ = [("train_spp", C.c_uint32), ("batch", C.c_uint32), ("steps_per_bounce", C.c_uint32),
+                ("lr", C.c_float), ("flags", C.c_uint32)]
+
+
 class chk_render_stats(C.Structure):
     _fields_ = [("paths", C.c_uint64), ("rays_extend", C.c_uint64), ("rays_shadow", C.c_uint64),
                 ("node_visits", C.c_uint64), ("tri_tests", C.c_uint64),
                 ("shadow_node_visits", C.c_uint64), ("shadow_tri_tests", C.c_uint64),
                 ("chunks", C.c_uint32), ("extend_launches", C.c_uint32), ("extend_ms", C.c_double),
-                ("shadow_ms", C.c_double), ("shade_ms", C.c_double), ("other_ms", C.c_double)]
+                ("shadow_ms", C.c_double), ("shade_ms", C.c_double), ("other_ms", C.c_double),
+                ("proposal_ms", C.c_double), ("train_steps", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 assert C.sizeof(chk_material) == 56
@@ -68,6 +90,7 @@ RENDER_SUMSQ = 0x2
 RENDER_PROFILE = 0x100
 JITTER_BOX, JITTER_REFERENCE = 0, 1
 MIS_BALANCE, MIS_LEARNED = 0, 1
+TRAIN_FREEZE_MIX, TRAIN_FREEZE_LIGHTS = 1, 2
 
 # Every exported symbol of include/chakum_c.h: name -> (restype, argtypes)
 _vp = C.c_void_p
@@ -98,6 +121,13 @@ SIGNATURES = {
                              C.POINTER(chk_render_stats), _vp]),
     "chk_render_count": (C.c_int, [_vp, C.POINTER(chk_camera), C.POINTER(chk_render_params), _vp,
                                    C.POINTER(chk_render_stats), _vp]),
+    "chk_render_batch": (C.c_int, [_vp, C.POINTER(chk_request), _u32, C.POINTER(chk_batch_tuning),
+                                   C.POINTER(C.c_double), C.POINTER(chk_batch_stats), _vp]),
+    "chk_batch_autotune": (C.c_int, [_vp, C.POINTER(chk_request), _u32, C.POINTER(chk_batch_tuning),
+                                     C.POINTER(C.c_double), _vp]),
+    "chk_render_learned": (C.c_int, [_vp, C.POINTER(chk_camera), C.POINTER(chk_render_params), _vp,
+                                     C.POINTER(chk_train_params), _vp, _vp,
+                                     C.POINTER(chk_render_stats), _vp]),
     "chk_proposal_create": (C.c_int, [_vp, _u32, C.c_uint64, C.POINTER(_vp)]),
     "chk_proposal_destroy": (C.c_int, [_vp]),
     "chk_proposal_set_params": (C.c_int, [_vp, _vp, C.c_size_t]),

@@ -17,7 +17,7 @@ void set_error(const std::string &msg) { g_err = msg; }
 
 chk_status render_impl(chk_scene_s *sc, const chk_camera *cam, const chk_render_params *P,
                        double *accum, double *sumsq, chk_render_stats *stats, bool count,
-                       hipStream_t st);
+                       chk_proposal_s *prop, const chk_train_params *train, hipStream_t st);
 
 namespace {
 
@@ -229,6 +229,7 @@ chk_status chk_destroy(chk_ctx_t c) {
   if (!c) return CHK_OK;
   (void)hipSetDevice(c->device);
   if (c->ltc_dev) (void)hipFree(c->ltc_dev);
+  if (c->batch_state) batch_state_free(c->batch_state);
   delete c;
   return CHK_OK;
 }
@@ -381,6 +382,19 @@ chk_status chk_scene_create(chk_ctx_t ctx, const float *tris, uint32_t n_tris,
   }
   sc->d.n_lights = used_lights;
   sc->d.n_leaf_tris = m;
+  {
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (const TriRec &t : bvh.tris)
+      for (int k = 0; k < 3; ++k) {
+        lo[k] = std::min(lo[k], std::min(t.a[k], std::min(t.b[k], t.c[k])));
+        hi[k] = std::max(hi[k], std::max(t.a[k], std::max(t.b[k], t.c[k])));
+      }
+    for (int k = 0; k < 3; ++k) {
+      const bool ok = m > 0 && hi[k] > lo[k];
+      sc->d.box_lo[k] = ok ? lo[k] : 0.0f;
+      sc->d.box_inv[k] = ok ? 1.0f / (hi[k] - lo[k]) : 1.0f;
+    }
+  }
   sc->stack_size = stack;
   chk_scene_info &I = sc->info;
   I.n_tris = n_tris;
@@ -403,6 +417,7 @@ chk_status chk_scene_destroy(chk_scene_t sc) {
   if (!sc) return CHK_OK;
   (void)hipSetDevice(sc->ctx->device);
   if (sc->work) (void)hipFree(sc->work);
+  if (sc->lwork) (void)hipFree(sc->lwork);
   if (sc->blob) (void)hipFree(sc->blob);
   delete sc;
   return CHK_OK;
@@ -432,8 +447,9 @@ chk_status chk_trace_closest(chk_scene_t sc, const chk_ray *rays, uint32_t n, ch
   CHK_HIP(hipSetDevice(sc->ctx->device));
   // no scratch: the stack depth bound of the tree is exact, so calls on
   // distinct streams share nothing but the read-only scene
-  CHK_HIP(launch_closest(sc, (const float4 *)rays, linear_range(n), (float4 *)hits, false, false,
-                         nullptr, (hipStream_t)stream));
+  CHK_HIP(launch_closest(sc, TraceShape{sc->stack_size, 128}, (const float4 *)rays,
+                         linear_range(n), (float4 *)hits, false, false, nullptr, SlotScenes{},
+                         (hipStream_t)stream));
   return CHK_OK;
   CHK_TRY_END
 }
@@ -451,8 +467,9 @@ chk_status chk_trace_any(chk_scene_t sc, const chk_ray *rays, uint32_t n, uint8_
     return CHK_ECAPACITY;
   }
   CHK_HIP(hipSetDevice(sc->ctx->device));
-  CHK_HIP(launch_any(sc, (const float4 *)rays, linear_range(n), occluded, nullptr, nullptr, false,
-                     false, nullptr, (hipStream_t)stream));
+  CHK_HIP(launch_any(sc, TraceShape{sc->stack_size, 128}, (const float4 *)rays,
+                     linear_range(n), occluded, nullptr, nullptr, false, false, nullptr,
+                     SlotScenes{}, (hipStream_t)stream));
   return CHK_OK;
   CHK_TRY_END
 }
@@ -513,7 +530,8 @@ chk_status chk_render(chk_scene_t sc, const chk_camera *camera, const chk_render
     return CHK_EINVAL;
   }
   CHK_HIP(hipSetDevice(sc->ctx->device));
-  return render_impl(sc, camera, params, accum, sumsq, stats, false, (hipStream_t)stream);
+  return render_impl(sc, camera, params, accum, sumsq, stats, false, nullptr, nullptr,
+                     (hipStream_t)stream);
   CHK_TRY_END
 }
 
@@ -526,7 +544,48 @@ chk_status chk_render_count(chk_scene_t sc, const chk_camera *camera,
     return CHK_EINVAL;
   }
   CHK_HIP(hipSetDevice(sc->ctx->device));
-  return render_impl(sc, camera, params, accum, nullptr, stats, true, (hipStream_t)stream);
+  return render_impl(sc, camera, params, accum, nullptr, stats, true, nullptr, nullptr,
+                     (hipStream_t)stream);
+  CHK_TRY_END
+}
+
+chk_status chk_render_learned(chk_scene_t sc, const chk_camera *camera,
+                              const chk_render_params *params, chk_proposal_t proposal,
+                              const chk_train_params *train, double *accum, double *sumsq,
+                              chk_render_stats *stats, void *stream) {
+  CHK_TRY_BEGIN
+  if (!sc || !proposal) {
+    set_error("chk_render_learned: null scene or proposal");
+    return CHK_EINVAL;
+  }
+  CHK_HIP(hipSetDevice(sc->ctx->device));
+  return render_impl(sc, camera, params, accum, sumsq, stats, false, proposal, train,
+                     (hipStream_t)stream);
+  CHK_TRY_END
+}
+
+chk_status chk_render_batch(chk_ctx_t ctx, const chk_request *reqs, uint32_t n,
+                            const chk_batch_tuning *tuning, double *latency_ms,
+                            chk_batch_stats *stats, void *stream) {
+  CHK_TRY_BEGIN
+  if (!ctx || (n && !reqs)) {
+    set_error("chk_render_batch: null context or requests");
+    return CHK_EINVAL;
+  }
+  CHK_HIP(hipSetDevice(ctx->device));
+  return batch_impl(ctx, reqs, n, tuning, latency_ms, stats, (hipStream_t)stream);
+  CHK_TRY_END
+}
+
+chk_status chk_batch_autotune(chk_ctx_t ctx, const chk_request *reqs, uint32_t n,
+                              chk_batch_tuning *best, double *best_ms, void *stream) {
+  CHK_TRY_BEGIN
+  if (!ctx || !best || (n && !reqs)) {
+    set_error("chk_batch_autotune: null argument");
+    return CHK_EINVAL;
+  }
+  CHK_HIP(hipSetDevice(ctx->device));
+  return batch_autotune(ctx, reqs, n, best, best_ms, (hipStream_t)stream);
   CHK_TRY_END
 }
 

@@ -46,6 +46,9 @@ struct DeviceScene {
   const LightRec *lights = nullptr;
   uint32_t n_lights = 0;
   uint32_t n_leaf_tris = 0;
+  // box of the BVH triangles (degenerate ones excluded): the learned
+  // proposal's position input is (x - box_lo) * box_inv in [0, 1]
+  float box_lo[3] = {0, 0, 0}, box_inv[3] = {1, 1, 1};
 };
 
 }  // namespace chk
@@ -54,6 +57,7 @@ struct chk_ctx_s {
   int device = 0;
   int n_cu = 256;
   double *ltc_dev = nullptr;  // packed LTC table (6 doubles per record, chk_ltc.hip)
+  void *batch_state = nullptr;  // queues, tables and cached graphs of chk_render_batch
 };
 
 struct chk_scene_s {
@@ -66,6 +70,8 @@ struct chk_scene_s {
   // render scratch (grown on demand)
   void *work = nullptr;
   size_t work_bytes = 0;
+  void *lwork = nullptr;  // learned-proposal buffers of the render
+  size_t lwork_bytes = 0;
 };
 
 namespace chk {
@@ -74,7 +80,7 @@ namespace chk {
 // (item (r, k) at index r * segcap + k).  Segment r holds count[r *
 // kSegStride] items when count is set, else the items of [r * segcap,
 // (r + 1) * segcap) below n (a plain array of n items).  segcap is a
-// multiple of 256 (the largest block size of the consumers).
+// multiple of 512 (the largest block size of the consumers).
 constexpr uint32_t kSegs = 8;
 constexpr uint32_t kSegStride = 64;  // counters 256 bytes apart
 struct SegRange {
@@ -88,26 +94,61 @@ struct SegRange {
 };
 inline uint32_t seg_capacity(uint32_t n) {
   const uint32_t per = (uint32_t)(((uint64_t)n + kSegs - 1) / kSegs);
-  return (per + 255u) & ~255u;
+  return (per + 511u) & ~511u;  // a multiple of every block size used (<= 512)
 }
 constexpr uint32_t kMaxTraceBatch = 1u << 31;  // rays per trace call
 inline SegRange linear_range(uint32_t n) { return SegRange{nullptr, n, seg_capacity(n)}; }
 
 // Kernel launchers (chk_trace.hip).
+// Launch shape: LDS stack entries per lane (32, 64 or 96, at least the
+// scene's stack bound) and threads per block (64, 128 or 256).
+struct TraceShape {
+  int stack;
+  int block;
+};
+// Per-ray scenes of a batched launch: ray i belongs to scene[slot_seg[i]]
+// (scene == nullptr: every ray uses the launch's scene).
+struct SlotScenes {
+  const DeviceScene *scene;
+  const uint32_t *slot_seg;
+};
 // Closest hit over rays (chk_ray layout) laid out by W.  leaf_ids:
 // report/skip leaf indices instead of original ones.  counters (dev, 4 x
 // u64, may be null): [node visits, tri tests] accumulated when `count`,
-// [2] = lanes whose stack overflowed, [3] += rays traced.
-hipError_t launch_closest(const chk_scene_s *s, const float4 *rays, const SegRange &W,
-                          float4 *hits, bool leaf_ids, bool count, unsigned long long *counters,
-                          hipStream_t st);
+// [2] = lanes whose stack overflowed, [3] += rays traced.  s may be null
+// when M.scene is set.
+hipError_t launch_closest(const chk_scene_s *s, const TraceShape &T, const float4 *rays,
+                          const SegRange &W, float4 *hits, bool leaf_ids, bool count,
+                          unsigned long long *counters, const SlotScenes &M, hipStream_t st);
 // Any hit.  occluded (may be null) receives 0/1; with path_L, unoccluded
 // rays add shadow_L[i].xyz to path_L[bits(shadow_L[i].w)].
-hipError_t launch_any(const chk_scene_s *s, const float4 *rays, const SegRange &W,
-                      uint8_t *occluded, const float4 *shadow_L, float4 *path_L, bool leaf_ids,
-                      bool count, unsigned long long *counters, hipStream_t st);
+hipError_t launch_any(const chk_scene_s *s, const TraceShape &T, const float4 *rays,
+                      const SegRange &W, uint8_t *occluded, const float4 *shadow_L,
+                      float4 *path_L, bool leaf_ids, bool count, unsigned long long *counters,
+                      const SlotScenes &M, hipStream_t st);
 hipError_t launch_zero_u32(uint32_t *p, uint32_t n, hipStream_t st);
 
+// Learned proposal (chk_mlp.hip).  Inference over the items of R: inputs
+// (7 per item), probs (32 per item), mix (1 per item), logits (33, may be null).
+struct TrainRec;
+hipError_t launch_mlp_eval(const chk_proposal_s *p, const float *inputs, const SegRange &R,
+                           float *probs, float *mix, float *logits, hipStream_t st);
+// One Adam step on `batch` records of a finished bounce (see chk_mlp.hip).
+chk_status proposal_train_step(chk_proposal_s *p, const SegRange &R, const TrainRec *rec,
+                               const float *inputs, const float *probs, const float *mix,
+                               const uint8_t *s_occ, uint32_t batch, uint32_t stride,
+                               uint32_t offset, float lr, uint32_t flags, hipStream_t st);
+uint32_t proposal_n_lights(const chk_proposal_s *p);
+const chk_ctx_s *proposal_ctx(const chk_proposal_s *p);
+
 int pick_stack_size(uint32_t bound);
+
+// Batched requests (chk_render.hip).
+chk_status batch_impl(chk_ctx_s *ctx, const chk_request *R, uint32_t n,
+                      const chk_batch_tuning *tuning, double *latency_ms, chk_batch_stats *stats,
+                      hipStream_t st);
+chk_status batch_autotune(chk_ctx_s *ctx, const chk_request *R, uint32_t n,
+                          chk_batch_tuning *best, double *best_ms, hipStream_t st);
+void batch_state_free(void *state);
 
 }  // namespace chk

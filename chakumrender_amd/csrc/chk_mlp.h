@@ -53,10 +53,21 @@ struct MlpDev {
   uint32_t n_lights;
 };
 
-// Per-point proposal inputs: 7 scalars in [0, 1] (position in the scene
-// box, normal and view direction as (theta/pi, phi/2pi)).
-struct PropBox {
-  float lo[3], inv_ext[3];
+// Per-vertex training record of the learned render (40 bytes), written by
+// the shading kernel at the vertex, completed by the shadow pass (occlusion
+// of the light sample) and by the next bounce (emission found by the BSDF
+// sample).  Scalars are luminances (mean of RGB) of the unweighted integrand
+// g = f * L * cos in solid-angle measure.
+struct TrainRec {
+  uint32_t light;    // light j drawn from q' (0xFFFFFFFF: no NEE sample)
+  float gl;          // g of the light sample, before visibility
+  float pl, pbl;     // light-sampling and BSDF pdfs of the light sample
+  float fb;          // f * cos * albedo of the BSDF sample
+  float le;          // emitted radiance its ray found (0: none)
+  float plb, pbb;    // light-sampling and BSDF pdfs of the BSDF sample
+  uint32_t sh_slot;  // shadow-queue slot of the light sample (0xFFFFFFFF: none)
+  uint32_t pad;
 };
+static_assert(sizeof(TrainRec) == 40, "TrainRec is 40 bytes");
 
 }  // namespace chk

@@ -132,21 +132,25 @@ __device__ inline void proposal_head(const f4 out[3], int lane, uint32_t L, floa
 }
 
 // ------------------------------------------------------------------ kernels
+// Inference over the items of W (segment r = blockIdx % kSegs; the waves of
+// a segment's blocks stride over its groups of 16 points).
 __global__ __launch_bounds__(256) void k_mlp_eval(MlpDev W, const float *__restrict__ inputs,
-                                                  uint32_t k, const uint32_t *__restrict__ kdev,
-                                                  float *__restrict__ probs,
+                                                  SegRange R, float *__restrict__ probs,
                                                   float *__restrict__ mix_out,
                                                   float *__restrict__ logits) {
-  if (kdev) k = min(k, *kdev);
   const int lane = threadIdx.x & 63;
-  const uint32_t wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  const uint32_t nw = gridDim.x * (blockDim.x >> 6);
+  const uint32_t r = blockIdx.x % kSegs;
+  const uint32_t k = R.count_of(r);
+  const size_t seg0 = (size_t)r * R.segcap;
+  const uint32_t wave = (blockIdx.x / kSegs) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const uint32_t nw = (gridDim.x / kSegs) * (blockDim.x >> 6);
   for (uint32_t g0 = wave * 16; g0 < k; g0 += nw * 16) {
-    const uint32_t p = g0 + (lane & 15);
-    const bool valid = p < k;
+    const uint32_t pl = g0 + (lane & 15);
+    const bool valid = pl < k;
+    const size_t p = seg0 + pl;
     float s[7];
 #pragma unroll
-    for (int i = 0; i < 7; ++i) s[i] = valid ? inputs[7 * (size_t)p + i] : 0.0f;
+    for (int i = 0; i < 7; ++i) s[i] = valid ? inputs[7 * p + i] : 0.0f;
     s4 x[4], h1[4], h2[4], h3[4];
     encode_b(s, lane, x);
     hidden(W, 0, x, lane, h1);
@@ -159,7 +163,7 @@ __global__ __launch_bounds__(256) void k_mlp_eval(MlpDev W, const float *__restr
     if (valid) {
       const int g = lane >> 4;
 #pragma unroll
-      for (int kk = 0; kk < 8; ++kk) probs[32 * (size_t)p + (kk >> 2) * 16 + 4 * g + (kk & 3)] = q[kk];
+      for (int kk = 0; kk < 8; ++kk) probs[32 * p + (kk >> 2) * 16 + 4 * g + (kk & 3)] = q[kk];
       if (g == 0) mix_out[p] = mix;
       if (logits) {
 #pragma unroll
@@ -167,7 +171,7 @@ __global__ __launch_bounds__(256) void k_mlp_eval(MlpDev W, const float *__restr
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
             const int o = t * 16 + 4 * g + v;
-            if (o < kMlpOut) logits[33 * (size_t)p + o] = out[t][v];
+            if (o < kMlpOut) logits[33 * p + o] = out[t][v];
           }
       }
     }
@@ -393,6 +397,141 @@ __global__ void k_mlp_pack(const float *__restrict__ P, uint16_t *__restrict__ F
   }
 }
 
+// ------------------------------------------------------------ online training
+// One training row per selected record of a bounce.  Both heads minimise
+// the second moment of the vertex's direct-light estimate (its variance up
+// to a constant):
+//  * light selection: the NEE part's second moment sum_j E[v_j^2] / q'_j
+//    (v_j: contribution of light j per unit selection probability; optimum
+//    q'_j ~ sqrt(E[v_j^2])), score-function estimate from j ~ q' with
+//    a = g_l V / p_l:  dL/dz_k = -a^2 (1 - eps) (q_j / q'_j) (delta_jk - q_k);
+//  * mixing weight c: F = c g_l/p_c(w_l) + (1-c) g_b/p_c(w_b),
+//    p_c = c p_l + (1-c) p_b.  The sample directions do not depend on c, so
+//    2 F dF/dc on the record's own pair is an unbiased estimate of
+//    dE[F^2]/dc.
+// Each head's rows are divided by the head's batch-mean magnitude
+// (k_train_norm).
+struct TrainArgs {
+  SegRange R;               // records of the bounce (counts per segment)
+  const TrainRec *rec;
+  const float *inputs, *probs, *mix;
+  const uint8_t *s_occ;     // occlusion per shadow slot (same segment)
+  uint32_t n_lights, batch, stride, offset, flags;
+};
+
+__device__ inline bool pick_record(const SegRange &R, uint32_t v, size_t &idx) {
+  for (uint32_t r = 0; r < kSegs; ++r) {
+    const uint32_t c = R.count_of(r);
+    if (v < c) {
+      idx = (size_t)r * R.segcap + v;
+      return true;
+    }
+    v -= c;
+  }
+  return false;
+}
+
+__global__ __launch_bounds__(256) void k_train_rows(TrainArgs T, float *__restrict__ bin,
+                                                    float *__restrict__ dl,
+                                                    float *__restrict__ wa,
+                                                    float *__restrict__ wh) {
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= T.batch) return;
+  float *row = dl + 33 * (size_t)b;
+  for (int k = 0; k < 33; ++k) row[k] = 0.0f;
+  for (int i = 0; i < 7; ++i) bin[7 * (size_t)b + i] = 0.0f;
+  wa[b] = 0.0f;
+  wh[b] = 0.0f;
+  size_t idx;
+  if (!pick_record(T.R, b * T.stride + T.offset, idx)) return;
+  const TrainRec R = T.rec[idx];
+  if (R.light >= T.n_lights) return;
+  for (int i = 0; i < 7; ++i) bin[7 * (size_t)b + i] = T.inputs[7 * idx + i];
+  const float *qp = T.probs + 32 * idx;
+  const double c = T.mix[idx];
+  const double eq = kPropEpsQ, ec = kPropEpsC, invL = 1.0 / T.n_lights;
+  // light part
+  const double vis = (R.sh_slot != 0xFFFFFFFFu && !T.s_occ[R.sh_slot]) ? 1.0 : 0.0;
+  const double gl = (double)R.gl * vis;
+  const double a = R.pl > 0.0f ? gl / (double)R.pl : 0.0;
+  if (a > 0.0 && isfinite(a * a) && !(T.flags & CHK_TRAIN_FREEZE_LIGHTS)) {
+    const double qpj = qp[R.light];
+    const double qj = (qpj - eq * invL) / (1.0 - eq);
+    const double s = -a * a * (1.0 - eq) * qj / qpj;
+    for (uint32_t k = 0; k < T.n_lights; ++k) {
+      const double qk = ((double)qp[k] - eq * invL) / (1.0 - eq);
+      row[k] = (float)(s * ((k == R.light ? 1.0 : 0.0) - qk));
+    }
+    wa[b] = (float)(a * a);
+  }
+  // mixing part: 2 F dF/dc
+  if (!(T.flags & CHK_TRAIN_FREEZE_MIX)) {
+    const double pl = R.pl, pbl = R.pbl, gb = (double)R.fb * R.le, plb = R.plb, pbb = R.pbb;
+    const double pcl = c * pl + (1.0 - c) * pbl, pcb = c * plb + (1.0 - c) * pbb;
+    const double al = (gl > 0.0 && pcl > 0.0) ? gl / pcl : 0.0;
+    const double ab = (gb > 0.0 && pcb > 0.0) ? gb / pcb : 0.0;
+    const double F = c * al + (1.0 - c) * ab;
+    double dF = al - ab;
+    if (al > 0.0) dF -= c * al * (pl - pbl) / pcl;
+    if (ab > 0.0) dF -= (1.0 - c) * ab * (plb - pbb) / pcb;
+    const double H = 2.0 * F * dF;
+    if (isfinite(H) && H != 0.0) {
+      const double sg = (c - ec) / (1.0 - 2.0 * ec);  // sigmoid of the mixing logit
+      const double gz = H * (1.0 - 2.0 * ec) * sg * (1.0 - sg);
+      row[32] = (float)gz;
+      wh[b] = (float)fabs(gz);
+    }
+  }
+}
+
+// Batch means of the two parts' magnitudes, in a fixed summation order.
+__global__ __launch_bounds__(1024) void k_train_norm(const float *__restrict__ wa,
+                                                     const float *__restrict__ wh, uint32_t batch,
+                                                     float *__restrict__ scale) {
+  __shared__ double sa[1024], sh[1024], na[1024], nh[1024];
+  double a = 0, h = 0, ca = 0, ch = 0;
+  for (uint32_t b = threadIdx.x; b < batch; b += 1024) {
+    a += wa[b];
+    h += wh[b];
+    ca += wa[b] > 0.0f ? 1.0 : 0.0;
+    ch += wh[b] > 0.0f ? 1.0 : 0.0;
+  }
+  sa[threadIdx.x] = a;
+  sh[threadIdx.x] = h;
+  na[threadIdx.x] = ca;
+  nh[threadIdx.x] = ch;
+  __syncthreads();
+  for (int w = 512; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      sa[threadIdx.x] += sa[threadIdx.x + w];
+      sh[threadIdx.x] += sh[threadIdx.x + w];
+      na[threadIdx.x] += na[threadIdx.x + w];
+      nh[threadIdx.x] += nh[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    // rows without a sample count as zero gradient in the mean over the batch
+    scale[0] = sa[0] > 0.0 ? (float)((double)batch / sa[0]) : 0.0f;
+    scale[1] = sh[0] > 0.0 ? (float)((double)batch / sh[0]) : 0.0f;
+  }
+}
+
+// Normalise each head by its batch mean magnitude.  No per-row clipping:
+// the largest rows are the samples of rarely chosen lights (weight
+// (g / (q'_j p_a))^2), and clipping them would bias the light head towards
+// the lights it already prefers.
+__global__ __launch_bounds__(256) void k_train_scale(float *__restrict__ dl, uint32_t batch,
+                                                     uint32_t n_lights,
+                                                     const float *__restrict__ scale) {
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= batch) return;
+  float *row = dl + 33 * (size_t)b;
+  const float sa = scale[0] / (float)batch, sh = scale[1] / (float)batch;
+  for (uint32_t k = 0; k < n_lights; ++k) row[k] *= sa;
+  row[32] *= sh;
+}
+
 }  // namespace chk
 
 // ====================================================================== API
@@ -407,6 +546,9 @@ struct chk_proposal_s {
   // training scratch
   void *scratch = nullptr;
   size_t scratch_bytes = 0;
+  // online-training batch (rows, deltas, magnitudes, scales, gradient)
+  void *train = nullptr;
+  size_t train_bytes = 0;
 };
 
 namespace chk {
@@ -420,14 +562,13 @@ MlpDev proposal_dev(const chk_proposal_s *p) {
   return W;
 }
 
-hipError_t launch_mlp_eval(const chk_proposal_s *p, const float *inputs, uint32_t k,
-                           const uint32_t *kdev, float *probs, float *mix, float *logits,
-                           hipStream_t st) {
-  if (k == 0) return hipSuccess;
-  const uint32_t groups = (k + 15) / 16;
-  const uint32_t blocks = std::min<uint32_t>((groups + 3) / 4, (uint32_t)p->ctx->n_cu * 8);
-  hipLaunchKernelGGL(k_mlp_eval, dim3(blocks), dim3(256), 0, st, proposal_dev(p), inputs, k, kdev,
-                     probs, mix, logits);
+hipError_t launch_mlp_eval(const chk_proposal_s *p, const float *inputs, const SegRange &R,
+                           float *probs, float *mix, float *logits, hipStream_t st) {
+  const uint32_t groups = R.segcap / 16;  // per segment
+  const uint32_t per_seg = std::max<uint32_t>(
+      1u, std::min<uint32_t>((groups + 3) / 4, (uint32_t)p->ctx->n_cu));
+  hipLaunchKernelGGL(k_mlp_eval, dim3(kSegs * per_seg), dim3(256), 0, st, proposal_dev(p), inputs,
+                     R, probs, mix, logits);
   return hipGetLastError();
 }
 
@@ -480,6 +621,60 @@ chk_status proposal_adam(chk_proposal_s *p, const float *grad, float lr, hipStre
   CHK_HIP(hipGetLastError());
   CHK_HIP(pack(p, st));
   return CHK_OK;
+}
+
+uint32_t proposal_n_lights(const chk_proposal_s *p) { return p->n_lights; }
+const chk_ctx_s *proposal_ctx(const chk_proposal_s *p) { return p->ctx; }
+
+// One online training step on `batch` records of a finished bounce: rows
+// b * stride + offset of the records in segment order (stride >= 1 spreads the
+// batch over the whole image).
+chk_status proposal_train_step(chk_proposal_s *p, const SegRange &R, const TrainRec *rec,
+                               const float *inputs, const float *probs, const float *mix,
+                               const uint8_t *s_occ, uint32_t batch, uint32_t stride,
+                               uint32_t offset, float lr, uint32_t flags, hipStream_t st) {
+  const size_t b_in = ((size_t)batch * 7 * 4 + 255) & ~(size_t)255;
+  const size_t b_dl = ((size_t)batch * 33 * 4 + 255) & ~(size_t)255;
+  const size_t b_w = ((size_t)batch * 4 + 255) & ~(size_t)255;
+  const size_t b_g = ((size_t)kMlpParams * 4 + 255) & ~(size_t)255;
+  const size_t need = b_in + b_dl + 2 * b_w + 256 + b_g;
+  if (p->train_bytes < need) {
+    if (p->train) (void)hipFree(p->train);
+    p->train = nullptr;
+    p->train_bytes = 0;
+    if (hipMalloc(&p->train, need) != hipSuccess) {
+      set_error("proposal: cannot allocate the training batch");
+      return CHK_ENOMEM;
+    }
+    p->train_bytes = need;
+  }
+  char *q = (char *)p->train;
+  float *bin = (float *)q;
+  float *dl = (float *)(q + b_in);
+  float *wa = (float *)(q + b_in + b_dl);
+  float *wh = (float *)(q + b_in + b_dl + b_w);
+  float *scale = (float *)(q + b_in + b_dl + 2 * b_w);
+  float *grad = (float *)(q + b_in + b_dl + 2 * b_w + 256);
+  TrainArgs T;
+  T.R = R;
+  T.rec = rec;
+  T.inputs = inputs;
+  T.probs = probs;
+  T.mix = mix;
+  T.s_occ = s_occ;
+  T.n_lights = p->n_lights;
+  T.batch = batch;
+  T.stride = std::max(1u, stride);
+  T.offset = offset;
+  T.flags = flags;
+  const dim3 g((batch + 255) / 256);
+  hipLaunchKernelGGL(k_train_rows, g, dim3(256), 0, st, T, bin, dl, wa, wh);
+  hipLaunchKernelGGL(k_train_norm, dim3(1), dim3(1024), 0, st, wa, wh, batch, scale);
+  hipLaunchKernelGGL(k_train_scale, g, dim3(256), 0, st, dl, batch, p->n_lights, scale);
+  CHK_HIP(hipGetLastError());
+  chk_status s = proposal_grad(p, bin, dl, batch, grad, st);
+  if (s != CHK_OK) return s;
+  return proposal_adam(p, grad, lr, st);
 }
 
 }  // namespace chk
@@ -542,7 +737,7 @@ chk_status chk_proposal_destroy(chk_proposal_t p) {
   if (!p) return CHK_OK;
   (void)hipSetDevice(p->ctx->device);
   for (void *q : {(void *)p->params, (void *)p->adam_m, (void *)p->adam_v, (void *)p->F,
-                  (void *)p->G, (void *)p->bias, p->scratch})
+                  (void *)p->G, (void *)p->bias, p->scratch, p->train})
     if (q) (void)hipFree(q);
   delete p;
   return CHK_OK;
@@ -580,8 +775,9 @@ chk_status chk_proposal_eval(chk_proposal_t p, const float *inputs, uint32_t k, 
     set_error("chk_proposal_eval: null argument");
     return CHK_EINVAL;
   }
+  if (k == 0) return CHK_OK;
   CHK_HIP(hipSetDevice(p->ctx->device));
-  CHK_HIP(launch_mlp_eval(p, inputs, k, nullptr, probs, mix, logits, (hipStream_t)stream));
+  CHK_HIP(launch_mlp_eval(p, inputs, linear_range(k), probs, mix, logits, (hipStream_t)stream));
   return CHK_OK;
 }
 

@@ -26,6 +26,7 @@
 
 #include "chk_device.h"
 #include "chk_internal.h"
+#include "chk_mlp.h"
 
 namespace chk {
 namespace {
@@ -39,7 +40,8 @@ enum Ctrl : uint32_t {
   kQ0 = 0,                        // live-ray counts, queue 0
   kQ1 = kSegs * kSegStride,       // live-ray counts, queue 1
   kQS = 2 * kSegs * kSegStride,   // shadow-ray counts
-  kCtrlWords = 3 * kSegs * kSegStride
+  kQP = 3 * kSegs * kSegStride,   // previous bounce's live-ray counts (training)
+  kCtrlWords = 4 * kSegs * kSegStride
 };
 // 64-bit statistics words: closest-hit group, then any-hit group, in the
 // order of the traversal kernels' counters
@@ -60,11 +62,14 @@ struct CamArgs {
   uint32_t W, H;
 };
 
+// Arguments of one request segment: the request's scene, camera and
+// parameters, and the pixels [pix0, pix0 + n_pix) whose paths occupy chunk
+// path ids [p0, p0 + n_pix * spp).
 struct RenderArgs {
   DeviceScene S;
   CamArgs cam;
   uint32_t spp, sample_offset, max_depth, jitter_mode, flags, key0, key1, req;
-  uint32_t pix0, n_pix;
+  uint32_t pix0, n_pix, p0;
   float inv_nl;
 };
 
@@ -111,7 +116,36 @@ Buffers carve(void *base, uint32_t qcap, uint32_t cap) {
   return B;
 }
 
-__device__ inline uint32_t lane_id() { return __lane_id(); }
+// Learned-proposal buffers, double-buffered by bounce parity.
+struct LBuffers {
+  uint32_t *prev[2];
+  float *inputs[2], *probs[2], *mix[2];
+  TrainRec *rec[2];
+  uint8_t *s_occ;
+};
+
+size_t lbuffers_bytes(uint32_t qcap) {
+  return 2 * (size_t)qcap * (4 + 28 + 128 + 4 + sizeof(TrainRec)) + (size_t)qcap + 16 * 256;
+}
+
+LBuffers lcarve(void *base, uint32_t qcap) {
+  LBuffers L;
+  char *p = (char *)base;
+  auto take = [&](size_t bytes) {
+    char *r = p;
+    p += (bytes + 255) & ~(size_t)255;
+    return (void *)r;
+  };
+  for (int k = 0; k < 2; ++k) {
+    L.prev[k] = (uint32_t *)take((size_t)qcap * 4);
+    L.inputs[k] = (float *)take((size_t)qcap * 28);
+    L.probs[k] = (float *)take((size_t)qcap * 128);
+    L.mix[k] = (float *)take((size_t)qcap * 4);
+    L.rec[k] = (TrainRec *)take((size_t)qcap * sizeof(TrainRec));
+  }
+  L.s_occ = (uint8_t *)take(qcap);
+  return L;
+}
 
 // Block-wide compacting append into two queues: every thread of the block
 // calls it once; threads with pa (pb) get consecutive slots after the
@@ -150,30 +184,48 @@ __global__ void k_reset(uint32_t *ctrl, unsigned long long *stats, int clear_sta
   if (clear_stats && i < kStatWords) stats[i] = 0ull;
 }
 
-// zero the output queue's and the shadow queue's segment counters
+// zero the output queue's and the shadow queue's segment counters; the
+// output queue held the previous bounce's live rays, whose counts are kept
+// in kQP for training on that bounce's records
 __global__ void k_reset_bounce(uint32_t *ctrl, uint32_t out_q) {
   const uint32_t t = threadIdx.x;
-  if (t < kSegs) ctrl[out_q + t * kSegStride] = 0u;
-  else if (t < 2 * kSegs) ctrl[kQS + (t - kSegs) * kSegStride] = 0u;
+  if (t < kSegs) {
+    ctrl[kQP + t * kSegStride] = ctrl[out_q + t * kSegStride];
+    ctrl[out_q + t * kSegStride] = 0u;
+  } else if (t < 2 * kSegs) {
+    ctrl[kQS + (t - kSegs) * kSegStride] = 0u;
+  }
 }
 
 // Camera rays: main.cpp:72-79 with prepareScene.cpp:24-33's cx, cy.
-// Path p of the chunk goes to segment p / per, per = ceil(n / kSegs).
-__global__ void k_camera(RenderArgs A, uint32_t segcap, float4 *__restrict__ q_ray,
-                         float4 *__restrict__ q_thr, uint32_t *__restrict__ q_pid,
-                         float4 *__restrict__ path_L, uint32_t *__restrict__ ctrl) {
-  const uint32_t n = A.n_pix * A.spp;
-  const uint32_t per = (n + kSegs - 1) / kSegs;
-  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p < kSegs) {
-    const uint32_t lo = p * per;
-    ctrl[kQ0 + p * kSegStride] = n > lo ? min(per, n - lo) : 0u;
+// Live-ray counts of the camera queue: the chunk's `total` paths, path p
+// in queue segment p / per, per = ceil(total / kSegs).
+__global__ void k_queue_counts(uint32_t *__restrict__ ctrl, uint32_t total) {
+  const uint32_t r = threadIdx.x;
+  const uint32_t per = (total + kSegs - 1) / kSegs;
+  if (r < kSegs) {
+    const uint32_t lo = r * per;
+    ctrl[kQ0 + r * kSegStride] = total > lo ? min(per, total - lo) : 0u;
   }
-  if (p >= n) return;
+}
+
+// Camera rays of one request segment (paths p0 .. p0 + n_pix * spp of a
+// chunk of `total` paths).  q_seg (batched launches): the segment index of
+// every slot.
+__global__ void k_camera(RenderArgs A, uint32_t segcap, uint32_t total,
+                         float4 *__restrict__ q_ray, float4 *__restrict__ q_thr,
+                         uint32_t *__restrict__ q_pid, float4 *__restrict__ path_L,
+                         uint32_t *__restrict__ q_seg, uint32_t seg) {
+  const uint32_t n = A.n_pix * A.spp;
+  const uint32_t per = (total + kSegs - 1) / kSegs;
+  const uint32_t lp = blockIdx.x * blockDim.x + threadIdx.x;
+  if (lp >= n) return;
+  const uint32_t p = A.p0 + lp;
   const uint32_t r = p / per;
   const size_t slot = (size_t)r * segcap + (p - r * per);
-  const uint32_t idx = A.pix0 + p / A.spp;
-  const uint32_t s = A.sample_offset + p % A.spp;
+  if (q_seg) q_seg[slot] = seg;
+  const uint32_t idx = A.pix0 + lp / A.spp;
+  const uint32_t s = A.sample_offset + lp % A.spp;
   const uint32_t row = idx / A.cam.W, x = idx % A.cam.W;
   const uint32_t y = A.cam.H - 1u - row;
   float jx = 0.34f, jy = 0.20f;  // main.cpp:23-24, k = 0
@@ -206,149 +258,294 @@ __device__ inline bool finite3(f3 a) {
   return isfinite(a.x) && isfinite(a.y) && isfinite(a.z);
 }
 
-// One vertex of every live path.  Block b shades items of segment
-// r = b % kSegs and appends to segment r of the output queues.
-__global__ __launch_bounds__(kShadeBlock) void k_shade(
-    RenderArgs A, uint32_t depth, uint32_t segcap, const float4 *__restrict__ in_ray,
-    const float4 *__restrict__ in_thr, const uint32_t *__restrict__ in_pid,
-    const float4 *__restrict__ hits, float4 *__restrict__ out_ray, float4 *__restrict__ out_thr,
-    uint32_t *__restrict__ out_pid, float4 *__restrict__ s_ray, float4 *__restrict__ s_L,
-    float4 *__restrict__ path_L, uint32_t *__restrict__ ctrl, uint32_t in_q, uint32_t out_q) {
+__device__ inline float lum(f3 a) { return (a.x + a.y + a.z) * (1.0f / 3.0f); }
+
+// Buffers of the learned proposal for one bounce (indexed like the in-queue).
+struct LearnedIO {
+  const float *probs, *mix;             // q' (32 per item) and c of this bounce
+  const float *probs_prev, *mix_prev;   // the same for the previous bounce
+  const uint32_t *in_prev;              // item's previous vertex (previous bounce index)
+  uint32_t *out_prev;                   // written for appended continuations
+  TrainRec *rec, *rec_prev;             // records of this / the previous bounce
+};
+
+// Per-slot request of a batched launch: tab[seg] are the arguments of the
+// request segment a path belongs to.
+struct BatchIO {
+  const RenderArgs *tab;
+  const uint32_t *in_seg;
+  uint32_t *out_seg, *s_seg;
+};
+
+// Shade the vertex of queue slot i (off < total): emission with its MIS
+// weight, one light sample, one BSDF sample.  Outputs the continuation ray,
+// the shadow ray and, for LEARNED, the vertex's training record.
+template <bool LEARNED>
+__device__ __forceinline__ void shade_vertex(const RenderArgs &A, uint32_t depth, size_t i,
+                                             const float4 *__restrict__ in_ray,
+                                             const float4 *__restrict__ in_thr,
+                                             const uint32_t *__restrict__ in_pid,
+                                             const float4 *__restrict__ hits,
+                                             float4 *__restrict__ path_L, const LearnedIO &P,
+                                             bool &emit_ray, bool &emit_sh, float4 &nr0,
+                                             float4 &nr1, float4 &nthr, float4 &sr0, float4 &sr1,
+                                             float4 &sl, uint32_t &pid, TrainRec &rec) {
   const DeviceScene &S = A.S;
-  const uint32_t r = blockIdx.x % kSegs, qb = blockIdx.x / kSegs;
-  const uint32_t total = ctrl[in_q + r * kSegStride];
-  if (qb * kShadeBlock >= total) return;  // whole block idle (uniform)
-  const uint32_t off = qb * kShadeBlock + threadIdx.x;
-  const size_t i = (size_t)r * segcap + off;
-  {
-    bool emit_ray = false, emit_sh = false;
-    float4 nr0, nr1, nthr, sr0, sr1, sl;
-    uint32_t pid = 0;
-    if (off < total) {
-      const float4 h = hits[i];
-      const uint32_t leaf = __float_as_uint(h.y);
-      pid = in_pid[i];
-      if (leaf != 0xFFFFFFFFu) {
-        const float4 r1 = in_ray[2 * (size_t)i + 1];
-        const float4 thr4 = in_thr[i];
-        const f3 d = xyz(r1);
-        const f3 thr = xyz(thr4);
-        const uint2 info = S.leaf_info[leaf];
-        const Mat m = load_mat(S.mats + info.x);
-        const float4 *tp = S.tris + 3 * (size_t)leaf;
-        const f3 A0 = xyz(tp[0]), B0 = xyz(tp[1]), C0 = xyz(tp[2]);
-        // reference normal (C-A)x(B-A) (geometryPrimitives.h:75-79)
-        const f3 Ng = normalize(cross(C0 - A0, B0 - A0));
-        if (m.light != CHK_LIGHT_NONE) {
-          f3 Le;
-          if (depth == 0) {
-            // directly seen emitter: getRadiance = radiance.maxnorm()
-            // (materialTypes.h:34-38, shader.cpp:354-356)
-            Le = (A.flags & CHK_RENDER_PHYSICAL_EMITTER) ? m.rad : maxnorm(m.rad);
+  const float4 h = hits[i];
+  const uint32_t leaf = __float_as_uint(h.y);
+  pid = in_pid[i];
+  if (leaf != 0xFFFFFFFFu) {
+    const float4 r1 = in_ray[2 * i + 1];
+    const float4 thr4 = in_thr[i];
+    const f3 d = xyz(r1);
+    const f3 thr = xyz(thr4);
+    const uint2 info = S.leaf_info[leaf];
+    const Mat m = load_mat(S.mats + info.x);
+    const float4 *tp = S.tris + 3 * (size_t)leaf;
+    const f3 A0 = xyz(tp[0]), B0 = xyz(tp[1]), C0 = xyz(tp[2]);
+    // reference normal (C-A)x(B-A) (geometryPrimitives.h:75-79)
+    const f3 Ng = normalize(cross(C0 - A0, B0 - A0));
+    if (m.light != CHK_LIGHT_NONE) {
+      f3 Le;
+      if (depth == 0) {
+        // directly seen emitter: getRadiance = radiance.maxnorm()
+        // (materialTypes.h:34-38, shader.cpp:354-356)
+        Le = (A.flags & CHK_RENDER_PHYSICAL_EMITTER) ? m.rad : maxnorm(m.rad);
+      } else {
+        float w = 1.0f;
+        if (info.y != 0xFFFFFFFFu) {
+          const float cl = fabsf(dot(d, Ng));
+          const float pb = thr4.w;
+          const float pa = h.x * h.x / (S.lights[info.y].area * cl);  // mis.cpp:55-66
+          if (LEARNED) {
+            const uint32_t prev = P.in_prev[i];
+            const float c = P.mix_prev[prev];
+            const float pl = P.probs_prev[32 * (size_t)prev + info.y] * pa;
+            w = (1.0f - c) * pb / (c * pl + (1.0f - c) * pb);
+            P.rec_prev[prev].le = lum(m.rad);
+            P.rec_prev[prev].plb = pl;
           } else {
-            float w = 1.0f;
-            if (info.y != 0xFFFFFFFFu) {
-              const float cl = fabsf(dot(d, Ng));
-              const float pl = A.inv_nl * h.x * h.x / (S.lights[info.y].area * cl);
-              const float pb = thr4.w;
-              w = pb / (pb + pl);  // balance heuristic, N_light = N_bsdf = 1 (mis.cpp:209-220)
-            }
-            Le = thr * m.rad * w;
+            const float pl = A.inv_nl * pa;
+            w = pb / (pb + pl);  // balance heuristic, N_light = N_bsdf = 1 (mis.cpp:209-220)
           }
-          if (finite3(Le)) {
-            float4 L = path_L[pid];
-            L.x += Le.x;
-            L.y += Le.y;
-            L.z += Le.z;
-            path_L[pid] = L;
-          }
-        } else if (depth < A.max_depth) {
-          const float bw = 1.0f - h.z - h.w;
-          const f3 X = A0 * bw + B0 * h.z + C0 * h.w;
-          const f3 n = dot(d, Ng) > 0.0f ? Ng * -1.0f : Ng;
-          const f3 wo = d * -1.0f;
-          const f3 wo_ref = normalize(n * (2.0f * dot(n, wo)) + d);  // lightSources.cpp:44-45
-          const f3 org = offset_ray(X, n);
-          const uint32_t pix = A.pix0 + pid / A.spp;
-          const uint32_t smp = A.sample_offset + pid % A.spp;
-          // ---- next-event estimation: one point on one mesh light
-          if (S.n_lights) {
-            const u4 r = philox(u4{pix, smp, (depth << 2) | CHK_STREAM_NEE, A.req}, A.key0, A.key1);
-            const uint32_t j = min((uint32_t)(u01(r.x) * (float)S.n_lights), S.n_lights - 1u);
-            const LightRec L = S.lights[j];
-            const float u3 = u01(r.w);
-            uint32_t k = L.first;
-            while (k + 1 < L.first + L.count && u3 >= S.light_tris[k].a_cdf.w) ++k;
-            const LightTri &lt = S.light_tris[k];
-            const f3 la = xyz(lt.a_cdf), lb = xyz(lt.b_area), lc = xyz(lt.c_leaf);
-            // MeshLight::getSamples point (lightSources.cpp:812-824)
-            const float e1 = sqrtf(u01(r.y)), e2 = u01(r.z);
-            const f3 yp = la * (1.0f - e1) + lb * (e1 * (1.0f - e2)) + lc * (e2 * e1);
-            const f3 dv = yp - X;
-            const float d2 = dot(dv, dv);
-            const float dist = sqrtf(d2);
-            const f3 wi = dv * (1.0f / dist);
-            const float cx = dot(wi, n);
-            const f3 nl = cross(lc - la, lb - la);
-            const float cl = fabsf(dot(wi, nl)) * rsqrtf(dot(nl, nl));
-            if (cx > 0.0f && cl > 0.0f) {
-              float f, pb;
-              brdf_pdf(m, n, wo_ref, wo, wi, f, pb);
-              const float pl = A.inv_nl * d2 / (L.area * cl);  // mis.cpp:55-66
-              const f3 c = thr * m.c * xyz(lt.rad) * (f * cx / (pl + pb));
-              if (finite3(c) && (c.x > 0.0f || c.y > 0.0f || c.z > 0.0f)) {
-                // aim from the offset origin at the sampled point, so the ray
-                // reaches the light exactly at t = sdist (shader.cpp:82-90)
-                const f3 sd = yp - org;
-                const float sdist = sqrtf(dot(sd, sd));
-                const f3 sdir = sd * (1.0f / sdist);
-                emit_sh = true;
-                sr0 = make_float4(org.x, org.y, org.z, sdist - 1e-5f);
-                sr1 = make_float4(sdir.x, sdir.y, sdir.z, __uint_as_float(leaf));
-                sl = make_float4(c.x, c.y, c.z, __uint_as_float(pid));
-              }
+        }
+        Le = thr * m.rad * w;
+      }
+      if (finite3(Le)) {
+        float4 L = path_L[pid];
+        L.x += Le.x;
+        L.y += Le.y;
+        L.z += Le.z;
+        path_L[pid] = L;
+      }
+    } else if (depth < A.max_depth) {
+      const float bw = 1.0f - h.z - h.w;
+      const f3 X = A0 * bw + B0 * h.z + C0 * h.w;
+      const f3 n = dot(d, Ng) > 0.0f ? Ng * -1.0f : Ng;
+      const f3 wo = d * -1.0f;
+      const f3 wo_ref = normalize(n * (2.0f * dot(n, wo)) + d);  // lightSources.cpp:44-45
+      const f3 org = offset_ray(X, n);
+      const uint32_t pix = A.pix0 + (pid - A.p0) / A.spp;
+      const uint32_t smp = A.sample_offset + (pid - A.p0) % A.spp;
+      const float c = LEARNED ? P.mix[i] : 0.5f;
+      // ---- next-event estimation: one point on one mesh light
+      if (S.n_lights) {
+        const u4 rr = philox(u4{pix, smp, (depth << 2) | CHK_STREAM_NEE, A.req}, A.key0, A.key1);
+        uint32_t j;
+        float qj;
+        if (LEARNED) {
+          // inverse CDF of q' over the scene's lights
+          const float *q = P.probs + 32 * i;
+          const float u = u01(rr.x);
+          float acc = 0.0f;
+          j = S.n_lights - 1u;
+          for (uint32_t k = 0; k + 1 < S.n_lights; ++k) {
+            acc += q[k];
+            if (u < acc) {
+              j = k;
+              break;
             }
           }
-          // ---- BSDF sample continues the path
-          {
-            const u4 r = philox(u4{pix, smp, (depth << 2) | CHK_STREAM_BSDF, A.req}, A.key0, A.key1);
-            const f3 wi = brdf_sample(m, n, wo_ref, wo, u01(r.x), u01(r.y), u01(r.z));
-            const float cs = dot(wi, n);
-            if (cs > 0.0f) {
-              float f, pb;
-              brdf_pdf(m, n, wo_ref, wo, wi, f, pb);
-              if (pb > 0.0f && isfinite(pb) && isfinite(f)) {
-                const f3 t2 = thr * m.c * (f * cs / pb);
-                if (finite3(t2) && (t2.x > 0.0f || t2.y > 0.0f || t2.z > 0.0f)) {
-                  emit_ray = true;
-                  nr0 = make_float4(org.x, org.y, org.z, kInf);
-                  nr1 = make_float4(wi.x, wi.y, wi.z, __uint_as_float(leaf));
-                  nthr = make_float4(t2.x, t2.y, t2.z, pb);
-                }
-              }
+          qj = q[j];
+        } else {
+          j = min((uint32_t)(u01(rr.x) * (float)S.n_lights), S.n_lights - 1u);
+          qj = A.inv_nl;
+        }
+        const LightRec L = S.lights[j];
+        const float u3 = u01(rr.w);
+        uint32_t k = L.first;
+        while (k + 1 < L.first + L.count && u3 >= S.light_tris[k].a_cdf.w) ++k;
+        const LightTri &lt = S.light_tris[k];
+        const f3 la = xyz(lt.a_cdf), lb = xyz(lt.b_area), lc = xyz(lt.c_leaf);
+        // MeshLight::getSamples point (lightSources.cpp:812-824)
+        const float e1 = sqrtf(u01(rr.y)), e2 = u01(rr.z);
+        const f3 yp = la * (1.0f - e1) + lb * (e1 * (1.0f - e2)) + lc * (e2 * e1);
+        const f3 dv = yp - X;
+        const float d2 = dot(dv, dv);
+        const float dist = sqrtf(d2);
+        const f3 wi = dv * (1.0f / dist);
+        const float cx = dot(wi, n);
+        const f3 nl = cross(lc - la, lb - la);
+        const float cl = fabsf(dot(wi, nl)) * rsqrtf(dot(nl, nl));
+        if (cx > 0.0f && cl > 0.0f) {
+          float f, pb;
+          brdf_pdf(m, n, wo_ref, wo, wi, f, pb);
+          const float pl = qj * d2 / (L.area * cl);  // mis.cpp:55-66
+          const f3 lrad = xyz(lt.rad);
+          // c-weighted balance heuristic; c = 1/2 gives f cos / (p_l + p_b)
+          const f3 cc = thr * m.c * lrad * (f * cx * (2.0f * c) / (2.0f * c * pl + 2.0f * (1.0f - c) * pb));
+          if (LEARNED) {
+            rec.light = j;
+            rec.gl = f * cx * lum(m.c * lrad);
+            rec.pl = pl;
+            rec.pbl = pb;
+          }
+          if (finite3(cc) && (cc.x > 0.0f || cc.y > 0.0f || cc.z > 0.0f)) {
+            // aim from the offset origin at the sampled point, so the ray
+            // reaches the light exactly at t = sdist (shader.cpp:82-90)
+            const f3 sd = yp - org;
+            const float sdist = sqrtf(dot(sd, sd));
+            const f3 sdir = sd * (1.0f / sdist);
+            emit_sh = true;
+            sr0 = make_float4(org.x, org.y, org.z, sdist - 1e-5f);
+            sr1 = make_float4(sdir.x, sdir.y, sdir.z, __uint_as_float(leaf));
+            sl = make_float4(cc.x, cc.y, cc.z, __uint_as_float(pid));
+          }
+        } else if (LEARNED) {
+          rec.light = j;  // a zero-contribution sample still trains q'
+          rec.pl = qj;
+        }
+      }
+      // ---- BSDF sample continues the path
+      {
+        const u4 rr = philox(u4{pix, smp, (depth << 2) | CHK_STREAM_BSDF, A.req}, A.key0, A.key1);
+        const f3 wi = brdf_sample(m, n, wo_ref, wo, u01(rr.x), u01(rr.y), u01(rr.z));
+        const float cs = dot(wi, n);
+        if (cs > 0.0f) {
+          float f, pb;
+          brdf_pdf(m, n, wo_ref, wo, wi, f, pb);
+          if (pb > 0.0f && isfinite(pb) && isfinite(f)) {
+            // the weight 2(1-c) times the 1/2 of the emission weight at
+            // the next vertex is 1 for the plain path
+            const f3 t2 = thr * m.c * (f * cs / pb);
+            if (LEARNED) {
+              rec.fb = f * cs * lum(m.c);
+              rec.pbb = pb;
+            }
+            if (finite3(t2) && (t2.x > 0.0f || t2.y > 0.0f || t2.z > 0.0f)) {
+              emit_ray = true;
+              nr0 = make_float4(org.x, org.y, org.z, kInf);
+              nr1 = make_float4(wi.x, wi.y, wi.z, __uint_as_float(leaf));
+              nthr = make_float4(t2.x, t2.y, t2.z, pb);
             }
           }
         }
       }
     }
-    uint32_t slot, sslot;
-    block_append2<kShadeBlock / 64>(emit_ray, emit_sh, ctrl + out_q + r * kSegStride,
-                                    ctrl + kQS + r * kSegStride, slot, sslot);
-    const size_t seg0 = (size_t)r * segcap;
-    if (emit_ray) {
-      const size_t o = seg0 + slot;
-      out_ray[2 * o] = nr0;
-      out_ray[2 * o + 1] = nr1;
-      out_thr[o] = nthr;
-      out_pid[o] = pid;
-    }
-    if (emit_sh) {
-      const size_t o = seg0 + sslot;
-      s_ray[2 * o] = sr0;
-      s_ray[2 * o + 1] = sr1;
-      s_L[o] = sl;
+  }
+}
+
+// One vertex of every live path.  Block b shades items of segment
+// r = b % kSegs and appends to segment r of the output queues.
+// LEARNED: the light is drawn from the proposal's q' instead of uniformly,
+// and light and BSDF samples are combined with the c-weighted balance
+// heuristic w_l = c p_l / (c p_l + (1-c) p_b), w_b = 1 - w_l (any c in
+// (0,1) keeps the two-sample estimator unbiased; c = 1/2 and q' = 1/L give
+// the plain path exactly).
+// MULTI: every slot names its request's arguments (Q.tab[Q.in_seg[i]]), so
+// one launch shades the paths of many requests and scenes.
+template <bool LEARNED, bool MULTI, int BS>
+__global__ __launch_bounds__(BS) void k_shade(
+    RenderArgs A0, uint32_t depth, uint32_t segcap, const float4 *__restrict__ in_ray,
+    const float4 *__restrict__ in_thr, const uint32_t *__restrict__ in_pid,
+    const float4 *__restrict__ hits, float4 *__restrict__ out_ray, float4 *__restrict__ out_thr,
+    uint32_t *__restrict__ out_pid, float4 *__restrict__ s_ray, float4 *__restrict__ s_L,
+    float4 *__restrict__ path_L, uint32_t *__restrict__ ctrl, uint32_t in_q, uint32_t out_q,
+    LearnedIO P, BatchIO Q) {
+  const uint32_t r = blockIdx.x % kSegs, qb = blockIdx.x / kSegs;
+  const uint32_t total = ctrl[in_q + r * kSegStride];
+  if (qb * BS >= total) return;  // whole block idle (uniform)
+  const uint32_t off = qb * BS + threadIdx.x;
+  const size_t i = (size_t)r * segcap + off;
+  bool emit_ray = false, emit_sh = false;
+  float4 nr0, nr1, nthr, sr0, sr1, sl;
+  uint32_t pid = 0, seg = 0;
+  TrainRec rec;
+  rec.light = 0xFFFFFFFFu;
+  rec.gl = rec.pl = rec.pbl = rec.fb = rec.le = rec.plb = rec.pbb = 0.0f;
+  rec.sh_slot = 0xFFFFFFFFu;
+  rec.pad = 0u;
+  if (off < total) {
+    if (MULTI) {
+      seg = Q.in_seg[i];
+      shade_vertex<LEARNED>(Q.tab[seg], depth, i, in_ray, in_thr, in_pid, hits, path_L, P,
+                            emit_ray, emit_sh, nr0, nr1, nthr, sr0, sr1, sl, pid, rec);
+    } else {
+      shade_vertex<LEARNED>(A0, depth, i, in_ray, in_thr, in_pid, hits, path_L, P, emit_ray,
+                            emit_sh, nr0, nr1, nthr, sr0, sr1, sl, pid, rec);
     }
   }
+  uint32_t slot, sslot;
+  block_append2<BS / 64>(emit_ray, emit_sh, ctrl + out_q + r * kSegStride,
+                         ctrl + kQS + r * kSegStride, slot, sslot);
+  const size_t seg0 = (size_t)r * segcap;
+  if (emit_ray) {
+    const size_t o = seg0 + slot;
+    out_ray[2 * o] = nr0;
+    out_ray[2 * o + 1] = nr1;
+    out_thr[o] = nthr;
+    out_pid[o] = pid;
+    if (LEARNED) P.out_prev[o] = (uint32_t)i;
+    if (MULTI) Q.out_seg[o] = seg;
+  }
+  if (emit_sh) {
+    const size_t o = seg0 + sslot;
+    s_ray[2 * o] = sr0;
+    s_ray[2 * o + 1] = sr1;
+    s_L[o] = sl;
+    if (LEARNED) rec.sh_slot = (uint32_t)o;
+    if (MULTI) Q.s_seg[o] = seg;
+  }
+  if (LEARNED && off < total) P.rec[i] = rec;
+}
+
+// Proposal inputs of every live vertex: position in the scene box, shading
+// normal and view direction as (theta/pi, phi/2pi), each in [0, 1].
+__device__ inline void dir_angles(f3 v, float &a, float &b) {
+  a = acosf(fminf(fmaxf(v.z, -1.0f), 1.0f)) * kInvPi;
+  const float ph = atan2f(v.y, v.x) * (0.5f * kInvPi);
+  b = ph < 0.0f ? ph + 1.0f : ph;
+}
+
+__global__ __launch_bounds__(256) void k_prop_in(RenderArgs A, uint32_t segcap,
+                                                 const float4 *__restrict__ in_ray,
+                                                 const float4 *__restrict__ hits,
+                                                 const uint32_t *__restrict__ ctrl, uint32_t in_q,
+                                                 float *__restrict__ inputs) {
+  const DeviceScene &S = A.S;
+  const uint32_t r = blockIdx.x % kSegs, qb = blockIdx.x / kSegs;
+  const uint32_t off = qb * 256 + threadIdx.x;
+  if (off >= ctrl[in_q + r * kSegStride]) return;
+  const size_t i = (size_t)r * segcap + off;
+  float s[7] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+  const float4 h = hits[i];
+  const uint32_t leaf = __float_as_uint(h.y);
+  if (leaf != 0xFFFFFFFFu) {
+    const f3 d = xyz(in_ray[2 * i + 1]);
+    const float4 *tp = S.tris + 3 * (size_t)leaf;
+    const f3 A0 = xyz(tp[0]), B0 = xyz(tp[1]), C0 = xyz(tp[2]);
+    const f3 Ng = normalize(cross(C0 - A0, B0 - A0));
+    const f3 n = dot(d, Ng) > 0.0f ? Ng * -1.0f : Ng;
+    const f3 X = A0 * (1.0f - h.z - h.w) + B0 * h.z + C0 * h.w;
+    s[0] = fminf(fmaxf((X.x - S.box_lo[0]) * S.box_inv[0], 0.0f), 1.0f);
+    s[1] = fminf(fmaxf((X.y - S.box_lo[1]) * S.box_inv[1], 0.0f), 1.0f);
+    s[2] = fminf(fmaxf((X.z - S.box_lo[2]) * S.box_inv[2], 0.0f), 1.0f);
+    dir_angles(n, s[3], s[4]);
+    dir_angles(d * -1.0f, s[5], s[6]);
+  }
+  float *o = inputs + 7 * i;
+#pragma unroll
+  for (int k = 0; k < 7; ++k) o[k] = s[k];
 }
 
 // Per-pixel FP64 accumulation in sample order (main.cpp:62,83).
@@ -357,7 +554,7 @@ __global__ void k_accumulate(RenderArgs A, const float4 *__restrict__ path_L,
   const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= A.n_pix) return;
   double s0 = 0, s1 = 0, s2 = 0, q0 = 0, q1 = 0, q2 = 0;
-  const float4 *p = path_L + (size_t)q * A.spp;
+  const float4 *p = path_L + A.p0 + (size_t)q * A.spp;
   for (uint32_t s = 0; s < A.spp; ++s) {
     const float4 L = p[s];
     const double a = L.x, b = L.y, c = L.z;
@@ -411,6 +608,8 @@ struct Profiler {
           s->shade_ms += ms;
         } else if (e.first == 2) {
           s->shadow_ms += ms;
+        } else if (e.first == 4) {
+          s->proposal_ms += ms;
         } else {
           s->other_ms += ms;
         }
@@ -422,51 +621,10 @@ struct Profiler {
   }
 };
 
-}  // namespace
-
-chk_status render_impl(chk_scene_s *sc, const chk_camera *cam, const chk_render_params *P,
-                       double *accum, double *sumsq, chk_render_stats *stats, bool count,
-                       hipStream_t st) {
-  if (!cam || !P || !accum) {
-    set_error("chk_render: null camera, params or accum");
-    return CHK_EINVAL;
-  }
-  if (cam->width == 0 || cam->height == 0 || P->spp == 0) {
-    set_error("chk_render: zero width, height or spp");
-    return CHK_EINVAL;
-  }
-  if (P->spp > kChunkPaths) {
-    set_error("chk_render: spp exceeds the paths-per-chunk capacity");
-    return CHK_ECAPACITY;
-  }
-  if (P->mis_mode != CHK_MIS_BALANCE) {
-    set_error("chk_render: mis_mode other than CHK_MIS_BALANCE is not available in this entry point");
-    return CHK_EINVAL;
-  }
-  const uint64_t n_pix_total = (uint64_t)cam->width * cam->height;
-  if (n_pix_total >= (1ull << 32)) {
-    set_error("chk_render: image too large");
-    return CHK_ECAPACITY;
-  }
-  const uint32_t pix_per_chunk = std::max<uint32_t>(1u, kChunkPaths / P->spp);
-  const uint32_t cap = (uint32_t)std::min<uint64_t>((uint64_t)pix_per_chunk * P->spp,
-                                                   n_pix_total * P->spp);
-  const uint32_t segcap = seg_capacity(cap);
-  const uint32_t qcap = kSegs * segcap;
-  const size_t need = buffers_bytes(qcap, cap);
-  if (sc->work_bytes < need) {
-    if (sc->work) (void)hipFree(sc->work);
-    sc->work = nullptr;
-    sc->work_bytes = 0;
-    if (hipMalloc(&sc->work, need) != hipSuccess) {
-      set_error("chk_render: cannot allocate the path queues");
-      return CHK_ENOMEM;
-    }
-    sc->work_bytes = need;
-  }
-  Buffers B = carve(sc->work, qcap, cap);
-
-  RenderArgs A;
+// Request arguments from the ABI records: the camera basis of
+// prepareScene.cpp:21-33 in double, the parameters, the scene.
+chk_status make_args(const chk_scene_s *sc, const chk_camera *cam, const chk_render_params *P,
+                     RenderArgs &A, const char *who) {
   std::memset(&A, 0, sizeof(A));
   A.S = sc->d;
   {
@@ -480,7 +638,7 @@ chk_status render_impl(chk_scene_s *sc, const chk_camera *cam, const chk_render_
                     d[0] * up[1] - d[1] * up[0]};
     double cl = std::sqrt(cx[0] * cx[0] + cx[1] * cx[1] + cx[2] * cx[2]);
     if (!(cl >= 1e-20)) {
-      set_error("chk_render: camera up and direction are parallel");
+      set_error(std::string(who) + ": camera up and direction are parallel");
       return CHK_EINVAL;
     }
     double sx = ((double)cam->width / cam->height) * std::tan(cam->fovx_deg * PI / 360.0) / cl;
@@ -507,12 +665,103 @@ chk_status render_impl(chk_scene_s *sc, const chk_camera *cam, const chk_render_
   A.key1 = P->seed_hi;
   A.req = 0;
   A.inv_nl = sc->d.n_lights ? 1.0f / (float)sc->d.n_lights : 0.0f;
+  return CHK_OK;
+}
+
+}  // namespace
+
+chk_status render_impl(chk_scene_s *sc, const chk_camera *cam, const chk_render_params *P,
+                       double *accum, double *sumsq, chk_render_stats *stats, bool count,
+                       chk_proposal_s *prop, const chk_train_params *train, hipStream_t st) {
+  if (!cam || !P || !accum) {
+    set_error("chk_render: null camera, params or accum");
+    return CHK_EINVAL;
+  }
+  const bool learned = prop != nullptr;
+  if (cam->width == 0 || cam->height == 0 || P->spp == 0) {
+    set_error("chk_render: zero width, height or spp");
+    return CHK_EINVAL;
+  }
+  if (P->spp > kChunkPaths) {
+    set_error("chk_render: spp exceeds the paths-per-chunk capacity");
+    return CHK_ECAPACITY;
+  }
+  if (P->mis_mode != (learned ? CHK_MIS_LEARNED : CHK_MIS_BALANCE)) {
+    set_error(learned ? "chk_render_learned: params->mis_mode must be CHK_MIS_LEARNED"
+                      : "chk_render: mis_mode other than CHK_MIS_BALANCE needs chk_render_learned");
+    return CHK_EINVAL;
+  }
+  bool training = false;
+  if (learned) {
+    if (proposal_ctx(prop) != sc->ctx) {
+      set_error("chk_render_learned: proposal and scene belong to different contexts");
+      return CHK_EINVAL;
+    }
+    if (proposal_n_lights(prop) != sc->d.n_lights) {
+      set_error("chk_render_learned: proposal has " + std::to_string(proposal_n_lights(prop)) +
+                " lights, the scene " + std::to_string(sc->d.n_lights));
+      return CHK_EINVAL;
+    }
+    if (train && P->sample_offset < train->train_spp) {
+      if (train->batch < 16 || train->batch > (1u << 16) || train->steps_per_bounce == 0 ||
+          !(train->lr > 0.0f) ||
+          (train->flags & ~(CHK_TRAIN_FREEZE_MIX | CHK_TRAIN_FREEZE_LIGHTS))) {
+        set_error("chk_render_learned: train batch must be 16..65536, steps >= 1, lr > 0, "
+                  "known flags");
+        return CHK_EINVAL;
+      }
+      training = true;
+    }
+  }
+  const uint64_t n_pix_total = (uint64_t)cam->width * cam->height;
+  if (n_pix_total >= (1ull << 32)) {
+    set_error("chk_render: image too large");
+    return CHK_ECAPACITY;
+  }
+  const uint32_t pix_per_chunk = std::max<uint32_t>(1u, kChunkPaths / P->spp);
+  const uint32_t cap = (uint32_t)std::min<uint64_t>((uint64_t)pix_per_chunk * P->spp,
+                                                   n_pix_total * P->spp);
+  const uint32_t segcap = seg_capacity(cap);
+  const uint32_t qcap = kSegs * segcap;
+  const size_t need = buffers_bytes(qcap, cap);
+  if (sc->work_bytes < need) {
+    if (sc->work) (void)hipFree(sc->work);
+    sc->work = nullptr;
+    sc->work_bytes = 0;
+    if (hipMalloc(&sc->work, need) != hipSuccess) {
+      set_error("chk_render: cannot allocate the path queues");
+      return CHK_ENOMEM;
+    }
+    sc->work_bytes = need;
+  }
+  Buffers B = carve(sc->work, qcap, cap);
+  LBuffers LB{};
+  if (learned) {
+    const size_t lneed = lbuffers_bytes(qcap);
+    if (sc->lwork_bytes < lneed) {
+      if (sc->lwork) (void)hipFree(sc->lwork);
+      sc->lwork = nullptr;
+      sc->lwork_bytes = 0;
+      if (hipMalloc(&sc->lwork, lneed) != hipSuccess) {
+        set_error("chk_render_learned: cannot allocate the proposal buffers");
+        return CHK_ENOMEM;
+      }
+      sc->lwork_bytes = lneed;
+    }
+    LB = lcarve(sc->lwork, qcap);
+  }
+
+  RenderArgs A;
+  {
+    const chk_status as = make_args(sc, cam, P, A, "chk_render");
+    if (as != CHK_OK) return as;
+  }
 
   Profiler prof;
   prof.on = (P->flags & CHK_RENDER_PROFILE) != 0;
   prof.st = st;
   const uint32_t sgrid = kSegs * (segcap / kShadeBlock);
-  uint32_t chunks = 0, ext_launches = 0;
+  uint32_t chunks = 0, ext_launches = 0, train_steps = 0;
   auto qbase = [](uint32_t q) { return q == 0 ? (uint32_t)kQ0 : (uint32_t)kQ1; };
 
   hipLaunchKernelGGL(k_reset, dim3((kCtrlWords + 255) / 256), dim3(256), 0, st, B.ctrl, B.stats,
@@ -525,31 +774,77 @@ chk_status render_impl(chk_scene_s *sc, const chk_camera *cam, const chk_render_
     prof.begin(3);
     hipLaunchKernelGGL(k_reset, dim3((kCtrlWords + 255) / 256), dim3(256), 0, st, B.ctrl, B.stats,
                        0);
+    hipLaunchKernelGGL(k_queue_counts, dim3(1), dim3(64), 0, st, B.ctrl, n_paths);
     hipLaunchKernelGGL(k_camera, dim3((n_paths + 255) / 256), dim3(256), 0, st, A, segcap,
-                       B.q_ray[0], B.q_thr[0], B.q_pid[0], B.path_L, B.ctrl);
+                       n_paths, B.q_ray[0], B.q_thr[0], B.q_pid[0], B.path_L, nullptr, 0u);
     prof.end();
     for (uint32_t depth = 0; depth <= P->max_depth; ++depth) {
       const uint32_t in = depth & 1u, out = in ^ 1u;
       hipLaunchKernelGGL(k_reset_bounce, dim3(1), dim3(64), 0, st, B.ctrl, qbase(out));
       prof.begin(0);
-      hipError_t e = launch_closest(sc, B.q_ray[in], SegRange{B.ctrl + qbase(in), 0u, segcap},
-                                    B.hits, true, count, B.stats + kNodeVisits, st);
+      hipError_t e = launch_closest(sc, TraceShape{sc->stack_size, 128}, B.q_ray[in],
+                                    SegRange{B.ctrl + qbase(in), 0u, segcap}, B.hits, true, count,
+                                    B.stats + kNodeVisits, SlotScenes{}, st);
       prof.end();
       ++ext_launches;
       if (e != hipSuccess) {
         set_error(std::string("chk_render: closest-hit launch: ") + hipGetErrorString(e));
         return CHK_EDEVICE;
       }
+      const SegRange inR{B.ctrl + qbase(in), 0u, segcap};
+      LearnedIO io{};
+      if (learned) {
+        prof.begin(4);
+        hipLaunchKernelGGL(k_prop_in, dim3(kSegs * (segcap / 256)), dim3(256), 0, st, A, segcap,
+                           B.q_ray[in], B.hits, B.ctrl, qbase(in), LB.inputs[in]);
+        e = launch_mlp_eval(prop, LB.inputs[in], inR, LB.probs[in], LB.mix[in], nullptr, st);
+        prof.end();
+        if (e != hipSuccess) {
+          set_error(std::string("chk_render_learned: proposal launch: ") + hipGetErrorString(e));
+          return CHK_EDEVICE;
+        }
+        io.probs = LB.probs[in];
+        io.mix = LB.mix[in];
+        io.probs_prev = LB.probs[out];
+        io.mix_prev = LB.mix[out];
+        io.in_prev = LB.prev[in];
+        io.out_prev = LB.prev[out];
+        io.rec = LB.rec[in];
+        io.rec_prev = LB.rec[out];
+      }
       prof.begin(1);
-      hipLaunchKernelGGL(k_shade, dim3(sgrid), dim3(kShadeBlock), 0, st, A, depth, segcap,
-                         B.q_ray[in], B.q_thr[in], B.q_pid[in], B.hits, B.q_ray[out],
-                         B.q_thr[out], B.q_pid[out], B.s_ray, B.s_L, B.path_L, B.ctrl, qbase(in),
-                         qbase(out));
+      if (learned)
+        hipLaunchKernelGGL((k_shade<true, false, kShadeBlock>), dim3(sgrid), dim3(kShadeBlock), 0,
+                           st, A, depth, segcap, B.q_ray[in], B.q_thr[in], B.q_pid[in], B.hits,
+                           B.q_ray[out], B.q_thr[out], B.q_pid[out], B.s_ray, B.s_L, B.path_L,
+                           B.ctrl, qbase(in), qbase(out), io, BatchIO{});
+      else
+        hipLaunchKernelGGL((k_shade<false, false, kShadeBlock>), dim3(sgrid), dim3(kShadeBlock), 0,
+                           st, A, depth, segcap, B.q_ray[in], B.q_thr[in], B.q_pid[in], B.hits,
+                           B.q_ray[out], B.q_thr[out], B.q_pid[out], B.s_ray, B.s_L, B.path_L,
+                           B.ctrl, qbase(in), qbase(out), io, BatchIO{});
       prof.end();
+      // the previous bounce's records are complete now (its shadow rays
+      // were traced, this bounce found the emitters its BSDF rays hit)
+      if (training && depth > 0) {
+        prof.begin(4);
+        const uint32_t stride = std::max(1u, cap / train->batch);
+        for (uint32_t t = 0; t < train->steps_per_bounce; ++t) {
+          chk_status ts = proposal_train_step(
+              prop, SegRange{B.ctrl + kQP, 0u, segcap}, LB.rec[out], LB.inputs[out],
+              LB.probs[out], LB.mix[out], LB.s_occ, train->batch, stride,
+              (uint32_t)(((uint64_t)t * stride) / train->steps_per_bounce), train->lr,
+              train->flags, st);
+          if (ts != CHK_OK) return ts;
+          ++train_steps;
+        }
+        prof.end();
+      }
       if (depth < P->max_depth) {
         prof.begin(2);
-        e = launch_any(sc, B.s_ray, SegRange{B.ctrl + kQS, 0u, segcap}, nullptr, B.s_L, B.path_L,
-                       true, count, B.stats + kShNodeVisits, st);
+        e = launch_any(sc, TraceShape{sc->stack_size, 128}, B.s_ray,
+                       SegRange{B.ctrl + kQS, 0u, segcap}, training ? LB.s_occ : nullptr, B.s_L,
+                       B.path_L, true, count, B.stats + kShNodeVisits, SlotScenes{}, st);
         prof.end();
         if (e != hipSuccess) {
           set_error(std::string("chk_render: any-hit launch: ") + hipGetErrorString(e));
@@ -577,6 +872,7 @@ chk_status render_impl(chk_scene_s *sc, const chk_camera *cam, const chk_render_
     stats->shadow_tri_tests = h[kShTriTests];
     stats->chunks = chunks;
     stats->extend_launches = ext_launches;
+    stats->train_steps = train_steps;
     prof.collect(stats);
     if (h[kOverflow] || h[kShOverflow]) {
       set_error("chk_render: traversal stack overflow");
@@ -585,6 +881,384 @@ chk_status render_impl(chk_scene_s *sc, const chk_camera *cam, const chk_render_
   } else {
     prof.collect(nullptr);
   }
+  return CHK_OK;
+}
+
+// ====================================================================== batching
+// Many requests (scenes, cameras, parameters) in one wavefront: requests are
+// ordered by cost (W * H * spp, shortest first) and packed into chunks of at
+// most chunk_paths paths; a chunk holds segments (pixel ranges) of several
+// requests.  Every queue slot carries its segment index, so the traversal
+// kernels find the ray's scene and the shading kernel the request's
+// arguments.  Per chunk the sequence is the single-request one; with
+// use_graph each chunk's launches are captured into a hipGraph once and
+// replayed (the executable graphs are kept for the next identical call).
+
+namespace {
+
+struct BatchSeg {
+  uint32_t req, pix0, n_pix, p0;
+};
+struct BatchChunk {
+  uint32_t first_seg, n_seg, total;
+};
+
+struct BatchState {
+  void *work = nullptr;
+  size_t work_bytes = 0;
+  void *tab = nullptr;
+  size_t tab_bytes = 0;
+  std::vector<unsigned char> key;  // request records + tuning of the cached graphs
+  std::vector<hipGraphExec_t> execs;
+  hipStream_t cap = nullptr;
+  void drop_graphs() {
+    for (hipGraphExec_t e : execs) (void)hipGraphExecDestroy(e);
+    execs.clear();
+    key.clear();
+  }
+  ~BatchState() {
+    drop_graphs();
+    if (cap) (void)hipStreamDestroy(cap);
+    if (work) (void)hipFree(work);
+    if (tab) (void)hipFree(tab);
+  }
+};
+
+chk_status ensure(void *&ptr, size_t &have, size_t need, const char *what) {
+  if (have >= need) return CHK_OK;
+  if (ptr) (void)hipFree(ptr);
+  ptr = nullptr;
+  have = 0;
+  if (hipMalloc(&ptr, need) != hipSuccess) {
+    set_error(std::string("chk_render_batch: cannot allocate ") + what);
+    return CHK_ENOMEM;
+  }
+  have = need;
+  return CHK_OK;
+}
+
+template <int BS>
+void launch_shade_multi(uint32_t grid, hipStream_t st, uint32_t depth, uint32_t segcap,
+                        const Buffers &B, uint32_t in, uint32_t out, uint32_t qin, uint32_t qout,
+                        const BatchIO &Q) {
+  hipLaunchKernelGGL((k_shade<false, true, BS>), dim3(grid), dim3(BS), 0, st, RenderArgs{}, depth,
+                     segcap, B.q_ray[in], B.q_thr[in], B.q_pid[in], B.hits, B.q_ray[out],
+                     B.q_thr[out], B.q_pid[out], B.s_ray, B.s_L, B.path_L, B.ctrl, qin, qout,
+                     LearnedIO{}, Q);
+}
+
+}  // namespace
+
+void batch_state_free(void *p) { delete static_cast<BatchState *>(p); }
+
+chk_status batch_impl(chk_ctx_s *ctx, const chk_request *R, uint32_t n,
+                      const chk_batch_tuning *tuning, double *latency_ms, chk_batch_stats *stats,
+                      hipStream_t st) {
+  chk_batch_tuning T{1u << 22, 128, 256, 0};
+  if (tuning) {
+    T = *tuning;
+    if (T.chunk_paths == 0) T.chunk_paths = 1u << 22;
+    if (T.trace_block == 0) T.trace_block = 128;
+    if (T.shade_block == 0) T.shade_block = 256;
+  }
+  if (T.chunk_paths < (1u << 16) || T.chunk_paths > (1u << 24) ||
+      (T.trace_block != 64 && T.trace_block != 128 && T.trace_block != 256) ||
+      (T.shade_block != 128 && T.shade_block != 256 && T.shade_block != 512)) {
+    set_error("chk_render_batch: tuning wants chunk_paths 2^16..2^24, trace_block 64/128/256, "
+              "shade_block 128/256/512");
+    return CHK_EINVAL;
+  }
+  if (stats) std::memset(stats, 0, sizeof(*stats));
+  if (n == 0) return CHK_OK;
+  // ---- requests
+  std::vector<RenderArgs> args(n);
+  uint32_t max_depth = 0;
+  int stack = 32;
+  for (uint32_t i = 0; i < n; ++i) {
+    const chk_request &q = R[i];
+    const std::string who = "chk_render_batch: request " + std::to_string(i);
+    if (!q.scene || q.scene->ctx != ctx || !q.accum) {
+      set_error(who + ": null scene or accum, or scene of another context");
+      return CHK_EINVAL;
+    }
+    if (q.camera.width == 0 || q.camera.height == 0 || q.params.spp == 0) {
+      set_error(who + ": zero width, height or spp");
+      return CHK_EINVAL;
+    }
+    if ((uint64_t)q.camera.width * q.camera.height >= (1ull << 32)) {
+      set_error(who + ": image too large");
+      return CHK_ECAPACITY;
+    }
+    if (q.params.mis_mode != CHK_MIS_BALANCE) {
+      set_error(who + ": batches render with CHK_MIS_BALANCE");
+      return CHK_EINVAL;
+    }
+    if (q.params.spp > T.chunk_paths) {
+      set_error(who + ": spp exceeds chunk_paths");
+      return CHK_ECAPACITY;
+    }
+    const chk_status as = make_args(q.scene, &q.camera, &q.params, args[i], who.c_str());
+    if (as != CHK_OK) return as;
+    max_depth = std::max(max_depth, q.params.max_depth);
+    stack = std::max(stack, q.scene->stack_size);
+  }
+  // ---- shortest request first, then pack chunks
+  std::vector<uint32_t> order(n);
+  for (uint32_t i = 0; i < n; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+    return (uint64_t)args[a].cam.W * args[a].cam.H * args[a].spp <
+           (uint64_t)args[b].cam.W * args[b].cam.H * args[b].spp;
+  });
+  std::vector<BatchSeg> segs;
+  std::vector<BatchChunk> chunks;
+  std::vector<uint32_t> last_chunk(n, 0);
+  uint64_t total_paths = 0;
+  {
+    BatchChunk cur{0, 0, 0};
+    for (uint32_t i : order) {
+      const uint32_t npix = args[i].cam.W * args[i].cam.H, spp = args[i].spp;
+      uint32_t pix = 0;
+      while (pix < npix) {
+        const uint32_t room = (T.chunk_paths - cur.total) / spp;
+        if (room == 0) {
+          chunks.push_back(cur);
+          cur = BatchChunk{(uint32_t)segs.size(), 0, 0};
+          continue;
+        }
+        const uint32_t take = std::min(room, npix - pix);
+        segs.push_back(BatchSeg{i, pix, take, cur.total});
+        ++cur.n_seg;
+        cur.total += take * spp;
+        pix += take;
+        total_paths += (uint64_t)take * spp;
+        last_chunk[i] = (uint32_t)chunks.size();
+      }
+    }
+    if (cur.n_seg) chunks.push_back(cur);
+  }
+  // ---- state, buffers, tables
+  BatchState *bs = static_cast<BatchState *>(ctx->batch_state);
+  if (!bs) {
+    bs = new BatchState();
+    ctx->batch_state = bs;
+  }
+  const uint32_t cap = T.chunk_paths;
+  const uint32_t segcap = seg_capacity(cap);
+  const uint32_t qcap = kSegs * segcap;
+  const size_t need = buffers_bytes(qcap, cap) + 3 * (size_t)qcap * 4 + 4 * 256;
+  std::vector<unsigned char> key(sizeof(chk_request) * n + sizeof(T));
+  std::memcpy(key.data(), R, sizeof(chk_request) * n);
+  std::memcpy(key.data() + sizeof(chk_request) * n, &T, sizeof(T));
+  const bool cached = T.use_graph && bs->key == key && !bs->execs.empty() &&
+                      bs->work_bytes >= need;
+  if (!cached) bs->drop_graphs();
+  chk_status cs = ensure(bs->work, bs->work_bytes, need, "the path queues");
+  if (cs != CHK_OK) return cs;
+  const size_t tab_need = segs.size() * (sizeof(RenderArgs) + sizeof(DeviceScene)) + 512;
+  cs = ensure(bs->tab, bs->tab_bytes, tab_need, "the request tables");
+  if (cs != CHK_OK) return cs;
+  Buffers B = carve(bs->work, qcap, cap);
+  uint32_t *segbuf = (uint32_t *)((char *)bs->work + ((buffers_bytes(qcap, cap) + 255) & ~(size_t)255));
+  uint32_t *q_seg[2] = {segbuf, segbuf + qcap};
+  uint32_t *s_seg = segbuf + 2 * (size_t)qcap;
+  std::vector<RenderArgs> seg_args(segs.size());
+  std::vector<DeviceScene> seg_scene(segs.size());
+  for (size_t k = 0; k < segs.size(); ++k) {
+    RenderArgs a = args[segs[k].req];
+    a.pix0 = segs[k].pix0;
+    a.n_pix = segs[k].n_pix;
+    a.p0 = segs[k].p0;
+    seg_args[k] = a;
+    seg_scene[k] = a.S;
+  }
+  RenderArgs *d_args = (RenderArgs *)bs->tab;
+  DeviceScene *d_scene =
+      (DeviceScene *)((char *)bs->tab + ((segs.size() * sizeof(RenderArgs) + 255) & ~(size_t)255));
+  CHK_HIP(hipStreamSynchronize(st));
+  CHK_HIP(hipMemcpy(d_args, seg_args.data(), segs.size() * sizeof(RenderArgs),
+                    hipMemcpyHostToDevice));
+  CHK_HIP(hipMemcpy(d_scene, seg_scene.data(), segs.size() * sizeof(DeviceScene),
+                    hipMemcpyHostToDevice));
+  const TraceShape shape{stack, (int)T.trace_block};
+  auto qbase = [](uint32_t q) { return q == 0 ? (uint32_t)kQ0 : (uint32_t)kQ1; };
+  // ---- the launches of one chunk
+  auto emit_chunk = [&](uint32_t c, hipStream_t s) -> hipError_t {
+    const BatchChunk &C = chunks[c];
+    hipLaunchKernelGGL(k_reset, dim3((kCtrlWords + 255) / 256), dim3(256), 0, s, B.ctrl, B.stats,
+                       c == 0 ? 1 : 0);
+    hipLaunchKernelGGL(k_queue_counts, dim3(1), dim3(64), 0, s, B.ctrl, C.total);
+    for (uint32_t k = 0; k < C.n_seg; ++k) {
+      const RenderArgs &a = seg_args[C.first_seg + k];
+      const uint32_t np = a.n_pix * a.spp;
+      hipLaunchKernelGGL(k_camera, dim3((np + 255) / 256), dim3(256), 0, s, a, segcap, C.total,
+                         B.q_ray[0], B.q_thr[0], B.q_pid[0], B.path_L, q_seg[0], k);
+    }
+    for (uint32_t depth = 0; depth <= max_depth; ++depth) {
+      const uint32_t in = depth & 1u, out = in ^ 1u;
+      hipLaunchKernelGGL(k_reset_bounce, dim3(1), dim3(64), 0, s, B.ctrl, qbase(out));
+      hipError_t e = launch_closest(nullptr, shape, B.q_ray[in],
+                                    SegRange{B.ctrl + qbase(in), 0u, segcap}, B.hits, true, false,
+                                    B.stats + kNodeVisits,
+                                    SlotScenes{d_scene + C.first_seg, q_seg[in]}, s);
+      if (e != hipSuccess) return e;
+      const BatchIO Q{d_args + C.first_seg, q_seg[in], q_seg[out], s_seg};
+      const uint32_t sgrid = kSegs * (segcap / T.shade_block);
+      if (T.shade_block == 128)
+        launch_shade_multi<128>(sgrid, s, depth, segcap, B, in, out, qbase(in), qbase(out), Q);
+      else if (T.shade_block == 512)
+        launch_shade_multi<512>(sgrid, s, depth, segcap, B, in, out, qbase(in), qbase(out), Q);
+      else
+        launch_shade_multi<256>(sgrid, s, depth, segcap, B, in, out, qbase(in), qbase(out), Q);
+      if (depth < max_depth) {
+        e = launch_any(nullptr, shape, B.s_ray, SegRange{B.ctrl + kQS, 0u, segcap}, nullptr,
+                       B.s_L, B.path_L, true, false, B.stats + kShNodeVisits,
+                       SlotScenes{d_scene + C.first_seg, s_seg}, s);
+        if (e != hipSuccess) return e;
+      }
+    }
+    for (uint32_t k = 0; k < C.n_seg; ++k) {
+      const RenderArgs &a = seg_args[C.first_seg + k];
+      const chk_request &q = R[segs[C.first_seg + k].req];
+      hipLaunchKernelGGL(k_accumulate, dim3((a.n_pix + 255) / 256), dim3(256), 0, s, a, B.path_L,
+                         q.accum, (q.params.flags & CHK_RENDER_SUMSQ) ? q.sumsq : nullptr);
+    }
+    return hipGetLastError();
+  };
+  // ---- capture (graph mode, not cached)
+  double capture_ms = 0.0;
+  if (T.use_graph && !cached) {
+    if (!bs->cap) CHK_HIP(hipStreamCreateWithFlags(&bs->cap, hipStreamNonBlocking));
+    const auto t0 = std::chrono::steady_clock::now();
+    for (uint32_t c = 0; c < chunks.size(); ++c) {
+      CHK_HIP(hipStreamBeginCapture(bs->cap, hipStreamCaptureModeThreadLocal));
+      const hipError_t e = emit_chunk(c, bs->cap);
+      hipGraph_t g = nullptr;
+      const hipError_t e2 = hipStreamEndCapture(bs->cap, &g);
+      if (e != hipSuccess || e2 != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        bs->drop_graphs();
+        set_error(std::string("chk_render_batch: graph capture: ") +
+                  hipGetErrorString(e != hipSuccess ? e : e2));
+        return CHK_EDEVICE;
+      }
+      hipGraphExec_t x = nullptr;
+      const hipError_t e3 = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e3 != hipSuccess) {
+        bs->drop_graphs();
+        set_error(std::string("chk_render_batch: graph instantiate: ") + hipGetErrorString(e3));
+        return CHK_EDEVICE;
+      }
+      bs->execs.push_back(x);
+    }
+    bs->key = key;
+    capture_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+  // ---- run, one event after each chunk
+  std::vector<hipEvent_t> ev(chunks.size() + 1, nullptr);
+  for (auto &e : ev) CHK_HIP(hipEventCreate(&e));
+  auto free_events = [&]() {
+    for (auto &e : ev)
+      if (e) (void)hipEventDestroy(e);
+  };
+  hipError_t err = hipEventRecord(ev[0], st);
+  for (uint32_t c = 0; c < chunks.size() && err == hipSuccess; ++c) {
+    err = T.use_graph ? hipGraphLaunch(bs->execs[c], st) : emit_chunk(c, st);
+    if (err == hipSuccess) err = hipEventRecord(ev[c + 1], st);
+  }
+  if (err == hipSuccess) err = hipEventSynchronize(ev[chunks.size()]);
+  if (err != hipSuccess) {
+    free_events();
+    set_error(std::string("chk_render_batch: ") + hipGetErrorString(err));
+    return CHK_EDEVICE;
+  }
+  if (latency_ms) {
+    for (uint32_t i = 0; i < n; ++i) {
+      float ms = 0.0f;
+      (void)hipEventElapsedTime(&ms, ev[0], ev[last_chunk[i] + 1]);
+      latency_ms[i] = ms;
+    }
+  }
+  if (stats) {
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, ev[0], ev[chunks.size()]);
+    unsigned long long h[kStatWords];
+    CHK_HIP(hipMemcpy(h, B.stats, sizeof(h), hipMemcpyDeviceToHost));
+    stats->paths = total_paths;
+    stats->rays_extend = h[kRaysExt];
+    stats->rays_shadow = h[kRaysShadow];
+    stats->chunks = (uint32_t)chunks.size();
+    stats->segments = (uint32_t)segs.size();
+    stats->batch_ms = ms;
+    stats->capture_ms = capture_ms;
+    stats->graph_cached = cached ? 1u : 0u;
+  }
+  free_events();
+  if (!T.use_graph) bs->drop_graphs();
+  return CHK_OK;
+}
+
+// Autotuner: time every (chunk_paths, trace_block, shade_block) candidate
+// on the batch with spp divided by 4 (at least 1), rendering into scratch
+// accumulators, then eager against graph replay for the fastest.
+chk_status batch_autotune(chk_ctx_s *ctx, const chk_request *R, uint32_t n,
+                          chk_batch_tuning *best, double *best_ms, hipStream_t st) {
+  if (n == 0) {
+    *best = chk_batch_tuning{1u << 22, 128, 256, 0};
+    return CHK_OK;
+  }
+  size_t maxpix = 0;
+  for (uint32_t i = 0; i < n; ++i)
+    maxpix = std::max<size_t>(maxpix, (size_t)R[i].camera.width * R[i].camera.height);
+  double *scratch = nullptr;
+  if (hipMalloc(&scratch, maxpix * 3 * sizeof(double) * 2) != hipSuccess) {
+    set_error("chk_batch_autotune: cannot allocate scratch accumulators");
+    return CHK_ENOMEM;
+  }
+  std::vector<chk_request> Q(R, R + n);
+  for (auto &q : Q) {
+    q.params.spp = std::max(1u, q.params.spp / 4);
+    // every request adds into the same scratch: the values are discarded
+    q.accum = scratch;
+    q.sumsq = scratch + maxpix * 3;
+  }
+  const uint32_t chunk_c[] = {1u << 21, 1u << 22, 1u << 23};
+  const uint32_t tb_c[] = {64, 128, 256};
+  const uint32_t sb_c[] = {128, 256, 512};
+  double bms = 1e30;
+  chk_batch_tuning bt{1u << 22, 128, 256, 0};
+  chk_status s = CHK_OK;
+  for (uint32_t c : chunk_c)
+    for (uint32_t tb : tb_c)
+      for (uint32_t sb : sb_c) {
+        chk_batch_tuning t{c, tb, sb, 0};
+        chk_batch_stats bst;
+        for (int rep = 0; rep < 2 && s == CHK_OK; ++rep)
+          s = batch_impl(ctx, Q.data(), n, &t, nullptr, &bst, st);
+        if (s != CHK_OK) break;
+        if (bst.batch_ms < bms) {
+          bms = bst.batch_ms;
+          bt = t;
+        }
+      }
+  if (s == CHK_OK) {
+    chk_batch_tuning g = bt;
+    g.use_graph = 1;
+    chk_batch_stats bst;
+    for (int rep = 0; rep < 3 && s == CHK_OK; ++rep) s = batch_impl(ctx, Q.data(), n, &g, nullptr, &bst, st);
+    if (s == CHK_OK && bst.batch_ms < bms) {
+      bms = bst.batch_ms;
+      bt = g;
+    }
+    BatchState *bs = static_cast<BatchState *>(ctx->batch_state);
+    if (bs) bs->drop_graphs();  // they render into the scratch accumulators
+  }
+  (void)hipStreamSynchronize(st);
+  (void)hipFree(scratch);
+  if (s != CHK_OK) return s;
+  *best = bt;
+  if (best_ms) *best_ms = bms;
   return CHK_OK;
 }
 

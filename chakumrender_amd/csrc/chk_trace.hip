@@ -17,13 +17,12 @@
 namespace chk {
 namespace {
 
-constexpr int kBlock = 128;
 
 __device__ inline float safe_rcp(float x) {
   return 1.0f / (fabsf(x) > 1e-20f ? x : copysignf(1e-20f, x));
 }
 
-template <bool ANY, bool COUNT, int STACK>
+template <bool ANY, bool COUNT, int STACK, int BS>
 __device__ inline void traverse(const DeviceScene &S, uint32_t *__restrict__ stk, f3 o, f3 d,
                                 float tmax, uint32_t skip, bool leaf_ids, float &t_out,
                                 uint32_t &id_out, float &u_out, float &v_out, bool &hit_any,
@@ -79,7 +78,7 @@ __device__ inline void traverse(const DeviceScene &S, uint32_t *__restrict__ stk
         // push far-to-near so the nearer ones pop first
         for (int c = nh - 1; c >= 1; --c) {
           if (sp < STACK) {
-            stk[sp * kBlock] = cid[c];
+            stk[sp * BS] = cid[c];
             ++sp;
           } else {
             overflow = 1;
@@ -111,7 +110,7 @@ __device__ inline void traverse(const DeviceScene &S, uint32_t *__restrict__ stk
     }
     if (sp == 0) break;
     --sp;
-    entry = stk[sp * kBlock];
+    entry = stk[sp * BS];
   }
   t_out = best == 0xFFFFFFFFu ? 1e30f : t;
   id_out = best;
@@ -133,14 +132,15 @@ __device__ inline float refine_t(const float4 *tp, f3 o, f3 d) {
   return (float)(num / den);
 }
 
-template <bool ANY, bool COUNT, int STACK>
-__global__ __launch_bounds__(kBlock) void k_trace(DeviceScene S, const float4 *__restrict__ rays,
-                                                  SegRange W, float4 *__restrict__ hits,
-                                                  uint8_t *__restrict__ occluded,
-                                                  const float4 *__restrict__ shadow_L,
-                                                  float4 *__restrict__ path_L, int leaf_ids,
-                                                  unsigned long long *__restrict__ counters) {
-  __shared__ uint32_t stack_mem[STACK * kBlock];
+template <bool ANY, bool COUNT, int STACK, int BS, bool MULTI>
+__global__ __launch_bounds__(BS) void k_trace(DeviceScene S0, const float4 *__restrict__ rays,
+                                              SegRange W, float4 *__restrict__ hits,
+                                              uint8_t *__restrict__ occluded,
+                                              const float4 *__restrict__ shadow_L,
+                                              float4 *__restrict__ path_L, int leaf_ids,
+                                              unsigned long long *__restrict__ counters,
+                                              SlotScenes M) {
+  __shared__ uint32_t stack_mem[STACK * BS];
   uint32_t *stk = stack_mem + threadIdx.x;
   const uint32_t lane = threadIdx.x & 63u;
   // block -> (segment, block within the segment); consecutive blocks go to
@@ -151,9 +151,9 @@ __global__ __launch_bounds__(kBlock) void k_trace(DeviceScene S, const float4 *_
     for (uint32_t k = 0; k < kSegs; ++k) tot += W.count_of(k);
     atomicAdd(&counters[3], tot);  // rays traced by this launch
   }
-  const uint32_t off = q * kBlock + threadIdx.x;
+  const uint32_t off = q * BS + threadIdx.x;
   const uint32_t cnt = W.count_of(r);
-  if (q * kBlock >= cnt) return;  // whole block idle (uniform)
+  if (q * BS >= cnt) return;  // whole block idle (uniform)
   uint32_t nv = 0, nt = 0, overflow = 0;
   if (off < cnt) {
     const size_t i = (size_t)r * W.segcap + off;
@@ -161,10 +161,21 @@ __global__ __launch_bounds__(kBlock) void k_trace(DeviceScene S, const float4 *_
     float t, u, v;
     uint32_t id, leaf;
     bool any;
-    traverse<ANY, COUNT, STACK>(S, stk, xyz(ot), xyz(ds), ot.w, __float_as_uint(ds.w),
-                                leaf_ids != 0, t, id, u, v, any, leaf, nv, nt, overflow);
-    // the caller-facing query reports t in FP64 accuracy
-    if (!ANY && !leaf_ids && id != 0xFFFFFFFFu)
+    if (MULTI) {
+      // the ray's own scene (a batched launch mixes requests)
+      const DeviceScene *ps = M.scene + M.slot_seg[i];
+      DeviceScene S;
+      S.nodes = ps->nodes;
+      S.tris = ps->tris;
+      traverse<ANY, COUNT, STACK, BS>(S, stk, xyz(ot), xyz(ds), ot.w, __float_as_uint(ds.w),
+                                      leaf_ids != 0, t, id, u, v, any, leaf, nv, nt, overflow);
+    } else {
+      traverse<ANY, COUNT, STACK, BS>(S0, stk, xyz(ot), xyz(ds), ot.w, __float_as_uint(ds.w),
+                                      leaf_ids != 0, t, id, u, v, any, leaf, nv, nt, overflow);
+    }
+    const DeviceScene &S = S0;
+  // the caller-facing query reports t in FP64 accuracy
+    if (!MULTI && !ANY && !leaf_ids && id != 0xFFFFFFFFu)
       t = refine_t(S.tris + 3 * (size_t)leaf, xyz(ot), xyz(ds));
     if (ANY) {
       if (occluded) occluded[i] = any ? 1 : 0;
@@ -204,29 +215,62 @@ __global__ void k_zero_u32(uint32_t *p, uint32_t n) {
   if (i < n) p[i] = 0u;
 }
 
-template <bool ANY, bool COUNT, int STACK>
+template <bool ANY, bool COUNT, int STACK, int BS, bool MULTI>
 hipError_t launch_t(const chk_scene_s *s, const float4 *rays, const SegRange &W, float4 *hits,
                     uint8_t *occ, const float4 *sL, float4 *pL, bool leaf_ids,
-                    unsigned long long *counters, hipStream_t st) {
-  const uint32_t grid = kSegs * (W.segcap / kBlock);
+                    unsigned long long *counters, const SlotScenes &M, hipStream_t st) {
+  const uint32_t grid = kSegs * (W.segcap / BS);
   if (grid == 0) return hipSuccess;
-  hipLaunchKernelGGL((k_trace<ANY, COUNT, STACK>), dim3(grid), dim3(kBlock), 0, st, s->d, rays, W,
-                     hits, occ, sL, pL, leaf_ids ? 1 : 0, counters);
+  hipLaunchKernelGGL((k_trace<ANY, COUNT, STACK, BS, MULTI>), dim3(grid), dim3(BS), 0, st,
+                     s ? s->d : DeviceScene{}, rays, W, hits, occ, sL, pL, leaf_ids ? 1 : 0,
+                     counters, M);
   return hipGetLastError();
 }
 
-template <bool ANY, bool COUNT>
-hipError_t launch_s(const chk_scene_s *s, const float4 *rays, const SegRange &W, float4 *hits,
-                    uint8_t *occ, const float4 *sL, float4 *pL, bool leaf_ids,
-                    unsigned long long *counters, hipStream_t st) {
-  switch (s->stack_size) {
+template <bool ANY, bool COUNT, bool MULTI, int BS>
+hipError_t launch_s(const chk_scene_s *s, int stack, const float4 *rays, const SegRange &W,
+                    float4 *hits, uint8_t *occ, const float4 *sL, float4 *pL, bool leaf_ids,
+                    unsigned long long *counters, const SlotScenes &M, hipStream_t st) {
+  switch (stack) {
     case 32:
-      return launch_t<ANY, COUNT, 32>(s, rays, W, hits, occ, sL, pL, leaf_ids, counters, st);
+      return launch_t<ANY, COUNT, 32, BS, MULTI>(s, rays, W, hits, occ, sL, pL, leaf_ids,
+                                                 counters, M, st);
     case 64:
-      return launch_t<ANY, COUNT, 64>(s, rays, W, hits, occ, sL, pL, leaf_ids, counters, st);
+      return launch_t<ANY, COUNT, 64, BS, MULTI>(s, rays, W, hits, occ, sL, pL, leaf_ids,
+                                                 counters, M, st);
     default:
-      return launch_t<ANY, COUNT, 96>(s, rays, W, hits, occ, sL, pL, leaf_ids, counters, st);
+      return launch_t<ANY, COUNT, 96, BS, MULTI>(s, rays, W, hits, occ, sL, pL, leaf_ids,
+                                                 counters, M, st);
   }
+}
+
+template <bool ANY>
+hipError_t launch_kind(const chk_scene_s *s, const TraceShape &T, const float4 *rays,
+                       const SegRange &W, float4 *hits, uint8_t *occ, const float4 *sL,
+                       float4 *pL, bool leaf_ids, bool count, unsigned long long *counters,
+                       const SlotScenes &M, hipStream_t st) {
+  const bool multi = M.scene != nullptr;
+  if (count)  // the counted render (work model): single scene, 128-thread blocks
+    return launch_s<ANY, true, false, 128>(s, T.stack, rays, W, hits, occ, sL, pL, leaf_ids,
+                                           counters, M, st);
+  if (multi) {
+    if (T.block == 64)
+      return launch_s<ANY, false, true, 64>(s, T.stack, rays, W, hits, occ, sL, pL, leaf_ids,
+                                            counters, M, st);
+    if (T.block == 256)
+      return launch_s<ANY, false, true, 256>(s, T.stack, rays, W, hits, occ, sL, pL, leaf_ids,
+                                             counters, M, st);
+    return launch_s<ANY, false, true, 128>(s, T.stack, rays, W, hits, occ, sL, pL, leaf_ids,
+                                           counters, M, st);
+  }
+  if (T.block == 64)
+    return launch_s<ANY, false, false, 64>(s, T.stack, rays, W, hits, occ, sL, pL, leaf_ids,
+                                           counters, M, st);
+  if (T.block == 256)
+    return launch_s<ANY, false, false, 256>(s, T.stack, rays, W, hits, occ, sL, pL, leaf_ids,
+                                            counters, M, st);
+  return launch_s<ANY, false, false, 128>(s, T.stack, rays, W, hits, occ, sL, pL, leaf_ids,
+                                          counters, M, st);
 }
 
 }  // namespace
@@ -238,24 +282,19 @@ int pick_stack_size(uint32_t bound) {
   return -1;
 }
 
-hipError_t launch_closest(const chk_scene_s *s, const float4 *rays, const SegRange &W,
-                          float4 *hits, bool leaf_ids, bool count, unsigned long long *counters,
-                          hipStream_t st) {
-  if (count)
-    return launch_s<false, true>(s, rays, W, hits, nullptr, nullptr, nullptr, leaf_ids, counters,
-                                 st);
-  return launch_s<false, false>(s, rays, W, hits, nullptr, nullptr, nullptr, leaf_ids, counters,
-                                st);
+hipError_t launch_closest(const chk_scene_s *s, const TraceShape &T, const float4 *rays,
+                          const SegRange &W, float4 *hits, bool leaf_ids, bool count,
+                          unsigned long long *counters, const SlotScenes &M, hipStream_t st) {
+  return launch_kind<false>(s, T, rays, W, hits, nullptr, nullptr, nullptr, leaf_ids, count,
+                            counters, M, st);
 }
 
-hipError_t launch_any(const chk_scene_s *s, const float4 *rays, const SegRange &W,
-                      uint8_t *occluded, const float4 *shadow_L, float4 *path_L, bool leaf_ids,
-                      bool count, unsigned long long *counters, hipStream_t st) {
-  if (count)
-    return launch_s<true, true>(s, rays, W, nullptr, occluded, shadow_L, path_L, leaf_ids,
-                                counters, st);
-  return launch_s<true, false>(s, rays, W, nullptr, occluded, shadow_L, path_L, leaf_ids,
-                               counters, st);
+hipError_t launch_any(const chk_scene_s *s, const TraceShape &T, const float4 *rays,
+                      const SegRange &W, uint8_t *occluded, const float4 *shadow_L,
+                      float4 *path_L, bool leaf_ids, bool count, unsigned long long *counters,
+                      const SlotScenes &M, hipStream_t st) {
+  return launch_kind<true>(s, T, rays, W, nullptr, occluded, shadow_L, path_L, leaf_ids, count,
+                           counters, M, st);
 }
 
 hipError_t launch_zero_u32(uint32_t *p, uint32_t n, hipStream_t st) {

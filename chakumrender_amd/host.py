@@ -244,6 +244,106 @@ def render(ds: abi.DeviceScene, params, camera=None, image: Image | None = None,
     return a, s, sd
 
 
+def render_learned(ds: abi.DeviceScene, params, proposal: "Proposal", train=None, camera=None,
+                   image: Image | None = None, stats=True, stream=None, download=True):
+    """chk_render_learned: params with mis_mode MIS_LEARNED; train a dict
+    (train_spp, batch, steps_per_bounce, lr) or None for inference only."""
+    cam = abi.camera_struct(camera or ds.scene.camera)
+    d = dict(params)
+    d.setdefault("mis_mode", abi.MIS_LEARNED)
+    P = abi.params_struct(d)
+    if image is None:
+        image = Image(cam.width, cam.height, sumsq=bool(P.flags & abi.RENDER_SUMSQ))
+    T = None
+    if train is not None:
+        T = abi.chk_train_params()
+        T.train_spp = int(train.get("train_spp", 32))
+        T.batch = int(train.get("batch", 16384))
+        T.steps_per_bounce = int(train.get("steps_per_bounce", 2))
+        T.lr = float(train.get("lr", 1e-3))
+        T.flags = int(train.get("flags", 0))
+    st = abi.chk_render_stats()
+    abi.check(ds.L.chk_render_learned(ds.h, C.byref(cam), C.byref(P), proposal.h,
+                                      C.byref(T) if T is not None else None, image.acc.ptr,
+                                      image.sq.ptr if image.sq is not None else None,
+                                      C.byref(st) if stats else None, stream),
+              "chk_render_learned")
+    sd = {k: getattr(st, k) for k, _ in abi.chk_render_stats._fields_} if stats else {}
+    if not download:
+        return image, sd
+    a, s2 = image.get()
+    return a, s2, sd
+
+
+def _requests(items):
+    """items: dicts (ds, params, camera=None) -> (chk_request array, Images)."""
+    arr = (abi.chk_request * len(items))()
+    images = []
+    for k, it in enumerate(items):
+        ds = it["ds"]
+        cam = abi.camera_struct(it.get("camera") or ds.scene.camera)
+        P = abi.params_struct(it["params"])
+        img = Image(cam.width, cam.height, sumsq=bool(P.flags & abi.RENDER_SUMSQ))
+        arr[k].scene = ds.h.value if hasattr(ds.h, "value") else ds.h
+        arr[k].camera = cam
+        arr[k].params = P
+        arr[k].accum = img.acc.ptr
+        arr[k].sumsq = img.sq.ptr if img.sq is not None else None
+        images.append(img)
+    return arr, images
+
+
+def _tuning(t):
+    if t is None:
+        return None
+    T = abi.chk_batch_tuning()
+    for k, _ in abi.chk_batch_tuning._fields_:
+        setattr(T, k, int(t.get(k, 0)) if isinstance(t, dict) else int(getattr(t, k)))
+    return T
+
+
+class Batch:
+    """A fixed set of requests and their accumulators; run() renders them all
+    with chk_render_batch (repeated runs add to the same images, and with
+    use_graph reuse the captured graphs)."""
+
+    def __init__(self, ctx: abi.Context, items):
+        self.ctx = ctx
+        self.n = len(items)
+        self.arr, self.images = _requests(items)
+
+    def run(self, tuning=None, stream=None):
+        T = _tuning(tuning)
+        lat = (C.c_double * self.n)()
+        st = abi.chk_batch_stats()
+        abi.check(self.ctx.L.chk_render_batch(self.ctx.h, self.arr, self.n,
+                                              C.byref(T) if T is not None else None, lat,
+                                              C.byref(st), stream), "chk_render_batch")
+        return list(lat), {k: getattr(st, k) for k, _ in abi.chk_batch_stats._fields_}
+
+    def get(self):
+        return [im.get() for im in self.images]
+
+
+def render_batch(ctx: abi.Context, items, tuning=None, stream=None):
+    """chk_render_batch over items (dicts with ds, params, optional camera).
+    Returns ([(accum, sumsq or None)], latency_ms list, stats dict)."""
+    b = Batch(ctx, items)
+    lat, sd = b.run(tuning, stream)
+    return b.get(), lat, sd
+
+
+def batch_autotune(ctx: abi.Context, items, stream=None):
+    """chk_batch_autotune: the fastest tuning (dict) and its batch time at spp / 4."""
+    arr, images = _requests(items)
+    T = abi.chk_batch_tuning()
+    ms = C.c_double(0)
+    abi.check(ctx.L.chk_batch_autotune(ctx.h, arr, len(items), C.byref(T), C.byref(ms), stream),
+              "chk_batch_autotune")
+    del images
+    return {k: getattr(T, k) for k, _ in abi.chk_batch_tuning._fields_}, ms.value
+
+
 def to_display(x):
     """toDisplayValue (main.cpp:27): int(pow(clamp(x,0,1), 1/2.2)*255 + .5)."""
     return (np.power(np.clip(x, 0.0, 1.0), 1.0 / 2.2) * 255 + 0.5).astype(np.int64)

@@ -87,6 +87,9 @@ typedef struct chk_render_stats {
   double shadow_ms;
   double shade_ms;
   double other_ms;
+  double proposal_ms;       /* proposal inputs, inference and training */
+  uint32_t train_steps;     /* Adam steps taken by chk_render_learned */
+  uint32_t reserved;
 } chk_render_stats;
 
 /* Time every kernel launch with HIP events into chk_render_stats. */
@@ -196,6 +199,55 @@ chk_status chk_render_count(chk_scene_t scene, const chk_camera *camera,
                             const chk_render_params *params, double *accum,
                             chk_render_stats *stats, void *stream);
 
+/* ---- batched requests ----------------------------------------------------
+ * No single reference counterpart: the reference renders one image per
+ * process (main.cpp:62-94).  Many render requests -- each with its own
+ * scene, camera, parameters (CHK_MIS_BALANCE) and accumulators -- run in one
+ * wavefront: requests are ordered by cost (W * H * spp, shortest first) and
+ * packed into chunks of tuning->chunk_paths paths; a chunk mixes requests
+ * and scenes.  Each request's image equals its chk_render image exactly
+ * (same keys, same per-path arithmetic, same per-pixel summation order). */
+typedef struct chk_request {
+  chk_scene_t scene;
+  chk_camera camera;
+  chk_render_params params;
+  double *accum;            /* W*H*3, dev: += per-pixel sums */
+  double *sumsq;            /* dev or NULL; used with CHK_RENDER_SUMSQ */
+} chk_request;
+
+typedef struct chk_batch_tuning {
+  uint32_t chunk_paths;     /* paths in flight per chunk, 2^16..2^24 (0: 2^22) */
+  uint32_t trace_block;     /* traversal threads per block: 64, 128, 256 (0: 128) */
+  uint32_t shade_block;     /* shading threads per block: 128, 256, 512 (0: 256) */
+  uint32_t use_graph;       /* capture each chunk into a hipGraph; the executable
+                               graphs are reused by the next identical call */
+} chk_batch_tuning;
+
+typedef struct chk_batch_stats {
+  uint64_t paths;
+  uint64_t rays_extend;
+  uint64_t rays_shadow;
+  uint32_t chunks;
+  uint32_t segments;        /* (request, pixel range) pieces */
+  double batch_ms;          /* device time from the first launch to the last */
+  double capture_ms;        /* host time spent capturing graphs (0 if reused) */
+  uint32_t graph_cached;    /* 1: the call replayed graphs of the previous call */
+  uint32_t reserved;
+} chk_batch_stats;
+
+/* Render n requests.  tuning NULL: defaults.  latency_ms (host, n, may be
+ * NULL): per request, device time from the batch start to the end of the
+ * chunk that finished it.  Synchronises the stream. */
+chk_status chk_render_batch(chk_ctx_t ctx, const chk_request *reqs, uint32_t n,
+                            const chk_batch_tuning *tuning, double *latency_ms,
+                            chk_batch_stats *stats, void *stream);
+/* Autotuner: times every chunk_paths x trace_block x shade_block candidate on
+ * the batch (spp / 4, scratch accumulators; the requests' own images are not
+ * touched), then eager launches against graph replay for the fastest.
+ * best_ms (may be NULL): its batch time at spp / 4. */
+chk_status chk_batch_autotune(chk_ctx_t ctx, const chk_request *reqs, uint32_t n,
+                              chk_batch_tuning *best, double *best_ms, void *stream);
+
 /* ---- learned light-selection proposal -----------------------------------
  * No reference counterpart (the reference gives every mesh light the same
  * sample count, mis.h:9, and the balance heuristic, mis.cpp:209-220).  A
@@ -224,6 +276,36 @@ chk_status chk_proposal_grad(chk_proposal_t p, const float *inputs, const float 
                              uint32_t k, float *grad, void *stream);
 /* One Adam step (beta 0.9 / 0.999, eps 1e-8) with learning rate lr. */
 chk_status chk_proposal_adam(chk_proposal_t p, const float *grad, float lr, void *stream);
+
+/* Online training inside chk_render_learned. */
+typedef struct chk_train_params {
+  uint32_t train_spp;        /* calls with params->sample_offset < train_spp train */
+  uint32_t batch;            /* records per Adam step, 16..65536 */
+  uint32_t steps_per_bounce; /* Adam steps after each bounce while training */
+  float lr;                  /* Adam learning rate */
+  uint32_t flags;            /* CHK_TRAIN_* */
+} chk_train_params;
+#define CHK_TRAIN_FREEZE_MIX 1u    /* no gradient into the mixing logit */
+#define CHK_TRAIN_FREEZE_LIGHTS 2u /* no gradient into the light logits */
+
+/* chk_render with the learned proposal: at every vertex the network (for
+ * the scene's n_lights lights) gives q' and c; the light of next-event
+ * estimation is drawn from q', and the light and BSDF samples are combined
+ * with weights c p_l / (c p_l + (1-c) p_b) and its complement, which keeps
+ * the estimator unbiased for any network output.  params->mis_mode must be
+ * CHK_MIS_LEARNED.  With train != NULL and params->sample_offset <
+ * train->train_spp the network is trained online on the render's own
+ * samples: after each bounce, train->steps_per_bounce Adam steps on the
+ * previous bounce's records (light choice, its shadowed contribution, and
+ * the emission found by the BSDF sample), minimising the KL divergence from
+ * the light-contribution distribution to q' and the second moment of the
+ * c-mixture.  An untrained proposal (zero output layer, q' = 1/L, c = 1/2)
+ * is the estimator of chk_render; the images differ only where the float
+ * CDF of q' puts a light-selection boundary one ulp away from floor(u L). */
+chk_status chk_render_learned(chk_scene_t scene, const chk_camera *camera,
+                              const chk_render_params *params, chk_proposal_t proposal,
+                              const chk_train_params *train, double *accum, double *sumsq,
+                              chk_render_stats *stats, void *stream);
 
 #ifdef __cplusplus
 }
