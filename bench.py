@@ -40,10 +40,15 @@ def parse():
     ap.add_argument("--depth", type=int, default=6)
     ap.add_argument("--freq", type=int, default=71, help="geodesic frequency (71 -> 100,820 triangles)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device to device 4 times (the known-size kernel that "
+                         "scales the FETCH_SIZE / WRITE_SIZE counters), then exit")
     ap.add_argument("--cpu-row-step", type=int, default=1)
     ap.add_argument("--cpu-spp", type=int, default=64)
-    ap.add_argument("--traffic", type=str, default=None,
-                    help="JSON file with measured HBM bytes per closest-hit launch")
+    ap.add_argument("--traffic", type=str,
+                    default=str(ROOT / "profiles" / "closest_hit_traffic_icosphere100k_1080p.json"),
+                    help="JSON with the measured HBM bytes per closest-hit launch (from the "
+                         "FETCH_SIZE / WRITE_SIZE runs recorded in that file)")
     # time-to-RMSE leg: learned proposal vs plain MIS (config 5 scene)
     ap.add_argument("--no-rmse", action="store_true")
     ap.add_argument("--rmse-width", type=int, default=480)
@@ -57,9 +62,120 @@ def parse():
     ap.add_argument("--rmse-target", type=float, default=1e-2)
     ap.add_argument("--train-spp", type=int, default=32)
     ap.add_argument("--train-batch", type=int, default=16384)
-    ap.add_argument("--train-steps", type=int, default=4, help="Adam steps per bounce")
+    ap.add_argument("--train-steps", type=int, default=64, help="Adam steps per bounce")
     ap.add_argument("--train-lr", type=float, default=1e-3)
+    # batching leg (config 4) and LTC leg (config 3)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-requests", type=int, default=64)
+    ap.add_argument("--batch-scenes", type=int, default=16)
+    ap.add_argument("--batch-reps", type=int, default=3)
+    ap.add_argument("--no-ltc", action="store_true")
+    ap.add_argument("--ltc-points", type=int, default=4 * 1024 * 1024)
+    ap.add_argument("--ltc-quads", type=int, default=16)
     return ap.parse_args()
+
+
+def batch_leg(args, ctx):
+    """Config 4: requests of log-uniform size 64..1024 per side, spp from
+    {1, 4, 16, 64, 256}, scenes from the Cornell generator with log-uniform
+    1k..500k triangles and 1..16 lights (a pool of batch_scenes scenes the
+    requests draw from).  Autotune, then time the batch."""
+    import dataclasses
+
+    import numpy as np
+
+    from chakumrender_amd import abi, host, scenes
+
+    rng = np.random.default_rng(0x5EED0004)
+    t0 = time.perf_counter()
+    pool = []
+    for k in range(args.batch_scenes):
+        tris = int(np.exp(rng.uniform(np.log(1e3), np.log(5e5))))
+        nl = int(rng.integers(1, 17))
+        sc = scenes.cornell(seed=0x5EED0400 + k, sphere_tris=max(tris // 2, 8), n_lights=nl,
+                            light_rng=nl > 1)
+        pool.append(abi.DeviceScene(ctx, sc))
+    items = []
+    for i in range(args.batch_requests):
+        w = int(np.exp(rng.uniform(np.log(64), np.log(1024))))
+        h = int(np.exp(rng.uniform(np.log(64), np.log(1024))))
+        spp = int(rng.choice([1, 4, 16, 64, 256]))
+        ds = pool[int(rng.integers(0, len(pool)))]
+        cam = dataclasses.replace(ds.scene.camera, width=w, height=h)
+        items.append(dict(ds=ds, camera=cam, params=dict(spp=spp, max_depth=4, seed=0x5EED0400 + i)))
+    setup_s = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    best, tune_ms = host.batch_autotune(ctx, items)
+    tune_s = time.perf_counter() - t1
+    b = host.Batch(ctx, items)
+    b.run(best)  # warm-up (and graph capture when chosen)
+    abi.sync()
+    lats, runs = [], []
+    t2 = time.perf_counter()
+    for _ in range(args.batch_reps):
+        lat, st = b.run(best)
+        lats.append(lat)
+        runs.append(st)
+    wall = (time.perf_counter() - t2) / args.batch_reps
+    paths = runs[-1]["paths"]
+    lat = np.array(lats[-1])
+    dev_ms = float(np.mean([r["batch_ms"] for r in runs]))
+    out = {
+        "workload": f"{args.batch_requests} requests, sizes 64..1024 log-uniform, spp in "
+                    f"{{1,4,16,64,256}}, depth 4, {args.batch_scenes} Cornell-generator scenes "
+                    f"1k..500k triangles, 1..16 lights",
+        "paths_per_batch": int(paths),
+        "mpaths_per_s": round(paths / (dev_ms / 1e3) / 1e6, 2),
+        "wall_mpaths_per_s": round(paths / wall / 1e6, 2),
+        "batch_ms": round(dev_ms, 3),
+        "latency_ms_p50": round(float(np.percentile(lat, 50)), 3),
+        "latency_ms_p99": round(float(np.percentile(lat, 99)), 3),
+        "tuning": best,
+        "autotune_s": round(tune_s, 2),
+        "chunks": runs[-1]["chunks"],
+        "segments": runs[-1]["segments"],
+        "graph_cached": runs[-1]["graph_cached"],
+        "setup_s": round(setup_s, 2),
+    }
+    for ds in pool:
+        ds.close()
+    return out
+
+
+def ltc_leg(args, ctx):
+    """Config 3: chk_ltc_eval on ltc_points points x ltc_quads quads
+    (ks = 1, the LTC part), FP64 arithmetic; pairs/s and the FP64 VALU
+    roofline at SURVEY's 180 FLOP per (point, quad) pair."""
+    import numpy as np
+
+    from chakumrender_amd import abi, host, scenes
+
+    P = scenes.ltc_points(args.ltc_points, n_quads=args.ltc_quads)
+    tris, rad = scenes.quads_to_tris(P["quads"], P["radiance"])
+    ls = host.LightSource(ctx, tris, rad)
+    k = args.ltc_points
+    bufs = [abi.DevBuf.from_array(np.ascontiguousarray(a, np.float32)) for a in
+            (P["x"], P["n"], P["v"], P["alpha"], np.ones(k, np.float32))]
+    rgb = abi.DevBuf(12 * k)
+    ls.eval_dev(bufs, k, rgb)
+    abi.sync()
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ls.eval_dev(bufs, k, rgb)
+    abi.sync()
+    dt = (time.perf_counter() - t0) / reps
+    pairs = k * args.ltc_quads
+    flop = 180.0 * pairs
+    return {
+        "workload": f"{k} points x {args.ltc_quads} quads ({2 * args.ltc_quads} triangles), "
+                    "GGX LTC part (ks = 1), FP64 arithmetic, FP32 in/out",
+        "ms": round(dt * 1e3, 3),
+        "gpairs_per_s": round(pairs / dt / 1e9, 3),
+        "roofline": {"bound": "fp64 valu", "achieved": round(flop / dt / 1e12, 3),
+                     "peak": 78.6, "unit": "TFLOP/s", "frac": round(flop / dt / 1e12 / 78.6, 4),
+                     "work_model": "180 FLOP per (point, quad) pair"},
+    }
 
 
 def progressive(render_pass, ref, pass_spp, max_spp, target, sync, W, H):
@@ -194,6 +310,15 @@ def main():
 
     from chakumrender_amd import abi, host, scenes
 
+    if args.calibrate_copy:
+        nb = args.calibrate_copy << 20
+        a, b = abi.DevBuf.zeros(nb), abi.DevBuf(nb)
+        for _ in range(4):
+            abi.check(abi.load().chk_memcpy(b.ptr, a.ptr, nb, 2, None), "chk_memcpy d2d")
+        abi.sync()
+        print(json.dumps({"copy_bytes_per_dispatch": nb, "dispatches": 4}))
+        return
+
     t0 = time.time()
     sc = scenes.icosphere_scene(freq=args.freq, width=args.width, height=args.height)
     ctx = abi.Context(0)
@@ -285,6 +410,10 @@ def main():
 
     if not args.no_rmse:
         line["time_to_rmse"] = rmse_leg(args, ctx)
+    if not args.no_batch:
+        line["batch"] = batch_leg(args, ctx)
+    if not args.no_ltc:
+        line["ltc"] = ltc_leg(args, ctx)
 
     if not args.no_cpu_baseline:
         import oracle

@@ -90,7 +90,7 @@ RENDER_SUMSQ = 0x2
 RENDER_PROFILE = 0x100
 JITTER_BOX, JITTER_REFERENCE = 0, 1
 MIS_BALANCE, MIS_LEARNED = 0, 1
-TRAIN_FREEZE_MIX, TRAIN_FREEZE_LIGHTS = 1, 2
+TRAIN_FREEZE_MIX, TRAIN_FREEZE_LIGHTS, TRAIN_LIGHTS_CHI2 = 1, 2, 4
 
 # Every exported symbol of include/chakum_c.h: name -> (restype, argtypes)
 _vp = C.c_void_p

@@ -66,7 +66,7 @@ struct TrainRec {
   float le;          // emitted radiance its ray found (0: none)
   float plb, pbb;    // light-sampling and BSDF pdfs of the BSDF sample
   uint32_t sh_slot;  // shadow-queue slot of the light sample (0xFFFFFFFF: none)
-  uint32_t pad;
+  uint32_t light_b;  // light the BSDF sample hit (0xFFFFFFFF: none)
 };
 static_assert(sizeof(TrainRec) == 40, "TrainRec is 40 bytes");
 

@@ -398,17 +398,20 @@ __global__ void k_mlp_pack(const float *__restrict__ P, uint16_t *__restrict__ F
 }
 
 // ------------------------------------------------------------ online training
-// One training row per selected record of a bounce.  Both heads minimise
-// the second moment of the vertex's direct-light estimate (its variance up
-// to a constant):
-//  * light selection: the NEE part's second moment sum_j E[v_j^2] / q'_j
-//    (v_j: contribution of light j per unit selection probability; optimum
-//    q'_j ~ sqrt(E[v_j^2])), score-function estimate from j ~ q' with
-//    a = g_l V / p_l:  dL/dz_k = -a^2 (1 - eps) (q_j / q'_j) (delta_jk - q_k);
-//  * mixing weight c: F = c g_l/p_c(w_l) + (1-c) g_b/p_c(w_b),
-//    p_c = c p_l + (1-c) p_b.  The sample directions do not depend on c, so
-//    2 F dF/dc on the record's own pair is an unbiased estimate of
-//    dE[F^2]/dc.
+// One training row per selected record of a finished bounce.  The records
+// were drawn with the proposal of their bounce (q'_s, c_s); the gradients are
+// taken at the current parameters (q'_c, c_c from a fresh forward pass over
+// the batch), with q'_s only in the importance weights, so every Adam step
+// sees where the network already is.
+//  * light selection, default: KL(p* || q'_c), p*_j proportional to light j's
+//    contribution: the sample j ~ q'_s with a = g_l V / p_l gives
+//    a grad(-log q'_c,j);  CHK_TRAIN_LIGHTS_CHI2: the second moment
+//    sum_j E[v_j^2] / q'_c,j of the NEE estimate, a^2 (q'_s,j / q'_c,j)
+//    grad(-log q'_c,j).  grad_z log q'_i = (1 - eps) q_i (e_i - q) / q'_i.
+//  * mixing weight c: F(c) = c g_l / p_c(w_l) + (1 - c) g_b / p_c(w_b),
+//    p_c = c p_l + (1 - c) p_b, is unbiased for any c (the directions do not
+//    depend on c), so 2 F dF/dc at c = c_c on the record's own pair is an
+//    unbiased estimate of dE[F^2]/dc.
 // Each head's rows are divided by the head's batch-mean magnitude
 // (k_train_norm).
 struct TrainArgs {
@@ -431,7 +434,20 @@ __device__ inline bool pick_record(const SegRange &R, uint32_t v, size_t &idx) {
   return false;
 }
 
-__global__ __launch_bounds__(256) void k_train_rows(TrainArgs T, float *__restrict__ bin,
+// Rows of the batch: record index (0xFFFFFFFF: none) and network input.
+__global__ __launch_bounds__(256) void k_train_gather(TrainArgs T, float *__restrict__ bin,
+                                                      uint32_t *__restrict__ ridx) {
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= T.batch) return;
+  size_t idx;
+  const bool ok = pick_record(T.R, b * T.stride + T.offset, idx) && T.rec[idx].light < T.n_lights;
+  ridx[b] = ok ? (uint32_t)idx : 0xFFFFFFFFu;
+  for (int i = 0; i < 7; ++i) bin[7 * (size_t)b + i] = ok ? T.inputs[7 * idx + i] : 0.0f;
+}
+
+__global__ __launch_bounds__(256) void k_train_rows(TrainArgs T, const uint32_t *__restrict__ ridx,
+                                                    const float *__restrict__ cur_probs,
+                                                    const float *__restrict__ cur_mix,
                                                     float *__restrict__ dl,
                                                     float *__restrict__ wa,
                                                     float *__restrict__ wh) {
@@ -439,34 +455,36 @@ __global__ __launch_bounds__(256) void k_train_rows(TrainArgs T, float *__restri
   if (b >= T.batch) return;
   float *row = dl + 33 * (size_t)b;
   for (int k = 0; k < 33; ++k) row[k] = 0.0f;
-  for (int i = 0; i < 7; ++i) bin[7 * (size_t)b + i] = 0.0f;
   wa[b] = 0.0f;
   wh[b] = 0.0f;
-  size_t idx;
-  if (!pick_record(T.R, b * T.stride + T.offset, idx)) return;
+  const uint32_t idx = ridx[b];
+  if (idx == 0xFFFFFFFFu) return;
   const TrainRec R = T.rec[idx];
-  if (R.light >= T.n_lights) return;
-  for (int i = 0; i < 7; ++i) bin[7 * (size_t)b + i] = T.inputs[7 * idx + i];
-  const float *qp = T.probs + 32 * idx;
-  const double c = T.mix[idx];
+  const float *qs = T.probs + 32 * (size_t)idx;  // proposal the samples were drawn with
+  const float *qc = cur_probs + 32 * (size_t)b;  // current proposal
+  const double c = cur_mix[b];
   const double eq = kPropEpsQ, ec = kPropEpsC, invL = 1.0 / T.n_lights;
-  // light part
   const double vis = (R.sh_slot != 0xFFFFFFFFu && !T.s_occ[R.sh_slot]) ? 1.0 : 0.0;
   const double gl = (double)R.gl * vis;
-  const double a = R.pl > 0.0f ? gl / (double)R.pl : 0.0;
-  if (a > 0.0 && isfinite(a * a) && !(T.flags & CHK_TRAIN_FREEZE_LIGHTS)) {
-    const double qpj = qp[R.light];
-    const double qj = (qpj - eq * invL) / (1.0 - eq);
-    const double s = -a * a * (1.0 - eq) * qj / qpj;
-    for (uint32_t k = 0; k < T.n_lights; ++k) {
-      const double qk = ((double)qp[k] - eq * invL) / (1.0 - eq);
-      row[k] = (float)(s * ((k == R.light ? 1.0 : 0.0) - qk));
+  const double pl = R.pl, pbl = R.pbl, gb = (double)R.fb * R.le, plb = R.plb, pbb = R.pbb;
+  // light part: w grad(-log q'_c,j), added as -w (1 - eps) (q_j/q'_j) (e_j - q) at q = q_c
+  if (!(T.flags & CHK_TRAIN_FREEZE_LIGHTS)) {
+    const double a = pl > 0.0 ? gl / pl : 0.0;
+    const uint32_t j = R.light;
+    const double qpc = qc[j];
+    const double w = (T.flags & CHK_TRAIN_LIGHTS_CHI2) ? a * a * (double)qs[j] / qpc : a;
+    if (w > 0.0 && isfinite(w)) {
+      const double qj = (qpc - eq * invL) / (1.0 - eq);
+      const double s = -w * (1.0 - eq) * qj / qpc;
+      for (uint32_t k = 0; k < T.n_lights; ++k) {
+        const double qk = ((double)qc[k] - eq * invL) / (1.0 - eq);
+        row[k] = (float)(s * ((k == j ? 1.0 : 0.0) - qk));
+      }
+      wa[b] = (float)w;
     }
-    wa[b] = (float)(a * a);
   }
-  // mixing part: 2 F dF/dc
+  // mixing part: 2 F dF/dc at the current c
   if (!(T.flags & CHK_TRAIN_FREEZE_MIX)) {
-    const double pl = R.pl, pbl = R.pbl, gb = (double)R.fb * R.le, plb = R.plb, pbb = R.pbb;
     const double pcl = c * pl + (1.0 - c) * pbl, pcb = c * plb + (1.0 - c) * pbb;
     const double al = (gl > 0.0 && pcl > 0.0) ? gl / pcl : 0.0;
     const double ab = (gb > 0.0 && pcb > 0.0) ? gb / pcb : 0.0;
@@ -633,11 +651,11 @@ chk_status proposal_train_step(chk_proposal_s *p, const SegRange &R, const Train
                                const float *inputs, const float *probs, const float *mix,
                                const uint8_t *s_occ, uint32_t batch, uint32_t stride,
                                uint32_t offset, float lr, uint32_t flags, hipStream_t st) {
-  const size_t b_in = ((size_t)batch * 7 * 4 + 255) & ~(size_t)255;
-  const size_t b_dl = ((size_t)batch * 33 * 4 + 255) & ~(size_t)255;
-  const size_t b_w = ((size_t)batch * 4 + 255) & ~(size_t)255;
-  const size_t b_g = ((size_t)kMlpParams * 4 + 255) & ~(size_t)255;
-  const size_t need = b_in + b_dl + 2 * b_w + 256 + b_g;
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t b_in = al((size_t)batch * 7 * 4), b_dl = al((size_t)batch * 33 * 4);
+  const size_t b_w = al((size_t)batch * 4), b_pr = al((size_t)batch * 32 * 4);
+  const size_t b_g = al((size_t)kMlpParams * 4);
+  const size_t need = b_in + b_dl + 4 * b_w + b_pr + 256 + b_g;
   if (p->train_bytes < need) {
     if (p->train) (void)hipFree(p->train);
     p->train = nullptr;
@@ -650,11 +668,22 @@ chk_status proposal_train_step(chk_proposal_s *p, const SegRange &R, const Train
   }
   char *q = (char *)p->train;
   float *bin = (float *)q;
-  float *dl = (float *)(q + b_in);
-  float *wa = (float *)(q + b_in + b_dl);
-  float *wh = (float *)(q + b_in + b_dl + b_w);
-  float *scale = (float *)(q + b_in + b_dl + 2 * b_w);
-  float *grad = (float *)(q + b_in + b_dl + 2 * b_w + 256);
+  q += b_in;
+  float *dl = (float *)q;
+  q += b_dl;
+  float *wa = (float *)q;
+  q += b_w;
+  float *wh = (float *)q;
+  q += b_w;
+  uint32_t *ridx = (uint32_t *)q;
+  q += b_w;
+  float *cmix = (float *)q;
+  q += b_w;
+  float *cprobs = (float *)q;
+  q += b_pr;
+  float *scale = (float *)q;
+  q += 256;
+  float *grad = (float *)q;
   TrainArgs T;
   T.R = R;
   T.rec = rec;
@@ -668,7 +697,9 @@ chk_status proposal_train_step(chk_proposal_s *p, const SegRange &R, const Train
   T.offset = offset;
   T.flags = flags;
   const dim3 g((batch + 255) / 256);
-  hipLaunchKernelGGL(k_train_rows, g, dim3(256), 0, st, T, bin, dl, wa, wh);
+  hipLaunchKernelGGL(k_train_gather, g, dim3(256), 0, st, T, bin, ridx);
+  CHK_HIP(launch_mlp_eval(p, bin, linear_range(batch), cprobs, cmix, nullptr, st));
+  hipLaunchKernelGGL(k_train_rows, g, dim3(256), 0, st, T, ridx, cprobs, cmix, dl, wa, wh);
   hipLaunchKernelGGL(k_train_norm, dim3(1), dim3(1024), 0, st, wa, wh, batch, scale);
   hipLaunchKernelGGL(k_train_scale, g, dim3(256), 0, st, dl, batch, p->n_lights, scale);
   CHK_HIP(hipGetLastError());

@@ -324,6 +324,7 @@ __device__ __forceinline__ void shade_vertex(const RenderArgs &A, uint32_t depth
             w = (1.0f - c) * pb / (c * pl + (1.0f - c) * pb);
             P.rec_prev[prev].le = lum(m.rad);
             P.rec_prev[prev].plb = pl;
+            P.rec_prev[prev].light_b = info.y;
           } else {
             const float pl = A.inv_nl * pa;
             w = pb / (pb + pl);  // balance heuristic, N_light = N_bsdf = 1 (mis.cpp:209-220)
@@ -474,7 +475,7 @@ __global__ __launch_bounds__(BS) void k_shade(
   rec.light = 0xFFFFFFFFu;
   rec.gl = rec.pl = rec.pbl = rec.fb = rec.le = rec.plb = rec.pbb = 0.0f;
   rec.sh_slot = 0xFFFFFFFFu;
-  rec.pad = 0u;
+  rec.light_b = 0xFFFFFFFFu;
   if (off < total) {
     if (MULTI) {
       seg = Q.in_seg[i];
@@ -705,7 +706,7 @@ chk_status render_impl(chk_scene_s *sc, const chk_camera *cam, const chk_render_
     if (train && P->sample_offset < train->train_spp) {
       if (train->batch < 16 || train->batch > (1u << 16) || train->steps_per_bounce == 0 ||
           !(train->lr > 0.0f) ||
-          (train->flags & ~(CHK_TRAIN_FREEZE_MIX | CHK_TRAIN_FREEZE_LIGHTS))) {
+          (train->flags & ~(CHK_TRAIN_FREEZE_MIX | CHK_TRAIN_FREEZE_LIGHTS | CHK_TRAIN_LIGHTS_CHI2))) {
         set_error("chk_render_learned: train batch must be 16..65536, steps >= 1, lr > 0, "
                   "known flags");
         return CHK_EINVAL;
@@ -892,7 +893,10 @@ chk_status render_impl(chk_scene_s *sc, const chk_camera *cam, const chk_render_
 // kernels find the ray's scene and the shading kernel the request's
 // arguments.  Per chunk the sequence is the single-request one; with
 // use_graph each chunk's launches are captured into a hipGraph once and
-// replayed (the executable graphs are kept for the next identical call).
+// replayed; the executable graphs are kept until a call with other
+// requests captures again or the buffers they name are reallocated.  Eager
+// calls in between leave them valid: every call re-uploads its own tables
+// before it launches anything.
 
 namespace {
 
@@ -924,8 +928,9 @@ struct BatchState {
   }
 };
 
-chk_status ensure(void *&ptr, size_t &have, size_t need, const char *what) {
+chk_status ensure(void *&ptr, size_t &have, size_t need, const char *what, bool &moved) {
   if (have >= need) return CHK_OK;
+  moved = true;
   if (ptr) (void)hipFree(ptr);
   ptr = nullptr;
   have = 0;
@@ -1049,14 +1054,16 @@ chk_status batch_impl(chk_ctx_s *ctx, const chk_request *R, uint32_t n,
   std::vector<unsigned char> key(sizeof(chk_request) * n + sizeof(T));
   std::memcpy(key.data(), R, sizeof(chk_request) * n);
   std::memcpy(key.data() + sizeof(chk_request) * n, &T, sizeof(T));
-  const bool cached = T.use_graph && bs->key == key && !bs->execs.empty() &&
-                      bs->work_bytes >= need;
-  if (!cached) bs->drop_graphs();
-  chk_status cs = ensure(bs->work, bs->work_bytes, need, "the path queues");
+  // captured graphs hold the buffer addresses: any reallocation drops them
+  bool moved = false;
+  chk_status cs = ensure(bs->work, bs->work_bytes, need, "the path queues", moved);
   if (cs != CHK_OK) return cs;
   const size_t tab_need = segs.size() * (sizeof(RenderArgs) + sizeof(DeviceScene)) + 512;
-  cs = ensure(bs->tab, bs->tab_bytes, tab_need, "the request tables");
+  cs = ensure(bs->tab, bs->tab_bytes, tab_need, "the request tables", moved);
   if (cs != CHK_OK) return cs;
+  if (moved) bs->drop_graphs();
+  const bool cached = T.use_graph && bs->key == key && !bs->execs.empty();
+  if (T.use_graph && !cached) bs->drop_graphs();
   Buffers B = carve(bs->work, qcap, cap);
   uint32_t *segbuf = (uint32_t *)((char *)bs->work + ((buffers_bytes(qcap, cap) + 255) & ~(size_t)255));
   uint32_t *q_seg[2] = {segbuf, segbuf + qcap};
@@ -1195,7 +1202,6 @@ chk_status batch_impl(chk_ctx_s *ctx, const chk_request *R, uint32_t n,
     stats->graph_cached = cached ? 1u : 0u;
   }
   free_events();
-  if (!T.use_graph) bs->drop_graphs();
   return CHK_OK;
 }
 

@@ -259,7 +259,7 @@ def render_learned(ds: abi.DeviceScene, params, proposal: "Proposal", train=None
         T = abi.chk_train_params()
         T.train_spp = int(train.get("train_spp", 32))
         T.batch = int(train.get("batch", 16384))
-        T.steps_per_bounce = int(train.get("steps_per_bounce", 2))
+        T.steps_per_bounce = int(train.get("steps_per_bounce", 16))
         T.lr = float(train.get("lr", 1e-3))
         T.flags = int(train.get("flags", 0))
     st = abi.chk_render_stats()

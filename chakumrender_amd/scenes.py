@@ -128,15 +128,17 @@ def uv_sphere(center, radius, slices, stacks):
     ph = np.linspace(0, 2 * math.pi, slices + 1)
     P = np.stack([np.outer(np.sin(th), np.cos(ph)), np.outer(np.cos(th), np.ones_like(ph)),
                   np.outer(np.sin(th), np.sin(ph))], -1) * radius + center
-    out = []
-    for i in range(stacks):
-        for j in range(slices):
-            a, b, c, d = P[i, j], P[i, j + 1], P[i + 1, j + 1], P[i + 1, j]
-            if i != 0:
-                out.append(np.r_[a, b, c])
-            if i != stacks - 1:
-                out.append(np.r_[a, c, d])
-    return np.array(out)
+    # per quad (i, j): triangle (a, b, c) unless i is the top stack, then
+    # (a, c, d) unless i is the bottom stack; quads in row-major order
+    a, b = P[:-1, :-1], P[:-1, 1:]
+    c, d = P[1:, 1:], P[1:, :-1]
+    t1 = np.concatenate([a, b, c], -1)  # (stacks, slices, 9)
+    t2 = np.concatenate([a, c, d], -1)
+    both = np.stack([t1, t2], 2)  # (stacks, slices, 2, 9)
+    keep = np.ones((stacks, slices, 2), bool)
+    keep[0, :, 0] = False
+    keep[stacks - 1, :, 1] = False
+    return both[keep]
 
 
 def _dummy_triangle():

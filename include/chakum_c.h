@@ -287,6 +287,9 @@ typedef struct chk_train_params {
 } chk_train_params;
 #define CHK_TRAIN_FREEZE_MIX 1u    /* no gradient into the mixing logit */
 #define CHK_TRAIN_FREEZE_LIGHTS 2u /* no gradient into the light logits */
+#define CHK_TRAIN_LIGHTS_CHI2 4u   /* light head: second moment of the NEE
+                                      estimate instead of the KL divergence to
+                                      the light-contribution distribution */
 
 /* chk_render with the learned proposal: at every vertex the network (for
  * the scene's n_lights lights) gives q' and c; the light of next-event
@@ -299,7 +302,7 @@ typedef struct chk_train_params {
  * previous bounce's records (light choice, its shadowed contribution, and
  * the emission found by the BSDF sample), minimising the KL divergence from
  * the light-contribution distribution to q' and the second moment of the
- * c-mixture.  An untrained proposal (zero output layer, q' = 1/L, c = 1/2)
+ * vertex's direct-light estimate over c.  An untrained proposal (zero output layer, q' = 1/L, c = 1/2)
  * is the estimator of chk_render; the images differ only where the float
  * CDF of q' puts a light-selection boundary one ulp away from floor(u L). */
 chk_status chk_render_learned(chk_scene_t scene, const chk_camera *camera,

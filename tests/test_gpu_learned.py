@@ -48,7 +48,7 @@ def test_training_is_unbiased_and_lowers_variance(ctx):
     nl = ds.info()["n_lights"]
     prop = host.Proposal(ctx, nl, seed=2)
     p0 = prop.get_params()
-    train = dict(train_spp=32, batch=4096, steps_per_bounce=4, lr=2e-3)
+    train = dict(train_spp=32, batch=4096, steps_per_bounce=32, lr=1e-3)
     # training pass: samples 0..31 of every pixel
     _, _, st = host.render_learned(ds, dict(spp=32, max_depth=4, seed=0x5EED), prop, train=train)
     p1 = prop.get_params()
@@ -70,7 +70,7 @@ def test_training_is_unbiased_and_lowers_variance(ctx):
           f"variance ratio learned/plain {vl / vp:.3f}")
     assert frac >= 0.99
     assert abs(ml.mean() - mp.mean()) / mp.mean() < 0.01
-    assert vl / vp < 0.8
+    assert vl / vp < 0.7
     prop.close()
 
 
