@@ -114,6 +114,7 @@ SIGNATURES = {
     "chk_scene_get_info": (C.c_int, [_vp, C.POINTER(chk_scene_info)]),
     "chk_trace_closest": (C.c_int, [_vp, _vp, _u32, _vp, _vp]),
     "chk_trace_any": (C.c_int, [_vp, _vp, _u32, _vp, _vp]),
+    "chk_trace_spheres": (C.c_int, [_vp, _u32, _vp, _u32, _vp, _vp]),
     "chk_brdf_eval": (C.c_int, [_vp, _u32, _vp, _vp, _vp, _vp, _vp, _u32, _u32, _vp, _vp, _vp]),
     "chk_brdf_sample": (C.c_int, [_vp, _u32, _vp, _vp, _vp, _vp, _vp, _u32, _u32, _vp, _vp, _vp]),
     "chk_ltc_eval": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _u32, _vp, _vp, _u32, _vp, _vp]),

@@ -474,6 +474,20 @@ chk_status chk_trace_any(chk_scene_t sc, const chk_ray *rays, uint32_t n, uint8_
   CHK_TRY_END
 }
 
+chk_status chk_trace_spheres(const float *spheres, uint32_t n_spheres, const chk_ray *rays,
+                             uint32_t n, chk_hit *hits, void *stream) {
+  CHK_TRY_BEGIN
+  if ((n && (!rays || !hits)) || (n_spheres && !spheres)) {
+    set_error("chk_trace_spheres: null argument");
+    return CHK_EINVAL;
+  }
+  if (n == 0) return CHK_OK;
+  CHK_HIP(launch_spheres(spheres, n_spheres, (const float4 *)rays, n, (float4 *)hits,
+                         (hipStream_t)stream));
+  return CHK_OK;
+  CHK_TRY_END
+}
+
 chk_status chk_brdf_eval(const chk_material *mats, uint32_t n_mats, const uint32_t *mat_idx,
                          const double *n, const double *wo_ref, const double *wo,
                          const double *wi, uint32_t k, uint32_t flags, double *f, double *pdf,

@@ -127,6 +127,9 @@ hipError_t launch_any(const chk_scene_s *s, const TraceShape &T, const float4 *r
                       float4 *path_L, bool leaf_ids, bool count, unsigned long long *counters,
                       const SlotScenes &M, hipStream_t st);
 hipError_t launch_zero_u32(uint32_t *p, uint32_t n, hipStream_t st);
+// Closest sphere per ray (spheres: ns x 4 floats, centre and radius).
+hipError_t launch_spheres(const float *spheres, uint32_t ns, const float4 *rays, uint32_t n,
+                          float4 *hits, hipStream_t st);
 
 // Learned proposal (chk_mlp.hip).  Inference over the items of R: inputs
 // (7 per item), probs (32 per item), mix (1 per item), logits (33, may be null).

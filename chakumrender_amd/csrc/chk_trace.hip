@@ -210,6 +210,41 @@ __global__ __launch_bounds__(BS) void k_trace(DeviceScene S0, const float4 *__re
   }
 }
 
+// Closest sphere per ray: Sphere::intersect (geometryPrimitives.cpp:12-27,
+// eps 1e-4) over every sphere in FP64, with the strict d < t of
+// BvhAccel::intersectS (bvhAccel.cpp:171) and the ray's tmax as the initial
+// t.  The sphere list is wave-uniform (scalar loads); spheres in the
+// configs are a handful, so a brute-force loop beats a tree.
+__global__ __launch_bounds__(256) void k_spheres(const float4 *__restrict__ sph, uint32_t ns,
+                                                 const float4 *__restrict__ rays, uint32_t n,
+                                                 float4 *__restrict__ hits) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 ot = rays[2 * (size_t)i], ds = rays[2 * (size_t)i + 1];
+  const double ox = ot.x, oy = ot.y, oz = ot.z, dx = ds.x, dy = ds.y, dz = ds.z;
+  const double eps = 1e-4;  // geometryPrimitives.h:39
+  double best = ot.w;
+  uint32_t id = 0xFFFFFFFFu;
+  for (uint32_t k = 0; k < ns; ++k) {
+    const float4 c = sph[k];
+    const double px = c.x - ox, py = c.y - oy, pz = c.z - oz;
+    const double b = px * dx + py * dy + pz * dz;
+    double det = b * b - (px * px + py * py + pz * pz) + (double)c.w * c.w;
+    if (det < 0.0) continue;
+    det = sqrt(det);
+    double t = b - det;
+    if (!(t > eps)) {
+      t = b + det;
+      if (!(t > eps)) continue;
+    }
+    if (t < best) {
+      best = t;
+      id = k;
+    }
+  }
+  hits[i] = make_float4(id == 0xFFFFFFFFu ? 1e30f : (float)best, __uint_as_float(id), 0.0f, 0.0f);
+}
+
 __global__ void k_zero_u32(uint32_t *p, uint32_t n) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0u;
@@ -295,6 +330,14 @@ hipError_t launch_any(const chk_scene_s *s, const TraceShape &T, const float4 *r
                       const SlotScenes &M, hipStream_t st) {
   return launch_kind<true>(s, T, rays, W, nullptr, occluded, shadow_L, path_L, leaf_ids, count,
                            counters, M, st);
+}
+
+hipError_t launch_spheres(const float *spheres, uint32_t ns, const float4 *rays, uint32_t n,
+                          float4 *hits, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_spheres, dim3((n + 255) / 256), dim3(256), 0, st,
+                     reinterpret_cast<const float4 *>(spheres), ns, rays, n, hits);
+  return hipGetLastError();
 }
 
 hipError_t launch_zero_u32(uint32_t *p, uint32_t n, hipStream_t st) {

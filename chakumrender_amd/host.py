@@ -84,6 +84,18 @@ class BvhAccel:
         return 1.0 - occ.astype(np.float64)
 
 
+def intersect_spheres(spheres, rays):
+    """BvhAccel::intersectS over a sphere list (n x 4: centre, radius) for
+    chk_ray records: (n, 4) hit records (t, sphere id bits, 0, 0)."""
+    L = abi.load()
+    sp = np.ascontiguousarray(spheres, np.float32).reshape(-1, 4)
+    r = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+    n = r.shape[0]
+    sd, rd, hd = DevBuf.from_array(sp), DevBuf.from_array(r), DevBuf(16 * max(n, 1))
+    abi.check(L.chk_trace_spheres(sd.ptr, sp.shape[0], rd.ptr, n, hd.ptr, None), "chk_trace_spheres")
+    return hd.download((n, 4), np.float32)
+
+
 class MaterialType:
     """Batched MaterialType (materialTypes.h:20-120) over a material table.
     FP64 directions in and FP64 results out, like the reference's Vec

@@ -148,6 +148,16 @@ chk_status chk_trace_closest(chk_scene_t scene, const chk_ray *rays, uint32_t n,
 chk_status chk_trace_any(chk_scene_t scene, const chk_ray *rays, uint32_t n, uint8_t *occluded,
                          void *stream);
 
+/* Closest sphere for n rays: Sphere::intersect (geometryPrimitives.cpp:12-27:
+ * nearest root beyond eps = 1e-4) in FP64 over the n_spheres spheres
+ * (n_spheres x 4 floats: centre, radius), keeping a hit only when it is
+ * strictly nearer than the ray's tmax and every earlier hit.  Replaces
+ * BvhAccel::intersectS (bvhAccel.cpp:151-198; pass tmax = INFINITY for its
+ * semantics).  hits[i].tri = original sphere index or 0xFFFFFFFF; u = v = 0.
+ * spheres, rays, hits: dev. */
+chk_status chk_trace_spheres(const float *spheres, uint32_t n_spheres, const chk_ray *rays,
+                             uint32_t n, chk_hit *hits, void *stream);
+
 /* BRDF value and sampling pdf for k configurations.  Replaces
  * BasePrimitive::brdf (geometryPrimitives.cpp:7-9 -> MaterialType::brdf,
  * materialTypes.cpp:9-50; headers/materialTypes.h:100) and

@@ -54,6 +54,9 @@ def lib():
         L.orc_aabb_intersect.argtypes = [_dp, _dp, _dp, _dp]
         L.orc_closest_brute.argtypes = [_dp, C.c_uint32, _dp, C.c_uint32, _ip, _dp, _dp, _dp, _dp, _dp]
         L.orc_occluded_brute.argtypes = [_dp, C.c_uint32, _dp, C.c_uint32, _bp]
+        L.orc_sphere_intersect.restype = C.c_double
+        L.orc_sphere_intersect.argtypes = [_dp, _dp, _dp]
+        L.orc_spheres_brute.argtypes = [_dp, C.c_uint32, _dp, C.c_uint32, _ip, _dp]
         L.orc_bvh_build.restype = _vp
         L.orc_bvh_build.argtypes = [_dp, C.c_uint32]
         L.orc_bvh_free.argtypes = [_vp]
@@ -87,6 +90,7 @@ def ref_lib():
         R = C.CDLL(str(REF_PATH))
         R.ref_bvh_closest.argtypes = [_dp, C.c_uint32, _dp, C.c_uint32, _ip, _dp, _dp]
         R.ref_bvh_occluded.argtypes = [_dp, C.c_uint32, _dp, C.c_uint32, _bp]
+        R.ref_bvh_spheres.argtypes = [_dp, C.c_uint32, _dp, C.c_uint32, _ip, _dp]
         R.ref_brdf.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                                C.c_double, C.c_uint32, _dp, _dp, _dp, _dp, _dp, _dp]
         R.ref_M_GGX.argtypes = [C.c_float, C.c_float, _dp, _dp, _dp, _dp]
@@ -109,6 +113,19 @@ def closest_brute(tris, rays):
     lib().orc_closest_brute(tris, tris.shape[0], rays, m, out["id"], out["t"], out["N"],
                             out["t2"], out["margin"], out["cos2"])
     return out
+
+
+def spheres_brute(spheres, rays):
+    """Sphere::intersect closest over a sphere list: (id, t)."""
+    spheres, rays = _d(spheres, (-1, 4)), _d(rays, (-1, 6))
+    m = rays.shape[0]
+    ids, t = np.empty(m, np.int32), np.empty(m)
+    lib().orc_spheres_brute(spheres, spheres.shape[0], rays, m, ids, t)
+    return ids, t
+
+
+def sphere_intersect(sphere, o, d):
+    return lib().orc_sphere_intersect(_d(sphere), _d(o), _d(d))
 
 
 def occluded_brute(tris, rays7):

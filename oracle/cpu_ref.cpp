@@ -9,6 +9,9 @@
  *       the compiled reference's BvhAccel::intersectT (ids mapped back to
  *       original indices), plus brute force == reference BVH on every ray and
  *       closed-form hits on axis-aligned triangles.
+ *   orc_sphere_intersect, orc_spheres_brute -- golden vectors from the
+ *       compiled reference's BvhAccel::intersectS over Sphere primitives;
+ *       closed-form hits (axis rays through the centre: t = |c - o| - r).
  *   orc_occluded_brute -- golden vectors from the compiled reference's
  *       shadow() semantics (intersectT closest hit + 1e-5 margin).
  *   orc_brdf_eval -- golden vectors from the compiled MaterialType::brdf /
@@ -777,6 +780,36 @@ double orc_tri_intersect(const double *A, const double *B, const double *C, cons
 double orc_aabb_intersect(const double *pmin, const double *pmax, const double *o,
                           const double *d) {
   return aabb_isect(ld(pmin), ld(pmax), ld(o), ld(d));
+}
+
+double orc_sphere_intersect(const double *sphere, const double *o, const double *d) {
+  const double eps = 1e-4;  // geometryPrimitives.h:39
+  const V3 op = v3(sphere[0] - o[0], sphere[1] - o[1], sphere[2] - o[2]);
+  const double b = op.x * d[0] + op.y * d[1] + op.z * d[2];
+  double det = b * b - (op.x * op.x + op.y * op.y + op.z * op.z) + sphere[3] * sphere[3];
+  if (det < 0) return INF;
+  det = std::sqrt(det);
+  double t;
+  return (t = b - det) > eps ? t : ((t = b + det) > eps ? t : INF);
+}
+
+void orc_spheres_brute(const double *spheres, uint32_t n, const double *rays, uint32_t m,
+                       int32_t *id, double *t) {
+#pragma omp parallel for schedule(static)
+  for (int64_t r = 0; r < (int64_t)m; ++r) {
+    const double *q = rays + 6 * r;
+    double best = INF;
+    int32_t bid = -1;
+    for (uint32_t i = 0; i < n; ++i) {
+      const double d = orc_sphere_intersect(spheres + 4 * (size_t)i, q, q + 3);
+      if (d && d < best) {  // bvhAccel.cpp:171
+        best = d;
+        bid = (int32_t)i;
+      }
+    }
+    id[r] = bid;
+    t[r] = best;
+  }
 }
 
 void orc_closest_brute(const double *tris, uint32_t n, const double *rays, uint32_t m,

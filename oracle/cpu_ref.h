@@ -40,6 +40,14 @@ void orc_closest_brute(const double *tris, uint32_t n, const double *rays, uint3
                        int32_t *id, double *t, double *N, double *t2, double *margin,
                        double *cos2);
 
+/* Sphere::intersect (geometryPrimitives.cpp:12-27): first root beyond
+ * eps = 1e-4 (geometryPrimitives.h:39), else 1e20.  Brute-force closest over
+ * n spheres (n x 4: center, radius) with the strict d < t of
+ * BvhAccel::intersectS (bvhAccel.cpp:171): id -1 when nothing is hit. */
+double orc_sphere_intersect(const double *sphere, const double *o, const double *d);
+void orc_spheres_brute(const double *spheres, uint32_t n, const double *rays, uint32_t m,
+                       int32_t *id, double *t);
+
 /* Shadow query of shader.cpp:76-97 for the triangle list alone: occluded iff
  * some triangle hit has 0 < t and t + 1e-5 < dist.  rays: m x 7 (o, d, dist). */
 void orc_occluded_brute(const double *tris, uint32_t n, const double *rays, uint32_t m,

@@ -69,6 +69,32 @@ void ref_bvh_closest(const double *tris, uint32_t n, const double *rays, uint32_
   delete[] list;
 }
 
+/* BvhAccel::intersectS over Sphere primitives (bvhAccel.cpp:151-198,
+ * geometryPrimitives.cpp:12-27): closest sphere id (ORIGINAL index, carried
+ * in the reflectance field like ref_bvh_closest) and t per ray.
+ * spheres: n x 4 (center, radius); rays: m x 6 (o, d). */
+void ref_bvh_spheres(const double *spheres, uint32_t n, const double *rays, uint32_t m,
+                     int32_t *id, double *t) {
+  Sphere *list = new Sphere[n];
+  for (uint32_t i = 0; i < n; ++i) {
+    const double *p = spheres + 4 * (size_t)i;
+    list[i] = Sphere(p[3], Vec(p[0], p[1], p[2]), Vec(1, 1, 1), (double)i, MaterialType());
+  }
+  BvhAccel *acc = new BvhAccel((uint8_t *)list, n, sizeof(Sphere));
+  acc->initAccel();
+  for (uint32_t r = 0; r < m; ++r) {
+    const double *q = rays + 6 * (size_t)r;
+    Ray ray(Vec(q[0], q[1], q[2]), Vec(q[3], q[4], q[5]));
+    double tt;
+    int rid;
+    bool hit = acc->intersectS(ray, tt, rid);
+    id[r] = hit ? (int32_t)list[rid].reflectance : -1;
+    t[r] = tt;
+  }
+  delete acc;
+  delete[] list;
+}
+
 /* shadow() of shader.cpp:88-90 for the triangle BVH: occluded iff the
  * closest hit has t + 1e-5 < dist.  rays: m x 7 (o, d, dist). */
 void ref_bvh_occluded(const double *tris, uint32_t n, const double *rays, uint32_t m,

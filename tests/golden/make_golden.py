@@ -10,6 +10,7 @@ Fixtures (inputs and expected outputs only):
     shadow.npz        shadow() semantics (intersectT + 1e-5 margin)
     brdf.npz          BasePrimitive::brdf and MaterialType::pdfEval
     ltc.npz           M_GGX and LightSource::getLightFromMeshSource_Analytic
+    spheres.npz       BvhAccel::intersectS over Sphere primitives
 """
 from __future__ import annotations
 
@@ -86,6 +87,26 @@ def gen_shadow(R, rng, sc):
     occ = np.empty(r7.shape[0], np.uint8)
     R.ref_bvh_occluded(np.ascontiguousarray(tris), tris.shape[0], r7, r7.shape[0], occ)
     np.savez_compressed(OUT / "shadow.npz", tris=sc.tris, rays7=r7, occluded=occ)
+
+
+def gen_spheres(R, rng):
+    """BvhAccel::intersectS over 64 random spheres (plus the scene front end's
+    dummy sphere, prepareScene.cpp:238) for rays from inside and outside."""
+    k = 64
+    c = rng.uniform(-2, 2, (k, 3))
+    r = rng.uniform(0.05, 0.6, k)
+    spheres = np.concatenate([np.c_[c, r], [[27.0, 1e5, 47.0, 1.0]]]).astype(np.float32).astype(np.float64)
+    m = 8192
+    o = rng.uniform(-3, 3, (m, 3))
+    o[: m // 4] = c[rng.integers(0, k, m // 4)]  # a quarter start inside a sphere
+    d = unit(rng.normal(size=(m, 3)))
+    rays = np.concatenate([o, d], 1).astype(np.float32).astype(np.float64)
+    rays[:, 3:] = unit(rays[:, 3:])
+    ids = np.empty(m, np.int32)
+    t = np.empty(m)
+    R.ref_bvh_spheres(np.ascontiguousarray(spheres), spheres.shape[0], np.ascontiguousarray(rays),
+                      m, ids, t)
+    np.savez_compressed(OUT / "spheres.npz", spheres=spheres, rays=rays, id=ids, t=t)
 
 
 def hemi(rng, n, k):
@@ -199,6 +220,7 @@ def main():
     gen_shadow(R, rng, sc)
     gen_brdf(R, rng)
     gen_ltc(R, rng)
+    gen_spheres(R, np.random.default_rng(20261204))  # own stream: earlier fixtures unchanged
     for p in sorted(OUT.glob("*.npz")):
         print(p.name, p.stat().st_size)
 

@@ -131,3 +131,38 @@ def test_empty_and_degenerate_inputs(ctx):
     assert not hit.any()
     hit, tt, ids = acc3.intersectT([[0.2, 0.2, 1.0]], [[0, 0, -1]])
     assert hit.all() and ids[0] == 0 and abs(tt[0] - 1.0) < 1e-6
+
+
+def test_spheres_match_reference_intersectS(golden):
+    """chk_trace_spheres against the compiled BvhAccel::intersectS: the GPU gets
+    the FP32 rounding of the unit directions, so ids must agree except where
+    the nearest two roots (or a root and eps) are within 1e-5 relative, and
+    t to 1e-6 relative."""
+    G = golden("spheres.npz")
+    sp, rays = G["spheres"], G["rays"]
+    r = host.make_rays(rays[:, :3], rays[:, 3:], tmax=np.full(rays.shape[0], np.inf, np.float32))
+    h = host.intersect_spheres(sp, r)
+    gid = h[:, 1].view(np.uint32).astype(np.int64)
+    gid[gid == 0xFFFFFFFF] = -1
+    # tie band from the oracle: roots of every sphere per ray
+    o, d = rays[:, :3], rays[:, 3:]
+    roots = np.full((rays.shape[0], sp.shape[0]), np.inf)
+    for k in range(sp.shape[0]):
+        op = sp[k, :3] - o
+        b = (op * d).sum(1)
+        det = b * b - (op * op).sum(1) + sp[k, 3] ** 2
+        sq = np.sqrt(np.maximum(det, 0))
+        t1, t2 = b - sq, b + sq
+        t = np.where(t1 > 1e-4, t1, np.where(t2 > 1e-4, t2, np.inf))
+        roots[:, k] = np.where(det >= 0, t, np.inf)
+    srt = np.sort(roots, 1)
+    near_tie = (np.abs(srt[:, 1] - srt[:, 0]) <= 1e-5 * np.abs(srt[:, 0])) | \
+               (np.abs(srt[:, 0] - 1e-4) <= 1e-5)
+    ok = ~near_tie
+    mism = (gid != G["id"]) & ok
+    print(f"rays {rays.shape[0]}, hits {(G['id'] >= 0).mean():.3f}, excluded {(~ok).sum()}, "
+          f"mismatches {mism.sum()}")
+    assert mism.sum() == 0
+    hit = (G["id"] >= 0) & ok
+    np.testing.assert_allclose(h[hit, 0], G["t"][hit], rtol=1e-6)
+    assert np.all(h[G["id"] < 0, 0] >= 1e29)

@@ -131,3 +131,26 @@ def test_live_reference_brdf_random_materials():
         ok = np.isfinite(f)
         np.testing.assert_allclose(fo[ok], f[ok], rtol=1e-10)
         np.testing.assert_allclose(po[ok], p[ok], rtol=1e-10)
+
+
+def test_spheres_match_reference_intersectS(golden):
+    """orc_spheres_brute against the compiled BvhAccel::intersectS over Sphere
+    primitives (same FP64 inputs): ids equal, t to 1e-12."""
+    G = golden("spheres.npz")
+    ids, t = oracle.spheres_brute(G["spheres"], G["rays"])
+    np.testing.assert_array_equal(ids, G["id"])
+    hit = G["id"] >= 0
+    assert 0.2 < hit.mean() < 0.95
+    np.testing.assert_allclose(t[hit], G["t"][hit], rtol=1e-12)
+
+
+def test_sphere_closed_form():
+    """A ray along an axis through the centre hits at |c - o| - r from
+    outside and at r - |c - o| from inside; roots at or below eps = 1e-4 are
+    skipped (geometryPrimitives.cpp:26)."""
+    s = np.array([1.0, 2.0, 3.0, 0.5])
+    assert oracle.sphere_intersect(s, [1.0, 2.0, 0.0], [0, 0, 1.0]) == pytest.approx(2.5, rel=1e-15)
+    assert oracle.sphere_intersect(s, [1.0, 2.0, 2.9], [0, 0, 1.0]) == pytest.approx(0.6, rel=1e-14)
+    assert oracle.sphere_intersect(s, [1.0, 2.0, 0.0], [0, 0, -1.0]) == 1e20
+    # origin on the surface: the root at t = 0 is below eps, the far one counts
+    assert oracle.sphere_intersect(s, [1.0, 2.0, 2.5], [0, 0, 1.0]) == pytest.approx(1.0, rel=1e-14)
