@@ -135,9 +135,12 @@ def test_empty_and_degenerate_inputs(ctx):
 
 def test_spheres_match_reference_intersectS(golden):
     """chk_trace_spheres against the compiled BvhAccel::intersectS: the GPU gets
-    the FP32 rounding of the unit directions, so ids must agree except where
-    the nearest two roots (or a root and eps) are within 1e-5 relative, and
-    t to 1e-6 relative."""
+    the FP32 rounding of the unit directions, so ids must agree with the
+    reference except where the nearest two roots (or a root and eps) are within
+    1e-5 relative.  That rounding moves t of near-tangent rays by more than
+    1e-6 (sqrt(det) amplifies it), so t is held to 1e-6 relative against the
+    oracle restatement (pinned to the same golden vectors on the CPU) run on
+    exactly the FP32-rounded rays the GPU sees, where ids must also agree."""
     G = golden("spheres.npz")
     sp, rays = G["spheres"], G["rays"]
     r = host.make_rays(rays[:, :3], rays[:, 3:], tmax=np.full(rays.shape[0], np.inf, np.float32))
@@ -163,6 +166,13 @@ def test_spheres_match_reference_intersectS(golden):
     print(f"rays {rays.shape[0]}, hits {(G['id'] >= 0).mean():.3f}, excluded {(~ok).sum()}, "
           f"mismatches {mism.sum()}")
     assert mism.sum() == 0
-    hit = (G["id"] >= 0) & ok
-    np.testing.assert_allclose(h[hit, 0], G["t"][hit], rtol=1e-6)
-    assert np.all(h[G["id"] < 0, 0] >= 1e29)
+    oid, ot = oracle.spheres_brute(sp, rays.astype(np.float32).astype(np.float64))
+    assert np.array_equal(gid[ok], oid[ok])
+    hit = (oid >= 0) & ok
+    np.testing.assert_allclose(h[hit, 0], ot[hit], rtol=1e-6)
+    assert np.all(h[oid < 0, 0] >= 1e29)
+    # against the reference's t on its exact unit directions, the input rounding
+    # alone bounds the difference away from tangency
+    far = (G["id"] >= 0) & ok & (np.abs(h[:, 0] - G["t"]) > 1e-6 * G["t"])
+    print(f"t beyond 1e-6 of the reference from direction rounding: {far.sum()}")
+    assert far.mean() < 0.01
