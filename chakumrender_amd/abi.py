@@ -81,6 +81,11 @@ class chk_render_stats(C.Structure):
                 ("proposal_ms", C.c_double), ("train_steps", C.c_uint32), ("reserved", C.c_uint32)]
 
 
+class chk_direct_params(C.Structure):
+    _fields_ = [("estimator", C.c_uint32), ("cv_samples", C.c_uint32), ("flags", C.c_uint32),
+                ("estimate", C.c_uint32), ("seed_lo", C.c_uint32), ("seed_hi", C.c_uint32)]
+
+
 assert C.sizeof(chk_material) == 56
 assert C.sizeof(chk_render_params) == 32
 
@@ -91,6 +96,9 @@ RENDER_PROFILE = 0x100
 JITTER_BOX, JITTER_REFERENCE = 0, 1
 MIS_BALANCE, MIS_LEARNED = 0, 1
 TRAIN_FREEZE_MIX, TRAIN_FREEZE_LIGHTS, TRAIN_LIGHTS_CHI2 = 1, 2, 4
+DIRECT_ANALYTIC, DIRECT_ANALYTIC_CV, DIRECT_MIS, DIRECT_MISCV = 0, 1, 2, 3
+DIRECT_VISIBILITY = 0x1
+MIS_LIGHT_SAMPLES, MIS_BSDF_SAMPLES = 10, 60
 
 # Every exported symbol of include/chakum_c.h: name -> (restype, argtypes)
 _vp = C.c_void_p
@@ -129,6 +137,12 @@ SIGNATURES = {
     "chk_render_learned": (C.c_int, [_vp, C.POINTER(chk_camera), C.POINTER(chk_render_params), _vp,
                                      C.POINTER(chk_train_params), _vp, _vp,
                                      C.POINTER(chk_render_stats), _vp]),
+    "chk_mis_weights": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _u32, _vp, _u32, _vp, _vp]),
+    "chk_direct_light": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _u32, C.POINTER(chk_direct_params),
+                                   _vp, _vp]),
+    "chk_render_direct": (C.c_int, [_vp, C.POINTER(chk_camera), C.POINTER(chk_render_params),
+                                    C.POINTER(chk_direct_params), _vp, _vp,
+                                    C.POINTER(chk_render_stats), _vp]),
     "chk_proposal_create": (C.c_int, [_vp, _u32, C.c_uint64, C.POINTER(_vp)]),
     "chk_proposal_destroy": (C.c_int, [_vp]),
     "chk_proposal_set_params": (C.c_int, [_vp, _vp, C.c_size_t]),

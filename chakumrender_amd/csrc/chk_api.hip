@@ -93,21 +93,6 @@ __global__ void k_fill_zero(uint4 *p, size_t n16, unsigned char *tail, size_t nt
 
 using namespace chk;
 
-#define CHK_TRY_BEGIN try {
-#define CHK_TRY_END                                   \
-  }                                                   \
-  catch (const std::bad_alloc &) {                    \
-    set_error("host allocation failed");              \
-    return CHK_ENOMEM;                                \
-  }                                                   \
-  catch (const std::exception &ex) {                  \
-    set_error(std::string("internal error: ") + ex.what()); \
-    return CHK_EINTERNAL;                             \
-  }                                                   \
-  catch (...) {                                       \
-    set_error("internal error");                      \
-    return CHK_EINTERNAL;                             \
-  }
 
 extern "C" {
 
@@ -287,6 +272,7 @@ chk_status chk_scene_create(chk_ctx_t ctx, const float *tris, uint32_t n_tris,
   // 300 bins, lightSources.h:151, lightSources.cpp:784-807)
   std::vector<LightTri> ltris;
   std::vector<LightRec> lrecs;
+  std::vector<double> larea64;
   uint32_t used_lights = 0;
   for (uint32_t j = 0; j < n_lights; ++j) {
     LightRec R;
@@ -317,6 +303,10 @@ chk_status chk_scene_create(chk_ctx_t ctx, const float *tris, uint32_t n_tris,
       const float *p = tris + 9 * (size_t)o;
       const chk_material &mt = mats[tri_mat[o]];
       cdf += areas[i] / area;
+      // the reference's bin: index = cdf * 300 clamped to 299; the last
+      // triangle also takes every bin after it (lightSources.cpp:793-806)
+      uint32_t bin_end = (uint32_t)(cdf * 300);
+      if (bin_end >= 300 || i + 1 == lights[j].n_tris) bin_end = 299;
       LightTri T;
       T.a_cdf = make_float4(p[0], p[1], p[2], i + 1 == lights[j].n_tris ? 2.0f : (float)cdf);
       T.b_area = make_float4(p[3], p[4], p[5], (float)areas[i]);
@@ -324,13 +314,14 @@ chk_status chk_scene_create(chk_ctx_t ctx, const float *tris, uint32_t n_tris,
       float lf;
       std::memcpy(&lf, &leaf, 4);
       T.c_leaf = make_float4(p[6], p[7], p[8], lf);
-      T.rad = make_float4(mt.radiance[0], mt.radiance[1], mt.radiance[2], 0.0f);
+      T.rad = make_float4(mt.radiance[0], mt.radiance[1], mt.radiance[2], (float)bin_end);
       ltris.push_back(T);
       ++R.count;
     }
     R.area = (float)area;
     R.pad = 0;
     lrecs.push_back(R);
+    larea64.push_back(area);
     ++used_lights;
   }
 
@@ -341,7 +332,16 @@ chk_status chk_scene_create(chk_ctx_t ctx, const float *tris, uint32_t n_tris,
   const size_t b_mats = align256(std::max<uint32_t>(n_mats, 1) * sizeof(chk_material));
   const size_t b_lt = align256(std::max<size_t>(ltris.size(), 1) * sizeof(LightTri));
   const size_t b_lr = align256(std::max<size_t>(lrecs.size(), 1) * sizeof(LightRec));
-  const size_t total = b_nodes + b_tris + b_info + b_mats + b_lt + b_lr;
+  // per original triangle (leaf, material) for entry points that name a
+  // triangle by its caller index
+  std::vector<uint32_t> oinfo(2 * (size_t)std::max(n_tris, 1u), 0u);
+  for (uint32_t o = 0; o < n_tris; ++o) {
+    oinfo[2 * (size_t)o] = bvh.orig_to_leaf[o] >= 0 ? (uint32_t)bvh.orig_to_leaf[o] : 0xFFFFFFFFu;
+    oinfo[2 * (size_t)o + 1] = tri_mat[o];
+  }
+  const size_t b_oi = align256(oinfo.size() * 4);
+  const size_t b_la = align256(std::max<size_t>(larea64.size(), 1) * sizeof(double));
+  const size_t total = b_nodes + b_tris + b_info + b_mats + b_lt + b_lr + b_oi + b_la;
   void *blob = nullptr;
   if (hipMalloc(&blob, total) != hipSuccess) {
     set_error("chk_scene_create: cannot allocate " + std::to_string(total) + " device bytes");
@@ -374,6 +374,13 @@ chk_status chk_scene_create(chk_ctx_t ctx, const float *tris, uint32_t n_tris,
   sc->d.lights = (const LightRec *)p;
   if (e == hipSuccess && !lrecs.empty())
     e = hipMemcpy(p, lrecs.data(), lrecs.size() * sizeof(LightRec), hipMemcpyHostToDevice);
+  p += b_lr;
+  sc->d.orig_info = (const uint2 *)p;
+  if (e == hipSuccess) e = hipMemcpy(p, oinfo.data(), oinfo.size() * 4, hipMemcpyHostToDevice);
+  p += b_oi;
+  sc->d.light_area = (const double *)p;
+  if (e == hipSuccess && !larea64.empty())
+    e = hipMemcpy(p, larea64.data(), larea64.size() * sizeof(double), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     set_error(std::string("chk_scene_create: upload: ") + hipGetErrorString(e));
     (void)hipFree(blob);
@@ -382,6 +389,8 @@ chk_status chk_scene_create(chk_ctx_t ctx, const float *tris, uint32_t n_tris,
   }
   sc->d.n_lights = used_lights;
   sc->d.n_leaf_tris = m;
+  sc->d.n_tris = n_tris;
+  sc->n_mats = n_mats;
   {
     float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (const TriRec &t : bvh.tris)
@@ -418,6 +427,7 @@ chk_status chk_scene_destroy(chk_scene_t sc) {
   (void)hipSetDevice(sc->ctx->device);
   if (sc->work) (void)hipFree(sc->work);
   if (sc->lwork) (void)hipFree(sc->lwork);
+  if (sc->dwork) (void)hipFree(sc->dwork);
   if (sc->blob) (void)hipFree(sc->blob);
   delete sc;
   return CHK_OK;

@@ -23,12 +23,31 @@ void set_error(const std::string &msg);
     }                                                                              \
   } while (0)
 
+// Entry points catch every C++ exception: none crosses the C ABI.
+#define CHK_TRY_BEGIN try {
+#define CHK_TRY_END                                   \
+  }                                                   \
+  catch (const std::bad_alloc &) {                    \
+    set_error("host allocation failed");              \
+    return CHK_ENOMEM;                                \
+  }                                                   \
+  catch (const std::exception &ex) {                  \
+    set_error(std::string("internal error: ") + ex.what()); \
+    return CHK_EINTERNAL;                             \
+  }                                                   \
+  catch (...) {                                       \
+    set_error("internal error");                      \
+    return CHK_EINTERNAL;                             \
+  }
+
 // Device-resident light data for next-event estimation.
 struct LightTri {
   float4 a_cdf;    // A, cumulative area fraction within its light
   float4 b_area;   // B, triangle area
   float4 c_leaf;   // C, leaf index (raw bits)
-  float4 rad;      // radiance (emitter material), w unused
+  float4 rad;      // radiance (emitter material); w: last bin of the
+                   // reference's 300-bin inverse CDF that maps to this
+                   // triangle (MeshLight::initMeshLight, lightSources.cpp:784-807)
 };
 struct LightRec {
   uint32_t first;  // into the LightTri table
@@ -44,8 +63,11 @@ struct DeviceScene {
   const chk_material *mats = nullptr;
   const LightTri *light_tris = nullptr;
   const LightRec *lights = nullptr;
+  const uint2 *orig_info = nullptr;    // per original triangle: (leaf or 0xFFFFFFFF, material)
+  const double *light_area = nullptr;  // per mesh light: MeshLight::area in FP64
   uint32_t n_lights = 0;
   uint32_t n_leaf_tris = 0;
+  uint32_t n_tris = 0;                 // original triangle count
   // box of the BVH triangles (degenerate ones excluded): the learned
   // proposal's position input is (x - box_lo) * box_inv in [0, 1]
   float box_lo[3] = {0, 0, 0}, box_inv[3] = {1, 1, 1};
@@ -56,7 +78,7 @@ struct DeviceScene {
 struct chk_ctx_s {
   int device = 0;
   int n_cu = 256;
-  double *ltc_dev = nullptr;  // packed LTC table (6 doubles per record, chk_ltc.hip)
+  double *ltc_dev = nullptr;  // packed LTC table (kLtcRec doubles per record, chk_ltc_dev.h)
   void *batch_state = nullptr;  // queues, tables and cached graphs of chk_render_batch
 };
 
@@ -72,6 +94,9 @@ struct chk_scene_s {
   size_t work_bytes = 0;
   void *lwork = nullptr;  // learned-proposal buffers of the render
   size_t lwork_bytes = 0;
+  void *dwork = nullptr;  // direct-lighting estimators' chunk buffers
+  size_t dwork_bytes = 0;
+  uint32_t n_mats = 0;
 };
 
 namespace chk {

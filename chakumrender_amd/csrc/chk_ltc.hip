@@ -18,107 +18,13 @@
 //
 // One shading point per lane; the light triangles are wave-uniform (scalar
 // loads); the two table rows come from L2.
-#include "chk_device.h"
 #include "chk_internal.h"
+#include "chk_ltc_dev.h"
 
 namespace chk {
 namespace {
 
 constexpr int kLtcBlock = 256;
-constexpr double kPi64 = 3.14159265359;  // mathPrimitives.h:4
-
-// integrateEdge (lightSources_CV.cpp:8-25): (v1 x v2).n * theta/sin(theta)
-// with v_i = p_i/|p_i|, from the unnormalised p1, p2, c = p1 x p2 and
-// cn = c.n: (v1 x v2).n = cn/(|p1||p2|) and sin(theta) = |c|/(|p1||p2|), so
-// the term is cn * theta/|c| with theta = atan2(|c|, p1.p2) (the
-// reference's acos(v1.v2), without its loss of precision near 0).
-__device__ inline double edge64(d3 p1, d3 p2, d3 c, double cn) {
-  const double s = sqrt(dot(c, c));
-  const double th = atan2(s, dot(p1, p2));
-  return th > 0.001 ? cn * (th / s) : cn / sqrt(dot(p1, p1) * dot(p2, p2));
-}
-
-// n.v for the clip plane: the LTC part clips against z = 0 in its own
-// frame (ZPLANE), the diffuse part against the plane through x with normal n.
-template <bool ZPLANE>
-__device__ inline double ndot(d3 n, d3 v) {
-  return ZPLANE ? v.z : dot(n, v);
-}
-
-// clipTriangle (lightSources_CV.cpp:28-126 and 129-219): the 4-point
-// polygon of the part of ABC above the plane; false if it has none.
-// nA, nB, nC: signs of the vertices (only their sign is read).
-template <bool ZPLANE>
-__device__ bool clip64(d3 A, d3 B, d3 C, d3 n, d3 P, double nA, double nB, double nC, d3 *clip) {
-  const double eps = 1e-15;
-  int cnt = 0, idx = 0;
-  const d3 dAB = A - B, dBC = B - C, dCA = C - A;
-  const double nAB = ndot<ZPLANE>(n, dAB), nBC = ndot<ZPLANE>(n, dBC),
-               nCA = ndot<ZPLANE>(n, dCA);
-  const double pA = ndot<ZPLANE>(n, P - A), pB = ndot<ZPLANE>(n, P - B),
-               pC = ndot<ZPLANE>(n, P - C);
-  if (nAB * nAB > eps) {
-    const d3 I = dAB * (pA / nAB) + A;
-    if (dot(I - A, B - I) > 0) {
-      ++cnt;
-      if (nA < 0) { clip[idx++] = I; clip[idx++] = B; }
-      else { clip[idx++] = A; clip[idx++] = I; }
-    }
-  } else if (pA * pA < eps) {
-    return false;
-  }
-  if (nBC * nBC > eps) {
-    const d3 I = dBC * (pB / nBC) + B;
-    if (dot(I - B, C - I) > 0) {
-      ++cnt;
-      if (idx < 4) {
-        if (nB < 0) { clip[idx++] = I; clip[idx++] = C; }
-        else { clip[idx++] = B; clip[idx++] = I; }
-      }
-    }
-  } else if (pB * pB < eps) {
-    return false;
-  }
-  if (nCA * nCA > eps) {
-    const d3 I = dCA * (pC / nCA) + C;
-    // the reference tests only the x components on this edge
-    if ((I.x - C.x) * (A.x - I.x) > 0) {
-      ++cnt;
-      if (idx < 4) {
-        if (nC < 0) { clip[idx++] = I; clip[idx++] = A; }
-        else { clip[idx++] = C; clip[idx++] = I; }
-      }
-    }
-  } else if (pC * pC < eps) {
-    return false;
-  }
-  return cnt > 1;
-}
-
-// |sum of edge terms| of one triangle (lightSources_CV.cpp:241-262 and
-// 293-314).  ZPLANE: LTC frame (P = 0, n = z).
-template <bool ZPLANE>
-__device__ double tri64(d3 A, d3 B, d3 C, d3 n, d3 P) {
-  const d3 p1 = A - P, p2 = B - P, p3 = C - P;
-  // signs of n.(V - P)/|V - P|, as the reference tests them
-  const double nA = ndot<ZPLANE>(n, p1), nB = ndot<ZPLANE>(n, p2), nC = ndot<ZPLANE>(n, p3);
-  double sum;
-  if (nA > 0 && nB > 0 && nC > 0) {
-    const d3 c1 = cross(p1, p2), c2 = cross(p2, p3), c3 = cross(p3, p1);
-    sum = edge64(p1, p2, c1, ndot<ZPLANE>(n, c1)) + edge64(p2, p3, c2, ndot<ZPLANE>(n, c2)) +
-          edge64(p3, p1, c3, ndot<ZPLANE>(n, c3));
-  } else if (nA < 0 && nB < 0 && nC < 0) {
-    return 0.0;
-  } else {
-    d3 cl[4] = {dk(0, 0, 0), dk(0, 0, 0), dk(0, 0, 0), dk(0, 0, 0)};
-    if (!clip64<ZPLANE>(A, B, C, n, P, nA, nB, nC, cl)) return 0.0;
-    const d3 q1 = cl[0] - P, q2 = cl[1] - P, q3 = cl[2] - P, q4 = cl[3] - P;
-    const d3 c1 = cross(q1, q2), c2 = cross(q2, q3), c3 = cross(q3, q4), c4 = cross(q4, q1);
-    sum = edge64(q1, q2, c1, ndot<ZPLANE>(n, c1)) + edge64(q2, q3, c2, ndot<ZPLANE>(n, c2)) +
-          edge64(q3, q4, c3, ndot<ZPLANE>(n, c3)) + edge64(q4, q1, c4, ndot<ZPLANE>(n, c4));
-  }
-  return fabs(sum);
-}
 
 __global__ __launch_bounds__(kLtcBlock) void k_ltc(const double *__restrict__ tab,
                                                    const float *__restrict__ X,
@@ -135,22 +41,9 @@ __global__ __launch_bounds__(kLtcBlock) void k_ltc(const double *__restrict__ ta
   const d3 x = dk(X[o], X[o + 1], X[o + 2]), n = dk(N[o], N[o + 1], N[o + 2]),
            wo = dk(V[o], V[o + 1], V[o + 2]);
   const double kss = ks[i];
-  // M_GGX((float)acos(n.wo), (float)alpha) (ltc.h:18-61, materialTypes.cpp:54):
-  // bins from the float theta and sqrt(alpha) in double, lerp with a float
-  // weight
-  const float theta = (float)acos(dot(n, wo));
-  const double tq = (double)theta / (0.5 * kPi64) * 64.0;
-  const int tb = max(0, min(63, (int)floor(tq)));
-  const int ab = max(0, min(63, (int)floor(sqrt((double)alpha[i]) * 64.0)));
-  const int tp = min(63, tb + 1);
-  const double *r0 = tab + 6 * (ab + tb * 64);
-  double m[6] = {r0[0], r0[1], r0[2], r0[3], r0[4], r0[5]};
-  if (tb != tp) {
-    const float mult = (float)(tq - (double)(float)tb);
-    const double *r1 = tab + 6 * (ab + tp * 64);
-#pragma unroll
-    for (int c = 0; c < 6; ++c) m[c] = m[c] + (r1[c] - m[c]) * (double)mult;
-  }
+  // M_GGX((float)acos(n.wo), (float)alpha) (ltc.h:18-61, materialTypes.cpp:54)
+  const LtcFit F = ltc_fit(tab, (float)acos(dot(n, wo)), (double)alpha[i]);
+  const double *m = F.mi;
   // frame of getBrdfApproxParam (materialTypes.cpp:59-68); rows of Minv * R
   const d3 T1 = normalize(wo - n * dot(n, wo));
   const d3 T2 = normalize(cross(n, T1));
@@ -180,7 +73,7 @@ __global__ __launch_bounds__(kLtcBlock) void k_ltc(const double *__restrict__ ta
     }
   }
   // Sum / (2 pi), then Analytic = LTC * amp * ks + diffuse * (1 - ks)
-  const double wl = m[5] * kss / (2.0 * kPi64), wd = (1.0 - kss) / (2.0 * kPi64);
+  const double wl = F.amp * kss / (2.0 * kPi64), wd = (1.0 - kss) / (2.0 * kPi64);
   rgb[o] = (float)(sl0 * wl + sd0 * wd);
   rgb[o + 1] = (float)(sl1 * wl + sd1 * wd);
   rgb[o + 2] = (float)(sl2 * wl + sd2 * wd);
@@ -199,23 +92,29 @@ chk_status chk_set_ltc_table(chk_ctx_t ctx, const double *table, size_t n_double
     return CHK_EINVAL;
   }
   CHK_HIP(hipSetDevice(ctx->device));
-  // packed row: Minv a[0], a[2], a[4], a[6], a[8] (= file entries 0, 6, 4,
-  // 2, 8 through mat3's column-major 9-argument ctor), amplitude
-  std::vector<double> p64(4096 * 6);
+  // packed record (kLtcRec doubles): Minv a[0], a[2], a[4], a[6], a[8]
+  // (= file entries 0, 6, 4, 2, 8 through mat3's column-major 9-argument
+  // ctor), amplitude, the same five entries of M, and det M as
+  // mat3::calcDet computes it (mathPrimitives.h:165-169)
+  std::vector<double> p64(4096 * (size_t)kLtcRec);
   for (int r = 0; r < 4096; ++r) {
-    const double *f = table + 19 * (size_t)r + 9;
+    const double *fm = table + 19 * (size_t)r, *fi = fm + 9;
     for (int z : {1, 3, 5, 7})
-      if (f[z] != 0.0) {
-        set_error("chk_set_ltc_table: Minv entry that should be zero is not");
+      if (fi[z] != 0.0 || fm[z] != 0.0) {
+        set_error("chk_set_ltc_table: an M or Minv entry that should be zero is not");
         return CHK_EINVAL;
       }
-    double *q = &p64[6 * (size_t)r];
-    q[0] = f[0];
-    q[1] = f[6];
-    q[2] = f[4];
-    q[3] = f[2];
-    q[4] = f[8];
-    q[5] = table[19 * (size_t)r + 18];
+    double *q = &p64[kLtcRec * (size_t)r];
+    const int pick[5] = {0, 6, 4, 2, 8};
+    for (int c = 0; c < 5; ++c) {
+      q[c] = fi[pick[c]];
+      q[6 + c] = fm[pick[c]];
+    }
+    q[5] = fm[18];
+    // a[] of M in mat3 order (column-major ctor), then calcDet
+    const double a[9] = {fm[0], fm[3], fm[6], fm[1], fm[4], fm[7], fm[2], fm[5], fm[8]};
+    q[11] = a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) +
+            a[2] * (a[3] * a[7] - a[4] * a[6]);
   }
   if (!ctx->ltc_dev) {
     if (hipMalloc(&ctx->ltc_dev, p64.size() * 8) != hipSuccess) {

@@ -8,6 +8,11 @@ C ABI, batched.  Names and argument meanings follow the reference:
                         getBrdfDirectionSampleRandomized)
   LightSource.getLightFromMeshSource_Analytic
                         lightSources_CV.cpp:323-333
+  SceneLights           the shadeDI estimators over a scene's mesh lights
+                        (lightSources.cpp:32-109, lightSources_CV.cpp:323-450)
+                        and MIS::getSamples' weights (mis.cpp:18-223)
+  render_direct         shadeDirectOnly (shader.cpp:339-363), the reference's
+                        shipped shade()
   render                main.cpp:62-94 around shade() (shader.cpp:362)
   write_ppm             main.cpp:27,98-117
 
@@ -156,6 +161,95 @@ class LightSource:
         rgb = DevBuf(12 * k)
         self.eval_dev(bufs, k, rgb)
         return rgb.download((k, 3), np.float32)
+
+
+class SceneLights:
+    """The reference's LightSource / MIS estimators over a scene's mesh
+    lights (chk_direct_light, chk_mis_weights).  Shading points are the
+    reference's (r.d, n, x, primitive): FP64 x, n (facing the viewer), d
+    (the ray direction), and prim, the original index of the triangle x lies
+    on.  Every call returns one estimate per point (FP64, k x 3)."""
+
+    def __init__(self, ds: abi.DeviceScene, seed=(0x5EED0006, 0)):
+        self.ds = ds
+        self.L = ds.L
+        self.seed = seed
+
+    def _pts(self, *arrs):
+        return [DevBuf.from_array(np.ascontiguousarray(a, np.float64).reshape(-1, 3)) for a in arrs]
+
+    def estimate(self, estimator, x, n, d, prim, estimate=0, cv_samples=0, flags=0, seed=None):
+        k = int(np.asarray(prim).size)
+        xd, nd, dd = self._pts(x, n, d)
+        pd = DevBuf.from_array(np.ascontiguousarray(prim, np.uint32))
+        P = abi.chk_direct_params()
+        P.estimator, P.cv_samples, P.flags, P.estimate = estimator, cv_samples, flags, estimate
+        P.seed_lo, P.seed_hi = seed or self.seed
+        out = DevBuf(24 * max(k, 1))
+        abi.check(self.L.chk_direct_light(self.ds.h, xd.ptr, nd.ptr, dd.ptr, pd.ptr, k, C.byref(P),
+                                          out.ptr, None), "chk_direct_light")
+        return out.download((k, 3), np.float64)
+
+    def getLightFromMeshSource_Analytic(self, x, n, d, prim):
+        """lightSources_CV.cpp:323-333 (deterministic)."""
+        return self.estimate(abi.DIRECT_ANALYTIC, x, n, d, prim)
+
+    def getLightFromMeshSource_AnalyticCV(self, x, n, d, prim, nSamples, estimate=0):
+        """lightSources_CV.cpp:335-397."""
+        return self.estimate(abi.DIRECT_ANALYTIC_CV, x, n, d, prim, estimate, cv_samples=nSamples)
+
+    def getLightFromAllSources_MIS(self, x, n, d, prim, estimate=0):
+        """lightSources.cpp:32-109."""
+        return self.estimate(abi.DIRECT_MIS, x, n, d, prim, estimate)
+
+    def getLightFromMeshSource_Analytic_MISCV(self, x, n, d, prim, estimate=0, visibility=False):
+        """lightSources_CV.cpp:399-450; visibility=True applies the shadow
+        test the reference computes and ignores."""
+        return self.estimate(abi.DIRECT_MISCV, x, n, d, prim, estimate,
+                             flags=abi.DIRECT_VISIBILITY if visibility else 0)
+
+    def mis_weights(self, x, n, wo_ref, wo, prim, samples):
+        """The weights MIS::getSamples returns for these samples
+        (mis.cpp:209-220); samples (k, S, 3) -> weights (k, S)."""
+        smp = np.ascontiguousarray(samples, np.float64)
+        k, S = smp.shape[0], smp.shape[1]
+        xd, nd, rd, od = self._pts(x, n, wo_ref, wo)
+        pd = DevBuf.from_array(np.ascontiguousarray(prim, np.uint32))
+        sd = DevBuf.from_array(smp.reshape(-1, 3))
+        w = DevBuf(8 * max(k * S, 1))
+        abi.check(self.L.chk_mis_weights(self.ds.h, xd.ptr, nd.ptr, rd.ptr, od.ptr, pd.ptr, k, sd.ptr,
+                                         S, w.ptr, None), "chk_mis_weights")
+        return w.download((k, S), np.float64)
+
+
+def direct_params(estimator, cv_samples=0, flags=0, seed=(0x5EED0006, 0)):
+    P = abi.chk_direct_params()
+    P.estimator, P.cv_samples, P.flags = estimator, cv_samples, flags
+    P.seed_lo, P.seed_hi = seed
+    return P
+
+
+def render_direct(ds: abi.DeviceScene, params, direct, camera=None, image: "Image | None" = None,
+                  stats=True, stream=None, download=True):
+    """chk_render_direct: the reference's shipped shade() (shadeDirectOnly,
+    shader.cpp:339-363) with the estimator of `direct` (a chk_direct_params
+    or a dict of its fields)."""
+    cam = abi.camera_struct(camera or ds.scene.camera)
+    P = abi.params_struct(params)
+    if isinstance(direct, dict):
+        direct = direct_params(**direct)
+    if image is None:
+        image = Image(cam.width, cam.height, sumsq=bool(P.flags & abi.RENDER_SUMSQ))
+    st = abi.chk_render_stats()
+    abi.check(ds.L.chk_render_direct(ds.h, C.byref(cam), C.byref(P), C.byref(direct),
+                                     image.acc.ptr, image.sq.ptr if image.sq is not None else None,
+                                     C.byref(st) if stats else None, stream),
+              "chk_render_direct")
+    sd = {k: getattr(st, k) for k, _ in abi.chk_render_stats._fields_} if stats else {}
+    if not download:
+        return image, sd
+    a, s2 = image.get()
+    return a, s2, sd
 
 
 class Proposal:

@@ -361,6 +361,75 @@ def cornell_large(seed=0x5EED0005, total_tris=2_000_000, n_lights=32, width=1920
                    height=height, light_rng=True)
 
 
+# ------------------------------------------------------------------ direct-lighting room
+def direct_room(seed=0x5EED0006, width=64, height=64, reflectance=None, fan=True, panel=True):
+    """A unit room for the shadeDI estimators: Lambert floor, GGX back wall,
+    Beckmann left wall, Phong right wall, a classic-Phong panel hanging under
+    the ceiling (it shadows part of the floor), and two mesh lights under the
+    ceiling: a 2-triangle quad and a 3-triangle fan of unequal areas (so the
+    reference's 300-bin inverse CDF is not exact).  `reflectance` overrides
+    every surface's reflectance; fan=False makes the second light a quad too,
+    panel=False drops the panel (nothing then occludes a light)."""
+    rng = np.random.default_rng(seed)
+
+    def R(r):
+        return r if reflectance is None else reflectance
+
+    mats = [
+        lambert_material(rng.uniform(0.3, 0.8, 3)),
+        _material(GGX, ks=0.5, alpha=0.3, reflectance=R(0.9), color=rng.uniform(0.4, 0.9, 3)),
+        _material(BECKMANN, ks=0.5, alpha=0.2, reflectance=R(1.0), color=rng.uniform(0.4, 0.9, 3)),
+        _material(PHONG, ks=0.333, alpha=2.0 / 0.3 ** 2 - 2.0, reflectance=R(0.8),
+                  color=rng.uniform(0.4, 0.9, 3)),
+        _material(CLASSIC_PHONG, ks=0.5, phong_exp=20.0, reflectance=R(0.9), color=(0.8, 0.8, 0.8)),
+    ]
+    if reflectance is not None:
+        mats[0]["reflectance"] = reflectance
+    obj = [quad((0.5, 0.0, 0.5), (0.5, 0, 0), (0, 0, 0.5)),   # floor
+           quad((0.5, 0.5, 0.0), (0.5, 0, 0), (0, 0.5, 0)),   # back
+           quad((0.0, 0.5, 0.5), (0, 0.5, 0), (0, 0, 0.5)),   # left
+           quad((1.0, 0.5, 0.5), (0, 0.5, 0), (0, 0, 0.5)),   # right
+           quad((0.45, 0.6, 0.5), (0.15, 0, 0), (0, 0, 0.12))]  # panel
+    omat = np.repeat(np.arange(5), 2)
+    if not panel:
+        obj, omat = obj[:4], omat[:8]
+    c = np.array([0.7, 0.999, 0.6])
+    if fan:
+        second = np.array([np.r_[c, c + (0.12, 0, -0.05), c + (0.02, 0, 0.1)],
+                           np.r_[c, c + (0.02, 0, 0.1), c + (-0.1, 0, 0.04)],
+                           np.r_[c, c + (-0.1, 0, 0.04), c + (-0.03, 0, -0.03)]])
+    else:
+        second = quad((0.7, 0.999, 0.6), (0.08, 0, 0), (0, 0, 0.07))
+    lights = [(quad((0.3, 0.999, 0.35), (0.12, 0, 0), (0, 0, 0.1)), (1.0, 0.9, 0.7), 12.0),
+              (second, (0.6, 0.8, 1.0), 20.0)]
+    cam = Camera((0.5, 0.5, 2.2), (0, 0, -1), (0, 1, 0), 40.0, 40.0, width, height)
+    return _assemble(np.concatenate(obj), omat, mats, lights, cam, "direct_room", {"seed": seed})
+
+
+def shading_points(scene, k, seed=0x5EED0007, eye=(0.5, 0.55, 0.95), tris=None):
+    """k shading points on the scene's non-emitter triangles (or on `tris`)
+    seen from `eye`: (x, n facing the eye, d = unit direction eye -> x, prim),
+    FP64 from the FP32 triangles, the arguments the reference's estimators
+    receive from shadeDirectOnly."""
+    rng = np.random.default_rng(seed)
+    T = scene.tris.astype(np.float64)
+    if tris is None:
+        tris = np.nonzero(scene.mats["light"][scene.tri_mat] == LIGHT_NONE)[0]
+        tris = tris[np.abs(T[tris]).max(1) < 1e5]  # not the dummy triangle
+    prim = rng.choice(np.asarray(tris), k).astype(np.uint32)
+    A, B, C = T[prim, 0:3], T[prim, 3:6], T[prim, 6:9]
+    u = rng.uniform(0.02, 0.98, (k, 2))
+    flip = u.sum(1) > 1
+    u[flip] = 1 - u[flip]
+    x = A + (B - A) * u[:, :1] + (C - A) * u[:, 1:]
+    d = x - np.asarray(eye, np.float64)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    n = np.cross(C - A, B - A)
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    n = np.where((n * d).sum(1, keepdims=True) > 0, -n, n)
+    return x, n, d, prim
+
+
 # ------------------------------------------------------------------ LTC points
 def ltc_points(n, seed=0x5EED0003, n_quads=16, alpha_range=(0.02, 1.0)):
     """Shading points for the isolated LTC kernel: positions U[-1,1]^3,

@@ -209,6 +209,50 @@ chk_status chk_render_count(chk_scene_t scene, const chk_camera *camera,
                             const chk_render_params *params, double *accum,
                             chk_render_stats *stats, void *stream);
 
+/* ---- direct lighting: the reference's shadeDI estimators ----------------
+ * Shading points are the reference's (r, n, x, primitive) arguments:
+ *   x     hit point (k x 3 doubles)
+ *   n     unit normal facing the viewer, as intersectT returns it (k x 3)
+ *   d     unit direction of the ray that hit x, r.d (k x 3)
+ *   prim  original index of the triangle x lies on: its material is the
+ *         BasePrimitive's (k)
+ * All FP64 in and out, like the reference's Vec arguments.  Light sources
+ * are the scene's mesh lights.  Materials must be GGX, Beckmann, Phong or
+ * classic Phong (the reference has no sampler for the others). */
+
+/* MIS weights of given samples.  Replaces the weight half of
+ * MIS::getSamples (mis.cpp:18-95,209-220): weight = 1 / (sum over the
+ * sampling techniques of N_k p_k(sample) + 1e-20) with N = 10 per mesh light
+ * (area sampling, solid-angle pdf d^2 / (A |cos|) from Ray(x, sample)) and
+ * N = 60 for the BSDF (MaterialType::pdfEval).  wo_ref and wo as the
+ * reference passes them (lightSources.cpp:44-45,60).  samples: k x S x 3
+ * unit directions (point-major); weights: k x S.  All dev. */
+chk_status chk_mis_weights(chk_scene_t scene, const double *x, const double *n,
+                           const double *wo_ref, const double *wo, const uint32_t *prim,
+                           uint32_t k, const double *samples, uint32_t S, double *weights,
+                           void *stream);
+/* One estimate of direct lighting per shading point, the value the
+ * reference's estimator returns (before shadeDI multiplies the colour):
+ * chk_direct_params selects getLightFromMeshSource_Analytic, ..._AnalyticCV,
+ * getLightFromAllSources_MIS or ..._Analytic_MISCV.  Samples come from
+ * Philox (counter in chakum_types.h) in the reference's allocation and
+ * distributions; shadow rays are traced on the scene.  ANALYTIC, ANALYTIC_CV
+ * and MISCV need chk_set_ltc_table.  x, n, d, prim, rgb (k x 3): dev. */
+chk_status chk_direct_light(chk_scene_t scene, const double *x, const double *n,
+                            const double *d, const uint32_t *prim, uint32_t k,
+                            const chk_direct_params *params, double *rgb, void *stream);
+/* Render with the reference's shipped shade() = shadeDirectOnly
+ * (shader.cpp:339-363): camera ray, closest hit, an emitter's
+ * radiance.maxnorm() (or its radiance with CHK_RENDER_PHYSICAL_EMITTER),
+ * else colour times the selected direct-lighting estimator.  params: spp,
+ * sample_offset, jitter_mode, seed and flags as chk_render (max_depth and
+ * mis_mode unused); direct->estimate unused (the sample index takes its
+ * place).  accum, sumsq: as chk_render. */
+chk_status chk_render_direct(chk_scene_t scene, const chk_camera *camera,
+                             const chk_render_params *params, const chk_direct_params *direct,
+                             double *accum, double *sumsq, chk_render_stats *stats,
+                             void *stream);
+
 /* ---- batched requests ----------------------------------------------------
  * No single reference counterpart: the reference renders one image per
  * process (main.cpp:62-94).  Many render requests -- each with its own

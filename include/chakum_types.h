@@ -102,6 +102,41 @@ typedef struct chk_render_params {
  * (the sumsq buffer receives W*H*3 doubles). */
 #define CHK_RENDER_SUMSQ            2u
 
+/* Direct-lighting estimators of the reference's shadeDI (shader.cpp:100-109),
+ * one per line of its selection list. */
+enum chk_direct_estimator {
+  CHK_DIRECT_ANALYTIC = 0,     /* getLightFromMeshSource_Analytic (lightSources_CV.cpp:323-333) */
+  CHK_DIRECT_ANALYTIC_CV = 1,  /* getLightFromMeshSource_AnalyticCV (lightSources_CV.cpp:335-397) */
+  CHK_DIRECT_MIS = 2,          /* getLightFromAllSources_MIS (lightSources.cpp:32-109) */
+  CHK_DIRECT_MISCV = 3         /* getLightFromMeshSource_Analytic_MISCV (lightSources_CV.cpp:399-450),
+                                  the estimator shadeDI ships with */
+};
+
+/* MISCV: apply the shadow test that the reference computes and then
+ * ignores (its visibility is 1 either way, lightSources_CV.cpp:433-438). */
+#define CHK_DIRECT_VISIBILITY 1u
+
+typedef struct chk_direct_params {
+  uint32_t estimator;    /* chk_direct_estimator */
+  uint32_t cv_samples;   /* ANALYTIC_CV: samples per mesh light (its nSamples; capped at
+                            1000 like MAX_MESH_SAMPLES, lightSources.cpp:24) */
+  uint32_t flags;        /* CHK_DIRECT_* bits */
+  uint32_t estimate;     /* estimate index (second Philox counter word; point entry only) */
+  uint32_t seed_lo;      /* Philox key */
+  uint32_t seed_hi;
+} chk_direct_params;
+
+/* Philox counter of the direct-lighting estimators:
+ *   ctr = { point index (or pixel), estimate index (or sample), slot, CHK_STREAM_DIRECT }
+ * with one call per sample slot; a 32-bit output x maps to (x + 0.5) * 2^-32.
+ * Slots 10 j .. 10 j + 9 sample mesh light j by area, slots 10 L .. 10 L + 59
+ * the BSDF (the first floor(60 ks) from the specular lobe), as
+ * MIS::getSamples (mis.cpp:97-191); ANALYTIC_CV uses slots j n .. j n + n - 1
+ * for light j. */
+#define CHK_STREAM_DIRECT 0x44490000u
+#define CHK_MIS_LIGHT_SAMPLES 10u  /* nSuArIS, mis.h:9 */
+#define CHK_MIS_BSDF_SAMPLES 60u   /* nBrIS, mis.h:11 */
+
 /* Philox counter layout shared by every implementation of the path:
  *   ctr = { pixel index, sample index, (depth << 2) | stream, request id }
  *   key = { seed_lo, seed_hi }

@@ -762,6 +762,249 @@ inline void philox(const uint32_t *ctr, const uint32_t *key, uint32_t *out) {
 }
 inline double u01(uint32_t x) { return (double)(x >> 8) * (1.0 / 16777216.0); }
 
+// ---------------------------------------------------------------- direct lighting
+// The estimators of shadeDI (shader.cpp:100-109) over mesh lights.
+// Philox counter {point, estimate, slot, CHK_STREAM_DIRECT}; u = (x + 0.5) 2^-32.
+inline double ud(uint32_t x) { return ((double)x + 0.5) * (1.0 / 4294967296.0); }
+
+struct DLight {
+  uint32_t first, count;  // run of scene triangles
+  double area;            // MeshLight::area (sum of Triangle::area)
+  uint32_t inv[300];      // MeshLight::invCDF
+};
+
+struct DScene {
+  std::vector<double> td;
+  std::vector<Tri> T;
+  RefBvh *B = nullptr;
+  std::vector<Mat> mats;
+  std::vector<uint32_t> tri_mat;
+  std::vector<DLight> L;
+  ~DScene() { delete B; }
+};
+
+// MeshLight::initMeshLight (lightSources.cpp:784-807).
+void init_mesh_light(const DScene &S, DLight &L) {
+  L.area = 0;
+  for (uint32_t i = 0; i < L.count; ++i) L.area += S.T[L.first + i].area;
+  double cdf = 0;
+  uint32_t last = 0;
+  for (uint32_t i = 0; i < L.count; ++i) {
+    cdf += S.T[L.first + i].area / L.area;
+    uint32_t index = (uint32_t)(cdf * 300);
+    if (index >= 300) index = 299;
+    for (uint32_t j = last; j <= index; ++j) L.inv[j] = i;
+    last = index + 1;
+  }
+  for (uint32_t j = last; j < 300; ++j) L.inv[j] = L.count - 1;
+}
+
+// MeshLight::getSamples, one point (lightSources.cpp:809-828): e picks the
+// bin, e1 = sqrt(u), e2 = u.
+V3 mesh_light_point(const DScene &S, const DLight &L, const uint32_t *r, uint32_t *tri) {
+  double e = ud(r[0]), e1 = std::sqrt(ud(r[1])), e2 = ud(r[2]);
+  uint32_t index = (uint32_t)(e * 300);
+  index = index >= 300 ? 299 : index;
+  *tri = L.first + L.inv[index];
+  const Tri &t = S.T[*tri];
+  return t.A * (1.0 - e1) + t.B * e1 * (1.0 - e2) + t.C * e2 * e1;
+}
+
+// MeshLight::intersect (lightSources.cpp:830-847): brute force, last
+// triangle first, `(d = ...) && d < t`.
+bool mesh_light_isect(const DScene &S, const DLight &L, V3 o, V3 d, uint32_t *tri, double *t,
+                      V3 *N) {
+  *t = INF;
+  *tri = 0xFFFFFFFFu;
+  for (int i = (int)L.count; i--;) {
+    V3 Nl;
+    double dd = tri_isect(S.T[L.first + i], o, d, &Nl, nullptr);
+    if (dd != 0 && dd < *t) {
+      *t = dd;
+      *N = Nl;
+      *tri = L.first + (uint32_t)i;
+    }
+  }
+  return *t < INF;
+}
+
+// shadow() (shader.cpp:76-97): closest hit + 1e-5 < distance.
+bool occluded(const DScene &S, V3 o, V3 d, double dist) {
+  double t;
+  V3 N;
+  int rid;
+  return bvh_closest(*S.B, o, d, 1, &t, &N, &rid, nullptr, nullptr) && t + 1e-5 < dist;
+}
+
+struct DPoint {
+  V3 x, n, d;
+  uint32_t prim;
+};
+struct DCtx {
+  Mat m;
+  V3 wo, wo_ref;
+  bool apx;  // BRDFApprox::isSet (materialTypes.cpp:52-74: GGX with force)
+  M3 M, Minv;
+  double amp;
+  V3 G;
+};
+
+// Per-point part of the estimators: wo_ref (lightSources.cpp:44-45),
+// getBrdfApproxParam and the analytic term G (lightSources_CV.cpp:323-333).
+void direct_setup(const DScene &S, const DPoint &P, bool need_G, DCtx *C) {
+  C->m = S.mats[S.tri_mat[P.prim]];
+  C->wo = P.d * -1.0;
+  C->wo_ref = norm(P.n * 2.0 * dot(P.n, C->wo) + P.d);
+  C->apx = C->m.brdf == CHK_BRDF_GGX;
+  C->amp = 0;
+  if (C->apx) brdf_approx(P.n, C->wo, C->m.alpha, &C->M, &C->Minv, &C->amp);
+  C->G = v3(0, 0, 0);
+  if (!need_G) return;
+  V3 sl = v3(0, 0, 0), sd = v3(0, 0, 0);
+  for (const DLight &L : S.L)
+    for (uint32_t i = 0; i < L.count; ++i) {
+      const Tri &t = S.T[L.first + i];
+      V3 rad = S.mats[S.tri_mat[L.first + i]].rad;
+      if (C->apx) sl = sl + rad * ltc_tri(t.A, t.B, t.C, P.x, C->Minv);
+      sd = sd + rad * diffuse_tri(t.A, t.B, t.C, P.n, P.x);
+    }
+  // getLightAnalytic * amplitude * ks: an unset BRDFApprox has amplitude 0
+  V3 lt = v3(sl.x / (2.0 * PI), sl.y / (2.0 * PI), sl.z / (2.0 * PI));
+  V3 df = v3(sd.x / (2.0 * PI), sd.y / (2.0 * PI), sd.z / (2.0 * PI));
+  C->G = C->apx ? lt * C->amp * C->m.ks : v3(0, 0, 0);
+  C->G = C->G + df * (1.0 - C->m.ks);
+}
+
+// BasePrimitive::brdf with the LTC approximation when it is set
+// (materialTypes.cpp:9-11,76-87; geometryPrimitives.cpp:7-9).
+double brdf_apx(const DCtx &C, V3 n, V3 wi) {
+  if (!C.apx) return brdf(C.m, n, C.wo_ref, C.wo, wi);
+  V3 Lo = norm(mv(C.Minv, wi));
+  V3 L_ = mv(C.M, Lo);
+  double l = length(L_);
+  double J = C.M.det / (l * l * l);
+  double D = 1.0 / PI * std::max(0.0, Lo.z);
+  double res = C.amp * D / J;
+  return (C.m.ks * res / dot(wi, n) + (1 - C.m.ks) / PI) * C.m.refl;
+}
+
+inline void dphilox(uint32_t a, uint32_t b, uint32_t slot, const uint32_t *key, uint32_t *r) {
+  uint32_t c[4] = {a, b, slot, CHK_STREAM_DIRECT};
+  philox(c, key, r);
+}
+
+// MIS::getSamples sample set (mis.cpp:127-138,180-191).
+void mis_samples(const DScene &S, const DCtx &C, const DPoint &P, uint32_t a, uint32_t b,
+                 const uint32_t *key, V3 *out) {
+  uint32_t slot = 0, r[4], tri;
+  for (const DLight &L : S.L)
+    for (uint32_t q = 0; q < CHK_MIS_LIGHT_SAMPLES; ++q, ++slot) {
+      dphilox(a, b, slot, key, r);
+      out[slot] = norm(mesh_light_point(S, L, r, &tri) - P.x);
+    }
+  // getBrdfDirectionSamples (materialTypes.cpp:118-134): floor(60 ks)
+  // specular-lobe samples, then cosine samples
+  const uint32_t nb = (uint32_t)(CHK_MIS_BSDF_SAMPLES * C.m.ks);
+  for (uint32_t q = 0; q < CHK_MIS_BSDF_SAMPLES; ++q, ++slot) {
+    dphilox(a, b, slot, key, r);
+    double u[3] = {q < nb ? -1.0 : 2.0, ud(r[1]), ud(r[2])};
+    out[slot] = brdf_sample(C.m, P.n, C.wo_ref, C.wo, u);
+  }
+}
+
+// 1 / (sum_k N_k pdfVal_k(s) + 1e-20) (mis.cpp:34-95,209-220).
+double mis_weight(const DScene &S, const DCtx &C, const DPoint &P, V3 s) {
+  double denom = 0;
+  for (const DLight &L : S.L) {
+    uint32_t tri;
+    double d;
+    V3 N = v3(0, 0, 0);
+    double p = 0;
+    if (mesh_light_isect(S, L, P.x, s, &tri, &d, &N)) {  // Ray r(x, sample)
+      double c1 = dot(s, N);
+      c1 = c1 < 0 ? -c1 : c1;
+      p = ((1.0 / L.area) * d * d) / c1;
+    }
+    denom += (double)CHK_MIS_LIGHT_SAMPLES * p;
+  }
+  denom += (double)CHK_MIS_BSDF_SAMPLES * pdf_eval(C.m, P.n, C.wo_ref, C.wo, s);
+  return 1.0 / (denom + 1e-20);
+}
+
+V3 direct_estimate(const DScene &S, const chk_direct_params &p, const DPoint &P, uint32_t a,
+                   uint32_t b) {
+  const uint32_t key[2] = {p.seed_lo, p.seed_hi};
+  DCtx C;
+  direct_setup(S, P, p.estimator != CHK_DIRECT_MIS, &C);
+  if (p.estimator == CHK_DIRECT_ANALYTIC) return C.G;
+  const V3 o = P.x + P.n * 1e-8;  // rr.o (lightSources.cpp:42, lightSources_CV.cpp:346,412)
+  V3 sum = v3(0, 0, 0);
+  if (p.estimator == CHK_DIRECT_ANALYTIC_CV) {
+    const uint32_t n = std::min(p.cv_samples, 1000u);
+    for (uint32_t j = 0; j < S.L.size(); ++j) {
+      const DLight &L = S.L[j];
+      V3 s = v3(0, 0, 0);
+      const double pdf = L.area;  // getSamples returns 1/pdf
+      for (uint32_t q = 0; q < n; ++q) {
+        uint32_t r[4], tri;
+        dphilox(a, b, j * n + q, key, r);
+        V3 y = mesh_light_point(S, L, r, &tri);
+        V3 w = norm(y - P.x);
+        double d = length(y - P.x);
+        double vis = occluded(S, o, w, d) ? 0.0 : 1.0;
+        double cs = dot(w, P.n);
+        double c1 = dot(w, S.T[tri].n);
+        c1 = c1 < 0 ? -c1 : c1;
+        if (cs < 0) continue;
+        double fa = brdf(C.m, P.n, C.wo_ref, C.wo, w);
+        double fx = brdf_apx(C, P.n, w);
+        V3 rad = S.mats[S.tri_mat[tri]].rad;
+        s = s + rad * ((vis * fa - fx) * cs * c1 / (d * d));
+      }
+      sum = sum + s * (pdf / n);
+    }
+    return sum + C.G;
+  }
+  // MIS and MISCV share MIS::getSamples
+  const uint32_t Sn = (uint32_t)S.L.size() * CHK_MIS_LIGHT_SAMPLES + CHK_MIS_BSDF_SAMPLES;
+  std::vector<V3> smp(Sn);
+  mis_samples(S, C, P, a, b, key, smp.data());
+  for (uint32_t i = 0; i < Sn; ++i) {
+    const V3 w = smp[i];
+    const double wt = mis_weight(S, C, P, w);
+    const double cs = dot(w, P.n);
+    if (cs < 0) continue;
+    if (p.estimator == CHK_DIRECT_MIS) {
+      // lightSources.cpp:64-80: the first mesh light hit and not shadowed
+      const double thr = brdf(C.m, P.n, C.wo_ref, C.wo, w) * cs * wt;
+      for (const DLight &L : S.L) {
+        uint32_t tri;
+        double d;
+        V3 N = v3(0, 0, 0);
+        if (!mesh_light_isect(S, L, o, w, &tri, &d, &N)) continue;
+        if (occluded(S, o, w, d)) continue;
+        sum = sum + S.mats[S.tri_mat[tri]].rad * thr;
+        break;
+      }
+    } else {
+      // lightSources_CV.cpp:420-445: the first mesh light hit
+      for (const DLight &L : S.L) {
+        uint32_t tri;
+        double d;
+        V3 N = v3(0, 0, 0);
+        if (!mesh_light_isect(S, L, o, w, &tri, &d, &N)) continue;
+        double vis = 1.0;
+        if ((p.flags & CHK_DIRECT_VISIBILITY) && occluded(S, o, w, d)) vis = 0.0;
+        double fa = brdf(C.m, P.n, C.wo_ref, C.wo, w);
+        double fx = brdf_apx(C, P.n, w);
+        sum = sum + S.mats[S.tri_mat[tri]].rad * ((vis * fa - fx) * cs * wt);
+        break;
+      }
+    }
+  }
+  return p.estimator == CHK_DIRECT_MISCV ? sum + C.G : sum;
+}
+
 }  // namespace
 
 // ====================================================================== C ABI
@@ -1157,6 +1400,134 @@ uint64_t orc_render(const float *tris_f, uint32_t n_tris, const uint32_t *tri_ma
   delete B;
   if (n_rays) *n_rays = rays;
   return paths;
+}
+
+void *orc_direct_scene(const float *tris, uint32_t n_tris, const uint32_t *tri_mat,
+                       const chk_material *mats, uint32_t n_mats,
+                       const chk_mesh_light *lights, uint32_t n_lights) {
+  DScene *S = new DScene();
+  S->td.resize((size_t)n_tris * 9);
+  for (size_t i = 0; i < S->td.size(); ++i) S->td[i] = tris[i];
+  S->T.resize(n_tris);
+  for (uint32_t i = 0; i < n_tris; ++i) S->T[i] = make_tri(&S->td[9 * (size_t)i]);
+  S->B = bvh_build(S->td.data(), n_tris);
+  S->mats.resize(n_mats);
+  for (uint32_t i = 0; i < n_mats; ++i) S->mats[i] = to_mat(mats[i]);
+  S->tri_mat.assign(tri_mat, tri_mat + n_tris);
+  S->L.resize(n_lights);
+  for (uint32_t j = 0; j < n_lights; ++j) {
+    S->L[j].first = lights[j].first_tri;
+    S->L[j].count = lights[j].n_tris;
+    init_mesh_light(*S, S->L[j]);
+  }
+  return S;
+}
+
+void orc_direct_scene_free(void *s) { delete (DScene *)s; }
+
+void orc_mis_weights(void *s, uint32_t k, const double *x, const double *n,
+                     const double *wo_ref, const double *wo, const uint32_t *prim,
+                     const double *samples, uint32_t S, double *weights) {
+  const DScene &D = *(const DScene *)s;
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < (int64_t)k; ++i) {
+    DPoint P{ld(x + 3 * i), ld(n + 3 * i), v3(0, 0, 0), prim[i]};
+    DCtx C;
+    C.m = D.mats[D.tri_mat[P.prim]];
+    C.wo_ref = ld(wo_ref + 3 * i);
+    C.wo = ld(wo + 3 * i);
+    for (uint32_t q = 0; q < S; ++q)
+      weights[(size_t)i * S + q] = mis_weight(D, C, P, ld(samples + 3 * ((size_t)i * S + q)));
+  }
+}
+
+void orc_mis_samples(void *s, const double *x, const double *n, const double *wo_ref,
+                     const double *wo, uint32_t prim, uint32_t point, uint32_t estimate,
+                     uint32_t seed_lo, uint32_t seed_hi, double *samples) {
+  const DScene &D = *(const DScene *)s;
+  DPoint P{ld(x), ld(n), v3(0, 0, 0), prim};
+  DCtx C;
+  C.m = D.mats[D.tri_mat[prim]];
+  C.wo_ref = ld(wo_ref);
+  C.wo = ld(wo);
+  const uint32_t key[2] = {seed_lo, seed_hi};
+  const uint32_t Sn = (uint32_t)D.L.size() * CHK_MIS_LIGHT_SAMPLES + CHK_MIS_BSDF_SAMPLES;
+  std::vector<V3> out(Sn);
+  mis_samples(D, C, P, point, estimate, key, out.data());
+  for (uint32_t q = 0; q < Sn; ++q) st(samples + 3 * q, out[q]);
+}
+
+void orc_direct_light(void *s, const chk_direct_params *p, uint32_t k, const double *x,
+                      const double *n, const double *d, const uint32_t *prim, int n_threads,
+                      double *rgb) {
+  const DScene &D = *(const DScene *)s;
+#ifdef _OPENMP
+  if (n_threads > 0) omp_set_num_threads(n_threads);
+#endif
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int64_t i = 0; i < (int64_t)k; ++i) {
+    DPoint P{ld(x + 3 * i), ld(n + 3 * i), ld(d + 3 * i), prim[i]};
+    st(rgb + 3 * i, direct_estimate(D, *p, P, (uint32_t)i, p->estimate));
+  }
+}
+
+void orc_render_direct(void *s, const chk_camera *cam, const chk_render_params *P,
+                       const chk_direct_params *dp, uint32_t row0, uint32_t row1,
+                       uint32_t row_step, int n_threads, double *accum, double *sumsq) {
+  const DScene &D = *(const DScene *)s;
+  const uint32_t W = cam->width, H = cam->height;
+  V3 cpos = v3(cam->pos[0], cam->pos[1], cam->pos[2]);
+  V3 cdir = norm(v3(cam->dir[0], cam->dir[1], cam->dir[2]));
+  V3 cx = norm(cross(cdir, v3(cam->up[0], cam->up[1], cam->up[2])));
+  cx = cx * ((double)W / H) * std::tan(cam->fovx_deg * PI / 360.0);
+  V3 cy = norm(cross(cx, cdir)) * std::tan(cam->fovy_deg * PI / 360.0);
+  const uint32_t key[2] = {P->seed_lo, P->seed_hi};
+#ifdef _OPENMP
+  if (n_threads > 0) omp_set_num_threads(n_threads);
+#endif
+  if (row_step == 0) row_step = 1;
+  const int64_t n_rows = row1 > row0 ? ((int64_t)row1 - row0 + row_step - 1) / row_step : 0;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int64_t ri = 0; ri < n_rows; ++ri) {
+    const int64_t row = (int64_t)row0 + ri * row_step;
+    uint32_t y = H - 1 - (uint32_t)row;  // idx = (h - y - 1) * w + x (main.cpp:72)
+    for (uint32_t x = 0; x < W; ++x) {
+      uint32_t idx = (uint32_t)row * W + x;
+      double acc[3] = {0, 0, 0}, acc2[3] = {0, 0, 0};
+      for (uint32_t sm = P->sample_offset; sm < P->sample_offset + P->spp; ++sm) {
+        double jx = 0.34, jy = 0.20;  // main.cpp:23-24, k = 0
+        if (P->jitter_mode == CHK_JITTER_BOX) {
+          uint32_t c[4] = {idx, sm, (0u << 2) | CHK_STREAM_CAM, 0u}, r4[4];
+          philox(c, key, r4);
+          jx = u01(r4[0]);
+          jy = u01(r4[1]);
+        }
+        V3 dir = cx * ((double(x) + jx) / W - .5) + cy * ((double(y) + jy) / H - .5) + cdir;
+        V3 d = norm(dir);
+        V3 L = v3(0, 0, 0);
+        double t;
+        V3 Ng;
+        int rid;
+        // shadeDirectOnly (shader.cpp:339-363)
+        if (bvh_closest(*D.B, cpos, d, 1, &t, &Ng, &rid, nullptr, nullptr)) {
+          uint32_t prim = D.B->ordered[rid];
+          const Mat &m = D.mats[D.tri_mat[prim]];
+          if (m.light != CHK_LIGHT_NONE) {
+            L = (P->flags & CHK_RENDER_PHYSICAL_EMITTER) ? m.rad : maxnorm(m.rad);
+          } else {
+            DPoint Q{cpos + d * t, Ng, d, prim};
+            L = mul(m.c, direct_estimate(D, *dp, Q, idx, sm));  // shadeDI: c.mult(light)
+          }
+        }
+        acc[0] += L.x; acc[1] += L.y; acc[2] += L.z;
+        acc2[0] += L.x * L.x; acc2[1] += L.y * L.y; acc2[2] += L.z * L.z;
+      }
+      for (int c = 0; c < 3; ++c) {
+        accum[3 * (size_t)idx + c] += acc[c];
+        if (sumsq) sumsq[3 * (size_t)idx + c] += acc2[c];
+      }
+    }
+  }
 }
 
 }  // extern "C"

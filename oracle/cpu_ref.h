@@ -137,6 +137,38 @@ uint64_t orc_render(const float *tris, uint32_t n_tris, const uint32_t *tri_mat,
                     uint32_t row0, uint32_t row1, uint32_t row_step, int n_threads,
                     double *accum, double *sumsq, uint64_t *n_rays);
 
+/* ---- direct lighting (shadeDI's estimators) ------------------------------ */
+/* A scene for the estimators: triangles (FP32 in, FP64 math), materials,
+ * mesh lights with MeshLight::initMeshLight's 300-bin inverse CDF
+ * (lightSources.cpp:784-807) and the reference-semantics BVH for shadow().
+ * Needs orc_ltc_set before the estimators that use the LTC fit. */
+void *orc_direct_scene(const float *tris, uint32_t n_tris, const uint32_t *tri_mat,
+                       const chk_material *mats, uint32_t n_mats,
+                       const chk_mesh_light *lights, uint32_t n_lights);
+void orc_direct_scene_free(void *scene);
+/* MIS weights (mis.cpp:18-95,209-220) of S given samples per point
+ * (samples k x S x 3, weights k x S). */
+void orc_mis_weights(void *scene, uint32_t k, const double *x, const double *n,
+                     const double *wo_ref, const double *wo, const uint32_t *prim,
+                     const double *samples, uint32_t S, double *weights);
+/* The MIS::getSamples sample set of one point (mis.cpp:97-191) drawn with
+ * the Philox counter of chakum_types.h: S = 10 L + 60 unit directions. */
+void orc_mis_samples(void *scene, const double *x, const double *n, const double *wo_ref,
+                     const double *wo, uint32_t prim, uint32_t point, uint32_t estimate,
+                     uint32_t seed_lo, uint32_t seed_hi, double *samples);
+/* One estimate per point of getLightFromMeshSource_Analytic / _AnalyticCV /
+ * getLightFromAllSources_MIS / _Analytic_MISCV (lightSources_CV.cpp:323-450,
+ * lightSources.cpp:32-109) with shadow() of shader.cpp:76-97 over the
+ * reference BVH.  x, n, d (k x 3), prim (k), rgb (k x 3). */
+void orc_direct_light(void *scene, const chk_direct_params *p, uint32_t k, const double *x,
+                      const double *n, const double *d, const uint32_t *prim, int n_threads,
+                      double *rgb);
+/* shadeDirectOnly (shader.cpp:339-363) in the pixel loop of main.cpp:62-94:
+ * rows row0, row0 + row_step, ... < row1; accum/sumsq as orc_render. */
+void orc_render_direct(void *scene, const chk_camera *cam, const chk_render_params *P,
+                       const chk_direct_params *dp, uint32_t row0, uint32_t row1,
+                       uint32_t row_step, int n_threads, double *accum, double *sumsq);
+
 #ifdef __cplusplus
 }
 #endif

@@ -15,6 +15,13 @@
  *   M_GGX                                  ltc.h:18-61 (compiled into materialTypes.o)
  *   LightSource::getLightFromMeshSource_Analytic
  *                                          lightSources_CV.cpp:323-333
+ *   LightSource::getLightFromMeshSource_AnalyticCV
+ *                                          lightSources_CV.cpp:335-397
+ *   LightSource::getLightFromAllSources_MIS
+ *                                          lightSources.cpp:32-109
+ *   MIS::getSamples                        mis.cpp:97-223
+ *   shadow()                               shader.cpp:76-97
+ *   loadAccels, loadLightSampler           objects.cpp:38-56, mis.cpp:225-234
  */
 #include <stdint.h>
 #include <cstring>
@@ -25,6 +32,8 @@
 #include "materialTypes.h"
 #include "bvhAccel.h"
 #include "lightSources.h"
+#include "mis.h"
+#include "objects.h"
 
 // Defined in materialTypes.o (ltc.h is a definition-bearing header).
 void M_GGX(const float theta, const float alpha, mat3 &M, mat3 &Minv, double &amplitude);
@@ -172,6 +181,123 @@ void ref_ltc_shade(uint32_t k, const double *x, const double *n, const double *v
     rgb[3 * i] = G.x;
     rgb[3 * i + 1] = G.y;
     rgb[3 * i + 2] = G.z;
+  }
+}
+
+/* The reference's scene state for its direct-lighting estimators, set up
+ * the way its front end does (prepareScene.cpp:180-241 then loadAccels() and
+ * loadLightSampler()): scene triangles in vTriangleList, each mesh light as
+ * a MeshLight of TriLights (LightSource::addMSource also appends them to
+ * vTriangleList), the dummy sphere and dummy triangle.
+ *   tris   n_tris x 9 doubles; tri_mat: material per triangle
+ *   md     n_mats x 14: brdf, ks, alpha, phong_exp, int_ior, ext_ior,
+ *          reflectance, colour[3], radiance[3], light
+ *   lights n_lights x 2: first triangle, triangle count (emitters) */
+static std::vector<Triangle> g_prims;  // the BasePrimitive of every input triangle
+
+void ref_direct_setup(const double *tris, uint32_t n_tris, const uint32_t *tri_mat,
+                      const double *md, uint32_t n_mats, const uint32_t *lights,
+                      uint32_t n_lights) {
+  (void)n_mats;
+  vTriangleList.clear();
+  vSphereList.clear();
+  std::vector<int> is_light(n_tris, 0);
+  for (uint32_t j = 0; j < n_lights; ++j)
+    for (uint32_t i = 0; i < lights[2 * j + 1]; ++i) is_light[lights[2 * j] + i] = 1;
+  g_prims.assign(n_tris, Triangle());
+  for (uint32_t o = 0; o < n_tris; ++o) {
+    const double *p = tris + 9 * (size_t)o, *m = md + 14 * (size_t)tri_mat[o];
+    MaterialType mt = make_material((int)m[0], m[1], m[2], m[3], m[4], m[5]);
+    g_prims[o] = Triangle(Vec(p[0], p[1], p[2]), Vec(p[3], p[4], p[5]), Vec(p[6], p[7], p[8]),
+                          Vec(m[7], m[8], m[9]), m[6], mt);
+    if (!is_light[o]) vTriangleList.push_back(g_prims[o]);
+  }
+  lSource = new LightSource();
+  for (uint32_t j = 0; j < n_lights; ++j) {
+    MeshLight ml;
+    for (uint32_t i = 0; i < lights[2 * j + 1]; ++i) {
+      const uint32_t o = lights[2 * j] + i;
+      const double *p = tris + 9 * (size_t)o, *m = md + 14 * (size_t)tri_mat[o];
+      TriLight tl(Vec(p[0], p[1], p[2]), Vec(p[3], p[4], p[5]), Vec(p[6], p[7], p[8]),
+                  Vec(1, 1, 1), 1.0);
+      tl.radiance = Vec(m[10], m[11], m[12]);
+      ml.add(tl);
+    }
+    ml.initMeshLight();
+    lSource->addMSource(ml);
+  }
+  // prepareScene.cpp:238,241
+  vSphereList.push_back(Sphere(1, Vec(27, 1e5, 47), Vec(.999, .999, .999), 1.0,
+                               MaterialType(200.2, 1)));
+  vTriangleList.push_back(Triangle(1e6, 1e6, 1e6, Vec(0.0, 0, 0.9), 0.9, MaterialType(1.0, 0.5)));
+  loadAccels();
+  loadLightSampler();
+}
+
+/* MIS::getSamples for k points (primitive = the point's input triangle):
+ * samples (k x 200 x 3) and weights (k x 200); returns the sample count. */
+uint32_t ref_mis_get_samples(uint32_t k, const double *x, const double *n, const double *wo_ref,
+                             const double *wo, const uint32_t *prim, double *samples,
+                             double *weights) {
+  uint32_t S = 0;
+  for (uint32_t i = 0; i < k; ++i) {
+    Vec smp[200];
+    double w[200] = {0};
+    S = lightSampler->getSamples(Vec(x[3 * i], x[3 * i + 1], x[3 * i + 2]),
+                                 Vec(n[3 * i], n[3 * i + 1], n[3 * i + 2]),
+                                 Vec(wo_ref[3 * i], wo_ref[3 * i + 1], wo_ref[3 * i + 2]),
+                                 Vec(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]), &g_prims[prim[i]],
+                                 smp, w);
+    for (uint32_t q = 0; q < S && q < 200; ++q) {
+      double *o = samples + 3 * ((size_t)i * 200 + q);
+      o[0] = smp[q].x;
+      o[1] = smp[q].y;
+      o[2] = smp[q].z;
+      weights[(size_t)i * 200 + q] = w[q];
+    }
+  }
+  return S;
+}
+
+/* reps calls of one estimator per point (0: getLightFromMeshSource_Analytic,
+ * 1: _AnalyticCV with cv samples, 2: getLightFromAllSources_MIS); mean and
+ * the variance of the mean from `batches` batch means (the estimators seed
+ * their generators from clock(), so successive calls can share streams). */
+void ref_direct_stats(int est, uint32_t cv, uint32_t reps, uint32_t batches, uint32_t k,
+                      const double *x, const double *n, const double *d, const uint32_t *prim,
+                      double *mean, double *var) {
+  if (batches == 0 || batches > reps) batches = reps;
+  const uint32_t per = reps / batches;
+  for (uint32_t i = 0; i < k; ++i) {
+    Vec X(x[3 * i], x[3 * i + 1], x[3 * i + 2]), N(n[3 * i], n[3 * i + 1], n[3 * i + 2]),
+        D(d[3 * i], d[3 * i + 1], d[3 * i + 2]);
+    Ray r(X - D, D);
+    std::vector<Vec> bm(batches);
+    for (uint32_t b = 0; b < batches; ++b) {
+      Vec s;
+      for (uint32_t q = 0; q < per; ++q) {
+        Vec v = est == 0   ? lSource->getLightFromMeshSource_Analytic(r, N, X, &g_prims[prim[i]])
+                : est == 1 ? lSource->getLightFromMeshSource_AnalyticCV(r, N, X, &g_prims[prim[i]], cv)
+                           : lSource->getLightFromAllSources_MIS(r, N, X, &g_prims[prim[i]]);
+        s = s + v;
+      }
+      bm[b] = s * (1.0 / per);
+    }
+    double m[3] = {0, 0, 0}, v2[3] = {0, 0, 0};
+    for (uint32_t b = 0; b < batches; ++b) {
+      m[0] += bm[b].x;
+      m[1] += bm[b].y;
+      m[2] += bm[b].z;
+    }
+    for (int c = 0; c < 3; ++c) m[c] /= batches;
+    for (uint32_t b = 0; b < batches; ++b) {
+      const double dv[3] = {bm[b].x - m[0], bm[b].y - m[1], bm[b].z - m[2]};
+      for (int c = 0; c < 3; ++c) v2[c] += dv[c] * dv[c];
+    }
+    for (int c = 0; c < 3; ++c) {
+      mean[3 * i + c] = m[c];
+      var[3 * i + c] = batches > 1 ? v2[c] / (batches - 1) / batches : 0.0;
+    }
   }
 }
 

@@ -11,6 +11,16 @@ Fixtures (inputs and expected outputs only):
     brdf.npz          BasePrimitive::brdf and MaterialType::pdfEval
     ltc.npz           M_GGX and LightSource::getLightFromMeshSource_Analytic
     spheres.npz       BvhAccel::intersectS over Sphere primitives
+    direct.npz        MIS::getSamples (samples and weights),
+                      getLightFromMeshSource_Analytic, and the mean and
+                      batch-means variance of 8192 calls of
+                      getLightFromAllSources_MIS and
+                      getLightFromMeshSource_AnalyticCV (16 samples per light)
+                      at 32 shading points of the direct-lighting room
+                      (chakumrender_amd.scenes.direct_room; the scene arrays
+                      are stored too).  The reference seeds its samplers from
+                      clock(), so the two statistical rows differ from run to
+                      run within their variance.
 """
 from __future__ import annotations
 
@@ -210,6 +220,22 @@ def gen_ltc(R, rng):
                         x2=x2, n2=n2, v2=v2, alpha2=a2, ks2=ks2, tris2=t2, rad2=r2, rgb2=rgb2)
 
 
+def gen_direct():
+    sc = scenes.direct_room()
+    x, n, d, prim = scenes.shading_points(sc, 32)
+    wo = -d
+    wo_ref = unit(n * 2.0 * (n * wo).sum(1, keepdims=True) + d)  # lightSources.cpp:44-45
+    Rd = oracle.RefDirect(sc)
+    smp, w = Rd.mis_get_samples(x, n, wo_ref, wo, prim)
+    analytic, _ = Rd.stats(0, x, n, d, prim, reps=1, batches=1)
+    mis_m, mis_v = Rd.stats(2, x, n, d, prim, reps=8192, batches=64)
+    cv_m, cv_v = Rd.stats(1, x, n, d, prim, reps=8192, cv_samples=16, batches=64)
+    np.savez_compressed(OUT / "direct.npz", tris=sc.tris, tri_mat=sc.tri_mat, mats=sc.mats,
+                        lights=sc.lights, x=x, n=n, d=d, prim=prim, wo_ref=wo_ref, wo=wo,
+                        samples=smp, weights=w, analytic=analytic, mis_mean=mis_m, mis_var=mis_v,
+                        cv_mean=cv_m, cv_var=cv_v, cv_samples=16, reps=8192)
+
+
 def main():
     oracle.build(ref=True)
     R = oracle.ref_lib()
@@ -221,6 +247,7 @@ def main():
     gen_brdf(R, rng)
     gen_ltc(R, rng)
     gen_spheres(R, np.random.default_rng(20261204))  # own stream: earlier fixtures unchanged
+    gen_direct()
     for p in sorted(OUT.glob("*.npz")):
         print(p.name, p.stat().st_size)
 

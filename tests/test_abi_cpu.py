@@ -48,6 +48,10 @@ def test_null_arguments_rejected_without_device():
     assert L.chk_ltc_eval(None, None, None, None, None, None, 0, None, None, 0, None, None) == 1
     assert L.chk_brdf_eval(None, 0, None, None, None, None, None, 5, 0, None, None, None) == 1
     assert b"null" in L.chk_last_error()
+    assert L.chk_direct_light(None, None, None, None, None, 4, None, None, None) == 1
+    assert L.chk_mis_weights(None, None, None, None, None, None, 4, None, 80, None, None) == 1
+    assert L.chk_render_direct(None, None, None, None, None, None, None, None) == 1
+    assert b"null" in L.chk_last_error()
 
 
 def test_struct_layouts_match_header():
@@ -56,11 +60,15 @@ def test_struct_layouts_match_header():
     assert C.sizeof(abi.chk_material) == 56
     assert C.sizeof(abi.chk_camera) == 4 * 11 + 8
     assert C.sizeof(abi.chk_render_params) == 32
+    assert C.sizeof(abi.chk_direct_params) == 24
 
 
 @pytest.mark.parametrize("flag,val", [("CHK_RENDER_PHYSICAL_EMITTER", abi.RENDER_PHYSICAL_EMITTER),
                                       ("CHK_RENDER_SUMSQ", abi.RENDER_SUMSQ),
-                                      ("CHK_RENDER_PROFILE", abi.RENDER_PROFILE)])
+                                      ("CHK_RENDER_PROFILE", abi.RENDER_PROFILE),
+                                      ("CHK_DIRECT_VISIBILITY", abi.DIRECT_VISIBILITY),
+                                      ("CHK_MIS_LIGHT_SAMPLES", abi.MIS_LIGHT_SAMPLES),
+                                      ("CHK_MIS_BSDF_SAMPLES", abi.MIS_BSDF_SAMPLES)])
 def test_flag_values_match_header(flag, val):
     txt = "".join(p.read_text() for p in (ROOT / "include").glob("*.h"))
     m = re.search(rf"#define {flag}\s+(0x[0-9a-fA-F]+|\d+)u?", txt)
