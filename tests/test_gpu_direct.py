@@ -5,10 +5,10 @@ Philox samples.
 
 Bars:
   * MIS weights of the reference's own samples: 1e-10 relative (FP64).
-  * Same samples as the oracle: every estimator, per point, within 1e-7
+  * Same samples as the oracle: every estimator, per point, within 1e-8
     relative (FP64 arithmetic; the GPU's polygon edge term uses atan2 where
     the reference uses acos) except points whose visibility differs between
-    the FP32 traversal and the FP64 brute force (at most 5% of points).
+    the FP32 traversal and the FP64 brute force (at most 2% of points).
   * Means over many estimates against the reference's: 4.5 sigma.
   * Renders: the oracle's shadeDirectOnly per pixel on the same keys, 3 sigma
     per pixel from the summed squares."""
@@ -59,10 +59,10 @@ def test_same_samples_as_oracle(env, name):
         o = D.direct_light(P, x, n, d, prim)
         assert np.isfinite(g).all()
         rel = np.abs(g - o) / np.maximum(np.abs(o), 1e-6)
-        bad = (rel > 1e-7).any(1)
+        bad = (rel > 1e-8).any(1)
         print(f"{name} e={e}: points off {bad.sum()} / {len(prim)}, max rel {rel.max():.2e}, "
               f"median rel {np.median(rel):.2e}")
-        assert bad.mean() <= 0.05, np.nonzero(bad)[0][:10]
+        assert bad.mean() <= 0.02, np.nonzero(bad)[0][:10]
         # the differing points are visibility decisions, so the totals agree
         assert abs(g.sum() - o.sum()) <= 0.01 * abs(o.sum())
 
