@@ -6,7 +6,7 @@ geodesic icosphere on a ground plane with 8 quad lights, 1920 x 1080 at
 per vertex, FP32 shading with FP64 per-pixel accumulation.  Scene, BVH and
 accumulator are resident in HBM before the timed region.
 
-Prints one JSON line (see the driver contract in the repository notes):
+Prints one JSON line:
 metric/value/unit, roofline of the dominant kernel (closest-hit traversal)
 measured with HIP events on the render stream over the timed region, and a
 CPU baseline (the FP64 oracle on the host cores, a bounded row sample).
@@ -70,6 +70,9 @@ def parse():
     ap.add_argument("--batch-scenes", type=int, default=16)
     ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--no-ltc", action="store_true")
+    ap.add_argument("--no-direct", action="store_true")
+    ap.add_argument("--direct-spp", type=int, default=1)
+    ap.add_argument("--direct-cpu-row-step", type=int, default=2)
     ap.add_argument("--ltc-points", type=int, default=4 * 1024 * 1024)
     ap.add_argument("--ltc-quads", type=int, default=16)
     return ap.parse_args()
@@ -145,7 +148,9 @@ def batch_leg(args, ctx):
 def ltc_leg(args, ctx):
     """Config 3: chk_ltc_eval on ltc_points points x ltc_quads quads
     (ks = 1, the LTC part), FP64 arithmetic; pairs/s and the FP64 VALU
-    roofline at SURVEY's 180 FLOP per (point, quad) pair."""
+    roofline at 180 FLOP per (point, quad) pair (unclipped quad: corner
+    transforms 72, normalisation 36, edge terms 60, combine 4, per-point
+    setup ~6; add/mul/compare 1, FMA 2, div/sqrt/acos/sin 1)."""
     import numpy as np
 
     from chakumrender_amd import abi, host, scenes
@@ -176,6 +181,57 @@ def ltc_leg(args, ctx):
                      "peak": 78.6, "unit": "TFLOP/s", "frac": round(flop / dt / 1e12 / 78.6, 4),
                      "work_model": "180 FLOP per (point, quad) pair"},
     }
+
+
+def direct_leg(args, ctx, sc, ds):
+    """The reference's shipped shade() (shadeDirectOnly with
+    getLightFromMeshSource_Analytic_MISCV, shader.cpp:100-109,339-363) and the
+    MIS estimator (getLightFromAllSources_MIS) through chk_render_direct on
+    the main scene: 10 area samples per mesh light + 60 BSDF samples per
+    camera hit, FP64.  The oracle's FP64 restatement of the same render runs
+    beside it on every direct_cpu_row_step-th row."""
+    import numpy as np
+
+    from chakumrender_amd import abi, host
+
+    out = {"workload": f"icosphere {sc.n_tris} tris, {sc.lights.shape[0]} mesh lights, "
+                       f"{args.width}x{args.height} {args.direct_spp} spp, shadeDirectOnly; "
+                       f"{10 * sc.lights.shape[0] + 60} MIS samples per camera hit, FP64"}
+    params = dict(spp=args.direct_spp, seed_lo=0x5EED0009)
+    image = host.Image(args.width, args.height)
+    for name, est, flags in (("miscv", abi.DIRECT_MISCV, 0), ("mis", abi.DIRECT_MIS, 0)):
+        dp = host.direct_params(est, flags=flags)
+        host.render_direct(ds, params, dp, image=image, download=False)  # warm-up
+        abi.sync()
+        reps = 2
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            _, st = host.render_direct(ds, params, dp, image=image, download=False)
+        abi.sync()
+        dt = (time.perf_counter() - t0) / reps
+        paths = args.width * args.height * args.direct_spp
+        row = {"ms": round(dt * 1e3, 2), "mpaths_per_s": round(paths / dt / 1e6, 3),
+               "samples_per_s_g": round(paths * (10 * sc.lights.shape[0] + 60) / dt / 1e9, 3),
+               "shadow_rays_per_s_g": round(st["rays_shadow"] / dt / 1e9, 3),
+               "chunks": st["chunks"]}
+        if not args.no_cpu_baseline:
+            import oracle
+
+            D = oracle.DirectScene(sc)
+            threads = min(16, os.cpu_count() or 1)
+            t1 = time.perf_counter()
+            D.render_direct(params, dp, row_step=args.direct_cpu_row_step, threads=threads)
+            ct = time.perf_counter() - t1
+            rows = len(range(0, args.height, args.direct_cpu_row_step))
+            cpaths = rows * args.width * args.direct_spp
+            row["cpu_baseline"] = {"value": round(cpaths / ct / 1e6, 4), "unit": "Mpaths/s",
+                                   "cores": threads, "kind": "port",
+                                   "sample": f"every {args.direct_cpu_row_step}th row "
+                                             f"({rows} rows x {args.width} px x {args.direct_spp} spp)",
+                                   "seconds": round(ct, 2)}
+            row["vs_cpu"] = round(row["mpaths_per_s"] / row["cpu_baseline"]["value"], 1)
+        out[name] = row
+    return out
 
 
 def progressive(render_pass, ref, pass_spp, max_spp, target, sync, W, H):
@@ -304,8 +360,8 @@ def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print(json.dumps({"error": "this build measures one GPU; --gpus above 1 is not supported "
-                                   "(a render shards by pixel or sample range, see DESIGN.md)"}))
+        print(json.dumps({"error": "the benchmark runs on one GPU; --gpus above 1 is not supported "
+                                   "(a render shards by pixel or sample range across GPUs)"}))
         sys.exit(2)
 
     from chakumrender_amd import abi, host, scenes
@@ -414,6 +470,8 @@ def main():
         line["batch"] = batch_leg(args, ctx)
     if not args.no_ltc:
         line["ltc"] = ltc_leg(args, ctx)
+    if not args.no_direct:
+        line["direct"] = direct_leg(args, ctx, sc, ds)
 
     if not args.no_cpu_baseline:
         import oracle
