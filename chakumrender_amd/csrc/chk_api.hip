@@ -273,6 +273,7 @@ chk_status chk_scene_create(chk_ctx_t ctx, const float *tris, uint32_t n_tris,
   std::vector<LightTri> ltris;
   std::vector<LightRec> lrecs;
   std::vector<double> larea64;
+  std::vector<LightTri64> ltri64;
   uint32_t used_lights = 0;
   for (uint32_t j = 0; j < n_lights; ++j) {
     LightRec R;
@@ -316,6 +317,27 @@ chk_status chk_scene_create(chk_ctx_t ctx, const float *tris, uint32_t n_tris,
       T.c_leaf = make_float4(p[6], p[7], p[8], lf);
       T.rad = make_float4(mt.radiance[0], mt.radiance[1], mt.radiance[2], (float)bin_end);
       ltris.push_back(T);
+      {
+        LightTri64 Q;
+        double v1[3], v2[3];
+        for (int c = 0; c < 3; ++c) {
+          Q.a[c] = p[c];
+          v1[c] = Q.v1[c] = (double)p[3 + c] - p[c];
+          v2[c] = Q.v2[c] = (double)p[6 + c] - p[c];
+        }
+        // (C-A) x (B-A), normalised as Vec::norm: v * (1 / sqrt(v.v))
+        const double nx = v2[1] * v1[2] - v2[2] * v1[1], ny = v2[2] * v1[0] - v2[0] * v1[2],
+                     nz = v2[0] * v1[1] - v2[1] * v1[0];
+        const double inv = 1.0 / std::sqrt(nx * nx + ny * ny + nz * nz);
+        Q.n[0] = nx * inv;
+        Q.n[1] = ny * inv;
+        Q.n[2] = nz * inv;
+        Q.b0 = v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2];
+        Q.b1 = v1[0] * v2[0] + v1[1] * v2[1] + v1[2] * v2[2];
+        Q.c1 = v2[0] * v2[0] + v2[1] * v2[1] + v2[2] * v2[2];
+        Q.den = Q.b1 * Q.b1 - Q.c1 * Q.b0;
+        ltri64.push_back(Q);
+      }
       ++R.count;
     }
     R.area = (float)area;
@@ -341,7 +363,8 @@ chk_status chk_scene_create(chk_ctx_t ctx, const float *tris, uint32_t n_tris,
   }
   const size_t b_oi = align256(oinfo.size() * 4);
   const size_t b_la = align256(std::max<size_t>(larea64.size(), 1) * sizeof(double));
-  const size_t total = b_nodes + b_tris + b_info + b_mats + b_lt + b_lr + b_oi + b_la;
+  const size_t b_l64 = align256(std::max<size_t>(ltri64.size(), 1) * sizeof(LightTri64));
+  const size_t total = b_nodes + b_tris + b_info + b_mats + b_lt + b_lr + b_oi + b_la + b_l64;
   void *blob = nullptr;
   if (hipMalloc(&blob, total) != hipSuccess) {
     set_error("chk_scene_create: cannot allocate " + std::to_string(total) + " device bytes");
@@ -381,6 +404,10 @@ chk_status chk_scene_create(chk_ctx_t ctx, const float *tris, uint32_t n_tris,
   sc->d.light_area = (const double *)p;
   if (e == hipSuccess && !larea64.empty())
     e = hipMemcpy(p, larea64.data(), larea64.size() * sizeof(double), hipMemcpyHostToDevice);
+  p += b_la;
+  sc->d.light64 = (const LightTri64 *)p;
+  if (e == hipSuccess && !ltri64.empty())
+    e = hipMemcpy(p, ltri64.data(), ltri64.size() * sizeof(LightTri64), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     set_error(std::string("chk_scene_create: upload: ") + hipGetErrorString(e));
     (void)hipFree(blob);

@@ -52,11 +52,11 @@ struct PointRec {
 __device__ inline double ud(uint32_t x) { return ((double)x + 0.5) * (1.0 / 4294967296.0); }
 __device__ inline d3 f4d(float4 v) { return dk(v.x, v.y, v.z); }
 
-// Triangle::intersect (geometryPrimitives.cpp:29-71) on FP64 vertices with
-// the constructor's unit normal (C-A)x(B-A); N receives the ray-facing one.
-__device__ inline double tri_isect64(d3 A, d3 B, d3 C, d3 o, d3 d, d3 *N) {
-  const d3 nn = cross(C - A, B - A);
-  const d3 n = nn * (1.0 / sqrt(dot(nn, nn)));
+// Triangle::intersect (geometryPrimitives.cpp:29-71) in FP64 with the
+// triangle's constants precomputed (LightTri64): the reference's operations
+// without recomputing the normal and the edge dot products per call.
+__device__ inline double tri_isect64p(const LightTri64 &T, d3 o, d3 d, d3 *N) {
+  const d3 n = dk(T.n[0], T.n[1], T.n[2]);
   double rn = dot(d, n);
   d3 Nf = n;
   if (rn > 0) {
@@ -65,15 +65,14 @@ __device__ inline double tri_isect64(d3 A, d3 B, d3 C, d3 o, d3 d, d3 *N) {
   }
   *N = Nf;
   if (rn * rn <= 1e-10) return kInf64;
+  const d3 A = dk(T.a[0], T.a[1], T.a[2]);
   const double t = dot(A - o, Nf) / rn;
   if (t < 0) return kInf64;
-  const d3 P = o + d * t;
-  const d3 v0 = P - A, v1 = B - A, v2 = C - A;
-  const double k0 = dot(v0, v1), k1 = dot(v0, v2);
-  const double b0 = dot(v1, v1), b1 = dot(v1, v2);
-  const double c0 = dot(v2, v1), c1 = dot(v2, v2);
-  const double gamma = (k0 * b1 - k1 * b0) / (c0 * b1 - c1 * b0);
-  const double beta = (k0 - gamma * c0) / b0;
+  const d3 v0 = (o + d * t) - A;
+  const double k0 = dot(v0, dk(T.v1[0], T.v1[1], T.v1[2]));
+  const double k1 = dot(v0, dk(T.v2[0], T.v2[1], T.v2[2]));
+  const double gamma = (k0 * T.b1 - k1 * T.b0) / T.den;
+  const double beta = (k0 - gamma * T.b1) / T.b0;  // c0 = v2.v1 = b1
   if (gamma + beta > 1) return kInf64;
   if (0 > gamma || gamma > 1) return kInf64;
   if (0 > beta || beta > 1) return kInf64;
@@ -88,9 +87,8 @@ __device__ inline bool light_isect(const DeviceScene &S, uint32_t j, d3 o, d3 d,
   *t = kInf64;
   *tri = kNone;
   for (int i = (int)L.count; i--;) {
-    const LightTri &lt = S.light_tris[L.first + i];
     d3 Nl;
-    const double dd = tri_isect64(f4d(lt.a_cdf), f4d(lt.b_area), f4d(lt.c_leaf), o, d, &Nl);
+    const double dd = tri_isect64p(S.light64[L.first + i], o, d, &Nl);
     if (dd != 0 && dd < *t) {
       *t = dd;
       *N = Nl;
@@ -299,10 +297,8 @@ __global__ __launch_bounds__(kDBlock) void k_dl_samples(DeviceScene S, const Poi
     const d3 dv = y - R.x;
     const double d = sqrt(dot(dv, dv));
     const double cs = dot(w, R.n);
-    const LightTri &lt = S.light_tris[tri];
-    const d3 A = f4d(lt.a_cdf), B = f4d(lt.b_area), C = f4d(lt.c_leaf);
-    const d3 tn = normalize(cross(C - A, B - A));
-    double c1 = dot(w, tn);
+    const LightTri64 &q = S.light64[tri];
+    double c1 = dot(w, dk(q.n[0], q.n[1], q.n[2]));  // mesh[ids[i]].t.n
     c1 = c1 < 0 ? -c1 : c1;
     if (cs < 0) {
       no_ray(I.ray, i);

@@ -49,6 +49,14 @@ struct LightTri {
                    // reference's 300-bin inverse CDF that maps to this
                    // triangle (MeshLight::initMeshLight, lightSources.cpp:784-807)
 };
+// FP64 constants of one light triangle for Triangle::intersect
+// (geometryPrimitives.cpp:29-71), from its FP32 vertices: A, v1 = B - A,
+// v2 = C - A, the unit normal (C-A)x(B-A), b0 = v1.v1, b1 = v1.v2 (= c0),
+// c1 = v2.v2 and den = c0 b1 - c1 b0.
+struct LightTri64 {
+  double a[3], v1[3], v2[3], n[3];
+  double b0, b1, c1, den;
+};
 struct LightRec {
   uint32_t first;  // into the LightTri table
   uint32_t count;
@@ -63,6 +71,7 @@ struct DeviceScene {
   const chk_material *mats = nullptr;
   const LightTri *light_tris = nullptr;
   const LightRec *lights = nullptr;
+  const LightTri64 *light64 = nullptr;  // parallel to light_tris
   const uint2 *orig_info = nullptr;    // per original triangle: (leaf or 0xFFFFFFFF, material)
   const double *light_area = nullptr;  // per mesh light: MeshLight::area in FP64
   uint32_t n_lights = 0;
