@@ -99,6 +99,9 @@ TRAIN_FREEZE_MIX, TRAIN_FREEZE_LIGHTS, TRAIN_LIGHTS_CHI2 = 1, 2, 4
 DIRECT_ANALYTIC, DIRECT_ANALYTIC_CV, DIRECT_MIS, DIRECT_MISCV = 0, 1, 2, 3
 DIRECT_VISIBILITY = 0x1
 MIS_LIGHT_SAMPLES, MIS_BSDF_SAMPLES = 10, 60
+SHADE_DIRECT_ONLY, SHADE_EXPLICIT, SHADE_IMPLICIT = 0, 1, 2
+SHADE_MAX_TRIES = 64
+SHADE_STREAM = 0x53480000  # CHK_STREAM_SHADE
 
 # Every exported symbol of include/chakum_c.h: name -> (restype, argtypes)
 _vp = C.c_void_p
@@ -143,6 +146,9 @@ SIGNATURES = {
     "chk_render_direct": (C.c_int, [_vp, C.POINTER(chk_camera), C.POINTER(chk_render_params),
                                     C.POINTER(chk_direct_params), _vp, _vp,
                                     C.POINTER(chk_render_stats), _vp]),
+    "chk_render_shade": (C.c_int, [_vp, C.POINTER(chk_camera), C.POINTER(chk_render_params), _u32,
+                                   C.POINTER(chk_direct_params), _vp, _vp,
+                                   C.POINTER(chk_render_stats), _vp]),
     "chk_proposal_create": (C.c_int, [_vp, _u32, C.c_uint64, C.POINTER(_vp)]),
     "chk_proposal_destroy": (C.c_int, [_vp]),
     "chk_proposal_set_params": (C.c_int, [_vp, _vp, C.c_size_t]),
@@ -204,9 +210,13 @@ def params_struct(p) -> chk_render_params:
     r.max_depth = int(d.get("max_depth", 4))
     r.mis_mode = int(d.get("mis_mode", MIS_BALANCE))
     r.jitter_mode = int(d.get("jitter_mode", JITTER_BOX))
-    seed = int(d.get("seed", 0x5EED))
-    r.seed_lo = seed & 0xFFFFFFFF
-    r.seed_hi = (seed >> 32) & 0xFFFFFFFF
+    if "seed_lo" in d or "seed_hi" in d:
+        r.seed_lo = int(d.get("seed_lo", 0)) & 0xFFFFFFFF
+        r.seed_hi = int(d.get("seed_hi", 0)) & 0xFFFFFFFF
+    else:
+        seed = int(d.get("seed", 0x5EED))
+        r.seed_lo = seed & 0xFFFFFFFF
+        r.seed_hi = (seed >> 32) & 0xFFFFFFFF
     r.flags = int(d.get("flags", 0))
     return r
 

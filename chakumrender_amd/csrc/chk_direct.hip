@@ -45,6 +45,8 @@ struct PointRec {
   d3 G;            // analytic term
   d3 pre;          // value of an inactive point (render: emitter or miss)
   uint32_t ca, cb; // Philox counter words 0 and 1
+  uint32_t cd;     // path vertex (added to the counter's stream word)
+  uint32_t path;   // the shader path this point belongs to
   uint32_t mat, leaf;
   uint32_t flags, nb;  // nb = floor(60 ks): specular-lobe BSDF slots
 };
@@ -209,6 +211,8 @@ __global__ __launch_bounds__(kDBlock) void k_dl_setup(
     mat = oi.y;
     ok = sampler_ok(S.mats[mat].brdf);
   }
+  R.cd = 0;
+  R.path = p;
   if (!ok) {
     atomicOr(err, 1u);
     R.flags = 0;
@@ -287,7 +291,7 @@ __global__ __launch_bounds__(kDBlock) void k_dl_samples(DeviceScene S, const Poi
     return;
   }
   const MatT<double> m = load_mat<double>(S.mats + R.mat);
-  const u4 r = philox(u4{R.ca, R.cb, slot, CHK_STREAM_DIRECT}, key0, key1);
+  const u4 r = philox(u4{R.ca, R.cb, slot, CHK_STREAM_DIRECT + R.cd}, key0, key1);
   if (EST == CHK_DIRECT_ANALYTIC_CV) {
     // lightSources_CV.cpp:357-385
     const uint32_t j = slot / cvn;
@@ -519,6 +523,8 @@ __global__ __launch_bounds__(kDBlock) void k_dr_points(DeviceScene S, const doub
   PointRec &R = pts[it];
   R.ca = pix0 + it / spp;
   R.cb = s0 + it % spp;
+  R.cd = 0;
+  R.path = it;
   R.flags = 0;
   R.pre = dk(0, 0, 0);
   const float4 h = hits[it];
@@ -570,6 +576,295 @@ __global__ __launch_bounds__(kDBlock) void k_dr_accum(const double *__restrict__
   }
 }
 
+// ------------------------------------------------------------ shaders
+// shadeExplicit and shadeImplicit (shader.cpp:111-207,276-337) as path
+// wavefronts: one path per (pixel, sample) of the chunk.
+enum PathState : uint32_t { kDead = 0, kReady = 1, kTracing = 2 };
+struct PathSt {
+  d3 T, col;     // throughput (prod of TP / (1 - q), or shadeImplicit's product), colour
+  d3 o, d;       // FP64 ray of the pending trace
+  float4 hit;    // the accepted hit of the next vertex (t, leaf)
+  uint32_t state, tries, pk, pad;
+};
+
+constexpr double kRR = 0.3;  // terminationProbability (shader.cpp:112)
+
+__device__ inline u4 shade_rng(uint32_t pix, uint32_t smp, uint32_t w, uint32_t k0, uint32_t k1) {
+  return philox(u4{pix, smp, w, CHK_STREAM_SHADE}, k0, k1);
+}
+// getHemiDirectionSamples' distribution (uniform, domainSampler.cpp:88-99), 1/pdf = 2 pi
+__device__ inline d3 hemi_dir(d3 n, double e1, double e2) {
+  const double ct = e1, st = sqrt(fmax(0.0, 1.0 - e1 * e1));
+  double sp, cp;
+  sincos(2.0 * kPi64 * e2, &sp, &cp);
+  d3 t, b;
+  onb(n, t, b);
+  return t * (st * cp) + b * (st * sp) + n * ct;
+}
+// getCosineDirectionSamples' distribution (domainSampler.cpp:154-189), 1/pdf = pi
+__device__ inline d3 cosine_dir(d3 n, double e1, double e2) {
+  const double st = sqrt(e1), ct = sqrt(1.0 - e1);
+  double sp, cp;
+  sincos(2.0 * kPi64 * e2, &sp, &cp);
+  d3 t, b;
+  onb(n, t, b);
+  return t * (st * cp) + b * (st * sp) + n * ct;
+}
+__device__ inline void put_ray(float4 *__restrict__ rays, uint32_t p, d3 o, d3 d, uint32_t skip) {
+  rays[2 * (size_t)p] = make_float4((float)o.x, (float)o.y, (float)o.z, kInf);
+  rays[2 * (size_t)p + 1] = make_float4((float)d.x, (float)d.y, (float)d.z, __uint_as_float(skip));
+}
+__device__ inline void kill_ray(float4 *__restrict__ rays, uint32_t p) {
+  rays[2 * (size_t)p].w = -1.0f;
+}
+__device__ inline bool is_emitter(const DeviceScene &S, float4 h, uint32_t *mat) {
+  const uint32_t leaf = __float_as_uint(h.y);
+  *mat = S.leaf_info[leaf].x;
+  return S.mats[*mat].light != CHK_LIGHT_NONE;
+}
+__device__ inline bool missed(const DeviceScene &S, float4 h) {
+  return !(h.x < kInf) || __float_as_uint(h.y) >= S.n_leaf_tris;
+}
+// an emitter's value as the shaders return it (materialTypes.h:34-38)
+__device__ inline d3 emitter_value(const chk_material &mt, bool physical) {
+  d3 r = dk(mt.radiance[0], mt.radiance[1], mt.radiance[2]);
+  if (physical) return r;
+  double mx = r.x;
+  if (mx < r.y) mx = r.y;
+  if (mx < r.z) mx = r.z;
+  return dk(r.x / mx, r.y / mx, r.z / mx);
+}
+// the facing unit normal of leaf triangle `leaf` for direction d
+__device__ inline d3 facing_normal(const DeviceScene &S, uint32_t leaf, d3 d) {
+  const float4 *tp = S.tris + 3 * (size_t)leaf;
+  const d3 A = f4d(tp[0]), B = f4d(tp[1]), C = f4d(tp[2]);
+  const d3 nn = cross(C - A, B - A);
+  d3 n = nn * (1.0 / sqrt(dot(nn, nn)));
+  if (dot(d, n) > 0) n = n * -1.0;
+  return n;
+}
+
+// Paths start at the camera: T = 1, col = 0, tracing the camera ray.
+__global__ __launch_bounds__(kDBlock) void k_sh_init(CamD cam, uint32_t P, const double *__restrict__ dirs,
+                                                     PathSt *__restrict__ st) {
+  const uint32_t p = blockIdx.x * kDBlock + threadIdx.x;
+  if (p >= P) return;
+  PathSt &s = st[p];
+  s.T = dk(1, 1, 1);
+  s.col = dk(0, 0, 0);
+  s.o = dk(cam.pos[0], cam.pos[1], cam.pos[2]);
+  s.d = dk(dirs[3 * (size_t)p], dirs[3 * (size_t)p + 1], dirs[3 * (size_t)p + 2]);
+  s.state = kTracing;
+  s.tries = 0;
+  s.pk = 0;
+}
+
+// shadeExplicit's first hit (shader.cpp:128-140): a miss is 0, an emitter
+// its value, anything else the first vertex.
+__global__ __launch_bounds__(kDBlock) void k_ex_camera_hit(DeviceScene S, uint32_t P, bool physical,
+                                                           const float4 *__restrict__ hits,
+                                                           PathSt *__restrict__ st,
+                                                           float4 *__restrict__ rays) {
+  const uint32_t p = blockIdx.x * kDBlock + threadIdx.x;
+  if (p >= P) return;
+  PathSt &s = st[p];
+  kill_ray(rays, p);  // traced again only once a continuation is drawn
+  const float4 h = hits[p];
+  uint32_t mat;
+  if (missed(S, h)) {
+    s.state = kDead;
+  } else if (is_emitter(S, h, &mat)) {
+    s.col = emitter_value(S.mats[mat], physical);
+    s.state = kDead;
+  } else {
+    s.hit = h;
+    s.state = kReady;
+  }
+}
+
+// Every path at a new vertex becomes a shading point (appended, one atomic
+// per wave; the order of points does not change any path's result).
+__global__ __launch_bounds__(kDBlock) void k_ex_gather(DeviceScene S, const double *__restrict__ ltc,
+                                                       uint32_t P, uint32_t pix0, uint32_t spp,
+                                                       uint32_t s0, uint32_t vertex, bool need_G,
+                                                       PathSt *__restrict__ st,
+                                                       PointRec *__restrict__ pts,
+                                                       uint32_t *__restrict__ count) {
+  const uint32_t p = blockIdx.x * kDBlock + threadIdx.x;
+  const bool act = p < P && st[p].state == kReady;
+  const unsigned long long b = __ballot(act);
+  if (b == 0) return;
+  const uint32_t lane = __lane_id();
+  const uint32_t first = (uint32_t)(__ffsll((long long)b) - 1);
+  uint32_t base = 0;
+  if (lane == first) base = atomicAdd(count, (uint32_t)__popcll(b));
+  base = __shfl(base, (int)first);
+  if (!act) return;
+  const uint32_t k = base + (uint32_t)__popcll(b & ((1ull << lane) - 1ull));
+  PathSt &s = st[p];
+  const uint32_t leaf = __float_as_uint(s.hit.y);
+  PointRec &R = pts[k];
+  R.ca = pix0 + p / spp;
+  R.cb = s0 + p % spp;
+  R.cd = vertex;
+  R.path = p;
+  setup_point(S, ltc, s.o + s.d * (double)s.hit.x, facing_normal(S, leaf, s.d), s.d,
+              S.leaf_info[leaf].x, leaf, need_G, R);
+  s.pk = k;
+}
+
+// After shadeDI at this vertex (val[k] = c * estimate): add it, play
+// Russian roulette, and draw the first continuation direction
+// (shader.cpp:146-160).
+__global__ __launch_bounds__(kDBlock) void k_ex_cont(uint32_t n, uint32_t vertex, uint32_t key0,
+                                                     uint32_t key1, const PointRec *__restrict__ pts,
+                                                     const double *__restrict__ val,
+                                                     PathSt *__restrict__ st,
+                                                     float4 *__restrict__ rays) {
+  const uint32_t k = blockIdx.x * kDBlock + threadIdx.x;
+  if (k >= n) return;
+  const PointRec &R = pts[k];
+  PathSt &s = st[R.path];
+  const d3 di = dk(val[3 * (size_t)k], val[3 * (size_t)k + 1], val[3 * (size_t)k + 2]);
+  s.col = s.col + s.T * di;
+  const double e = ud(shade_rng(R.ca, R.cb, (vertex << 8) | 0xFFu, key0, key1).x);
+  if (e < kRR || vertex + 1 >= 100) {
+    s.state = kDead;
+    kill_ray(rays, R.path);
+    return;
+  }
+  const u4 r = shade_rng(R.ca, R.cb, vertex << 8, key0, key1);
+  const d3 w = hemi_dir(R.n, ud(r.x), ud(r.y));
+  s.o = R.x + R.n * 1e-6;  // secondaryRay.o (shader.cpp:154)
+  s.d = w;
+  s.tries = 0;
+  s.state = kTracing;
+  put_ray(rays, R.path, s.o, w, R.leaf);
+}
+
+// The continuation's trace came back: a miss ends the path (TP = 0), an
+// emitter draws again, anything else is accepted with
+// TP = c * (2 pi cos brdf) and T *= TP / (1 - q) (shader.cpp:162-201).
+__global__ __launch_bounds__(kDBlock) void k_ex_retry(DeviceScene S, uint32_t P, uint32_t pix0,
+                                                      uint32_t spp, uint32_t s0, uint32_t vertex,
+                                                      uint32_t key0, uint32_t key1,
+                                                      const float4 *__restrict__ hits,
+                                                      const PointRec *__restrict__ pts,
+                                                      PathSt *__restrict__ st,
+                                                      float4 *__restrict__ rays,
+                                                      uint32_t *__restrict__ count) {
+  const uint32_t p = blockIdx.x * kDBlock + threadIdx.x;
+  bool again = false;
+  if (p < P && st[p].state == kTracing) {
+    PathSt &s = st[p];
+    const float4 h = hits[p];
+    uint32_t mat;
+    if (missed(S, h)) {
+      s.state = kDead;
+      kill_ray(rays, p);
+    } else if (is_emitter(S, h, &mat)) {
+      if (++s.tries >= CHK_SHADE_MAX_TRIES) {
+        s.state = kDead;
+        kill_ray(rays, p);
+      } else {
+        const PointRec &R = pts[s.pk];
+        const u4 r = shade_rng(pix0 + p / spp, s0 + p % spp, (vertex << 8) | s.tries, key0, key1);
+        s.d = hemi_dir(R.n, ud(r.x), ud(r.y));
+        put_ray(rays, p, s.o, s.d, R.leaf);
+        again = true;
+      }
+    } else {
+      const PointRec &R = pts[s.pk];
+      const MatT<double> m = load_mat<double>(S.mats + R.mat);
+      const double cosine = dot(s.d, R.n);
+      double f, pdf;
+      brdf_pdf(m, R.n, R.wo_ref, R.wo, s.d, f, pdf);
+      const double product = 2.0 * kPi64 * cosine * f;  // pdf * cosine * brdf, pdf = 2 pi
+      const d3 TP = m.c * product;
+      s.T = dk(s.T.x * TP.x / (1 - kRR), s.T.y * TP.y / (1 - kRR), s.T.z * TP.z / (1 - kRR));
+      s.hit = h;
+      s.state = kReady;
+      kill_ray(rays, p);
+    }
+  }
+  const unsigned long long b = __ballot(again);
+  if (b && __lane_id() == (uint32_t)(__ffsll((long long)b) - 1))
+    atomicAdd(count, (uint32_t)__popcll(b));
+}
+
+// One vertex of shadeImplicit (shader.cpp:285-333): a miss is 0, an
+// emitter ends the path with radiance.maxnorm() times the product, anything
+// else draws a cosine direction and multiplies the product by
+// c * (pi * brdf).  After the fourth vertex the path is 0.
+__global__ __launch_bounds__(kDBlock) void k_im_step(DeviceScene S, uint32_t P, uint32_t pix0,
+                                                     uint32_t spp, uint32_t s0, uint32_t vertex,
+                                                     bool physical, uint32_t key0, uint32_t key1,
+                                                     const float4 *__restrict__ hits,
+                                                     PathSt *__restrict__ st,
+                                                     float4 *__restrict__ rays) {
+  const uint32_t p = blockIdx.x * kDBlock + threadIdx.x;
+  if (p >= P) return;
+  PathSt &s = st[p];
+  if (s.state != kTracing) return;
+  const float4 h = hits[p];
+  uint32_t mat;
+  if (missed(S, h)) {
+    s.col = dk(0, 0, 0);
+    s.state = kDead;
+    kill_ray(rays, p);
+    return;
+  }
+  if (is_emitter(S, h, &mat)) {
+    s.col = emitter_value(S.mats[mat], physical) * s.T;
+    s.state = kDead;
+    kill_ray(rays, p);
+    return;
+  }
+  if (vertex >= 3) {
+    s.state = kDead;
+    kill_ray(rays, p);
+    return;
+  }
+  const uint32_t leaf = __float_as_uint(h.y);
+  const d3 x = s.o + s.d * (double)h.x;
+  const d3 n = facing_normal(S, leaf, s.d);
+  const MatT<double> m = load_mat<double>(S.mats + mat);
+  const d3 wo = s.d * -1.0;
+  const d3 wo_ref = normalize(n * 2.0 * dot(n, wo) + s.d);
+  const u4 r = shade_rng(pix0 + p / spp, s0 + p % spp, vertex << 8, key0, key1);
+  const d3 w = cosine_dir(n, ud(r.x), ud(r.y));
+  double f, pdf;
+  brdf_pdf(m, n, wo_ref, wo, w, f, pdf);
+  s.T = s.T * (kPi64 * 1.0 * f);  // product * (pdfInv * cosine * brdf), cosine = 1
+  s.T = m.c * s.T;
+  s.o = x + n * 1e-6;
+  s.d = w;
+  put_ray(rays, p, s.o, w, leaf);
+}
+
+// Per-pixel FP64 sums of the chunk's path colours in sample order.
+__global__ __launch_bounds__(kDBlock) void k_sh_accum(const PathSt *__restrict__ st, uint32_t pix0,
+                                                      uint32_t n_pix, uint32_t spp,
+                                                      double *__restrict__ accum,
+                                                      double *__restrict__ sumsq) {
+  const uint32_t q = blockIdx.x * kDBlock + threadIdx.x;
+  if (q >= n_pix) return;
+  double a[3] = {0, 0, 0}, b[3] = {0, 0, 0};
+  for (uint32_t s = 0; s < spp; ++s) {
+    const d3 c = st[(size_t)q * spp + s].col;
+    const double v[3] = {c.x, c.y, c.z};
+    for (int j = 0; j < 3; ++j) {
+      a[j] += v[j];
+      b[j] += v[j] * v[j];
+    }
+  }
+  const size_t o = 3 * ((size_t)pix0 + q);
+  for (int j = 0; j < 3; ++j) {
+    accum[o + j] += a[j];
+    if (sumsq) sumsq[o + j] += b[j];
+  }
+}
+
 // ------------------------------------------------------------ host side
 inline size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
 
@@ -601,19 +896,23 @@ chk_status check_params(const chk_scene_s *sc, const chk_direct_params *p, const
 struct DWork {
   PointRec *pts;
   uint32_t *err;
+  uint32_t *count;  // 64 bytes after err
   Items I;
   uint8_t *occ;
   double *val;     // P x 3
   float4 *crays;   // render: P camera rays
   float4 *chits;
   double *cdirs;   // P x 3
+  PathSt *paths;   // render with a path shader: P paths
 };
 size_t dwork_bytes(uint32_t P, uint32_t n_slots, bool render) {
   const size_t I = (size_t)P * n_slots;
   size_t b = al256((size_t)P * sizeof(PointRec)) + 256;
   b += 3 * al256(I * 8) + al256(I * 4) + al256(I * 32) + al256(I);
   b += al256((size_t)P * 24);
-  if (render) b += al256((size_t)P * 32) + al256((size_t)P * 16) + al256((size_t)P * 24);
+  if (render)
+    b += al256((size_t)P * 32) + al256((size_t)P * 16) + al256((size_t)P * 24) +
+         al256((size_t)P * sizeof(PathSt));
   return b;
 }
 chk_status dwork_get(chk_scene_s *sc, uint32_t P, uint32_t n_slots, bool render, DWork &W) {
@@ -634,6 +933,7 @@ chk_status dwork_get(chk_scene_s *sc, uint32_t P, uint32_t n_slots, bool render,
   W.pts = (PointRec *)q;
   q += al256((size_t)P * sizeof(PointRec));
   W.err = (uint32_t *)q;
+  W.count = W.err + 16;
   q += 256;
   W.I.a = (double *)q;
   q += al256(I * 8);
@@ -651,12 +951,15 @@ chk_status dwork_get(chk_scene_s *sc, uint32_t P, uint32_t n_slots, bool render,
   q += al256((size_t)P * 24);
   W.crays = W.chits = nullptr;
   W.cdirs = nullptr;
+  W.paths = nullptr;
   if (render) {
     W.crays = (float4 *)q;
     q += al256((size_t)P * 32);
     W.chits = (float4 *)q;
     q += al256((size_t)P * 16);
     W.cdirs = (double *)q;
+    q += al256((size_t)P * 24);
+    W.paths = (PathSt *)q;
   }
   return CHK_OK;
 }
@@ -841,28 +1144,37 @@ chk_status chk_direct_light(chk_scene_t sc, const double *x, const double *n, co
   CHK_TRY_END
 }
 
-chk_status chk_render_direct(chk_scene_t sc, const chk_camera *cam, const chk_render_params *P,
-                             const chk_direct_params *direct, double *accum, double *sumsq,
-                             chk_render_stats *stats, void *stream) {
+chk_status chk_render_shade(chk_scene_t sc, const chk_camera *cam, const chk_render_params *P,
+                            uint32_t shader, const chk_direct_params *direct, double *accum,
+                            double *sumsq, chk_render_stats *stats, void *stream) {
   CHK_TRY_BEGIN
   if (!sc || !cam || !P || !accum) {
-    set_error("chk_render_direct: null scene, camera, params or accum");
+    set_error("chk_render_shade: null scene, camera, params or accum");
     return CHK_EINVAL;
   }
-  chk_status cs = check_params(sc, direct, "chk_render_direct");
-  if (cs != CHK_OK) return cs;
+  if (shader > CHK_SHADE_IMPLICIT) {
+    set_error("chk_render_shade: unknown shader");
+    return CHK_EINVAL;
+  }
+  const bool implicit = shader == CHK_SHADE_IMPLICIT;
+  chk_direct_params dp{};
+  if (!implicit) {
+    chk_status cs = check_params(sc, direct, "chk_render_shade");
+    if (cs != CHK_OK) return cs;
+    dp = *direct;
+  }
   if (cam->width == 0 || cam->height == 0 || P->spp == 0) {
-    set_error("chk_render_direct: zero width, height or spp");
+    set_error("chk_render_shade: zero width, height or spp");
     return CHK_EINVAL;
   }
   if ((uint64_t)cam->width * cam->height >= (1ull << 32)) {
-    set_error("chk_render_direct: image too large");
+    set_error("chk_render_shade: image too large");
     return CHK_ECAPACITY;
   }
   CHK_HIP(hipSetDevice(sc->ctx->device));
   std::string why;
   if (!materials_ok(sc, why)) {
-    set_error("chk_render_direct: " + why);
+    set_error("chk_render_shade: " + why);
     return CHK_EINVAL;
   }
   const hipStream_t st = (hipStream_t)stream;
@@ -879,7 +1191,7 @@ chk_status chk_render_direct(chk_scene_t sc, const chk_camera *cam, const chk_re
                     d[0] * up[1] - d[1] * up[0]};
     const double cl = std::sqrt(cx[0] * cx[0] + cx[1] * cx[1] + cx[2] * cx[2]);
     if (!(cl >= 1e-20)) {
-      set_error("chk_render_direct: camera up and direction are parallel");
+      set_error("chk_render_shade: camera up and direction are parallel");
       return CHK_EINVAL;
     }
     const double sx = ((double)cam->width / cam->height) * std::tan(cam->fovx_deg * PI / 360.0) / cl;
@@ -897,26 +1209,26 @@ chk_status chk_render_direct(chk_scene_t sc, const chk_camera *cam, const chk_re
     C.W = cam->width;
     C.H = cam->height;
   }
-  chk_direct_params dp = *direct;
-  const uint32_t n_slots = n_slots_of(sc, dp);
+  const uint32_t n_slots = implicit ? 0u : n_slots_of(sc, dp);
   const uint32_t n_pix = cam->width * cam->height, spp = P->spp;
   // whole pixels per chunk
   const uint32_t cap_items = chunk_points(n_slots, 0xFFFFFFFFu);
   if (spp > cap_items) {
-    set_error("chk_render_direct: spp exceeds the items-per-chunk capacity (" +
+    set_error("chk_render_shade: spp exceeds the items-per-chunk capacity (" +
               std::to_string(cap_items) + ")");
     return CHK_ECAPACITY;
   }
   const uint32_t pix_per = std::max(1u, cap_items / spp);
   const uint32_t Pc = (uint32_t)std::min<uint64_t>((uint64_t)pix_per * spp, (uint64_t)n_pix * spp);
   DWork W;
-  cs = dwork_get(sc, Pc, n_slots, true, W);
+  chk_status cs = dwork_get(sc, Pc, n_slots, true, W);
   if (cs != CHK_OK) return cs;
-  const bool need_G = dp.estimator != CHK_DIRECT_MIS;
+  const bool need_G = !implicit && dp.estimator != CHK_DIRECT_MIS;
   const bool physical = (P->flags & CHK_RENDER_PHYSICAL_EMITTER) != 0;
   const TraceShape shape{sc->stack_size, 128};
-  uint64_t shadow = 0;
-  uint32_t chunks = 0;
+  const double *ltc = (const double *)sc->ctx->ltc_dev;
+  uint64_t shadow = 0, extend = 0;
+  uint32_t chunks = 0, launches = 0;
   for (uint32_t pix0 = 0; pix0 < n_pix; pix0 += pix_per) {
     const uint32_t np = std::min(pix_per, n_pix - pix0);
     const uint32_t items = np * spp;
@@ -926,13 +1238,72 @@ chk_status chk_render_direct(chk_scene_t sc, const chk_camera *cam, const chk_re
     CHK_HIP(hipGetLastError());
     CHK_HIP(launch_closest(sc, shape, W.crays, linear_range(items), W.chits, true, false, nullptr,
                            SlotScenes{}, st));
-    hipLaunchKernelGGL(k_dr_points, g, b, 0, st, sc->d, (const double *)sc->ctx->ltc_dev, C, pix0,
-                       items, spp, P->sample_offset, physical, need_G, W.chits, W.cdirs, W.pts);
+    extend += items;
+    ++launches;
+    if (shader == CHK_SHADE_DIRECT_ONLY) {
+      hipLaunchKernelGGL(k_dr_points, g, b, 0, st, sc->d, ltc, C, pix0, items, spp,
+                         P->sample_offset, physical, need_G, W.chits, W.cdirs, W.pts);
+      CHK_HIP(hipGetLastError());
+      cs = run_items(sc, W, items, dp, true, &shadow, st);
+      if (cs != CHK_OK) return cs;
+      hipLaunchKernelGGL(k_dr_accum, dim3((np + kDBlock - 1) / kDBlock), dim3(kDBlock), 0, st,
+                         W.val, pix0, np, spp, accum,
+                         (P->flags & CHK_RENDER_SUMSQ) ? sumsq : nullptr);
+      CHK_HIP(hipGetLastError());
+      ++chunks;
+      continue;
+    }
+    hipLaunchKernelGGL(k_sh_init, g, b, 0, st, C, items, W.cdirs, W.paths);
     CHK_HIP(hipGetLastError());
-    cs = run_items(sc, W, items, dp, true, &shadow, st);
-    if (cs != CHK_OK) return cs;
-    hipLaunchKernelGGL(k_dr_accum, dim3((np + kDBlock - 1) / kDBlock), dim3(kDBlock), 0, st,
-                       W.val, pix0, np, spp, accum,
+    if (implicit) {
+      for (uint32_t v = 0; v < 4; ++v) {
+        hipLaunchKernelGGL(k_im_step, g, b, 0, st, sc->d, items, pix0, spp, P->sample_offset, v,
+                           physical, P->seed_lo, P->seed_hi, W.chits, W.paths, W.crays);
+        CHK_HIP(hipGetLastError());
+        if (v == 3) break;
+        CHK_HIP(launch_closest(sc, shape, W.crays, linear_range(items), W.chits, true, false,
+                               nullptr, SlotScenes{}, st));
+        ++launches;
+        extend += items;  // upper bound: rays of paths still alive
+      }
+    } else {
+      hipLaunchKernelGGL(k_ex_camera_hit, g, b, 0, st, sc->d, items, physical, W.chits, W.paths,
+                         W.crays);
+      CHK_HIP(hipGetLastError());
+      for (uint32_t v = 0; v < 100; ++v) {
+        // shadeDI at every live path's vertex
+        CHK_HIP(launch_zero_u32(W.count, 1, st));
+        hipLaunchKernelGGL(k_ex_gather, g, b, 0, st, sc->d, ltc, items, pix0, spp,
+                           P->sample_offset, v, need_G, W.paths, W.pts, W.count);
+        CHK_HIP(hipGetLastError());
+        uint32_t n_act = 0;
+        CHK_HIP(hipMemcpyAsync(&n_act, W.count, 4, hipMemcpyDeviceToHost, st));
+        CHK_HIP(hipStreamSynchronize(st));
+        if (n_act == 0) break;
+        cs = run_items(sc, W, n_act, dp, true, &shadow, st);
+        if (cs != CHK_OK) return cs;
+        hipLaunchKernelGGL(k_ex_cont, dim3((n_act + kDBlock - 1) / kDBlock), dim3(kDBlock), 0, st,
+                           n_act, v, P->seed_lo, P->seed_hi, W.pts, W.val, W.paths, W.crays);
+        CHK_HIP(hipGetLastError());
+        // continuation: draw again while the ray hits an emitter
+        for (uint32_t t = 0; t < CHK_SHADE_MAX_TRIES; ++t) {
+          CHK_HIP(launch_closest(sc, shape, W.crays, linear_range(items), W.chits, true, false,
+                                 nullptr, SlotScenes{}, st));
+          ++launches;
+          CHK_HIP(launch_zero_u32(W.count, 1, st));
+          hipLaunchKernelGGL(k_ex_retry, g, b, 0, st, sc->d, items, pix0, spp, P->sample_offset, v,
+                             P->seed_lo, P->seed_hi, W.chits, W.pts, W.paths, W.crays, W.count);
+          CHK_HIP(hipGetLastError());
+          uint32_t again = 0;
+          CHK_HIP(hipMemcpyAsync(&again, W.count, 4, hipMemcpyDeviceToHost, st));
+          CHK_HIP(hipStreamSynchronize(st));
+          extend += n_act;  // upper bound: rays traced this round
+          if (again == 0) break;
+        }
+      }
+    }
+    hipLaunchKernelGGL(k_sh_accum, dim3((np + kDBlock - 1) / kDBlock), dim3(kDBlock), 0, st,
+                       W.paths, pix0, np, spp, accum,
                        (P->flags & CHK_RENDER_SUMSQ) ? sumsq : nullptr);
     CHK_HIP(hipGetLastError());
     ++chunks;
@@ -941,15 +1312,21 @@ chk_status chk_render_direct(chk_scene_t sc, const chk_camera *cam, const chk_re
   if (stats) {
     std::memset(stats, 0, sizeof(*stats));
     stats->paths = (uint64_t)n_pix * spp;
-    stats->rays_extend = (uint64_t)n_pix * spp;
+    stats->rays_extend = extend;
     stats->rays_shadow = shadow;
     stats->chunks = chunks;
-    stats->extend_launches = chunks;
+    stats->extend_launches = launches;
     stats->other_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   }
   return CHK_OK;
   CHK_TRY_END
+}
+
+chk_status chk_render_direct(chk_scene_t sc, const chk_camera *cam, const chk_render_params *P,
+                             const chk_direct_params *direct, double *accum, double *sumsq,
+                             chk_render_stats *stats, void *stream) {
+  return chk_render_shade(sc, cam, P, CHK_SHADE_DIRECT_ONLY, direct, accum, sumsq, stats, stream);
 }
 
 }  // extern "C"

@@ -13,6 +13,8 @@ C ABI, batched.  Names and argument meanings follow the reference:
                         and MIS::getSamples' weights (mis.cpp:18-223)
   render_direct         shadeDirectOnly (shader.cpp:339-363), the reference's
                         shipped shade()
+  render_shade          any of its shaders: shadeDirectOnly, shadeExplicit
+                        (shader.cpp:111-207), shadeImplicit (:276-337)
   render                main.cpp:62-94 around shade() (shader.cpp:362)
   write_ppm             main.cpp:27,98-117
 
@@ -227,6 +229,31 @@ def direct_params(estimator, cv_samples=0, flags=0, seed=(0x5EED0006, 0)):
     P.estimator, P.cv_samples, P.flags = estimator, cv_samples, flags
     P.seed_lo, P.seed_hi = seed
     return P
+
+
+def render_shade(ds: abi.DeviceScene, params, shader, direct=None, camera=None,
+                 image: "Image | None" = None, stats=True, stream=None, download=True):
+    """chk_render_shade: one of the reference's shaders (abi.SHADE_DIRECT_ONLY,
+    SHADE_EXPLICIT, SHADE_IMPLICIT; shader.cpp:111-378) with the direct-lighting
+    estimator of `direct` (a chk_direct_params or a dict of its fields; unused
+    by shadeImplicit)."""
+    cam = abi.camera_struct(camera or ds.scene.camera)
+    P = abi.params_struct(params)
+    if isinstance(direct, dict):
+        direct = direct_params(**direct)
+    if image is None:
+        image = Image(cam.width, cam.height, sumsq=bool(P.flags & abi.RENDER_SUMSQ))
+    st = abi.chk_render_stats()
+    abi.check(ds.L.chk_render_shade(ds.h, C.byref(cam), C.byref(P), int(shader),
+                                    C.byref(direct) if direct is not None else None,
+                                    image.acc.ptr, image.sq.ptr if image.sq is not None else None,
+                                    C.byref(st) if stats else None, stream),
+              "chk_render_shade")
+    sd = {k: getattr(st, k) for k, _ in abi.chk_render_stats._fields_} if stats else {}
+    if not download:
+        return image, sd
+    a, s2 = image.get()
+    return a, s2, sd
 
 
 def render_direct(ds: abi.DeviceScene, params, direct, camera=None, image: "Image | None" = None,

@@ -362,14 +362,17 @@ def cornell_large(seed=0x5EED0005, total_tris=2_000_000, n_lights=32, width=1920
 
 
 # ------------------------------------------------------------------ direct-lighting room
-def direct_room(seed=0x5EED0006, width=64, height=64, reflectance=None, fan=True, panel=True):
+def direct_room(seed=0x5EED0006, width=64, height=64, reflectance=None, fan=True, panel=True,
+                light_scale=1.0, light_intensity=None):
     """A unit room for the shadeDI estimators: Lambert floor, GGX back wall,
     Beckmann left wall, Phong right wall, a classic-Phong panel hanging under
     the ceiling (it shadows part of the floor), and two mesh lights under the
     ceiling: a 2-triangle quad and a 3-triangle fan of unequal areas (so the
     reference's 300-bin inverse CDF is not exact).  `reflectance` overrides
     every surface's reflectance; fan=False makes the second light a quad too,
-    panel=False drops the panel (nothing then occludes a light)."""
+    panel=False drops the panel (nothing then occludes a light);
+    light_scale shrinks both lights about their centres; light_intensity
+    replaces both intensities (1.0: radiance.maxnorm() is the radiance)."""
     rng = np.random.default_rng(seed)
 
     def R(r):
@@ -394,14 +397,16 @@ def direct_room(seed=0x5EED0006, width=64, height=64, reflectance=None, fan=True
     if not panel:
         obj, omat = obj[:4], omat[:8]
     c = np.array([0.7, 0.999, 0.6])
+    k = float(light_scale)
     if fan:
-        second = np.array([np.r_[c, c + (0.12, 0, -0.05), c + (0.02, 0, 0.1)],
-                           np.r_[c, c + (0.02, 0, 0.1), c + (-0.1, 0, 0.04)],
-                           np.r_[c, c + (-0.1, 0, 0.04), c + (-0.03, 0, -0.03)]])
+        second = np.array([np.r_[c, c + k * np.array((0.12, 0, -0.05)), c + k * np.array((0.02, 0, 0.1))],
+                           np.r_[c, c + k * np.array((0.02, 0, 0.1)), c + k * np.array((-0.1, 0, 0.04))],
+                           np.r_[c, c + k * np.array((-0.1, 0, 0.04)), c + k * np.array((-0.03, 0, -0.03))]])
     else:
-        second = quad((0.7, 0.999, 0.6), (0.08, 0, 0), (0, 0, 0.07))
-    lights = [(quad((0.3, 0.999, 0.35), (0.12, 0, 0), (0, 0, 0.1)), (1.0, 0.9, 0.7), 12.0),
-              (second, (0.6, 0.8, 1.0), 20.0)]
+        second = quad((0.7, 0.999, 0.6), (0.08 * k, 0, 0), (0, 0, 0.07 * k))
+    i0, i1 = (12.0, 20.0) if light_intensity is None else (light_intensity, light_intensity)
+    lights = [(quad((0.3, 0.999, 0.35), (0.12 * k, 0, 0), (0, 0, 0.1 * k)), (1.0, 0.9, 0.7), i0),
+              (second, (0.6, 0.8, 1.0), i1)]
     cam = Camera((0.5, 0.5, 2.2), (0, 0, -1), (0, 1, 0), 40.0, 40.0, width, height)
     return _assemble(np.concatenate(obj), omat, mats, lights, cam, "direct_room", {"seed": seed})
 

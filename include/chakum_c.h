@@ -252,6 +252,22 @@ chk_status chk_render_direct(chk_scene_t scene, const chk_camera *camera,
                              const chk_render_params *params, const chk_direct_params *direct,
                              double *accum, double *sumsq, chk_render_stats *stats,
                              void *stream);
+/* Render with one of the reference's shaders (chk_shader), the compile-time
+ * choice of shade() (shader.cpp:362-378):
+ *   CHK_SHADE_DIRECT_ONLY  as chk_render_direct
+ *   CHK_SHADE_EXPLICIT     shadeExplicit (shader.cpp:111-207): colour times
+ *                          the selected direct-lighting estimator at every
+ *                          vertex, Russian roulette with probability 0.3,
+ *                          uniform-hemisphere continuation that draws again
+ *                          while it hits an emitter (CHK_SHADE_MAX_TRIES)
+ *   CHK_SHADE_IMPLICIT     shadeImplicit (shader.cpp:276-337): up to 4
+ *                          cosine-sampled vertices, an emitter's
+ *                          radiance.maxnorm() ends the path (direct unused)
+ * params and accumulators as chk_render_direct. */
+chk_status chk_render_shade(chk_scene_t scene, const chk_camera *camera,
+                            const chk_render_params *params, uint32_t shader,
+                            const chk_direct_params *direct, double *accum, double *sumsq,
+                            chk_render_stats *stats, void *stream);
 
 /* ---- batched requests ----------------------------------------------------
  * No single reference counterpart: the reference renders one image per

@@ -127,13 +127,29 @@ typedef struct chk_direct_params {
 } chk_direct_params;
 
 /* Philox counter of the direct-lighting estimators:
- *   ctr = { point index (or pixel), estimate index (or sample), slot, CHK_STREAM_DIRECT }
+ *   ctr = { point index (or pixel), estimate index (or sample), slot,
+ *           CHK_STREAM_DIRECT + path vertex (0 except in shadeExplicit) }
  * with one call per sample slot; a 32-bit output x maps to (x + 0.5) * 2^-32.
  * Slots 10 j .. 10 j + 9 sample mesh light j by area, slots 10 L .. 10 L + 59
  * the BSDF (the first floor(60 ks) from the specular lobe), as
  * MIS::getSamples (mis.cpp:97-191); ANALYTIC_CV uses slots j n .. j n + n - 1
  * for light j. */
 #define CHK_STREAM_DIRECT 0x44490000u
+
+/* The reference's shaders (shader.cpp:362-378 selects one at compile time). */
+enum chk_shader {
+  CHK_SHADE_DIRECT_ONLY = 0,  /* shadeDirectOnly (shader.cpp:339-363), the shipped one */
+  CHK_SHADE_EXPLICIT = 1,     /* shadeExplicit (shader.cpp:111-207) */
+  CHK_SHADE_IMPLICIT = 2      /* shadeImplicit (shader.cpp:276-337) */
+};
+/* Philox counter of the shaders' own draws:
+ *   ctr = { pixel, sample, (vertex << 8) | k, CHK_STREAM_SHADE }
+ * k = the try index of a continuation direction, 0xFF for Russian roulette. */
+#define CHK_STREAM_SHADE 0x53480000u
+/* shadeExplicit draws a new direction while the last one hit an emitter
+ * (shader.cpp:159-183, unbounded there); here at most this many draws, after
+ * which the path ends as if the ray had escaped. */
+#define CHK_SHADE_MAX_TRIES 64u
 #define CHK_MIS_LIGHT_SAMPLES 10u  /* nSuArIS, mis.h:9 */
 #define CHK_MIS_BSDF_SAMPLES 60u   /* nBrIS, mis.h:11 */
 

@@ -85,8 +85,8 @@ def lib():
         L.orc_mis_samples.argtypes = [_vp, _dp, _dp, _dp, _dp, C.c_uint32, C.c_uint32, C.c_uint32,
                                       C.c_uint32, C.c_uint32, _dp]
         L.orc_direct_light.argtypes = [_vp, _vp, C.c_uint32, _dp, _dp, _dp, _up, C.c_int, _dp]
-        L.orc_render_direct.argtypes = [_vp, _vp, _vp, _vp, C.c_uint32, C.c_uint32, C.c_uint32,
-                                        C.c_int, _dp, _vp]
+        L.orc_render_shade.argtypes = [_vp, _vp, _vp, C.c_uint32, _vp, C.c_uint32, C.c_uint32,
+                                       C.c_uint32, C.c_int, _dp, _vp]
         table = np.fromfile(LTC_TABLE, "<f8")
         L.orc_ltc_set(np.ascontiguousarray(table))
         _lib = L
@@ -319,7 +319,9 @@ class DirectScene:
                                int(threads), out)
         return out
 
-    def render_direct(self, params, direct, rows=None, threads=0, sumsq=False, row_step=1):
+    def render_shade(self, shader, params, direct, rows=None, threads=0, sumsq=False,
+                     row_step=1):
+        """orc_render_shade: shader 0 shadeDirectOnly, 1 shadeExplicit, 2 shadeImplicit."""
         from chakumrender_amd import abi
 
         cam = abi.camera_struct(self.scene.camera)
@@ -328,10 +330,13 @@ class DirectScene:
         r0, r1 = (0, H) if rows is None else rows
         acc = np.zeros((H, W, 3))
         sq = np.zeros((H, W, 3)) if sumsq else None
-        lib().orc_render_direct(self.h, C.addressof(cam), C.addressof(P), C.addressof(direct),
-                                r0, r1, int(row_step), int(threads), acc,
-                                sq.ctypes.data if sq is not None else None)
+        lib().orc_render_shade(self.h, C.addressof(cam), C.addressof(P), int(shader),
+                               C.addressof(direct), r0, r1, int(row_step), int(threads), acc,
+                               sq.ctypes.data if sq is not None else None)
         return acc, sq
+
+    def render_direct(self, params, direct, **kw):
+        return self.render_shade(0, params, direct, **kw)
 
 
 class _quiet:

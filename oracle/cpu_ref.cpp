@@ -888,25 +888,26 @@ double brdf_apx(const DCtx &C, V3 n, V3 wi) {
   return (C.m.ks * res / dot(wi, n) + (1 - C.m.ks) / PI) * C.m.refl;
 }
 
-inline void dphilox(uint32_t a, uint32_t b, uint32_t slot, const uint32_t *key, uint32_t *r) {
-  uint32_t c[4] = {a, b, slot, CHK_STREAM_DIRECT};
+inline void dphilox(uint32_t a, uint32_t b, uint32_t slot, uint32_t v, const uint32_t *key,
+                    uint32_t *r) {
+  uint32_t c[4] = {a, b, slot, CHK_STREAM_DIRECT + v};
   philox(c, key, r);
 }
 
 // MIS::getSamples sample set (mis.cpp:127-138,180-191).
 void mis_samples(const DScene &S, const DCtx &C, const DPoint &P, uint32_t a, uint32_t b,
-                 const uint32_t *key, V3 *out) {
+                 uint32_t v, const uint32_t *key, V3 *out) {
   uint32_t slot = 0, r[4], tri;
   for (const DLight &L : S.L)
     for (uint32_t q = 0; q < CHK_MIS_LIGHT_SAMPLES; ++q, ++slot) {
-      dphilox(a, b, slot, key, r);
+      dphilox(a, b, slot, v, key, r);
       out[slot] = norm(mesh_light_point(S, L, r, &tri) - P.x);
     }
   // getBrdfDirectionSamples (materialTypes.cpp:118-134): floor(60 ks)
   // specular-lobe samples, then cosine samples
   const uint32_t nb = (uint32_t)(CHK_MIS_BSDF_SAMPLES * C.m.ks);
   for (uint32_t q = 0; q < CHK_MIS_BSDF_SAMPLES; ++q, ++slot) {
-    dphilox(a, b, slot, key, r);
+    dphilox(a, b, slot, v, key, r);
     double u[3] = {q < nb ? -1.0 : 2.0, ud(r[1]), ud(r[2])};
     out[slot] = brdf_sample(C.m, P.n, C.wo_ref, C.wo, u);
   }
@@ -932,7 +933,7 @@ double mis_weight(const DScene &S, const DCtx &C, const DPoint &P, V3 s) {
 }
 
 V3 direct_estimate(const DScene &S, const chk_direct_params &p, const DPoint &P, uint32_t a,
-                   uint32_t b) {
+                   uint32_t b, uint32_t v = 0) {
   const uint32_t key[2] = {p.seed_lo, p.seed_hi};
   DCtx C;
   direct_setup(S, P, p.estimator != CHK_DIRECT_MIS, &C);
@@ -947,7 +948,7 @@ V3 direct_estimate(const DScene &S, const chk_direct_params &p, const DPoint &P,
       const double pdf = L.area;  // getSamples returns 1/pdf
       for (uint32_t q = 0; q < n; ++q) {
         uint32_t r[4], tri;
-        dphilox(a, b, j * n + q, key, r);
+        dphilox(a, b, j * n + q, v, key, r);
         V3 y = mesh_light_point(S, L, r, &tri);
         V3 w = norm(y - P.x);
         double d = length(y - P.x);
@@ -968,7 +969,7 @@ V3 direct_estimate(const DScene &S, const chk_direct_params &p, const DPoint &P,
   // MIS and MISCV share MIS::getSamples
   const uint32_t Sn = (uint32_t)S.L.size() * CHK_MIS_LIGHT_SAMPLES + CHK_MIS_BSDF_SAMPLES;
   std::vector<V3> smp(Sn);
-  mis_samples(S, C, P, a, b, key, smp.data());
+  mis_samples(S, C, P, a, b, v, key, smp.data());
   for (uint32_t i = 0; i < Sn; ++i) {
     const V3 w = smp[i];
     const double wt = mis_weight(S, C, P, w);
@@ -1003,6 +1004,133 @@ V3 direct_estimate(const DScene &S, const chk_direct_params &p, const DPoint &P,
     }
   }
   return p.estimator == CHK_DIRECT_MISCV ? sum + C.G : sum;
+}
+
+// ---------------------------------------------------------------- reference shaders
+// Philox counter of the shaders' own draws: {pixel, sample, vertex << 8 | k,
+// CHK_STREAM_SHADE}; k = try index of a direction, 0xFF for Russian roulette.
+inline void sphilox(uint32_t pix, uint32_t smp, uint32_t w, const uint32_t *key, uint32_t *r) {
+  uint32_t c[4] = {pix, smp, w, CHK_STREAM_SHADE};
+  philox(c, key, r);
+}
+// getCosineDirectionSamples distribution (domainSampler.cpp:154-189), 1/pdf pi
+V3 cosine_dir(V3 n, double e1, double e2) {
+  double st = std::sqrt(e1), ct = std::sqrt(1.0 - e1), phi = 2 * PI * e2;
+  V3 T, Bv;
+  onb(n, &T, &Bv);
+  return T * (st * std::cos(phi)) + Bv * (st * std::sin(phi)) + n * ct;
+}
+// getHemiDirectionSamples distribution (uniform, domainSampler.cpp:88-99), 1/pdf 2 pi
+V3 hemi_dir(V3 n, double e1, double e2) {
+  double ct = e1, st = std::sqrt(std::max(0.0, 1.0 - e1 * e1)), phi = 2 * PI * e2;
+  V3 T, Bv;
+  onb(n, &T, &Bv);
+  return T * (st * std::cos(phi)) + Bv * (st * std::sin(phi)) + n * ct;
+}
+inline V3 vdiv(V3 a, double s) { return v3(a.x / s, a.y / s, a.z / s); }
+
+struct Hit {
+  bool hit;
+  double t;
+  V3 N;
+  uint32_t prim;
+};
+Hit closest(const DScene &S, V3 o, V3 d) {
+  Hit h;
+  int rid;
+  h.hit = bvh_closest(*S.B, o, d, 1, &h.t, &h.N, &rid, nullptr, nullptr);
+  h.prim = h.hit ? S.B->ordered[rid] : 0u;
+  return h;
+}
+
+// shadeImplicit (shader.cpp:276-337): up to 4 vertices, cosine sampling,
+// an emitter ends the path with radiance.maxnorm() times the product.
+V3 shade_implicit(const DScene &S, V3 o, V3 d, uint32_t pix, uint32_t smp, const uint32_t *key,
+                  bool physical) {
+  V3 product = v3(1, 1, 1), col = v3(0, 0, 0);
+  for (uint32_t i = 0; i < 4; ++i) {
+    Hit h = closest(S, o, d);
+    if (!h.hit) {
+      col = v3(0, 0, 0);
+      break;
+    }
+    const Mat &m = S.mats[S.tri_mat[h.prim]];
+    V3 x = o + d * h.t, n = h.N;
+    if (m.light != CHK_LIGHT_NONE) {
+      col = mul(physical ? m.rad : maxnorm(m.rad), product);
+      break;
+    }
+    V3 wo = d * -1.0;
+    V3 wo_ref = norm(n * 2.0 * dot(n, wo) + d);
+    uint32_t r[4];
+    sphilox(pix, smp, i << 8, key, r);
+    V3 w = cosine_dir(n, ud(r[0]), ud(r[1]));
+    double b = brdf(m, n, wo_ref, wo, w);
+    product = product * (PI * 1.0 * b);  // pdfInv * cosine * brdf, cosine = 1
+    product = mul(m.c, product);
+    o = x + n * 1e-6;
+    d = w;
+  }
+  return col;
+}
+
+// shadeExplicit (shader.cpp:111-207): shadeDI at every vertex, Russian
+// roulette with probability 0.3, uniform-hemisphere continuation that draws
+// again while it hits an emitter (at most CHK_SHADE_MAX_TRIES draws; the
+// reference loops until it misses one), and DI_i + col * TP_i / 0.7 from
+// the last vertex back.
+V3 shade_explicit(const DScene &S, const chk_direct_params &dp, V3 o, V3 d, uint32_t pix,
+                  uint32_t smp, const uint32_t *key, bool physical) {
+  const double q = 0.3;
+  Hit h = closest(S, o, d);
+  if (!h.hit) return v3(0, 0, 0);
+  const Mat *m = &S.mats[S.tri_mat[h.prim]];
+  V3 x = o + d * h.t, n = h.N, rd = d;
+  uint32_t prim = h.prim;
+  if (m->light != CHK_LIGHT_NONE) return physical ? m->rad : maxnorm(m->rad);
+  std::vector<V3> DI, TP;
+  for (uint32_t i = 0; i < 100; ++i) {
+    uint32_t r[4];
+    sphilox(pix, smp, (i << 8) | 0xFFu, key, r);
+    const double e = ud(r[0]);
+    DPoint P{x, n, rd, prim};
+    DI.push_back(mul(m->c, direct_estimate(S, dp, P, pix, smp, i)));  // shadeDI
+    if (e < q) {
+      TP.push_back(v3(0, 0, 0));
+      break;
+    }
+    V3 wo = rd * -1.0;
+    V3 wo_ref = norm(n * 2.0 * dot(n, wo) + rd);
+    V3 so = x + n * 1e-6;
+    V3 w = v3(0, 0, 1);
+    Hit nh;
+    int status = 2;
+    for (uint32_t k = 0; k < CHK_SHADE_MAX_TRIES; ++k) {
+      sphilox(pix, smp, (i << 8) | k, key, r);
+      w = hemi_dir(n, ud(r[0]), ud(r[1]));
+      nh = closest(S, so, w);
+      if (!nh.hit) break;  // the ray escapes
+      if (S.mats[S.tri_mat[nh.prim]].light != CHK_LIGHT_NONE) continue;
+      status = 1;
+      break;
+    }
+    if (status == 2) {
+      TP.push_back(v3(0, 0, 0));
+      break;
+    }
+    double cosine = dot(w, n);
+    double b = brdf(*m, n, wo_ref, wo, w);
+    double product = 2 * PI * cosine * b;  // pdf (= 1/pdf, 2 pi) * cosine * brdf
+    TP.push_back(m->c * product);
+    rd = w;
+    x = so + w * nh.t;
+    n = nh.N;
+    prim = nh.prim;
+    m = &S.mats[S.tri_mat[prim]];
+  }
+  V3 col = DI.back();
+  for (int i = (int)DI.size() - 2; i >= 0; --i) col = DI[i] + vdiv(mul(col, TP[i]), 1 - q);
+  return col;
 }
 
 }  // namespace
@@ -1453,7 +1581,7 @@ void orc_mis_samples(void *s, const double *x, const double *n, const double *wo
   const uint32_t key[2] = {seed_lo, seed_hi};
   const uint32_t Sn = (uint32_t)D.L.size() * CHK_MIS_LIGHT_SAMPLES + CHK_MIS_BSDF_SAMPLES;
   std::vector<V3> out(Sn);
-  mis_samples(D, C, P, point, estimate, key, out.data());
+  mis_samples(D, C, P, point, estimate, 0, key, out.data());
   for (uint32_t q = 0; q < Sn; ++q) st(samples + 3 * q, out[q]);
 }
 
@@ -1471,9 +1599,9 @@ void orc_direct_light(void *s, const chk_direct_params *p, uint32_t k, const dou
   }
 }
 
-void orc_render_direct(void *s, const chk_camera *cam, const chk_render_params *P,
-                       const chk_direct_params *dp, uint32_t row0, uint32_t row1,
-                       uint32_t row_step, int n_threads, double *accum, double *sumsq) {
+void orc_render_shade(void *s, const chk_camera *cam, const chk_render_params *P, uint32_t shader,
+                      const chk_direct_params *dp, uint32_t row0, uint32_t row1,
+                      uint32_t row_step, int n_threads, double *accum, double *sumsq) {
   const DScene &D = *(const DScene *)s;
   const uint32_t W = cam->width, H = cam->height;
   V3 cpos = v3(cam->pos[0], cam->pos[1], cam->pos[2]);
@@ -1508,8 +1636,13 @@ void orc_render_direct(void *s, const chk_camera *cam, const chk_render_params *
         double t;
         V3 Ng;
         int rid;
-        // shadeDirectOnly (shader.cpp:339-363)
-        if (bvh_closest(*D.B, cpos, d, 1, &t, &Ng, &rid, nullptr, nullptr)) {
+        const bool phys = (P->flags & CHK_RENDER_PHYSICAL_EMITTER) != 0;
+        if (shader == CHK_SHADE_IMPLICIT) {
+          L = shade_implicit(D, cpos, d, idx, sm, key, phys);
+        } else if (shader == CHK_SHADE_EXPLICIT) {
+          L = shade_explicit(D, *dp, cpos, d, idx, sm, key, phys);
+        } else if (bvh_closest(*D.B, cpos, d, 1, &t, &Ng, &rid, nullptr, nullptr)) {
+          // shadeDirectOnly (shader.cpp:339-363)
           uint32_t prim = D.B->ordered[rid];
           const Mat &m = D.mats[D.tri_mat[prim]];
           if (m.light != CHK_LIGHT_NONE) {

@@ -163,11 +163,13 @@ void orc_mis_samples(void *scene, const double *x, const double *n, const double
 void orc_direct_light(void *scene, const chk_direct_params *p, uint32_t k, const double *x,
                       const double *n, const double *d, const uint32_t *prim, int n_threads,
                       double *rgb);
-/* shadeDirectOnly (shader.cpp:339-363) in the pixel loop of main.cpp:62-94:
- * rows row0, row0 + row_step, ... < row1; accum/sumsq as orc_render. */
-void orc_render_direct(void *scene, const chk_camera *cam, const chk_render_params *P,
-                       const chk_direct_params *dp, uint32_t row0, uint32_t row1,
-                       uint32_t row_step, int n_threads, double *accum, double *sumsq);
+/* The pixel loop of main.cpp:62-94 around one of the reference's shaders
+ * (chk_shader): shadeDirectOnly (shader.cpp:339-363), shadeExplicit
+ * (:111-207, with dp's estimator in shadeDI) or shadeImplicit (:276-337).
+ * Rows row0, row0 + row_step, ... < row1; accum/sumsq as orc_render. */
+void orc_render_shade(void *scene, const chk_camera *cam, const chk_render_params *P,
+                      uint32_t shader, const chk_direct_params *dp, uint32_t row0, uint32_t row1,
+                      uint32_t row_step, int n_threads, double *accum, double *sumsq);
 
 #ifdef __cplusplus
 }

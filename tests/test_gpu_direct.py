@@ -156,3 +156,70 @@ def test_errors(ctx, G):
     assert np.isfinite(v).all()
     del SL2, ds2
     c2.close()
+
+
+SHADERS = [(abi.SHADE_IMPLICIT, "mis"), (abi.SHADE_EXPLICIT, "miscv"), (abi.SHADE_EXPLICIT, "mis")]
+
+
+@pytest.mark.parametrize("shader,name", SHADERS)
+def test_shader_render_matches_oracle(ctx, shader, name):
+    """shadeImplicit / shadeExplicit on the same keys as the oracle: paths
+    agree sample by sample unless the FP32 and FP64 traversals part at an
+    edge; per-pixel means agree within 4 sigma."""
+    sc = scenes.direct_room(width=24, height=24)
+    ds = abi.DeviceScene(ctx, sc)
+    dp = host.direct_params(**ESTIMATORS[name])
+    spp = 8
+    P = dict(spp=spp, seed_lo=0x5EED000B, flags=abi.RENDER_SUMSQ)
+    a, s2, st = host.render_shade(ds, P, shader, dp)
+    oa, os2 = oracle.DirectScene(sc).render_shade(shader, P, dp, sumsq=True)
+    gm, om = a / spp, oa / spp
+    var = np.maximum(s2 / spp - gm ** 2, 0) / spp + np.maximum(os2 / spp - om ** 2, 0) / spp
+    z = np.abs(gm - om) / np.sqrt(var + 1e-18)
+    close = (np.abs(gm - om) <= 1e-5 * np.maximum(np.abs(om), 1e-3)).all(2)
+    iz = abs(gm.mean() - om.mean()) / np.sqrt(var.sum() / var.size ** 2)
+    print(f"shader {shader} {name}: pixels equal to 1e-5 {close.mean():.3f}, max z {z.max():.2f}, "
+          f"image z {iz:.2f}, rays {st['rays_extend']}, chunks {st['chunks']}")
+    assert st["paths"] == 24 * 24 * spp
+    assert close.mean() >= 0.8
+    assert (z <= 4.0).mean() >= 0.99
+    assert iz < 4.0
+
+
+def test_explicit_roulette_stop_equals_direct_only(ctx):
+    sc = scenes.direct_room(width=24, height=24)
+    ds = abi.DeviceScene(ctx, sc)
+    dp = host.direct_params(abi.DIRECT_MISCV)
+    P = dict(spp=1, seed_lo=0x5EED000C)
+    ex, _, _ = host.render_shade(ds, P, abi.SHADE_EXPLICIT, dp)
+    do, _, _ = host.render_direct(ds, P, dp)
+    stop = np.zeros((24, 24), bool)
+    for idx in range(24 * 24):
+        r = oracle.philox([idx, 0, 0xFF, abi.SHADE_STREAM], [0x5EED000C, 0])
+        stop[idx // 24, idx % 24] = (r[0] + 0.5) / 2 ** 32 < 0.3
+    np.testing.assert_array_equal(ex[stop], do[stop])
+
+
+@pytest.mark.parametrize("shader", [abi.SHADE_EXPLICIT, abi.SHADE_IMPLICIT])
+def test_shader_deterministic_and_split_invariant(ctx, shader):
+    sc = scenes.direct_room(width=16, height=12)
+    ds = abi.DeviceScene(ctx, sc)
+    dp = host.direct_params(abi.DIRECT_MISCV)
+
+    def run(spp, off):
+        return host.render_shade(ds, dict(spp=spp, sample_offset=off, seed_lo=3), shader, dp)[0]
+
+    a1, a2 = run(8, 0), run(8, 0)
+    np.testing.assert_array_equal(a1, a2)
+    np.testing.assert_allclose(run(3, 0) + run(5, 3), a1, rtol=1e-12, atol=1e-14)
+
+
+def test_shader_errors(ctx):
+    sc = scenes.direct_room(width=8, height=8)
+    ds = abi.DeviceScene(ctx, sc)
+    with pytest.raises(abi.ChkError, match="shader"):
+        host.render_shade(ds, dict(spp=1), 7, host.direct_params(abi.DIRECT_MIS))
+    with pytest.raises(abi.ChkError, match="null params"):
+        host.render_shade(ds, dict(spp=1), abi.SHADE_EXPLICIT, None)
+    a, _, _ = host.render_shade(ds, dict(spp=2), abi.SHADE_IMPLICIT, None)  # no estimator needed
+    assert np.isfinite(a).all()
