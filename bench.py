@@ -184,12 +184,14 @@ def ltc_leg(args, ctx):
 
 
 def direct_leg(args, ctx, sc, ds):
-    """The reference's shipped shade() (shadeDirectOnly with
-    getLightFromMeshSource_Analytic_MISCV, shader.cpp:100-109,339-363) and the
-    MIS estimator (getLightFromAllSources_MIS) through chk_render_direct on
-    the main scene: 10 area samples per mesh light + 60 BSDF samples per
-    camera hit, FP64.  The oracle's FP64 restatement of the same render runs
-    beside it on every direct_cpu_row_step-th row."""
+    """The reference's shaders through chk_render_shade on the main scene:
+    its shipped shade() (shadeDirectOnly with
+    getLightFromMeshSource_Analytic_MISCV, shader.cpp:100-109,339-363), the
+    same with getLightFromAllSources_MIS, shadeExplicit (MISCV at every
+    vertex, shader.cpp:111-207) and shadeImplicit (shader.cpp:276-337).  The
+    direct-lighting estimators take 10 area samples per mesh light + 60 BSDF
+    samples per shading point, FP64.  The oracle's FP64 restatement of the
+    same render runs beside each on every direct_cpu_row_step-th row."""
     import numpy as np
 
     from chakumrender_amd import abi, host
@@ -199,28 +201,33 @@ def direct_leg(args, ctx, sc, ds):
                        f"{10 * sc.lights.shape[0] + 60} MIS samples per camera hit, FP64"}
     params = dict(spp=args.direct_spp, seed_lo=0x5EED0009)
     image = host.Image(args.width, args.height)
-    for name, est, flags in (("miscv", abi.DIRECT_MISCV, 0), ("mis", abi.DIRECT_MIS, 0)):
-        dp = host.direct_params(est, flags=flags)
-        host.render_direct(ds, params, dp, image=image, download=False)  # warm-up
+    rows_ = (("miscv", abi.SHADE_DIRECT_ONLY, abi.DIRECT_MISCV),
+             ("mis", abi.SHADE_DIRECT_ONLY, abi.DIRECT_MIS),
+             ("explicit_miscv", abi.SHADE_EXPLICIT, abi.DIRECT_MISCV),
+             ("implicit", abi.SHADE_IMPLICIT, abi.DIRECT_MIS))
+    for name, shader, est in rows_:
+        dp = host.direct_params(est)
+        host.render_shade(ds, params, shader, dp, image=image, download=False)  # warm-up
         abi.sync()
         reps = 2
         t0 = time.perf_counter()
         for _ in range(reps):
-            _, st = host.render_direct(ds, params, dp, image=image, download=False)
+            _, st = host.render_shade(ds, params, shader, dp, image=image, download=False)
         abi.sync()
         dt = (time.perf_counter() - t0) / reps
         paths = args.width * args.height * args.direct_spp
         row = {"ms": round(dt * 1e3, 2), "mpaths_per_s": round(paths / dt / 1e6, 3),
-               "samples_per_s_g": round(paths * (10 * sc.lights.shape[0] + 60) / dt / 1e9, 3),
                "shadow_rays_per_s_g": round(st["rays_shadow"] / dt / 1e9, 3),
-               "chunks": st["chunks"]}
+               "trace_launches": st["extend_launches"], "chunks": st["chunks"]}
+        if shader == abi.SHADE_DIRECT_ONLY:
+            row["samples_per_s_g"] = round(paths * (10 * sc.lights.shape[0] + 60) / dt / 1e9, 3)
         if not args.no_cpu_baseline:
             import oracle
 
             D = oracle.DirectScene(sc)
             threads = min(16, os.cpu_count() or 1)
             t1 = time.perf_counter()
-            D.render_direct(params, dp, row_step=args.direct_cpu_row_step, threads=threads)
+            D.render_shade(shader, params, dp, row_step=args.direct_cpu_row_step, threads=threads)
             ct = time.perf_counter() - t1
             rows = len(range(0, args.height, args.direct_cpu_row_step))
             cpaths = rows * args.width * args.direct_spp
