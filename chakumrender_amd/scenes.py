@@ -142,8 +142,9 @@ def uv_sphere(center, radius, slices, stacks):
 
 
 def _dummy_triangle():
-    # prepareScene.cpp:241: Triangle(1e6, 1e6, 1e6, ...) -> all three vertices (1e6,1e6,1e6)
-    return np.full((1, 9), 1e6)
+    # prepareScene.cpp:241: Triangle(1e6, 1e6, 1e6, ...) converts each 1e6
+    # through Vec(double x) into the vertex (1e6, 0, 0)
+    return np.array([[1e6, 0, 0] * 3], np.float64)
 
 
 def _assemble(obj_tris, obj_mat, mats, light_quads, camera, name, meta):

@@ -19,6 +19,7 @@ import numpy as np
 HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "liboracle.so"
 REF_PATH = HERE / "_ref" / "libchkref.so"
+REF_ROOT = Path(os.environ.get("CHK_REFERENCE", "/root/reference"))
 LTC_TABLE = HERE.parent / "chakumrender_amd" / "data" / "ltc_ggx_64.bin"
 
 _dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
@@ -33,7 +34,7 @@ def build(ref: bool | None = None):
     """Compile liboracle.so (and the reference library when the reference
     checkout is present, or when ref=True)."""
     if ref is None:
-        ref = Path(os.environ.get("CHK_REFERENCE", "/root/reference")).is_dir()
+        ref = REF_ROOT.is_dir()
     target = ["ref"] if ref else ["all"]
     subprocess.run(["make", "-s", "-C", str(HERE), "-j8"] + target, check=True)
 

@@ -22,6 +22,9 @@
  *   MIS::getSamples                        mis.cpp:97-223
  *   shadow()                               shader.cpp:76-97
  *   loadAccels, loadLightSampler           objects.cpp:38-56, mis.cpp:225-234
+ *   initSceneParser, prepareScreenCamera, prepareObjects, prepareLightSources
+ *                                          prepareScene.cpp:1-281 (with sceneParser.cpp
+ *                                          and the vendored tinyxml2)
  */
 #include <stdint.h>
 #include <cstring>
@@ -34,6 +37,7 @@
 #include "lightSources.h"
 #include "mis.h"
 #include "objects.h"
+#include "prepareScene.h"
 
 // Defined in materialTypes.o (ltc.h is a definition-bearing header).
 void M_GGX(const float theta, const float alpha, mat3 &M, mat3 &Minv, double &amplitude);
@@ -299,6 +303,104 @@ void ref_direct_stats(int est, uint32_t cv, uint32_t reps, uint32_t batches, uin
       var[3 * i + c] = batches > 1 ? v2[c] / (batches - 1) / batches : 0.0;
     }
   }
+}
+
+/* The reference's scene front end on one XML file, in main.cpp's order
+ * (main.cpp:33-45): initSceneParser, prepareScreenCamera, prepareObjects,
+ * prepareLightSources.  Outputs (capacity `cap` triangles / spheres):
+ *   counts  [n_tris, n_mesh_lights, n_tri_lights, n_spheres, n_sphere_sources,
+ *            n_point_sources, width, height]
+ *   cam     camPos, camDir, cx, cy (12 doubles)
+ *   tris    vTriangleList in order: A, B, C (9 doubles each)
+ *   tmat    per triangle 15 doubles: brdf, light, specularCoef, alpha,
+ *           phongExp, intIOR, extIOR, reflectance, c[3], radiance[3]
+ *   mesh    triangles per mesh light, then per triangle light (1 each)
+ *   sph     vSphereList: centre, radius, then the 15 material doubles
+ *   ssrc    sphere sources: centre, radius, radiance[3] (7 doubles each)
+ * Returns 0 on a parse error. */
+static void put_mat(double *o, const MaterialType &m, const Vec &c, double refl) {
+  o[0] = m.b;
+  o[1] = m.l;
+  o[2] = m.specularCoef;
+  o[3] = m.alpha;
+  o[4] = m.phongExp;
+  o[5] = m.intIOR;
+  o[6] = m.extIOR;
+  o[7] = refl;
+  o[8] = c.x;
+  o[9] = c.y;
+  o[10] = c.z;
+  o[11] = m.radiance.x;
+  o[12] = m.radiance.y;
+  o[13] = m.radiance.z;
+  o[14] = 0;
+}
+
+int ref_parse_scene(const char *path, uint32_t *counts, double *cam, double *tris, double *tmat,
+                    uint32_t *mesh, double *sph, double *ssrc, uint32_t cap) {
+  vTriangleList.clear();
+  vSphereList.clear();
+  if (!initSceneParser(path)) return 0;
+  uint32_t w, h;
+  Vec camPos, camDir, cx, cy;
+  if (!prepareScreenCamera(w, h, camPos, camDir, cx, cy, false)) return 0;
+  if (!prepareObjects(false)) return 0;
+  if (!prepareLightSources(false)) return 0;
+  const Vec cv[4] = {camPos, camDir, cx, cy};
+  for (int k = 0; k < 4; ++k) {
+    cam[3 * k] = cv[k].x;
+    cam[3 * k + 1] = cv[k].y;
+    cam[3 * k + 2] = cv[k].z;
+  }
+  uint32_t nm, nt, nss;
+  MeshLight *ml = lSource->getMeshLightPtr(nm);
+  TriLight *tl = lSource->getTriLightPtr(nt);
+  SphereSource *ss = lSource->getSphereSourcePtr(nss);
+  const uint32_t nT = (uint32_t)vTriangleList.size(), nS = (uint32_t)vSphereList.size();
+  if (nT > cap || nS > cap || nm + nt > cap || nss > cap) return 0;
+  for (uint32_t i = 0; i < nT; ++i) {
+    const Triangle &t = vTriangleList[i];
+    const Vec v[3] = {t.A, t.B, t.C};
+    for (int k = 0; k < 3; ++k) {
+      tris[9 * i + 3 * k] = v[k].x;
+      tris[9 * i + 3 * k + 1] = v[k].y;
+      tris[9 * i + 3 * k + 2] = v[k].z;
+    }
+    put_mat(tmat + 15 * (size_t)i, t.m, t.c, t.reflectance);
+  }
+  for (uint32_t j = 0; j < nm; ++j) mesh[j] = (uint32_t)ml[j].mesh.size();
+  for (uint32_t j = 0; j < nt; ++j) mesh[nm + j] = 1;
+  for (uint32_t i = 0; i < nS; ++i) {
+    const Sphere &q = vSphereList[i];
+    double *o = sph + 19 * (size_t)i;
+    o[0] = q.p.x;
+    o[1] = q.p.y;
+    o[2] = q.p.z;
+    o[3] = q.r;
+    put_mat(o + 4, q.m, q.c, q.reflectance);
+  }
+  for (uint32_t i = 0; i < nss; ++i) {
+    double *o = ssrc + 7 * (size_t)i;
+    o[0] = ss[i].p.p.x;
+    o[1] = ss[i].p.p.y;
+    o[2] = ss[i].p.p.z;
+    o[3] = ss[i].p.r;
+    o[4] = ss[i].radiance.x;
+    o[5] = ss[i].radiance.y;
+    o[6] = ss[i].radiance.z;
+  }
+  counts[0] = nT;
+  counts[1] = nm;
+  counts[2] = nt;
+  counts[3] = nS;
+  counts[4] = nss;
+  counts[5] = sceneParser->findPointSources(false);
+  counts[6] = w;
+  counts[7] = h;
+  delete[] ml;
+  delete[] tl;
+  delete[] ss;
+  return 1;
 }
 
 }  // extern "C"

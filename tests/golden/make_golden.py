@@ -11,6 +11,10 @@ Fixtures (inputs and expected outputs only):
     brdf.npz          BasePrimitive::brdf and MaterialType::pdfEval
     ltc.npz           M_GGX and LightSource::getLightFromMeshSource_Analytic
     spheres.npz       BvhAccel::intersectS over Sphere primitives
+    scene_xml.npz     the reference's two scene files (inputs) and what its own
+                      front end builds from them: triangle list with
+                      materials, mesh-light sizes, spheres, sphere sources,
+                      camera
     direct.npz        MIS::getSamples (samples and weights),
                       getLightFromMeshSource_Analytic, and the mean and
                       batch-means variance of 8192 calls of
@@ -236,18 +240,58 @@ def gen_direct():
                         cv_mean=cv_m, cv_var=cv_v, cv_samples=16, reps=8192)
 
 
+def gen_scene_xml(R):
+    """The reference's own front end (initSceneParser, prepareScreenCamera,
+    prepareObjects, prepareLightSources) on its two scene files; the files
+    themselves are stored as the inputs."""
+    import ctypes as C
+
+    dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+    up = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+    R.ref_parse_scene.restype = C.c_int
+    R.ref_parse_scene.argtypes = [C.c_char_p, up, dp, dp, dp, up, dp, dp, C.c_uint32]
+    out = {}
+    cap = 4096
+    for name in ("scene0", "scene1"):
+        path = oracle.REF_ROOT / f"{name}.xml"
+        counts = np.zeros(8, np.uint32)
+        cam, tris, tmat = np.zeros(12), np.zeros((cap, 9)), np.zeros((cap, 15))
+        mesh, sph, ssrc = np.zeros(cap, np.uint32), np.zeros((cap, 19)), np.zeros((cap, 7))
+        with oracle._quiet():
+            ok = R.ref_parse_scene(str(path).encode(), counts, cam, tris, tmat, mesh, sph, ssrc, cap)
+        assert ok, name
+        n, nm, nt, ns, nss = (int(c) for c in counts[:5])
+        out.update({f"{name}_xml": np.array(path.read_text()), f"{name}_counts": counts,
+                    f"{name}_cam": cam, f"{name}_tris": tris[:n], f"{name}_tmat": tmat[:n],
+                    f"{name}_mesh": mesh[:nm + nt], f"{name}_spheres": sph[:ns],
+                    f"{name}_sphere_sources": ssrc[:nss]})
+    np.savez_compressed(OUT / "scene_xml.npz", **out)
+
+
 def main():
+    """python tests/golden/make_golden.py [fixture ...]: all fixtures, or the
+    named ones (bvh_closest, shadow, brdf, ltc, spheres, direct, scene_xml)."""
     oracle.build(ref=True)
     R = oracle.ref_lib()
     if R is None:
         raise SystemExit("oracle/_ref/libchkref.so missing: the reference checkout is needed")
+    want = set(sys.argv[1:])
+
+    def on(name):
+        return not want or name in want
+
     rng = np.random.default_rng(20261203)
-    sc = gen_bvh(R, rng)
-    gen_shadow(R, rng, sc)
-    gen_brdf(R, rng)
-    gen_ltc(R, rng)
-    gen_spheres(R, np.random.default_rng(20261204))  # own stream: earlier fixtures unchanged
-    gen_direct()
+    sc = gen_bvh(R, rng)  # the fixtures below share this generator's stream
+    if on("shadow") or on("brdf") or on("ltc") or not want:
+        gen_shadow(R, rng, sc)
+        gen_brdf(R, rng)
+        gen_ltc(R, rng)
+    if on("spheres"):
+        gen_spheres(R, np.random.default_rng(20261204))  # own stream: earlier fixtures unchanged
+    if on("direct"):
+        gen_direct()
+    if on("scene_xml"):
+        gen_scene_xml(R)
     for p in sorted(OUT.glob("*.npz")):
         print(p.name, p.stat().st_size)
 

@@ -223,3 +223,23 @@ def test_shader_errors(ctx):
         host.render_shade(ds, dict(spp=1), abi.SHADE_EXPLICIT, None)
     a, _, _ = host.render_shade(ds, dict(spp=2), abi.SHADE_IMPLICIT, None)  # no estimator needed
     assert np.isfinite(a).all()
+
+
+def test_reference_scene0_renders_like_oracle(ctx, golden):
+    """The reference's own scene0.xml through the XML front end and the
+    shipped shader (shadeDirectOnly with MISCV), reduced to 128 x 96."""
+    from chakumrender_amd import scene_xml
+
+    G = golden("scene_xml.npz")
+    sc = scene_xml.load_scene_xml(str(G["scene0_xml"]), strict=True).with_camera(width=128,
+                                                                                 height=96)
+    ds = abi.DeviceScene(ctx, sc)
+    dp = host.direct_params(abi.DIRECT_MISCV)
+    P = dict(spp=2, seed_lo=0x5EED000D, flags=abi.RENDER_SUMSQ)
+    a, _, st = host.render_direct(ds, P, dp)
+    oa, _ = oracle.DirectScene(sc).render_direct(P, dp)
+    close = (np.abs(a - oa) <= 1e-5 * np.maximum(np.abs(oa), 1e-3)).all(2)
+    print(f"scene0: pixels equal to 1e-5 {close.mean():.3f}, mean {a.mean():.5f} vs {oa.mean():.5f}")
+    assert a.mean() > 0
+    assert close.mean() >= 0.95
+    assert abs(a.mean() - oa.mean()) <= 2e-3 * abs(oa.mean())
