@@ -32,6 +32,59 @@ class chk_mesh_light(C.Structure):
     _fields_ = [("first_tri", C.c_uint32), ("n_tris", C.c_uint32)]
 
 
+class chk_sphere(C.Structure):
+    _fields_ = [("center", C.c_double * 3), ("radius", C.c_double), ("mat", C.c_uint32),
+                ("pad", C.c_uint32)]
+
+
+class chk_point_source(C.Structure):
+    _fields_ = [("pos", C.c_double * 3), ("power", C.c_double * 3)]
+
+
+class chk_scene_desc(C.Structure):
+    _fields_ = [("tris", C.c_void_p), ("tri_mat", C.c_void_p), ("mats", C.c_void_p),
+                ("mesh_lights", C.c_void_p), ("tri_lights", C.c_void_p), ("spheres", C.c_void_p),
+                ("sphere_sources", C.c_void_p), ("point_sources", C.c_void_p),
+                ("n_tris", C.c_uint32), ("n_mats", C.c_uint32), ("n_mesh_lights", C.c_uint32),
+                ("n_tri_lights", C.c_uint32), ("n_spheres", C.c_uint32),
+                ("n_sphere_sources", C.c_uint32), ("n_point_sources", C.c_uint32),
+                ("reserved", C.c_uint32)]
+
+
+def scene_desc(scene):
+    """chk_scene_desc of a scenes.Scene and the arrays it points into (keep
+    them alive for the call)."""
+    arrs = {
+        "tris": np.ascontiguousarray(scene.tris, np.float32).reshape(-1, 9),
+        "tri_mat": np.ascontiguousarray(scene.tri_mat, np.uint32),
+        "mats": np.ascontiguousarray(scene.mats),
+        "mesh_lights": np.ascontiguousarray(scene.lights),
+        "tri_lights": np.ascontiguousarray(getattr(scene, "tri_lights", np.zeros(0)), np.uint32),
+        "spheres": np.ascontiguousarray(getattr(scene, "spheres", np.zeros(0, _SPHERE_NP))),
+        "sphere_sources": np.ascontiguousarray(getattr(scene, "sphere_sources", np.zeros(0)),
+                                               np.uint32),
+        "point_sources": np.ascontiguousarray(getattr(scene, "point_sources",
+                                                      np.zeros(0, _POINT_NP))),
+    }
+    assert arrs["spheres"].dtype.itemsize == C.sizeof(chk_sphere)
+    assert arrs["point_sources"].dtype.itemsize == C.sizeof(chk_point_source)
+    d = chk_scene_desc()
+    for k, a in arrs.items():
+        setattr(d, k, a.ctypes.data if a.shape[0] else None)
+    d.n_tris = arrs["tris"].shape[0]
+    d.n_mats = arrs["mats"].shape[0]
+    d.n_mesh_lights = arrs["mesh_lights"].shape[0]
+    d.n_tri_lights = arrs["tri_lights"].shape[0]
+    d.n_spheres = arrs["spheres"].shape[0]
+    d.n_sphere_sources = arrs["sphere_sources"].shape[0]
+    d.n_point_sources = arrs["point_sources"].shape[0]
+    return d, arrs
+
+
+_SPHERE_NP = np.dtype([("center", "<f8", (3,)), ("radius", "<f8"), ("mat", "<u4"), ("pad", "<u4")])
+_POINT_NP = np.dtype([("pos", "<f8", (3,)), ("power", "<f8", (3,))])
+
+
 class chk_camera(C.Structure):
     _fields_ = [("pos", C.c_float * 3), ("dir", C.c_float * 3), ("up", C.c_float * 3),
                 ("fovx_deg", C.c_float), ("fovy_deg", C.c_float), ("width", C.c_uint32),
@@ -121,6 +174,7 @@ SIGNATURES = {
     "chk_destroy": (C.c_int, [_vp]),
     "chk_set_ltc_table": (C.c_int, [_vp, _vp, C.c_size_t]),
     "chk_scene_create": (C.c_int, [_vp, _vp, _u32, _vp, _vp, _u32, _vp, _u32, C.POINTER(_vp)]),
+    "chk_scene_create_desc": (C.c_int, [_vp, C.POINTER(chk_scene_desc), C.POINTER(_vp)]),
     "chk_scene_destroy": (C.c_int, [_vp]),
     "chk_scene_get_info": (C.c_int, [_vp, C.POINTER(chk_scene_info)]),
     "chk_trace_closest": (C.c_int, [_vp, _vp, _u32, _vp, _vp]),
@@ -310,14 +364,10 @@ class DeviceScene:
         self.ctx = ctx
         self.L = ctx.L
         self.scene = scene
-        tris = np.ascontiguousarray(scene.tris, np.float32)
-        tri_mat = np.ascontiguousarray(scene.tri_mat, np.uint32)
-        mats = np.ascontiguousarray(scene.mats)
-        lights = np.ascontiguousarray(scene.lights)
+        d, keep = scene_desc(scene)
         h = C.c_void_p()
-        check(self.L.chk_scene_create(ctx.h, tris.ctypes.data, tris.shape[0], tri_mat.ctypes.data,
-                                      mats.ctypes.data, mats.shape[0], lights.ctypes.data,
-                                      lights.shape[0], C.byref(h)), "chk_scene_create")
+        check(self.L.chk_scene_create_desc(ctx.h, C.byref(d), C.byref(h)), "chk_scene_create_desc")
+        del keep
         self.h = h
 
     def info(self) -> dict:

@@ -94,6 +94,26 @@ __global__ void k_fill_zero(uint4 *p, size_t n16, unsigned char *tail, size_t nt
 using namespace chk;
 
 
+namespace {
+// The path tracer of chk_render and its batched and learned forms traces
+// triangles and samples mesh lights only.
+chk_status reject_extras(const chk_scene_s *sc, const char *who) {
+  if (sc && sc->extras) {
+    set_error(std::string(who) + ": the scene has spheres or triangle, sphere or point sources; "
+              "only the direct-lighting entry points and chk_render_shade render those");
+    return CHK_EINVAL;
+  }
+  return CHK_OK;
+}
+chk_status reject_extras(const chk_request *reqs, uint32_t n, const char *who) {
+  for (uint32_t i = 0; i < n; ++i) {
+    const chk_status s = reject_extras(reqs[i].scene, who);
+    if (s != CHK_OK) return s;
+  }
+  return CHK_OK;
+}
+}  // namespace
+
 extern "C" {
 
 const char *chk_status_string(chk_status s) {
@@ -222,9 +242,33 @@ chk_status chk_destroy(chk_ctx_t c) {
 chk_status chk_scene_create(chk_ctx_t ctx, const float *tris, uint32_t n_tris,
                             const uint32_t *tri_mat, const chk_material *mats, uint32_t n_mats,
                             const chk_mesh_light *lights, uint32_t n_lights, chk_scene_t *out) {
+  chk_scene_desc D;
+  std::memset(&D, 0, sizeof(D));
+  D.tris = tris;
+  D.n_tris = n_tris;
+  D.tri_mat = tri_mat;
+  D.mats = mats;
+  D.n_mats = n_mats;
+  D.mesh_lights = lights;
+  D.n_mesh_lights = n_lights;
+  return chk_scene_create_desc(ctx, &D, out);
+}
+
+chk_status chk_scene_create_desc(chk_ctx_t ctx, const chk_scene_desc *desc, chk_scene_t *out) {
   CHK_TRY_BEGIN
-  if (!ctx || !out || (n_tris && (!tris || !tri_mat)) || (n_mats && !mats) ||
-      (n_lights && !lights)) {
+  if (!ctx || !out || !desc) {
+    set_error("chk_scene_create: null argument");
+    return CHK_EINVAL;
+  }
+  const chk_scene_desc &D = *desc;
+  const float *tris = D.tris;
+  const uint32_t *tri_mat = D.tri_mat;
+  const chk_material *mats = D.mats;
+  const uint32_t n_tris = D.n_tris, n_mats = D.n_mats;
+  const uint32_t n_lights = D.n_mesh_lights;
+  if ((n_tris && (!tris || !tri_mat)) || (n_mats && !mats) || (n_lights && !D.mesh_lights) ||
+      (D.n_tri_lights && !D.tri_lights) || (D.n_spheres && !D.spheres) ||
+      (D.n_sphere_sources && !D.sphere_sources) || (D.n_point_sources && !D.point_sources)) {
     set_error("chk_scene_create: null argument");
     return CHK_EINVAL;
   }
@@ -235,6 +279,48 @@ chk_status chk_scene_create(chk_ctx_t ctx, const float *tris, uint32_t n_tris,
                 std::to_string(tri_mat[i]) + " >= n_mats");
       return CHK_EINVAL;
     }
+  if (D.n_spheres > kMaxSpheres || D.n_tri_lights > kMaxSources ||
+      D.n_sphere_sources > kMaxSources || D.n_point_sources > kMaxSources) {
+    set_error("chk_scene_create: more than " + std::to_string(kMaxSpheres) + " spheres or " +
+              std::to_string(kMaxSources) + " triangle, sphere or point sources");
+    return CHK_ECAPACITY;
+  }
+  // light runs: the mesh lights, then one run per triangle light
+  std::vector<chk_mesh_light> runs(D.mesh_lights, D.mesh_lights + n_lights);
+  for (uint32_t j = 0; j < D.n_tri_lights; ++j) {
+    const uint32_t o = D.tri_lights[j];
+    if (o >= n_tris || mats[tri_mat[o]].light == CHK_LIGHT_NONE) {
+      set_error("chk_scene_create: triangle light " + std::to_string(j) +
+                " names no emitter triangle");
+      return CHK_EINVAL;
+    }
+    runs.push_back(chk_mesh_light{o, 1u});
+  }
+  for (uint32_t j = 0; j < D.n_spheres; ++j) {
+    const chk_sphere &q = D.spheres[j];
+    if (!(q.radius > 0) || !std::isfinite(q.radius) || !std::isfinite(q.center[0]) ||
+        !std::isfinite(q.center[1]) || !std::isfinite(q.center[2]) || q.mat >= n_mats) {
+      set_error("chk_scene_create: sphere " + std::to_string(j) +
+                " has a bad centre, radius or material");
+      return CHK_EINVAL;
+    }
+  }
+  for (uint32_t j = 0; j < D.n_sphere_sources; ++j) {
+    const uint32_t s = D.sphere_sources[j];
+    if (s >= D.n_spheres || mats[D.spheres[s].mat].light == CHK_LIGHT_NONE) {
+      set_error("chk_scene_create: sphere source " + std::to_string(j) +
+                " names no emitter sphere");
+      return CHK_EINVAL;
+    }
+  }
+  for (uint32_t j = 0; j < D.n_point_sources; ++j)
+    for (int c = 0; c < 3; ++c)
+      if (!std::isfinite(D.point_sources[j].pos[c]) || !std::isfinite(D.point_sources[j].power[c])) {
+        set_error("chk_scene_create: point source " + std::to_string(j) + " is not finite");
+        return CHK_EINVAL;
+      }
+  const chk_mesh_light *lights = runs.data();
+  const uint32_t n_runs = (uint32_t)runs.size();
   for (uint32_t j = 0; j < n_lights; ++j) {
     if (lights[j].n_tris == 0 || (uint64_t)lights[j].first_tri + lights[j].n_tris > n_tris) {
       set_error("chk_scene_create: mesh light " + std::to_string(j) + " is empty or out of range");
@@ -275,7 +361,7 @@ chk_status chk_scene_create(chk_ctx_t ctx, const float *tris, uint32_t n_tris,
   std::vector<double> larea64;
   std::vector<LightTri64> ltri64;
   uint32_t used_lights = 0;
-  for (uint32_t j = 0; j < n_lights; ++j) {
+  for (uint32_t j = 0; j < n_runs; ++j) {
     LightRec R;
     R.first = (uint32_t)ltris.size();
     R.count = 0;
@@ -295,7 +381,8 @@ chk_status chk_scene_create(chk_ctx_t ctx, const float *tris, uint32_t n_tris,
       area += a;
     }
     if (!(area > 0)) {
-      set_error("chk_scene_create: mesh light " + std::to_string(j) + " has zero area");
+      set_error(std::string("chk_scene_create: ") + (j < n_lights ? "mesh light " : "triangle light ") +
+                std::to_string(j < n_lights ? j : j - n_lights) + " has zero area");
       return CHK_EINVAL;
     }
     double cdf = 0;
@@ -364,7 +451,20 @@ chk_status chk_scene_create(chk_ctx_t ctx, const float *tris, uint32_t n_tris,
   const size_t b_oi = align256(oinfo.size() * 4);
   const size_t b_la = align256(std::max<size_t>(larea64.size(), 1) * sizeof(double));
   const size_t b_l64 = align256(std::max<size_t>(ltri64.size(), 1) * sizeof(LightTri64));
-  const size_t total = b_nodes + b_tris + b_info + b_mats + b_lt + b_lr + b_oi + b_la + b_l64;
+  // spheres and the other sources
+  std::vector<SphereRec> sph(std::max<uint32_t>(D.n_spheres, 1u));
+  std::vector<uint32_t> smat(std::max<uint32_t>(D.n_spheres, 1u), 0u);
+  for (uint32_t j = 0; j < D.n_spheres; ++j) {
+    for (int c = 0; c < 3; ++c) sph[j].c[c] = D.spheres[j].center[c];
+    sph[j].r = D.spheres[j].radius;
+    smat[j] = D.spheres[j].mat;
+  }
+  const size_t b_sp = align256(sph.size() * sizeof(SphereRec));
+  const size_t b_sm = align256(smat.size() * 4);
+  const size_t b_ss = align256(std::max<uint32_t>(D.n_sphere_sources, 1u) * 4);
+  const size_t b_pt = align256(std::max<uint32_t>(D.n_point_sources, 1u) * sizeof(chk_point_source));
+  const size_t total = b_nodes + b_tris + b_info + b_mats + b_lt + b_lr + b_oi + b_la + b_l64 +
+                       b_sp + b_sm + b_ss + b_pt;
   void *blob = nullptr;
   if (hipMalloc(&blob, total) != hipSuccess) {
     set_error("chk_scene_create: cannot allocate " + std::to_string(total) + " device bytes");
@@ -408,13 +508,34 @@ chk_status chk_scene_create(chk_ctx_t ctx, const float *tris, uint32_t n_tris,
   sc->d.light64 = (const LightTri64 *)p;
   if (e == hipSuccess && !ltri64.empty())
     e = hipMemcpy(p, ltri64.data(), ltri64.size() * sizeof(LightTri64), hipMemcpyHostToDevice);
+  p += b_l64;
+  sc->d.spheres = (const SphereRec *)p;
+  if (e == hipSuccess) e = hipMemcpy(p, sph.data(), sph.size() * sizeof(SphereRec), hipMemcpyHostToDevice);
+  p += b_sp;
+  sc->d.sphere_mat = (const uint32_t *)p;
+  if (e == hipSuccess) e = hipMemcpy(p, smat.data(), smat.size() * 4, hipMemcpyHostToDevice);
+  p += b_sm;
+  sc->d.sph_src = (const uint32_t *)p;
+  if (e == hipSuccess && D.n_sphere_sources)
+    e = hipMemcpy(p, D.sphere_sources, D.n_sphere_sources * 4, hipMemcpyHostToDevice);
+  p += b_ss;
+  sc->d.points = (const chk_point_source *)p;
+  if (e == hipSuccess && D.n_point_sources)
+    e = hipMemcpy(p, D.point_sources, D.n_point_sources * sizeof(chk_point_source),
+                  hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     set_error(std::string("chk_scene_create: upload: ") + hipGetErrorString(e));
     (void)hipFree(blob);
     delete sc;
     return CHK_EDEVICE;
   }
-  sc->d.n_lights = used_lights;
+  sc->d.n_lights = n_lights;
+  sc->d.n_tri_lights = D.n_tri_lights;
+  sc->d.n_spheres = D.n_spheres;
+  sc->d.n_sph_src = D.n_sphere_sources;
+  sc->d.n_points = D.n_point_sources;
+  sc->extras = D.n_tri_lights || D.n_spheres || D.n_sphere_sources || D.n_point_sources;
+  (void)used_lights;
   sc->d.n_leaf_tris = m;
   sc->d.n_tris = n_tris;
   sc->n_mats = n_mats;
@@ -440,7 +561,7 @@ chk_status chk_scene_create(chk_ctx_t ctx, const float *tris, uint32_t n_tris,
   I.max_depth = bvh.stats.max_depth;
   I.stack_bound = bvh.stats.stack_bound;
   I.stack_size = (uint32_t)stack;
-  I.n_lights = used_lights;
+  I.n_lights = n_lights;
   I.device_bytes = total;
   I.sah_cost = bvh.stats.sah_cost;
   I.build_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
@@ -580,6 +701,7 @@ chk_status chk_render(chk_scene_t sc, const chk_camera *camera, const chk_render
     set_error("chk_render: null scene");
     return CHK_EINVAL;
   }
+  if (reject_extras(sc, "chk_render") != CHK_OK) return CHK_EINVAL;
   CHK_HIP(hipSetDevice(sc->ctx->device));
   return render_impl(sc, camera, params, accum, sumsq, stats, false, nullptr, nullptr,
                      (hipStream_t)stream);
@@ -594,6 +716,7 @@ chk_status chk_render_count(chk_scene_t sc, const chk_camera *camera,
     set_error("chk_render_count: null scene or stats");
     return CHK_EINVAL;
   }
+  if (reject_extras(sc, "chk_render_count") != CHK_OK) return CHK_EINVAL;
   CHK_HIP(hipSetDevice(sc->ctx->device));
   return render_impl(sc, camera, params, accum, nullptr, stats, true, nullptr, nullptr,
                      (hipStream_t)stream);
@@ -609,6 +732,7 @@ chk_status chk_render_learned(chk_scene_t sc, const chk_camera *camera,
     set_error("chk_render_learned: null scene or proposal");
     return CHK_EINVAL;
   }
+  if (reject_extras(sc, "chk_render_learned") != CHK_OK) return CHK_EINVAL;
   CHK_HIP(hipSetDevice(sc->ctx->device));
   return render_impl(sc, camera, params, accum, sumsq, stats, false, proposal, train,
                      (hipStream_t)stream);
@@ -623,6 +747,7 @@ chk_status chk_render_batch(chk_ctx_t ctx, const chk_request *reqs, uint32_t n,
     set_error("chk_render_batch: null context or requests");
     return CHK_EINVAL;
   }
+  if (reject_extras(reqs, n, "chk_render_batch") != CHK_OK) return CHK_EINVAL;
   CHK_HIP(hipSetDevice(ctx->device));
   return batch_impl(ctx, reqs, n, tuning, latency_ms, stats, (hipStream_t)stream);
   CHK_TRY_END
@@ -635,6 +760,7 @@ chk_status chk_batch_autotune(chk_ctx_t ctx, const chk_request *reqs, uint32_t n
     set_error("chk_batch_autotune: null argument");
     return CHK_EINVAL;
   }
+  if (reject_extras(reqs, n, "chk_batch_autotune") != CHK_OK) return CHK_EINVAL;
   CHK_HIP(hipSetDevice(ctx->device));
   return batch_autotune(ctx, reqs, n, best, best_ms, (hipStream_t)stream);
   CHK_TRY_END

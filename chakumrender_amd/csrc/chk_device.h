@@ -187,6 +187,9 @@ __device__ inline T beck_D(T a2, T c, T s2) {
 // Phong D (materialTypes.cpp:39,107): (e+2)/(2 pi) cos^e
 template <typename T>
 __device__ inline T phong_D(T e, T c, T s2) {
+  // cos <= 0 (a direction below the surface, which pdfEval still weighs):
+  // pow's own value, finite for an integral exponent and NaN otherwise
+  if (!(c > T(0))) return (e + T(2)) * (T(0.5) * inv_pi_v<T>()) * pw(c, e);
   const T omc = s2 / (T(1) + c);  // 1 - cos, accurate near 1
   return (e + T(2)) * (T(0.5) * inv_pi_v<T>()) * ex(e * lg1p(-omc));
 }

@@ -115,6 +115,159 @@ __device__ inline d3 light_point(const DeviceScene &S, uint32_t j, u4 r, uint32_
          f4d(lt.c_leaf) * e2 * e1;
 }
 
+// ------------------------------------------------------------ spheres, sources
+// Emitter code of a sample item: a light-triangle index (mesh or triangle
+// light), or a sphere-source / point-source index under its tag.  kOccS
+// marks a sample whose shadow ray a sphere blocks.
+constexpr uint32_t kTagSph = 0x10000000u, kTagPt = 0x20000000u, kTagMask = 0x30000000u;
+constexpr uint32_t kCodeMask = 0x0FFFFFFFu, kOccS = 0x80000000u;
+// A closest hit on a sphere: leaf word kHitSph | sphere, its FP64 t in the
+// hit's z (low word) and w (high word).
+constexpr uint32_t kHitSph = 0x80000000u;
+
+// Sphere::intersect (geometryPrimitives.cpp:12-27): the first root beyond
+// eps = 1e-4 (geometryPrimitives.h:39), else INF.
+__device__ inline double sph_isect(const SphereRec &q, d3 o, d3 d) {
+  const d3 op = dk(q.c[0], q.c[1], q.c[2]) - o;
+  const double b = dot(op, d);
+  double det = b * b - dot(op, op) + q.r * q.r;
+  if (det < 0) return kInf64;
+  det = sqrt(det);
+  double t = b - det;
+  if (t > 1e-4) return t;
+  t = b + det;
+  return t > 1e-4 ? t : kInf64;
+}
+// BvhAccel::intersectS (bvhAccel.cpp:151-198) as a loop over the spheres:
+// the nearest hit under the strict d < t (wave-uniform loads).
+__device__ inline uint32_t sph_closest(const DeviceScene &S, d3 o, d3 d, double *t) {
+  *t = kInf64;
+  uint32_t id = kNone;
+  for (uint32_t k = 0; k < S.n_spheres; ++k) {
+    const double dd = sph_isect(S.spheres[k], o, d);
+    if (dd < *t) {
+      *t = dd;
+      id = k;
+    }
+  }
+  return id;
+}
+// The sphere half of shadow() (shader.cpp:83-85).
+__device__ inline bool sph_occluded(const DeviceScene &S, d3 o, d3 d, double dist) {
+  double t;
+  return sph_closest(S, o, d, &t) != kNone && t + 1e-5 < dist;
+}
+
+// The sampling cone of sphere source j seen from x (mis.cpp:103-115): axis
+// w = (c - x).norm(), sin(theta_max) = r / |c - x| and the cap area
+// 2 pi (1 - cos theta_max) (domainSampler.cpp:325).  A point inside the
+// source takes theta_max = pi / 2 (the reference's asin has no value there).
+struct Cone {
+  d3 w;
+  double cos_max, cap;
+};
+__device__ inline Cone sph_cone(const DeviceScene &S, uint32_t j, d3 x) {
+  const SphereRec &q = S.spheres[S.sph_src[j]];
+  const d3 v = dk(q.c[0], q.c[1], q.c[2]) - x;
+  double sn = q.r / sqrt(dot(v, v));
+  sn = sn > 1.0 ? 1.0 : sn;
+  Cone C;
+  C.w = v * (1.0 / sqrt(dot(v, v)));
+  C.cap = 2.0 * kPi64 * (1 - cos(asin(sn)));
+  C.cos_max = sqrt(1 - sn * sn);
+  return C;
+}
+// SphericalSampler::getSolidDirectionSamples, one direction
+// (domainSampler.cpp:320-369): cos theta = 1 - A e1 / (2 pi), phi = 2 pi e2
+// in the frame (newX, newY, w) the reference builds from w.
+__device__ inline d3 cone_dir(const Cone &C, double e1, double e2) {
+  const d3 w = C.w;
+  d3 X;
+  if (w.x != 0)
+    X = dk((-w.y - w.z) / w.x, 1.0, 1.0);
+  else if (w.y != 0)
+    X = dk(1.0, -w.z / w.y, 1.0);
+  else
+    X = dk(1.0, 1.0, 0.0);
+  X = X * (1.0 / sqrt(dot(X, X)));
+  d3 Y = cross(w, X);
+  Y = Y * (1.0 / sqrt(dot(Y, Y)));
+  const double ct = 1.0 - (C.cap * e1) / (2.0 * kPi64);
+  const double st = sin(acos(ct));
+  double sp, cp;
+  sincos(2.0 * kPi64 * e2, &sp, &cp);
+  const double a = st * cp, b = st * sp;
+  return dk(a * X.x + b * Y.x + ct * w.x, a * X.y + b * Y.y + ct * w.y,
+            a * X.z + b * Y.z + ct * w.z);
+}
+// The radiance of an emitter code.
+__device__ inline d3 code_radiance(const DeviceScene &S, uint32_t code) {
+  const uint32_t j = code & kCodeMask;
+  if ((code & kTagMask) == kTagSph) {
+    const chk_material &m = S.mats[S.sphere_mat[S.sph_src[j]]];
+    return dk(m.radiance[0], m.radiance[1], m.radiance[2]);
+  }
+  if ((code & kTagMask) == kTagPt) {
+    const chk_point_source &L = S.points[j];
+    return dk(L.power[0], L.power[1], L.power[2]);
+  }
+  return f4d(S.light_tris[j].rad);
+}
+
+// Closest-hit records (float4 t, leaf word, z, w) of the shading kernels.
+__device__ inline bool hit_is_sph(float4 h) {
+  const uint32_t l = __float_as_uint(h.y);
+  return l != kNone && (l & kHitSph);
+}
+__device__ inline double hit_t(float4 h) {
+  if (hit_is_sph(h)) return __hiloint2double(__float_as_int(h.w), __float_as_int(h.z));
+  return (double)h.x;
+}
+__device__ inline bool missed(const DeviceScene &S, float4 h) {
+  return !(h.x < kInf) || (!hit_is_sph(h) && __float_as_uint(h.y) >= S.n_leaf_tris);
+}
+__device__ inline uint32_t hit_mat(const DeviceScene &S, float4 h) {
+  const uint32_t l = __float_as_uint(h.y);
+  return hit_is_sph(h) ? S.sphere_mat[l & ~kHitSph] : S.leaf_info[l].x;
+}
+// the BVH leaf a ray leaving the hit skips (none for a sphere)
+__device__ inline uint32_t hit_leaf(float4 h) {
+  return hit_is_sph(h) ? kNone : __float_as_uint(h.y);
+}
+// The unit normal facing the ray: Triangle::intersect's (C-A)x(B-A) turned
+// against d (geometryPrimitives.cpp:29-71), or the sphere's c - x turned
+// against d (shader.cpp:41-47).
+__device__ inline d3 hit_normal(const DeviceScene &S, float4 h, d3 x, d3 d) {
+  d3 n;
+  if (hit_is_sph(h)) {
+    const SphereRec &q = S.spheres[__float_as_uint(h.y) & ~kHitSph];
+    n = dk(q.c[0], q.c[1], q.c[2]) - x;
+  } else {
+    const float4 *tp = S.tris + 3 * (size_t)__float_as_uint(h.y);
+    const d3 A = f4d(tp[0]), B = f4d(tp[1]), C = f4d(tp[2]);
+    n = cross(C - A, B - A);
+  }
+  n = n * (1.0 / sqrt(dot(n, n)));
+  if (dot(d, n) > 0) n = n * -1.0;
+  return n;
+}
+// A caller's primitive: a triangle index below n_tris, else a sphere
+// (n_tris + sphere index).  Returns false when it names neither.
+__device__ inline bool prim_info(const DeviceScene &S, uint32_t o, uint32_t *mat, uint32_t *leaf) {
+  if (o < S.n_tris) {
+    const uint2 oi = S.orig_info[o];
+    *leaf = oi.x;
+    *mat = oi.y;
+    return true;
+  }
+  if (o - S.n_tris < S.n_spheres) {
+    *leaf = kNone;
+    *mat = S.sphere_mat[o - S.n_tris];
+    return true;
+  }
+  return false;
+}
+
 // BasePrimitive::brdf with the LTC approximation when it is set
 // (materialTypes.cpp:9-11,76-87; geometryPrimitives.cpp:7-9).
 __device__ inline double brdf_apx(const MatT<double> &m, const PointRec &P, d3 wi) {
@@ -202,15 +355,9 @@ __global__ __launch_bounds__(kDBlock) void k_dl_setup(
   if (p >= P) return;
   const size_t g = (size_t)p0 + p;
   PointRec &R = pts[p];
-  const uint32_t o = prim[g];
   uint32_t mat = 0, leaf = kNone;
-  bool ok = o < S.n_tris;
-  if (ok) {
-    const uint2 oi = S.orig_info[o];
-    leaf = oi.x;
-    mat = oi.y;
-    ok = sampler_ok(S.mats[mat].brdf);
-  }
+  bool ok = prim_info(S, prim[g], &mat, &leaf);
+  if (ok) ok = sampler_ok(S.mats[mat].brdf);
   R.cd = 0;
   R.path = p;
   if (!ok) {
@@ -246,11 +393,17 @@ __device__ inline void no_ray(float4 *__restrict__ ray, size_t i) {
   ray[2 * i + 1] = make_float4(0.0f, 0.0f, 1.0f, __uint_as_float(kNone));
 }
 
-// 1 / (sum_k N_k pdfVal_k(s) + 1e-20) (mis.cpp:34-95,209-220).
+// 1 / (sum_k N_k pdfVal_k(s) + 1e-20) (mis.cpp:34-95,209-220), the
+// techniques in MIS::getSamples' order: the sphere sources' cones, the mesh
+// lights and the triangle lights (10 samples each), the BSDF (60).
 __device__ inline double mis_weight(const DeviceScene &S, const MatT<double> &m, d3 x, d3 n,
                                     d3 wo_ref, d3 wo, d3 s) {
   double denom = 0;
-  for (uint32_t j = 0; j < S.n_lights; ++j) {
+  for (uint32_t j = 0; j < S.n_sph_src; ++j) {  // SoAnISSP (mis.cpp:35-39)
+    const Cone C = sph_cone(S, j, x);
+    denom += (double)CHK_MIS_LIGHT_SAMPLES * (dot(s, C.w) >= C.cos_max ? 1.0 / C.cap : 0.);
+  }
+  for (uint32_t j = 0; j < S.n_lights + S.n_tri_lights; ++j) {  // SuArISME, SuArISTR
     uint32_t tri;
     double d;
     d3 N = dk(0, 0, 0);
@@ -276,6 +429,21 @@ struct Items {
   float4 *ray;
 };
 
+// Sample slots of MIS::getSamples (mis.cpp:97-191): 10 per sphere source
+// (cone), mesh light and triangle light (area), then 60 BSDF slots; the MIS
+// estimator adds one slot per point source after them.
+struct Slots {
+  uint32_t sph, mesh, tri, mis;  // first mesh slot, first triangle-light slot, first BSDF slot, total
+};
+__device__ inline Slots mis_slots(const DeviceScene &S) {
+  Slots L;
+  L.sph = S.n_sph_src * CHK_MIS_LIGHT_SAMPLES;
+  L.mesh = L.sph + S.n_lights * CHK_MIS_LIGHT_SAMPLES;
+  L.tri = L.mesh + S.n_tri_lights * CHK_MIS_LIGHT_SAMPLES;
+  L.mis = L.tri + CHK_MIS_BSDF_SAMPLES;
+  return L;
+}
+
 template <int EST>
 __global__ __launch_bounds__(kDBlock) void k_dl_samples(DeviceScene S, const PointRec *__restrict__ pts,
                                                         uint32_t P, uint32_t n_slots, uint32_t cvn,
@@ -291,9 +459,10 @@ __global__ __launch_bounds__(kDBlock) void k_dl_samples(DeviceScene S, const Poi
     return;
   }
   const MatT<double> m = load_mat<double>(S.mats + R.mat);
-  const u4 r = philox(u4{R.ca, R.cb, slot, CHK_STREAM_DIRECT + R.cd}, key0, key1);
+  const d3 o = R.x + R.n * 1e-8;  // rr.o (lightSources.cpp:42, lightSources_CV.cpp:346,412)
   if (EST == CHK_DIRECT_ANALYTIC_CV) {
     // lightSources_CV.cpp:357-385
+    const u4 r = philox(u4{R.ca, R.cb, slot, CHK_STREAM_DIRECT + R.cd}, key0, key1);
     const uint32_t j = slot / cvn;
     uint32_t tri;
     const d3 y = light_point(S, j, r, &tri);
@@ -313,18 +482,52 @@ __global__ __launch_bounds__(kDBlock) void k_dl_samples(DeviceScene S, const Poi
     I.a[i] = fa;
     I.b[i] = brdf_apx(m, R, w);
     I.c[i] = cs * c1 / (d * d);
-    I.tri[i] = tri;
-    shadow_ray(R, y, I.ray, i);
+    if (S.n_spheres && sph_occluded(S, o, w, d)) {
+      I.tri[i] = tri | kOccS;
+      no_ray(I.ray, i);
+    } else {
+      I.tri[i] = tri;
+      shadow_ray(R, y, I.ray, i);
+    }
     return;
   }
-  // MIS::getSamples sample (mis.cpp:127-138,180-191)
-  const uint32_t nls = S.n_lights * CHK_MIS_LIGHT_SAMPLES;
+  const Slots L = mis_slots(S);
+  if (EST == CHK_DIRECT_MIS && slot >= L.mis) {
+    // point source j (lightSources.cpp:47-58): radiance / (4 pi |p - x|^2)
+    // * cosine * brdf, after its shadow test
+    const uint32_t j = slot - L.mis;
+    const chk_point_source &Q = S.points[j];
+    const d3 dv = dk(Q.pos[0], Q.pos[1], Q.pos[2]) - R.x;
+    const d3 w = dv * (1.0 / sqrt(dot(dv, dv)));
+    const double cs = dot(w, R.n);
+    if (cs < 0) {
+      no_ray(I.ray, i);
+      return;
+    }
+    double fa, pdf;
+    brdf_pdf(m, R.n, R.wo_ref, R.wo, w, fa, pdf);
+    I.a[i] = cs * fa;
+    I.b[i] = 0.0;
+    I.c[i] = 4.0 * kPi64 * dot(dv, dv);
+    if (S.n_spheres && sph_occluded(S, o, w, sqrt(dot(dv, dv)))) {
+      I.tri[i] = kTagPt | j | kOccS;
+      no_ray(I.ray, i);
+    } else {
+      I.tri[i] = kTagPt | j;
+      shadow_ray(R, dk(Q.pos[0], Q.pos[1], Q.pos[2]), I.ray, i);
+    }
+    return;
+  }
+  // the sample of this slot (mis.cpp:103-191)
+  const u4 r = philox(u4{R.ca, R.cb, slot, CHK_STREAM_DIRECT + R.cd}, key0, key1);
   d3 w;
-  if (slot < nls) {
+  if (slot < L.sph) {
+    w = cone_dir(sph_cone(S, slot / CHK_MIS_LIGHT_SAMPLES, R.x), ud(r.x), ud(r.y));
+  } else if (slot < L.tri) {
     uint32_t tri;
-    w = normalize(light_point(S, slot / CHK_MIS_LIGHT_SAMPLES, r, &tri) - R.x);
+    w = normalize(light_point(S, (slot - L.sph) / CHK_MIS_LIGHT_SAMPLES, r, &tri) - R.x);
   } else {
-    const uint32_t q = slot - nls;
+    const uint32_t q = slot - L.tri;
     w = brdf_sample(m, R.n, R.wo_ref, R.wo, q < R.nb ? -1.0 : 2.0, ud(r.y), ud(r.z));
   }
   const double wt = mis_weight(S, m, R.x, R.n, R.wo_ref, R.wo, w);
@@ -333,45 +536,63 @@ __global__ __launch_bounds__(kDBlock) void k_dl_samples(DeviceScene S, const Poi
     no_ray(I.ray, i);
     return;
   }
-  const d3 o = R.x + R.n * 1e-8;  // rr.o (lightSources.cpp:42, lightSources_CV.cpp:412)
-  uint32_t pick = kNone, ptri = kNone;
+  uint32_t code = kNone;
   double pd = 0;
   if (EST == CHK_DIRECT_MIS) {
-    // lightSources.cpp:64-80: the first mesh light in order that the ray
-    // hits and shadow() passes.  Every light triangle is in the scene, so a
-    // light farther than the nearest light hit + 1e-5 is shadowed by it:
-    // the candidate is the first light within 1e-5 of the nearest hit.
+    // lightSources.cpp:60-106: the first source in the order mesh lights,
+    // sphere sources, triangle lights that the ray hits and shadow() passes.
+    // Every source's geometry is in the scene, so a source farther than the
+    // nearest source hit + 1e-5 is shadowed by it: the candidate is the
+    // first source within 1e-5 of the nearest hit.
     double dmin = kInf64;
-    for (uint32_t j = 0; j < S.n_lights; ++j) {
+    for (uint32_t j = 0; j < S.n_lights + S.n_tri_lights; ++j) {
       uint32_t tri;
       double d;
       d3 N;
       if (light_isect(S, j, o, w, &tri, &d, &N) && d < dmin) dmin = d;
     }
-    for (uint32_t j = 0; j < S.n_lights && pick == kNone; ++j) {
+    for (uint32_t j = 0; j < S.n_sph_src; ++j) {
+      const double d = sph_isect(S.spheres[S.sph_src[j]], o, w);
+      if (d < dmin) dmin = d;
+    }
+    for (uint32_t j = 0; j < S.n_lights && code == kNone; ++j) {
       uint32_t tri;
       double d;
       d3 N;
       if (light_isect(S, j, o, w, &tri, &d, &N) && d <= dmin + 1e-5) {
-        pick = j;
-        ptri = tri;
+        code = tri;
+        pd = d;
+      }
+    }
+    for (uint32_t j = 0; j < S.n_sph_src && code == kNone; ++j) {
+      const double d = sph_isect(S.spheres[S.sph_src[j]], o, w);
+      if (d < kInf64 && d <= dmin + 1e-5) {
+        code = kTagSph | j;
+        pd = d;
+      }
+    }
+    for (uint32_t j = S.n_lights; j < S.n_lights + S.n_tri_lights && code == kNone; ++j) {
+      uint32_t tri;
+      double d;
+      d3 N;
+      if (light_isect(S, j, o, w, &tri, &d, &N) && d <= dmin + 1e-5) {
+        code = tri;
         pd = d;
       }
     }
   } else {
     // lightSources_CV.cpp:420-445: the first mesh light in order hit
-    for (uint32_t j = 0; j < S.n_lights && pick == kNone; ++j) {
+    for (uint32_t j = 0; j < S.n_lights && code == kNone; ++j) {
       uint32_t tri;
       double d;
       d3 N;
       if (light_isect(S, j, o, w, &tri, &d, &N)) {
-        pick = j;
-        ptri = tri;
+        code = tri;
         pd = d;
       }
     }
   }
-  if (pick == kNone) {
+  if (code == kNone) {
     no_ray(I.ray, i);
     return;
   }
@@ -386,11 +607,17 @@ __global__ __launch_bounds__(kDBlock) void k_dl_samples(DeviceScene S, const Poi
     I.b[i] = brdf_apx(m, R, w);
     I.c[i] = cs * wt;
   }
-  I.tri[i] = ptri;
-  if (EST == CHK_DIRECT_MIS || (flags & CHK_DIRECT_VISIBILITY))
-    shadow_ray(R, o + w * pd, I.ray, i);
-  else
+  if (EST == CHK_DIRECT_MIS || (flags & CHK_DIRECT_VISIBILITY)) {
+    if (S.n_spheres && sph_occluded(S, o, w, pd)) {
+      code |= kOccS;
+      no_ray(I.ray, i);
+    } else {
+      shadow_ray(R, o + w * pd, I.ray, i);
+    }
+  } else {
     no_ray(I.ray, i);
+  }
+  I.tri[i] = code;
 }
 
 // Per point: the reference's sums in its order.  out: k x 3 doubles (the
@@ -415,23 +642,30 @@ __global__ __launch_bounds__(kDBlock) void k_dl_reduce(DeviceScene S, const Poin
           const size_t i = (size_t)(j * cvn + q) * P + p;
           const uint32_t tri = I.tri[i];
           if (tri == kNone) continue;
-          const double V = occ[i] ? 0.0 : 1.0;
-          s = s + f4d(S.light_tris[tri].rad) * ((V * I.a[i] - I.b[i]) * I.c[i]);
+          const double V = (occ[i] || (tri & kOccS)) ? 0.0 : 1.0;
+          s = s + f4d(S.light_tris[tri & ~kOccS].rad) * ((V * I.a[i] - I.b[i]) * I.c[i]);
         }
         sum = sum + s * (S.light_area[j] / cvn);
       }
       sum = sum + R.G;
     } else if (EST != CHK_DIRECT_ANALYTIC) {
-      for (uint32_t slot = 0; slot < n_slots; ++slot) {
+      // MIS: the point sources first (lightSources.cpp:47-58), then the samples
+      const uint32_t first = EST == CHK_DIRECT_MIS ? mis_slots(S).mis : 0u;
+      for (uint32_t k = 0; k < n_slots; ++k) {
+        const uint32_t slot = first + k < n_slots ? first + k : first + k - n_slots;
         const size_t i = (size_t)slot * P + p;
-        const uint32_t tri = I.tri[i];
-        if (tri == kNone) continue;
-        const double V = (use_vis && occ[i]) ? 0.0 : 1.0;
+        const uint32_t code = I.tri[i];
+        if (code == kNone) continue;
+        const double V = ((use_vis && occ[i]) || (code & kOccS)) ? 0.0 : 1.0;
+        const d3 rad = code_radiance(S, code & ~kOccS);
         if (EST == CHK_DIRECT_MIS) {
           if (V == 0.0) continue;
-          sum = sum + f4d(S.light_tris[tri].rad) * I.a[i];
+          if ((code & kTagMask) == kTagPt)
+            sum = sum + dk(rad.x / I.c[i], rad.y / I.c[i], rad.z / I.c[i]) * I.a[i];
+          else
+            sum = sum + rad * I.a[i];
         } else {
-          sum = sum + f4d(S.light_tris[tri].rad) * ((V * I.a[i] - I.b[i]) * I.c[i]);
+          sum = sum + rad * ((V * I.a[i] - I.b[i]) * I.c[i]);
         }
       }
       if (EST == CHK_DIRECT_MISCV) sum = sum + R.G;
@@ -458,13 +692,12 @@ __global__ __launch_bounds__(kDBlock) void k_mis_weights(
   const size_t i = (size_t)blockIdx.x * kDBlock + threadIdx.x;
   if (i >= (size_t)k * n_s) return;
   const size_t p = i / n_s;
-  const uint32_t o = prim[p];
-  if (o >= S.n_tris) {
+  uint32_t mat, leaf;
+  if (!prim_info(S, prim[p], &mat, &leaf)) {
     atomicOr(err, 1u);
     wout[i] = 0.0;
     return;
   }
-  const uint32_t mat = S.orig_info[o].y;
   const MatT<double> m = load_mat<double>(S.mats + mat);
   const d3 x = dk(X[3 * p], X[3 * p + 1], X[3 * p + 2]);
   const d3 n = dk(N[3 * p], N[3 * p + 1], N[3 * p + 2]);
@@ -528,9 +761,8 @@ __global__ __launch_bounds__(kDBlock) void k_dr_points(DeviceScene S, const doub
   R.flags = 0;
   R.pre = dk(0, 0, 0);
   const float4 h = hits[it];
-  const uint32_t leaf = __float_as_uint(h.y);
-  if (!(h.x < kInf) || leaf >= S.n_leaf_tris) return;
-  const uint32_t mat = S.leaf_info[leaf].x;
+  if (missed(S, h)) return;
+  const uint32_t mat = hit_mat(S, h);
   const chk_material &mt = S.mats[mat];
   if (mt.light != CHK_LIGHT_NONE) {
     d3 r = dk(mt.radiance[0], mt.radiance[1], mt.radiance[2]);
@@ -546,12 +778,8 @@ __global__ __launch_bounds__(kDBlock) void k_dr_points(DeviceScene S, const doub
   if (!sampler_ok(mt.brdf)) return;  // rejected on the host before launch
   const d3 d = dk(dirs[3 * (size_t)it], dirs[3 * (size_t)it + 1], dirs[3 * (size_t)it + 2]);
   const d3 o = dk(cam.pos[0], cam.pos[1], cam.pos[2]);
-  const float4 *tp = S.tris + 3 * (size_t)leaf;
-  const d3 A = f4d(tp[0]), B = f4d(tp[1]), C = f4d(tp[2]);
-  const d3 nn = cross(C - A, B - A);
-  d3 n = nn * (1.0 / sqrt(dot(nn, nn)));
-  if (dot(d, n) > 0) n = n * -1.0;  // Triangle::intersect's facing normal
-  setup_point(S, ltc, o + d * (double)h.x, n, d, mat, leaf, need_G, R);
+  const d3 x = o + d * hit_t(h);
+  setup_point(S, ltc, x, hit_normal(S, h, x, d), d, mat, hit_leaf(h), need_G, R);
 }
 
 // Per-pixel FP64 sums of the chunk's items in sample order (main.cpp:62,83).
@@ -618,12 +846,8 @@ __device__ inline void kill_ray(float4 *__restrict__ rays, uint32_t p) {
   rays[2 * (size_t)p].w = -1.0f;
 }
 __device__ inline bool is_emitter(const DeviceScene &S, float4 h, uint32_t *mat) {
-  const uint32_t leaf = __float_as_uint(h.y);
-  *mat = S.leaf_info[leaf].x;
+  *mat = hit_mat(S, h);
   return S.mats[*mat].light != CHK_LIGHT_NONE;
-}
-__device__ inline bool missed(const DeviceScene &S, float4 h) {
-  return !(h.x < kInf) || __float_as_uint(h.y) >= S.n_leaf_tris;
 }
 // an emitter's value as the shaders return it (materialTypes.h:34-38)
 __device__ inline d3 emitter_value(const chk_material &mt, bool physical) {
@@ -633,15 +857,6 @@ __device__ inline d3 emitter_value(const chk_material &mt, bool physical) {
   if (mx < r.y) mx = r.y;
   if (mx < r.z) mx = r.z;
   return dk(r.x / mx, r.y / mx, r.z / mx);
-}
-// the facing unit normal of leaf triangle `leaf` for direction d
-__device__ inline d3 facing_normal(const DeviceScene &S, uint32_t leaf, d3 d) {
-  const float4 *tp = S.tris + 3 * (size_t)leaf;
-  const d3 A = f4d(tp[0]), B = f4d(tp[1]), C = f4d(tp[2]);
-  const d3 nn = cross(C - A, B - A);
-  d3 n = nn * (1.0 / sqrt(dot(nn, nn)));
-  if (dot(d, n) > 0) n = n * -1.0;
-  return n;
 }
 
 // Paths start at the camera: T = 1, col = 0, tracing the camera ray.
@@ -657,6 +872,34 @@ __global__ __launch_bounds__(kDBlock) void k_sh_init(CamD cam, uint32_t P, const
   s.state = kTracing;
   s.tries = 0;
   s.pk = 0;
+}
+
+// The sphere half of the shaders' closest hit (shader.cpp:128-136,344-351):
+// after the triangle trace, the nearest sphere replaces the triangle hit when
+// tS < tT (a tie keeps the triangle).  The FP64 ray is the path's (st) or the
+// camera's (dirs).  Rays with a negative tmax are not traced.
+__global__ __launch_bounds__(kDBlock) void k_sph_closest(DeviceScene S, uint32_t n, CamD cam,
+                                                         const double *__restrict__ dirs,
+                                                         const PathSt *__restrict__ st,
+                                                         const float4 *__restrict__ rays,
+                                                         float4 *__restrict__ hits) {
+  const uint32_t p = blockIdx.x * kDBlock + threadIdx.x;
+  if (p >= n || !(rays[2 * (size_t)p].w > 0.0f)) return;
+  d3 o, d;
+  if (st) {
+    o = st[p].o;
+    d = st[p].d;
+  } else {
+    o = dk(cam.pos[0], cam.pos[1], cam.pos[2]);
+    d = dk(dirs[3 * (size_t)p], dirs[3 * (size_t)p + 1], dirs[3 * (size_t)p + 2]);
+  }
+  const float4 h = hits[p];
+  const double tT = missed(S, h) ? kInf64 : (double)h.x;
+  double tS;
+  const uint32_t id = sph_closest(S, o, d, &tS);
+  if (id != kNone && tS < tT)
+    hits[p] = make_float4((float)tS, __uint_as_float(kHitSph | id),
+                          __int_as_float(__double2loint(tS)), __int_as_float(__double2hiint(tS)));
 }
 
 // shadeExplicit's first hit (shader.cpp:128-140): a miss is 0, an emitter
@@ -702,14 +945,14 @@ __global__ __launch_bounds__(kDBlock) void k_ex_gather(DeviceScene S, const doub
   if (!act) return;
   const uint32_t k = base + (uint32_t)__popcll(b & ((1ull << lane) - 1ull));
   PathSt &s = st[p];
-  const uint32_t leaf = __float_as_uint(s.hit.y);
   PointRec &R = pts[k];
   R.ca = pix0 + p / spp;
   R.cb = s0 + p % spp;
   R.cd = vertex;
   R.path = p;
-  setup_point(S, ltc, s.o + s.d * (double)s.hit.x, facing_normal(S, leaf, s.d), s.d,
-              S.leaf_info[leaf].x, leaf, need_G, R);
+  const d3 x = s.o + s.d * hit_t(s.hit);
+  setup_point(S, ltc, x, hit_normal(S, s.hit, x, s.d), s.d, hit_mat(S, s.hit), hit_leaf(s.hit),
+              need_G, R);
   s.pk = k;
 }
 
@@ -825,9 +1068,9 @@ __global__ __launch_bounds__(kDBlock) void k_im_step(DeviceScene S, uint32_t P, 
     kill_ray(rays, p);
     return;
   }
-  const uint32_t leaf = __float_as_uint(h.y);
-  const d3 x = s.o + s.d * (double)h.x;
-  const d3 n = facing_normal(S, leaf, s.d);
+  const uint32_t leaf = hit_leaf(h);
+  const d3 x = s.o + s.d * hit_t(h);
+  const d3 n = hit_normal(S, h, x, s.d);
   const MatT<double> m = load_mat<double>(S.mats + mat);
   const d3 wo = s.d * -1.0;
   const d3 wo_ref = normalize(n * 2.0 * dot(n, wo) + s.d);
@@ -868,11 +1111,19 @@ __global__ __launch_bounds__(kDBlock) void k_sh_accum(const PathSt *__restrict__
 // ------------------------------------------------------------ host side
 inline size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
 
+// Slots per point.  The mesh-light estimators return 0 without a mesh light
+// (lightSources_CV.cpp:324,336,400), getLightFromAllSources_MIS without any
+// source (lightSources.cpp:33).
 uint32_t n_slots_of(const chk_scene_s *sc, const chk_direct_params &p) {
+  const DeviceScene &S = sc->d;
+  const uint32_t n_mis =
+      (S.n_sph_src + S.n_lights + S.n_tri_lights) * CHK_MIS_LIGHT_SAMPLES + CHK_MIS_BSDF_SAMPLES;
   switch (p.estimator) {
     case CHK_DIRECT_ANALYTIC: return 0;
-    case CHK_DIRECT_ANALYTIC_CV: return sc->d.n_lights * std::min(p.cv_samples, 1000u);
-    default: return sc->d.n_lights * CHK_MIS_LIGHT_SAMPLES + CHK_MIS_BSDF_SAMPLES;
+    case CHK_DIRECT_ANALYTIC_CV: return S.n_lights * std::min(p.cv_samples, 1000u);
+    case CHK_DIRECT_MIS:
+      return (S.n_lights || S.n_tri_lights || S.n_sph_src || S.n_points) ? n_mis + S.n_points : 0;
+    default: return S.n_lights ? n_mis : 0;
   }
 }
 
@@ -1089,7 +1340,7 @@ chk_status chk_mis_weights(chk_scene_t sc, const double *x, const double *n, con
   CHK_HIP(hipMemcpyAsync(&err, W.err, 4, hipMemcpyDeviceToHost, st));
   CHK_HIP(hipStreamSynchronize(st));
   if (err) {
-    set_error("chk_mis_weights: a prim index is not below the scene's triangle count");
+    set_error("chk_mis_weights: a prim index names no triangle or sphere");
     return CHK_EINVAL;
   }
   return CHK_OK;
@@ -1136,8 +1387,8 @@ chk_status chk_direct_light(chk_scene_t sc, const double *x, const double *n, co
   CHK_HIP(hipMemcpyAsync(&err, W.err, 4, hipMemcpyDeviceToHost, st));
   CHK_HIP(hipStreamSynchronize(st));
   if (err) {
-    set_error("chk_direct_light: a prim index is out of range or names a material without a "
-              "BSDF sampler (GGX, Beckmann, Phong, classic Phong)");
+    set_error("chk_direct_light: a prim index names no triangle or sphere, or names a material "
+              "without a BSDF sampler (GGX, Beckmann, Phong, classic Phong)");
     return CHK_EINVAL;
   }
   return CHK_OK;
@@ -1229,6 +1480,13 @@ chk_status chk_render_shade(chk_scene_t sc, const chk_camera *cam, const chk_ren
   const double *ltc = (const double *)sc->ctx->ltc_dev;
   uint64_t shadow = 0, extend = 0;
   uint32_t chunks = 0, launches = 0;
+  // spheres after every triangle trace (camera rays: paths == nullptr)
+  auto spheres = [&](uint32_t items, const PathSt *paths) -> hipError_t {
+    if (sc->d.n_spheres == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_sph_closest, dim3((items + kDBlock - 1) / kDBlock), dim3(kDBlock), 0, st,
+                       sc->d, items, C, W.cdirs, paths, W.crays, W.chits);
+    return hipGetLastError();
+  };
   for (uint32_t pix0 = 0; pix0 < n_pix; pix0 += pix_per) {
     const uint32_t np = std::min(pix_per, n_pix - pix0);
     const uint32_t items = np * spp;
@@ -1238,6 +1496,7 @@ chk_status chk_render_shade(chk_scene_t sc, const chk_camera *cam, const chk_ren
     CHK_HIP(hipGetLastError());
     CHK_HIP(launch_closest(sc, shape, W.crays, linear_range(items), W.chits, true, false, nullptr,
                            SlotScenes{}, st));
+    CHK_HIP(spheres(items, nullptr));
     extend += items;
     ++launches;
     if (shader == CHK_SHADE_DIRECT_ONLY) {
@@ -1263,6 +1522,7 @@ chk_status chk_render_shade(chk_scene_t sc, const chk_camera *cam, const chk_ren
         if (v == 3) break;
         CHK_HIP(launch_closest(sc, shape, W.crays, linear_range(items), W.chits, true, false,
                                nullptr, SlotScenes{}, st));
+        CHK_HIP(spheres(items, W.paths));
         ++launches;
         extend += items;  // upper bound: rays of paths still alive
       }
@@ -1289,6 +1549,7 @@ chk_status chk_render_shade(chk_scene_t sc, const chk_camera *cam, const chk_ren
         for (uint32_t t = 0; t < CHK_SHADE_MAX_TRIES; ++t) {
           CHK_HIP(launch_closest(sc, shape, W.crays, linear_range(items), W.chits, true, false,
                                  nullptr, SlotScenes{}, st));
+          CHK_HIP(spheres(items, W.paths));
           ++launches;
           CHK_HIP(launch_zero_u32(W.count, 1, st));
           hipLaunchKernelGGL(k_ex_retry, g, b, 0, st, sc->d, items, pix0, spp, P->sample_offset, v,

@@ -63,6 +63,11 @@ struct LightRec {
   float area;      // total area of the mesh light
   float pad;
 };
+// A sphere primitive in FP64 (Sphere, geometryPrimitives.h:38-52).
+struct SphereRec {
+  double c[3];
+  double r;
+};
 
 struct DeviceScene {
   const Node4 *nodes = nullptr;
@@ -74,7 +79,15 @@ struct DeviceScene {
   const LightTri64 *light64 = nullptr;  // parallel to light_tris
   const uint2 *orig_info = nullptr;    // per original triangle: (leaf or 0xFFFFFFFF, material)
   const double *light_area = nullptr;  // per mesh light: MeshLight::area in FP64
-  uint32_t n_lights = 0;
+  uint32_t n_lights = 0;               // mesh lights: LightRec 0 .. n_lights - 1
+  // Sources beyond the mesh lights and the sphere list, used by the direct-
+  // lighting estimators and the reference shaders.  Triangle lights are
+  // LightRec n_lights .. n_lights + n_tri_lights - 1 (one triangle each).
+  const SphereRec *spheres = nullptr;
+  const uint32_t *sphere_mat = nullptr;
+  const uint32_t *sph_src = nullptr;   // SphereSource j -> its sphere
+  const chk_point_source *points = nullptr;
+  uint32_t n_tri_lights = 0, n_spheres = 0, n_sph_src = 0, n_points = 0;
   uint32_t n_leaf_tris = 0;
   uint32_t n_tris = 0;                 // original triangle count
   // box of the BVH triangles (degenerate ones excluded): the learned
@@ -106,6 +119,9 @@ struct chk_scene_s {
   void *dwork = nullptr;  // direct-lighting estimators' chunk buffers
   size_t dwork_bytes = 0;
   uint32_t n_mats = 0;
+  // spheres, triangle, sphere or point sources: only the direct-lighting
+  // entry points and chk_render_shade handle them
+  bool extras = false;
 };
 
 namespace chk {
@@ -131,6 +147,11 @@ inline uint32_t seg_capacity(uint32_t n) {
   return (per + 511u) & ~511u;  // a multiple of every block size used (<= 512)
 }
 constexpr uint32_t kMaxTraceBatch = 1u << 31;  // rays per trace call
+// Spheres are tested by brute force in FP64 (the reference's sphere BVH
+// gives the same closest hit), so their number is bounded; every triangle,
+// sphere or point source costs sample slots per shading point.
+constexpr uint32_t kMaxSpheres = 4096;
+constexpr uint32_t kMaxSources = 1024;
 inline SegRange linear_range(uint32_t n) { return SegRange{nullptr, n, seg_capacity(n)}; }
 
 // Kernel launchers (chk_trace.hip).

@@ -13,11 +13,11 @@ Geometry is computed in FP64 with the reference's operations and order
 (quaternion rotations of transformations.cpp:26-42 applied as
 R_1 ... R_k S v + t, tempToColor of mathPrimitives.cpp:32-60) and then
 rounded to the FP32 the C ABI takes; the FP64 vertices stay in
-scene.meta["tris64"].  A triangle light becomes a one-triangle mesh light
-after the mesh lights: MIS::getSamples and the estimators treat the two
-alike (10 area samples, pdf d^2 / (A |cos|)).  Point sources, sphere
-sources and sphere objects are parsed into scene.meta; the renders do not
-draw them (strict=True rejects such scenes).
+scene.meta["tris64"].  Sphere objects, the dummy sphere and the sphere
+sources' visible spheres fill scene.spheres in vSphereList's order (each
+with a material of its own after the triangles'), triangle lights
+scene.tri_lights, sphere sources scene.sphere_sources (their spheres) and
+point sources scene.point_sources, as LightSource keeps them.
 """
 from __future__ import annotations
 
@@ -225,9 +225,10 @@ def _triangles(el, with_material):
     return out
 
 
-def load_scene_xml(path_or_text, strict=False):
-    """A reference scene file as a Scene (FP32 triangles, one material per
-    triangle, mesh lights, camera).  path_or_text: a path or the XML text."""
+def load_scene_xml(path_or_text):
+    """A reference scene file as a Scene (FP32 triangles with one material
+    each, spheres, mesh, triangle, sphere and point sources, camera).
+    path_or_text: a path or the XML text."""
     src = str(path_or_text)
     text = Path(src).read_text() if not src.lstrip().startswith("<") else src
     text = re.sub(r"<\?xml[^>]*\?>", "", text)
@@ -274,12 +275,21 @@ def load_scene_xml(path_or_text, strict=False):
             tris.append(verts)
             mats.append(_emitter(col, rad))
         lights.append((first, len(tris) - first))
+    tri_lights = []
     for src_el in doc.findall("TRISOURCE"):
         verts, col, _, rad = _triangles(src_el, False)[0]
-        lights.append((len(tris), 1))
+        tri_lights.append(len(tris))
         tris.append(verts)
         mats.append(_emitter(col, rad))
-    points = len(doc.findall("POINTSOURCE"))
+    points = []
+    for s in doc.findall("POINTSOURCE"):
+        # parsePointSource (sceneParser.cpp:76-90); PointSource(pos, col,
+        # power): BaseSource radiance = col.maxnorm() * power
+        c, col, pw = _vec3(s, "POSITION"), _color(s), _double(s, "POWER")
+        if c is None or col is None or pw is None:
+            raise ValueError("Point Source couldn't be parsed.")
+        m = max(col)
+        points.append((c, tuple(ch / m * pw for ch in col)))
     sphere_sources = []
     for s in doc.findall("SPHERESOURCE"):
         r, c, col = _double(s, "RADIUS"), _vec3(s, "POSITION"), _color(s)
@@ -296,24 +306,28 @@ def load_scene_xml(path_or_text, strict=False):
                             material=scenes._material(scenes.BRDF_NONE, int_ior=0.0, ext_ior=0.0,
                                                       reflectance=0.0, color=(0, 0, 0),
                                                       light=scenes.LIGHT_VOLUME, radiance=emit)))
-    unsupported = []
-    if points:
-        unsupported.append(f"{points} POINTSOURCE")
-    if sphere_sources:
-        unsupported.append(f"{len(sphere_sources)} SPHERESOURCE")
-    n_obj_spheres = len(spheres) - 1 - len(sphere_sources)
-    if n_obj_spheres:
-        unsupported.append(f"{n_obj_spheres} sphere objects")
-    if strict and unsupported:
-        raise ValueError("the renders draw triangles and mesh/triangle lights only; this scene "
-                         "also has " + ", ".join(unsupported))
     t64 = np.array([[c for v in t for c in v] for t in tris], np.float64)
+    n_tri_mats = len(mats)
+    sph = np.zeros(len(spheres), scenes.SPHERE_DTYPE)
+    for i, q in enumerate(spheres):
+        sph[i]["center"] = q["center"]
+        sph[i]["radius"] = q["radius"]
+        sph[i]["mat"] = len(mats)
+        mats.append(q["material"])
+    n_src = len(sphere_sources)
+    pts = np.zeros(len(points), scenes.POINT_DTYPE)
+    for i, (c, w) in enumerate(points):
+        pts[i]["pos"] = c
+        pts[i]["power"] = w
     mats_arr = np.zeros(len(mats), scenes.MATERIAL_DTYPE)
     for i, m in enumerate(mats):
         mats_arr[i] = m
     camera = scenes.Camera(tuple(pos), tuple(look), tuple(up), float(fovx), float(fovy), int(W),
                            int(H))
-    meta = {"tris64": t64, "spheres": spheres, "sphere_sources": sphere_sources,
-            "point_sources": points, "unsupported": unsupported, "source": src[:200]}
-    return scenes.Scene(t64.astype(np.float32), np.arange(len(mats), dtype=np.uint32), mats_arr,
-                        np.array(lights, scenes.LIGHT_DTYPE), camera, "xml", meta)
+    meta = {"tris64": t64, "sphere_sources": sphere_sources, "source": src[:200]}
+    return scenes.Scene(t64.astype(np.float32), np.arange(n_tri_mats, dtype=np.uint32), mats_arr,
+                        np.array(lights, scenes.LIGHT_DTYPE), camera, "xml", meta, spheres=sph,
+                        tri_lights=np.array(tri_lights, np.uint32),
+                        sphere_sources=np.arange(len(spheres) - n_src, len(spheres),
+                                                 dtype=np.uint32),
+                        point_sources=pts)

@@ -15,7 +15,7 @@ oracle and the tests.
 from __future__ import annotations
 
 import math
-from dataclasses import dataclass, field
+from dataclasses import dataclass, field, replace
 
 import numpy as np
 
@@ -36,6 +36,9 @@ MATERIAL_DTYPE = np.dtype(
 assert MATERIAL_DTYPE.itemsize == 56
 
 LIGHT_DTYPE = np.dtype([("first_tri", "<u4"), ("n_tris", "<u4")])
+# chk_sphere (Sphere with its material index) and chk_point_source
+SPHERE_DTYPE = np.dtype([("center", "<f8", (3,)), ("radius", "<f8"), ("mat", "<u4"), ("pad", "<u4")])
+POINT_DTYPE = np.dtype([("pos", "<f8", (3,)), ("power", "<f8", (3,))])
 
 # chk_brdf_type / chk_light_type (materialTypes.h:6-7)
 BRDF_NONE, CLASSIC_PHONG, GGX, BECKMANN, PHONG, GGXAPPROX = range(6)
@@ -74,6 +77,11 @@ class Scene:
     camera: Camera
     name: str = ""
     meta: dict = field(default_factory=dict)
+    # the other primitives and sources of chk_scene_desc
+    spheres: np.ndarray = field(default_factory=lambda: np.zeros(0, SPHERE_DTYPE))
+    tri_lights: np.ndarray = field(default_factory=lambda: np.zeros(0, np.uint32))
+    sphere_sources: np.ndarray = field(default_factory=lambda: np.zeros(0, np.uint32))
+    point_sources: np.ndarray = field(default_factory=lambda: np.zeros(0, POINT_DTYPE))
 
     @property
     def n_tris(self):
@@ -81,7 +89,7 @@ class Scene:
 
     def with_camera(self, **kw):
         cam = Camera(**{**self.camera.__dict__, **kw})
-        return Scene(self.tris, self.tri_mat, self.mats, self.lights, cam, self.name, dict(self.meta))
+        return replace(self, camera=cam, meta=dict(self.meta))
 
 
 def _material(brdf, ks=0.0, alpha=0.0, phong_exp=0.0, int_ior=1.5, ext_ior=1.0,
@@ -410,6 +418,96 @@ def direct_room(seed=0x5EED0006, width=64, height=64, reflectance=None, fan=True
               (second, (0.6, 0.8, 1.0), i1)]
     cam = Camera((0.5, 0.5, 2.2), (0, 0, -1), (0, 1, 0), 40.0, 40.0, width, height)
     return _assemble(np.concatenate(obj), omat, mats, lights, cam, "direct_room", {"seed": seed})
+
+
+def sources_room(seed=0x5EED0008, width=64, height=64, mesh=True, sphere_source=True,
+                 tri_light=True, points=True, spheres=True):
+    """direct_room with every other primitive and source kind of the
+    reference: two spheres on the floor (GGX and Phong), a sphere source
+    (LightSource::addSSource: the source is also a visible VOLUME sphere), a
+    triangle light (TriLight, PLANAR) on the left wall and two point sources
+    (PointSource).  mesh=False drops the two mesh lights."""
+    sc = direct_room(seed=seed, width=width, height=height)
+    rng = np.random.default_rng(seed)
+    mats = list(sc.mats)
+    cut = sc.n_tris if mesh else int(sc.lights["first_tri"].min())  # lights are the tail
+    tris = [sc.tris[:cut]]
+    tri_mat = [sc.tri_mat[:cut]]
+    lights = sc.lights if mesh else np.zeros(0, LIGHT_DTYPE)
+    tl = []
+    if tri_light:
+        mi = len(mats)
+        m = emitter_material((0.9, 1.0, 0.8), 9.0)
+        m["light"] = LIGHT_PLANAR
+        mats.append(m)
+        tl.append(sum(t.shape[0] for t in tris))
+        tris.append(np.array([[0.001, 0.75, 0.3, 0.001, 0.85, 0.45, 0.001, 0.7, 0.5]], np.float32))
+        tri_mat.append(np.array([mi], np.uint32))
+    sph = []
+    ss = []
+    if spheres:
+        g = len(mats)
+        mats.append(_material(GGX, ks=0.6, alpha=0.25, reflectance=0.95,
+                              color=rng.uniform(0.4, 0.9, 3)))
+        mats.append(_material(PHONG, ks=0.4, alpha=30.0, reflectance=0.9,
+                              color=rng.uniform(0.4, 0.9, 3)))
+        sph.append(((0.3, 0.15, 0.35), 0.15, g))
+        sph.append(((0.74, 0.12, 0.3), 0.12, g + 1))
+    if sphere_source:
+        mi = len(mats)
+        col = np.array([1.0, 0.85, 0.6])
+        m = _material(BRDF_NONE, reflectance=0.0, color=(0, 0, 0), light=LIGHT_VOLUME,
+                      radiance=col / col.max() * 14.0)
+        mats.append(m)
+        ss.append(len(sph))
+        sph.append(((0.78, 0.82, 0.62), 0.05, mi))
+    pts = []
+    if points:
+        pts.append(((0.5, 0.9, 0.8), (0.9, 0.9, 0.9)))
+        pts.append(((0.2, 0.7, 0.25), (0.6, 0.4, 0.2)))
+    mats_arr = np.zeros(len(mats), MATERIAL_DTYPE)
+    for i, m in enumerate(mats):
+        mats_arr[i] = m
+    sph_arr = np.zeros(len(sph), SPHERE_DTYPE)
+    for i, (c, r, m) in enumerate(sph):
+        sph_arr[i]["center"] = c
+        sph_arr[i]["radius"] = r
+        sph_arr[i]["mat"] = m
+    pt_arr = np.zeros(len(pts), POINT_DTYPE)
+    for i, (p, w) in enumerate(pts):
+        pt_arr[i]["pos"] = p
+        pt_arr[i]["power"] = w
+    return Scene(np.concatenate(tris).astype(np.float32), np.concatenate(tri_mat).astype(np.uint32),
+                 mats_arr, np.asarray(lights, LIGHT_DTYPE), sc.camera, "sources_room",
+                 {"seed": seed}, spheres=sph_arr, tri_lights=np.array(tl, np.uint32),
+                 sphere_sources=np.array(ss, np.uint32), point_sources=pt_arr)
+
+
+def sphere_points(scene, k, seed=0x5EED0009, eye=(0.5, 0.55, 0.95), spheres=None):
+    """k shading points on the scene's non-emitter spheres (or on `spheres`)
+    seen from `eye`: (x, n, d, prim) like shading_points, prim = n_tris +
+    sphere index and n the outward normal (the sphere hit's normal turned
+    against d, shader.cpp:41-47)."""
+    rng = np.random.default_rng(seed)
+    S = scene.spheres
+    if spheres is None:
+        spheres = np.nonzero(scene.mats["light"][S["mat"]] == LIGHT_NONE)[0]
+    sid = rng.choice(np.asarray(spheres), k)
+    c, r = S["center"][sid], S["radius"][sid][:, None]
+    e = np.asarray(eye, np.float64)
+    x = np.empty((k, 3))
+    for i in range(k):
+        while True:
+            u = rng.normal(size=3)
+            u /= np.linalg.norm(u)
+            if np.dot(u, e - c[i]) > 0.2 * np.linalg.norm(e - c[i]):
+                break
+        x[i] = c[i] + r[i] * u
+    n = x - c
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    d = x - e
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return x, n, d, (scene.n_tris + sid).astype(np.uint32)
 
 
 def shading_points(scene, k, seed=0x5EED0007, eye=(0.5, 0.55, 0.95), tris=None):

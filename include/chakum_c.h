@@ -136,6 +136,16 @@ chk_status chk_set_ltc_table(chk_ctx_t ctx, const double *table_host, size_t n_d
 chk_status chk_scene_create(chk_ctx_t ctx, const float *tris, uint32_t n_tris,
                             const uint32_t *tri_mat, const chk_material *mats, uint32_t n_mats,
                             const chk_mesh_light *lights, uint32_t n_lights, chk_scene_t *out);
+/* The whole scene of the reference's prepareScene (prepareScene.cpp:54-240:
+ * prepareLights + prepareObjects): triangles and spheres with their
+ * materials, mesh, triangle, sphere and point light sources
+ * (chk_scene_desc).  chk_scene_create is this call with triangles and mesh
+ * lights only.  Spheres and the triangle, sphere and point sources are seen by
+ * the direct-lighting entry points and chk_render_shade; chk_render,
+ * chk_render_count, chk_render_batch and chk_render_learned (the mesh-light
+ * path tracer) reject a scene that has them.  At most 4096 spheres and 1024
+ * sources of each of those kinds (CHK_ECAPACITY). */
+chk_status chk_scene_create_desc(chk_ctx_t ctx, const chk_scene_desc *desc, chk_scene_t *out);
 chk_status chk_scene_destroy(chk_scene_t scene);
 chk_status chk_scene_get_info(chk_scene_t scene, chk_scene_info *info);
 
@@ -214,17 +224,20 @@ chk_status chk_render_count(chk_scene_t scene, const chk_camera *camera,
  *   x     hit point (k x 3 doubles)
  *   n     unit normal facing the viewer, as intersectT returns it (k x 3)
  *   d     unit direction of the ray that hit x, r.d (k x 3)
- *   prim  original index of the triangle x lies on: its material is the
- *         BasePrimitive's (k)
- * All FP64 in and out, like the reference's Vec arguments.  Light sources
- * are the scene's mesh lights.  Materials must be GGX, Beckmann, Phong or
- * classic Phong (the reference has no sampler for the others). */
+ *   prim  the primitive x lies on, whose material is the BasePrimitive's:
+ *         the original triangle index, or n_tris + sphere index (k)
+ * All FP64 in and out, like the reference's Vec arguments.  The light
+ * sources are the scene's (chk_scene_desc): the mesh-light estimators see the
+ * mesh lights, getLightFromAllSources_MIS every kind.  Materials must be GGX,
+ * Beckmann, Phong or classic Phong (the reference has no sampler for the
+ * others). */
 
 /* MIS weights of given samples.  Replaces the weight half of
  * MIS::getSamples (mis.cpp:18-95,209-220): weight = 1 / (sum over the
- * sampling techniques of N_k p_k(sample) + 1e-20) with N = 10 per mesh light
- * (area sampling, solid-angle pdf d^2 / (A |cos|) from Ray(x, sample)) and
- * N = 60 for the BSDF (MaterialType::pdfEval).  wo_ref and wo as the
+ * sampling techniques of N_k p_k(sample) + 1e-20) with N = 10 per sphere
+ * source (uniform cone, 1 / cap area inside it), per mesh light and per
+ * triangle light (area sampling, solid-angle pdf d^2 / (A |cos|) from
+ * Ray(x, sample)), and N = 60 for the BSDF (MaterialType::pdfEval).  wo_ref and wo as the
  * reference passes them (lightSources.cpp:44-45,60).  samples: k x S x 3
  * unit directions (point-major); weights: k x S.  All dev. */
 chk_status chk_mis_weights(chk_scene_t scene, const double *x, const double *n,

@@ -59,6 +59,46 @@ typedef struct chk_mesh_light {
   uint32_t n_tris;
 } chk_mesh_light;
 
+/* A sphere primitive: Sphere (headers/geometryPrimitives.h:38-52) with its
+ * material (colour, reflectance and MaterialType in a chk_material).  FP64
+ * like the reference's Sphere.  40 bytes. */
+typedef struct chk_sphere {
+  double center[3];
+  double radius;
+  uint32_t mat;
+  uint32_t pad;
+} chk_sphere;
+
+/* A point light: PointSource (headers/lightSources.h:22-31).  power is its
+ * BaseSource radiance field, col.maxnorm() * power (lightSources.h:12-16);
+ * the radiance at x is power / (4 pi |p - x|^2).  48 bytes. */
+typedef struct chk_point_source {
+  double pos[3];
+  double power[3];
+} chk_point_source;
+
+/* Everything a scene holds, in the order LightSource and the object lists
+ * keep it (lightSources.h:158-203, objects.h:7-17).  Light geometry is part
+ * of the primitive lists, as LightSource::addMSource / addTSource / addSSource
+ * append it there:
+ *   mesh_lights     MeshLight: runs of emitter triangles
+ *   tri_lights      TriLight: one emitter triangle each (triangle indices)
+ *   sphere_sources  SphereSource: emitter spheres (sphere indices); the
+ *                   sphere is both the visible sphere and the source
+ *   point_sources   PointSource: not geometry */
+typedef struct chk_scene_desc {
+  const float *tris;            /* n_tris x 9 (A, B, C) */
+  const uint32_t *tri_mat;      /* n_tris */
+  const chk_material *mats;     /* n_mats */
+  const chk_mesh_light *mesh_lights;
+  const uint32_t *tri_lights;
+  const chk_sphere *spheres;
+  const uint32_t *sphere_sources;
+  const chk_point_source *point_sources;
+  uint32_t n_tris, n_mats, n_mesh_lights, n_tri_lights;
+  uint32_t n_spheres, n_sphere_sources, n_point_sources, reserved;
+} chk_scene_desc;
+
 /* Pinhole camera with the reference's conventions (prepareScene.cpp:12-36,
  * main.cpp:72-79): cx = |dir x up| * (w/h) * tan(fovx/2),
  * cy = |cx x dir| * tan(fovy/2); fov in degrees. */
@@ -130,10 +170,12 @@ typedef struct chk_direct_params {
  *   ctr = { point index (or pixel), estimate index (or sample), slot,
  *           CHK_STREAM_DIRECT + path vertex (0 except in shadeExplicit) }
  * with one call per sample slot; a 32-bit output x maps to (x + 0.5) * 2^-32.
- * Slots 10 j .. 10 j + 9 sample mesh light j by area, slots 10 L .. 10 L + 59
- * the BSDF (the first floor(60 ks) from the specular lobe), as
- * MIS::getSamples (mis.cpp:97-191); ANALYTIC_CV uses slots j n .. j n + n - 1
- * for light j. */
+ * The slots follow MIS::getSamples (mis.cpp:97-191): 10 per sphere source
+ * (its cone), then 10 per mesh light and 10 per triangle light (by area), then
+ * 60 for the BSDF (the first floor(60 ks) from the specular lobe); the MIS
+ * estimator adds one slot per point source after them (no draw).  With mesh
+ * lights only, slots 10 j .. 10 j + 9 are mesh light j.  ANALYTIC_CV uses
+ * slots j n .. j n + n - 1 for mesh light j. */
 #define CHK_STREAM_DIRECT 0x44490000u
 
 /* The reference's shaders (shader.cpp:362-378 selects one at compile time). */

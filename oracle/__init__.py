@@ -81,6 +81,10 @@ def lib():
         L.orc_direct_scene.restype = _vp
         L.orc_direct_scene.argtypes = [_fp, C.c_uint32, _up, _vp, C.c_uint32, _vp, C.c_uint32]
         L.orc_direct_scene_free.argtypes = [_vp]
+        L.orc_direct_scene_desc.restype = _vp
+        L.orc_direct_scene_desc.argtypes = [_vp]
+        L.orc_mis_count.restype = C.c_uint32
+        L.orc_mis_count.argtypes = [_vp]
         L.orc_mis_weights.argtypes = [_vp, C.c_uint32, _dp, _dp, _dp, _dp, _up, _dp, C.c_uint32,
                                       _dp]
         L.orc_mis_samples.argtypes = [_vp, _dp, _dp, _dp, _dp, C.c_uint32, C.c_uint32, C.c_uint32,
@@ -107,6 +111,9 @@ def ref_lib():
         R.ref_M_GGX.argtypes = [C.c_float, C.c_float, _dp, _dp, _dp, _dp]
         R.ref_ltc_shade.argtypes = [C.c_uint32, _dp, _dp, _dp, _dp, _dp, _dp, _dp, C.c_uint32, _dp]
         R.ref_direct_setup.argtypes = [_dp, C.c_uint32, _up, _dp, C.c_uint32, _up, C.c_uint32]
+        R.ref_direct_setup_ex.argtypes = [_dp, C.c_uint32, _up, _dp, C.c_uint32, _up, C.c_uint32,
+                                          _vp, C.c_uint32, _vp, _vp, C.c_uint32, _vp, C.c_uint32,
+                                          _vp, C.c_uint32]
         R.ref_mis_get_samples.restype = C.c_uint32
         R.ref_mis_get_samples.argtypes = [C.c_uint32, _dp, _dp, _dp, _dp, _up, _dp, _dp]
         R.ref_direct_stats.argtypes = [C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
@@ -279,13 +286,14 @@ class DirectScene:
     scene (triangles, materials, mesh lights)."""
 
     def __init__(self, scene):
+        from chakumrender_amd import abi
+
         self.scene = scene
         self.mats = np.ascontiguousarray(scene.mats)
         self.lights = np.ascontiguousarray(scene.lights)
-        self.h = lib().orc_direct_scene(np.ascontiguousarray(scene.tris, np.float32), scene.n_tris,
-                                        np.ascontiguousarray(scene.tri_mat, np.uint32),
-                                        self.mats.ctypes.data, self.mats.shape[0],
-                                        self.lights.ctypes.data, self.lights.shape[0])
+        d, keep = abi.scene_desc(scene)
+        self.h = lib().orc_direct_scene_desc(C.addressof(d))
+        del keep
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -294,7 +302,9 @@ class DirectScene:
 
     @property
     def n_samples(self):
-        return 10 * self.lights.shape[0] + 60
+        """MIS::getSamples' sample count: 10 per sphere source, mesh light and
+        triangle light, 60 BSDF."""
+        return int(lib().orc_mis_count(self.h))
 
     def mis_weights(self, x, n, wo_ref, wo, prim, samples):
         smp = _d(samples)
@@ -386,10 +396,29 @@ class RefDirect:
         md[:, 13] = mats["light"]
         lights = np.ascontiguousarray(scene.lights)
         lt = np.ascontiguousarray(np.stack([lights["first_tri"], lights["n_tris"]], 1), np.uint32)
+        tris = _d(np.asarray(scene.tris, np.float32).astype(np.float64), (-1, 9))
+        tri_mat = np.ascontiguousarray(scene.tri_mat, np.uint32)
+        sph = np.asarray(getattr(scene, "spheres", np.zeros(0)))
+        tl = np.ascontiguousarray(getattr(scene, "tri_lights", np.zeros(0)), np.uint32)
+        ss = np.ascontiguousarray(getattr(scene, "sphere_sources", np.zeros(0)), np.uint32)
+        pts = np.asarray(getattr(scene, "point_sources", np.zeros(0)))
         with _quiet():
-            R.ref_direct_setup(_d(np.asarray(scene.tris, np.float32).astype(np.float64), (-1, 9)),
-                               scene.n_tris, np.ascontiguousarray(scene.tri_mat, np.uint32),
-                               np.ascontiguousarray(md), md.shape[0], lt, lt.shape[0])
+            if sph.shape[0] or tl.shape[0] or ss.shape[0] or pts.shape[0]:
+                # the scene's spheres as given (a scene file's list holds the
+                # dummy sphere of prepareScene.cpp:238 already)
+                sd = np.ascontiguousarray(np.concatenate(
+                    [sph["center"], sph["radius"][:, None]], 1) if sph.shape[0] else np.zeros((0, 4)))
+                sm = np.ascontiguousarray(sph["mat"] if sph.shape[0] else np.zeros(0), np.uint32)
+                pd = np.ascontiguousarray(np.concatenate(
+                    [pts["pos"], pts["power"]], 1) if pts.shape[0] else np.zeros((0, 6)))
+                ptr = lambda a: a.ctypes.data if a.shape[0] else None  # noqa: E731
+                R.ref_direct_setup_ex(tris, scene.n_tris, tri_mat, np.ascontiguousarray(md),
+                                      md.shape[0], lt, lt.shape[0], ptr(tl), tl.shape[0], ptr(sd),
+                                      ptr(sm), sd.shape[0], ptr(ss), ss.shape[0], ptr(pd),
+                                      pd.shape[0])
+            else:
+                R.ref_direct_setup(tris, scene.n_tris, tri_mat, np.ascontiguousarray(md),
+                                   md.shape[0], lt, lt.shape[0])
         self.R = R
         self.n_lights = lt.shape[0]
 

@@ -61,6 +61,7 @@ inline V3 operator+(V3 a, V3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
 inline V3 operator-(V3 a, V3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
 inline V3 operator*(V3 a, double s) { return v3(a.x * s, a.y * s, a.z * s); }
 inline V3 mul(V3 a, V3 b) { return v3(a.x * b.x, a.y * b.y, a.z * b.z); }
+inline V3 vdiv(V3 a, double s) { return v3(a.x / s, a.y / s, a.z / s); }
 inline double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 // Vec::operator% (mathPrimitives.cpp:16)
 inline V3 cross(V3 a, V3 b) {
@@ -773,15 +774,63 @@ struct DLight {
   uint32_t inv[300];      // MeshLight::invCDF
 };
 
+struct DSph {  // Sphere (geometryPrimitives.h:38-52)
+  V3 c;
+  double r;
+  uint32_t mat;
+};
+struct DPt {  // PointSource (lightSources.h:22-31)
+  V3 p, power;
+};
+
+// The scene of prepareScene (prepareScene.cpp:54-240): the object lists and
+// LightSource's mList, tList, sList and pList.
 struct DScene {
   std::vector<double> td;
   std::vector<Tri> T;
   RefBvh *B = nullptr;
   std::vector<Mat> mats;
   std::vector<uint32_t> tri_mat;
-  std::vector<DLight> L;
+  std::vector<DLight> L;    // mesh lights
+  std::vector<DLight> TL;   // triangle lights, one triangle each
+  std::vector<DSph> sph;    // spheres
+  std::vector<uint32_t> ssrc;  // sphere sources -> their sphere
+  std::vector<DPt> pts;     // point sources
+  uint32_t n_tris = 0;
   ~DScene() { delete B; }
 };
+
+// A primitive: a triangle index below n_tris, else n_tris + sphere index.
+inline const Mat &prim_mat(const DScene &S, uint32_t prim) {
+  return prim < S.n_tris ? S.mats[S.tri_mat[prim]] : S.mats[S.sph[prim - S.n_tris].mat];
+}
+
+// Sphere::intersect (geometryPrimitives.cpp:12-27): the first root beyond
+// eps = 1e-4 (geometryPrimitives.h:39), else INF.
+double sph_isect(const DSph &q, V3 o, V3 d) {
+  const V3 op = q.c - o;
+  const double b = dot(op, d);
+  double det = b * b - dot(op, op) + q.r * q.r;
+  if (det < 0) return INF;
+  det = std::sqrt(det);
+  double t;
+  return (t = b - det) > 1e-4 ? t : ((t = b + det) > 1e-4 ? t : INF);
+}
+// BvhAccel::intersectS (bvhAccel.cpp:151-198) over the list: the nearest
+// sphere under the strict d < t (a tie keeps the earlier sphere; the
+// reference's sphere BVH order may keep another).  -1 on a miss.
+int sph_closest(const DScene &S, V3 o, V3 d, double *t) {
+  *t = INF;
+  int id = -1;
+  for (size_t k = 0; k < S.sph.size(); ++k) {
+    const double dd = sph_isect(S.sph[k], o, d);
+    if (dd && dd < *t) {
+      *t = dd;
+      id = (int)k;
+    }
+  }
+  return id;
+}
 
 // MeshLight::initMeshLight (lightSources.cpp:784-807).
 void init_mesh_light(const DScene &S, DLight &L) {
@@ -828,12 +877,53 @@ bool mesh_light_isect(const DScene &S, const DLight &L, V3 o, V3 d, uint32_t *tr
   return *t < INF;
 }
 
-// shadow() (shader.cpp:76-97): closest hit + 1e-5 < distance.
+// shadow() (shader.cpp:76-97): the closest sphere, then the closest
+// triangle, + 1e-5 < distance.
 bool occluded(const DScene &S, V3 o, V3 d, double dist) {
   double t;
+  if (sph_closest(S, o, d, &t) >= 0 && t + 1e-5 < dist) return true;
   V3 N;
   int rid;
   return bvh_closest(*S.B, o, d, 1, &t, &N, &rid, nullptr, nullptr) && t + 1e-5 < dist;
+}
+
+// The sampling cone of sphere source j from x (mis.cpp:103-115): axis
+// (c - x).norm(), sin(theta_max) = r / |c - x|, cap area 2 pi (1 - cos
+// theta_max) (domainSampler.cpp:325).  Inside the source theta_max = pi / 2
+// (the reference's asin has no value there).
+struct DCone {
+  V3 w;
+  double cos_max, cap;
+};
+DCone sph_cone(const DScene &S, uint32_t j, V3 x) {
+  const DSph &q = S.sph[S.ssrc[j]];
+  const V3 v = q.c - x;
+  double sn = q.r / length(v);
+  sn = sn > 1.0 ? 1.0 : sn;
+  DCone C;
+  C.w = norm(v);
+  C.cap = 2.0 * PI * (1 - std::cos(std::asin(sn)));
+  C.cos_max = std::sqrt(1 - sn * sn);
+  return C;
+}
+// SphericalSampler::getSolidDirectionSamples, one direction
+// (domainSampler.cpp:320-369), in the frame it builds from w.
+V3 cone_dir(const DCone &C, double e1, double e2) {
+  const V3 w = C.w;
+  V3 X;
+  if (w.x != 0)
+    X = v3((-w.y - w.z) / w.x, 1.0, 1.0);
+  else if (w.y != 0)
+    X = v3(1.0, -w.z / w.y, 1.0);
+  else
+    X = v3(1.0, 1.0, 0.0);
+  X = norm(X);
+  const V3 Y = norm(cross(w, X));
+  const double ct = 1.0 - (C.cap * e1) / (2.0 * PI);
+  const double th = std::acos(ct);
+  const double sth = std::sin(th), phi = 2 * PI * e2;
+  const double a = sth * std::cos(phi), b = sth * std::sin(phi);
+  return v3(a * X.x + b * Y.x + ct * w.x, a * X.y + b * Y.y + ct * w.y, a * X.z + b * Y.z + ct * w.z);
 }
 
 struct DPoint {
@@ -852,7 +942,7 @@ struct DCtx {
 // Per-point part of the estimators: wo_ref (lightSources.cpp:44-45),
 // getBrdfApproxParam and the analytic term G (lightSources_CV.cpp:323-333).
 void direct_setup(const DScene &S, const DPoint &P, bool need_G, DCtx *C) {
-  C->m = S.mats[S.tri_mat[P.prim]];
+  C->m = prim_mat(S, P.prim);
   C->wo = P.d * -1.0;
   C->wo_ref = norm(P.n * 2.0 * dot(P.n, C->wo) + P.d);
   C->apx = C->m.brdf == CHK_BRDF_GGX;
@@ -894,11 +984,32 @@ inline void dphilox(uint32_t a, uint32_t b, uint32_t slot, uint32_t v, const uin
   philox(c, key, r);
 }
 
-// MIS::getSamples sample set (mis.cpp:127-138,180-191).
+// Size of MIS::getSamples' sample set: 10 per sphere source, mesh light and
+// triangle light, 60 for the BSDF (mis.h:8-12).
+uint32_t mis_count(const DScene &S) {
+  return (uint32_t)(S.ssrc.size() + S.L.size() + S.TL.size()) * CHK_MIS_LIGHT_SAMPLES +
+         CHK_MIS_BSDF_SAMPLES;
+}
+
+// MIS::getSamples sample set (mis.cpp:103-191): sphere-source cones, mesh
+// lights, triangle lights (getTriLightSurfaceSamples, domainSampler.cpp:
+// 138-150, the distribution of a one-triangle mesh light), the BSDF.
 void mis_samples(const DScene &S, const DCtx &C, const DPoint &P, uint32_t a, uint32_t b,
                  uint32_t v, const uint32_t *key, V3 *out) {
   uint32_t slot = 0, r[4], tri;
+  for (uint32_t j = 0; j < S.ssrc.size(); ++j) {
+    const DCone K = sph_cone(S, j, P.x);
+    for (uint32_t q = 0; q < CHK_MIS_LIGHT_SAMPLES; ++q, ++slot) {
+      dphilox(a, b, slot, v, key, r);
+      out[slot] = cone_dir(K, ud(r[0]), ud(r[1]));
+    }
+  }
   for (const DLight &L : S.L)
+    for (uint32_t q = 0; q < CHK_MIS_LIGHT_SAMPLES; ++q, ++slot) {
+      dphilox(a, b, slot, v, key, r);
+      out[slot] = norm(mesh_light_point(S, L, r, &tri) - P.x);
+    }
+  for (const DLight &L : S.TL)
     for (uint32_t q = 0; q < CHK_MIS_LIGHT_SAMPLES; ++q, ++slot) {
       dphilox(a, b, slot, v, key, r);
       out[slot] = norm(mesh_light_point(S, L, r, &tri) - P.x);
@@ -913,15 +1024,31 @@ void mis_samples(const DScene &S, const DCtx &C, const DPoint &P, uint32_t a, ui
   }
 }
 
-// 1 / (sum_k N_k pdfVal_k(s) + 1e-20) (mis.cpp:34-95,209-220).
+// 1 / (sum_k N_k pdfVal_k(s) + 1e-20) (mis.cpp:34-95,209-220), the
+// techniques in getSamples' order.
 double mis_weight(const DScene &S, const DCtx &C, const DPoint &P, V3 s) {
   double denom = 0;
+  for (uint32_t j = 0; j < S.ssrc.size(); ++j) {  // SoAnISSP (mis.cpp:35-39)
+    const DCone K = sph_cone(S, j, P.x);
+    denom += (double)CHK_MIS_LIGHT_SAMPLES * (dot(s, K.w) >= K.cos_max ? 1.0 / K.cap : 0.);
+  }
   for (const DLight &L : S.L) {
     uint32_t tri;
     double d;
     V3 N = v3(0, 0, 0);
     double p = 0;
     if (mesh_light_isect(S, L, P.x, s, &tri, &d, &N)) {  // Ray r(x, sample)
+      double c1 = dot(s, N);
+      c1 = c1 < 0 ? -c1 : c1;
+      p = ((1.0 / L.area) * d * d) / c1;
+    }
+    denom += (double)CHK_MIS_LIGHT_SAMPLES * p;
+  }
+  for (const DLight &L : S.TL) {  // SuArISTR (mis.cpp:68-78)
+    V3 N = v3(0, 0, 0);
+    double p = 0;
+    const double d = tri_isect(S.T[L.first], P.x, s, &N, nullptr);
+    if (d < INF) {
       double c1 = dot(s, N);
       c1 = c1 < 0 ? -c1 : c1;
       p = ((1.0 / L.area) * d * d) / c1;
@@ -935,6 +1062,13 @@ double mis_weight(const DScene &S, const DCtx &C, const DPoint &P, V3 s) {
 V3 direct_estimate(const DScene &S, const chk_direct_params &p, const DPoint &P, uint32_t a,
                    uint32_t b, uint32_t v = 0) {
   const uint32_t key[2] = {p.seed_lo, p.seed_hi};
+  // the mesh-light estimators return 0 without a mesh light
+  // (lightSources_CV.cpp:324,336,400), the MIS one without any source
+  // (lightSources.cpp:33)
+  if (p.estimator != CHK_DIRECT_MIS && S.L.empty()) return v3(0, 0, 0);
+  if (p.estimator == CHK_DIRECT_MIS && S.L.empty() && S.TL.empty() && S.ssrc.empty() &&
+      S.pts.empty())
+    return v3(0, 0, 0);
   DCtx C;
   direct_setup(S, P, p.estimator != CHK_DIRECT_MIS, &C);
   if (p.estimator == CHK_DIRECT_ANALYTIC) return C.G;
@@ -966,8 +1100,20 @@ V3 direct_estimate(const DScene &S, const chk_direct_params &p, const DPoint &P,
     }
     return sum + C.G;
   }
+  if (p.estimator == CHK_DIRECT_MIS) {
+    // point sources (lightSources.cpp:47-58)
+    for (const DPt &Q : S.pts) {
+      const V3 dv = Q.p - P.x;
+      const V3 w = norm(dv);
+      const double cs = dot(w, P.n);
+      if (cs < 0) continue;
+      if (occluded(S, o, w, length(dv))) continue;
+      const double sq = dot(dv, dv);
+      sum = sum + vdiv(Q.power, 4.0 * PI * sq) * (cs * brdf(C.m, P.n, C.wo_ref, C.wo, w));
+    }
+  }
   // MIS and MISCV share MIS::getSamples
-  const uint32_t Sn = (uint32_t)S.L.size() * CHK_MIS_LIGHT_SAMPLES + CHK_MIS_BSDF_SAMPLES;
+  const uint32_t Sn = mis_count(S);
   std::vector<V3> smp(Sn);
   mis_samples(S, C, P, a, b, v, key, smp.data());
   for (uint32_t i = 0; i < Sn; ++i) {
@@ -976,8 +1122,10 @@ V3 direct_estimate(const DScene &S, const chk_direct_params &p, const DPoint &P,
     const double cs = dot(w, P.n);
     if (cs < 0) continue;
     if (p.estimator == CHK_DIRECT_MIS) {
-      // lightSources.cpp:64-80: the first mesh light hit and not shadowed
+      // lightSources.cpp:60-106: the first source hit and not shadowed, in
+      // the order mesh lights, sphere sources, triangle lights
       const double thr = brdf(C.m, P.n, C.wo_ref, C.wo, w) * cs * wt;
+      bool hit = false;
       for (const DLight &L : S.L) {
         uint32_t tri;
         double d;
@@ -985,7 +1133,24 @@ V3 direct_estimate(const DScene &S, const chk_direct_params &p, const DPoint &P,
         if (!mesh_light_isect(S, L, o, w, &tri, &d, &N)) continue;
         if (occluded(S, o, w, d)) continue;
         sum = sum + S.mats[S.tri_mat[tri]].rad * thr;
+        hit = true;
         break;
+      }
+      for (uint32_t j = 0; j < S.ssrc.size() && !hit; ++j) {
+        const DSph &q = S.sph[S.ssrc[j]];
+        const double d = sph_isect(q, o, w);
+        if (d >= INF) continue;
+        if (occluded(S, o, w, d)) continue;
+        sum = sum + S.mats[q.mat].rad * thr;
+        hit = true;
+      }
+      for (uint32_t j = 0; j < S.TL.size() && !hit; ++j) {
+        V3 N;
+        const double d = tri_isect(S.T[S.TL[j].first], o, w, &N, nullptr);
+        if (d >= INF) continue;
+        if (occluded(S, o, w, d)) continue;
+        sum = sum + S.mats[S.tri_mat[S.TL[j].first]].rad * thr;
+        hit = true;
       }
     } else {
       // lightSources_CV.cpp:420-445: the first mesh light hit
@@ -1027,7 +1192,6 @@ V3 hemi_dir(V3 n, double e1, double e2) {
   onb(n, &T, &Bv);
   return T * (st * std::cos(phi)) + Bv * (st * std::sin(phi)) + n * ct;
 }
-inline V3 vdiv(V3 a, double s) { return v3(a.x / s, a.y / s, a.z / s); }
 
 struct Hit {
   bool hit;
@@ -1035,11 +1199,25 @@ struct Hit {
   V3 N;
   uint32_t prim;
 };
+// The shaders' closest hit (shader.cpp:128-136,344-351): the triangle hit
+// (intersectTriangle) unless a sphere is strictly nearer (intersectSphere,
+// whose normal is c - x turned against d, shader.cpp:41-47).
 Hit closest(const DScene &S, V3 o, V3 d) {
   Hit h;
   int rid;
   h.hit = bvh_closest(*S.B, o, d, 1, &h.t, &h.N, &rid, nullptr, nullptr);
   h.prim = h.hit ? S.B->ordered[rid] : 0u;
+  const double tT = h.hit ? h.t : INF;
+  double tS;
+  const int id = sph_closest(S, o, d, &tS);
+  if (id >= 0 && tS < tT) {
+    h.hit = true;
+    h.t = tS;
+    V3 N = norm(S.sph[id].c - (o + d * tS));
+    if (dot(d, N) > 0) N = N * -1.0;
+    h.N = N;
+    h.prim = S.n_tris + (uint32_t)id;
+  }
   return h;
 }
 
@@ -1054,7 +1232,7 @@ V3 shade_implicit(const DScene &S, V3 o, V3 d, uint32_t pix, uint32_t smp, const
       col = v3(0, 0, 0);
       break;
     }
-    const Mat &m = S.mats[S.tri_mat[h.prim]];
+    const Mat &m = prim_mat(S, h.prim);
     V3 x = o + d * h.t, n = h.N;
     if (m.light != CHK_LIGHT_NONE) {
       col = mul(physical ? m.rad : maxnorm(m.rad), product);
@@ -1084,7 +1262,7 @@ V3 shade_explicit(const DScene &S, const chk_direct_params &dp, V3 o, V3 d, uint
   const double q = 0.3;
   Hit h = closest(S, o, d);
   if (!h.hit) return v3(0, 0, 0);
-  const Mat *m = &S.mats[S.tri_mat[h.prim]];
+  const Mat *m = &prim_mat(S, h.prim);
   V3 x = o + d * h.t, n = h.N, rd = d;
   uint32_t prim = h.prim;
   if (m->light != CHK_LIGHT_NONE) return physical ? m->rad : maxnorm(m->rad);
@@ -1110,7 +1288,7 @@ V3 shade_explicit(const DScene &S, const chk_direct_params &dp, V3 o, V3 d, uint
       w = hemi_dir(n, ud(r[0]), ud(r[1]));
       nh = closest(S, so, w);
       if (!nh.hit) break;  // the ray escapes
-      if (S.mats[S.tri_mat[nh.prim]].light != CHK_LIGHT_NONE) continue;
+      if (prim_mat(S, nh.prim).light != CHK_LIGHT_NONE) continue;
       status = 1;
       break;
     }
@@ -1126,7 +1304,7 @@ V3 shade_explicit(const DScene &S, const chk_direct_params &dp, V3 o, V3 d, uint
     x = so + w * nh.t;
     n = nh.N;
     prim = nh.prim;
-    m = &S.mats[S.tri_mat[prim]];
+    m = &prim_mat(S, prim);
   }
   V3 col = DI.back();
   for (int i = (int)DI.size() - 2; i >= 0; --i) col = DI[i] + vdiv(mul(col, TP[i]), 1 - q);
@@ -1530,26 +1708,58 @@ uint64_t orc_render(const float *tris_f, uint32_t n_tris, const uint32_t *tri_ma
   return paths;
 }
 
-void *orc_direct_scene(const float *tris, uint32_t n_tris, const uint32_t *tri_mat,
-                       const chk_material *mats, uint32_t n_mats,
-                       const chk_mesh_light *lights, uint32_t n_lights) {
+void *orc_direct_scene_desc(const chk_scene_desc *D) {
+  const uint32_t n_tris = D->n_tris;
   DScene *S = new DScene();
+  S->n_tris = n_tris;
   S->td.resize((size_t)n_tris * 9);
-  for (size_t i = 0; i < S->td.size(); ++i) S->td[i] = tris[i];
+  for (size_t i = 0; i < S->td.size(); ++i) S->td[i] = D->tris[i];
   S->T.resize(n_tris);
   for (uint32_t i = 0; i < n_tris; ++i) S->T[i] = make_tri(&S->td[9 * (size_t)i]);
   S->B = bvh_build(S->td.data(), n_tris);
-  S->mats.resize(n_mats);
-  for (uint32_t i = 0; i < n_mats; ++i) S->mats[i] = to_mat(mats[i]);
-  S->tri_mat.assign(tri_mat, tri_mat + n_tris);
-  S->L.resize(n_lights);
-  for (uint32_t j = 0; j < n_lights; ++j) {
-    S->L[j].first = lights[j].first_tri;
-    S->L[j].count = lights[j].n_tris;
+  S->mats.resize(D->n_mats);
+  for (uint32_t i = 0; i < D->n_mats; ++i) S->mats[i] = to_mat(D->mats[i]);
+  S->tri_mat.assign(D->tri_mat, D->tri_mat + n_tris);
+  S->L.resize(D->n_mesh_lights);
+  for (uint32_t j = 0; j < D->n_mesh_lights; ++j) {
+    S->L[j].first = D->mesh_lights[j].first_tri;
+    S->L[j].count = D->mesh_lights[j].n_tris;
     init_mesh_light(*S, S->L[j]);
+  }
+  S->TL.resize(D->n_tri_lights);
+  for (uint32_t j = 0; j < D->n_tri_lights; ++j) {
+    S->TL[j].first = D->tri_lights[j];
+    S->TL[j].count = 1;
+    init_mesh_light(*S, S->TL[j]);  // area = Triangle::area
+  }
+  for (uint32_t j = 0; j < D->n_spheres; ++j) {
+    const chk_sphere &q = D->spheres[j];
+    S->sph.push_back(DSph{v3(q.center[0], q.center[1], q.center[2]), q.radius, q.mat});
+  }
+  S->ssrc.assign(D->sphere_sources, D->sphere_sources + D->n_sphere_sources);
+  for (uint32_t j = 0; j < D->n_point_sources; ++j) {
+    const chk_point_source &q = D->point_sources[j];
+    S->pts.push_back(DPt{v3(q.pos[0], q.pos[1], q.pos[2]), v3(q.power[0], q.power[1], q.power[2])});
   }
   return S;
 }
+
+void *orc_direct_scene(const float *tris, uint32_t n_tris, const uint32_t *tri_mat,
+                       const chk_material *mats, uint32_t n_mats,
+                       const chk_mesh_light *lights, uint32_t n_lights) {
+  chk_scene_desc D;
+  std::memset(&D, 0, sizeof(D));
+  D.tris = tris;
+  D.n_tris = n_tris;
+  D.tri_mat = tri_mat;
+  D.mats = mats;
+  D.n_mats = n_mats;
+  D.mesh_lights = lights;
+  D.n_mesh_lights = n_lights;
+  return orc_direct_scene_desc(&D);
+}
+
+uint32_t orc_mis_count(void *s) { return mis_count(*(const DScene *)s); }
 
 void orc_direct_scene_free(void *s) { delete (DScene *)s; }
 
@@ -1561,7 +1771,7 @@ void orc_mis_weights(void *s, uint32_t k, const double *x, const double *n,
   for (int64_t i = 0; i < (int64_t)k; ++i) {
     DPoint P{ld(x + 3 * i), ld(n + 3 * i), v3(0, 0, 0), prim[i]};
     DCtx C;
-    C.m = D.mats[D.tri_mat[P.prim]];
+    C.m = prim_mat(D, P.prim);
     C.wo_ref = ld(wo_ref + 3 * i);
     C.wo = ld(wo + 3 * i);
     for (uint32_t q = 0; q < S; ++q)
@@ -1575,11 +1785,11 @@ void orc_mis_samples(void *s, const double *x, const double *n, const double *wo
   const DScene &D = *(const DScene *)s;
   DPoint P{ld(x), ld(n), v3(0, 0, 0), prim};
   DCtx C;
-  C.m = D.mats[D.tri_mat[prim]];
+  C.m = prim_mat(D, prim);
   C.wo_ref = ld(wo_ref);
   C.wo = ld(wo);
   const uint32_t key[2] = {seed_lo, seed_hi};
-  const uint32_t Sn = (uint32_t)D.L.size() * CHK_MIS_LIGHT_SAMPLES + CHK_MIS_BSDF_SAMPLES;
+  const uint32_t Sn = mis_count(D);
   std::vector<V3> out(Sn);
   mis_samples(D, C, P, point, estimate, 0, key, out.data());
   for (uint32_t q = 0; q < Sn; ++q) st(samples + 3 * q, out[q]);
@@ -1633,23 +1843,22 @@ void orc_render_shade(void *s, const chk_camera *cam, const chk_render_params *P
         V3 dir = cx * ((double(x) + jx) / W - .5) + cy * ((double(y) + jy) / H - .5) + cdir;
         V3 d = norm(dir);
         V3 L = v3(0, 0, 0);
-        double t;
-        V3 Ng;
-        int rid;
         const bool phys = (P->flags & CHK_RENDER_PHYSICAL_EMITTER) != 0;
         if (shader == CHK_SHADE_IMPLICIT) {
           L = shade_implicit(D, cpos, d, idx, sm, key, phys);
         } else if (shader == CHK_SHADE_EXPLICIT) {
           L = shade_explicit(D, *dp, cpos, d, idx, sm, key, phys);
-        } else if (bvh_closest(*D.B, cpos, d, 1, &t, &Ng, &rid, nullptr, nullptr)) {
+        } else {
           // shadeDirectOnly (shader.cpp:339-363)
-          uint32_t prim = D.B->ordered[rid];
-          const Mat &m = D.mats[D.tri_mat[prim]];
-          if (m.light != CHK_LIGHT_NONE) {
-            L = (P->flags & CHK_RENDER_PHYSICAL_EMITTER) ? m.rad : maxnorm(m.rad);
-          } else {
-            DPoint Q{cpos + d * t, Ng, d, prim};
-            L = mul(m.c, direct_estimate(D, *dp, Q, idx, sm));  // shadeDI: c.mult(light)
+          const Hit h = closest(D, cpos, d);
+          if (h.hit) {
+            const Mat &m = prim_mat(D, h.prim);
+            if (m.light != CHK_LIGHT_NONE) {
+              L = phys ? m.rad : maxnorm(m.rad);
+            } else {
+              DPoint Q{cpos + d * h.t, h.N, d, h.prim};
+              L = mul(m.c, direct_estimate(D, *dp, Q, idx, sm));  // shadeDI: c.mult(light)
+            }
           }
         }
         acc[0] += L.x; acc[1] += L.y; acc[2] += L.z;

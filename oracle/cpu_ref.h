@@ -145,14 +145,24 @@ uint64_t orc_render(const float *tris, uint32_t n_tris, const uint32_t *tri_mat,
 void *orc_direct_scene(const float *tris, uint32_t n_tris, const uint32_t *tri_mat,
                        const chk_material *mats, uint32_t n_mats,
                        const chk_mesh_light *lights, uint32_t n_lights);
+/* The same with every kind of primitive and source of chk_scene_desc:
+ * spheres (Sphere::intersect, geometryPrimitives.cpp:12-27, in shadow() and
+ * the shaders' closest hit), triangle lights (TriLight), sphere sources
+ * (SphereSource: cone sampling, mis.cpp:103-115, domainSampler.cpp:320-369)
+ * and point sources (PointSource, lightSources.cpp:47-58).  A primitive
+ * index at or above n_tris names sphere index - n_tris. */
+void *orc_direct_scene_desc(const chk_scene_desc *desc);
 void orc_direct_scene_free(void *scene);
+/* Size of MIS::getSamples' sample set for this scene (10 per sphere source,
+ * mesh light and triangle light, 60 BSDF). */
+uint32_t orc_mis_count(void *scene);
 /* MIS weights (mis.cpp:18-95,209-220) of S given samples per point
  * (samples k x S x 3, weights k x S). */
 void orc_mis_weights(void *scene, uint32_t k, const double *x, const double *n,
                      const double *wo_ref, const double *wo, const uint32_t *prim,
                      const double *samples, uint32_t S, double *weights);
 /* The MIS::getSamples sample set of one point (mis.cpp:97-191) drawn with
- * the Philox counter of chakum_types.h: S = 10 L + 60 unit directions. */
+ * the Philox counter of chakum_types.h: orc_mis_count unit directions. */
 void orc_mis_samples(void *scene, const double *x, const double *n, const double *wo_ref,
                      const double *wo, uint32_t prim, uint32_t point, uint32_t estimate,
                      uint32_t seed_lo, uint32_t seed_hi, double *samples);

@@ -189,25 +189,43 @@ void ref_ltc_shade(uint32_t k, const double *x, const double *n, const double *v
 }
 
 /* The reference's scene state for its direct-lighting estimators, set up
- * the way its front end does (prepareScene.cpp:180-241 then loadAccels() and
+ * the way its front end does (prepareScene.cpp:54-241 then loadAccels() and
  * loadLightSampler()): scene triangles in vTriangleList, each mesh light as
  * a MeshLight of TriLights (LightSource::addMSource also appends them to
- * vTriangleList), the dummy sphere and dummy triangle.
- *   tris   n_tris x 9 doubles; tri_mat: material per triangle
- *   md     n_mats x 14: brdf, ks, alpha, phong_exp, int_ior, ext_ior,
- *          reflectance, colour[3], radiance[3], light
- *   lights n_lights x 2: first triangle, triangle count (emitters) */
+ * vTriangleList), triangle lights by addTSource, sphere objects in
+ * vSphereList, sphere sources by addSSource (which appends the source's
+ * sphere), point sources by addPSource, and the dummy triangle.
+ *   tris    n_tris x 9 doubles; tri_mat: material per triangle
+ *   md      n_mats x 14: brdf, ks, alpha, phong_exp, int_ior, ext_ior,
+ *           reflectance, colour[3], radiance[3], light
+ *   lights  n_lights x 2: first triangle, triangle count (emitters)
+ *   tl      n_tl triangle indices (emitters)
+ *   sph     n_sph x 4: centre, radius; sph_mat: material per sphere
+ *   ssrc    n_ss sphere indices (emitters)
+ *   pts     n_pts x 6: position, power (the PointSource radiance field)
+ * The primitive of a shading point is an input triangle index, or n_tris +
+ * sphere index. */
 static std::vector<Triangle> g_prims;  // the BasePrimitive of every input triangle
+static std::vector<Sphere> g_sph;      // ... and of every input sphere
+static uint32_t g_ntris = 0;
 
-void ref_direct_setup(const double *tris, uint32_t n_tris, const uint32_t *tri_mat,
-                      const double *md, uint32_t n_mats, const uint32_t *lights,
-                      uint32_t n_lights) {
+static BasePrimitive *prim_ptr(uint32_t prim) {
+  return prim < g_ntris ? (BasePrimitive *)&g_prims[prim] : (BasePrimitive *)&g_sph[prim - g_ntris];
+}
+
+void ref_direct_setup_ex(const double *tris, uint32_t n_tris, const uint32_t *tri_mat,
+                         const double *md, uint32_t n_mats, const uint32_t *lights,
+                         uint32_t n_lights, const uint32_t *tl, uint32_t n_tl, const double *sph,
+                         const uint32_t *sph_mat, uint32_t n_sph, const uint32_t *ssrc,
+                         uint32_t n_ss, const double *pts, uint32_t n_pts) {
   (void)n_mats;
   vTriangleList.clear();
   vSphereList.clear();
+  g_ntris = n_tris;
   std::vector<int> is_light(n_tris, 0);
   for (uint32_t j = 0; j < n_lights; ++j)
     for (uint32_t i = 0; i < lights[2 * j + 1]; ++i) is_light[lights[2 * j] + i] = 1;
+  for (uint32_t j = 0; j < n_tl; ++j) is_light[tl[j]] = 1;
   g_prims.assign(n_tris, Triangle());
   for (uint32_t o = 0; o < n_tris; ++o) {
     const double *p = tris + 9 * (size_t)o, *m = md + 14 * (size_t)tri_mat[o];
@@ -216,26 +234,70 @@ void ref_direct_setup(const double *tris, uint32_t n_tris, const uint32_t *tri_m
                           Vec(m[7], m[8], m[9]), m[6], mt);
     if (!is_light[o]) vTriangleList.push_back(g_prims[o]);
   }
+  std::vector<int> is_src(n_sph, 0);
+  for (uint32_t j = 0; j < n_ss; ++j) is_src[ssrc[j]] = 1;
+  g_sph.assign(n_sph, Sphere());
+  for (uint32_t j = 0; j < n_sph; ++j) {
+    const double *q = sph + 4 * (size_t)j, *m = md + 14 * (size_t)sph_mat[j];
+    MaterialType mt = make_material((int)m[0], m[1], m[2], m[3], m[4], m[5]);
+    g_sph[j] = Sphere(q[3], Vec(q[0], q[1], q[2]), Vec(m[7], m[8], m[9]), m[6], mt);
+    if (!is_src[j]) vSphereList.push_back(g_sph[j]);
+  }
   lSource = new LightSource();
   for (uint32_t j = 0; j < n_lights; ++j) {
     MeshLight ml;
     for (uint32_t i = 0; i < lights[2 * j + 1]; ++i) {
       const uint32_t o = lights[2 * j] + i;
       const double *p = tris + 9 * (size_t)o, *m = md + 14 * (size_t)tri_mat[o];
-      TriLight tl(Vec(p[0], p[1], p[2]), Vec(p[3], p[4], p[5]), Vec(p[6], p[7], p[8]),
-                  Vec(1, 1, 1), 1.0);
-      tl.radiance = Vec(m[10], m[11], m[12]);
-      ml.add(tl);
+      TriLight t(Vec(p[0], p[1], p[2]), Vec(p[3], p[4], p[5]), Vec(p[6], p[7], p[8]),
+                 Vec(1, 1, 1), 1.0);
+      t.radiance = Vec(m[10], m[11], m[12]);
+      ml.add(t);
     }
     ml.initMeshLight();
     lSource->addMSource(ml);
   }
-  // prepareScene.cpp:238,241
-  vSphereList.push_back(Sphere(1, Vec(27, 1e5, 47), Vec(.999, .999, .999), 1.0,
-                               MaterialType(200.2, 1)));
+  for (uint32_t j = 0; j < n_tl; ++j) {
+    const uint32_t o = tl[j];
+    const double *p = tris + 9 * (size_t)o, *m = md + 14 * (size_t)tri_mat[o];
+    TriLight t(Vec(p[0], p[1], p[2]), Vec(p[3], p[4], p[5]), Vec(p[6], p[7], p[8]), Vec(1, 1, 1),
+               1.0);
+    t.radiance = Vec(m[10], m[11], m[12]);
+    t.t.m = MaterialType(PLANAR, t.radiance);
+    lSource->addTSource(t);
+  }
+  for (uint32_t j = 0; j < n_ss; ++j) {
+    const double *q = sph + 4 * (size_t)ssrc[j], *m = md + 14 * (size_t)sph_mat[ssrc[j]];
+    SphereSource src(q[3], Vec(q[0], q[1], q[2]), Vec(1, 1, 1), 1.0);
+    src.radiance = Vec(m[10], m[11], m[12]);
+    src.p = Sphere(q[3], Vec(q[0], q[1], q[2]), Vec(0, 0, 0), 0, MaterialType(VOLUME, src.radiance));
+    lSource->addSSource(src);
+  }
+  for (uint32_t j = 0; j < n_pts; ++j) {
+    const double *q = pts + 6 * (size_t)j;
+    PointSource ps(Vec(q[0], q[1], q[2]), Vec(1, 1, 1), 1.0);
+    ps.radiance = Vec(q[3], q[4], q[5]);
+    lSource->addPSource(ps);
+  }
+  // prepareScene.cpp:241
   vTriangleList.push_back(Triangle(1e6, 1e6, 1e6, Vec(0.0, 0, 0.9), 0.9, MaterialType(1.0, 0.5)));
   loadAccels();
   loadLightSampler();
+}
+
+/* ref_direct_setup_ex with mesh lights only and the dummy sphere of
+ * prepareScene.cpp:238. */
+void ref_direct_setup(const double *tris, uint32_t n_tris, const uint32_t *tri_mat,
+                      const double *md, uint32_t n_mats, const uint32_t *lights,
+                      uint32_t n_lights) {
+  std::vector<double> m(md, md + 14 * (size_t)n_mats);
+  // MaterialType(200.2, 1): classic Phong, exponent 200.2, ks 1, colour .999
+  const double dm[14] = {1, 1, 0, 200.2, 0, 0, 1.0, .999, .999, .999, 0, 0, 0, 0};
+  m.insert(m.end(), dm, dm + 14);
+  const double dummy[4] = {27, 1e5, 47, 1};
+  const uint32_t dmat = n_mats;
+  ref_direct_setup_ex(tris, n_tris, tri_mat, m.data(), n_mats + 1, lights, n_lights, NULL, 0,
+                      dummy, &dmat, 1, NULL, 0, NULL, 0);
 }
 
 /* MIS::getSamples for k points (primitive = the point's input triangle):
@@ -250,7 +312,7 @@ uint32_t ref_mis_get_samples(uint32_t k, const double *x, const double *n, const
     S = lightSampler->getSamples(Vec(x[3 * i], x[3 * i + 1], x[3 * i + 2]),
                                  Vec(n[3 * i], n[3 * i + 1], n[3 * i + 2]),
                                  Vec(wo_ref[3 * i], wo_ref[3 * i + 1], wo_ref[3 * i + 2]),
-                                 Vec(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]), &g_prims[prim[i]],
+                                 Vec(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]), prim_ptr(prim[i]),
                                  smp, w);
     for (uint32_t q = 0; q < S && q < 200; ++q) {
       double *o = samples + 3 * ((size_t)i * 200 + q);
@@ -280,9 +342,10 @@ void ref_direct_stats(int est, uint32_t cv, uint32_t reps, uint32_t batches, uin
     for (uint32_t b = 0; b < batches; ++b) {
       Vec s;
       for (uint32_t q = 0; q < per; ++q) {
-        Vec v = est == 0   ? lSource->getLightFromMeshSource_Analytic(r, N, X, &g_prims[prim[i]])
-                : est == 1 ? lSource->getLightFromMeshSource_AnalyticCV(r, N, X, &g_prims[prim[i]], cv)
-                           : lSource->getLightFromAllSources_MIS(r, N, X, &g_prims[prim[i]]);
+        BasePrimitive *bp = prim_ptr(prim[i]);
+        Vec v = est == 0   ? lSource->getLightFromMeshSource_Analytic(r, N, X, bp)
+                : est == 1 ? lSource->getLightFromMeshSource_AnalyticCV(r, N, X, bp, cv)
+                           : lSource->getLightFromAllSources_MIS(r, N, X, bp);
         s = s + v;
       }
       bm[b] = s * (1.0 / per);

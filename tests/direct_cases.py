@@ -14,6 +14,17 @@ def fixture_scene(G):
                         np.asarray(G["mats"]), np.asarray(G["lights"]), cam, "direct_room")
 
 
+def sources_scene(G, prefix=""):
+    """A scene stored in tests/golden/sources.npz (prefix "" or "p_")."""
+    cam = scenes.sources_room().camera
+    g = lambda k: np.asarray(G[prefix + k])  # noqa: E731
+    return scenes.Scene(g("tris").astype(np.float32), g("tri_mat").astype(np.uint32), g("mats"),
+                        g("lights"), cam, "sources_room", spheres=g("spheres"),
+                        tri_lights=g("tri_lights").astype(np.uint32),
+                        sphere_sources=g("sphere_sources").astype(np.uint32),
+                        point_sources=g("point_sources"))
+
+
 def means(fn, k, E):
     """Mean and variance of the mean over E estimates fn(e) -> (k, 3)."""
     acc, acc2 = np.zeros((k, 3)), np.zeros((k, 3))

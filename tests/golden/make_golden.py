@@ -240,6 +240,45 @@ def gen_direct():
                         cv_mean=cv_m, cv_var=cv_v, cv_samples=16, reps=8192)
 
 
+def _scene_arrays(prefix, sc):
+    return {f"{prefix}{k}": getattr(sc, k) for k in
+            ("tris", "tri_mat", "mats", "lights", "spheres", "tri_lights", "sphere_sources",
+             "point_sources")}
+
+
+def _points(sc, k_tri, k_sph):
+    a = scenes.shading_points(sc, k_tri, seed=3)
+    b = scenes.sphere_points(sc, k_sph, seed=4)
+    return tuple(np.concatenate([u, v]) for u, v in zip(a, b))
+
+
+def gen_sources():
+    """sources_room: spheres, a sphere source, a triangle light, point
+    sources and mesh lights.  The reference's MIS::getSamples (samples and
+    weights), its getLightFromMeshSource_Analytic value and its
+    getLightFromAllSources_MIS and _AnalyticCV means at points on triangles
+    and on spheres; and its getLightFromAllSources_MIS values on a scene whose
+    only sources are the point sources (no random draw then matters)."""
+    sc = scenes.sources_room()
+    x, n, d, prim = _points(sc, 24, 8)
+    wo = -d
+    wo_ref = unit(n * 2.0 * (n * wo).sum(1, keepdims=True) + d)  # lightSources.cpp:44-45
+    Rd = oracle.RefDirect(sc)
+    smp, w = Rd.mis_get_samples(x, n, wo_ref, wo, prim)
+    analytic, _ = Rd.stats(0, x, n, d, prim, reps=1, batches=1)
+    mis_m, mis_v = Rd.stats(2, x, n, d, prim, reps=8192, batches=64)
+    cv_m, cv_v = Rd.stats(1, x, n, d, prim, reps=8192, cv_samples=16, batches=64)
+    sp = scenes.sources_room(mesh=False, sphere_source=False, tri_light=False)
+    px, pn, pd, pp = _points(sp, 24, 8)
+    Rp = oracle.RefDirect(sp)
+    pts_mis, _ = Rp.stats(2, px, pn, pd, pp, reps=1, batches=1)
+    np.savez_compressed(OUT / "sources.npz", **_scene_arrays("", sc), **_scene_arrays("p_", sp),
+                        x=x, n=n, d=d, prim=prim, wo_ref=wo_ref, wo=wo, samples=smp, weights=w,
+                        analytic=analytic, mis_mean=mis_m, mis_var=mis_v, cv_mean=cv_m,
+                        cv_var=cv_v, cv_samples=16, reps=8192, px=px, pn=pn, pd=pd, pprim=pp,
+                        pts_mis=pts_mis)
+
+
 def gen_scene_xml(R):
     """The reference's own front end (initSceneParser, prepareScreenCamera,
     prepareObjects, prepareLightSources) on its two scene files; the files
@@ -270,7 +309,7 @@ def gen_scene_xml(R):
 
 def main():
     """python tests/golden/make_golden.py [fixture ...]: all fixtures, or the
-    named ones (bvh_closest, shadow, brdf, ltc, spheres, direct, scene_xml)."""
+    named ones (bvh_closest, shadow, brdf, ltc, spheres, direct, scene_xml, sources)."""
     oracle.build(ref=True)
     R = oracle.ref_lib()
     if R is None:
@@ -292,6 +331,8 @@ def main():
         gen_direct()
     if on("scene_xml"):
         gen_scene_xml(R)
+    if on("sources"):
+        gen_sources()
     for p in sorted(OUT.glob("*.npz")):
         print(p.name, p.stat().st_size)
 

@@ -55,26 +55,48 @@ def test_camera_basis_equals_reference(G, name):
     np.testing.assert_allclose(cy, cam[3], rtol=1e-13)
 
 
-def test_spheres_and_unsupported_sources(G):
+def test_spheres_and_sources(G):
+    """scene1.xml: sphere objects, the dummy sphere and the sphere source's
+    visible sphere in vSphereList's order with their materials, and the
+    sphere source, equal to the reference's parsed scene."""
     sc = scene_xml.load_scene_xml(str(G["scene1_xml"]))
     ref = G["scene1_spheres"]
-    sp = sc.meta["spheres"]
-    assert len(sp) == ref.shape[0]
-    for s, r in zip(sp, ref):
+    assert len(sc.spheres) == ref.shape[0]
+    for s, r in zip(sc.spheres, ref):
         np.testing.assert_allclose(list(s["center"]) + [s["radius"]], r[:4], rtol=1e-15)
-        m = s["material"]
+        m = sc.mats[s["mat"]]
         got = np.r_[m["brdf"], m["light"], m["ks"], m["alpha"], m["phong_exp"], m["int_ior"],
                     m["ext_ior"], m["reflectance"], m["color"]].astype(np.float32)
         np.testing.assert_array_equal(got, r[4:15].astype(np.float32))
     ss = G["scene1_sphere_sources"]
-    assert len(sc.meta["sphere_sources"]) == ss.shape[0] == 1
-    s = sc.meta["sphere_sources"][0]
-    col = np.array(s["color"])
-    np.testing.assert_allclose(col / col.max() * s["radiance"], ss[0, 4:7], rtol=1e-15)
-    assert sc.meta["unsupported"]
-    with pytest.raises(ValueError, match="SPHERESOURCE"):
-        scene_xml.load_scene_xml(str(G["scene1_xml"]), strict=True)
-    assert not scene_xml.load_scene_xml(str(G["scene0_xml"]), strict=True).meta["unsupported"]
+    assert len(sc.sphere_sources) == ss.shape[0] == 1
+    q = sc.spheres[sc.sphere_sources[0]]
+    np.testing.assert_allclose(list(q["center"]) + [q["radius"]], ss[0, :4], rtol=1e-15)
+    np.testing.assert_allclose(sc.mats[q["mat"]]["radiance"], ss[0, 4:7].astype(np.float32),
+                               rtol=0)
+    assert sc.mats[q["mat"]]["light"] == scenes.LIGHT_VOLUME
+    assert len(sc.tri_lights) == len(sc.point_sources) == 0
+    s0 = scene_xml.load_scene_xml(str(G["scene0_xml"]))
+    assert len(s0.spheres) == 1 and len(s0.sphere_sources) == 0  # the dummy sphere only
+
+
+def test_point_and_triangle_sources():
+    """POINTSOURCE (parsePointSource, sceneParser.cpp:76-90) and TRISOURCE
+    (prepareScene.cpp:87-111): power and radiance are col.maxnorm() times
+    the value; the triangle light's triangle follows the objects."""
+    text = ("<SCREEN><WIDTH>4</WIDTH><HEIGHT>4</HEIGHT></SCREEN><CAMERA><POSITION>0 0 0</POSITION>"
+            "<LOOKAT>0 0 -1</LOOKAT><UP>0 1 0</UP><FOVX>60</FOVX><FOVY>60</FOVY></CAMERA>"
+            "<POINTSOURCE><POSITION>1 2 3</POSITION><COLOR>0.5 1.0 0.25</COLOR><POWER>8</POWER>"
+            "</POINTSOURCE>"
+            "<TRISOURCE><TR><V1>0 0 -1</V1><V2>1 0 -1</V2><V3>0 1 -1</V3><COLOR>2 1 1</COLOR>"
+            "<RADIANCE>3</RADIANCE></TR></TRISOURCE>")
+    sc = scene_xml.load_scene_xml(text)
+    np.testing.assert_array_equal(sc.point_sources["pos"][0], [1, 2, 3])
+    np.testing.assert_array_equal(sc.point_sources["power"][0], [4.0, 8.0, 2.0])
+    assert sc.tri_lights.tolist() == [sc.n_tris - 1] and len(sc.lights) == 0
+    m = sc.mats[sc.tri_mat[sc.tri_lights[0]]]
+    assert m["light"] == scenes.LIGHT_PLANAR
+    np.testing.assert_array_equal(m["radiance"], np.float32([3.0, 1.5, 1.5]))
 
 
 def test_temperature_colours():
