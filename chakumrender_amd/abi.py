@@ -41,10 +41,16 @@ class chk_point_source(C.Structure):
     _fields_ = [("pos", C.c_double * 3), ("power", C.c_double * 3)]
 
 
+class chk_env_source(C.Structure):
+    _fields_ = [("image", C.c_void_p), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("axis", C.c_uint32), ("pad", C.c_uint32), ("radiance", C.c_double * 3)]
+
+
 class chk_scene_desc(C.Structure):
     _fields_ = [("tris", C.c_void_p), ("tri_mat", C.c_void_p), ("mats", C.c_void_p),
                 ("mesh_lights", C.c_void_p), ("tri_lights", C.c_void_p), ("spheres", C.c_void_p),
                 ("sphere_sources", C.c_void_p), ("point_sources", C.c_void_p),
+                ("env", C.POINTER(chk_env_source)),
                 ("n_tris", C.c_uint32), ("n_mats", C.c_uint32), ("n_mesh_lights", C.c_uint32),
                 ("n_tri_lights", C.c_uint32), ("n_spheres", C.c_uint32),
                 ("n_sphere_sources", C.c_uint32), ("n_point_sources", C.c_uint32),
@@ -71,6 +77,17 @@ def scene_desc(scene):
     d = chk_scene_desc()
     for k, a in arrs.items():
         setattr(d, k, a.ctypes.data if a.shape[0] else None)
+    env = getattr(scene, "env", None)
+    if env is not None:
+        img = np.ascontiguousarray(env.image, np.float64)
+        assert img.ndim == 3 and img.shape[2] == 3
+        e = chk_env_source()
+        e.image = img.ctypes.data
+        e.height, e.width = img.shape[0], img.shape[1]
+        e.axis = int(env.axis)
+        e.radiance[:] = [float(v) for v in env.radiance]
+        arrs["env_image"], arrs["env"] = img, e
+        d.env = C.pointer(e)
     d.n_tris = arrs["tris"].shape[0]
     d.n_mats = arrs["mats"].shape[0]
     d.n_mesh_lights = arrs["mesh_lights"].shape[0]

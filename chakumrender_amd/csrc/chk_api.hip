@@ -99,7 +99,8 @@ namespace {
 // triangles and samples mesh lights only.
 chk_status reject_extras(const chk_scene_s *sc, const char *who) {
   if (sc && sc->extras) {
-    set_error(std::string(who) + ": the scene has spheres or triangle, sphere or point sources; "
+    set_error(std::string(who) + ": the scene has spheres or triangle, sphere, point or "
+              "environment sources; "
               "only the direct-lighting entry points and chk_render_shade render those");
     return CHK_EINVAL;
   }
@@ -312,6 +313,22 @@ chk_status chk_scene_create_desc(chk_ctx_t ctx, const chk_scene_desc *desc, chk_
                 " names no emitter sphere");
       return CHK_EINVAL;
     }
+  }
+  if (D.env) {
+    const chk_env_source &E = *D.env;
+    const uint64_t px = (uint64_t)E.width * E.height;
+    if (!E.image || E.width == 0 || E.height == 0 || E.axis > 2 || px >= kMaxEnvPixels ||
+        !std::isfinite(E.radiance[0]) || !std::isfinite(E.radiance[1]) ||
+        !std::isfinite(E.radiance[2])) {
+      set_error("chk_scene_create: the environment light needs an image of 1 .. 2^28 pixels, "
+                "axis 0, 1 or 2 and a finite radiance");
+      return CHK_EINVAL;
+    }
+    for (uint64_t i = 0; i < 3 * px; ++i)
+      if (!std::isfinite(E.image[i])) {
+        set_error("chk_scene_create: the environment image is not finite");
+        return CHK_EINVAL;
+      }
   }
   for (uint32_t j = 0; j < D.n_point_sources; ++j)
     for (int c = 0; c < 3; ++c)
@@ -534,7 +551,16 @@ chk_status chk_scene_create_desc(chk_ctx_t ctx, const chk_scene_desc *desc, chk_
   sc->d.n_spheres = D.n_spheres;
   sc->d.n_sph_src = D.n_sphere_sources;
   sc->d.n_points = D.n_point_sources;
-  sc->extras = D.n_tri_lights || D.n_spheres || D.n_sphere_sources || D.n_point_sources;
+  sc->extras = D.n_tri_lights || D.n_spheres || D.n_sphere_sources || D.n_point_sources || D.env;
+  if (D.env) {
+    std::string why;
+    if (!build_env(sc, *D.env, why)) {
+      set_error("chk_scene_create: " + why);
+      (void)hipFree(blob);
+      delete sc;
+      return CHK_EDEVICE;
+    }
+  }
   (void)used_lights;
   sc->d.n_leaf_tris = m;
   sc->d.n_tris = n_tris;
@@ -576,6 +602,7 @@ chk_status chk_scene_destroy(chk_scene_t sc) {
   if (sc->work) (void)hipFree(sc->work);
   if (sc->lwork) (void)hipFree(sc->lwork);
   if (sc->dwork) (void)hipFree(sc->dwork);
+  if (sc->env_mem) (void)hipFree(sc->env_mem);
   if (sc->blob) (void)hipFree(sc->blob);
   delete sc;
   return CHK_OK;

@@ -23,6 +23,7 @@
 #include <cmath>
 #include <cstring>
 
+#include "chk_env_dev.h"
 #include "chk_internal.h"
 #include "chk_ltc_dev.h"
 
@@ -117,9 +118,10 @@ __device__ inline d3 light_point(const DeviceScene &S, uint32_t j, u4 r, uint32_
 
 // ------------------------------------------------------------ spheres, sources
 // Emitter code of a sample item: a light-triangle index (mesh or triangle
-// light), or a sphere-source / point-source index under its tag.  kOccS
-// marks a sample whose shadow ray a sphere blocks.
-constexpr uint32_t kTagSph = 0x10000000u, kTagPt = 0x20000000u, kTagMask = 0x30000000u;
+// light), or a sphere-source / point-source index or an environment pixel
+// under its tag.  kOccS marks a sample whose shadow ray a sphere blocks.
+constexpr uint32_t kTagSph = 0x10000000u, kTagPt = 0x20000000u, kTagEnv = 0x30000000u;
+constexpr uint32_t kTagMask = 0x30000000u;
 constexpr uint32_t kCodeMask = 0x0FFFFFFFu, kOccS = 0x80000000u;
 // A closest hit on a sphere: leaf word kHitSph | sphere, its FP64 t in the
 // hit's z (low word) and w (high word).
@@ -203,6 +205,7 @@ __device__ inline d3 cone_dir(const Cone &C, double e1, double e2) {
 // The radiance of an emitter code.
 __device__ inline d3 code_radiance(const DeviceScene &S, uint32_t code) {
   const uint32_t j = code & kCodeMask;
+  if ((code & kTagMask) == kTagEnv) return env_pixel_radiance(S.env, j);
   if ((code & kTagMask) == kTagSph) {
     const chk_material &m = S.mats[S.sphere_mat[S.sph_src[j]]];
     return dk(m.radiance[0], m.radiance[1], m.radiance[2]);
@@ -388,6 +391,16 @@ __device__ inline void shadow_ray(const PointRec &P, d3 y, float4 *__restrict__ 
   ray[2 * i] = make_float4(org.x, org.y, org.z, (float)(dist - 1e-5));
   ray[2 * i + 1] = make_float4((float)dir.x, (float)dir.y, (float)dir.z, __uint_as_float(P.leaf));
 }
+// A shadow ray along unit w for distance dist (shadow(rr, dist) with a far
+// dist): same origin and margin as shadow_ray.
+__device__ inline void shadow_dir(const PointRec &P, d3 w, double dist, float4 *__restrict__ ray,
+                                  size_t i) {
+  const f3 xf = mk((float)P.x.x, (float)P.x.y, (float)P.x.z);
+  const f3 nf = mk((float)P.n.x, (float)P.n.y, (float)P.n.z);
+  const f3 org = offset_ray(xf, nf);
+  ray[2 * i] = make_float4(org.x, org.y, org.z, (float)(dist - 1e-5));
+  ray[2 * i + 1] = make_float4((float)w.x, (float)w.y, (float)w.z, __uint_as_float(P.leaf));
+}
 __device__ inline void no_ray(float4 *__restrict__ ray, size_t i) {
   ray[2 * i] = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
   ray[2 * i + 1] = make_float4(0.0f, 0.0f, 1.0f, __uint_as_float(kNone));
@@ -415,6 +428,9 @@ __device__ inline double mis_weight(const DeviceScene &S, const MatT<double> &m,
     }
     denom += (double)CHK_MIS_LIGHT_SAMPLES * p;
   }
+  if (S.env.on)  // EnMpIS (mis.cpp:84-87): grey(L) / normalizationArea
+    denom += (double)CHK_MIS_LIGHT_SAMPLES *
+             (grey(env_pixel_radiance(S.env, env_pixel(S.env, s.x, s.y, s.z))) * (1.0 / S.env.area));
   double f, pdf;
   brdf_pdf(m, n, wo_ref, wo, s, f, pdf);
   denom += (double)CHK_MIS_BSDF_SAMPLES * pdf;
@@ -430,17 +446,19 @@ struct Items {
 };
 
 // Sample slots of MIS::getSamples (mis.cpp:97-191): 10 per sphere source
-// (cone), mesh light and triangle light (area), then 60 BSDF slots; the MIS
-// estimator adds one slot per point source after them.
+// (cone), mesh light and triangle light (area), 10 for the environment light
+// (importance), then 60 BSDF slots; the MIS estimator adds one slot per
+// point source after them.
 struct Slots {
-  uint32_t sph, mesh, tri, mis;  // first mesh slot, first triangle-light slot, first BSDF slot, total
+  uint32_t sph, mesh, tri, env, mis;  // the first slot after each group; mis = total
 };
 __device__ inline Slots mis_slots(const DeviceScene &S) {
   Slots L;
   L.sph = S.n_sph_src * CHK_MIS_LIGHT_SAMPLES;
   L.mesh = L.sph + S.n_lights * CHK_MIS_LIGHT_SAMPLES;
   L.tri = L.mesh + S.n_tri_lights * CHK_MIS_LIGHT_SAMPLES;
-  L.mis = L.tri + CHK_MIS_BSDF_SAMPLES;
+  L.env = L.tri + (S.env.on ? CHK_MIS_LIGHT_SAMPLES : 0u);
+  L.mis = L.env + CHK_MIS_BSDF_SAMPLES;
   return L;
 }
 
@@ -526,8 +544,10 @@ __global__ __launch_bounds__(kDBlock) void k_dl_samples(DeviceScene S, const Poi
   } else if (slot < L.tri) {
     uint32_t tri;
     w = normalize(light_point(S, (slot - L.sph) / CHK_MIS_LIGHT_SAMPLES, r, &tri) - R.x);
+  } else if (slot < L.env) {
+    w = env_sample(S.env, ud(r.x), ud(r.y));  // EnMpIS (mis.cpp:157-169)
   } else {
-    const uint32_t q = slot - L.tri;
+    const uint32_t q = slot - L.env;
     w = brdf_sample(m, R.n, R.wo_ref, R.wo, q < R.nb ? -1.0 : 2.0, ud(r.y), ud(r.z));
   }
   const double wt = mis_weight(S, m, R.x, R.n, R.wo_ref, R.wo, w);
@@ -592,6 +612,14 @@ __global__ __launch_bounds__(kDBlock) void k_dl_samples(DeviceScene S, const Poi
       }
     }
   }
+  bool env = false;
+  if (EST == CHK_DIRECT_MIS && code == kNone && S.env.on) {
+    // lightSources.cpp:102-104: no source hit; the environment counts when
+    // nothing at all is hit (shadow(rr, INF))
+    code = kTagEnv | env_pixel(S.env, w.x, w.y, w.z);
+    pd = kInf64;
+    env = true;
+  }
   if (code == kNone) {
     no_ray(I.ray, i);
     return;
@@ -611,6 +639,8 @@ __global__ __launch_bounds__(kDBlock) void k_dl_samples(DeviceScene S, const Poi
     if (S.n_spheres && sph_occluded(S, o, w, pd)) {
       code |= kOccS;
       no_ray(I.ray, i);
+    } else if (env) {
+      shadow_dir(R, w, pd, I.ray, i);
     } else {
       shadow_ray(R, o + w * pd, I.ray, i);
     }
@@ -1117,12 +1147,15 @@ inline size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
 uint32_t n_slots_of(const chk_scene_s *sc, const chk_direct_params &p) {
   const DeviceScene &S = sc->d;
   const uint32_t n_mis =
-      (S.n_sph_src + S.n_lights + S.n_tri_lights) * CHK_MIS_LIGHT_SAMPLES + CHK_MIS_BSDF_SAMPLES;
+      (S.n_sph_src + S.n_lights + S.n_tri_lights + (S.env.on ? 1u : 0u)) * CHK_MIS_LIGHT_SAMPLES +
+      CHK_MIS_BSDF_SAMPLES;
   switch (p.estimator) {
     case CHK_DIRECT_ANALYTIC: return 0;
     case CHK_DIRECT_ANALYTIC_CV: return S.n_lights * std::min(p.cv_samples, 1000u);
     case CHK_DIRECT_MIS:
-      return (S.n_lights || S.n_tri_lights || S.n_sph_src || S.n_points) ? n_mis + S.n_points : 0;
+      return (S.n_lights || S.n_tri_lights || S.n_sph_src || S.n_points || S.env.on)
+                 ? n_mis + S.n_points
+                 : 0;
     default: return S.n_lights ? n_mis : 0;
   }
 }

@@ -63,6 +63,16 @@ struct LightRec {
   float area;      // total area of the mesh light
   float pad;
 };
+// An environment light (EnvSource, lightSources.h:33-109) with the tables
+// of its initSamples (lightSources.cpp:740-782).
+struct EnvDev {
+  const double *img = nullptr;      // h x w x 3
+  const uint32_t *inv_m = nullptr;  // inverse CDF of the marginal, kEnvBins entries
+  const uint32_t *inv_c = nullptr;  // inverse CDFs of the conditionals, kEnvRows x kEnvBins
+  double rad[3] = {0, 0, 0};        // BaseSource radiance (col.maxnorm() * intensity)
+  double area = 0;                  // normalizationArea
+  uint32_t w = 0, h = 0, axis = 0, on = 0;
+};
 // A sphere primitive in FP64 (Sphere, geometryPrimitives.h:38-52).
 struct SphereRec {
   double c[3];
@@ -88,6 +98,7 @@ struct DeviceScene {
   const uint32_t *sph_src = nullptr;   // SphereSource j -> its sphere
   const chk_point_source *points = nullptr;
   uint32_t n_tri_lights = 0, n_spheres = 0, n_sph_src = 0, n_points = 0;
+  EnvDev env;
   uint32_t n_leaf_tris = 0;
   uint32_t n_tris = 0;                 // original triangle count
   // box of the BVH triangles (degenerate ones excluded): the learned
@@ -122,6 +133,7 @@ struct chk_scene_s {
   // spheres, triangle, sphere or point sources: only the direct-lighting
   // entry points and chk_render_shade handle them
   bool extras = false;
+  void *env_mem = nullptr;  // environment image and tables
 };
 
 namespace chk {
@@ -152,6 +164,11 @@ constexpr uint32_t kMaxTraceBatch = 1u << 31;  // rays per trace call
 // sphere or point source costs sample slots per shading point.
 constexpr uint32_t kMaxSpheres = 4096;
 constexpr uint32_t kMaxSources = 1024;
+// An environment image's pixel index must fit an emitter code (28 bits).
+constexpr uint64_t kMaxEnvPixels = 1ull << 28;
+// Builds the environment light's device image and tables into sc (scene
+// creation; synchronises).  Returns false with a message on failure.
+bool build_env(chk_scene_s *sc, const chk_env_source &E, std::string &err);
 inline SegRange linear_range(uint32_t n) { return SegRange{nullptr, n, seg_capacity(n)}; }
 
 // Kernel launchers (chk_trace.hip).

@@ -27,6 +27,8 @@ libchakum.so.
 from __future__ import annotations
 
 import ctypes as C
+import re
+from pathlib import Path
 
 import numpy as np
 
@@ -492,3 +494,58 @@ def write_ppm(path, image):
     with open(path, "w") as f:
         f.write("P3\n%d %d\n%d\n" % (W, H, 255))
         f.write("".join("%d %d %d " % (a, b, c) for a, b, c in v))
+
+
+def read_ppm6(path):
+    """readImage (ppm.cpp:36-115) of a binary P6 file: the first line (at
+    most 15 characters, fgets) must start with "P6"; lines starting with '#'
+    right after it are skipped; then width and height, then the maximum
+    component, which must be 255; the rest of that line is skipped and width
+    * height * 3 bytes follow.  Returns (H, W, 3) float64 of byte / 255.0,
+    row 0 first.  Raises ValueError where readImage returns 0."""
+    data = Path(path).read_bytes()
+    nl = data.find(b"\n")
+    first = data[:min(15, nl + 1 if nl >= 0 else len(data))]
+    if len(first) < 2 or first[:2] != b"P6":
+        raise ValueError("Invalid image format (must be 'P6')")
+    pos = len(first)
+    while pos < len(data) and data[pos:pos + 1] == b"#":
+        e = data.find(b"\n", pos)
+        pos = len(data) if e < 0 else e + 1
+
+    def scan_int(pos):  # fscanf "%d": whitespace, an optional sign, digits
+        while pos < len(data) and data[pos:pos + 1].isspace():
+            pos += 1
+        m = re.match(rb"[+-]?\d+", data[pos:pos + 32])
+        if not m:
+            raise ValueError("Invalid image size or rgb component")
+        return int(m.group()), pos + m.end()
+
+    w, pos = scan_int(pos)
+    h, pos = scan_int(pos)
+    maxval, pos = scan_int(pos)
+    if maxval != 255:
+        raise ValueError("does not have 8-bits components")
+    e = data.find(b"\n", pos)
+    if e < 0:
+        raise ValueError("no pixel data")
+    raw = data[e + 1:e + 1 + 3 * w * h]
+    if w <= 0 or h <= 0 or len(raw) != 3 * w * h:
+        raise ValueError("Error loading image")
+    return np.frombuffer(raw, np.uint8).reshape(h, w, 3).astype(np.float64) / 255.0
+
+
+def write_ppm6(path, image, comment=None):
+    """writeImage (ppm.cpp:117-152): "P6\\n", an optional "# comment\\n",
+    "%d %d\\n255\\n", then per component (min(v, 1) * 255) converted to a byte
+    as the reference's x86 build converts it (truncated toward zero, the low
+    8 bits kept).  image: (H, W, 3), row 0 first."""
+    img = np.asarray(image, np.float64)
+    H, W, _ = img.shape
+    v = np.trunc(np.where(img > 1, 1.0, img) * 255).astype(np.int64) & 0xFF
+    with open(path, "wb") as f:
+        f.write(b"P6\n")
+        if comment is not None:
+            f.write(b"# " + comment.encode() + b"\n")
+        f.write(b"%d %d\n%d\n" % (W, H, 255))
+        f.write(v.astype(np.uint8).tobytes())

@@ -69,6 +69,25 @@ class Camera:
 
 
 @dataclass
+class EnvMap:
+    """An environment light EnvSource(image, w, h, axis, col, intensity)
+    (lightSources.h:33-109): image (h, w, 3) float64 with row 0 at theta = 0
+    about the world axis `axis` (0 x, 1 y, 2 z).  The reference's front end
+    would add EnvMap(readImage("env1.ppm"), 1, (1, 1, 1), 2)
+    (prepareScene.cpp:150-154, commented out there)."""
+    image: np.ndarray
+    axis: int = 1
+    col: tuple = (1.0, 1.0, 1.0)
+    intensity: float = 1.0
+
+    @property
+    def radiance(self):
+        """BaseSource: col.maxnorm() * intensity (lightSources.h:12-16)."""
+        c = np.asarray(self.col, np.float64)
+        return tuple(float(v) for v in c / c.max() * self.intensity)
+
+
+@dataclass
 class Scene:
     tris: np.ndarray          # (n, 9) float32: A, B, C
     tri_mat: np.ndarray       # (n,) uint32 material index per triangle
@@ -82,6 +101,7 @@ class Scene:
     tri_lights: np.ndarray = field(default_factory=lambda: np.zeros(0, np.uint32))
     sphere_sources: np.ndarray = field(default_factory=lambda: np.zeros(0, np.uint32))
     point_sources: np.ndarray = field(default_factory=lambda: np.zeros(0, POINT_DTYPE))
+    env: EnvMap | None = None
 
     @property
     def n_tris(self):
@@ -481,6 +501,25 @@ def sources_room(seed=0x5EED0008, width=64, height=64, mesh=True, sphere_source=
                  mats_arr, np.asarray(lights, LIGHT_DTYPE), sc.camera, "sources_room",
                  {"seed": seed}, spheres=sph_arr, tri_lights=np.array(tl, np.uint32),
                  sphere_sources=np.array(ss, np.uint32), point_sources=pt_arr)
+
+
+def sky_image(seed=0x5EED000A, width=64, height=32, sun=(0.35, 0.6)):
+    """A small 8-bit sky for an environment light: a blue gradient above the
+    horizon, darker ground below and a saturated sun disc at (theta, phi) =
+    sun * (pi, 2 pi); values k / 255 like a P6 file's (readImage,
+    ppm.cpp:106-110).  (height, width, 3) float64."""
+    rng = np.random.default_rng(seed)
+    th = (np.arange(height) + 0.5) / height * math.pi
+    ph = (np.arange(width) + 0.5) / width * 2 * math.pi
+    T, P = np.meshgrid(th, ph, indexing="ij")
+    up = np.cos(T)
+    sky = np.stack([0.35 + 0.25 * up, 0.5 + 0.25 * up, 0.9 + 0.1 * up], -1)
+    ground = np.stack([0.25, 0.2, 0.15], -1) * (0.6 + 0.4 * rng.random((height, width, 1)))
+    img = np.where(up[..., None] > 0, sky, ground)
+    st, sp = sun[0] * math.pi, sun[1] * 2 * math.pi
+    cosd = np.sin(T) * math.sin(st) * np.cos(P - sp) + np.cos(T) * math.cos(st)
+    img = np.where((cosd > math.cos(0.25))[..., None], 1.0, img)
+    return np.round(np.clip(img, 0, 1) * 255) / 255.0
 
 
 def sphere_points(scene, k, seed=0x5EED0009, eye=(0.5, 0.55, 0.95), spheres=None):

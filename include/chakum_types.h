@@ -77,6 +77,21 @@ typedef struct chk_point_source {
   double power[3];
 } chk_point_source;
 
+/* An environment light: EnvSource (headers/lightSources.h:33-109).
+ *   image     height x width x 3 doubles; row 0 is theta = 0 about the map's
+ *             polar axis (the first row readImage loads, ppm.cpp:36-115)
+ *   axis      the world axis that is the map's polar axis: 0 x, 1 y, 2 z
+ *             (EnvSource::getRadiance, lightSources.h:80-94)
+ *   radiance  the BaseSource radiance getRadiance multiplies each pixel by,
+ *             col.maxnorm() * intensity
+ * The library copies the image and builds initSamples' tables
+ * (lightSources.cpp:662-782) on the GPU.  40 bytes. */
+typedef struct chk_env_source {
+  const double *image;
+  uint32_t width, height, axis, pad;
+  double radiance[3];
+} chk_env_source;
+
 /* Everything a scene holds, in the order LightSource and the object lists
  * keep it (lightSources.h:158-203, objects.h:7-17).  Light geometry is part
  * of the primitive lists, as LightSource::addMSource / addTSource / addSSource
@@ -85,7 +100,8 @@ typedef struct chk_point_source {
  *   tri_lights      TriLight: one emitter triangle each (triangle indices)
  *   sphere_sources  SphereSource: emitter spheres (sphere indices); the
  *                   sphere is both the visible sphere and the source
- *   point_sources   PointSource: not geometry */
+ *   point_sources   PointSource: not geometry
+ *   env             EnvSource or NULL (LightSource::addESource keeps one) */
 typedef struct chk_scene_desc {
   const float *tris;            /* n_tris x 9 (A, B, C) */
   const uint32_t *tri_mat;      /* n_tris */
@@ -95,6 +111,7 @@ typedef struct chk_scene_desc {
   const chk_sphere *spheres;
   const uint32_t *sphere_sources;
   const chk_point_source *point_sources;
+  const chk_env_source *env;
   uint32_t n_tris, n_mats, n_mesh_lights, n_tri_lights;
   uint32_t n_spheres, n_sphere_sources, n_point_sources, reserved;
 } chk_scene_desc;

@@ -85,6 +85,10 @@ def lib():
         L.orc_direct_scene_desc.argtypes = [_vp]
         L.orc_mis_count.restype = C.c_uint32
         L.orc_mis_count.argtypes = [_vp]
+        L.orc_env_tables.restype = C.c_double
+        L.orc_env_tables.argtypes = [_vp, _up, _up]
+        L.orc_env_radiance.argtypes = [_vp, C.c_uint32, _dp, _dp]
+        L.orc_env_sample.argtypes = [_vp, C.c_uint32, _dp, _dp]
         L.orc_mis_weights.argtypes = [_vp, C.c_uint32, _dp, _dp, _dp, _dp, _up, _dp, C.c_uint32,
                                       _dp]
         L.orc_mis_samples.argtypes = [_vp, _dp, _dp, _dp, _dp, C.c_uint32, C.c_uint32, C.c_uint32,
@@ -114,6 +118,12 @@ def ref_lib():
         R.ref_direct_setup_ex.argtypes = [_dp, C.c_uint32, _up, _dp, C.c_uint32, _up, C.c_uint32,
                                           _vp, C.c_uint32, _vp, _vp, C.c_uint32, _vp, C.c_uint32,
                                           _vp, C.c_uint32]
+        R.ref_env_setup.argtypes = [_dp, C.c_uint32, C.c_uint32, C.c_uint32, _dp, C.c_double]
+        R.ref_env_area.restype = C.c_double
+        R.ref_env_radiance.argtypes = [C.c_uint32, _dp, _dp]
+        R.ref_env_samples.argtypes = [C.c_uint32, _dp]
+        R.ref_read_ppm.argtypes = [C.c_char_p, _dp, C.c_uint32, _up]
+        R.ref_write_ppm.argtypes = [C.c_char_p, _dp, C.c_uint32, C.c_uint32]
         R.ref_mis_get_samples.restype = C.c_uint32
         R.ref_mis_get_samples.argtypes = [C.c_uint32, _dp, _dp, _dp, _dp, _up, _dp, _dp]
         R.ref_direct_stats.argtypes = [C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
@@ -306,6 +316,25 @@ class DirectScene:
         triangle light, 60 BSDF."""
         return int(lib().orc_mis_count(self.h))
 
+    def env_tables(self):
+        """(normalizationArea, inverse marginal CDF (2000), inverse
+        conditional CDFs (3000, 2000)) of the scene's environment light."""
+        im, ic = np.zeros(2000, np.uint32), np.zeros((3000, 2000), np.uint32)
+        area = lib().orc_env_tables(self.h, im, ic)
+        return area, im, ic
+
+    def env_radiance(self, dirs):
+        d = _d(dirs, (-1, 3))
+        out = np.empty_like(d)
+        lib().orc_env_radiance(self.h, d.shape[0], d, out)
+        return out
+
+    def env_sample(self, u):
+        u = _d(u, (-1, 2))
+        out = np.empty((u.shape[0], 3))
+        lib().orc_env_sample(self.h, u.shape[0], u, out)
+        return out
+
     def mis_weights(self, x, n, wo_ref, wo, prim, samples):
         smp = _d(samples)
         k, S = smp.shape[0], smp.shape[1]
@@ -402,8 +431,9 @@ class RefDirect:
         tl = np.ascontiguousarray(getattr(scene, "tri_lights", np.zeros(0)), np.uint32)
         ss = np.ascontiguousarray(getattr(scene, "sphere_sources", np.zeros(0)), np.uint32)
         pts = np.asarray(getattr(scene, "point_sources", np.zeros(0)))
+        env = getattr(scene, "env", None)
         with _quiet():
-            if sph.shape[0] or tl.shape[0] or ss.shape[0] or pts.shape[0]:
+            if sph.shape[0] or tl.shape[0] or ss.shape[0] or pts.shape[0] or env is not None:
                 # the scene's spheres as given (a scene file's list holds the
                 # dummy sphere of prepareScene.cpp:238 already)
                 sd = np.ascontiguousarray(np.concatenate(
@@ -419,8 +449,26 @@ class RefDirect:
             else:
                 R.ref_direct_setup(tris, scene.n_tris, tri_mat, np.ascontiguousarray(md),
                                    md.shape[0], lt, lt.shape[0])
+            if env is not None:
+                img = _d(env.image)
+                R.ref_env_setup(img.reshape(-1, 3), img.shape[1], img.shape[0], int(env.axis),
+                                _d(env.col), float(env.intensity))
         self.R = R
         self.n_lights = lt.shape[0]
+
+    def env_area(self):
+        return self.R.ref_env_area()
+
+    def env_radiance(self, dirs):
+        d = _d(dirs, (-1, 3))
+        out = np.empty_like(d)
+        self.R.ref_env_radiance(d.shape[0], d, out)
+        return out
+
+    def env_samples(self, n):
+        out = np.zeros((n, 3))
+        self.R.ref_env_samples(n, out)
+        return out
 
     def mis_get_samples(self, x, n, wo_ref, wo, prim):
         """MIS::getSamples per point: (samples (k, S, 3), weights (k, S))."""

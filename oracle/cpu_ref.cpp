@@ -782,6 +782,18 @@ struct DSph {  // Sphere (geometryPrimitives.h:38-52)
 struct DPt {  // PointSource (lightSources.h:22-31)
   V3 p, power;
 };
+// EnvSource (lightSources.h:33-109) with the tables of initSamples
+// (lightSources.cpp:680-782): normalizationArea, the 2000-bin inverse CDF of
+// the 3000-row marginal and, per row, of the 4000-column conditional.
+constexpr uint32_t kEnvAreaRows = 2000, kEnvRows = 3000, kEnvCols = 4000, kEnvBins = 2000;
+struct DEnv {
+  bool on = false;
+  std::vector<double> img;  // h x w x 3
+  uint32_t w = 0, h = 0, axis = 0;
+  V3 rad;                   // BaseSource radiance (col.maxnorm() * intensity)
+  double area = 0;
+  std::vector<uint32_t> inv_m, inv_c;
+};
 
 // The scene of prepareScene (prepareScene.cpp:54-240): the object lists and
 // LightSource's mList, tList, sList and pList.
@@ -796,6 +808,7 @@ struct DScene {
   std::vector<DSph> sph;    // spheres
   std::vector<uint32_t> ssrc;  // sphere sources -> their sphere
   std::vector<DPt> pts;     // point sources
+  DEnv env;                 // environment light (on or not)
   uint32_t n_tris = 0;
   ~DScene() { delete B; }
 };
@@ -875,6 +888,126 @@ bool mesh_light_isect(const DScene &S, const DLight &L, V3 o, V3 d, uint32_t *tr
     }
   }
   return *t < INF;
+}
+
+// ---------------------------------------------------------------- environment light
+inline double grey(V3 L) { return L.x * 0.3 + L.y * 0.59 + L.z * 0.11; }  // mathPrimitives.h:10
+// (uint32_t) of a double as the reference's x86 build converts it: NaN -> 0
+inline uint32_t to_u32(double v) { return v == v ? (uint32_t)v : 0u; }
+
+// EnvSource::getThetaPhi + getRadiance (lightSources.h:57-101): the pixel of
+// unit direction d.  Where a rounding excess makes the reference's acos or
+// asin NaN, its row or column conversion gives 0 (to_u32); the row is
+// clamped to the image (the reference reads one row past it at theta = pi).
+uint32_t env_pixel(const DEnv &E, V3 d) {
+  double x, y, z;
+  if (E.axis == 0) {
+    x = d.y; y = d.z; z = d.x;
+  } else if (E.axis == 1) {
+    x = d.x; y = d.z; z = d.y;
+  } else {
+    x = d.x; y = d.y; z = d.z;
+  }
+  const double theta = std::acos(z);
+  const double st = std::sqrt(1 - z * z);
+  double phi;
+  if (st < 1e-12) {
+    phi = 0.0;
+  } else {
+    x /= st;
+    y /= st;
+    if (y >= 0 && x >= 0)
+      phi = std::asin(y);
+    else if (y >= 0 && x <= 0)
+      phi = PI - std::asin(y);
+    else if (y <= 0 && x <= 0)
+      phi = PI + std::asin(-y);
+    else
+      phi = 2 * PI - std::asin(-y);
+  }
+  uint32_t col = to_u32((phi / (2 * PI)) * E.w), row = to_u32((theta / PI) * E.h);
+  col = col >= E.w ? E.w - 1 : col;
+  row = row >= E.h ? E.h - 1 : row;
+  return row * E.w + col;
+}
+inline V3 env_pixel_radiance(const DEnv &E, uint32_t p) {
+  return v3(E.img[3 * (size_t)p] * E.rad.x, E.img[3 * (size_t)p + 1] * E.rad.y,
+            E.img[3 * (size_t)p + 2] * E.rad.z);  // envMap[.].mult(radiance)
+}
+inline V3 env_radiance(const DEnv &E, V3 d) { return env_pixel_radiance(E, env_pixel(E, d)); }
+
+// invertedCDF (lightSources.cpp:662-678): bin k of the running sum takes
+// index i and fills the unset bins below it; bins past the total take the
+// last index.
+void inverted_cdf(const double *pdf, uint32_t length, double delta, uint32_t *inv, uint32_t nb) {
+  double sum = 0;
+  for (uint32_t j = 0; j < nb; ++j) inv[j] = 0;
+  for (uint32_t i = 0; i < length; ++i) {
+    sum += pdf[i] * delta;
+    const uint32_t k = to_u32(sum * nb);
+    inv[k >= nb ? nb - 1 : k] = i;
+    for (uint32_t j = k - 1; j < nb && inv[j] == 0; j--) inv[j] = i;
+  }
+  for (uint32_t j = to_u32(sum * nb); j < nb; ++j) inv[j] = length - 1;
+}
+
+// Sum over phi of grey(getRadiance) on row i of a rows x 4000 grid
+// (calcArea / calcMarginalDensity, lightSources.cpp:680-722).
+double env_row_sum(const DEnv &E, uint32_t rows, uint32_t i, double *row_vals) {
+  const double dth = PI / rows, dph = 2.0 * PI / kEnvCols;
+  const double st = std::sin(dth * i), ct = std::cos(dth * i);
+  double sum = 0;
+  for (uint32_t j = 0; j < kEnvCols; ++j) {
+    const double cp = std::cos(dph * j), sp = std::sin(dph * j);
+    const double g = grey(env_radiance(E, v3(st * cp, st * sp, ct)));
+    if (row_vals) row_vals[j] = g;
+    sum += g;
+  }
+  return sum;
+}
+
+// EnvSource::initSamples (lightSources.cpp:740-782) up to the inverse CDFs.
+void env_init(DEnv &E) {
+  std::vector<double> rs(kEnvAreaRows);
+#pragma omp parallel for schedule(dynamic, 8)
+  for (int64_t i = 0; i < (int64_t)kEnvAreaRows; ++i) rs[i] = env_row_sum(E, kEnvAreaRows, (uint32_t)i, nullptr);
+  const double dth_a = PI / kEnvAreaRows, dph = 2.0 * PI / kEnvCols;
+  double area = 0;
+  for (uint32_t i = 0; i < kEnvAreaRows; ++i) area += rs[i] * std::sin(dth_a * i);
+  E.area = area * dth_a * dph;
+  // marginal density (calcMarginalDensity, :702-722)
+  const double dth = PI / kEnvRows;
+  std::vector<double> pm(kEnvRows);
+#pragma omp parallel for schedule(dynamic, 8)
+  for (int64_t i = 0; i < (int64_t)kEnvRows; ++i) {
+    const double st = std::sin(dth * i);
+    pm[i] = (env_row_sum(E, kEnvRows, (uint32_t)i, nullptr) * st * dph) / E.area;
+  }
+  E.inv_m.assign(kEnvBins, 0);
+  inverted_cdf(pm.data(), kEnvRows, PI / kEnvRows, E.inv_m.data(), kEnvBins);
+  // conditional density per row (calcConditionalDensity, :724-738) and its
+  // inverse CDF
+  E.inv_c.assign((size_t)kEnvRows * kEnvBins, 0);
+#pragma omp parallel for schedule(dynamic, 8)
+  for (int64_t i = 0; i < (int64_t)kEnvRows; ++i) {
+    std::vector<double> g(kEnvCols);
+    env_row_sum(E, kEnvRows, (uint32_t)i, g.data());
+    const double st = std::sin(dth * i);
+    for (uint32_t j = 0; j < kEnvCols; ++j) g[j] = g[j] * st / (E.area * pm[i]);
+    inverted_cdf(g.data(), kEnvCols, 2 * PI / kEnvCols, &E.inv_c[(size_t)i * kEnvBins], kEnvBins);
+  }
+}
+
+// One importance sample from (e1, e2) (lightSources.cpp:765-779).
+V3 env_sample(const DEnv &E, double e1, double e2) {
+  const uint32_t ti = E.inv_m[(uint32_t)(e1 * kEnvBins)];
+  const double theta = ti * (PI / kEnvRows);
+  const uint32_t pi_ = E.inv_c[(size_t)ti * kEnvBins + (uint32_t)(e2 * kEnvBins)];
+  const double phi = pi_ * (2.0 * PI) / kEnvCols;
+  double x = std::sin(theta), y = x;
+  x *= std::cos(phi);
+  y *= std::sin(phi);
+  return v3(x, y, std::cos(theta));
 }
 
 // shadow() (shader.cpp:76-97): the closest sphere, then the closest
@@ -987,7 +1120,8 @@ inline void dphilox(uint32_t a, uint32_t b, uint32_t slot, uint32_t v, const uin
 // Size of MIS::getSamples' sample set: 10 per sphere source, mesh light and
 // triangle light, 60 for the BSDF (mis.h:8-12).
 uint32_t mis_count(const DScene &S) {
-  return (uint32_t)(S.ssrc.size() + S.L.size() + S.TL.size()) * CHK_MIS_LIGHT_SAMPLES +
+  return (uint32_t)(S.ssrc.size() + S.L.size() + S.TL.size() + (S.env.on ? 1 : 0)) *
+             CHK_MIS_LIGHT_SAMPLES +
          CHK_MIS_BSDF_SAMPLES;
 }
 
@@ -1013,6 +1147,11 @@ void mis_samples(const DScene &S, const DCtx &C, const DPoint &P, uint32_t a, ui
     for (uint32_t q = 0; q < CHK_MIS_LIGHT_SAMPLES; ++q, ++slot) {
       dphilox(a, b, slot, v, key, r);
       out[slot] = norm(mesh_light_point(S, L, r, &tri) - P.x);
+    }
+  if (S.env.on)  // EnMpIS (mis.cpp:157-169)
+    for (uint32_t q = 0; q < CHK_MIS_LIGHT_SAMPLES; ++q, ++slot) {
+      dphilox(a, b, slot, v, key, r);
+      out[slot] = env_sample(S.env, ud(r[0]), ud(r[1]));
     }
   // getBrdfDirectionSamples (materialTypes.cpp:118-134): floor(60 ks)
   // specular-lobe samples, then cosine samples
@@ -1055,6 +1194,8 @@ double mis_weight(const DScene &S, const DCtx &C, const DPoint &P, V3 s) {
     }
     denom += (double)CHK_MIS_LIGHT_SAMPLES * p;
   }
+  if (S.env.on)  // EnMpIS (mis.cpp:84-87): grey(L) / normalizationArea
+    denom += (double)CHK_MIS_LIGHT_SAMPLES * (grey(env_radiance(S.env, s)) * (1.0 / S.env.area));
   denom += (double)CHK_MIS_BSDF_SAMPLES * pdf_eval(C.m, P.n, C.wo_ref, C.wo, s);
   return 1.0 / (denom + 1e-20);
 }
@@ -1067,7 +1208,7 @@ V3 direct_estimate(const DScene &S, const chk_direct_params &p, const DPoint &P,
   // (lightSources.cpp:33)
   if (p.estimator != CHK_DIRECT_MIS && S.L.empty()) return v3(0, 0, 0);
   if (p.estimator == CHK_DIRECT_MIS && S.L.empty() && S.TL.empty() && S.ssrc.empty() &&
-      S.pts.empty())
+      S.pts.empty() && !S.env.on)
     return v3(0, 0, 0);
   DCtx C;
   direct_setup(S, P, p.estimator != CHK_DIRECT_MIS, &C);
@@ -1152,6 +1293,8 @@ V3 direct_estimate(const DScene &S, const chk_direct_params &p, const DPoint &P,
         sum = sum + S.mats[S.tri_mat[S.TL[j].first]].rad * thr;
         hit = true;
       }
+      // lightSources.cpp:102-104: nothing hit at all
+      if (!hit && S.env.on && !occluded(S, o, w, INF)) sum = sum + env_radiance(S.env, w) * thr;
     } else {
       // lightSources_CV.cpp:420-445: the first mesh light hit
       for (const DLight &L : S.L) {
@@ -1741,6 +1884,16 @@ void *orc_direct_scene_desc(const chk_scene_desc *D) {
     const chk_point_source &q = D->point_sources[j];
     S->pts.push_back(DPt{v3(q.pos[0], q.pos[1], q.pos[2]), v3(q.power[0], q.power[1], q.power[2])});
   }
+  if (D->env) {
+    DEnv &E = S->env;
+    E.on = true;
+    E.w = D->env->width;
+    E.h = D->env->height;
+    E.axis = D->env->axis;
+    E.img.assign(D->env->image, D->env->image + 3 * (size_t)E.w * E.h);
+    E.rad = v3(D->env->radiance[0], D->env->radiance[1], D->env->radiance[2]);
+    env_init(E);
+  }
   return S;
 }
 
@@ -1760,6 +1913,24 @@ void *orc_direct_scene(const float *tris, uint32_t n_tris, const uint32_t *tri_m
 }
 
 uint32_t orc_mis_count(void *s) { return mis_count(*(const DScene *)s); }
+
+double orc_env_tables(void *s, uint32_t *inv_marginal, uint32_t *inv_conditional) {
+  const DEnv &E = ((const DScene *)s)->env;
+  if (!E.on) return 0.0;
+  if (inv_marginal) std::copy(E.inv_m.begin(), E.inv_m.end(), inv_marginal);
+  if (inv_conditional) std::copy(E.inv_c.begin(), E.inv_c.end(), inv_conditional);
+  return E.area;
+}
+
+void orc_env_radiance(void *s, uint32_t k, const double *dirs, double *out) {
+  const DEnv &E = ((const DScene *)s)->env;
+  for (uint32_t i = 0; i < k; ++i) st(out + 3 * i, env_radiance(E, ld(dirs + 3 * i)));
+}
+
+void orc_env_sample(void *s, uint32_t k, const double *u, double *out) {
+  const DEnv &E = ((const DScene *)s)->env;
+  for (uint32_t i = 0; i < k; ++i) st(out + 3 * i, env_sample(E, u[2 * i], u[2 * i + 1]));
+}
 
 void orc_direct_scene_free(void *s) { delete (DScene *)s; }
 

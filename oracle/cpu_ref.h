@@ -156,6 +156,14 @@ void orc_direct_scene_free(void *scene);
 /* Size of MIS::getSamples' sample set for this scene (10 per sphere source,
  * mesh light and triangle light, 60 BSDF). */
 uint32_t orc_mis_count(void *scene);
+/* The scene's environment light (EnvSource, lightSources.h:33-109,
+ * lightSources.cpp:662-782): normalizationArea (returned; 0 without one),
+ * the 2000-entry inverse CDF of the marginal and the 3000 x 2000 inverse CDFs
+ * of the conditionals (either may be NULL); getRadiance of k unit
+ * directions; the importance sample of k pairs (e1, e2). */
+double orc_env_tables(void *scene, uint32_t *inv_marginal, uint32_t *inv_conditional);
+void orc_env_radiance(void *scene, uint32_t k, const double *dirs, double *out);
+void orc_env_sample(void *scene, uint32_t k, const double *u, double *out);
 /* MIS weights (mis.cpp:18-95,209-220) of S given samples per point
  * (samples k x S x 3, weights k x S). */
 void orc_mis_weights(void *scene, uint32_t k, const double *x, const double *n,

@@ -38,6 +38,8 @@
 #include "mis.h"
 #include "objects.h"
 #include "prepareScene.h"
+#include "ppm.h"
+#include <unistd.h>
 
 // Defined in materialTypes.o (ltc.h is a definition-bearing header).
 void M_GGX(const float theta, const float alpha, mat3 &M, mat3 &Minv, double &amplitude);
@@ -298,6 +300,72 @@ void ref_direct_setup(const double *tris, uint32_t n_tris, const uint32_t *tri_m
   const uint32_t dmat = n_mats;
   ref_direct_setup_ex(tris, n_tris, tri_mat, m.data(), n_mats + 1, lights, n_lights, NULL, 0,
                       dummy, &dmat, 1, NULL, 0, NULL, 0);
+}
+
+/* An EnvSource (lightSources.h:33-109) added to the scene of the last
+ * ref_direct_setup(_ex) call, the way prepareLightSources would
+ * (prepareScene.cpp:150-154, commented out there): EnvSource(image, w, h,
+ * axis, col, intensity) runs initSamples (lightSources.cpp:662-782), then
+ * loadLightSampler() hands it to MIS.  image: h x w x 3 doubles. */
+static EnvSource *g_env = NULL;
+void ref_env_setup(const double *image, uint32_t w, uint32_t h, uint32_t axis, const double *col,
+                   double intensity) {
+  std::vector<Vec> img((size_t)w * h);
+  for (size_t i = 0; i < img.size(); ++i) img[i] = Vec(image[3 * i], image[3 * i + 1], image[3 * i + 2]);
+  g_env = new EnvSource(&img[0], w, h, axis, Vec(col[0], col[1], col[2]), intensity);
+  lSource->addESource(*g_env);
+  loadLightSampler();
+}
+double ref_env_area() { return lSource->getEnvSourcePtr()->normalizationArea; }
+/* EnvSource::getRadiance for k unit directions (k x 3 in, k x 3 out). */
+void ref_env_radiance(uint32_t k, const double *dirs, double *out) {
+  EnvSource *e = lSource->getEnvSourcePtr();
+  for (uint32_t i = 0; i < k; ++i) {
+    Vec L = e->getRadiance(Vec(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]));
+    out[3 * i] = L.x;
+    out[3 * i + 1] = L.y;
+    out[3 * i + 2] = L.z;
+  }
+}
+/* The first n of initSamples' importance samples (n x 3). */
+void ref_env_samples(uint32_t n, double *out) {
+  EnvSource *e = lSource->getEnvSourcePtr();
+  for (uint32_t i = 0; i < n && i < EnvSource::nSamples * EnvSource::multiplier; ++i) {
+    out[3 * i] = e->impSamples[i].x;
+    out[3 * i + 1] = e->impSamples[i].y;
+    out[3 * i + 2] = e->impSamples[i].z;
+  }
+}
+
+/* readImage / writeImage (ppm.cpp:36-152) in directory `dir`, where they
+ * open their fixed file names (env1.ppm, env1_cpy.ppm).  ref_read_ppm
+ * returns readImage's status, the size in wh and up to cap pixels. */
+int ref_read_ppm(const char *dir, double *img, uint32_t cap, uint32_t *wh) {
+  char cwd[4096];
+  if (!getcwd(cwd, sizeof cwd) || chdir(dir) != 0) return -1;
+  Vec *image = NULL;
+  uint32_t w = 0, h = 0;
+  const int ok = readImage(image, w, h);
+  if (chdir(cwd) != 0) return -1;
+  if (!ok) return 0;
+  wh[0] = w;
+  wh[1] = h;
+  for (uint32_t i = 0; i < w * h && i < cap; ++i) {
+    img[3 * i] = image[i].x;
+    img[3 * i + 1] = image[i].y;
+    img[3 * i + 2] = image[i].z;
+  }
+  delete[] image;
+  return 1;
+}
+int ref_write_ppm(const char *dir, const double *img, uint32_t w, uint32_t h) {
+  char cwd[4096];
+  if (!getcwd(cwd, sizeof cwd) || chdir(dir) != 0) return -1;
+  std::vector<Vec> image((size_t)w * h);
+  for (size_t i = 0; i < image.size(); ++i) image[i] = Vec(img[3 * i], img[3 * i + 1], img[3 * i + 2]);
+  const int ok = writeImage(&image[0], w, h);
+  if (chdir(cwd) != 0) return -1;
+  return ok;
 }
 
 /* MIS::getSamples for k points (primitive = the point's input triangle):

@@ -279,6 +279,62 @@ def gen_sources():
                         pts_mis=pts_mis)
 
 
+def env_scene(axis=1):
+    """sources_room under the sky environment light."""
+    sc = scenes.sources_room()
+    sc.env = scenes.EnvMap(scenes.sky_image(), axis, (1.0, 0.9, 0.8), 2.0)
+    return sc
+
+
+def gen_env(R):
+    """EnvSource (lightSources.h:33-109, lightSources.cpp:662-782) over the
+    sky image, polar axis y, in sources_room: the reference's
+    normalizationArea, getRadiance of random directions, the (theta, phi)
+    grid indices of the first 20,000 of its importance samples, its
+    MIS::getSamples (samples and weights) and its getLightFromAllSources_MIS
+    means at points on triangles and spheres; normalizationArea with the x and
+    z axes; and readImage / writeImage (ppm.cpp:36-152) on P6 files."""
+    import tempfile
+
+    sc = env_scene(1)
+    x, n, d, prim = _points(sc, 16, 8)
+    wo = -d
+    wo_ref = unit(n * 2.0 * (n * wo).sum(1, keepdims=True) + d)  # lightSources.cpp:44-45
+    Rd = oracle.RefDirect(sc)
+    area = Rd.env_area()
+    rng = np.random.default_rng(20261205)
+    dirs = unit(rng.normal(size=(1024, 3)))
+    rad = Rd.env_radiance(dirs)
+    s = Rd.env_samples(20000)
+    ti = np.rint(np.arccos(np.clip(s[:, 2], -1, 1)) / (np.pi / 3000)).astype(np.uint16)
+    pj = (np.rint((np.arctan2(s[:, 1], s[:, 0]) % (2 * np.pi)) / (2 * np.pi / 4000)) % 4000).astype(np.uint16)
+    smp, w = Rd.mis_get_samples(x, n, wo_ref, wo, prim)
+    mis_m, mis_v = Rd.stats(2, x, n, d, prim, reps=8192, batches=64)
+    areas = {}
+    for ax in (0, 2):
+        areas[ax] = oracle.RefDirect(env_scene(ax)).env_area()
+    # P6: a file with a comment line read by readImage; writeImage's bytes
+    img = scenes.sky_image(width=7, height=5)
+    raw = np.round(img * 255).astype(np.uint8)
+    ppm_in = b"P6\n# a comment\n# another\n7 5\n255\n" + raw.tobytes()
+    out_img = np.linspace(-0.6, 1.4, 7 * 5 * 3).reshape(5, 7, 3)
+    with tempfile.TemporaryDirectory() as tmp:
+        Path(tmp, "env1.ppm").write_bytes(ppm_in)
+        buf, wh = np.zeros((64, 3)), np.zeros(2, np.uint32)
+        assert R.ref_read_ppm(tmp.encode(), buf, 64, wh) == 1
+        ppm_in_ref = buf[:int(wh[0] * wh[1])].reshape(int(wh[1]), int(wh[0]), 3)
+        assert R.ref_write_ppm(tmp.encode(), np.ascontiguousarray(out_img.reshape(-1, 3)), 7, 5) == 1
+        ppm_out_ref = Path(tmp, "env1_cpy.ppm").read_bytes()
+    np.savez_compressed(OUT / "env.npz", env_image=sc.env.image, env_axis=1,
+                        env_col=np.array(sc.env.col), env_intensity=sc.env.intensity,
+                        area=area, area_axis0=areas[0], area_axis2=areas[2], dirs=dirs, rad=rad,
+                        sample_theta_idx=ti, sample_phi_idx=pj, x=x, n=n, d=d, prim=prim,
+                        wo_ref=wo_ref, wo=wo, samples=smp, weights=w, mis_mean=mis_m,
+                        mis_var=mis_v, reps=8192, ppm_in=np.frombuffer(ppm_in, np.uint8),
+                        ppm_in_ref=ppm_in_ref, ppm_out_img=out_img,
+                        ppm_out_ref=np.frombuffer(ppm_out_ref, np.uint8))
+
+
 def gen_scene_xml(R):
     """The reference's own front end (initSceneParser, prepareScreenCamera,
     prepareObjects, prepareLightSources) on its two scene files; the files
@@ -309,7 +365,7 @@ def gen_scene_xml(R):
 
 def main():
     """python tests/golden/make_golden.py [fixture ...]: all fixtures, or the
-    named ones (bvh_closest, shadow, brdf, ltc, spheres, direct, scene_xml, sources)."""
+    named ones (bvh_closest, shadow, brdf, ltc, spheres, direct, scene_xml, sources, env)."""
     oracle.build(ref=True)
     R = oracle.ref_lib()
     if R is None:
@@ -333,6 +389,8 @@ def main():
         gen_scene_xml(R)
     if on("sources"):
         gen_sources()
+    if on("env"):
+        gen_env(R)
     for p in sorted(OUT.glob("*.npz")):
         print(p.name, p.stat().st_size)
 

@@ -61,6 +61,39 @@ def test_struct_layouts_match_header():
     assert C.sizeof(abi.chk_camera) == 4 * 11 + 8
     assert C.sizeof(abi.chk_render_params) == 32
     assert C.sizeof(abi.chk_direct_params) == 24
+    assert C.sizeof(abi.chk_sphere) == 40 and C.sizeof(abi.chk_point_source) == 48
+    assert C.sizeof(abi.chk_env_source) == 48
+
+
+LAYOUT_STRUCTS = ["chk_material", "chk_mesh_light", "chk_sphere", "chk_point_source",
+                  "chk_env_source", "chk_scene_desc", "chk_camera", "chk_render_params",
+                  "chk_direct_params", "chk_scene_info", "chk_render_stats", "chk_request",
+                  "chk_batch_tuning", "chk_batch_stats", "chk_train_params"]
+
+
+def test_ctypes_layouts_equal_the_c_compilers(tmp_path):
+    """sizeof and every field offset of the ctypes records equal what gcc
+    makes of include/*.h."""
+    import subprocess
+
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "chakum_c.h"', "int main(void) {"]
+    for name in LAYOUT_STRUCTS:
+        lines.append(f'  printf("{name} %zu\\n", sizeof({name}));')
+        for f, _ in getattr(abi, name)._fields_:
+            lines.append(f'  printf("{name}.{f} %zu\\n", offsetof({name}, {f}));')
+    lines.append("  return 0;\n}")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", str(ROOT / "include"), str(src), "-o", str(exe)],
+                   check=True)
+    got = dict(l.split() for l in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                  text=True).stdout.splitlines())
+    for name in LAYOUT_STRUCTS:
+        T = getattr(abi, name)
+        assert int(got[name]) == C.sizeof(T), name
+        for f, _ in T._fields_:
+            assert int(got[f"{name}.{f}"]) == getattr(T, f).offset, (name, f)
 
 
 @pytest.mark.parametrize("flag,val", [("CHK_RENDER_PHYSICAL_EMITTER", abi.RENDER_PHYSICAL_EMITTER),

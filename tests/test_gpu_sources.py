@@ -169,7 +169,7 @@ def test_scene_errors(ctx):
     with pytest.raises(abi.ChkError, match="4096 spheres"):
         abi.DeviceScene(ctx, bad(spheres=many, sphere_sources=np.zeros(0, np.uint32)))
     ds = abi.DeviceScene(ctx, sc)
-    with pytest.raises(abi.ChkError, match="spheres or triangle, sphere or point sources"):
+    with pytest.raises(abi.ChkError, match="spheres or triangle, sphere, point or environment sources"):
         host.render(ds, dict(spp=1))
     SL = host.SceneLights(ds)
     x, n, d, prim = _points(sc, 2, 2, 5)
