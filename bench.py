@@ -73,6 +73,9 @@ def parse():
     ap.add_argument("--no-direct", action="store_true")
     ap.add_argument("--direct-spp", type=int, default=1)
     ap.add_argument("--direct-cpu-row-step", type=int, default=2)
+    ap.add_argument("--no-sources", action="store_true")
+    ap.add_argument("--sources-spp", type=int, default=16)
+    ap.add_argument("--sources-cpu-row-step", type=int, default=1)
     ap.add_argument("--ltc-points", type=int, default=4 * 1024 * 1024)
     ap.add_argument("--ltc-quads", type=int, default=16)
     return ap.parse_args()
@@ -237,6 +240,72 @@ def direct_leg(args, ctx, sc, ds):
                                              f"({rows} rows x {args.width} px x {args.direct_spp} spp)",
                                    "seconds": round(ct, 2)}
             row["vs_cpu"] = round(row["mpaths_per_s"] / row["cpu_baseline"]["value"], 1)
+        out[name] = row
+    return out
+
+
+def sources_leg(args, ctx):
+    """Spheres and the other light sources through chk_render_shade with
+    getLightFromAllSources_MIS (shadeDirectOnly, shader.cpp:339-363): the
+    reference's own scene1.xml (sphere walls and balls, a sphere source;
+    cone sampling) at its 512 x 384, and a 1080p room holding every kind
+    (spheres, a sphere source, a triangle light, point sources, mesh lights
+    and an 8-bit sky environment map).  The oracle's FP64 render of the same
+    runs beside each on a bounded sample of rows."""
+    from chakumrender_amd import abi, host, scene_xml, scenes
+
+    import numpy as np
+
+    G = np.load(ROOT / "tests" / "golden" / "scene_xml.npz")
+    s1 = scene_xml.load_scene_xml(str(G["scene1_xml"]))
+    room = scenes.sources_room(width=args.width, height=args.height)
+    room.env = scenes.EnvMap(scenes.sky_image(), 1, (1.0, 0.9, 0.8), 2.0)
+    legs = (("scene1_mis", s1, args.sources_spp, args.sources_cpu_row_step,
+             f"scene1.xml (reference): {len(s1.spheres)} spheres incl. the 1e5-radius walls, "
+             f"1 sphere source, {s1.camera.width}x{s1.camera.height} {args.sources_spp} spp, "
+             "shadeDirectOnly + getLightFromAllSources_MIS (10 cone + 60 BSDF samples), FP64"),
+            ("all_sources_mis", room, 1, args.sources_cpu_row_step * 2,
+             f"room: 3 spheres, 1 sphere source, 1 triangle light, 2 point sources, 2 mesh "
+             f"lights, 64x32 sky map; {args.width}x{args.height} 1 spp, shadeDirectOnly + "
+             "getLightFromAllSources_MIS (110 samples + 2 point lights per hit), FP64"))
+    out = {}
+    dp = host.direct_params(abi.DIRECT_MIS)
+    for name, sc, spp, row_step, workload in legs:
+        t0 = time.perf_counter()
+        ds = abi.DeviceScene(ctx, sc)
+        setup = time.perf_counter() - t0
+        W, H = sc.camera.width, sc.camera.height
+        params = dict(spp=spp, seed_lo=0x5EED0013)
+        image = host.Image(W, H)
+        host.render_shade(ds, params, abi.SHADE_DIRECT_ONLY, dp, image=image, download=False)
+        abi.sync()
+        reps = 2
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            _, st = host.render_shade(ds, params, abi.SHADE_DIRECT_ONLY, dp, image=image,
+                                      download=False)
+        abi.sync()
+        dt = (time.perf_counter() - t0) / reps
+        paths = W * H * spp
+        row = {"workload": workload, "ms": round(dt * 1e3, 2),
+               "mpaths_per_s": round(paths / dt / 1e6, 3),
+               "shadow_rays_per_s_g": round(st["rays_shadow"] / dt / 1e9, 3),
+               "scene_setup_s": round(setup, 3)}
+        if not args.no_cpu_baseline:
+            import oracle
+
+            D = oracle.DirectScene(sc)
+            threads = min(16, os.cpu_count() or 1)
+            t1 = time.perf_counter()
+            D.render_shade(abi.SHADE_DIRECT_ONLY, params, dp, row_step=row_step, threads=threads)
+            ct = time.perf_counter() - t1
+            rows = len(range(0, H, row_step))
+            row["cpu_baseline"] = {"value": round(rows * W * spp / ct / 1e6, 4), "unit": "Mpaths/s",
+                                   "cores": threads, "kind": "port",
+                                   "sample": f"every {row_step}th row ({rows} rows x {W} px x {spp} spp)",
+                                   "seconds": round(ct, 2)}
+            row["vs_cpu"] = round(row["mpaths_per_s"] / row["cpu_baseline"]["value"], 1)
+        ds.close()
         out[name] = row
     return out
 
@@ -479,6 +548,8 @@ def main():
         line["ltc"] = ltc_leg(args, ctx)
     if not args.no_direct:
         line["direct"] = direct_leg(args, ctx, sc, ds)
+    if not args.no_sources:
+        line["sources"] = sources_leg(args, ctx)
 
     if not args.no_cpu_baseline:
         import oracle

@@ -408,15 +408,20 @@ __device__ inline void no_ray(float4 *__restrict__ ray, size_t i) {
 
 // 1 / (sum_k N_k pdfVal_k(s) + 1e-20) (mis.cpp:34-95,209-220), the
 // techniques in MIS::getSamples' order: the sphere sources' cones, the mesh
-// lights and the triangle lights (10 samples each), the BSDF (60).
+// lights and the triangle lights (10 samples each), the environment (10),
+// the BSDF (60).  EXT = false: a scene with mesh lights only (the other
+// counts are 0), compiled without their code.
+template <bool EXT>
 __device__ inline double mis_weight(const DeviceScene &S, const MatT<double> &m, d3 x, d3 n,
                                     d3 wo_ref, d3 wo, d3 s) {
   double denom = 0;
-  for (uint32_t j = 0; j < S.n_sph_src; ++j) {  // SoAnISSP (mis.cpp:35-39)
-    const Cone C = sph_cone(S, j, x);
-    denom += (double)CHK_MIS_LIGHT_SAMPLES * (dot(s, C.w) >= C.cos_max ? 1.0 / C.cap : 0.);
-  }
-  for (uint32_t j = 0; j < S.n_lights + S.n_tri_lights; ++j) {  // SuArISME, SuArISTR
+  if (EXT)
+    for (uint32_t j = 0; j < S.n_sph_src; ++j) {  // SoAnISSP (mis.cpp:35-39)
+      const Cone C = sph_cone(S, j, x);
+      denom += (double)CHK_MIS_LIGHT_SAMPLES * (dot(s, C.w) >= C.cos_max ? 1.0 / C.cap : 0.);
+    }
+  const uint32_t n_area = S.n_lights + (EXT ? S.n_tri_lights : 0u);
+  for (uint32_t j = 0; j < n_area; ++j) {  // SuArISME, SuArISTR
     uint32_t tri;
     double d;
     d3 N = dk(0, 0, 0);
@@ -428,7 +433,7 @@ __device__ inline double mis_weight(const DeviceScene &S, const MatT<double> &m,
     }
     denom += (double)CHK_MIS_LIGHT_SAMPLES * p;
   }
-  if (S.env.on)  // EnMpIS (mis.cpp:84-87): grey(L) / normalizationArea
+  if (EXT && S.env.on)  // EnMpIS (mis.cpp:84-87): grey(L) / normalizationArea
     denom += (double)CHK_MIS_LIGHT_SAMPLES *
              (grey(env_pixel_radiance(S.env, env_pixel(S.env, s.x, s.y, s.z))) * (1.0 / S.env.area));
   double f, pdf;
@@ -437,8 +442,10 @@ __device__ inline double mis_weight(const DeviceScene &S, const MatT<double> &m,
   return 1.0 / (denom + 1e-20);
 }
 
-// Item terms: the contribution is rad(tri) * ((V a - b) c) with V the
-// visibility of the item's shadow ray (1 when the estimator ignores it).
+// Item terms: the contribution is rad(code) * (V ? a : b) with V the
+// visibility of the item's shadow ray (1 when the estimator ignores it); a
+// and b are the sample's term for V = 1 and V = 0, each formed in the
+// reference's order.  A point-source item contributes rad / c * a.
 struct Items {
   double *a, *b, *c;
   uint32_t *tri;
@@ -462,7 +469,22 @@ __device__ inline Slots mis_slots(const DeviceScene &S) {
   return L;
 }
 
-template <int EST>
+// The direction of sample slot `slot` from its Philox draw r (mis.cpp:103-191).
+template <bool EXT>
+__device__ inline d3 slot_dir(const DeviceScene &S, const PointRec &R, const MatT<double> &m,
+                              const Slots &L, uint32_t slot, u4 r) {
+  if (EXT && slot < L.sph)
+    return cone_dir(sph_cone(S, slot / CHK_MIS_LIGHT_SAMPLES, R.x), ud(r.x), ud(r.y));
+  if (slot < L.tri) {
+    uint32_t tri;
+    return normalize(light_point(S, (slot - L.sph) / CHK_MIS_LIGHT_SAMPLES, r, &tri) - R.x);
+  }
+  if (EXT && slot < L.env) return env_sample(S.env, ud(r.x), ud(r.y));  // EnMpIS (mis.cpp:157-169)
+  const uint32_t q = slot - L.env;
+  return brdf_sample(m, R.n, R.wo_ref, R.wo, q < R.nb ? -1.0 : 2.0, ud(r.y), ud(r.z));
+}
+
+template <int EST, bool EXT>
 __global__ __launch_bounds__(kDBlock) void k_dl_samples(DeviceScene S, const PointRec *__restrict__ pts,
                                                         uint32_t P, uint32_t n_slots, uint32_t cvn,
                                                         uint32_t flags, uint32_t key0,
@@ -497,10 +519,11 @@ __global__ __launch_bounds__(kDBlock) void k_dl_samples(DeviceScene S, const Poi
     }
     double fa, pdf;
     brdf_pdf(m, R.n, R.wo_ref, R.wo, w, fa, pdf);
-    I.a[i] = fa;
-    I.b[i] = brdf_apx(m, R, w);
-    I.c[i] = cs * c1 / (d * d);
-    if (S.n_spheres && sph_occluded(S, o, w, d)) {
+    // (visibility * brdfActual - brdfApproximate) * cosine * cosine1 / (d * d)
+    const double fx = brdf_apx(m, R, w);
+    I.a[i] = (fa - fx) * cs * c1 / (d * d);
+    I.b[i] = (0.0 * fa - fx) * cs * c1 / (d * d);
+    if (EXT && S.n_spheres && sph_occluded(S, o, w, d)) {
       I.tri[i] = tri | kOccS;
       no_ray(I.ray, i);
     } else {
@@ -510,7 +533,7 @@ __global__ __launch_bounds__(kDBlock) void k_dl_samples(DeviceScene S, const Poi
     return;
   }
   const Slots L = mis_slots(S);
-  if (EST == CHK_DIRECT_MIS && slot >= L.mis) {
+  if (EXT && EST == CHK_DIRECT_MIS && slot >= L.mis) {
     // point source j (lightSources.cpp:47-58): radiance / (4 pi |p - x|^2)
     // * cosine * brdf, after its shadow test
     const uint32_t j = slot - L.mis;
@@ -536,21 +559,11 @@ __global__ __launch_bounds__(kDBlock) void k_dl_samples(DeviceScene S, const Poi
     }
     return;
   }
-  // the sample of this slot (mis.cpp:103-191)
   const u4 r = philox(u4{R.ca, R.cb, slot, CHK_STREAM_DIRECT + R.cd}, key0, key1);
-  d3 w;
-  if (slot < L.sph) {
-    w = cone_dir(sph_cone(S, slot / CHK_MIS_LIGHT_SAMPLES, R.x), ud(r.x), ud(r.y));
-  } else if (slot < L.tri) {
-    uint32_t tri;
-    w = normalize(light_point(S, (slot - L.sph) / CHK_MIS_LIGHT_SAMPLES, r, &tri) - R.x);
-  } else if (slot < L.env) {
-    w = env_sample(S.env, ud(r.x), ud(r.y));  // EnMpIS (mis.cpp:157-169)
-  } else {
-    const uint32_t q = slot - L.env;
-    w = brdf_sample(m, R.n, R.wo_ref, R.wo, q < R.nb ? -1.0 : 2.0, ud(r.y), ud(r.z));
-  }
-  const double wt = mis_weight(S, m, R.x, R.n, R.wo_ref, R.wo, w);
+  const d3 w = slot_dir<EXT>(S, R, m, L, slot, r);
+  // the weight is needed only for a sample above the horizon that reaches a
+  // source (the reference computes every weight in getSamples and skips the
+  // others, lightSources.cpp:62-66, lightSources_CV.cpp:416-420)
   const double cs = dot(w, R.n);
   if (cs < 0) {
     no_ray(I.ray, i);
@@ -565,16 +578,17 @@ __global__ __launch_bounds__(kDBlock) void k_dl_samples(DeviceScene S, const Poi
     // nearest source hit + 1e-5 is shadowed by it: the candidate is the
     // first source within 1e-5 of the nearest hit.
     double dmin = kInf64;
-    for (uint32_t j = 0; j < S.n_lights + S.n_tri_lights; ++j) {
+    for (uint32_t j = 0; j < S.n_lights + (EXT ? S.n_tri_lights : 0u); ++j) {
       uint32_t tri;
       double d;
       d3 N;
       if (light_isect(S, j, o, w, &tri, &d, &N) && d < dmin) dmin = d;
     }
-    for (uint32_t j = 0; j < S.n_sph_src; ++j) {
-      const double d = sph_isect(S.spheres[S.sph_src[j]], o, w);
-      if (d < dmin) dmin = d;
-    }
+    if (EXT)
+      for (uint32_t j = 0; j < S.n_sph_src; ++j) {
+        const double d = sph_isect(S.spheres[S.sph_src[j]], o, w);
+        if (d < dmin) dmin = d;
+      }
     for (uint32_t j = 0; j < S.n_lights && code == kNone; ++j) {
       uint32_t tri;
       double d;
@@ -584,14 +598,14 @@ __global__ __launch_bounds__(kDBlock) void k_dl_samples(DeviceScene S, const Poi
         pd = d;
       }
     }
-    for (uint32_t j = 0; j < S.n_sph_src && code == kNone; ++j) {
+    for (uint32_t j = 0; EXT && j < S.n_sph_src && code == kNone; ++j) {
       const double d = sph_isect(S.spheres[S.sph_src[j]], o, w);
       if (d < kInf64 && d <= dmin + 1e-5) {
         code = kTagSph | j;
         pd = d;
       }
     }
-    for (uint32_t j = S.n_lights; j < S.n_lights + S.n_tri_lights && code == kNone; ++j) {
+    for (uint32_t j = S.n_lights; EXT && j < S.n_lights + S.n_tri_lights && code == kNone; ++j) {
       uint32_t tri;
       double d;
       d3 N;
@@ -613,7 +627,7 @@ __global__ __launch_bounds__(kDBlock) void k_dl_samples(DeviceScene S, const Poi
     }
   }
   bool env = false;
-  if (EST == CHK_DIRECT_MIS && code == kNone && S.env.on) {
+  if (EXT && EST == CHK_DIRECT_MIS && code == kNone && S.env.on) {
     // lightSources.cpp:102-104: no source hit; the environment counts when
     // nothing at all is hit (shadow(rr, INF))
     code = kTagEnv | env_pixel(S.env, w.x, w.y, w.z);
@@ -624,19 +638,20 @@ __global__ __launch_bounds__(kDBlock) void k_dl_samples(DeviceScene S, const Poi
     no_ray(I.ray, i);
     return;
   }
+  const double wt = mis_weight<EXT>(S, m, R.x, R.n, R.wo_ref, R.wo, w);
   double fa, pdf;
   brdf_pdf(m, R.n, R.wo_ref, R.wo, w, fa, pdf);
   if (EST == CHK_DIRECT_MIS) {
     I.a[i] = fa * cs * wt;  // throughput
     I.b[i] = 0.0;
-    I.c[i] = 1.0;
   } else {
-    I.a[i] = fa;
-    I.b[i] = brdf_apx(m, R, w);
-    I.c[i] = cs * wt;
+    // (visibility * brdfActual - brdfApproximate) * cosine * weights[i]
+    const double fx = brdf_apx(m, R, w);
+    I.a[i] = (fa - fx) * cs * wt;
+    I.b[i] = (0.0 * fa - fx) * cs * wt;
   }
   if (EST == CHK_DIRECT_MIS || (flags & CHK_DIRECT_VISIBILITY)) {
-    if (S.n_spheres && sph_occluded(S, o, w, pd)) {
+    if (EXT && S.n_spheres && sph_occluded(S, o, w, pd)) {
       code |= kOccS;
       no_ray(I.ray, i);
     } else if (env) {
@@ -648,6 +663,58 @@ __global__ __launch_bounds__(kDBlock) void k_dl_samples(DeviceScene S, const Poi
     no_ray(I.ray, i);
   }
   I.tri[i] = code;
+}
+
+// getLightFromMeshSource_Analytic_MISCV as shipped, without the shadow test
+// it ignores (lightSources_CV.cpp:399-450), in one pass per point: the slots
+// in order, each term formed and added as the reference forms and adds it,
+// then G.  No item buffers and no shadow rays; every lane of a wave works on
+// the same slot, so the light / BSDF branches stay wave-uniform.  out as
+// k_dl_reduce.
+template <bool EXT>
+__global__ __launch_bounds__(kDBlock) void k_dl_miscv_fused(DeviceScene S,
+                                                            const PointRec *__restrict__ pts,
+                                                            uint32_t P, uint32_t n_slots,
+                                                            uint32_t key0, uint32_t key1,
+                                                            bool colour, double *__restrict__ out) {
+  const uint32_t p = blockIdx.x * kDBlock + threadIdx.x;
+  if (p >= P) return;
+  const PointRec &R = pts[p];
+  d3 v = R.pre;
+  if (R.flags & kActive) {
+    const MatT<double> m = load_mat<double>(S.mats + R.mat);
+    const d3 o = R.x + R.n * 1e-8;  // rr.o (lightSources_CV.cpp:412)
+    const Slots L = mis_slots(S);
+    d3 sum = dk(0, 0, 0);
+    for (uint32_t slot = 0; slot < n_slots; ++slot) {
+      const u4 r = philox(u4{R.ca, R.cb, slot, CHK_STREAM_DIRECT + R.cd}, key0, key1);
+      const d3 w = slot_dir<EXT>(S, R, m, L, slot, r);
+      const double cs = dot(w, R.n);
+      if (cs < 0) continue;
+      uint32_t code = kNone;
+      for (uint32_t j = 0; j < S.n_lights && code == kNone; ++j) {
+        uint32_t tri;
+        double d;
+        d3 N;
+        if (light_isect(S, j, o, w, &tri, &d, &N)) code = tri;
+      }
+      if (code == kNone) continue;
+      const double wt = mis_weight<EXT>(S, m, R.x, R.n, R.wo_ref, R.wo, w);
+      double fa, pdf;
+      brdf_pdf(m, R.n, R.wo_ref, R.wo, w, fa, pdf);
+      const double fx = brdf_apx(m, R, w);
+      sum = sum + f4d(S.light_tris[code].rad) * ((1.0 * fa - fx) * cs * wt);
+    }
+    sum = sum + R.G;
+    v = sum;
+    if (colour) {
+      const chk_material &mt = S.mats[R.mat];
+      v = dk(mt.color[0] * sum.x, mt.color[1] * sum.y, mt.color[2] * sum.z);
+    }
+  }
+  out[3 * (size_t)p] = v.x;
+  out[3 * (size_t)p + 1] = v.y;
+  out[3 * (size_t)p + 2] = v.z;
 }
 
 // Per point: the reference's sums in its order.  out: k x 3 doubles (the
@@ -673,7 +740,7 @@ __global__ __launch_bounds__(kDBlock) void k_dl_reduce(DeviceScene S, const Poin
           const uint32_t tri = I.tri[i];
           if (tri == kNone) continue;
           const double V = (occ[i] || (tri & kOccS)) ? 0.0 : 1.0;
-          s = s + f4d(S.light_tris[tri & ~kOccS].rad) * ((V * I.a[i] - I.b[i]) * I.c[i]);
+          s = s + f4d(S.light_tris[tri & ~kOccS].rad) * (V != 0.0 ? I.a[i] : I.b[i]);
         }
         sum = sum + s * (S.light_area[j] / cvn);
       }
@@ -695,7 +762,7 @@ __global__ __launch_bounds__(kDBlock) void k_dl_reduce(DeviceScene S, const Poin
           else
             sum = sum + rad * I.a[i];
         } else {
-          sum = sum + rad * ((V * I.a[i] - I.b[i]) * I.c[i]);
+          sum = sum + rad * (V != 0.0 ? I.a[i] : I.b[i]);
         }
       }
       if (EST == CHK_DIRECT_MISCV) sum = sum + R.G;
@@ -734,7 +801,7 @@ __global__ __launch_bounds__(kDBlock) void k_mis_weights(
   const d3 wr = dk(WR[3 * p], WR[3 * p + 1], WR[3 * p + 2]);
   const d3 wo = dk(WO[3 * p], WO[3 * p + 1], WO[3 * p + 2]);
   const d3 s = dk(smp[3 * i], smp[3 * i + 1], smp[3 * i + 2]);
-  wout[i] = mis_weight(S, m, x, n, wr, wo, s);
+  wout[i] = mis_weight<true>(S, m, x, n, wr, wo, s);
 }
 
 // ------------------------------------------------------------ render pieces
@@ -1257,11 +1324,23 @@ template <int EST>
 void launch_items(chk_scene_s *sc, const DWork &W, uint32_t P, uint32_t n_slots, uint32_t cvn,
                   const chk_direct_params &p, hipStream_t st) {
   const size_t I = (size_t)P * n_slots;
-  if (I) {
-    hipLaunchKernelGGL(k_dl_samples<EST>, dim3((unsigned)((I + kDBlock - 1) / kDBlock)),
-                       dim3(kDBlock), 0, st, sc->d, W.pts, P, n_slots, cvn, p.flags, p.seed_lo,
-                       p.seed_hi, W.I);
-  }
+  if (!I) return;
+  const dim3 g((unsigned)((I + kDBlock - 1) / kDBlock)), b(kDBlock);
+  if (sc->extras)
+    hipLaunchKernelGGL((k_dl_samples<EST, true>), g, b, 0, st, sc->d, W.pts, P, n_slots, cvn,
+                       p.flags, p.seed_lo, p.seed_hi, W.I);
+  else
+    hipLaunchKernelGGL((k_dl_samples<EST, false>), g, b, 0, st, sc->d, W.pts, P, n_slots, cvn,
+                       p.flags, p.seed_lo, p.seed_hi, W.I);
+}
+
+// MISCV as shipped runs fused (k_dl_miscv_fused): no per-item storage.
+bool fused(const chk_direct_params &p) {
+  return p.estimator == CHK_DIRECT_MISCV && !(p.flags & CHK_DIRECT_VISIBILITY);
+}
+// Slots that need per-item storage (0 for the fused estimator).
+uint32_t item_slots(const chk_scene_s *sc, const chk_direct_params &p) {
+  return fused(p) ? 0u : n_slots_of(sc, p);
 }
 
 bool uses_vis(const chk_direct_params &p) {
@@ -1274,6 +1353,17 @@ chk_status run_items(chk_scene_s *sc, const DWork &W, uint32_t P, const chk_dire
                      bool colour, uint64_t *shadow_rays, hipStream_t st) {
   const uint32_t n_slots = n_slots_of(sc, p);
   const uint32_t cvn = std::min(p.cv_samples, 1000u);
+  if (fused(p)) {
+    const dim3 g((P + kDBlock - 1) / kDBlock), b(kDBlock);
+    if (sc->extras)
+      hipLaunchKernelGGL(k_dl_miscv_fused<true>, g, b, 0, st, sc->d, W.pts, P, n_slots, p.seed_lo,
+                         p.seed_hi, colour, W.val);
+    else
+      hipLaunchKernelGGL(k_dl_miscv_fused<false>, g, b, 0, st, sc->d, W.pts, P, n_slots,
+                         p.seed_lo, p.seed_hi, colour, W.val);
+    CHK_HIP(hipGetLastError());
+    return CHK_OK;
+  }
   switch (p.estimator) {
     case CHK_DIRECT_ANALYTIC_CV:
       launch_items<CHK_DIRECT_ANALYTIC_CV>(sc, W, P, n_slots, cvn, p, st);
@@ -1398,7 +1488,7 @@ chk_status chk_direct_light(chk_scene_t sc, const double *x, const double *n, co
   CHK_HIP(hipSetDevice(sc->ctx->device));
   const hipStream_t st = (hipStream_t)stream;
   const chk_direct_params p = *params;
-  const uint32_t n_slots = n_slots_of(sc, p);
+  const uint32_t n_slots = item_slots(sc, p);
   const uint32_t Pc = chunk_points(n_slots, k);
   DWork W;
   cs = dwork_get(sc, Pc, n_slots, false, W);
@@ -1493,7 +1583,7 @@ chk_status chk_render_shade(chk_scene_t sc, const chk_camera *cam, const chk_ren
     C.W = cam->width;
     C.H = cam->height;
   }
-  const uint32_t n_slots = implicit ? 0u : n_slots_of(sc, dp);
+  const uint32_t n_slots = implicit ? 0u : item_slots(sc, dp);
   const uint32_t n_pix = cam->width * cam->height, spp = P->spp;
   // whole pixels per chunk
   const uint32_t cap_items = chunk_points(n_slots, 0xFFFFFFFFu);
