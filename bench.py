@@ -74,6 +74,9 @@ def parse():
     ap.add_argument("--direct-spp", type=int, default=1)
     ap.add_argument("--direct-cpu-row-step", type=int, default=2)
     ap.add_argument("--no-sources", action="store_true")
+    ap.add_argument("--no-calibration", action="store_true")
+    ap.add_argument("--calib-rays", type=int, default=20000)
+    ap.add_argument("--calib-points", type=int, default=4000)
     ap.add_argument("--sources-spp", type=int, default=16)
     ap.add_argument("--sources-cpu-row-step", type=int, default=1)
     ap.add_argument("--ltc-points", type=int, default=4 * 1024 * 1024)
@@ -307,6 +310,78 @@ def sources_leg(args, ctx):
             row["vs_cpu"] = round(row["mpaths_per_s"] / row["cpu_baseline"]["value"], 1)
         ds.close()
         out[name] = row
+    return out
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def calibration_leg(args, sc):
+    """The compiled reference (oracle/_ref, built from its own sources)
+    against the FP64 restatement on identical inputs, one core each:
+    BvhAccel::intersectT over the reference's midpoint BVH (the restatement
+    without t-pruning, as the reference) on camera rays of the main scene,
+    and getLightFromMeshSource_Analytic at points on it.  The ratio says how
+    the port's CPU baseline compares with the reference's own code."""
+    import numpy as np
+
+    import oracle
+    from chakumrender_amd import abi, host, scenes
+
+    R = oracle.ref_lib()
+    if R is None:
+        return {"skipped": "oracle/_ref/libchkref.so not built"}
+    L = oracle.lib()
+    L.orc_set_threads(1)
+    out = {"cores": 1, "cpu_model": cpu_model()}
+    tris = np.ascontiguousarray(sc.tris, np.float64)
+    pos, d0, cx, cy = sc.camera.basis()
+    rng = np.random.default_rng(11)
+    m = args.calib_rays
+    u = rng.random((m, 2))
+    dirs = cx * (u[:, :1] - 0.5) + cy * (u[:, 1:] - 0.5) + d0
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    rays = np.ascontiguousarray(np.c_[np.repeat(pos[None], m, 0), dirs])
+    ids_r, t_r, n_r = np.empty(m, np.int32), np.empty(m), np.empty((m, 3))
+    t0 = time.perf_counter()
+    R.ref_bvh_closest(tris, tris.shape[0], rays, 0, ids_r, t_r, n_r)  # the build alone
+    t_build = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    R.ref_bvh_closest(tris, tris.shape[0], rays, m, ids_r, t_r, n_r)
+    t_ref = time.perf_counter() - t0 - t_build
+    bvh = oracle.RefBvh(tris)
+    t0 = time.perf_counter()
+    ids_p = bvh.closest(rays, prune=False)["id"]
+    t_port = time.perf_counter() - t0
+    out["intersectT"] = {"rays": m, "triangles": int(tris.shape[0]),
+                         "reference_us_per_ray": round(t_ref / m * 1e6, 3),
+                         "port_us_per_ray": round(t_port / m * 1e6, 3),
+                         "reference_over_port": round(t_ref / t_port, 3),
+                         "ids_equal": float((ids_r == ids_p).mean())}
+    x, n, d, prim = scenes.shading_points(sc, args.calib_points, seed=12,
+                                          eye=tuple(float(v) for v in pos))
+    Rd = oracle.RefDirect(sc)
+    t0 = time.perf_counter()
+    ref_v, _ = Rd.stats(0, x, n, d, prim, reps=1, batches=1)
+    t_ref = time.perf_counter() - t0
+    D = oracle.DirectScene(sc)
+    t0 = time.perf_counter()
+    port_v = D.direct_light(host.direct_params(abi.DIRECT_ANALYTIC), x, n, d, prim, threads=1)
+    t_port = time.perf_counter() - t0
+    out["getLightFromMeshSource_Analytic"] = {
+        "points": int(len(prim)), "mesh_lights": int(sc.lights.shape[0]),
+        "reference_us_per_point": round(t_ref / len(prim) * 1e6, 3),
+        "port_us_per_point": round(t_port / len(prim) * 1e6, 3),
+        "reference_over_port": round(t_ref / t_port, 3),
+        "max_rel_diff": float(np.max(np.abs(ref_v - port_v) / np.maximum(np.abs(ref_v), 1e-12)))}
+    L.orc_set_threads(min(16, os.cpu_count() or 1))
     return out
 
 
@@ -550,6 +625,8 @@ def main():
         line["direct"] = direct_leg(args, ctx, sc, ds)
     if not args.no_sources:
         line["sources"] = sources_leg(args, ctx)
+    if not args.no_cpu_baseline and not args.no_calibration:
+        line["cpu_calibration"] = calibration_leg(args, sc)
 
     if not args.no_cpu_baseline:
         import oracle
@@ -568,6 +645,7 @@ def main():
             "kind": "port",
             "sample": f"FP64 oracle, every {args.cpu_row_step}th row ({n_rows} rows x {args.width}) "
                       f"at {args.cpu_spp} spp = {cpaths} paths, {crays} rays, {dt:.1f} s",
+            "cpu_model": cpu_model(),
         }
         line["speedup_vs_cpu"] = round(value / line["cpu_baseline"]["value"], 2)
     print(json.dumps(line), flush=True)

@@ -78,6 +78,7 @@ def lib():
         L.orc_render.argtypes = [_fp, C.c_uint32, _up, _vp, C.c_uint32, _vp, C.c_uint32, _vp, _vp,
                                  C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, _dp, _vp,
                                  C.POINTER(C.c_uint64)]
+        L.orc_set_threads.argtypes = [C.c_int]
         L.orc_direct_scene.restype = _vp
         L.orc_direct_scene.argtypes = [_fp, C.c_uint32, _up, _vp, C.c_uint32, _vp, C.c_uint32]
         L.orc_direct_scene_free.argtypes = [_vp]

@@ -1678,6 +1678,14 @@ void orc_ltc_brdf_cos(uint32_t k, const double *n, const double *wo, const doubl
   }
 }
 
+void orc_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
 void orc_philox4x32_10(const uint32_t *ctr, const uint32_t *key, uint32_t *out) {
   philox(ctr, key, out);
 }

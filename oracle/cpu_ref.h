@@ -113,6 +113,9 @@ void orc_ltc_shade(uint32_t k, const double *x, const double *n, const double *v
 void orc_ltc_brdf_cos(uint32_t k, const double *n, const double *wo, const double *wi,
                       const double *alpha, double *val);
 
+/* Threads of the OpenMP loops that take no thread count (n > 0). */
+void orc_set_threads(int n);
+
 /* ---- RNG ----------------------------------------------------------------- */
 /* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11). */
 void orc_philox4x32_10(const uint32_t *ctr, const uint32_t *key, uint32_t *out);
