@@ -565,6 +565,16 @@ chk_status chk_scene_create_desc(chk_ctx_t ctx, const chk_scene_desc *desc, chk_
   sc->d.n_leaf_tris = m;
   sc->d.n_tris = n_tris;
   sc->n_mats = n_mats;
+  for (uint32_t i = 0; i < n_mats; ++i) {
+    const uint32_t b = mats[i].brdf;
+    const bool ok = b == CHK_BRDF_CLASSIC_PHONG || b == CHK_BRDF_GGX || b == CHK_BRDF_BECKMANN ||
+                    b == CHK_BRDF_PHONG;
+    if (mats[i].light == CHK_LIGHT_NONE && !ok) {
+      sc->bad_sampler_mat = (int)i;
+      sc->bad_sampler_brdf = b;
+      break;
+    }
+  }
   {
     float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (const TriRec &t : bvh.tris)

@@ -1404,25 +1404,13 @@ chk_status run_items(chk_scene_s *sc, const DWork &W, uint32_t P, const chk_dire
   return CHK_OK;
 }
 
-// Whether every non-emitter material has a BSDF sampler.
+// Whether every non-emitter material has a BSDF sampler (found at scene
+// creation).
 bool materials_ok(const chk_scene_s *sc, std::string &why) {
-  std::vector<chk_material> m(sc->n_mats);
-  if (m.empty()) return true;
-  if (hipMemcpy(m.data(), sc->d.mats, m.size() * sizeof(chk_material), hipMemcpyDeviceToHost) !=
-      hipSuccess) {
-    why = "cannot read the materials";
-    return false;
-  }
-  for (size_t i = 0; i < m.size(); ++i) {
-    const uint32_t b = m[i].brdf;
-    const bool ok = b == CHK_BRDF_CLASSIC_PHONG || b == CHK_BRDF_GGX || b == CHK_BRDF_BECKMANN ||
-                    b == CHK_BRDF_PHONG;
-    if (m[i].light == CHK_LIGHT_NONE && !ok) {
-      why = "material " + std::to_string(i) + " has no BSDF sampler (brdf " + std::to_string(b) + ")";
-      return false;
-    }
-  }
-  return true;
+  if (sc->bad_sampler_mat < 0) return true;
+  why = "material " + std::to_string(sc->bad_sampler_mat) + " has no BSDF sampler (brdf " +
+        std::to_string(sc->bad_sampler_brdf) + ")";
+  return false;
 }
 
 }  // namespace

@@ -134,6 +134,9 @@ struct chk_scene_s {
   // entry points and chk_render_shade handle them
   bool extras = false;
   void *env_mem = nullptr;  // environment image and tables
+  // the first non-emitter material without a BSDF sampler (-1: none)
+  int bad_sampler_mat = -1;
+  uint32_t bad_sampler_brdf = 0;
 };
 
 namespace chk {
