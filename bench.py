@@ -60,6 +60,7 @@ def parse():
     ap.add_argument("--rmse-pass-spp", type=int, default=32)
     ap.add_argument("--rmse-max-spp", type=int, default=8192)
     ap.add_argument("--rmse-target", type=float, default=1e-2)
+    ap.add_argument("--rmse-check-spp", type=int, default=64)
     ap.add_argument("--train-spp", type=int, default=32)
     ap.add_argument("--train-batch", type=int, default=16384)
     ap.add_argument("--train-steps", type=int, default=64, help="Adam steps per bounce")
@@ -464,6 +465,31 @@ def rmse_leg(args, ctx):
     ref_s = time.perf_counter() - t0
     print(f"# rmse leg: {sc.n_tris} tris, {nl} lights, {W}x{H}, reference {args.rmse_ref_spp} spp "
           f"in {ref_s:.1f} s, mean {ref.mean():.4f}", file=sys.stderr, flush=True)
+    ref_check = None
+    if not args.no_cpu_baseline:
+        # the reference image against the FP64 oracle's own render of a
+        # 64-row band with independent keys: per-pixel z from the oracle's
+        # sample variance (the GPU reference's is 1 / 1024 of it at most)
+        import numpy as np
+
+        import oracle
+
+        r0 = max(0, H // 2 - 32)
+        r1 = min(H, r0 + 64)
+        n = args.rmse_check_spp
+        t1 = time.perf_counter()
+        acc, sq, _, _ = oracle.render(sc, dict(base, spp=n, seed=0x5EED0F77), rows=(r0, r1),
+                                      threads=min(16, os.cpu_count() or 1), sumsq=True)
+        m = acc[r0:r1] / n
+        var = np.maximum(sq[r0:r1] / n - m * m, 0) / n
+        z = np.abs(ref[r0:r1] - m) / np.sqrt(var + 1e-30)
+        band = ref[r0:r1]
+        ref_check = {"rows": [r0, r1], "cpu_spp": n, "within_3sigma": round(float((z <= 3).mean()), 4),
+                     "band_mean_gpu_ref": round(float(band.mean()), 6),
+                     "band_mean_cpu": round(float(m.mean()), 6),
+                     "band_mean_z": round(float(abs(band.mean() - m.mean()) /
+                                                np.sqrt(var.sum() / var.size ** 2)), 2),
+                     "cpu_seconds": round(time.perf_counter() - t1, 2)}
 
     def plain_pass(off, img):
         host.render(ds, dict(base, spp=args.rmse_pass_spp, sample_offset=off, seed=0x5EED0005),
@@ -501,6 +527,7 @@ def rmse_leg(args, ctx):
         "learned_mpaths_per_s": round(W * H * cl[-1][1] / cl[-1][0] / 1e6, 2),
         "train": dict(train, pass_spp=args.rmse_pass_spp),
         "reference_seconds": round(ref_s, 2),
+        "reference_check": ref_check,
     }
     prop.close()
     ds.close()
