@@ -203,3 +203,23 @@ def test_sphere_closest_replaces_triangle_only_when_nearer(ctx):
                                     abi.SHADE_DIRECT_ONLY, host.direct_params(abi.DIRECT_MIS))
         np.testing.assert_allclose(a[3:5, 3:5].reshape(-1, 3), np.broadcast_to(want, (4, 3)),
                                    rtol=1e-7)
+
+
+def test_scene_without_triangles(ctx):
+    """Spheres, a sphere source and point sources but no triangle at all (an
+    empty BVH): the shaders render what the oracle renders."""
+    sc0 = scenes.sources_room(width=24, height=24)
+    sc = scenes.Scene(np.zeros((0, 9), np.float32), np.zeros(0, np.uint32), sc0.mats,
+                      np.zeros(0, scenes.LIGHT_DTYPE), sc0.camera, spheres=sc0.spheres,
+                      sphere_sources=sc0.sphere_sources, point_sources=sc0.point_sources)
+    ds = abi.DeviceScene(ctx, sc)
+    assert ds.info()["n_tris"] == 0
+    dp = host.direct_params(abi.DIRECT_MIS)
+    P = dict(spp=4, seed_lo=0x5EED0014)
+    for shader in (abi.SHADE_DIRECT_ONLY, abi.SHADE_EXPLICIT, abi.SHADE_IMPLICIT):
+        a, _, _ = host.render_shade(ds, P, shader, dp)
+        oa, _ = oracle.DirectScene(sc).render_shade(shader, P, dp)
+        close = (np.abs(a - oa) <= 1e-5 * np.maximum(np.abs(oa), 1e-3)).all(2)
+        print(f"no triangles, shader {shader}: pixels equal {close.mean():.3f}, mean {a.mean():.5f}")
+        assert close.mean() >= 0.95
+    assert a.mean() >= 0 and np.isfinite(a).all()
