@@ -148,7 +148,7 @@ class chk_render_stats(C.Structure):
                 ("shadow_node_visits", C.c_uint64), ("shadow_tri_tests", C.c_uint64),
                 ("chunks", C.c_uint32), ("extend_launches", C.c_uint32), ("extend_ms", C.c_double),
                 ("shadow_ms", C.c_double), ("shade_ms", C.c_double), ("other_ms", C.c_double),
-                ("proposal_ms", C.c_double), ("train_steps", C.c_uint32), ("reserved", C.c_uint32)]
+                ("proposal_ms", C.c_double), ("train_steps", C.c_uint32), ("stack_retries", C.c_uint32)]
 
 
 class chk_direct_params(C.Structure):
@@ -193,6 +193,7 @@ SIGNATURES = {
     "chk_scene_create": (C.c_int, [_vp, _vp, _u32, _vp, _vp, _u32, _vp, _u32, C.POINTER(_vp)]),
     "chk_scene_create_desc": (C.c_int, [_vp, C.POINTER(chk_scene_desc), C.POINTER(_vp)]),
     "chk_scene_destroy": (C.c_int, [_vp]),
+    "chk_scene_set_trace_stack": (C.c_int, [_vp, _u32]),
     "chk_scene_get_info": (C.c_int, [_vp, C.POINTER(chk_scene_info)]),
     "chk_trace_closest": (C.c_int, [_vp, _vp, _u32, _vp, _vp]),
     "chk_trace_any": (C.c_int, [_vp, _vp, _u32, _vp, _vp]),
@@ -386,6 +387,10 @@ class DeviceScene:
         check(self.L.chk_scene_create_desc(ctx.h, C.byref(d), C.byref(h)), "chk_scene_create_desc")
         del keep
         self.h = h
+
+    def set_trace_stack(self, entries: int):
+        """chk_scene_set_trace_stack: 8 or 32 LDS stack entries per lane."""
+        check(self.L.chk_scene_set_trace_stack(self.h, int(entries)), "chk_scene_set_trace_stack")
 
     def info(self) -> dict:
         i = chk_scene_info()

@@ -618,6 +618,15 @@ chk_status chk_scene_destroy(chk_scene_t sc) {
   return CHK_OK;
 }
 
+chk_status chk_scene_set_trace_stack(chk_scene_t sc, uint32_t entries) {
+  if (!sc || (entries != 8 && entries != 32)) {
+    set_error("chk_scene_set_trace_stack: null scene, or entries not 8 or 32");
+    return CHK_EINVAL;
+  }
+  sc->shallow_stack = (int)entries;
+  return CHK_OK;
+}
+
 chk_status chk_scene_get_info(chk_scene_t sc, chk_scene_info *info) {
   if (!sc || !info) {
     set_error("chk_scene_get_info: null argument");

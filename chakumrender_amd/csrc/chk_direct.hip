@@ -1255,6 +1255,8 @@ struct DWork {
   float4 *chits;
   double *cdirs;   // P x 3
   PathSt *paths;   // render with a path shader: P paths
+  uint32_t *ovf;   // the traversal's overflow list (TraceShape::ovf)
+  uint32_t ovf_cap;
 };
 size_t dwork_bytes(uint32_t P, uint32_t n_slots, bool render) {
   const size_t I = (size_t)P * n_slots;
@@ -1264,6 +1266,7 @@ size_t dwork_bytes(uint32_t P, uint32_t n_slots, bool render) {
   if (render)
     b += al256((size_t)P * 32) + al256((size_t)P * 16) + al256((size_t)P * 24) +
          al256((size_t)P * sizeof(PathSt));
+  b += al256(ovf_words((uint32_t)std::max<size_t>(I, P)) * 4);
   return b;
 }
 chk_status dwork_get(chk_scene_s *sc, uint32_t P, uint32_t n_slots, bool render, DWork &W) {
@@ -1311,7 +1314,10 @@ chk_status dwork_get(chk_scene_s *sc, uint32_t P, uint32_t n_slots, bool render,
     W.cdirs = (double *)q;
     q += al256((size_t)P * 24);
     W.paths = (PathSt *)q;
+    q += al256((size_t)P * sizeof(PathSt));
   }
+  W.ovf_cap = (uint32_t)std::max<size_t>(I, P);
+  W.ovf = (uint32_t *)q;
   return CHK_OK;
 }
 
@@ -1376,7 +1382,7 @@ chk_status run_items(chk_scene_s *sc, const DWork &W, uint32_t P, const chk_dire
   const size_t I = (size_t)P * n_slots;
   const bool vis = uses_vis(p) && I > 0;
   if (vis) {
-    const TraceShape shape{sc->stack_size, 128};
+    const TraceShape shape{sc->stack_size, 128, W.ovf, W.ovf_cap, sc->shallow_stack};
     CHK_HIP(launch_any(sc, shape, W.I.ray, linear_range((uint32_t)I), W.occ, nullptr, nullptr,
                        true, false, nullptr, SlotScenes{}, st));
     if (shadow_rays) *shadow_rays += I;
@@ -1587,7 +1593,7 @@ chk_status chk_render_shade(chk_scene_t sc, const chk_camera *cam, const chk_ren
   if (cs != CHK_OK) return cs;
   const bool need_G = !implicit && dp.estimator != CHK_DIRECT_MIS;
   const bool physical = (P->flags & CHK_RENDER_PHYSICAL_EMITTER) != 0;
-  const TraceShape shape{sc->stack_size, 128};
+  const TraceShape shape{sc->stack_size, 128, W.ovf, W.ovf_cap, sc->shallow_stack};
   const double *ltc = (const double *)sc->ctx->ltc_dev;
   uint64_t shadow = 0, extend = 0;
   uint32_t chunks = 0, launches = 0;

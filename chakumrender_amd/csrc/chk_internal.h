@@ -135,6 +135,7 @@ struct chk_scene_s {
   bool extras = false;
   void *env_mem = nullptr;  // environment image and tables
   // the first non-emitter material without a BSDF sampler (-1: none)
+  int shallow_stack = 32;  // chk_scene_set_trace_stack (8 or 32)
   int bad_sampler_mat = -1;
   uint32_t bad_sampler_brdf = 0;
 };
@@ -180,7 +181,16 @@ inline SegRange linear_range(uint32_t n) { return SegRange{nullptr, n, seg_capac
 struct TraceShape {
   int stack;
   int block;
+  // Optional overflow list (ovf[0] = count, ovf[64 ...] = ray indices, room
+  // for ovf_cap rays, at least the launch's ray slots).  With it, a tree
+  // whose worst-case stack exceeds 32 entries is traced with a 32-entry stack
+  // and the rays that overflow it are traced again with the full stack.
+  uint32_t *ovf = nullptr;
+  uint32_t ovf_cap = 0;
+  int shallow = 32;  // the shorter stack (8 only for single-scene 128-thread launches)
 };
+// Words of an overflow list for n ray slots.
+inline size_t ovf_words(uint32_t n) { return 64 + (size_t)n; }
 // Per-ray scenes of a batched launch: ray i belongs to scene[slot_seg[i]]
 // (scene == nullptr: every ray uses the launch's scene).
 struct SlotScenes {

@@ -83,6 +83,7 @@ struct Buffers {
   float4 *s_ray;
   float4 *s_L;
   float4 *path_L;
+  uint32_t *ovf;  // the traversal's overflow list (TraceShape::ovf), qcap rays
 };
 
 // qcap: queue slots (kSegs * segcap); cap: paths per chunk
@@ -91,6 +92,7 @@ size_t buffers_bytes(uint32_t qcap, uint32_t cap) {
   b += 2 * (size_t)qcap * (32 + 16 + 4);
   b += (size_t)qcap * (16 + 32 + 16);
   b += (size_t)cap * 16;
+  b += ovf_words(qcap) * 4;
   return b + 16 * 256;
 }
 
@@ -113,6 +115,7 @@ Buffers carve(void *base, uint32_t qcap, uint32_t cap) {
   B.s_ray = (float4 *)take((size_t)qcap * 32);
   B.s_L = (float4 *)take((size_t)qcap * 16);
   B.path_L = (float4 *)take((size_t)cap * 16);
+  B.ovf = (uint32_t *)take(ovf_words(qcap) * 4);
   return B;
 }
 
@@ -736,6 +739,7 @@ chk_status render_impl(chk_scene_s *sc, const chk_camera *cam, const chk_render_
     sc->work_bytes = need;
   }
   Buffers B = carve(sc->work, qcap, cap);
+  CHK_HIP(launch_zero_u32(B.ovf, 2, st));  // overflow count, retries
   LBuffers LB{};
   if (learned) {
     const size_t lneed = lbuffers_bytes(qcap);
@@ -783,7 +787,7 @@ chk_status render_impl(chk_scene_s *sc, const chk_camera *cam, const chk_render_
       const uint32_t in = depth & 1u, out = in ^ 1u;
       hipLaunchKernelGGL(k_reset_bounce, dim3(1), dim3(64), 0, st, B.ctrl, qbase(out));
       prof.begin(0);
-      hipError_t e = launch_closest(sc, TraceShape{sc->stack_size, 128}, B.q_ray[in],
+      hipError_t e = launch_closest(sc, TraceShape{sc->stack_size, 128, B.ovf, qcap, sc->shallow_stack}, B.q_ray[in],
                                     SegRange{B.ctrl + qbase(in), 0u, segcap}, B.hits, true, count,
                                     B.stats + kNodeVisits, SlotScenes{}, st);
       prof.end();
@@ -843,7 +847,7 @@ chk_status render_impl(chk_scene_s *sc, const chk_camera *cam, const chk_render_
       }
       if (depth < P->max_depth) {
         prof.begin(2);
-        e = launch_any(sc, TraceShape{sc->stack_size, 128}, B.s_ray,
+        e = launch_any(sc, TraceShape{sc->stack_size, 128, B.ovf, qcap, sc->shallow_stack}, B.s_ray,
                        SegRange{B.ctrl + kQS, 0u, segcap}, training ? LB.s_occ : nullptr, B.s_L,
                        B.path_L, true, count, B.stats + kShNodeVisits, SlotScenes{}, st);
         prof.end();
@@ -863,7 +867,10 @@ chk_status render_impl(chk_scene_s *sc, const chk_camera *cam, const chk_render_
     std::memset(stats, 0, sizeof(*stats));
     unsigned long long h[kStatWords];
     CHK_HIP(hipMemcpyAsync(h, B.stats, sizeof(h), hipMemcpyDeviceToHost, st));
+    uint32_t retries = 0;
+    CHK_HIP(hipMemcpyAsync(&retries, B.ovf + 1, 4, hipMemcpyDeviceToHost, st));
     CHK_HIP(hipStreamSynchronize(st));
+    stats->stack_retries = retries;
     stats->paths = n_pix_total * P->spp;
     stats->rays_extend = h[kRaysExt];
     stats->rays_shadow = h[kRaysShadow];
@@ -1086,7 +1093,7 @@ chk_status batch_impl(chk_ctx_s *ctx, const chk_request *R, uint32_t n,
                     hipMemcpyHostToDevice));
   CHK_HIP(hipMemcpy(d_scene, seg_scene.data(), segs.size() * sizeof(DeviceScene),
                     hipMemcpyHostToDevice));
-  const TraceShape shape{stack, (int)T.trace_block};
+  const TraceShape shape{stack, (int)T.trace_block, B.ovf, qcap};
   auto qbase = [](uint32_t q) { return q == 0 ? (uint32_t)kQ0 : (uint32_t)kQ1; };
   // ---- the launches of one chunk
   auto emit_chunk = [&](uint32_t c, hipStream_t s) -> hipError_t {

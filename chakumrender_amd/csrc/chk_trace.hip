@@ -132,6 +132,52 @@ __device__ inline float refine_t(const float4 *tp, f3 o, f3 d) {
   return (float)(num / den);
 }
 
+// One ray through the launch's scene or, in a batched launch, its own.
+template <bool ANY, bool COUNT, int STACK, int BS, bool MULTI>
+__device__ inline void trace_ray(const DeviceScene &S0, const SlotScenes &M, uint32_t *stk, size_t i,
+                                 float4 ot, float4 ds, int leaf_ids, float &t, uint32_t &id,
+                                 float &u, float &v, bool &any, uint32_t &leaf, uint32_t &nv,
+                                 uint32_t &nt, uint32_t &overflow) {
+  if (MULTI) {
+    const DeviceScene *ps = M.scene + M.slot_seg[i];
+    DeviceScene S;
+    S.nodes = ps->nodes;
+    S.tris = ps->tris;
+    traverse<ANY, COUNT, STACK, BS>(S, stk, xyz(ot), xyz(ds), ot.w, __float_as_uint(ds.w),
+                                    leaf_ids != 0, t, id, u, v, any, leaf, nv, nt, overflow);
+  } else {
+    traverse<ANY, COUNT, STACK, BS>(S0, stk, xyz(ot), xyz(ds), ot.w, __float_as_uint(ds.w),
+                                    leaf_ids != 0, t, id, u, v, any, leaf, nv, nt, overflow);
+  }
+}
+
+// A traced ray's outputs: the closest hit (t in FP64 accuracy for the
+// caller-facing query), or the occlusion flag and, for an unoccluded shadow
+// ray, its contribution added to its path.
+template <bool ANY, bool MULTI>
+__device__ inline void finish_ray(const DeviceScene &S, size_t i, float4 ot, float4 ds, float t,
+                                  uint32_t id, float u, float v, bool any, uint32_t leaf,
+                                  float4 *__restrict__ hits, uint8_t *__restrict__ occluded,
+                                  const float4 *__restrict__ shadow_L,
+                                  float4 *__restrict__ path_L, int leaf_ids) {
+  if (!MULTI && !ANY && !leaf_ids && id != 0xFFFFFFFFu)
+    t = refine_t(S.tris + 3 * (size_t)leaf, xyz(ot), xyz(ds));
+  if (ANY) {
+    if (occluded) occluded[i] = any ? 1 : 0;
+    if (path_L && !any) {
+      const float4 c = shadow_L[i];
+      const uint32_t pid = __float_as_uint(c.w);
+      float4 L = path_L[pid];
+      L.x += c.x;
+      L.y += c.y;
+      L.z += c.z;
+      path_L[pid] = L;
+    }
+  } else {
+    hits[i] = make_float4(t, __uint_as_float(id), u, v);
+  }
+}
+
 template <bool ANY, bool COUNT, int STACK, int BS, bool MULTI>
 __global__ __launch_bounds__(BS) void k_trace(DeviceScene S0, const float4 *__restrict__ rays,
                                               SegRange W, float4 *__restrict__ hits,
@@ -139,7 +185,8 @@ __global__ __launch_bounds__(BS) void k_trace(DeviceScene S0, const float4 *__re
                                               const float4 *__restrict__ shadow_L,
                                               float4 *__restrict__ path_L, int leaf_ids,
                                               unsigned long long *__restrict__ counters,
-                                              SlotScenes M) {
+                                              SlotScenes M, uint32_t *__restrict__ ovf,
+                                              uint32_t ovf_cap) {
   __shared__ uint32_t stack_mem[STACK * BS];
   uint32_t *stk = stack_mem + threadIdx.x;
   const uint32_t lane = threadIdx.x & 63u;
@@ -161,36 +208,22 @@ __global__ __launch_bounds__(BS) void k_trace(DeviceScene S0, const float4 *__re
     float t, u, v;
     uint32_t id, leaf;
     bool any;
-    if (MULTI) {
-      // the ray's own scene (a batched launch mixes requests)
-      const DeviceScene *ps = M.scene + M.slot_seg[i];
-      DeviceScene S;
-      S.nodes = ps->nodes;
-      S.tris = ps->tris;
-      traverse<ANY, COUNT, STACK, BS>(S, stk, xyz(ot), xyz(ds), ot.w, __float_as_uint(ds.w),
-                                      leaf_ids != 0, t, id, u, v, any, leaf, nv, nt, overflow);
-    } else {
-      traverse<ANY, COUNT, STACK, BS>(S0, stk, xyz(ot), xyz(ds), ot.w, __float_as_uint(ds.w),
-                                      leaf_ids != 0, t, id, u, v, any, leaf, nv, nt, overflow);
-    }
-    const DeviceScene &S = S0;
-  // the caller-facing query reports t in FP64 accuracy
-    if (!MULTI && !ANY && !leaf_ids && id != 0xFFFFFFFFu)
-      t = refine_t(S.tris + 3 * (size_t)leaf, xyz(ot), xyz(ds));
-    if (ANY) {
-      if (occluded) occluded[i] = any ? 1 : 0;
-      if (path_L && !any) {
-        const float4 c = shadow_L[i];
-        const uint32_t pid = __float_as_uint(c.w);
-        float4 L = path_L[pid];
-        L.x += c.x;
-        L.y += c.y;
-        L.z += c.z;
-        path_L[pid] = L;
+    trace_ray<ANY, COUNT, STACK, BS, MULTI>(S0, M, stk, i, ot, ds, leaf_ids, t, id, u, v, any, leaf,
+                                            nv, nt, overflow);
+    bool later = false;
+    if (overflow && ovf) {
+      // the stack was too small for this ray: listed, traced again with the
+      // full stack by k_trace_retry
+      const uint32_t k = atomicAdd(ovf, 1u);
+      if (k < ovf_cap) {
+        ovf[64 + k] = (uint32_t)i;
+        overflow = 0;
+        later = true;
       }
-    } else {
-      hits[i] = make_float4(t, __uint_as_float(id), u, v);
     }
+    if (!later)
+      finish_ray<ANY, MULTI>(S0, i, ot, ds, t, id, u, v, any, leaf, hits, occluded, shadow_L,
+                             path_L, leaf_ids);
   }
   // wave-uniform condition: every lane of the wave takes part in the shuffles
   if (COUNT || __any(overflow != 0)) {
@@ -207,6 +240,37 @@ __global__ __launch_bounds__(BS) void k_trace(DeviceScene S0, const float4 *__re
       }
       if (c) atomicAdd(&counters[2], c);
     }
+  }
+}
+
+// The rays k_trace listed as overflowing its stack, traced with a stack of
+// STACK entries (the tree's worst case); results written as k_trace writes
+// them.  A grid-stride loop over the list, whose length only the device
+// knows.
+template <bool ANY, int STACK, int BS, bool MULTI>
+__global__ __launch_bounds__(BS) void k_trace_retry(DeviceScene S0, const float4 *__restrict__ rays,
+                                                    float4 *__restrict__ hits,
+                                                    uint8_t *__restrict__ occluded,
+                                                    const float4 *__restrict__ shadow_L,
+                                                    float4 *__restrict__ path_L, int leaf_ids,
+                                                    unsigned long long *__restrict__ counters,
+                                                    SlotScenes M, const uint32_t *__restrict__ ovf,
+                                                    uint32_t ovf_cap) {
+  __shared__ uint32_t stack_mem[STACK * BS];
+  uint32_t *stk = stack_mem + threadIdx.x;
+  const uint32_t n = ovf[0] < ovf_cap ? ovf[0] : ovf_cap;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && n) atomicAdd(const_cast<uint32_t *>(ovf) + 1, n);
+  for (uint32_t k = blockIdx.x * BS + threadIdx.x; k < n; k += gridDim.x * BS) {
+    const size_t i = ovf[64 + k];
+    const float4 ot = rays[2 * i], ds = rays[2 * i + 1];
+    float t, u, v;
+    uint32_t id, leaf, nv = 0, nt = 0, overflow = 0;
+    bool any;
+    trace_ray<ANY, false, STACK, BS, MULTI>(S0, M, stk, i, ot, ds, leaf_ids, t, id, u, v, any,
+                                            leaf, nv, nt, overflow);
+    finish_ray<ANY, MULTI>(S0, i, ot, ds, t, id, u, v, any, leaf, hits, occluded, shadow_L, path_L,
+                           leaf_ids);
+    if (overflow && counters) atomicAdd(&counters[2], 1ull);
   }
 }
 
@@ -253,30 +317,50 @@ __global__ void k_zero_u32(uint32_t *p, uint32_t n) {
 template <bool ANY, bool COUNT, int STACK, int BS, bool MULTI>
 hipError_t launch_t(const chk_scene_s *s, const float4 *rays, const SegRange &W, float4 *hits,
                     uint8_t *occ, const float4 *sL, float4 *pL, bool leaf_ids,
-                    unsigned long long *counters, const SlotScenes &M, hipStream_t st) {
+                    unsigned long long *counters, const SlotScenes &M, hipStream_t st,
+                    uint32_t *ovf, uint32_t ovf_cap) {
   const uint32_t grid = kSegs * (W.segcap / BS);
   if (grid == 0) return hipSuccess;
   hipLaunchKernelGGL((k_trace<ANY, COUNT, STACK, BS, MULTI>), dim3(grid), dim3(BS), 0, st,
                      s ? s->d : DeviceScene{}, rays, W, hits, occ, sL, pL, leaf_ids ? 1 : 0,
-                     counters, M);
+                     counters, M, ovf, ovf_cap);
   return hipGetLastError();
 }
 
 template <bool ANY, bool COUNT, bool MULTI, int BS>
 hipError_t launch_s(const chk_scene_s *s, int stack, const float4 *rays, const SegRange &W,
                     float4 *hits, uint8_t *occ, const float4 *sL, float4 *pL, bool leaf_ids,
-                    unsigned long long *counters, const SlotScenes &M, hipStream_t st) {
+                    unsigned long long *counters, const SlotScenes &M, hipStream_t st,
+                    uint32_t *ovf = nullptr, uint32_t ovf_cap = 0) {
   switch (stack) {
     case 32:
       return launch_t<ANY, COUNT, 32, BS, MULTI>(s, rays, W, hits, occ, sL, pL, leaf_ids,
-                                                 counters, M, st);
+                                                 counters, M, st, ovf, ovf_cap);
     case 64:
       return launch_t<ANY, COUNT, 64, BS, MULTI>(s, rays, W, hits, occ, sL, pL, leaf_ids,
-                                                 counters, M, st);
+                                                 counters, M, st, ovf, ovf_cap);
     default:
       return launch_t<ANY, COUNT, 96, BS, MULTI>(s, rays, W, hits, occ, sL, pL, leaf_ids,
-                                                 counters, M, st);
+                                                 counters, M, st, ovf, ovf_cap);
   }
+}
+
+// The listed rays again with the tree's full stack (fixed grid: one block
+// per CU; an empty list costs one short launch).
+template <bool ANY, bool MULTI>
+hipError_t launch_retry(const chk_scene_s *s, int stack, const float4 *rays, float4 *hits,
+                        uint8_t *occ, const float4 *sL, float4 *pL, bool leaf_ids,
+                        unsigned long long *counters, const SlotScenes &M, hipStream_t st,
+                        const uint32_t *ovf, uint32_t ovf_cap) {
+  const DeviceScene S = s ? s->d : DeviceScene{};
+  const dim3 g(256), b(128);
+  if (stack <= 64)
+    hipLaunchKernelGGL((k_trace_retry<ANY, 64, 128, MULTI>), g, b, 0, st, S, rays, hits, occ, sL,
+                       pL, leaf_ids ? 1 : 0, counters, M, ovf, ovf_cap);
+  else
+    hipLaunchKernelGGL((k_trace_retry<ANY, 96, 128, MULTI>), g, b, 0, st, S, rays, hits, occ, sL,
+                       pL, leaf_ids ? 1 : 0, counters, M, ovf, ovf_cap);
+  return hipGetLastError();
 }
 
 template <bool ANY>
@@ -288,6 +372,46 @@ hipError_t launch_kind(const chk_scene_s *s, const TraceShape &T, const float4 *
   if (count)  // the counted render (work model): single scene, 128-thread blocks
     return launch_s<ANY, true, false, 128>(s, T.stack, rays, W, hits, occ, sL, pL, leaf_ids,
                                            counters, M, st);
+  const int shallow = (T.shallow == 8 && !multi && T.block == 128) ? 8 : 32;
+  if (T.ovf && T.stack > shallow) {
+    // deep tree: a 32-entry stack for every ray (twice the occupancy), the
+    // rare overflowing ray again with the full stack
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, T.ovf, 1u);
+    hipError_t e;
+    if (shallow == 8) {
+      e = launch_t<ANY, false, 8, 128, false>(s, rays, W, hits, occ, sL, pL, leaf_ids, counters,
+                                               M, st, T.ovf, T.ovf_cap);
+      if (e != hipSuccess) return e;
+      return launch_retry<ANY, false>(s, T.stack, rays, hits, occ, sL, pL, leaf_ids, counters, M,
+                                      st, T.ovf, T.ovf_cap);
+    }
+    if (multi) {
+      e = T.block == 64    ? launch_s<ANY, false, true, 64>(s, 32, rays, W, hits, occ, sL, pL,
+                                                           leaf_ids, counters, M, st, T.ovf,
+                                                           T.ovf_cap)
+          : T.block == 256 ? launch_s<ANY, false, true, 256>(s, 32, rays, W, hits, occ, sL, pL,
+                                                            leaf_ids, counters, M, st, T.ovf,
+                                                            T.ovf_cap)
+                           : launch_s<ANY, false, true, 128>(s, 32, rays, W, hits, occ, sL, pL,
+                                                            leaf_ids, counters, M, st, T.ovf,
+                                                            T.ovf_cap);
+      if (e != hipSuccess) return e;
+      return launch_retry<ANY, true>(s, T.stack, rays, hits, occ, sL, pL, leaf_ids, counters, M,
+                                     st, T.ovf, T.ovf_cap);
+    }
+    e = T.block == 64    ? launch_s<ANY, false, false, 64>(s, 32, rays, W, hits, occ, sL, pL,
+                                                          leaf_ids, counters, M, st, T.ovf,
+                                                          T.ovf_cap)
+        : T.block == 256 ? launch_s<ANY, false, false, 256>(s, 32, rays, W, hits, occ, sL, pL,
+                                                           leaf_ids, counters, M, st, T.ovf,
+                                                           T.ovf_cap)
+                         : launch_s<ANY, false, false, 128>(s, 32, rays, W, hits, occ, sL, pL,
+                                                           leaf_ids, counters, M, st, T.ovf,
+                                                           T.ovf_cap);
+    if (e != hipSuccess) return e;
+    return launch_retry<ANY, false>(s, T.stack, rays, hits, occ, sL, pL, leaf_ids, counters, M, st,
+                                    T.ovf, T.ovf_cap);
+  }
   if (multi) {
     if (T.block == 64)
       return launch_s<ANY, false, true, 64>(s, T.stack, rays, W, hits, occ, sL, pL, leaf_ids,

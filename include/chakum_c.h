@@ -89,7 +89,8 @@ typedef struct chk_render_stats {
   double other_ms;
   double proposal_ms;       /* proposal inputs, inference and training */
   uint32_t train_steps;     /* Adam steps taken by chk_render_learned */
-  uint32_t reserved;
+  uint32_t stack_retries;   /* rays traced again with the tree's full traversal stack after
+                               overflowing the shorter one (chk_scene_set_trace_stack) */
 } chk_render_stats;
 
 /* Time every kernel launch with HIP events into chk_render_stats. */
@@ -147,6 +148,15 @@ chk_status chk_scene_create(chk_ctx_t ctx, const float *tris, uint32_t n_tris,
  * sources of each of those kinds (CHK_ECAPACITY). */
 chk_status chk_scene_create_desc(chk_ctx_t ctx, const chk_scene_desc *desc, chk_scene_t *out);
 chk_status chk_scene_destroy(chk_scene_t scene);
+/* Traversal stack per lane (no reference counterpart: the reference's
+ * traversal keeps a 64-entry stack and never checks it, bvhAccel.cpp:208).
+ * A tree whose worst case needs more than `entries` (8 or 32, default 32)
+ * is traced by the renders with `entries` LDS entries per lane, at higher
+ * occupancy, and a ray that overflows them is traced again with the full
+ * stack; results are the same either way.  chk_render_stats.stack_retries
+ * counts those rays.  8 is for testing the second trace: rays of ordinary
+ * scenes overflow it. */
+chk_status chk_scene_set_trace_stack(chk_scene_t scene, uint32_t entries);
 chk_status chk_scene_get_info(chk_scene_t scene, chk_scene_info *info);
 
 /* Closest hit for n rays.  Replaces BvhAccel::intersectT

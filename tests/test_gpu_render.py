@@ -128,3 +128,55 @@ def test_render_stats_and_errors(ctx):
         host.render(ds, dict(spp=0, max_depth=3))
     with pytest.raises(abi.ChkError):
         host.render(ds, dict(spp=1, max_depth=3), camera=scenes.Camera((0, 0, 0), (0, 1, 0), (0, 1, 0), 40, 40, 8, 8))
+
+
+def layered_scene(n=2048, size=48):
+    """n parallel unit quads stacked along -z in front of the camera, a quad
+    light above: a ray down the stack hits most children of the nodes on its
+    way (stacks of 10 to 15 entries), so an 8-entry stack overflows."""
+    rng = np.random.default_rng(5)
+    quads = [scenes.quad((0.5, 0.5, -0.002 * k), (0.5, 0, 0), (0, 0.5, 0)) for k in range(n)]
+    mats = [scenes.lambert_material(rng.uniform(0.3, 0.8, 3))]
+    light = [(scenes.quad((0.5, 1.2, 0.2), (0.3, 0, 0), (0, 0, 0.3)), (1.0, 1.0, 1.0), 10.0)]
+    cam = scenes.Camera((0.5, 0.5, 1.5), (0, 0, -1), (0, 1, 0), 30.0, 30.0, size, size)
+    return scenes._assemble(np.concatenate(quads), np.zeros(2 * n, np.uint32), mats, light, cam,
+                            "layers", {})
+
+
+def test_short_stack_retries_are_exact(ctx):
+    """A tree whose worst-case stack exceeds an 8-entry stack: the rays that
+    overflow it are traced again with the full stack, and the renders (path
+    tracer, and the shaders' closest-hit and shadow traces) equal the
+    32-entry renders bit for bit."""
+    sc = layered_scene()
+    ds = abi.DeviceScene(ctx, sc)
+    assert 8 < ds.info()["stack_bound"] <= 32
+    P = dict(spp=8, max_depth=4, seed=0x5EED0020)
+    dp = host.direct_params(abi.DIRECT_MIS)
+    a32, _, s32 = host.render(ds, P)
+    d32, _, _ = host.render_direct(ds, dict(spp=2, seed_lo=0x5EED0020), dp)
+    assert s32["stack_retries"] == 0
+    ds.set_trace_stack(8)
+    a16, _, s16 = host.render(ds, P)
+    d16, _, _ = host.render_direct(ds, dict(spp=2, seed_lo=0x5EED0020), dp)
+    print(f"8-entry stack: {s16['stack_retries']} of {s16['rays_extend'] + s16['rays_shadow']} "
+          "rays traced again")
+    assert s16["stack_retries"] > 0
+    np.testing.assert_array_equal(a16, a32)
+    np.testing.assert_array_equal(d16, d32)
+    with pytest.raises(abi.ChkError, match="8 or 32"):
+        ds.set_trace_stack(24)
+
+
+def test_deep_tree_short_stack_equals_full_stack(ctx):
+    """A tree whose worst-case stack (40) exceeds 32 entries is rendered
+    with the 32-entry stack plus retries; the counted render traces every ray
+    with the full stack: the images are equal."""
+    sc = scenes.cornell(sphere_tris=100_000, width=32, height=32)
+    ds = abi.DeviceScene(ctx, sc)
+    assert ds.info()["stack_bound"] > 32
+    P = dict(spp=4, max_depth=4, seed=0x5EED0021)
+    a, _, st = host.render(ds, P)
+    c, _, _ = host.render(ds, P, count=True)
+    print(f"deep tree: {st['stack_retries']} rays traced again")
+    np.testing.assert_array_equal(a, c)
