@@ -20,13 +20,17 @@ __device__ inline uint16_t f2bf(float f) {
   u += 0x7FFFu + ((u >> 16) & 1u);
   return (uint16_t)(u >> 16);
 }
+// Round to nearest even, as f2bf does for every finite value; gfx950 converts
+// two values per v_cvt_pk_bf16_f32.
+__device__ inline short bf16_bits(float a) { return __builtin_bit_cast(short, (__bf16)a); }
+typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
+typedef float fl2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u2 __attribute__((ext_vector_type(2)));
+__device__ inline uint32_t pack2(float a, float b) {
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector((fl2{a, b}), bf2));
+}
 __device__ inline s4 pack4(float a, float b, float c, float d) {
-  s4 r;
-  r.x = (short)f2bf(a);
-  r.y = (short)f2bf(b);
-  r.z = (short)f2bf(c);
-  r.w = (short)f2bf(d);
-  return r;
+  return __builtin_bit_cast(s4, (u2{pack2(a, b), pack2(c, d)}));
 }
 __device__ inline float bf2f(short b) { return __uint_as_float(((uint32_t)(uint16_t)b) << 16); }
 
@@ -63,35 +67,47 @@ __device__ inline const s4 *frag_g(const MlpDev &W, int l) {
   return reinterpret_cast<const s4 *>(W.G + (l < 3 ? l * kFragLayerF : 3 * kFragLayerF));
 }
 
-// One hidden layer: out tiles (fp32 + bias), then ReLU -> bf16 B operands.
-__device__ inline void hidden(const MlpDev &W, int l, const s4 in[4], int lane, s4 out[4]) {
-  const s4 *F = frag_f(W, l);
-  const float *b = W.bias + 64 * l;
+// One layer over NG groups of 16 points: the accumulators start at the bias,
+// each A fragment (from F, global or LDS) feeds the NG groups' MFMAs.  OT
+// output tiles; hidden layers then apply ReLU and round to bf16 B operands.
+template <int NG, int OT>
+__device__ inline void layer_acc(const s4 *F, const float *b, const s4 (&in)[NG][4], int lane,
+                                 f4 (&acc)[NG][OT]) {
   const int g = lane >> 4;
 #pragma unroll
-  for (int ot = 0; ot < 4; ++ot) {
-    f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+  for (int ot = 0; ot < OT; ++ot) {
+    const f4 bias = *reinterpret_cast<const f4 *>(b + ot * 16 + 4 * g);
+    const s4 a0 = F[ot * 4 * 64 + lane];
 #pragma unroll
-    for (int kt = 0; kt < 4; ++kt) acc = mfma(F[(ot * 4 + kt) * 64 + lane], in[kt], acc);
-    const int o = ot * 16 + 4 * g;
-    out[ot] = pack4(fmaxf(acc[0] + b[o], 0.0f), fmaxf(acc[1] + b[o + 1], 0.0f),
-                    fmaxf(acc[2] + b[o + 2], 0.0f), fmaxf(acc[3] + b[o + 3], 0.0f));
+    for (int n = 0; n < NG; ++n) acc[n][ot] = mfma(a0, in[n][0], bias);
+#pragma unroll
+    for (int kt = 1; kt < 4; ++kt) {
+      const s4 a = F[(ot * 4 + kt) * 64 + lane];
+#pragma unroll
+      for (int n = 0; n < NG; ++n) acc[n][ot] = mfma(a, in[n][kt], acc[n][ot]);
+    }
   }
 }
+// ReLU after rounding: bf16 bit patterns order as signed 16-bit integers
+// (negative values and -0 below +0), so max(bits, 0) is bf16(max(x, 0)) up to
+// the sign of zero, two lanes per v_pk_max_i16.
+__device__ inline s4 relu_bf16(s4 v) { return __builtin_elementwise_max(v, s4{0, 0, 0, 0}); }
+template <int NG>
+__device__ inline void hidden_n(const s4 *F, const float *b, const s4 (&in)[NG][4], int lane,
+                                s4 (&out)[NG][4]) {
+  f4 acc[NG][4];
+  layer_acc<NG, 4>(F, b, in, lane, acc);
+#pragma unroll
+  for (int n = 0; n < NG; ++n)
+#pragma unroll
+    for (int ot = 0; ot < 4; ++ot)
+      out[n][ot] = relu_bf16(pack4(acc[n][ot][0], acc[n][ot][1], acc[n][ot][2], acc[n][ot][3]));
+}
 
-__device__ inline void output_layer(const MlpDev &W, const s4 in[4], int lane, f4 out[3]) {
-  const s4 *F = frag_f(W, 3);
-  const float *b = W.bias + 192;
-  const int g = lane >> 4;
-#pragma unroll
-  for (int ot = 0; ot < 3; ++ot) {
-    f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-    for (int kt = 0; kt < 4; ++kt) acc = mfma(F[(ot * 4 + kt) * 64 + lane], in[kt], acc);
-    const int o = ot * 16 + 4 * g;
-    for (int v = 0; v < 4; ++v) acc[v] += b[o + v];
-    out[ot] = acc;
-  }
+__device__ inline void hidden(const MlpDev &W, int l, const s4 in[4], int lane, s4 out[4]) {
+  const s4 (&i1)[1][4] = *reinterpret_cast<const s4 (*)[1][4]>(in);
+  s4 (&o1)[1][4] = *reinterpret_cast<s4 (*)[1][4]>(out);
+  hidden_n<1>(frag_f(W, l), W.bias + 64 * l, i1, lane, o1);
 }
 
 // Softmax over the first n_lights logits of each point (column), defensive
@@ -120,11 +136,11 @@ __device__ inline void proposal_head(const f4 out[3], int lane, uint32_t L, floa
   s += __shfl_xor(s, 16);
   s += __shfl_xor(s, 32);
   sum_e = s;
-  const float invL = 1.0f / (float)L;
+  const float invL = 1.0f / (float)L, inv_s = 1.0f / s;
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     const int o = (k >> 2) * 16 + 4 * g + (k & 3);
-    q[k] = (uint32_t)o < L ? (1.0f - kPropEpsQ) * (e[k] / s) + kPropEpsQ * invL : 0.0f;
+    q[k] = (uint32_t)o < L ? (1.0f - kPropEpsQ) * (e[k] * inv_s) + kPropEpsQ * invL : 0.0f;
   }
   // mixing logit: output 32 = tile 2, row 0 -> lanes of group 0, v = 0
   const float zc = __shfl(out[2][0], lane & 15);
@@ -132,47 +148,118 @@ __device__ inline void proposal_head(const f4 out[3], int lane, uint32_t L, floa
 }
 
 // ------------------------------------------------------------------ kernels
-// Inference over the items of W (segment r = blockIdx % kSegs; the waves of
-// a segment's blocks stride over its groups of 16 points).
-__global__ __launch_bounds__(256) void k_mlp_eval(MlpDev W, const float *__restrict__ inputs,
-                                                  SegRange R, float *__restrict__ probs,
-                                                  float *__restrict__ mix_out,
-                                                  float *__restrict__ logits) {
-  const int lane = threadIdx.x & 63;
+// Inference over the items of R.  A block stages the forward fragments and
+// the biases in LDS once; each wave then takes 32 points (two groups of 16)
+// at a time from its segment (r = blockIdx % kSegs):
+//  * encoding: the 28 sincospi values of a point (7 inputs x 4 octaves) are
+//    spread over its two lanes (octaves h and h + 2, h = lane / 32) and
+//    written as bf16 into the wave's LDS table in B-operand order, so each
+//    lane then reads its four K tiles with four 8-byte loads;
+//  * every A fragment read from LDS feeds both groups' MFMAs.
+constexpr int kEvalBlock = 512;
+constexpr int kEvalWaves = kEvalBlock / 64;
+
+struct EvalLds {
+  s4 F[kFragTotal / 4];
+  __attribute__((aligned(16))) float bias[240];
+  s4 enc[kEvalWaves][2][4][64];  // [wave][group][K tile][lane]
+};
+static_assert(sizeof(EvalLds) <= 65536, "eval LDS fits one block's static limit");
+
+// Orders the wave's LDS writes before its reads of other lanes' entries (and
+// the reads of one iteration before the writes of the next).
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// Feature f of point pt (0..31) in the table: group pt / 16, K tile f / 16,
+// lane 16 ((f / 4) % 4) + pt % 16, element f % 4.  The index is the sum of
+// a point part and a feature part.
+__device__ constexpr uint32_t enc_pt(uint32_t pt) { return ((pt >> 4) * 256 + (pt & 15)) << 2; }
+__device__ constexpr uint32_t enc_f(uint32_t f) {
+  return (((f >> 4) * 64 + ((f >> 2) & 3) * 16) << 2) + (f & 3);
+}
+__device__ inline void enc_put(uint16_t *T, uint32_t ipt, uint32_t ifeat, float v) {
+  T[ipt + ifeat] = (uint16_t)bf16_bits(v);
+}
+
+__global__ __launch_bounds__(kEvalBlock) void k_mlp_eval(MlpDev W, const float *__restrict__ inputs,
+                                                         SegRange R, float *__restrict__ probs,
+                                                         float *__restrict__ mix_out,
+                                                         float *__restrict__ logits) {
+  __shared__ EvalLds S;
+  const s4 *gF = reinterpret_cast<const s4 *>(W.F);
+  for (uint32_t i = threadIdx.x; i < kFragTotal / 4; i += kEvalBlock) S.F[i] = gF[i];
+  for (uint32_t i = threadIdx.x; i < 240; i += kEvalBlock) S.bias[i] = W.bias[i];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  uint16_t *T = reinterpret_cast<uint16_t *>(&S.enc[wv][0][0][0]);
+  // feature 63 is the constant 1 (K tile 3, lane group 3, element 3)
+  if (lane < 32) T[enc_pt(lane) + enc_f(63)] = 0x3F80;
+  __syncthreads();
+
   const uint32_t r = blockIdx.x % kSegs;
   const uint32_t k = R.count_of(r);
   const size_t seg0 = (size_t)r * R.segcap;
-  const uint32_t wave = (blockIdx.x / kSegs) * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  const uint32_t nw = (gridDim.x / kSegs) * (blockDim.x >> 6);
-  for (uint32_t g0 = wave * 16; g0 < k; g0 += nw * 16) {
-    const uint32_t pl = g0 + (lane & 15);
-    const bool valid = pl < k;
-    const size_t p = seg0 + pl;
-    float s[7];
+  const uint32_t wave = (blockIdx.x / kSegs) * kEvalWaves + wv;
+  const uint32_t nw = (gridDim.x / kSegs) * kEvalWaves;
+  const int g = lane >> 4;
+  const uint32_t pt = lane & 31, h = lane >> 5;
+  for (uint32_t g0 = wave * 32; g0 < k; g0 += nw * 32) {
+    wave_lds_sync();
+    {
+      const bool valid = g0 + pt < k;
+      const float *in = inputs + 7 * (seg0 + g0 + pt);
+      const uint32_t ipt = enc_pt(pt);
 #pragma unroll
-    for (int i = 0; i < 7; ++i) s[i] = valid ? inputs[7 * p + i] : 0.0f;
-    s4 x[4], h1[4], h2[4], h3[4];
-    encode_b(s, lane, x);
-    hidden(W, 0, x, lane, h1);
-    hidden(W, 1, h1, lane, h2);
-    hidden(W, 2, h2, lane, h3);
-    f4 out[3];
-    output_layer(W, h3, lane, out);
-    float q[8], mix, se, mx;
-    proposal_head(out, lane, W.n_lights, q, mix, se, mx);
-    if (valid) {
-      const int g = lane >> 4;
+      for (int i = 0; i < 7; ++i) {
+        const float x = valid ? in[i] : 0.0f;
+        if (h == 0) enc_put(T, ipt, enc_f(9 * i), x);
 #pragma unroll
-      for (int kk = 0; kk < 8; ++kk) probs[32 * p + (kk >> 2) * 16 + 4 * g + (kk & 3)] = q[kk];
-      if (g == 0) mix_out[p] = mix;
-      if (logits) {
+        for (int o = 0; o < 2; ++o) {
+          // octave h + 2 o: sin(2^oct pi x) is feature 9 i + 1 + oct, cos 9 i + 5 + oct
+          const uint32_t oct = h + 2 * o;
+          float sn, cs;
+          sincospif((float)(1u << oct) * x, &sn, &cs);
+          const uint32_t fs = h ? enc_f(9 * i + 2 + 2 * o) : enc_f(9 * i + 1 + 2 * o);
+          const uint32_t fc = h ? enc_f(9 * i + 6 + 2 * o) : enc_f(9 * i + 5 + 2 * o);
+          enc_put(T, ipt, fs, sn);
+          enc_put(T, ipt, fc, cs);
+        }
+      }
+    }
+    wave_lds_sync();
+    s4 x[2][4], h1[2][4], h2[2][4];
 #pragma unroll
-        for (int t = 0; t < 3; ++t)
+    for (int n = 0; n < 2; ++n)
 #pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            const int o = t * 16 + 4 * g + v;
-            if (o < kMlpOut) logits[33 * p + o] = out[t][v];
-          }
+      for (int kt = 0; kt < 4; ++kt) x[n][kt] = S.enc[wv][n][kt][lane];
+    hidden_n<2>(S.F, S.bias, x, lane, h1);
+    hidden_n<2>(S.F + kFragLayerF / 4, S.bias + 64, h1, lane, h2);
+    hidden_n<2>(S.F + 2 * kFragLayerF / 4, S.bias + 128, h2, lane, h1);
+    f4 out[2][3];
+    layer_acc<2, 3>(S.F + 3 * kFragLayerF / 4, S.bias + 192, h1, lane, out);
+#pragma unroll
+    for (int n = 0; n < 2; ++n) {
+      float q[8], mix, se, mx;
+      proposal_head(out[n], lane, W.n_lights, q, mix, se, mx);
+      const uint32_t pl = g0 + 16 * n + (lane & 15);
+      if (pl < k) {
+        const size_t p = seg0 + pl;
+        float *pr = probs + 32 * p + 4 * g;
+        *reinterpret_cast<f4 *>(pr) = f4{q[0], q[1], q[2], q[3]};
+        *reinterpret_cast<f4 *>(pr + 16) = f4{q[4], q[5], q[6], q[7]};
+        if (g == 0) mix_out[p] = mix;
+        if (logits) {
+#pragma unroll
+          for (int t = 0; t < 3; ++t)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              const int o = t * 16 + 4 * g + v;
+              if (o < kMlpOut) logits[33 * p + o] = out[n][t][v];
+            }
+        }
       }
     }
   }
@@ -582,11 +669,14 @@ MlpDev proposal_dev(const chk_proposal_s *p) {
 
 hipError_t launch_mlp_eval(const chk_proposal_s *p, const float *inputs, const SegRange &R,
                            float *probs, float *mix, float *logits, hipStream_t st) {
-  const uint32_t groups = R.segcap / 16;  // per segment
+  // probs rows are 128 bytes, written as two 16-byte stores per lane
+  if (((uintptr_t)probs & 15) != 0) return hipErrorInvalidValue;
+  const uint32_t groups = R.segcap / 32;  // per segment, 32 points per wave step
   const uint32_t per_seg = std::max<uint32_t>(
-      1u, std::min<uint32_t>((groups + 3) / 4, (uint32_t)p->ctx->n_cu));
-  hipLaunchKernelGGL(k_mlp_eval, dim3(kSegs * per_seg), dim3(256), 0, st, proposal_dev(p), inputs,
-                     R, probs, mix, logits);
+      1u, std::min<uint32_t>((groups + kEvalWaves - 1) / kEvalWaves,
+                             std::max<uint32_t>(1u, (uint32_t)p->ctx->n_cu / 4)));
+  hipLaunchKernelGGL(k_mlp_eval, dim3(kSegs * per_seg), dim3(kEvalBlock), 0, st, proposal_dev(p),
+                     inputs, R, probs, mix, logits);
   return hipGetLastError();
 }
 
