@@ -325,9 +325,11 @@ __device__ __forceinline__ void shade_vertex(const RenderArgs &A, uint32_t depth
             const float c = P.mix_prev[prev];
             const float pl = P.probs_prev[32 * (size_t)prev + info.y] * pa;
             w = (1.0f - c) * pb / (c * pl + (1.0f - c) * pb);
-            P.rec_prev[prev].le = lum(m.rad);
-            P.rec_prev[prev].plb = pl;
-            P.rec_prev[prev].light_b = info.y;
+            if (P.rec_prev) {
+              P.rec_prev[prev].le = lum(m.rad);
+              P.rec_prev[prev].plb = pl;
+              P.rec_prev[prev].light_b = info.y;
+            }
           } else {
             const float pl = A.inv_nl * pa;
             w = pb / (pb + pl);  // balance heuristic, N_light = N_bsdf = 1 (mis.cpp:209-220)
@@ -358,19 +360,31 @@ __device__ __forceinline__ void shade_vertex(const RenderArgs &A, uint32_t depth
         uint32_t j;
         float qj;
         if (LEARNED) {
-          // inverse CDF of q' over the scene's lights
-          const float *q = P.probs + 32 * i;
+          // inverse CDF of q' over the scene's lights: the row (128 bytes)
+          // is loaded at once, the walk runs in registers
+          const float4 *q4 = reinterpret_cast<const float4 *>(P.probs + 32 * i);
+          float4 qv[8];
+#pragma unroll
+          for (int t = 0; t < 8; ++t) qv[t] = q4[t];
           const float u = u01(rr.x);
           float acc = 0.0f;
           j = S.n_lights - 1u;
-          for (uint32_t k = 0; k + 1 < S.n_lights; ++k) {
-            acc += q[k];
-            if (u < acc) {
-              j = k;
-              break;
+          qj = 0.0f;
+          bool found = false;
+#pragma unroll
+          for (uint32_t k = 0; k < 32; ++k) {
+            const float4 v = qv[k >> 2];
+            const float qk = (k & 3) == 0 ? v.x : (k & 3) == 1 ? v.y : (k & 3) == 2 ? v.z : v.w;
+            if (!found && k + 1 < S.n_lights) {
+              acc += qk;
+              if (u < acc) {
+                j = k;
+                qj = qk;
+                found = true;
+              }
             }
+            if (k == S.n_lights - 1u && !found) qj = qk;
           }
-          qj = q[j];
         } else {
           j = min((uint32_t)(u01(rr.x) * (float)S.n_lights), S.n_lights - 1u);
           qj = A.inv_nl;
@@ -510,7 +524,7 @@ __global__ __launch_bounds__(BS) void k_shade(
     if (LEARNED) rec.sh_slot = (uint32_t)o;
     if (MULTI) Q.s_seg[o] = seg;
   }
-  if (LEARNED && off < total) P.rec[i] = rec;
+  if (LEARNED && off < total && P.rec) P.rec[i] = rec;
 }
 
 // Proposal inputs of every live vertex: position in the scene box, shading
@@ -814,8 +828,9 @@ chk_status render_impl(chk_scene_s *sc, const chk_camera *cam, const chk_render_
         io.mix_prev = LB.mix[out];
         io.in_prev = LB.prev[in];
         io.out_prev = LB.prev[out];
-        io.rec = LB.rec[in];
-        io.rec_prev = LB.rec[out];
+        // records only feed the training steps of a training pass
+        io.rec = training ? LB.rec[in] : nullptr;
+        io.rec_prev = training ? LB.rec[out] : nullptr;
       }
       prof.begin(1);
       if (learned)
