@@ -71,6 +71,8 @@ def parse():
     ap.add_argument("--batch-scenes", type=int, default=16)
     ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--no-ltc", action="store_true")
+    ap.add_argument("--no-mlp", action="store_true")
+    ap.add_argument("--mlp-queries", type=int, default=480 * 270 * 32)
     ap.add_argument("--no-direct", action="store_true")
     ap.add_argument("--direct-spp", type=int, default=1)
     ap.add_argument("--direct-cpu-row-step", type=int, default=2)
@@ -187,6 +189,49 @@ def ltc_leg(args, ctx):
         "roofline": {"bound": "fp64 valu", "achieved": round(flop / dt / 1e12, 3),
                      "peak": 78.6, "unit": "TFLOP/s", "frac": round(flop / dt / 1e12 / 78.6, 4),
                      "work_model": "180 FLOP per (point, quad) pair"},
+    }
+
+
+def mlp_leg(args, ctx):
+    """The proposal network's inference (chk_proposal_eval -> k_mlp_eval) on
+    mlp_queries vertices, the live vertices of one bounce of the
+    time-to-RMSE leg's 32-spp pass at 480x270.  Work model: 28,800 FLOP per
+    query (2 x (3 x 64 x 64 + 64 x 33) multiply-adds of the four layers)
+    against the dense BF16 MFMA peak; HBM: 28 B of inputs in, 128 B of pmf
+    and 4 B of mixing weight out per query."""
+    import numpy as np
+
+    from chakumrender_amd import abi, host
+
+    k = args.mlp_queries
+    pr = host.Proposal(ctx, 32, seed=5)
+    s = np.random.default_rng(7).random((k, 7), dtype=np.float32)
+    sd = abi.DevBuf.from_array(s)
+    probs, mix = abi.DevBuf(128 * k), abi.DevBuf(4 * k)
+    L = pr.L
+
+    def run():
+        abi.check(L.chk_proposal_eval(pr.h, sd.ptr, k, probs.ptr, mix.ptr, None, None),
+                  "chk_proposal_eval")
+
+    run()
+    abi.sync()
+    reps = 10
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        run()
+    abi.sync()
+    dt = (time.perf_counter() - t0) / reps
+    flop = 28800.0 * k
+    return {
+        "workload": f"{k} queries, 7 inputs -> 64-wide encoding -> 3 x 64 ReLU -> 33 outputs, "
+                    "BF16 MFMA with FP32 accumulation, 32-light softmax + mixing sigmoid",
+        "ms": round(dt * 1e3, 4),
+        "gqueries_per_s": round(k / dt / 1e9, 3),
+        "hbm_gbs": round(160.0 * k / dt / 1e9, 1),
+        "roofline": {"bound": "mfma", "achieved": round(flop / dt / 1e12, 2), "peak": 2500.0,
+                     "unit": "TFLOP/s", "frac": round(flop / dt / 1e12 / 2500.0, 4),
+                     "work_model": "28,800 FLOP per query"},
     }
 
 
@@ -648,6 +693,8 @@ def main():
         line["batch"] = batch_leg(args, ctx)
     if not args.no_ltc:
         line["ltc"] = ltc_leg(args, ctx)
+    if not args.no_mlp:
+        line["mlp"] = mlp_leg(args, ctx)
     if not args.no_direct:
         line["direct"] = direct_leg(args, ctx, sc, ds)
     if not args.no_sources:

@@ -154,7 +154,10 @@ __device__ inline void proposal_head(const f4 out[3], int lane, uint32_t L, floa
 //  * encoding: the 28 sincospi values of a point (7 inputs x 4 octaves) are
 //    spread over its two lanes (octaves h and h + 2, h = lane / 32) and
 //    written as bf16 into the wave's LDS table in B-operand order, so each
-//    lane then reads its four K tiles with four 8-byte loads;
+//    lane then reads its four K tiles with four 8-byte loads.  One
+//    sincospif per octave keeps the values those of the training kernel's
+//    encoding (a double-angle step from the octave below would save a
+//    tenth of the time and move a few features by a bf16 step);
 //  * every A fragment read from LDS feeds both groups' MFMAs.
 constexpr int kEvalBlock = 512;
 constexpr int kEvalWaves = kEvalBlock / 64;
@@ -897,6 +900,10 @@ chk_status chk_proposal_eval(chk_proposal_t p, const float *inputs, uint32_t k, 
     return CHK_EINVAL;
   }
   if (k == 0) return CHK_OK;
+  if (((uintptr_t)probs & 15) != 0) {
+    set_error("chk_proposal_eval: probs must be 16-byte aligned");
+    return CHK_EINVAL;
+  }
   CHK_HIP(hipSetDevice(p->ctx->device));
   CHK_HIP(launch_mlp_eval(p, inputs, linear_range(k), probs, mix, logits, (hipStream_t)stream));
   return CHK_OK;

@@ -360,7 +360,8 @@ chk_status chk_proposal_destroy(chk_proposal_t p);
 /* host float[14625]; set resets the Adam state. */
 chk_status chk_proposal_set_params(chk_proposal_t p, const float *host, size_t n);
 chk_status chk_proposal_get_params(chk_proposal_t p, float *host, size_t n);
-/* inputs k x 7, probs k x 32 (q'), mix k (c), logits k x 33 (may be NULL): dev. */
+/* inputs k x 7, probs k x 32 (q', 16-byte aligned), mix k (c), logits k x 33
+ * (may be NULL): dev. */
 chk_status chk_proposal_eval(chk_proposal_t p, const float *inputs, uint32_t k, float *probs,
                              float *mix, float *logits, void *stream);
 /* Parameter gradient (dev float[14625], overwritten) of a loss whose
