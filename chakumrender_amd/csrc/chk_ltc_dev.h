@@ -37,13 +37,33 @@ __device__ inline double ndot(d3 n, d3 v) {
   return ZPLANE ? v.z : dot(n, v);
 }
 
+// Points of the clipped polygon, written in pairs at positions 0-1 then 2-3
+// (a third pair is dropped); named registers rather than an array indexed
+// at run time, which would live in scratch memory.
+__device__ inline d3 sel3(bool c, d3 a, d3 b) {
+  return dk(c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z);
+}
+struct Clip4 {
+  d3 q0, q1, q2, q3;
+  int idx;
+  __device__ void put(d3 X, d3 Y) {
+    const bool a = idx == 0, b = idx == 2;
+    q0 = sel3(a, X, q0);
+    q1 = sel3(a, Y, q1);
+    q2 = sel3(b, X, q2);
+    q3 = sel3(b, Y, q3);
+    idx += 2;
+  }
+};
+
 // clipTriangle (lightSources_CV.cpp:28-126 and 129-219): the 4-point
 // polygon of the part of ABC above the plane; false if it has none.
 // nA, nB, nC: signs of the vertices (only their sign is read).
 template <bool ZPLANE>
-__device__ bool clip64(d3 A, d3 B, d3 C, d3 n, d3 P, double nA, double nB, double nC, d3 *clip) {
+__device__ bool clip64(d3 A, d3 B, d3 C, d3 n, d3 P, double nA, double nB, double nC,
+                       Clip4 &clip) {
   const double eps = 1e-15;
-  int cnt = 0, idx = 0;
+  int cnt = 0;
   const d3 dAB = A - B, dBC = B - C, dCA = C - A;
   const double nAB = ndot<ZPLANE>(n, dAB), nBC = ndot<ZPLANE>(n, dBC),
                nCA = ndot<ZPLANE>(n, dCA);
@@ -53,8 +73,8 @@ __device__ bool clip64(d3 A, d3 B, d3 C, d3 n, d3 P, double nA, double nB, doubl
     const d3 I = dAB * (pA / nAB) + A;
     if (dot(I - A, B - I) > 0) {
       ++cnt;
-      if (nA < 0) { clip[idx++] = I; clip[idx++] = B; }
-      else { clip[idx++] = A; clip[idx++] = I; }
+      if (nA < 0) clip.put(I, B);
+      else clip.put(A, I);
     }
   } else if (pA * pA < eps) {
     return false;
@@ -63,10 +83,8 @@ __device__ bool clip64(d3 A, d3 B, d3 C, d3 n, d3 P, double nA, double nB, doubl
     const d3 I = dBC * (pB / nBC) + B;
     if (dot(I - B, C - I) > 0) {
       ++cnt;
-      if (idx < 4) {
-        if (nB < 0) { clip[idx++] = I; clip[idx++] = C; }
-        else { clip[idx++] = B; clip[idx++] = I; }
-      }
+      if (nB < 0) clip.put(I, C);
+      else clip.put(B, I);
     }
   } else if (pB * pB < eps) {
     return false;
@@ -76,10 +94,8 @@ __device__ bool clip64(d3 A, d3 B, d3 C, d3 n, d3 P, double nA, double nB, doubl
     // the reference tests only the x components on this edge
     if ((I.x - C.x) * (A.x - I.x) > 0) {
       ++cnt;
-      if (idx < 4) {
-        if (nC < 0) { clip[idx++] = I; clip[idx++] = A; }
-        else { clip[idx++] = C; clip[idx++] = I; }
-      }
+      if (nC < 0) clip.put(I, A);
+      else clip.put(C, I);
     }
   } else if (pC * pC < eps) {
     return false;
@@ -102,9 +118,11 @@ __device__ double tri64(d3 A, d3 B, d3 C, d3 n, d3 P) {
   } else if (nA < 0 && nB < 0 && nC < 0) {
     return 0.0;
   } else {
-    d3 cl[4] = {dk(0, 0, 0), dk(0, 0, 0), dk(0, 0, 0), dk(0, 0, 0)};
+    Clip4 cl;
+    cl.q0 = cl.q1 = cl.q2 = cl.q3 = dk(0, 0, 0);
+    cl.idx = 0;
     if (!clip64<ZPLANE>(A, B, C, n, P, nA, nB, nC, cl)) return 0.0;
-    const d3 q1 = cl[0] - P, q2 = cl[1] - P, q3 = cl[2] - P, q4 = cl[3] - P;
+    const d3 q1 = cl.q0 - P, q2 = cl.q1 - P, q3 = cl.q2 - P, q4 = cl.q3 - P;
     const d3 c1 = cross(q1, q2), c2 = cross(q2, q3), c3 = cross(q3, q4), c4 = cross(q4, q1);
     sum = edge64(q1, q2, c1, ndot<ZPLANE>(n, c1)) + edge64(q2, q3, c2, ndot<ZPLANE>(n, c2)) +
           edge64(q3, q4, c3, ndot<ZPLANE>(n, c3)) + edge64(q4, q1, c4, ndot<ZPLANE>(n, c4));
