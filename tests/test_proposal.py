@@ -68,6 +68,24 @@ gpu = pytest.mark.gpu
 
 
 @gpu
+def test_gpu_eval_rejects_misaligned_probs(ctx):
+    """Rows of q' are written with 16-byte stores: a probs pointer off that
+    alignment is refused on the host, before any launch."""
+    import ctypes as C
+
+    from chakumrender_amd import abi, host
+
+    pr = host.Proposal(ctx, 4, seed=1)
+    k = 16
+    s = abi.DevBuf.from_array(np.zeros((k, 7), np.float32))
+    probs, mix = abi.DevBuf(128 * k + 16), abi.DevBuf(4 * k)
+    off = C.c_void_p(probs.ptr.value + 4)
+    st = pr.L.chk_proposal_eval(pr.h, s.ptr, k, off, mix.ptr, None, None)
+    assert st == 1 and b"16-byte aligned" in pr.L.chk_last_error()
+    assert pr.L.chk_proposal_eval(pr.h, s.ptr, k, probs.ptr, mix.ptr, None, None) == 0
+
+
+@gpu
 def test_gpu_init_params_match_cpu(ctx):
     from chakumrender_amd import host
 
