@@ -98,23 +98,15 @@ def batch_leg(args, ctx):
 
     from chakumrender_amd import abi, host, scenes
 
-    rng = np.random.default_rng(0x5EED0004)
     t0 = time.perf_counter()
-    pool = []
-    for k in range(args.batch_scenes):
-        tris = int(np.exp(rng.uniform(np.log(1e3), np.log(5e5))))
-        nl = int(rng.integers(1, 17))
-        sc = scenes.cornell(seed=0x5EED0400 + k, sphere_tris=max(tris // 2, 8), n_lights=nl,
-                            light_rng=nl > 1)
-        pool.append(abi.DeviceScene(ctx, sc))
+    pool_sc, reqs = scenes.batch_workload(n_requests=args.batch_requests, n_scenes=args.batch_scenes)
+    pool = [abi.DeviceScene(ctx, sc) for sc in pool_sc]
     items = []
-    for i in range(args.batch_requests):
-        w = int(np.exp(rng.uniform(np.log(64), np.log(1024))))
-        h = int(np.exp(rng.uniform(np.log(64), np.log(1024))))
-        spp = int(rng.choice([1, 4, 16, 64, 256]))
-        ds = pool[int(rng.integers(0, len(pool)))]
-        cam = dataclasses.replace(ds.scene.camera, width=w, height=h)
-        items.append(dict(ds=ds, camera=cam, params=dict(spp=spp, max_depth=4, seed=0x5EED0400 + i)))
+    for q in reqs:
+        ds = pool[q["scene"]]
+        cam = dataclasses.replace(ds.scene.camera, width=q["width"], height=q["height"])
+        items.append(dict(ds=ds, camera=cam, params=dict(spp=q["spp"], max_depth=q["max_depth"],
+                                                         seed=q["seed"])))
     setup_s = time.perf_counter() - t0
     t1 = time.perf_counter()
     best, tune_ms = host.batch_autotune(ctx, items)

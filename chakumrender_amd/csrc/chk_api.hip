@@ -1,6 +1,7 @@
 // C ABI of libchakum (include/chakum_c.h): context, scene upload, trace,
 // BRDF and render entry points.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -589,6 +590,10 @@ chk_status chk_scene_create_desc(chk_ctx_t ctx, const chk_scene_desc *desc, chk_
     }
   }
   sc->stack_size = stack;
+  {
+    static std::atomic<uint64_t> next_gen{1};
+    sc->gen = next_gen.fetch_add(1);
+  }
   chk_scene_info &I = sc->info;
   I.n_tris = n_tris;
   I.n_tris_bvh = m;

@@ -136,6 +136,9 @@ struct chk_scene_s {
   void *env_mem = nullptr;  // environment image and tables
   // the first non-emitter material without a BSDF sampler (-1: none)
   int shallow_stack = 32;  // chk_scene_set_trace_stack (8 or 32)
+  // process-unique creation number: a batch's cached graphs are keyed on it,
+  // so a new scene at a freed scene's address never replays them
+  uint64_t gen = 0;
   int bad_sampler_mat = -1;
   uint32_t bad_sampler_brdf = 0;
 };

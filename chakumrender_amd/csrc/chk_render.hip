@@ -1073,9 +1073,19 @@ chk_status batch_impl(chk_ctx_s *ctx, const chk_request *R, uint32_t n,
   const uint32_t segcap = seg_capacity(cap);
   const uint32_t qcap = kSegs * segcap;
   const size_t need = buffers_bytes(qcap, cap) + 3 * (size_t)qcap * 4 + 4 * 256;
-  std::vector<unsigned char> key(sizeof(chk_request) * n + sizeof(T));
+  // graph key: the request records, the tuning, and per request its scene's
+  // creation number and stack (a graph bakes in the traversal kernel chosen
+  // for the stack; a scene handle's address alone can be reused)
+  std::vector<unsigned char> key(sizeof(chk_request) * n + sizeof(T) + 12 * (size_t)n);
   std::memcpy(key.data(), R, sizeof(chk_request) * n);
   std::memcpy(key.data() + sizeof(chk_request) * n, &T, sizeof(T));
+  for (uint32_t i = 0; i < n; ++i) {
+    unsigned char *k = key.data() + sizeof(chk_request) * n + sizeof(T) + 12 * (size_t)i;
+    const uint64_t g = R[i].scene->gen;
+    const uint32_t sz = (uint32_t)R[i].scene->stack_size;
+    std::memcpy(k, &g, 8);
+    std::memcpy(k + 8, &sz, 4);
+  }
   // captured graphs hold the buffer addresses: any reallocation drops them
   bool moved = false;
   chk_status cs = ensure(bs->work, bs->work_bytes, need, "the path queues", moved);
@@ -1209,11 +1219,21 @@ chk_status batch_impl(chk_ctx_s *ctx, const chk_request *R, uint32_t n,
       latency_ms[i] = ms;
     }
   }
+  unsigned long long h[kStatWords];
+  err = hipMemcpy(h, B.stats, sizeof(h), hipMemcpyDeviceToHost);
+  if (err != hipSuccess) {
+    free_events();
+    set_error(std::string("chk_render_batch: statistics readback: ") + hipGetErrorString(err));
+    return CHK_EDEVICE;
+  }
+  if (h[kOverflow] || h[kShOverflow]) {
+    free_events();
+    set_error("chk_render_batch: traversal stack overflow");
+    return CHK_EINTERNAL;
+  }
   if (stats) {
     float ms = 0.0f;
     (void)hipEventElapsedTime(&ms, ev[0], ev[chunks.size()]);
-    unsigned long long h[kStatWords];
-    CHK_HIP(hipMemcpy(h, B.stats, sizeof(h), hipMemcpyDeviceToHost));
     stats->paths = total_paths;
     stats->rays_extend = h[kRaysExt];
     stats->rays_shadow = h[kRaysShadow];

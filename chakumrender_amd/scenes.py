@@ -390,6 +390,31 @@ def cornell_large(seed=0x5EED0005, total_tris=2_000_000, n_lights=32, width=1920
                    height=height, light_rng=True)
 
 
+def batch_workload(seed=0x5EED0004, n_requests=64, n_scenes=16, min_tris=1_000,
+                   max_tris=500_000, max_lights=16, sizes=(64, 1024), spp=(1, 4, 16, 64, 256),
+                   max_depth=4):
+    """Config-4 serving workload: a pool of `n_scenes` Cornell-generator
+    scenes of log-uniform [min_tris, max_tris] sphere triangles and 1..max_lights
+    lights, and `n_requests` requests drawing a scene from the pool, each
+    side log-uniform in `sizes`, spp from `spp`.  Returns (scenes, requests),
+    each request a dict(scene=index, width, height, spp, max_depth, seed)."""
+    rng = np.random.default_rng(seed)
+    pool = []
+    for k in range(n_scenes):
+        tris = int(np.exp(rng.uniform(np.log(min_tris), np.log(max_tris))))
+        nl = int(rng.integers(1, max_lights + 1))
+        pool.append(cornell(seed=0x5EED0400 + k, sphere_tris=max(tris // 2, 8), n_lights=nl,
+                            light_rng=nl > 1))
+    reqs = []
+    for i in range(n_requests):
+        w = int(np.exp(rng.uniform(np.log(sizes[0]), np.log(sizes[1]))))
+        h = int(np.exp(rng.uniform(np.log(sizes[0]), np.log(sizes[1]))))
+        n = int(rng.choice(list(spp)))
+        reqs.append(dict(scene=int(rng.integers(0, len(pool))), width=w, height=h, spp=n,
+                         max_depth=max_depth, seed=0x5EED0400 + i))
+    return pool, reqs
+
+
 # ------------------------------------------------------------------ direct-lighting room
 def direct_room(seed=0x5EED0006, width=64, height=64, reflectance=None, fan=True, panel=True,
                 light_scale=1.0, light_intensity=None):
