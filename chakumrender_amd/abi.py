@@ -203,6 +203,8 @@ SIGNATURES = {
     "chk_ltc_eval": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _u32, _vp, _vp, _u32, _vp, _vp]),
     "chk_render": (C.c_int, [_vp, C.POINTER(chk_camera), C.POINTER(chk_render_params), _vp, _vp,
                              C.POINTER(chk_render_stats), _vp]),
+    "chk_render_noalloc": (C.c_int, [_vp, C.POINTER(chk_camera), C.POINTER(chk_render_params), _vp,
+                                     _vp, C.POINTER(chk_render_stats), _vp]),
     "chk_render_count": (C.c_int, [_vp, C.POINTER(chk_camera), C.POINTER(chk_render_params), _vp,
                                    C.POINTER(chk_render_stats), _vp]),
     "chk_render_batch": (C.c_int, [_vp, C.POINTER(chk_request), _u32, C.POINTER(chk_batch_tuning),

@@ -16,10 +16,6 @@ namespace chk {
 thread_local std::string g_err;
 void set_error(const std::string &msg) { g_err = msg; }
 
-chk_status render_impl(chk_scene_s *sc, const chk_camera *cam, const chk_render_params *P,
-                       double *accum, double *sumsq, chk_render_stats *stats, bool count,
-                       chk_proposal_s *prop, const chk_train_params *train, hipStream_t st);
-
 namespace {
 
 // Load a direction given in FP64 at precision T and normalise it there: the
@@ -227,6 +223,9 @@ chk_status chk_create(int device, chk_ctx_t *out) {
   chk_ctx_s *c = new chk_ctx_s();
   c->device = device;
   c->n_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+  // the device's copy of chk_render_noalloc's scratch array; a failed
+  // lookup is reported by that entry point
+  (void)noalloc_scratch_init(device);
   *out = c;
   return CHK_OK;
   CHK_TRY_END

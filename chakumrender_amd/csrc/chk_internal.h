@@ -234,6 +234,24 @@ const chk_ctx_s *proposal_ctx(const chk_proposal_s *p);
 
 int pick_stack_size(uint32_t bound);
 
+// Caller-provided render workspace: the path queues, counters and overflow
+// list of render_impl carved from [ptr, ptr + bytes) instead of the scene's
+// growable buffer; the chunk size is the largest that fits.
+struct Workspace {
+  void *ptr;
+  size_t bytes;
+};
+// The render path (chk_render.hip).  ws == nullptr: the scene's own buffers,
+// grown with hipMalloc on demand.
+chk_status render_impl(chk_scene_s *sc, const chk_camera *cam, const chk_render_params *P,
+                       double *accum, double *sumsq, chk_render_stats *stats, bool count,
+                       chk_proposal_s *prop, const chk_train_params *train, hipStream_t st,
+                       const Workspace *ws = nullptr);
+// Looks up this device's copy of the fixed render scratch array (called by
+// chk_create with the device current, once per device; later calls return
+// the first result).
+hipError_t noalloc_scratch_init(int device);
+
 // Batched requests (chk_render.hip).
 chk_status batch_impl(chk_ctx_s *ctx, const chk_request *R, uint32_t n,
                       const chk_batch_tuning *tuning, double *latency_ms, chk_batch_stats *stats,

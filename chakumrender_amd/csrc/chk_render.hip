@@ -23,6 +23,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 #include "chk_device.h"
 #include "chk_internal.h"
@@ -94,6 +95,20 @@ size_t buffers_bytes(uint32_t qcap, uint32_t cap) {
   b += (size_t)cap * 16;
   b += ovf_words(qcap) * 4;
   return b + 16 * 256;
+}
+
+// The largest chunk (a multiple of 4096 paths, at most kChunkPaths) whose
+// buffers fit in `bytes`; 0 when not even 4096 paths fit.
+uint32_t workspace_chunk_paths(size_t bytes) {
+  uint32_t lo = 0, hi = kChunkPaths / 4096u;
+  while (lo < hi) {  // largest k with buffers_bytes(qcap(4096 k), 4096 k) <= bytes
+    const uint32_t mid = (lo + hi + 1) / 2, cap = mid * 4096u;
+    if (buffers_bytes(kSegs * seg_capacity(cap), cap) <= bytes)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo * 4096u;
 }
 
 Buffers carve(void *base, uint32_t qcap, uint32_t cap) {
@@ -690,7 +705,8 @@ chk_status make_args(const chk_scene_s *sc, const chk_camera *cam, const chk_ren
 
 chk_status render_impl(chk_scene_s *sc, const chk_camera *cam, const chk_render_params *P,
                        double *accum, double *sumsq, chk_render_stats *stats, bool count,
-                       chk_proposal_s *prop, const chk_train_params *train, hipStream_t st) {
+                       chk_proposal_s *prop, const chk_train_params *train, hipStream_t st,
+                       const Workspace *ws) {
   if (!cam || !P || !accum) {
     set_error("chk_render: null camera, params or accum");
     return CHK_EINVAL;
@@ -700,8 +716,12 @@ chk_status render_impl(chk_scene_s *sc, const chk_camera *cam, const chk_render_
     set_error("chk_render: zero width, height or spp");
     return CHK_EINVAL;
   }
-  if (P->spp > kChunkPaths) {
-    set_error("chk_render: spp exceeds the paths-per-chunk capacity");
+  // paths per chunk: fixed, or the most a caller's workspace holds
+  const uint32_t chunk_paths = ws ? workspace_chunk_paths(ws->bytes) : kChunkPaths;
+  if (P->spp > chunk_paths) {
+    set_error("chk_render: spp " + std::to_string(P->spp) + " exceeds the " +
+              std::to_string(chunk_paths) + " paths per chunk" +
+              (ws ? " that the fixed scratch holds" : ""));
     return CHK_ECAPACITY;
   }
   if (P->mis_mode != (learned ? CHK_MIS_LEARNED : CHK_MIS_BALANCE)) {
@@ -736,13 +756,20 @@ chk_status render_impl(chk_scene_s *sc, const chk_camera *cam, const chk_render_
     set_error("chk_render: image too large");
     return CHK_ECAPACITY;
   }
-  const uint32_t pix_per_chunk = std::max<uint32_t>(1u, kChunkPaths / P->spp);
+  const uint32_t pix_per_chunk = std::max<uint32_t>(1u, chunk_paths / P->spp);
   const uint32_t cap = (uint32_t)std::min<uint64_t>((uint64_t)pix_per_chunk * P->spp,
                                                    n_pix_total * P->spp);
   const uint32_t segcap = seg_capacity(cap);
   const uint32_t qcap = kSegs * segcap;
   const size_t need = buffers_bytes(qcap, cap);
-  if (sc->work_bytes < need) {
+  void *work = sc->work;
+  if (ws) {
+    if (need > ws->bytes) {  // workspace_chunk_paths guarantees it fits
+      set_error("chk_render: internal workspace sizing error");
+      return CHK_EINTERNAL;
+    }
+    work = ws->ptr;
+  } else if (sc->work_bytes < need) {
     if (sc->work) (void)hipFree(sc->work);
     sc->work = nullptr;
     sc->work_bytes = 0;
@@ -751,8 +778,9 @@ chk_status render_impl(chk_scene_s *sc, const chk_camera *cam, const chk_render_
       return CHK_ENOMEM;
     }
     sc->work_bytes = need;
+    work = sc->work;
   }
-  Buffers B = carve(sc->work, qcap, cap);
+  Buffers B = carve(work, qcap, cap);
   CHK_HIP(launch_zero_u32(B.ovf, 2, st));  // overflow count, retries
   LBuffers LB{};
   if (learned) {
@@ -1311,3 +1339,78 @@ chk_status batch_autotune(chk_ctx_s *ctx, const chk_request *R, uint32_t n,
 }
 
 }  // namespace chk
+
+// ====================================================================== no-allocation render
+// chk_render for callers that cannot allocate device memory (code under HIP
+// graph capture, serving loops, harnesses that forbid hipMalloc): the path
+// queues live in one fixed array of device memory that the library holds
+// for the life of the process, one copy per device.  The host-side name
+// `scratch` is a stand-in; every use goes through the device address that
+// hipGetSymbolAddress returned for the current device.
+
+static __device__ __align__(256) unsigned char scratch[CHK_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+bool g_scratch_done[kMaxDevices] = {};
+hipError_t g_scratch_err[kMaxDevices] = {};
+void *g_scratch_ptr[kMaxDevices] = {};
+
+}  // namespace
+
+namespace chk {
+
+hipError_t noalloc_scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_done[device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    g_scratch_err[device] = e;
+    g_scratch_ptr[device] = e == hipSuccess ? p : nullptr;
+    g_scratch_done[device] = true;
+  }
+  return g_scratch_err[device];
+}
+
+}  // namespace chk
+
+extern "C" chk_status chk_render_noalloc(chk_scene_t sc, const chk_camera *camera,
+                                         const chk_render_params *params, double *accum,
+                                         double *sumsq, chk_render_stats *stats, void *stream) {
+  using namespace chk;
+  CHK_TRY_BEGIN
+  if (!sc) {
+    set_error("chk_render_noalloc: null scene");
+    return CHK_EINVAL;
+  }
+  if (sc->extras) {
+    set_error("chk_render_noalloc: the scene has spheres or triangle, sphere, point or "
+              "environment sources; only the direct-lighting entry points and "
+              "chk_render_shade render those");
+    return CHK_EINVAL;
+  }
+  if (params && (params->flags & CHK_RENDER_PROFILE)) {
+    set_error("chk_render_noalloc: CHK_RENDER_PROFILE creates events; not available here");
+    return CHK_EINVAL;
+  }
+  const int dev = sc->ctx->device;
+  CHK_HIP(hipSetDevice(dev));
+  void *ptr = nullptr;
+  {
+    const hipError_t e = noalloc_scratch_init(dev);
+    if (e != hipSuccess) {
+      set_error(std::string("chk_render_noalloc: no scratch array on this device: ") +
+                hipGetErrorString(e));
+      return CHK_EDEVICE;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    ptr = g_scratch_ptr[dev];
+  }
+  const Workspace ws{ptr, (size_t)CHK_NOALLOC_SCRATCH_BYTES};
+  return render_impl(sc, camera, params, accum, sumsq, stats, false, nullptr, nullptr,
+                     (hipStream_t)stream, &ws);
+  CHK_TRY_END
+}

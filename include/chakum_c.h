@@ -229,6 +229,25 @@ chk_status chk_render_count(chk_scene_t scene, const chk_camera *camera,
                             const chk_render_params *params, double *accum,
                             chk_render_stats *stats, void *stream);
 
+/* chk_render for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference, which renders on the host.  Same
+ * arguments, results and errors as chk_render -- every image equals
+ * chk_render's bit for bit -- but the path queues, counters and overflow
+ * list live in a fixed array of CHK_NOALLOC_SCRATCH_BYTES of device memory
+ * that the library holds on each device it runs on until the process ends,
+ * so the call makes no hipMalloc/hipFree.  Paths are rendered in chunks of
+ * at most 356,352 (the most that array holds), so spp above that is
+ * CHK_ECAPACITY.  Not available with CHK_RENDER_PROFILE (CHK_EINVAL).  All
+ * calls on one device share its array: calls that overlap on one device
+ * (other streams, other threads) race and must be serialised by the
+ * caller.  Each call clears the counters it uses; nothing else carries
+ * over between calls. */
+#define CHK_NOALLOC_SCRATCH_BYTES (64u << 20)
+chk_status chk_render_noalloc(chk_scene_t scene, const chk_camera *camera,
+                              const chk_render_params *params, double *accum, double *sumsq,
+                              chk_render_stats *stats, void *stream);
+
 /* ---- direct lighting: the reference's shadeDI estimators ----------------
  * Shading points are the reference's (r, n, x, primitive) arguments:
  *   x     hit point (k x 3 doubles)
