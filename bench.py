@@ -51,8 +51,13 @@ def parse():
                          "FETCH_SIZE / WRITE_SIZE runs recorded in that file)")
     # time-to-RMSE leg: learned proposal vs plain MIS (config 5 scene)
     ap.add_argument("--no-rmse", action="store_true")
-    ap.add_argument("--rmse-width", type=int, default=480)
-    ap.add_argument("--rmse-height", type=int, default=270)
+    ap.add_argument("--rmse-width", type=int, default=1920)
+    ap.add_argument("--rmse-height", type=int, default=1080)
+    ap.add_argument("--rmse-row-step", type=int, default=16,
+                    help="the reference (and the RMSE) covers every N-th row of the frame")
+    ap.add_argument("--rmse-full-reference", action="store_true",
+                    help="also render the reference for every row and report the RMSE of the "
+                         "final images over the whole frame (about 4 minutes at 1080p)")
     ap.add_argument("--rmse-tris", type=int, default=2_000_000)
     ap.add_argument("--rmse-lights", type=int, default=32)
     ap.add_argument("--rmse-depth", type=int, default=4)
@@ -124,6 +129,22 @@ def batch_leg(args, ctx):
     paths = runs[-1]["paths"]
     lat = np.array(lats[-1])
     dev_ms = float(np.mean([r["batch_ms"] for r in runs]))
+    # the same requests one chk_render call each, in the batch's order
+    # (shortest first), into their own accumulators
+    imgs = [host.Image(it["camera"].width, it["camera"].height) for it in items]
+    order = sorted(range(len(items)), key=lambda k: items[k]["camera"].width *
+                   items[k]["camera"].height * items[k]["params"]["spp"])
+    for k in order:  # warm-up (buffers)
+        host.render(items[k]["ds"], items[k]["params"], camera=items[k]["camera"], image=imgs[k],
+                    stats=False, download=False)
+    abi.sync()
+    t3 = time.perf_counter()
+    for k in order:
+        host.render(items[k]["ds"], items[k]["params"], camera=items[k]["camera"], image=imgs[k],
+                    stats=False, download=False)
+    abi.sync()
+    seq_ms = (time.perf_counter() - t3) * 1e3
+    del imgs
     out = {
         "workload": f"{args.batch_requests} requests, sizes 64..1024 log-uniform, spp in "
                     f"{{1,4,16,64,256}}, depth 4, {args.batch_scenes} Cornell-generator scenes "
@@ -132,6 +153,8 @@ def batch_leg(args, ctx):
         "mpaths_per_s": round(paths / (dev_ms / 1e3) / 1e6, 2),
         "wall_mpaths_per_s": round(paths / wall / 1e6, 2),
         "batch_ms": round(dev_ms, 3),
+        "sequential_ms": round(seq_ms, 3),
+        "sequential_mpaths_per_s": round(paths / (seq_ms / 1e3) / 1e6, 2),
         "latency_ms_p50": round(float(np.percentile(lat, 50)), 3),
         "latency_ms_p99": round(float(np.percentile(lat, 99)), 3),
         "tuning": best,
@@ -269,7 +292,7 @@ def direct_leg(args, ctx, sc, ds):
             import oracle
 
             D = oracle.DirectScene(sc)
-            threads = min(16, os.cpu_count() or 1)
+            threads = cpu_threads()
             t1 = time.perf_counter()
             D.render_shade(shader, params, dp, row_step=args.direct_cpu_row_step, threads=threads)
             ct = time.perf_counter() - t1
@@ -336,7 +359,7 @@ def sources_leg(args, ctx):
             import oracle
 
             D = oracle.DirectScene(sc)
-            threads = min(16, os.cpu_count() or 1)
+            threads = cpu_threads()
             t1 = time.perf_counter()
             D.render_shade(abi.SHADE_DIRECT_ONLY, params, dp, row_step=row_step, threads=threads)
             ct = time.perf_counter() - t1
@@ -359,6 +382,39 @@ def cpu_model():
     except OSError:
         pass
     return "unknown"
+
+
+def cpu_threads():
+    """Threads for the CPU baselines: the CPUs this process may run on
+    (sched_getaffinity), or fewer when OMP_NUM_THREADS sets the host share
+    (the GPU boxes allot each GPU 16 CPUs and preset it to that)."""
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return min(n, int(env)) if env.isdigit() and int(env) > 0 else n
+
+
+def cpu_info():
+    """The host the CPU baselines run on, from /proc/cpuinfo and the
+    scheduler: model, logical CPUs, physical cores, those this process may
+    use, and the threads the baselines use."""
+    info = {"cpu_model": cpu_model(), "logical_cpus": os.cpu_count(),
+            "affinity_cpus": len(os.sched_getaffinity(0)), "threads_used": cpu_threads(),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+    try:
+        cores, phys = set(), None
+        for line in open("/proc/cpuinfo"):
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "physical id":
+                phys = v
+            elif k == "core id":
+                cores.add((phys, v))
+            elif k == "flags" and "flags" not in info:
+                info["avx512"] = "avx512f" in v.split()
+        info["physical_cores"] = len(cores) or None
+    except OSError:
+        pass
+    return info
 
 
 def calibration_leg(args, sc):
@@ -419,19 +475,21 @@ def calibration_leg(args, sc):
         "port_us_per_point": round(t_port / len(prim) * 1e6, 3),
         "reference_over_port": round(t_ref / t_port, 3),
         "max_rel_diff": float(np.max(np.abs(ref_v - port_v) / np.maximum(np.abs(ref_v), 1e-12)))}
-    L.orc_set_threads(min(16, os.cpu_count() or 1))
+    L.orc_set_threads(cpu_threads())
     return out
 
 
-def progressive(render_pass, ref, pass_spp, max_spp, target, sync, W, H):
-    """Render passes of pass_spp until the RMSE against ref reaches target or
-    max_spp is reached; returns [(gpu seconds so far, spp, rmse)].  Only the
+def progressive(render_pass, ref, rows, pass_spp, max_spp, target, sync, W, H):
+    """Render whole-frame passes of pass_spp until the RMSE against ref (the
+    reference's rows `rows`) reaches target or max_spp is reached; returns
+    ([(gpu seconds so far, spp, rmse)], the final image's sums).  Only the
     render calls are timed (the RMSE readback is not)."""
     from chakumrender_amd import host
 
     img = host.Image(W, H)
     t = 0.0
     curve = []
+    a = None
     for k in range(max_spp // pass_spp):
         sync()
         t0 = time.perf_counter()
@@ -440,11 +498,11 @@ def progressive(render_pass, ref, pass_spp, max_spp, target, sync, W, H):
         t += time.perf_counter() - t0
         n = (k + 1) * pass_spp
         a, _ = img.get()
-        err = float(((a / n - ref) ** 2).mean() ** 0.5)
+        err = float(((a[rows] / n - ref) ** 2).mean() ** 0.5)
         curve.append((t, n, err))
         if err <= target:
             break
-    return curve
+    return curve, a
 
 
 def time_to(curve, target):
@@ -480,8 +538,17 @@ def rmse_at(curve, t_query):
 
 
 def rmse_leg(args, ctx):
-    """Time-to-RMSE of plain MIS and of the learned proposal against a
-    plain-MIS reference of rmse_ref_spp samples (FP64 accumulation)."""
+    """Time-to-RMSE of plain MIS and of the learned proposal on the config-5
+    scene (2M triangles, 32 quad lights) at the config's 1920 x 1080.
+
+    The reference is plain MIS at rmse_ref_spp samples with FP64
+    accumulation, rendered for every rmse_row_step-th row of the frame (the
+    render's window: each row exactly as in a whole-frame render); the
+    RMSE of the whole-frame passes is taken over those rows.  A systematic
+    sample of 1/16 of the rows is an unbiased estimate of the whole-frame
+    RMSE; --rmse-full-reference renders every row to measure both."""
+    import numpy as np
+
     from chakumrender_amd import abi, host, scenes
 
     W, H = args.rmse_width, args.rmse_height
@@ -489,39 +556,42 @@ def rmse_leg(args, ctx):
     ds = abi.DeviceScene(ctx, sc)
     nl = ds.info()["n_lights"]
     base = dict(max_depth=args.rmse_depth)
+    step = max(1, args.rmse_row_step)
+    rows = np.arange(step // 2, H, step)
+
+    def reference(rlist):
+        img = host.Image(W, H)
+        for r in rlist:
+            host.render(ds, dict(base, spp=args.rmse_ref_spp, seed=0x5EED0F00,
+                                 window=(0, int(r), W, 1)), image=img, stats=False, download=False)
+        return img.get()[0] / args.rmse_ref_spp
+
     t0 = time.perf_counter()
-    ref_img = host.Image(W, H)
-    chunk = max(1, min(args.rmse_ref_spp, (1 << 22) // (W * H) * 4))
-    done = 0
-    while done < args.rmse_ref_spp:
-        n = min(chunk, args.rmse_ref_spp - done)
-        host.render(ds, dict(base, spp=n, sample_offset=done, seed=0x5EED0F00), image=ref_img,
-                    stats=False, download=False)
-        done += n
-    ref = ref_img.get()[0] / args.rmse_ref_spp
+    ref_full = reference(rows)
+    ref = ref_full[rows]
     ref_s = time.perf_counter() - t0
     print(f"# rmse leg: {sc.n_tris} tris, {nl} lights, {W}x{H}, reference {args.rmse_ref_spp} spp "
-          f"in {ref_s:.1f} s, mean {ref.mean():.4f}", file=sys.stderr, flush=True)
+          f"over {len(rows)} rows in {ref_s:.1f} s, mean {ref.mean():.4f}", file=sys.stderr, flush=True)
     ref_check = None
     if not args.no_cpu_baseline:
-        # the reference image against the FP64 oracle's own render of a
-        # 64-row band with independent keys: per-pixel z from the oracle's
-        # sample variance (the GPU reference's is 1 / 1024 of it at most)
-        import numpy as np
-
+        # the reference rows near the middle of the frame against the FP64
+        # oracle's own render of the same rows with independent keys:
+        # per-pixel z from the oracle's sample variance (the GPU reference's
+        # is 1 / 1024 of it at most)
         import oracle
 
-        r0 = max(0, H // 2 - 32)
-        r1 = min(H, r0 + 64)
+        mid = rows[(rows >= H // 2 - 4 * step) & (rows < H // 2 + 4 * step)]
         n = args.rmse_check_spp
         t1 = time.perf_counter()
-        acc, sq, _, _ = oracle.render(sc, dict(base, spp=n, seed=0x5EED0F77), rows=(r0, r1),
-                                      threads=min(16, os.cpu_count() or 1), sumsq=True)
-        m = acc[r0:r1] / n
-        var = np.maximum(sq[r0:r1] / n - m * m, 0) / n
-        z = np.abs(ref[r0:r1] - m) / np.sqrt(var + 1e-30)
-        band = ref[r0:r1]
-        ref_check = {"rows": [r0, r1], "cpu_spp": n, "within_3sigma": round(float((z <= 3).mean()), 4),
+        acc, sq, _, _ = oracle.render(sc, dict(base, spp=n, seed=0x5EED0F77),
+                                      rows=(int(mid[0]), int(mid[-1]) + 1), row_step=step,
+                                      threads=cpu_threads(), sumsq=True)
+        m = acc[mid] / n
+        var = np.maximum(sq[mid] / n - m * m, 0) / n
+        z = np.abs(ref_full[mid] - m) / np.sqrt(var + 1e-30)
+        band = ref_full[mid]
+        ref_check = {"rows": [int(v) for v in mid], "cpu_spp": n,
+                     "within_3sigma": round(float((z <= 3).mean()), 4),
                      "band_mean_gpu_ref": round(float(band.mean()), 6),
                      "band_mean_cpu": round(float(m.mean()), 6),
                      "band_mean_z": round(float(abs(band.mean() - m.mean()) /
@@ -541,10 +611,10 @@ def rmse_leg(args, ctx):
                                      seed=0x5EED0005), prop, train=train, image=img,
                             stats=False, download=False)
 
-    cp = progressive(plain_pass, ref, args.rmse_pass_spp, args.rmse_max_spp, args.rmse_target,
-                     abi.sync, W, H)
-    cl = progressive(learned_pass, ref, args.rmse_pass_spp, args.rmse_max_spp, args.rmse_target,
-                     abi.sync, W, H)
+    cp, img_p = progressive(plain_pass, ref, rows, args.rmse_pass_spp, args.rmse_max_spp,
+                            args.rmse_target, abi.sync, W, H)
+    cl, img_l = progressive(learned_pass, ref, rows, args.rmse_pass_spp, args.rmse_max_spp,
+                            args.rmse_target, abi.sync, W, H)
     tp, tl = time_to(cp, args.rmse_target), time_to(cl, args.rmse_target)
     # equal time: the RMSE each reaches at the time the faster one stops
     t_eq = min(cp[-1][0], cl[-1][0])
@@ -552,7 +622,8 @@ def rmse_leg(args, ctx):
         "workload": f"cornell-style {sc.n_tris} tris, {nl} quad lights, {W}x{H}, depth "
                     f"{args.rmse_depth}; reference plain MIS {args.rmse_ref_spp} spp FP64",
         "target": args.rmse_target,
-        "rmse": "per-channel linear radiance over all pixels",
+        "rmse": f"per-channel linear radiance over every {step}th row of the frame "
+                f"({len(rows)} rows x {W} px), whole-frame passes of {args.rmse_pass_spp} spp",
         "plain_s": None if tp is None else round(tp, 4),
         "learned_s": None if tl is None else round(tl, 4),
         "ratio_learned_over_plain": None if (tp is None or tl is None) else round(tl / tp, 4),
@@ -563,9 +634,23 @@ def rmse_leg(args, ctx):
         "plain_mpaths_per_s": round(W * H * cp[-1][1] / cp[-1][0] / 1e6, 2),
         "learned_mpaths_per_s": round(W * H * cl[-1][1] / cl[-1][0] / 1e6, 2),
         "train": dict(train, pass_spp=args.rmse_pass_spp),
+        "reference_rows": int(len(rows)),
         "reference_seconds": round(ref_s, 2),
         "reference_check": ref_check,
     }
+    if args.rmse_full_reference:
+        # every row of the reference: the final images' RMSE over the whole
+        # frame against the row-sample estimate
+        t1 = time.perf_counter()
+        full = reference(range(H))
+        out["full_reference"] = {
+            "seconds": round(time.perf_counter() - t1, 1),
+            "plain_rmse_frame": float(((img_p / cp[-1][1] - full) ** 2).mean() ** 0.5),
+            "plain_rmse_rows": cp[-1][2],
+            "learned_rmse_frame": float(((img_l / cl[-1][1] - full) ** 2).mean() ** 0.5),
+            "learned_rmse_rows": cl[-1][2],
+            "reference_rows_equal": bool(np.array_equal(full[rows], ref)),
+        }
     prop.close()
     ds.close()
     return out
@@ -695,15 +780,40 @@ def main():
         line["cpu_calibration"] = calibration_leg(args, sc)
 
     if not args.no_cpu_baseline:
+        import numpy as np
+
         import oracle
 
-        threads = min(16, os.cpu_count() or 1)
+        threads = cpu_threads()
         rows = (0, args.height)
+        # independent keys: the oracle's frame is also the statistical check
+        # of the GPU's headline frame
         t1 = time.perf_counter()
-        _, _, cpaths, crays = oracle.render(sc, dict(params, spp=args.cpu_spp), rows=rows,
-                                            row_step=args.cpu_row_step, threads=threads)
+        acc_c, sq_c, cpaths, crays = oracle.render(sc, dict(params, spp=args.cpu_spp,
+                                                            seed=params["seed"] ^ 0xC0FFEE),
+                                                   rows=rows, row_step=args.cpu_row_step,
+                                                   threads=threads, sumsq=True)
         dt = time.perf_counter() - t1
         n_rows = len(range(0, args.height, args.cpu_row_step))
+        # the headline frame again with sums of squares (untimed), same keys
+        # as the timed renders
+        acc_g, sq_g, _ = host.render(ds, dict(params, flags=abi.RENDER_SUMSQ))
+        r = slice(0, args.height, args.cpu_row_step)
+        ng, nc = args.spp, args.cpu_spp
+        mg, mc = acc_g[r] / ng, acc_c[r] / nc
+        se = np.sqrt(np.maximum(sq_g[r] / ng - mg * mg, 0) / ng +
+                     np.maximum(sq_c[r] / nc - mc * mc, 0) / nc)
+        with np.errstate(all="ignore"):
+            z = np.where(se > 0, np.abs(mg - mc) / se, np.where(np.abs(mg - mc) <= 1e-6, 0.0, np.inf))
+        line["parity_vs_oracle"] = {
+            "what": "headline frame (the timed render's keys) against the FP64 oracle's frame with "
+                    "independent keys, per pixel channel, sigma from both sample variances",
+            "pixel_channels": int(z.size),
+            "within_3sigma": round(float(np.mean(z <= 3)), 5),
+            "max_z": round(float(z.max()), 2),
+            "image_mean_gpu": round(float(mg.mean()), 6),
+            "image_mean_oracle": round(float(mc.mean()), 6),
+        }
         line["cpu_baseline"] = {
             "value": round(cpaths / dt / 1e6, 4),
             "unit": "Mpaths/s",
@@ -711,7 +821,7 @@ def main():
             "kind": "port",
             "sample": f"FP64 oracle, every {args.cpu_row_step}th row ({n_rows} rows x {args.width}) "
                       f"at {args.cpu_spp} spp = {cpaths} paths, {crays} rays, {dt:.1f} s",
-            "cpu_model": cpu_model(),
+            "host": cpu_info(),
         }
         line["speedup_vs_cpu"] = round(value / line["cpu_baseline"]["value"], 2)
     print(json.dumps(line), flush=True)

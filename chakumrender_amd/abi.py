@@ -111,7 +111,7 @@ class chk_camera(C.Structure):
 class chk_render_params(C.Structure):
     _fields_ = [("spp", C.c_uint32), ("sample_offset", C.c_uint32), ("max_depth", C.c_uint32),
                 ("mis_mode", C.c_uint32), ("jitter_mode", C.c_uint32), ("seed_lo", C.c_uint32),
-                ("seed_hi", C.c_uint32), ("flags", C.c_uint32)]
+                ("seed_hi", C.c_uint32), ("flags", C.c_uint32), ("window", C.c_uint32 * 4)]
 
 
 class chk_scene_info(C.Structure):
@@ -142,6 +142,17 @@ class chk_train_params(C.Structure):
                 ("lr", C.c_float), ("flags", C.c_uint32)]
 
 
+class chk_train_record(C.Structure):
+    _fields_ = [("light", C.c_uint32), ("gl", C.c_float), ("pl", C.c_float), ("pbl", C.c_float),
+                ("fb", C.c_float), ("le", C.c_float), ("plb", C.c_float), ("pbb", C.c_float),
+                ("sh_slot", C.c_uint32), ("light_b", C.c_uint32)]
+
+
+# the same record as a numpy dtype (arrays of records for chk_proposal_loss_grad)
+TRAIN_RECORD_DTYPE = np.dtype([(n, "<u4" if t is C.c_uint32 else "<f4")
+                               for n, t in chk_train_record._fields_])
+
+
 class chk_render_stats(C.Structure):
     _fields_ = [("paths", C.c_uint64), ("rays_extend", C.c_uint64), ("rays_shadow", C.c_uint64),
                 ("node_visits", C.c_uint64), ("tri_tests", C.c_uint64),
@@ -157,7 +168,7 @@ class chk_direct_params(C.Structure):
 
 
 assert C.sizeof(chk_material) == 56
-assert C.sizeof(chk_render_params) == 32
+assert C.sizeof(chk_render_params) == 48
 
 # flags (chakum_types.h, chakum_c.h)
 RENDER_PHYSICAL_EMITTER = 0x1
@@ -230,6 +241,7 @@ SIGNATURES = {
     "chk_proposal_eval": (C.c_int, [_vp, _vp, _u32, _vp, _vp, _vp, _vp]),
     "chk_proposal_grad": (C.c_int, [_vp, _vp, _vp, _u32, _vp, _vp]),
     "chk_proposal_adam": (C.c_int, [_vp, _vp, C.c_float, _vp]),
+    "chk_proposal_loss_grad": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _u32, _u32, _vp, _vp]),
 }
 N_PROPOSAL_PARAMS = 14625
 
@@ -292,6 +304,8 @@ def params_struct(p) -> chk_render_params:
         r.seed_lo = seed & 0xFFFFFFFF
         r.seed_hi = (seed >> 32) & 0xFFFFFFFF
     r.flags = int(d.get("flags", 0))
+    for i, v in enumerate(d.get("window", (0, 0, 0, 0))):
+        r.window[i] = int(v)
     return r
 
 
@@ -391,7 +405,7 @@ class DeviceScene:
         self.h = h
 
     def set_trace_stack(self, entries: int):
-        """chk_scene_set_trace_stack: 8 or 32 LDS stack entries per lane."""
+        """chk_scene_set_trace_stack: 8, 16 or 32 LDS stack entries per lane."""
         check(self.L.chk_scene_set_trace_stack(self.h, int(entries)), "chk_scene_set_trace_stack")
 
     def info(self) -> dict:

@@ -623,8 +623,8 @@ chk_status chk_scene_destroy(chk_scene_t sc) {
 }
 
 chk_status chk_scene_set_trace_stack(chk_scene_t sc, uint32_t entries) {
-  if (!sc || (entries != 8 && entries != 32)) {
-    set_error("chk_scene_set_trace_stack: null scene, or entries not 8 or 32");
+  if (!sc || (entries != 8 && entries != 16 && entries != 32)) {
+    set_error("chk_scene_set_trace_stack: null scene, or entries not 8, 16 or 32");
     return CHK_EINVAL;
   }
   sc->shallow_stack = (int)entries;

@@ -1524,6 +1524,11 @@ chk_status chk_render_shade(chk_scene_t sc, const chk_camera *cam, const chk_ren
     set_error("chk_render_shade: unknown shader");
     return CHK_EINVAL;
   }
+  if (P->window[0] | P->window[1] | P->window[2] | P->window[3]) {
+    set_error("chk_render_shade: params->window is for chk_render; the shaders render whole "
+              "images");
+    return CHK_EINVAL;
+  }
   const bool implicit = shader == CHK_SHADE_IMPLICIT;
   chk_direct_params dp{};
   if (!implicit) {

@@ -135,7 +135,7 @@ struct chk_scene_s {
   bool extras = false;
   void *env_mem = nullptr;  // environment image and tables
   // the first non-emitter material without a BSDF sampler (-1: none)
-  int shallow_stack = 32;  // chk_scene_set_trace_stack (8 or 32)
+  int shallow_stack = 16;  // chk_scene_set_trace_stack (8, 16 or 32)
   // process-unique creation number: a batch's cached graphs are keyed on it,
   // so a new scene at a freed scene's address never replays them
   uint64_t gen = 0;
@@ -190,7 +190,7 @@ struct TraceShape {
   // and the rays that overflow it are traced again with the full stack.
   uint32_t *ovf = nullptr;
   uint32_t ovf_cap = 0;
-  int shallow = 32;  // the shorter stack (8 only for single-scene 128-thread launches)
+  int shallow = 16;  // the shorter stack: 16 or 32 (8 only for single-scene 128-thread launches)
 };
 // Words of an overflow list for n ray slots.
 inline size_t ovf_words(uint32_t n) { return 64 + (size_t)n; }

@@ -737,13 +737,16 @@ chk_status proposal_adam(chk_proposal_s *p, const float *grad, float lr, hipStre
 uint32_t proposal_n_lights(const chk_proposal_s *p) { return p->n_lights; }
 const chk_ctx_s *proposal_ctx(const chk_proposal_s *p) { return p->ctx; }
 
-// One online training step on `batch` records of a finished bounce: rows
-// b * stride + offset of the records in segment order (stride >= 1 spreads the
-// batch over the whole image).
-chk_status proposal_train_step(chk_proposal_s *p, const SegRange &R, const TrainRec *rec,
+// The loss gradient with respect to the 33 logits of `batch` records of a
+// finished bounce (rows b * stride + offset of the records in segment order;
+// stride >= 1 spreads the batch over the whole image) at the current
+// parameters: *bin_out the batch's network inputs, *dl_out its dlogits
+// (batch x 33), both in the proposal's training buffer.
+chk_status proposal_train_rows(chk_proposal_s *p, const SegRange &R, const TrainRec *rec,
                                const float *inputs, const float *probs, const float *mix,
                                const uint8_t *s_occ, uint32_t batch, uint32_t stride,
-                               uint32_t offset, float lr, uint32_t flags, hipStream_t st) {
+                               uint32_t offset, uint32_t flags, hipStream_t st, float **bin_out,
+                               float **dl_out, float **grad_out) {
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
   const size_t b_in = al((size_t)batch * 7 * 4), b_dl = al((size_t)batch * 33 * 4);
   const size_t b_w = al((size_t)batch * 4), b_pr = al((size_t)batch * 32 * 4);
@@ -796,7 +799,22 @@ chk_status proposal_train_step(chk_proposal_s *p, const SegRange &R, const Train
   hipLaunchKernelGGL(k_train_norm, dim3(1), dim3(1024), 0, st, wa, wh, batch, scale);
   hipLaunchKernelGGL(k_train_scale, g, dim3(256), 0, st, dl, batch, p->n_lights, scale);
   CHK_HIP(hipGetLastError());
-  chk_status s = proposal_grad(p, bin, dl, batch, grad, st);
+  *bin_out = bin;
+  *dl_out = dl;
+  *grad_out = grad;
+  return CHK_OK;
+}
+
+// One online training step on `batch` records of a finished bounce.
+chk_status proposal_train_step(chk_proposal_s *p, const SegRange &R, const TrainRec *rec,
+                               const float *inputs, const float *probs, const float *mix,
+                               const uint8_t *s_occ, uint32_t batch, uint32_t stride,
+                               uint32_t offset, float lr, uint32_t flags, hipStream_t st) {
+  float *bin = nullptr, *dl = nullptr, *grad = nullptr;
+  chk_status s = proposal_train_rows(p, R, rec, inputs, probs, mix, s_occ, batch, stride, offset,
+                                     flags, st, &bin, &dl, &grad);
+  if (s != CHK_OK) return s;
+  s = proposal_grad(p, bin, dl, batch, grad, st);
   if (s != CHK_OK) return s;
   return proposal_adam(p, grad, lr, st);
 }
@@ -917,6 +935,35 @@ chk_status chk_proposal_grad(chk_proposal_t p, const float *inputs, const float 
   }
   CHK_HIP(hipSetDevice(p->ctx->device));
   return proposal_grad(p, inputs, dlogits, k, grad, (hipStream_t)stream);
+}
+
+chk_status chk_proposal_loss_grad(chk_proposal_t p, const chk_train_record *rec,
+                                  const float *inputs, const float *probs,
+                                  const uint8_t *occluded, uint32_t k, uint32_t flags,
+                                  float *dlogits, void *stream) {
+  CHK_TRY_BEGIN
+  static_assert(sizeof(chk_train_record) == sizeof(TrainRec), "record layouts");
+  if (!p || (k && (!rec || !inputs || !probs || !occluded || !dlogits))) {
+    set_error("chk_proposal_loss_grad: null argument");
+    return CHK_EINVAL;
+  }
+  if (k > (1u << 16) ||
+      (flags & ~(CHK_TRAIN_FREEZE_MIX | CHK_TRAIN_FREEZE_LIGHTS | CHK_TRAIN_LIGHTS_CHI2))) {
+    set_error("chk_proposal_loss_grad: at most 65536 records and known CHK_TRAIN_* flags");
+    return CHK_EINVAL;
+  }
+  if (k == 0) return CHK_OK;
+  CHK_HIP(hipSetDevice(p->ctx->device));
+  const hipStream_t st = (hipStream_t)stream;
+  float *bin = nullptr, *dl = nullptr, *grad = nullptr;
+  const chk_status s = proposal_train_rows(p, linear_range(k), (const TrainRec *)rec, inputs,
+                                           probs, nullptr, occluded, k, 1, 0, flags, st, &bin,
+                                           &dl, &grad);
+  if (s != CHK_OK) return s;
+  CHK_HIP(hipMemcpyAsync(dlogits, dl, (size_t)k * 33 * sizeof(float), hipMemcpyDeviceToDevice,
+                         st));
+  return CHK_OK;
+  CHK_TRY_END
 }
 
 chk_status chk_proposal_adam(chk_proposal_t p, const float *grad, float lr, void *stream) {

@@ -32,7 +32,7 @@
 namespace chk {
 namespace {
 
-constexpr uint32_t kChunkPaths = 1u << 22;  // paths in flight per chunk
+constexpr uint32_t kChunkPaths = 1u << 24;  // paths in flight per chunk
 constexpr int kShadeBlock = 256;
 
 // control words in the work buffer: per-segment queue counters, each
@@ -70,9 +70,18 @@ struct RenderArgs {
   DeviceScene S;
   CamArgs cam;
   uint32_t spp, sample_offset, max_depth, jitter_mode, flags, key0, key1, req;
-  uint32_t pix0, n_pix, p0;
+  uint32_t pix0, n_pix, p0;  // pixels: indices into the window, row-major
+  uint32_t win_x0, win_y0, win_w, win_h;
   float inv_nl;
 };
+
+// Image pixel (row-major index into the W x H image, row 0 at the top) of
+// window pixel li: keys and accumulators use it, so a window renders its
+// pixels exactly as the whole image does.
+__host__ __device__ inline uint32_t image_pixel(const RenderArgs &A, uint32_t li) {
+  if (A.win_w == A.cam.W) return A.win_y0 * A.cam.W + li;  // full rows
+  return (A.win_y0 + li / A.win_w) * A.cam.W + A.win_x0 + li % A.win_w;
+}
 
 struct Buffers {
   uint32_t *ctrl;
@@ -242,7 +251,7 @@ __global__ void k_camera(RenderArgs A, uint32_t segcap, uint32_t total,
   const uint32_t r = p / per;
   const size_t slot = (size_t)r * segcap + (p - r * per);
   if (q_seg) q_seg[slot] = seg;
-  const uint32_t idx = A.pix0 + lp / A.spp;
+  const uint32_t idx = image_pixel(A, A.pix0 + lp / A.spp);
   const uint32_t s = A.sample_offset + lp % A.spp;
   const uint32_t row = idx / A.cam.W, x = idx % A.cam.W;
   const uint32_t y = A.cam.H - 1u - row;
@@ -366,7 +375,7 @@ __device__ __forceinline__ void shade_vertex(const RenderArgs &A, uint32_t depth
       const f3 wo = d * -1.0f;
       const f3 wo_ref = normalize(n * (2.0f * dot(n, wo)) + d);  // lightSources.cpp:44-45
       const f3 org = offset_ray(X, n);
-      const uint32_t pix = A.pix0 + (pid - A.p0) / A.spp;
+      const uint32_t pix = image_pixel(A, A.pix0 + (pid - A.p0) / A.spp);
       const uint32_t smp = A.sample_offset + (pid - A.p0) % A.spp;
       const float c = LEARNED ? P.mix[i] : 0.5f;
       // ---- next-event estimation: one point on one mesh light
@@ -598,7 +607,7 @@ __global__ void k_accumulate(RenderArgs A, const float4 *__restrict__ path_L,
     q1 += b * b;
     q2 += c * c;
   }
-  const size_t o = 3 * ((size_t)A.pix0 + q);
+  const size_t o = 3 * (size_t)image_pixel(A, A.pix0 + q);
   accum[o] += s0;
   accum[o + 1] += s1;
   accum[o + 2] += s2;
@@ -698,6 +707,22 @@ chk_status make_args(const chk_scene_s *sc, const chk_camera *cam, const chk_ren
   A.key1 = P->seed_hi;
   A.req = 0;
   A.inv_nl = sc->d.n_lights ? 1.0f / (float)sc->d.n_lights : 0.0f;
+  const uint32_t *w = P->window;
+  if ((w[0] | w[1] | w[2] | w[3]) == 0) {
+    A.win_x0 = A.win_y0 = 0;
+    A.win_w = cam->width;
+    A.win_h = cam->height;
+  } else {
+    if (w[2] == 0 || w[3] == 0 || w[0] >= cam->width || w[1] >= cam->height ||
+        w[2] > cam->width - w[0] || w[3] > cam->height - w[1]) {
+      set_error(std::string(who) + ": window (x0, y0, w, h) is empty or leaves the image");
+      return CHK_EINVAL;
+    }
+    A.win_x0 = w[0];
+    A.win_y0 = w[1];
+    A.win_w = w[2];
+    A.win_h = w[3];
+  }
   return CHK_OK;
 }
 
@@ -751,11 +776,17 @@ chk_status render_impl(chk_scene_s *sc, const chk_camera *cam, const chk_render_
       training = true;
     }
   }
-  const uint64_t n_pix_total = (uint64_t)cam->width * cam->height;
-  if (n_pix_total >= (1ull << 32)) {
+  if ((uint64_t)cam->width * cam->height >= (1ull << 32)) {
     set_error("chk_render: image too large");
     return CHK_ECAPACITY;
   }
+  RenderArgs A;
+  {
+    const chk_status as = make_args(sc, cam, P, A, "chk_render");
+    if (as != CHK_OK) return as;
+  }
+  // pixels rendered: the window's, in row-major window order
+  const uint64_t n_pix_total = (uint64_t)A.win_w * A.win_h;
   const uint32_t pix_per_chunk = std::max<uint32_t>(1u, chunk_paths / P->spp);
   const uint32_t cap = (uint32_t)std::min<uint64_t>((uint64_t)pix_per_chunk * P->spp,
                                                    n_pix_total * P->spp);
@@ -798,12 +829,6 @@ chk_status render_impl(chk_scene_s *sc, const chk_camera *cam, const chk_render_
     LB = lcarve(sc->lwork, qcap);
   }
 
-  RenderArgs A;
-  {
-    const chk_status as = make_args(sc, cam, P, A, "chk_render");
-    if (as != CHK_OK) return as;
-  }
-
   Profiler prof;
   prof.on = (P->flags & CHK_RENDER_PROFILE) != 0;
   prof.st = st;
@@ -829,7 +854,8 @@ chk_status render_impl(chk_scene_s *sc, const chk_camera *cam, const chk_render_
       const uint32_t in = depth & 1u, out = in ^ 1u;
       hipLaunchKernelGGL(k_reset_bounce, dim3(1), dim3(64), 0, st, B.ctrl, qbase(out));
       prof.begin(0);
-      hipError_t e = launch_closest(sc, TraceShape{sc->stack_size, 128, B.ovf, qcap, sc->shallow_stack}, B.q_ray[in],
+      hipError_t e = launch_closest(sc, TraceShape{sc->stack_size, 128, B.ovf, qcap, sc->shallow_stack},
+                                    B.q_ray[in],
                                     SegRange{B.ctrl + qbase(in), 0u, segcap}, B.hits, true, count,
                                     B.stats + kNodeVisits, SlotScenes{}, st);
       prof.end();
@@ -890,7 +916,8 @@ chk_status render_impl(chk_scene_s *sc, const chk_camera *cam, const chk_render_
       }
       if (depth < P->max_depth) {
         prof.begin(2);
-        e = launch_any(sc, TraceShape{sc->stack_size, 128, B.ovf, qcap, sc->shallow_stack}, B.s_ray,
+        e = launch_any(sc, TraceShape{sc->stack_size, 128, B.ovf, qcap, sc->shallow_stack},
+                       B.s_ray,
                        SegRange{B.ctrl + kQS, 0u, segcap}, training ? LB.s_occ : nullptr, B.s_L,
                        B.path_L, true, count, B.stats + kShNodeVisits, SlotScenes{}, st);
         prof.end();
@@ -1009,17 +1036,17 @@ void batch_state_free(void *p) { delete static_cast<BatchState *>(p); }
 chk_status batch_impl(chk_ctx_s *ctx, const chk_request *R, uint32_t n,
                       const chk_batch_tuning *tuning, double *latency_ms, chk_batch_stats *stats,
                       hipStream_t st) {
-  chk_batch_tuning T{1u << 22, 128, 256, 0};
+  chk_batch_tuning T{1u << 24, 128, 256, 0};
   if (tuning) {
     T = *tuning;
-    if (T.chunk_paths == 0) T.chunk_paths = 1u << 22;
+    if (T.chunk_paths == 0) T.chunk_paths = 1u << 24;
     if (T.trace_block == 0) T.trace_block = 128;
     if (T.shade_block == 0) T.shade_block = 256;
   }
-  if (T.chunk_paths < (1u << 16) || T.chunk_paths > (1u << 24) ||
+  if (T.chunk_paths < (1u << 16) || T.chunk_paths > (1u << 26) ||
       (T.trace_block != 64 && T.trace_block != 128 && T.trace_block != 256) ||
       (T.shade_block != 128 && T.shade_block != 256 && T.shade_block != 512)) {
-    set_error("chk_render_batch: tuning wants chunk_paths 2^16..2^24, trace_block 64/128/256, "
+    set_error("chk_render_batch: tuning wants chunk_paths 2^16..2^26, trace_block 64/128/256, "
               "shade_block 128/256/512");
     return CHK_EINVAL;
   }
@@ -1061,8 +1088,8 @@ chk_status batch_impl(chk_ctx_s *ctx, const chk_request *R, uint32_t n,
   std::vector<uint32_t> order(n);
   for (uint32_t i = 0; i < n; ++i) order[i] = i;
   std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
-    return (uint64_t)args[a].cam.W * args[a].cam.H * args[a].spp <
-           (uint64_t)args[b].cam.W * args[b].cam.H * args[b].spp;
+    return (uint64_t)args[a].win_w * args[a].win_h * args[a].spp <
+           (uint64_t)args[b].win_w * args[b].win_h * args[b].spp;
   });
   std::vector<BatchSeg> segs;
   std::vector<BatchChunk> chunks;
@@ -1071,7 +1098,7 @@ chk_status batch_impl(chk_ctx_s *ctx, const chk_request *R, uint32_t n,
   {
     BatchChunk cur{0, 0, 0};
     for (uint32_t i : order) {
-      const uint32_t npix = args[i].cam.W * args[i].cam.H, spp = args[i].spp;
+      const uint32_t npix = args[i].win_w * args[i].win_h, spp = args[i].spp;
       uint32_t pix = 0;
       while (pix < npix) {
         const uint32_t room = (T.chunk_paths - cur.total) / spp;
@@ -1146,7 +1173,7 @@ chk_status batch_impl(chk_ctx_s *ctx, const chk_request *R, uint32_t n,
                     hipMemcpyHostToDevice));
   CHK_HIP(hipMemcpy(d_scene, seg_scene.data(), segs.size() * sizeof(DeviceScene),
                     hipMemcpyHostToDevice));
-  const TraceShape shape{stack, (int)T.trace_block, B.ovf, qcap};
+  const TraceShape shape{stack, (int)T.trace_block, B.ovf, qcap, 16};
   auto qbase = [](uint32_t q) { return q == 0 ? (uint32_t)kQ0 : (uint32_t)kQ1; };
   // ---- the launches of one chunk
   auto emit_chunk = [&](uint32_t c, hipStream_t s) -> hipError_t {
@@ -1281,7 +1308,7 @@ chk_status batch_impl(chk_ctx_s *ctx, const chk_request *R, uint32_t n,
 chk_status batch_autotune(chk_ctx_s *ctx, const chk_request *R, uint32_t n,
                           chk_batch_tuning *best, double *best_ms, hipStream_t st) {
   if (n == 0) {
-    *best = chk_batch_tuning{1u << 22, 128, 256, 0};
+    *best = chk_batch_tuning{1u << 24, 128, 256, 0};
     return CHK_OK;
   }
   size_t maxpix = 0;
@@ -1299,11 +1326,11 @@ chk_status batch_autotune(chk_ctx_s *ctx, const chk_request *R, uint32_t n,
     q.accum = scratch;
     q.sumsq = scratch + maxpix * 3;
   }
-  const uint32_t chunk_c[] = {1u << 21, 1u << 22, 1u << 23};
+  const uint32_t chunk_c[] = {1u << 22, 1u << 23, 1u << 24, 1u << 25};
   const uint32_t tb_c[] = {64, 128, 256};
   const uint32_t sb_c[] = {128, 256, 512};
   double bms = 1e30;
-  chk_batch_tuning bt{1u << 22, 128, 256, 0};
+  chk_batch_tuning bt{1u << 24, 128, 256, 0};
   chk_status s = CHK_OK;
   for (uint32_t c : chunk_c)
     for (uint32_t tb : tb_c)

@@ -178,6 +178,12 @@ __device__ inline void finish_ray(const DeviceScene &S, size_t i, float4 ot, flo
   }
 }
 
+// One ray per thread: block b traces rays q * BS ... of segment
+// r = b % kSegs, q = b / kSegs; the grid covers the segments' capacity and
+// blocks past a segment's live count exit at once.  (A persistent grid whose
+// waves fetched batches from a per-segment cursor measured no faster once
+// chunks hold 2^24 paths: the idle blocks of a launch cost less than the
+// fetch loop.)
 template <bool ANY, bool COUNT, int STACK, int BS, bool MULTI>
 __global__ __launch_bounds__(BS) void k_trace(DeviceScene S0, const float4 *__restrict__ rays,
                                               SegRange W, float4 *__restrict__ hits,
@@ -315,10 +321,10 @@ __global__ void k_zero_u32(uint32_t *p, uint32_t n) {
 }
 
 template <bool ANY, bool COUNT, int STACK, int BS, bool MULTI>
-hipError_t launch_t(const chk_scene_s *s, const float4 *rays, const SegRange &W, float4 *hits,
-                    uint8_t *occ, const float4 *sL, float4 *pL, bool leaf_ids,
-                    unsigned long long *counters, const SlotScenes &M, hipStream_t st,
-                    uint32_t *ovf, uint32_t ovf_cap) {
+hipError_t launch_t(const chk_scene_s *s, const TraceShape &T, const float4 *rays,
+                    const SegRange &W, float4 *hits, uint8_t *occ, const float4 *sL, float4 *pL,
+                    bool leaf_ids, unsigned long long *counters, const SlotScenes &M,
+                    hipStream_t st, uint32_t *ovf, uint32_t ovf_cap) {
   const uint32_t grid = kSegs * (W.segcap / BS);
   if (grid == 0) return hipSuccess;
   hipLaunchKernelGGL((k_trace<ANY, COUNT, STACK, BS, MULTI>), dim3(grid), dim3(BS), 0, st,
@@ -327,22 +333,33 @@ hipError_t launch_t(const chk_scene_s *s, const float4 *rays, const SegRange &W,
   return hipGetLastError();
 }
 
-template <bool ANY, bool COUNT, bool MULTI, int BS>
-hipError_t launch_s(const chk_scene_s *s, int stack, const float4 *rays, const SegRange &W,
-                    float4 *hits, uint8_t *occ, const float4 *sL, float4 *pL, bool leaf_ids,
-                    unsigned long long *counters, const SlotScenes &M, hipStream_t st,
-                    uint32_t *ovf = nullptr, uint32_t ovf_cap = 0) {
-  switch (stack) {
-    case 32:
-      return launch_t<ANY, COUNT, 32, BS, MULTI>(s, rays, W, hits, occ, sL, pL, leaf_ids,
-                                                 counters, M, st, ovf, ovf_cap);
-    case 64:
-      return launch_t<ANY, COUNT, 64, BS, MULTI>(s, rays, W, hits, occ, sL, pL, leaf_ids,
-                                                 counters, M, st, ovf, ovf_cap);
-    default:
-      return launch_t<ANY, COUNT, 96, BS, MULTI>(s, rays, W, hits, occ, sL, pL, leaf_ids,
-                                                 counters, M, st, ovf, ovf_cap);
+// The traversal at a given stack depth (8 only for single-scene 128-thread
+// launches) and block size.
+template <bool ANY, bool COUNT, bool MULTI>
+hipError_t launch_s(const chk_scene_s *s, int stack, const TraceShape &T, const float4 *rays,
+                    const SegRange &W, float4 *hits, uint8_t *occ, const float4 *sL, float4 *pL,
+                    bool leaf_ids, unsigned long long *counters, const SlotScenes &M,
+                    hipStream_t st, uint32_t *ovf = nullptr, uint32_t ovf_cap = 0) {
+#define CHK_LAUNCH(STK, BSZ)                                                                 \
+  return launch_t<ANY, COUNT, STK, BSZ, MULTI>(s, T, rays, W, hits, occ, sL, pL, leaf_ids,    \
+                                               counters, M, st, ovf, ovf_cap)
+#define CHK_BLOCKS(STK)            \
+  switch (T.block) {               \
+    case 64: CHK_LAUNCH(STK, 64);  \
+    case 256: CHK_LAUNCH(STK, 256); \
+    default: CHK_LAUNCH(STK, 128); \
   }
+  if (!MULTI && !COUNT && stack == 8 && T.block == 128) {
+    CHK_LAUNCH(8, 128);
+  }
+  switch (stack) {
+    case 16: CHK_BLOCKS(16);
+    case 32: CHK_BLOCKS(32);
+    case 64: CHK_BLOCKS(64);
+    default: CHK_BLOCKS(96);
+  }
+#undef CHK_BLOCKS
+#undef CHK_LAUNCH
 }
 
 // The listed rays again with the tree's full stack (fixed grid: one block
@@ -354,7 +371,10 @@ hipError_t launch_retry(const chk_scene_s *s, int stack, const float4 *rays, flo
                         const uint32_t *ovf, uint32_t ovf_cap) {
   const DeviceScene S = s ? s->d : DeviceScene{};
   const dim3 g(256), b(128);
-  if (stack <= 64)
+  if (stack <= 32)
+    hipLaunchKernelGGL((k_trace_retry<ANY, 32, 128, MULTI>), g, b, 0, st, S, rays, hits, occ, sL,
+                       pL, leaf_ids ? 1 : 0, counters, M, ovf, ovf_cap);
+  else if (stack <= 64)
     hipLaunchKernelGGL((k_trace_retry<ANY, 64, 128, MULTI>), g, b, 0, st, S, rays, hits, occ, sL,
                        pL, leaf_ids ? 1 : 0, counters, M, ovf, ovf_cap);
   else
@@ -369,67 +389,30 @@ hipError_t launch_kind(const chk_scene_s *s, const TraceShape &T, const float4 *
                        float4 *pL, bool leaf_ids, bool count, unsigned long long *counters,
                        const SlotScenes &M, hipStream_t st) {
   const bool multi = M.scene != nullptr;
-  if (count)  // the counted render (work model): single scene, 128-thread blocks
-    return launch_s<ANY, true, false, 128>(s, T.stack, rays, W, hits, occ, sL, pL, leaf_ids,
-                                           counters, M, st);
-  const int shallow = (T.shallow == 8 && !multi && T.block == 128) ? 8 : 32;
+  if (count)  // the counted render (work model): single scene, 128-thread blocks, full stack
+    return launch_s<ANY, true, false>(s, T.stack, TraceShape{T.stack, 128}, rays, W, hits, occ,
+                                      sL, pL, leaf_ids, counters, M, st);
+  int shallow = T.shallow;
+  if (shallow == 8 && (multi || T.block != 128)) shallow = 16;
   if (T.ovf && T.stack > shallow) {
-    // deep tree: a 32-entry stack for every ray (twice the occupancy), the
-    // rare overflowing ray again with the full stack
+    // a stack shorter than the tree's worst case for every ray (more waves
+    // per CU), the rare overflowing ray again with the full stack
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, T.ovf, 1u);
-    hipError_t e;
-    if (shallow == 8) {
-      e = launch_t<ANY, false, 8, 128, false>(s, rays, W, hits, occ, sL, pL, leaf_ids, counters,
-                                               M, st, T.ovf, T.ovf_cap);
-      if (e != hipSuccess) return e;
-      return launch_retry<ANY, false>(s, T.stack, rays, hits, occ, sL, pL, leaf_ids, counters, M,
-                                      st, T.ovf, T.ovf_cap);
-    }
-    if (multi) {
-      e = T.block == 64    ? launch_s<ANY, false, true, 64>(s, 32, rays, W, hits, occ, sL, pL,
-                                                           leaf_ids, counters, M, st, T.ovf,
-                                                           T.ovf_cap)
-          : T.block == 256 ? launch_s<ANY, false, true, 256>(s, 32, rays, W, hits, occ, sL, pL,
-                                                            leaf_ids, counters, M, st, T.ovf,
-                                                            T.ovf_cap)
-                           : launch_s<ANY, false, true, 128>(s, 32, rays, W, hits, occ, sL, pL,
-                                                            leaf_ids, counters, M, st, T.ovf,
-                                                            T.ovf_cap);
-      if (e != hipSuccess) return e;
-      return launch_retry<ANY, true>(s, T.stack, rays, hits, occ, sL, pL, leaf_ids, counters, M,
-                                     st, T.ovf, T.ovf_cap);
-    }
-    e = T.block == 64    ? launch_s<ANY, false, false, 64>(s, 32, rays, W, hits, occ, sL, pL,
-                                                          leaf_ids, counters, M, st, T.ovf,
-                                                          T.ovf_cap)
-        : T.block == 256 ? launch_s<ANY, false, false, 256>(s, 32, rays, W, hits, occ, sL, pL,
-                                                           leaf_ids, counters, M, st, T.ovf,
-                                                           T.ovf_cap)
-                         : launch_s<ANY, false, false, 128>(s, 32, rays, W, hits, occ, sL, pL,
-                                                           leaf_ids, counters, M, st, T.ovf,
-                                                           T.ovf_cap);
+    hipError_t e = multi ? launch_s<ANY, false, true>(s, shallow, T, rays, W, hits, occ, sL, pL,
+                                                      leaf_ids, counters, M, st, T.ovf, T.ovf_cap)
+                         : launch_s<ANY, false, false>(s, shallow, T, rays, W, hits, occ, sL, pL,
+                                                       leaf_ids, counters, M, st, T.ovf,
+                                                       T.ovf_cap);
     if (e != hipSuccess) return e;
-    return launch_retry<ANY, false>(s, T.stack, rays, hits, occ, sL, pL, leaf_ids, counters, M, st,
-                                    T.ovf, T.ovf_cap);
+    return multi ? launch_retry<ANY, true>(s, T.stack, rays, hits, occ, sL, pL, leaf_ids,
+                                           counters, M, st, T.ovf, T.ovf_cap)
+                 : launch_retry<ANY, false>(s, T.stack, rays, hits, occ, sL, pL, leaf_ids,
+                                            counters, M, st, T.ovf, T.ovf_cap);
   }
-  if (multi) {
-    if (T.block == 64)
-      return launch_s<ANY, false, true, 64>(s, T.stack, rays, W, hits, occ, sL, pL, leaf_ids,
-                                            counters, M, st);
-    if (T.block == 256)
-      return launch_s<ANY, false, true, 256>(s, T.stack, rays, W, hits, occ, sL, pL, leaf_ids,
-                                             counters, M, st);
-    return launch_s<ANY, false, true, 128>(s, T.stack, rays, W, hits, occ, sL, pL, leaf_ids,
-                                           counters, M, st);
-  }
-  if (T.block == 64)
-    return launch_s<ANY, false, false, 64>(s, T.stack, rays, W, hits, occ, sL, pL, leaf_ids,
-                                           counters, M, st);
-  if (T.block == 256)
-    return launch_s<ANY, false, false, 256>(s, T.stack, rays, W, hits, occ, sL, pL, leaf_ids,
-                                            counters, M, st);
-  return launch_s<ANY, false, false, 128>(s, T.stack, rays, W, hits, occ, sL, pL, leaf_ids,
-                                          counters, M, st);
+  return multi ? launch_s<ANY, false, true>(s, T.stack, T, rays, W, hits, occ, sL, pL, leaf_ids,
+                                            counters, M, st)
+               : launch_s<ANY, false, false>(s, T.stack, T, rays, W, hits, occ, sL, pL,
+                                             leaf_ids, counters, M, st);
 }
 
 }  // namespace

@@ -321,6 +321,23 @@ class Proposal:
         abi.check(self.L.chk_proposal_grad(self.h, sd.ptr, dd.ptr, k, g.ptr, None), "chk_proposal_grad")
         return g.download((abi.N_PROPOSAL_PARAMS,), np.float32), g
 
+    def loss_grad(self, records, inputs, probs, occluded, flags=0):
+        """chk_proposal_loss_grad: the online-training dlogits (k, 33) of
+        `records` (abi.TRAIN_RECORD_DTYPE) drawn with pmfs `probs` (k, 32)
+        at network inputs `inputs` (k, 7); occluded: the occlusion flags the
+        records' sh_slot index."""
+        rec = np.ascontiguousarray(records, abi.TRAIN_RECORD_DTYPE)
+        k = rec.shape[0]
+        bufs = [DevBuf.from_array(rec),
+                DevBuf.from_array(np.ascontiguousarray(inputs, np.float32).reshape(k, 7)),
+                DevBuf.from_array(np.ascontiguousarray(probs, np.float32).reshape(k, 32)),
+                DevBuf.from_array(np.ascontiguousarray(occluded, np.uint8).reshape(-1))]
+        out = DevBuf(132 * k)
+        abi.check(self.L.chk_proposal_loss_grad(self.h, bufs[0].ptr, bufs[1].ptr, bufs[2].ptr,
+                                                bufs[3].ptr, k, int(flags), out.ptr, None),
+                  "chk_proposal_loss_grad")
+        return out.download((k, 33), np.float32)
+
     def adam(self, grad_buf, lr=1e-3):
         abi.check(self.L.chk_proposal_adam(self.h, grad_buf.ptr, float(lr), None), "chk_proposal_adam")
 

@@ -150,12 +150,12 @@ chk_status chk_scene_create_desc(chk_ctx_t ctx, const chk_scene_desc *desc, chk_
 chk_status chk_scene_destroy(chk_scene_t scene);
 /* Traversal stack per lane (no reference counterpart: the reference's
  * traversal keeps a 64-entry stack and never checks it, bvhAccel.cpp:208).
- * A tree whose worst case needs more than `entries` (8 or 32, default 32)
- * is traced by the renders with `entries` LDS entries per lane, at higher
- * occupancy, and a ray that overflows them is traced again with the full
- * stack; results are the same either way.  chk_render_stats.stack_retries
- * counts those rays.  8 is for testing the second trace: rays of ordinary
- * scenes overflow it. */
+ * A tree whose worst case needs more than `entries` (8, 16 or 32, default
+ * 16) is traced by the renders with `entries` LDS entries per lane, at
+ * higher occupancy, and a ray that overflows them is traced again with the
+ * full stack; results are the same either way.
+ * chk_render_stats.stack_retries counts those rays.  8 is for testing the
+ * second trace: rays of ordinary scenes overflow it. */
 chk_status chk_scene_set_trace_stack(chk_scene_t scene, uint32_t entries);
 chk_status chk_scene_get_info(chk_scene_t scene, chk_scene_info *info);
 
@@ -328,7 +328,7 @@ typedef struct chk_request {
 } chk_request;
 
 typedef struct chk_batch_tuning {
-  uint32_t chunk_paths;     /* paths in flight per chunk, 2^16..2^24 (0: 2^22) */
+  uint32_t chunk_paths;     /* paths in flight per chunk, 2^16..2^26 (0: 2^24) */
   uint32_t trace_block;     /* traversal threads per block: 64, 128, 256 (0: 128) */
   uint32_t shade_block;     /* shading threads per block: 128, 256, 512 (0: 256) */
   uint32_t use_graph;       /* capture each chunk into a hipGraph; the executable
@@ -389,6 +389,43 @@ chk_status chk_proposal_grad(chk_proposal_t p, const float *inputs, const float 
                              uint32_t k, float *grad, void *stream);
 /* One Adam step (beta 0.9 / 0.999, eps 1e-8) with learning rate lr. */
 chk_status chk_proposal_adam(chk_proposal_t p, const float *grad, float lr, void *stream);
+
+/* One training record of chk_render_learned: a vertex's light sample and
+ * BSDF sample as the learned render writes them (40 bytes).
+ *   light     light j drawn from q' (0xFFFFFFFF: no light sample)
+ *   gl        f cos * albedo * radiance of the light sample, before visibility
+ *   pl, pbl   light-sampling (q'_j times the area pdf) and BSDF pdfs of it
+ *   fb        f cos * albedo of the BSDF sample
+ *   le        emitted radiance the BSDF sample's ray found (0: none)
+ *   plb, pbb  light-sampling and BSDF pdfs of the BSDF sample
+ *   sh_slot   index of the light sample's occlusion flag (0xFFFFFFFF: none)
+ *   light_b   light the BSDF sample hit (0xFFFFFFFF: none) */
+typedef struct chk_train_record {
+  uint32_t light;
+  float gl, pl, pbl, fb, le, plb, pbb;
+  uint32_t sh_slot, light_b;
+} chk_train_record;
+
+/* The loss gradient chk_render_learned's online training feeds to
+ * chk_proposal_grad, for k records at the proposal's current parameters
+ * (no reference counterpart).  Per record, with q'_c and c the current
+ * network's outputs for inputs[i] and q'_s = probs[i] the pmf the samples
+ * were drawn with:
+ *   light head: a = gl V / pl (V = 1 - occluded[sh_slot]) times
+ *     grad_z(-log q'_c,j) -- the KL divergence from the light-contribution
+ *     distribution to q'_c -- or, with CHK_TRAIN_LIGHTS_CHI2, a^2 q'_s,j /
+ *     q'_c,j times it (the second moment of the light estimate);
+ *   mixing head: d F^2 / dz_32 of the vertex's two-sample estimate
+ *     F = c g_l / (c pl + (1-c) pbl) + (1-c) g_b / (c plb + (1-c) pbb);
+ *   the light head's rows divided by the batch's sum of their weights (a,
+ *   or a^2 q'_s,j / q'_c,j), the mixing head's by the batch's sum of
+ *   |d F^2 / dz_32|.
+ * rec, inputs (k x 7), probs (k x 32), occluded (indexed by sh_slot),
+ * dlogits (k x 33, out): dev.  k <= 65536; flags: CHK_TRAIN_*. */
+chk_status chk_proposal_loss_grad(chk_proposal_t p, const chk_train_record *rec,
+                                  const float *inputs, const float *probs,
+                                  const uint8_t *occluded, uint32_t k, uint32_t flags,
+                                  float *dlogits, void *stream);
 
 /* Online training inside chk_render_learned. */
 typedef struct chk_train_params {

@@ -149,6 +149,10 @@ typedef struct chk_render_params {
   uint32_t seed_lo;         /* Philox key */
   uint32_t seed_hi;
   uint32_t flags;           /* CHK_RENDER_* bits */
+  uint32_t window[4];       /* x0, y0, w, h: render only this rectangle of the image (row 0
+                               at the top; each pixel exactly as in a whole-image render);
+                               all zero = the whole image.  chk_render and its count, learned,
+                               no-allocation and batched forms */
 } chk_render_params;
 
 /* A directly seen emitter returns its radiance (physical) instead of

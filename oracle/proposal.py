@@ -123,3 +123,129 @@ def init_params(seed):
         for i in range(64 * 64):
             p[base + i] = np.float32((2.0 * nxt() - 1.0) * lim)
     return p
+
+
+# ------------------------------------------------------------ online-training loss
+# Restatement of the learned render's training rows (chk_mlp.hip k_train_rows,
+# k_train_norm, k_train_scale; no reference counterpart: the reference gives
+# every light the same sample count, mis.h:9).  Records are arrays of the
+# library's chk_train_record (chakumrender_amd.abi.TRAIN_RECORD_DTYPE).
+NO_INDEX = 0xFFFFFFFF
+TRAIN_FREEZE_MIX, TRAIN_FREEZE_LIGHTS, TRAIN_LIGHTS_CHI2 = 1, 2, 4
+
+
+def qprime(z, n_lights):
+    """Defensive pmf q' = (1 - eps) softmax(z[:L]) + eps / L of logits z (k, 33)."""
+    zl = np.asarray(z, np.float64)[:, :n_lights]
+    e = np.exp(zl - zl.max(1, keepdims=True))
+    return (1 - EPS_Q) * e / e.sum(1, keepdims=True) + EPS_Q / n_lights
+
+
+def mixing(z32):
+    """Mixing weight c = eps + (1 - 2 eps) sigmoid(z_32)."""
+    return EPS_C + (1 - 2 * EPS_C) / (1 + np.exp(-np.asarray(z32, np.float64)))
+
+
+def record_terms(rec, occluded):
+    """Per record: visible light-sample value g_l V, its pdfs (p_l, p_b),
+    the BSDF sample's value g_b = f cos * L_e and its pdfs (p_l, p_b)."""
+    rec = np.asarray(rec)
+    occ = np.asarray(occluded, np.uint8)
+    slot = rec["sh_slot"].astype(np.int64)
+    has = slot != NO_INDEX
+    vis = np.zeros(len(rec))
+    vis[has] = 1.0 - occ[slot[has]]
+    gl = rec["gl"].astype(np.float64) * vis
+    gb = rec["fb"].astype(np.float64) * rec["le"].astype(np.float64)
+    return (gl, rec["pl"].astype(np.float64), rec["pbl"].astype(np.float64), gb,
+            rec["plb"].astype(np.float64), rec["pbb"].astype(np.float64))
+
+
+def train_rows(rec, qs, qc, c, occluded, n_lights, flags=0):
+    """Unnormalised loss gradients (k, 33) with respect to the logits at the
+    current proposal (pmf qc (k, >= L), mixing weight c (k,)), and the two
+    heads' per-row weights (light: the importance weight w; mixing: |row 32|).
+    qs: the pmf each record's light was drawn with."""
+    k = len(rec)
+    L = n_lights
+    rows = np.zeros((k, 33))
+    wa, wh = np.zeros(k), np.zeros(k)
+    qs = np.asarray(qs, np.float64)
+    qc = np.asarray(qc, np.float64)
+    c = np.asarray(c, np.float64)
+    gl, pl, pbl, gb, plb, pbb = record_terms(rec, occluded)
+    light = np.asarray(rec)["light"].astype(np.int64)
+    for b in range(k):
+        j = light[b]
+        if j >= L:
+            continue
+        if not flags & TRAIN_FREEZE_LIGHTS:
+            a = gl[b] / pl[b] if pl[b] > 0 else 0.0
+            w = a * a * qs[b, j] / qc[b, j] if flags & TRAIN_LIGHTS_CHI2 else a
+            if w > 0 and np.isfinite(w):
+                q = (qc[b, :L] - EPS_Q / L) / (1 - EPS_Q)  # the softmax part
+                onehot = np.zeros(L)
+                onehot[j] = 1.0
+                rows[b, :L] = -w * (1 - EPS_Q) * q[j] / qc[b, j] * (onehot - q)
+                wa[b] = w
+        if not flags & TRAIN_FREEZE_MIX:
+            cb = c[b]
+            pcl = cb * pl[b] + (1 - cb) * pbl[b]
+            pcb = cb * plb[b] + (1 - cb) * pbb[b]
+            al = gl[b] / pcl if (gl[b] > 0 and pcl > 0) else 0.0
+            ab = gb[b] / pcb if (gb[b] > 0 and pcb > 0) else 0.0
+            F = cb * al + (1 - cb) * ab
+            dF = al - ab
+            if al > 0:
+                dF -= cb * al * (pl[b] - pbl[b]) / pcl
+            if ab > 0:
+                dF -= (1 - cb) * ab * (plb[b] - pbb[b]) / pcb
+            H = 2 * F * dF
+            if np.isfinite(H) and H != 0:
+                sg = (cb - EPS_C) / (1 - 2 * EPS_C)
+                rows[b, 32] = H * (1 - 2 * EPS_C) * sg * (1 - sg)
+                wh[b] = abs(rows[b, 32])
+    return rows, wa, wh
+
+
+def train_dlogits(rec, qs, qc, c, occluded, n_lights, flags=0):
+    """The dlogits a training step feeds to the backward pass: each head's
+    rows divided by the batch's sum of that head's row weights."""
+    rows, wa, wh = train_rows(rec, qs, qc, c, occluded, n_lights, flags)
+    out = rows.copy()
+    sa, sh = wa.sum(), wh.sum()
+    out[:, :n_lights] *= (1.0 / sa) if sa > 0 else 0.0
+    out[:, 32] *= (1.0 / sh) if sh > 0 else 0.0
+    return out
+
+
+# The objectives whose logit gradients the rows are (for the finite-difference
+# checks): per record, as functions of the current logits z (k, 33).
+def light_objective(z, rec, qs, occluded, n_lights, chi2=False):
+    """Per record: a (-log q'_j(z)) (KL form) or a^2 q'_s,j / q'_j(z)
+    (second-moment form), a = g_l V / p_l; 0 for records without a light."""
+    qp = qprime(z, n_lights)
+    gl, pl, _, _, _, _ = record_terms(rec, occluded)
+    j = np.asarray(rec)["light"].astype(np.int64)
+    ok = j < n_lights
+    a = np.where(pl > 0, gl / np.where(pl > 0, pl, 1), 0.0)
+    jj = np.where(ok, j, 0)
+    qj = qp[np.arange(len(j)), jj]
+    if chi2:
+        v = a * a * np.asarray(qs, np.float64)[np.arange(len(j)), jj] / qj
+    else:
+        v = -a * np.log(qj)
+    return np.where(ok, v, 0.0)
+
+
+def mix_objective(z, rec, occluded, n_lights):
+    """Per record: F(c(z_32))^2, F the vertex's two-sample estimate
+    c g_l / p_c(l) + (1 - c) g_b / p_c(b); 0 for records without a light."""
+    gl, pl, pbl, gb, plb, pbb = record_terms(rec, occluded)
+    c = mixing(np.asarray(z, np.float64)[:, 32])
+    pcl, pcb = c * pl + (1 - c) * pbl, c * plb + (1 - c) * pbb
+    al = np.where((gl > 0) & (pcl > 0), gl / np.where(pcl > 0, pcl, 1), 0.0)
+    ab = np.where((gb > 0) & (pcb > 0), gb / np.where(pcb > 0, pcb, 1), 0.0)
+    ok = np.asarray(rec)["light"].astype(np.int64) < n_lights
+    F = c * al + (1 - c) * ab
+    return np.where(ok, F * F, 0.0)

@@ -59,7 +59,7 @@ def test_struct_layouts_match_header():
     assert "56 bytes" in txt
     assert C.sizeof(abi.chk_material) == 56
     assert C.sizeof(abi.chk_camera) == 4 * 11 + 8
-    assert C.sizeof(abi.chk_render_params) == 32
+    assert C.sizeof(abi.chk_render_params) == 48
     assert C.sizeof(abi.chk_direct_params) == 24
     assert C.sizeof(abi.chk_sphere) == 40 and C.sizeof(abi.chk_point_source) == 48
     assert C.sizeof(abi.chk_env_source) == 48
@@ -68,7 +68,7 @@ def test_struct_layouts_match_header():
 LAYOUT_STRUCTS = ["chk_material", "chk_mesh_light", "chk_sphere", "chk_point_source",
                   "chk_env_source", "chk_scene_desc", "chk_camera", "chk_render_params",
                   "chk_direct_params", "chk_scene_info", "chk_render_stats", "chk_request",
-                  "chk_batch_tuning", "chk_batch_stats", "chk_train_params"]
+                  "chk_batch_tuning", "chk_batch_stats", "chk_train_params", "chk_train_record"]
 
 
 def test_ctypes_layouts_equal_the_c_compilers(tmp_path):

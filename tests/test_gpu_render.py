@@ -147,24 +147,27 @@ def test_short_stack_retries_are_exact(ctx):
     """A tree whose worst-case stack exceeds an 8-entry stack: the rays that
     overflow it are traced again with the full stack, and the renders (path
     tracer, and the shaders' closest-hit and shadow traces) equal the
-    32-entry renders bit for bit."""
+    32-entry renders bit for bit, with 16 entries (the default) and with 8."""
     sc = layered_scene()
     ds = abi.DeviceScene(ctx, sc)
     assert 8 < ds.info()["stack_bound"] <= 32
     P = dict(spp=8, max_depth=4, seed=0x5EED0020)
     dp = host.direct_params(abi.DIRECT_MIS)
+    ds.set_trace_stack(32)
     a32, _, s32 = host.render(ds, P)
     d32, _, _ = host.render_direct(ds, dict(spp=2, seed_lo=0x5EED0020), dp)
     assert s32["stack_retries"] == 0
-    ds.set_trace_stack(8)
-    a16, _, s16 = host.render(ds, P)
-    d16, _, _ = host.render_direct(ds, dict(spp=2, seed_lo=0x5EED0020), dp)
-    print(f"8-entry stack: {s16['stack_retries']} of {s16['rays_extend'] + s16['rays_shadow']} "
-          "rays traced again")
-    assert s16["stack_retries"] > 0
-    np.testing.assert_array_equal(a16, a32)
-    np.testing.assert_array_equal(d16, d32)
-    with pytest.raises(abi.ChkError, match="8 or 32"):
+    for entries in (16, 8):
+        ds.set_trace_stack(entries)
+        a, _, st = host.render(ds, P)
+        d, _, _ = host.render_direct(ds, dict(spp=2, seed_lo=0x5EED0020), dp)
+        print(f"{entries}-entry stack: {st['stack_retries']} of "
+              f"{st['rays_extend'] + st['rays_shadow']} rays traced again")
+        if entries == 8:
+            assert st["stack_retries"] > 0
+        np.testing.assert_array_equal(a, a32)
+        np.testing.assert_array_equal(d, d32)
+    with pytest.raises(abi.ChkError, match="8, 16 or 32"):
         ds.set_trace_stack(24)
 
 
@@ -180,3 +183,40 @@ def test_deep_tree_short_stack_equals_full_stack(ctx):
     c, _, _ = host.render(ds, P, count=True)
     print(f"deep tree: {st['stack_retries']} rays traced again")
     np.testing.assert_array_equal(a, c)
+
+
+def _window_cases():
+    return [(5, 3, 17, 9), (0, 10, 40, 1), (0, 0, 40, 30), (39, 29, 1, 1)]
+
+
+@pytest.mark.parametrize("win", _window_cases())
+def test_render_window_equals_whole_image(ctx, win):
+    """params.window renders a rectangle of the image: its pixels equal the
+    whole-image render's bit for bit (keys and accumulators use the image
+    pixel), every other pixel is untouched; the same through the batch."""
+    sc = scenes.cornell(width=40, height=30, sphere_tris=2000, n_lights=3, light_rng=True)
+    ds = abi.DeviceScene(ctx, sc)
+    P = dict(spp=8, max_depth=4, seed=0x5EED0041, flags=abi.RENDER_SUMSQ)
+    full, fsq, _ = host.render(ds, P)
+    x0, y0, w, h = win
+    part, psq, st = host.render(ds, dict(P, window=win))
+    assert st["paths"] == w * h * 8
+    np.testing.assert_array_equal(part[y0:y0 + h, x0:x0 + w], full[y0:y0 + h, x0:x0 + w])
+    np.testing.assert_array_equal(psq[y0:y0 + h, x0:x0 + w], fsq[y0:y0 + h, x0:x0 + w])
+    mask = np.ones((30, 40), bool)
+    mask[y0:y0 + h, x0:x0 + w] = False
+    assert not part[mask].any()
+    got, _, bst = host.render_batch(ctx, [dict(ds=ds, params=dict(P, window=win)),
+                                          dict(ds=ds, params=P)])
+    assert bst["paths"] == (w * h + 40 * 30) * 8
+    np.testing.assert_array_equal(got[0][0], part)
+    np.testing.assert_array_equal(got[1][0], full)
+
+
+def test_render_window_rejected_outside_the_image(ctx):
+    ds = abi.DeviceScene(ctx, scenes.cornell(width=16, height=8))
+    for bad in [(0, 0, 17, 1), (15, 0, 2, 1), (0, 8, 1, 1), (0, 0, 0, 4)]:
+        with pytest.raises(abi.ChkError, match="window"):
+            host.render(ds, dict(spp=1, max_depth=1, window=bad))
+    with pytest.raises(abi.ChkError, match="window"):
+        host.render_direct(ds, dict(spp=1, window=(0, 0, 4, 4)), host.direct_params(abi.DIRECT_MIS))

@@ -169,3 +169,162 @@ def test_gpu_adam_step(ctx):
     # first Adam step: m_hat = g, v_hat = g^2 -> step = lr * g / (|g| + eps)
     expect = p0 - 1e-3 * g / (np.abs(g) + 1e-8)
     np.testing.assert_allclose(p1, expect, rtol=1e-5, atol=1e-7)
+
+
+# ------------------------------------------------------------ online-training loss
+def random_records(rng, k, n_lights, n_slots=None):
+    """Training records of the learned render's shape: a light drawn per
+    record (some without one), positive values and pdfs, occlusion slots."""
+    from chakumrender_amd import abi
+
+    n_slots = n_slots or k
+    rec = np.zeros(k, abi.TRAIN_RECORD_DTYPE)
+    rec["light"] = rng.integers(0, n_lights, k)
+    rec["light"][rng.random(k) < 0.1] = P.NO_INDEX          # no light sample
+    rec["gl"] = rng.exponential(1.0, k) * (rng.random(k) < 0.8)
+    rec["pl"] = rng.uniform(0.05, 3.0, k)
+    rec["pbl"] = rng.uniform(0.05, 3.0, k)
+    rec["fb"] = rng.exponential(0.5, k)
+    rec["le"] = rng.exponential(2.0, k) * (rng.random(k) < 0.5)
+    rec["plb"] = rng.uniform(0.05, 3.0, k)
+    rec["pbb"] = rng.uniform(0.05, 3.0, k)
+    rec["sh_slot"] = rng.permutation(n_slots)[:k]
+    rec["sh_slot"][rng.random(k) < 0.1] = P.NO_INDEX
+    rec["light_b"] = P.NO_INDEX
+    occ = (rng.random(n_slots) < 0.3).astype(np.uint8)
+    return rec, occ
+
+
+def defensive_pmf(rng, k, n_lights):
+    q = rng.dirichlet(np.ones(n_lights), k)
+    return (1 - P.EPS_Q) * q + P.EPS_Q / n_lights
+
+
+@pytest.mark.parametrize("n_lights", [32, 5])
+@pytest.mark.parametrize("chi2", [False, True])
+def test_training_rows_are_gradients_of_their_objectives(n_lights, chi2):
+    """Each record's row is the logit gradient of its objective: the light
+    head of a (-log q'_j) (KL) or a^2 q'_s,j / q'_j (second moment), the
+    mixing head of F(c)^2 -- central differences of the objectives."""
+    rng = np.random.default_rng(11 + n_lights + chi2)
+    k = 48
+    rec, occ = random_records(rng, k, n_lights)
+    z = rng.normal(scale=1.5, size=(k, 33))
+    qs = defensive_pmf(rng, k, n_lights)
+    qc = P.qprime(z, n_lights)
+    c = P.mixing(z[:, 32])
+    rows, wa, wh = P.train_rows(rec, qs, qc, c, occ, n_lights,
+                                flags=P.TRAIN_LIGHTS_CHI2 if chi2 else 0)
+    h = 1e-6
+    for col in list(range(n_lights)) + [32]:
+        zp, zm = z.copy(), z.copy()
+        zp[:, col] += h
+        zm[:, col] -= h
+        if col == 32:
+            fd = (P.mix_objective(zp, rec, occ, n_lights) - P.mix_objective(zm, rec, occ, n_lights)) / (2 * h)
+        else:
+            fd = (P.light_objective(zp, rec, qs, occ, n_lights, chi2)
+                  - P.light_objective(zm, rec, qs, occ, n_lights, chi2)) / (2 * h)
+        np.testing.assert_allclose(rows[:, col], fd, rtol=1e-5, atol=1e-7 * np.abs(fd).max() + 1e-12,
+                                   err_msg=f"logit {col}")
+    # rows beyond the scene's lights stay zero; weights match the rows' heads
+    assert not rows[:, n_lights:32].any()
+    assert np.all((wa > 0) == np.any(rows[:, :n_lights] != 0, axis=1))
+    np.testing.assert_array_equal(wh, np.abs(rows[:, 32]))
+
+
+@pytest.mark.parametrize("chi2", [False, True])
+def test_light_rows_estimate_the_objective_over_all_lights(chi2):
+    """One shading configuration with known per-light values g_j and area pdfs
+    p_j, lights drawn from q'_s: the expectation over j ~ q'_s of the row is
+    the logit gradient of Z KL(p* || q') with p*_j = (g_j / p_j) / Z (KL
+    form), or of the light estimate's second moment sum_j g_j^2 / (q'_j
+    p_j^2) (second-moment form), both by central differences."""
+    from chakumrender_amd import abi
+
+    rng = np.random.default_rng(5 + chi2)
+    L = 7
+    g = rng.exponential(1.0, L)
+    parea = rng.uniform(0.2, 2.0, L)
+    qs = defensive_pmf(rng, 1, L)[0]
+    z0 = rng.normal(size=33)
+    rec = np.zeros(L, abi.TRAIN_RECORD_DTYPE)
+    rec["light"] = np.arange(L)
+    rec["gl"] = g
+    rec["pl"] = qs * parea                     # the record's light pdf q'_s,j p_j
+    rec["pbl"] = rec["plb"] = rec["pbb"] = 1.0
+    rec["sh_slot"] = 0
+    rec["light_b"] = P.NO_INDEX
+    occ = np.zeros(1, np.uint8)
+    z = np.repeat(z0[None], L, 0)
+    qc = P.qprime(z, L)
+    rows, _, _ = P.train_rows(rec, np.repeat(qs[None], L, 0), qc, P.mixing(z[:, 32]), occ, L,
+                              flags=P.TRAIN_FREEZE_MIX | (P.TRAIN_LIGHTS_CHI2 if chi2 else 0))
+    expect = (qs[:, None] * rows[:, :L]).sum(0)
+
+    def objective(zz):
+        q = P.qprime(zz[None], L)[0]
+        if chi2:
+            return float(np.sum(g ** 2 / (q * parea ** 2)))
+        v = g / parea
+        Z = v.sum()
+        p_star = v / Z
+        return float(Z * np.sum(p_star * np.log(p_star / q)))
+
+    h = 1e-6
+    fd = np.array([(objective(z0 + h * np.eye(33)[i]) - objective(z0 - h * np.eye(33)[i])) / (2 * h)
+                   for i in range(L)])
+    np.testing.assert_allclose(expect, fd, rtol=1e-6, atol=1e-9)
+
+
+def test_training_dlogits_normalisation():
+    """Each head is divided by its batch sum of row weights: the dlogits are
+    unchanged when every value (g_l and f L_e: the light values g_l and the
+    emission L_e) is scaled by one constant, the
+    light head's weights then sum to 1, and a batch without any light value
+    gives a zero light head."""
+    rng = np.random.default_rng(21)
+    k, L = 64, 16
+    rec, occ = random_records(rng, k, L)
+    z = rng.normal(size=(k, 33))
+    qs = defensive_pmf(rng, k, L)
+    qc, c = P.qprime(z, L), P.mixing(z[:, 32])
+    d1 = P.train_dlogits(rec, qs, qc, c, occ, L)
+    r2 = rec.copy()
+    for f in ("gl", "le"):
+        r2[f] = r2[f] * 37.0
+    d2 = P.train_dlogits(r2, qs, qc, c, occ, L)
+    np.testing.assert_allclose(d2, d1, rtol=1e-6, atol=1e-12)
+    _, wa, _ = P.train_rows(rec, qs, qc, c, occ, L)
+    assert abs((wa / wa.sum()).sum() - 1) < 1e-12
+    r3 = rec.copy()
+    r3["gl"] = 0
+    d3 = P.train_dlogits(r3, qs, qc, c, occ, L)
+    assert not d3[:, :L].any() and d3[:, 32].any()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("flags", [0, 4, 1, 2])
+def test_gpu_training_loss_matches_oracle(ctx, flags):
+    """chk_proposal_loss_grad (the learned render's training rows) against the
+    oracle's restatement on the same records, at the q'_c and c the GPU's own
+    inference gives for the records' inputs."""
+    from chakumrender_amd import host
+
+    rng = np.random.default_rng(31 + flags)
+    L, k = 32, 4096
+    pr = host.Proposal(ctx, L, seed=3)
+    pr.set_params(random_params(rng))
+    rec, occ = random_records(rng, k, L, n_slots=2 * k)
+    s = inputs(rng, k)
+    qs = defensive_pmf(rng, k, L).astype(np.float32)
+    print(f"loss grad: {k} records, flags {flags}", flush=True)
+    got = pr.loss_grad(rec, s, qs, occ, flags)
+    qc, c, _ = pr.eval(s)
+    want = P.train_dlogits(rec, qs, qc, c, occ, L, flags)
+    scale = np.abs(want).max(0) + 1e-30
+    err = np.abs(got - want) / scale
+    print(f"max error relative to each logit's largest row: {err.max():.2e}")
+    assert err.max() < 1e-5
+    np.testing.assert_array_equal(got[:, L:32], 0)
+    pr.close()
