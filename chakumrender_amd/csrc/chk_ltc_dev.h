@@ -19,15 +19,66 @@ constexpr double kPi64 = 3.14159265359;  // mathPrimitives.h:4
 // of both matrices are zero in the fit.
 constexpr int kLtcRec = 12;
 
+// 1/x and 1/sqrt(x) for positive normal x: the hardware approximation and
+// two Newton steps (full FP64 precision), cheaper than a divide or sqrt.
+__device__ inline double rcp64(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(fma(-x, r, 1.0), r, r);
+  return fma(fma(-x, r, 1.0), r, r);
+}
+__device__ inline double rsq64(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  y = fma(0.5 * y, fma(-x * y, y, 1.0), y);
+  return fma(0.5 * y, fma(-x * y, y, 1.0), y);
+}
+
+// atan2(y, x) for y >= 0 (an angle in [0, pi]): t = min/max of (y, |x|) in
+// [0, 1], atan(t) = t P(t^2) with P the degree-16 Chebyshev fit of
+// atan(sqrt(w))/sqrt(w) on [0, 1] (relative error 7.7e-15 in FP64;
+// gen_atan_poly.py), then the octant.  Branch free.
+__device__ inline double atan2_upper(double y, double x) {
+  const double ax = fabs(x);
+  const double mx = fmax(y, ax), mn = fmin(y, ax);
+  const double t = mx > 0.0 ? mn * rcp64(mx) : 0.0;
+  const double w = t * t;
+  double p = 7.063142635888046e-05;
+  p = fma(p, w, -0.0006774105399645076);
+  p = fma(p, w, 0.0030659957028108006);
+  p = fma(p, w, -0.008792435542217635);
+  p = fma(p, w, 0.018198570485395787);
+  p = fma(p, w, -0.029589012194344372);
+  p = fma(p, w, 0.04051778977125066);
+  p = fma(p, w, -0.04977996675955047);
+  p = fma(p, w, 0.05794192316881845);
+  p = fma(p, w, -0.06646145790221254);
+  p = fma(p, w, 0.07688811662401294);
+  p = fma(p, w, -0.09090489647799033);
+  p = fma(p, w, 0.11111077584971772);
+  p = fma(p, w, -0.142857126460504);
+  p = fma(p, w, 0.19999999957481826);
+  p = fma(p, w, -0.3333333333289398);
+  p = fma(p, w, 0.9999999999999925);
+  double a = t * p;
+  a = y > ax ? 1.5707963267948966 - a : a;
+  return x < 0.0 ? 3.141592653589793 - a : a;
+}
+
 // integrateEdge (lightSources_CV.cpp:8-25): (v1 x v2).n * theta/sin(theta)
 // with v_i = p_i/|p_i|, from the unnormalised p1, p2, c = p1 x p2 and
 // cn = c.n: (v1 x v2).n = cn/(|p1||p2|) and sin(theta) = |c|/(|p1||p2|), so
 // the term is cn * theta/|c| with theta = atan2(|c|, p1.p2) (the
-// reference's acos(v1.v2), without its loss of precision near 0).
+// reference's acos(v1.v2), without its loss of precision near 0).  For
+// theta <= 0.001 the reference uses the limit cn/(|p1||p2|), which is
+// cn (theta/|c|) (sin(theta)/theta), here with the series 1 - theta^2/6
+// (1 - theta^2/20) (exact to 1e-22 there).  A zero edge (p1 = p2) gives 0.
 __device__ inline double edge64(d3 p1, d3 p2, d3 c, double cn) {
-  const double s = sqrt(dot(c, c));
-  const double th = atan2(s, dot(p1, p2));
-  return th > 0.001 ? cn * (th / s) : cn / sqrt(dot(p1, p1) * dot(p2, p2));
+  const double cc = dot(c, c);
+  const double rs = rsq64(cc > 0.0 ? cc : 1.0);
+  const double th = atan2_upper(cc * rs, dot(p1, p2));
+  const double q = th * rs;
+  const double t2 = th * th;
+  const double g = th > 0.001 ? q : q * fma(-t2 * (1.0 / 6.0), fma(-t2, 1.0 / 20.0, 1.0), 1.0);
+  return cc > 0.0 ? cn * g : 0.0;
 }
 
 // n.v for the clip plane: the LTC part clips against z = 0 in its own
@@ -46,88 +97,84 @@ __device__ inline d3 sel3(bool c, d3 a, d3 b) {
 struct Clip4 {
   d3 q0, q1, q2, q3;
   int idx;
-  __device__ void put(d3 X, d3 Y) {
-    const bool a = idx == 0, b = idx == 2;
+  __device__ void put(bool on, d3 X, d3 Y) {
+    const bool a = on && idx == 0, b = on && idx == 2;
     q0 = sel3(a, X, q0);
     q1 = sel3(a, Y, q1);
     q2 = sel3(b, X, q2);
     q3 = sel3(b, Y, q3);
-    idx += 2;
+    idx += on ? 2 : 0;
   }
 };
 
 // clipTriangle (lightSources_CV.cpp:28-126 and 129-219): the 4-point
 // polygon of the part of ABC above the plane; false if it has none.
-// nA, nB, nC: signs of the vertices (only their sign is read).
+// nA, nB, nC: signs of the vertices (only their sign is read).  Evaluated
+// without branches (the reference's early returns become a flag), so the
+// lanes of a wave that clip and those that do not stay converged.
 template <bool ZPLANE>
 __device__ bool clip64(d3 A, d3 B, d3 C, d3 n, d3 P, double nA, double nB, double nC,
                        Clip4 &clip) {
   const double eps = 1e-15;
   int cnt = 0;
+  bool dead = false;
   const d3 dAB = A - B, dBC = B - C, dCA = C - A;
   const double nAB = ndot<ZPLANE>(n, dAB), nBC = ndot<ZPLANE>(n, dBC),
                nCA = ndot<ZPLANE>(n, dCA);
   const double pA = ndot<ZPLANE>(n, P - A), pB = ndot<ZPLANE>(n, P - B),
                pC = ndot<ZPLANE>(n, P - C);
-  if (nAB * nAB > eps) {
-    const d3 I = dAB * (pA / nAB) + A;
-    if (dot(I - A, B - I) > 0) {
-      ++cnt;
-      if (nA < 0) clip.put(I, B);
-      else clip.put(A, I);
-    }
-  } else if (pA * pA < eps) {
-    return false;
+  {
+    const bool cuts = nAB * nAB > eps;
+    const d3 I = dAB * (pA / (cuts ? nAB : 1.0)) + A;
+    const bool hit = cuts && dot(I - A, B - I) > 0;
+    dead = dead || (!cuts && pA * pA < eps);
+    cnt += hit ? 1 : 0;
+    clip.put(hit, sel3(nA < 0, I, A), sel3(nA < 0, B, I));
   }
-  if (nBC * nBC > eps) {
-    const d3 I = dBC * (pB / nBC) + B;
-    if (dot(I - B, C - I) > 0) {
-      ++cnt;
-      if (nB < 0) clip.put(I, C);
-      else clip.put(B, I);
-    }
-  } else if (pB * pB < eps) {
-    return false;
+  {
+    const bool cuts = nBC * nBC > eps;
+    const d3 I = dBC * (pB / (cuts ? nBC : 1.0)) + B;
+    const bool hit = cuts && dot(I - B, C - I) > 0;
+    dead = dead || (!cuts && pB * pB < eps);
+    cnt += hit ? 1 : 0;
+    clip.put(hit, sel3(nB < 0, I, B), sel3(nB < 0, C, I));
   }
-  if (nCA * nCA > eps) {
-    const d3 I = dCA * (pC / nCA) + C;
+  {
+    const bool cuts = nCA * nCA > eps;
+    const d3 I = dCA * (pC / (cuts ? nCA : 1.0)) + C;
     // the reference tests only the x components on this edge
-    if ((I.x - C.x) * (A.x - I.x) > 0) {
-      ++cnt;
-      if (nC < 0) clip.put(I, A);
-      else clip.put(C, I);
-    }
-  } else if (pC * pC < eps) {
-    return false;
+    const bool hit = cuts && (I.x - C.x) * (A.x - I.x) > 0;
+    dead = dead || (!cuts && pC * pC < eps);
+    cnt += hit ? 1 : 0;
+    clip.put(hit, sel3(nC < 0, I, C), sel3(nC < 0, A, I));
   }
-  return cnt > 1;
+  return cnt > 1 && !dead;
 }
 
 // |sum of edge terms| of one triangle (lightSources_CV.cpp:241-262 and
-// 293-314).  ZPLANE: LTC frame (P = 0, n = z).
+// 293-314).  ZPLANE: LTC frame (P = 0, n = z).  Every lane evaluates one
+// 4-point polygon: the clipped one, or ABC with C repeated (a zero edge, so
+// the sum is the reference's three-edge sum in its order); the clip itself
+// runs only when some lane of the wave straddles the plane.
 template <bool ZPLANE>
 __device__ double tri64(d3 A, d3 B, d3 C, d3 n, d3 P) {
   const d3 p1 = A - P, p2 = B - P, p3 = C - P;
   // signs of n.(V - P)/|V - P|, as the reference tests them
   const double nA = ndot<ZPLANE>(n, p1), nB = ndot<ZPLANE>(n, p2), nC = ndot<ZPLANE>(n, p3);
-  double sum;
-  if (nA > 0 && nB > 0 && nC > 0) {
-    const d3 c1 = cross(p1, p2), c2 = cross(p2, p3), c3 = cross(p3, p1);
-    sum = edge64(p1, p2, c1, ndot<ZPLANE>(n, c1)) + edge64(p2, p3, c2, ndot<ZPLANE>(n, c2)) +
-          edge64(p3, p1, c3, ndot<ZPLANE>(n, c3));
-  } else if (nA < 0 && nB < 0 && nC < 0) {
-    return 0.0;
-  } else {
-    Clip4 cl;
-    cl.q0 = cl.q1 = cl.q2 = cl.q3 = dk(0, 0, 0);
-    cl.idx = 0;
-    if (!clip64<ZPLANE>(A, B, C, n, P, nA, nB, nC, cl)) return 0.0;
-    const d3 q1 = cl.q0 - P, q2 = cl.q1 - P, q3 = cl.q2 - P, q4 = cl.q3 - P;
-    const d3 c1 = cross(q1, q2), c2 = cross(q2, q3), c3 = cross(q3, q4), c4 = cross(q4, q1);
-    sum = edge64(q1, q2, c1, ndot<ZPLANE>(n, c1)) + edge64(q2, q3, c2, ndot<ZPLANE>(n, c2)) +
-          edge64(q3, q4, c3, ndot<ZPLANE>(n, c3)) + edge64(q4, q1, c4, ndot<ZPLANE>(n, c4));
-  }
-  return fabs(sum);
+  const bool above = nA > 0 && nB > 0 && nC > 0;
+  const bool below = nA < 0 && nB < 0 && nC < 0;
+  Clip4 cl;
+  cl.q0 = cl.q1 = cl.q2 = cl.q3 = dk(0, 0, 0);
+  cl.idx = 0;
+  bool clipped = false;
+  if (__any(!above && !below))  // wave-uniform
+    clipped = clip64<ZPLANE>(A, B, C, n, P, nA, nB, nC, cl);
+  const d3 q1 = sel3(above, p1, cl.q0 - P), q2 = sel3(above, p2, cl.q1 - P),
+           q3 = sel3(above, p3, cl.q2 - P), q4 = sel3(above, p3, cl.q3 - P);
+  const d3 c1 = cross(q1, q2), c2 = cross(q2, q3), c3 = cross(q3, q4), c4 = cross(q4, q1);
+  const double sum = edge64(q1, q2, c1, ndot<ZPLANE>(n, c1)) + edge64(q2, q3, c2, ndot<ZPLANE>(n, c2)) +
+                     edge64(q3, q4, c3, ndot<ZPLANE>(n, c3)) + edge64(q4, q1, c4, ndot<ZPLANE>(n, c4));
+  return (above || (!below && clipped)) ? fabs(sum) : 0.0;
 }
 
 
