@@ -182,6 +182,7 @@ DIRECT_VISIBILITY = 0x1
 MIS_LIGHT_SAMPLES, MIS_BSDF_SAMPLES = 10, 60
 SHADE_DIRECT_ONLY, SHADE_EXPLICIT, SHADE_IMPLICIT = 0, 1, 2
 SHADE_MAX_TRIES = 64
+ACCEL_BVH, ACCEL_DUMMY = 0, 1
 SHADE_STREAM = 0x53480000  # CHK_STREAM_SHADE
 
 # Every exported symbol of include/chakum_c.h: name -> (restype, argtypes)
@@ -206,6 +207,7 @@ SIGNATURES = {
     "chk_scene_destroy": (C.c_int, [_vp]),
     "chk_scene_set_trace_stack": (C.c_int, [_vp, _u32]),
     "chk_scene_get_info": (C.c_int, [_vp, C.POINTER(chk_scene_info)]),
+    "chk_scene_set_accel": (C.c_int, [_vp, _u32]),
     "chk_trace_closest": (C.c_int, [_vp, _vp, _u32, _vp, _vp]),
     "chk_trace_any": (C.c_int, [_vp, _vp, _u32, _vp, _vp]),
     "chk_trace_spheres": (C.c_int, [_vp, _u32, _vp, _u32, _vp, _vp]),
@@ -403,6 +405,10 @@ class DeviceScene:
         check(self.L.chk_scene_create_desc(ctx.h, C.byref(d), C.byref(h)), "chk_scene_create_desc")
         del keep
         self.h = h
+
+    def set_accel(self, kind: int):
+        """chk_scene_set_accel: ACCEL_BVH or ACCEL_DUMMY for the trace entry points."""
+        check(self.L.chk_scene_set_accel(self.h, int(kind)), "chk_scene_set_accel")
 
     def set_trace_stack(self, entries: int):
         """chk_scene_set_trace_stack: 8, 16 or 32 LDS stack entries per lane."""

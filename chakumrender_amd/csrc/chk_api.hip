@@ -617,6 +617,7 @@ chk_status chk_scene_destroy(chk_scene_t sc) {
   if (sc->lwork) (void)hipFree(sc->lwork);
   if (sc->dwork) (void)hipFree(sc->dwork);
   if (sc->env_mem) (void)hipFree(sc->env_mem);
+  if (sc->dummy_box) (void)hipFree(sc->dummy_box);
   if (sc->blob) (void)hipFree(sc->blob);
   delete sc;
   return CHK_OK;
@@ -629,6 +630,36 @@ chk_status chk_scene_set_trace_stack(chk_scene_t sc, uint32_t entries) {
   }
   sc->shallow_stack = (int)entries;
   return CHK_OK;
+}
+
+chk_status chk_scene_set_accel(chk_scene_t sc, uint32_t kind) {
+  CHK_TRY_BEGIN
+  if (!sc || (kind != CHK_ACCEL_BVH && kind != CHK_ACCEL_DUMMY)) {
+    set_error("chk_scene_set_accel: null scene, or kind not CHK_ACCEL_BVH / CHK_ACCEL_DUMMY");
+    return CHK_EINVAL;
+  }
+  if (kind == CHK_ACCEL_DUMMY && !sc->dummy_box && sc->d.n_tris) {
+    CHK_HIP(hipSetDevice(sc->ctx->device));
+    // dummyAccel.cpp:13-23: nBoxT = (int)sqrt(n) runs of nBoxT triangles
+    const uint32_t runs = std::max<uint32_t>(1u, (uint32_t)std::sqrt((double)sc->d.n_tris));
+    float4 *box = nullptr;
+    if (hipMalloc(&box, (size_t)runs * 2 * sizeof(float4)) != hipSuccess) {
+      set_error("chk_scene_set_accel: cannot allocate the run boxes");
+      return CHK_ENOMEM;
+    }
+    const hipError_t e = launch_dummy_boxes(sc, runs, runs, box, nullptr);
+    const hipError_t e2 = e == hipSuccess ? hipDeviceSynchronize() : e;
+    if (e2 != hipSuccess) {
+      (void)hipFree(box);
+      set_error(std::string("chk_scene_set_accel: ") + hipGetErrorString(e2));
+      return CHK_EDEVICE;
+    }
+    sc->dummy_box = box;
+    sc->dummy_runs = runs;
+  }
+  sc->accel = kind;
+  return CHK_OK;
+  CHK_TRY_END
 }
 
 chk_status chk_scene_get_info(chk_scene_t sc, chk_scene_info *info) {
@@ -653,6 +684,11 @@ chk_status chk_trace_closest(chk_scene_t sc, const chk_ray *rays, uint32_t n, ch
     return CHK_ECAPACITY;
   }
   CHK_HIP(hipSetDevice(sc->ctx->device));
+  if (sc->accel == CHK_ACCEL_DUMMY) {
+    CHK_HIP(launch_dummy_trace(sc, false, sc->dummy_box, sc->dummy_runs, (const float4 *)rays, n,
+                               (float4 *)hits, nullptr, (hipStream_t)stream));
+    return CHK_OK;
+  }
   // no scratch: the stack depth bound of the tree is exact, so calls on
   // distinct streams share nothing but the read-only scene
   CHK_HIP(launch_closest(sc, TraceShape{sc->stack_size, 128}, (const float4 *)rays,
@@ -675,6 +711,11 @@ chk_status chk_trace_any(chk_scene_t sc, const chk_ray *rays, uint32_t n, uint8_
     return CHK_ECAPACITY;
   }
   CHK_HIP(hipSetDevice(sc->ctx->device));
+  if (sc->accel == CHK_ACCEL_DUMMY) {
+    CHK_HIP(launch_dummy_trace(sc, true, sc->dummy_box, sc->dummy_runs, (const float4 *)rays, n,
+                               nullptr, occluded, (hipStream_t)stream));
+    return CHK_OK;
+  }
   CHK_HIP(launch_any(sc, TraceShape{sc->stack_size, 128}, (const float4 *)rays,
                      linear_range(n), occluded, nullptr, nullptr, false, false, nullptr,
                      SlotScenes{}, (hipStream_t)stream));

@@ -139,6 +139,11 @@ struct chk_scene_s {
   // process-unique creation number: a batch's cached graphs are keyed on it,
   // so a new scene at a freed scene's address never replays them
   uint64_t gen = 0;
+  // accelerator of the trace entry points (chk_scene_set_accel); the run
+  // boxes of CHK_ACCEL_DUMMY, built on first use
+  uint32_t accel = 0;
+  float4 *dummy_box = nullptr;
+  uint32_t dummy_runs = 0;
   int bad_sampler_mat = -1;
   uint32_t bad_sampler_brdf = 0;
 };
@@ -215,6 +220,13 @@ hipError_t launch_any(const chk_scene_s *s, const TraceShape &T, const float4 *r
                       float4 *path_L, bool leaf_ids, bool count, unsigned long long *counters,
                       const SlotScenes &M, hipStream_t st);
 hipError_t launch_zero_u32(uint32_t *p, uint32_t n, hipStream_t st);
+// DummyAccel (chk_scene_set_accel): the run boxes of the scene's original
+// triangle order, and closest / any hit over them.
+hipError_t launch_dummy_boxes(const chk_scene_s *s, uint32_t n_runs, uint32_t run_len,
+                              float4 *box, hipStream_t st);
+hipError_t launch_dummy_trace(const chk_scene_s *s, bool any, const float4 *box, uint32_t n_runs,
+                              const float4 *rays, uint32_t n, float4 *hits, uint8_t *occluded,
+                              hipStream_t st);
 // Closest sphere per ray (spheres: ns x 4 floats, centre and radius).
 hipError_t launch_spheres(const float *spheres, uint32_t ns, const float4 *rays, uint32_t n,
                           float4 *hits, hipStream_t st);

@@ -26,6 +26,26 @@ namespace {
 
 constexpr int kLtcBlock = 256;
 
+// The LTC frame of triangle j seen from x: its vertices relative to x,
+// multiplied by the rows of Minv * R.
+__device__ inline void ltc_vertices(const float *__restrict__ lt, uint32_t j, d3 x, d3 R0, d3 R1,
+                                    d3 R2, d3 &e1, d3 &e2, d3 &e3) {
+  const float *p = lt + 9 * (size_t)j;
+  const d3 a = dk(p[0], p[1], p[2]) - x, b = dk(p[3], p[4], p[5]) - x,
+           c = dk(p[6], p[7], p[8]) - x;
+  e1 = dk(dot(R0, a), dot(R1, a), dot(R2, a));
+  e2 = dk(dot(R0, b), dot(R1, b), dot(R2, b));
+  e3 = dk(dot(R0, c), dot(R1, c), dot(R2, c));
+}
+
+// Light triangles are taken 32 at a time.  First pass: every lane
+// integrates the triangles entirely above its horizon (three edges, no
+// clipping; the value of tri64<true>) and marks those that straddle it.
+// Second pass: each lane walks its own marked triangles (the lanes of a wave
+// in step, each on its next one), clipping and integrating them (tri64's
+// clip branch); a lane with none left repeats a harmless evaluation.  The
+// clip code thus runs only as often as the lane with the most straddling
+// triangles needs it.  Triangles below the horizon contribute 0 in both.
 __global__ __launch_bounds__(kLtcBlock) void k_ltc(const double *__restrict__ tab,
                                                    const float *__restrict__ X,
                                                    const float *__restrict__ N,
@@ -51,25 +71,54 @@ __global__ __launch_bounds__(kLtcBlock) void k_ltc(const double *__restrict__ ta
   double sl0 = 0, sl1 = 0, sl2 = 0, sd0 = 0, sd1 = 0, sd2 = 0;
   const d3 zero = dk(0, 0, 0), zax = dk(0, 0, 1);
   const bool do_ltc = kss != 0.0, do_dif = kss != 1.0;
-  for (uint32_t j = 0; j < nlt; ++j) {
-    const float *p = lt + 9 * (size_t)j;
-    const d3 A = dk(p[0], p[1], p[2]), B = dk(p[3], p[4], p[5]), C = dk(p[6], p[7], p[8]);
-    const double r0c = lrad[3 * j], r1c = lrad[3 * j + 1], r2c = lrad[3 * j + 2];
-    if (do_ltc) {
-      const d3 a = A - x, b = B - x, c = C - x;
-      const d3 e1 = dk(dot(R0, a), dot(R1, a), dot(R2, a));
-      const d3 e2 = dk(dot(R0, b), dot(R1, b), dot(R2, b));
-      const d3 e3 = dk(dot(R0, c), dot(R1, c), dot(R2, c));
-      const double s = tri64<true>(e1, e2, e3, zax, zero);
-      sl0 += r0c * s;
-      sl1 += r1c * s;
-      sl2 += r2c * s;
+  for (uint32_t j0 = 0; j0 < nlt; j0 += 32) {
+    const uint32_t jn = min(32u, nlt - j0);
+    uint32_t straddle = 0;
+    for (uint32_t jj = 0; jj < jn; ++jj) {
+      const uint32_t j = j0 + jj;
+      const double r0c = lrad[3 * j], r1c = lrad[3 * j + 1], r2c = lrad[3 * j + 2];
+      if (do_ltc) {
+        d3 e1, e2, e3;
+        ltc_vertices(lt, j, x, R0, R1, R2, e1, e2, e3);
+        const bool above = e1.z > 0 && e2.z > 0 && e3.z > 0;
+        const bool below = e1.z < 0 && e2.z < 0 && e3.z < 0;
+        straddle |= (!above && !below) ? 1u << jj : 0u;
+        const d3 c1 = cross(e1, e2), c2 = cross(e2, e3), c3 = cross(e3, e1);
+        const double sum = edge64(e1, e2, c1, c1.z) + edge64(e2, e3, c2, c2.z) +
+                           edge64(e3, e1, c3, c3.z);
+        const double sv = above ? fabs(sum) : 0.0;
+        sl0 += r0c * sv;
+        sl1 += r1c * sv;
+        sl2 += r2c * sv;
+      }
+      if (do_dif) {
+        const float *p = lt + 9 * (size_t)j;
+        const d3 A = dk(p[0], p[1], p[2]), B = dk(p[3], p[4], p[5]), C = dk(p[6], p[7], p[8]);
+        const double sv = tri64<false>(A, B, C, n, x);
+        sd0 += r0c * sv;
+        sd1 += r1c * sv;
+        sd2 += r2c * sv;
+      }
     }
-    if (do_dif) {
-      const double s = tri64<false>(A, B, C, n, x);
-      sd0 += r0c * s;
-      sd1 += r1c * s;
-      sd2 += r2c * s;
+    while (__any(straddle != 0)) {  // wave-uniform loop
+      const bool mine = straddle != 0;
+      const uint32_t jj = mine ? (uint32_t)__builtin_ctz(straddle) : 0u;
+      straddle &= straddle - 1u;
+      const uint32_t j = j0 + jj;
+      d3 e1, e2, e3;
+      ltc_vertices(lt, j, x, R0, R1, R2, e1, e2, e3);
+      Clip4 cl;
+      cl.q0 = cl.q1 = cl.q2 = cl.q3 = zero;
+      cl.idx = 0;
+      const bool ok = clip64<true>(e1, e2, e3, zax, zero, e1.z, e2.z, e3.z, cl);
+      const d3 c1 = cross(cl.q0, cl.q1), c2 = cross(cl.q1, cl.q2), c3 = cross(cl.q2, cl.q3),
+               c4 = cross(cl.q3, cl.q0);
+      const double sum = edge64(cl.q0, cl.q1, c1, c1.z) + edge64(cl.q1, cl.q2, c2, c2.z) +
+                         edge64(cl.q2, cl.q3, c3, c3.z) + edge64(cl.q3, cl.q0, c4, c4.z);
+      const double sv = (mine && ok) ? fabs(sum) : 0.0;
+      sl0 += lrad[3 * j] * sv;
+      sl1 += lrad[3 * j + 1] * sv;
+      sl2 += lrad[3 * j + 2] * sv;
     }
   }
   // Sum / (2 pi), then Analytic = LTC * amp * ks + diffuse * (1 - ks)

@@ -1019,14 +1019,22 @@ chk_status ensure(void *&ptr, size_t &have, size_t need, const char *what, bool 
   return CHK_OK;
 }
 
+// The shading launch of a batch chunk: MULTI (every slot names its request)
+// or, for a chunk that holds one request, the single-request kernel.
 template <int BS>
-void launch_shade_multi(uint32_t grid, hipStream_t st, uint32_t depth, uint32_t segcap,
+void launch_shade_batch(uint32_t grid, hipStream_t st, uint32_t depth, uint32_t segcap,
                         const Buffers &B, uint32_t in, uint32_t out, uint32_t qin, uint32_t qout,
-                        const BatchIO &Q) {
-  hipLaunchKernelGGL((k_shade<false, true, BS>), dim3(grid), dim3(BS), 0, st, RenderArgs{}, depth,
-                     segcap, B.q_ray[in], B.q_thr[in], B.q_pid[in], B.hits, B.q_ray[out],
-                     B.q_thr[out], B.q_pid[out], B.s_ray, B.s_L, B.path_L, B.ctrl, qin, qout,
-                     LearnedIO{}, Q);
+                        const BatchIO &Q, const RenderArgs *single) {
+  if (single)
+    hipLaunchKernelGGL((k_shade<false, false, BS>), dim3(grid), dim3(BS), 0, st, *single, depth,
+                       segcap, B.q_ray[in], B.q_thr[in], B.q_pid[in], B.hits, B.q_ray[out],
+                       B.q_thr[out], B.q_pid[out], B.s_ray, B.s_L, B.path_L, B.ctrl, qin, qout,
+                       LearnedIO{}, BatchIO{});
+  else
+    hipLaunchKernelGGL((k_shade<false, true, BS>), dim3(grid), dim3(BS), 0, st, RenderArgs{}, depth,
+                       segcap, B.q_ray[in], B.q_thr[in], B.q_pid[in], B.hits, B.q_ray[out],
+                       B.q_thr[out], B.q_pid[out], B.s_ray, B.s_L, B.path_L, B.ctrl, qin, qout,
+                       LearnedIO{}, Q);
 }
 
 }  // namespace
@@ -1178,6 +1186,11 @@ chk_status batch_impl(chk_ctx_s *ctx, const chk_request *R, uint32_t n,
   // ---- the launches of one chunk
   auto emit_chunk = [&](uint32_t c, hipStream_t s) -> hipError_t {
     const BatchChunk &C = chunks[c];
+    // a chunk of one request (a large request fills chunks alone) runs the
+    // single-scene kernels: no per-slot scene and argument lookups
+    const RenderArgs *single = C.n_seg == 1 ? &seg_args[C.first_seg] : nullptr;
+    const chk_scene_s *ssc = single ? R[segs[C.first_seg].req].scene : nullptr;
+    const TraceShape sshape{ssc ? ssc->stack_size : stack, (int)T.trace_block, B.ovf, qcap, 16};
     hipLaunchKernelGGL(k_reset, dim3((kCtrlWords + 255) / 256), dim3(256), 0, s, B.ctrl, B.stats,
                        c == 0 ? 1 : 0);
     hipLaunchKernelGGL(k_queue_counts, dim3(1), dim3(64), 0, s, B.ctrl, C.total);
@@ -1185,28 +1198,38 @@ chk_status batch_impl(chk_ctx_s *ctx, const chk_request *R, uint32_t n,
       const RenderArgs &a = seg_args[C.first_seg + k];
       const uint32_t np = a.n_pix * a.spp;
       hipLaunchKernelGGL(k_camera, dim3((np + 255) / 256), dim3(256), 0, s, a, segcap, C.total,
-                         B.q_ray[0], B.q_thr[0], B.q_pid[0], B.path_L, q_seg[0], k);
+                         B.q_ray[0], B.q_thr[0], B.q_pid[0], B.path_L, single ? nullptr : q_seg[0],
+                         k);
     }
     for (uint32_t depth = 0; depth <= max_depth; ++depth) {
       const uint32_t in = depth & 1u, out = in ^ 1u;
       hipLaunchKernelGGL(k_reset_bounce, dim3(1), dim3(64), 0, s, B.ctrl, qbase(out));
-      hipError_t e = launch_closest(nullptr, shape, B.q_ray[in],
-                                    SegRange{B.ctrl + qbase(in), 0u, segcap}, B.hits, true, false,
-                                    B.stats + kNodeVisits,
-                                    SlotScenes{d_scene + C.first_seg, q_seg[in]}, s);
+      hipError_t e =
+          single ? launch_closest(ssc, sshape, B.q_ray[in], SegRange{B.ctrl + qbase(in), 0u, segcap},
+                                  B.hits, true, false, B.stats + kNodeVisits, SlotScenes{}, s)
+                 : launch_closest(nullptr, shape, B.q_ray[in],
+                                  SegRange{B.ctrl + qbase(in), 0u, segcap}, B.hits, true, false,
+                                  B.stats + kNodeVisits,
+                                  SlotScenes{d_scene + C.first_seg, q_seg[in]}, s);
       if (e != hipSuccess) return e;
       const BatchIO Q{d_args + C.first_seg, q_seg[in], q_seg[out], s_seg};
       const uint32_t sgrid = kSegs * (segcap / T.shade_block);
       if (T.shade_block == 128)
-        launch_shade_multi<128>(sgrid, s, depth, segcap, B, in, out, qbase(in), qbase(out), Q);
+        launch_shade_batch<128>(sgrid, s, depth, segcap, B, in, out, qbase(in), qbase(out), Q,
+                                single);
       else if (T.shade_block == 512)
-        launch_shade_multi<512>(sgrid, s, depth, segcap, B, in, out, qbase(in), qbase(out), Q);
+        launch_shade_batch<512>(sgrid, s, depth, segcap, B, in, out, qbase(in), qbase(out), Q,
+                                single);
       else
-        launch_shade_multi<256>(sgrid, s, depth, segcap, B, in, out, qbase(in), qbase(out), Q);
+        launch_shade_batch<256>(sgrid, s, depth, segcap, B, in, out, qbase(in), qbase(out), Q,
+                                single);
       if (depth < max_depth) {
-        e = launch_any(nullptr, shape, B.s_ray, SegRange{B.ctrl + kQS, 0u, segcap}, nullptr,
-                       B.s_L, B.path_L, true, false, B.stats + kShNodeVisits,
-                       SlotScenes{d_scene + C.first_seg, s_seg}, s);
+        e = single ? launch_any(ssc, sshape, B.s_ray, SegRange{B.ctrl + kQS, 0u, segcap}, nullptr,
+                                B.s_L, B.path_L, true, false, B.stats + kShNodeVisits,
+                                SlotScenes{}, s)
+                   : launch_any(nullptr, shape, B.s_ray, SegRange{B.ctrl + kQS, 0u, segcap},
+                                nullptr, B.s_L, B.path_L, true, false, B.stats + kShNodeVisits,
+                                SlotScenes{d_scene + C.first_seg, s_seg}, s);
         if (e != hipSuccess) return e;
       }
     }
@@ -1302,9 +1325,11 @@ chk_status batch_impl(chk_ctx_s *ctx, const chk_request *R, uint32_t n,
   return CHK_OK;
 }
 
-// Autotuner: time every (chunk_paths, trace_block, shade_block) candidate
-// on the batch with spp divided by 4 (at least 1), rendering into scratch
-// accumulators, then eager against graph replay for the fastest.
+// Autotuner: time every (chunk_paths, trace_block) candidate on the batch
+// itself (the chunk size that suits a request mix depends on the mix at
+// its real spp), rendering into scratch accumulators, then eager against
+// graph replay for the fastest.  The first call of each chunk size grows
+// the buffers; each candidate is timed on its second call.
 chk_status batch_autotune(chk_ctx_s *ctx, const chk_request *R, uint32_t n,
                           chk_batch_tuning *best, double *best_ms, hipStream_t st) {
   if (n == 0) {
@@ -1321,30 +1346,27 @@ chk_status batch_autotune(chk_ctx_s *ctx, const chk_request *R, uint32_t n,
   }
   std::vector<chk_request> Q(R, R + n);
   for (auto &q : Q) {
-    q.params.spp = std::max(1u, q.params.spp / 4);
     // every request adds into the same scratch: the values are discarded
     q.accum = scratch;
     q.sumsq = scratch + maxpix * 3;
   }
   const uint32_t chunk_c[] = {1u << 22, 1u << 23, 1u << 24, 1u << 25};
-  const uint32_t tb_c[] = {64, 128, 256};
-  const uint32_t sb_c[] = {128, 256, 512};
+  const uint32_t tb_c[] = {64, 128};
   double bms = 1e30;
   chk_batch_tuning bt{1u << 24, 128, 256, 0};
   chk_status s = CHK_OK;
   for (uint32_t c : chunk_c)
-    for (uint32_t tb : tb_c)
-      for (uint32_t sb : sb_c) {
-        chk_batch_tuning t{c, tb, sb, 0};
-        chk_batch_stats bst;
-        for (int rep = 0; rep < 2 && s == CHK_OK; ++rep)
-          s = batch_impl(ctx, Q.data(), n, &t, nullptr, &bst, st);
-        if (s != CHK_OK) break;
-        if (bst.batch_ms < bms) {
-          bms = bst.batch_ms;
-          bt = t;
-        }
+    for (uint32_t tb : tb_c) {
+      chk_batch_tuning t{c, tb, 256, 0};
+      chk_batch_stats bst;
+      for (int rep = 0; rep < (tb == tb_c[0] ? 2 : 1) && s == CHK_OK; ++rep)
+        s = batch_impl(ctx, Q.data(), n, &t, nullptr, &bst, st);
+      if (s != CHK_OK) break;
+      if (bst.batch_ms < bms) {
+        bms = bst.batch_ms;
+        bt = t;
       }
+    }
   if (s == CHK_OK) {
     chk_batch_tuning g = bt;
     g.use_graph = 1;

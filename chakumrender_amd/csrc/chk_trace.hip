@@ -315,6 +315,88 @@ __global__ __launch_bounds__(256) void k_spheres(const float4 *__restrict__ sph,
   hits[i] = make_float4(id == 0xFFFFFFFFu ? 1e30f : (float)best, __uint_as_float(id), 0.0f, 0.0f);
 }
 
+// ---------------------------------------------------------------- DummyAccel
+// The reference's other accelerator (DummyAccel, dummyAccel.cpp:10-59): the
+// triangles in their original order split into floor(sqrt(n)) runs of
+// floor(sqrt(n)) (the last run takes the rest), one box per run; a ray tests
+// the triangles of every run whose box it crosses, in order, keeping a hit
+// only when strictly nearer (so the first of equally near triangles wins).
+// Run boxes: (lo, first) (hi, end) as two float4.
+__global__ __launch_bounds__(256) void k_dummy_boxes(DeviceScene S, uint32_t n_runs,
+                                                     uint32_t run_len, float4 *__restrict__ box) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n_runs) return;
+  const uint32_t first = g * run_len, end = g + 1 == n_runs ? S.n_tris : first + run_len;
+  float lo[3] = {kInf, kInf, kInf}, hi[3] = {-kInf, -kInf, -kInf};
+  for (uint32_t t = first; t < end; ++t) {
+    const uint32_t leaf = S.orig_info[t].x;
+    if (leaf == 0xFFFFFFFFu) continue;  // zero-area triangle: never hit
+    const float4 *tp = S.tris + 3 * (size_t)leaf;
+    for (int v = 0; v < 3; ++v) {
+      const float4 q = tp[v];
+      lo[0] = fminf(lo[0], q.x), hi[0] = fmaxf(hi[0], q.x);
+      lo[1] = fminf(lo[1], q.y), hi[1] = fmaxf(hi[1], q.y);
+      lo[2] = fminf(lo[2], q.z), hi[2] = fmaxf(hi[2], q.z);
+    }
+  }
+  box[2 * g] = make_float4(lo[0], lo[1], lo[2], __uint_as_float(first));
+  box[2 * g + 1] = make_float4(hi[0], hi[1], hi[2], __uint_as_float(end));
+}
+
+// One ray per lane; the runs and their triangles are wave-uniform.
+template <bool ANY>
+__global__ __launch_bounds__(256) void k_dummy_trace(DeviceScene S, const float4 *__restrict__ box,
+                                                     uint32_t n_runs,
+                                                     const float4 *__restrict__ rays, uint32_t n,
+                                                     float4 *__restrict__ hits,
+                                                     uint8_t *__restrict__ occluded) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = i < n;
+  const float4 ot = live ? rays[2 * (size_t)i] : make_float4(0, 0, 0, 0);
+  const float4 ds = live ? rays[2 * (size_t)i + 1] : make_float4(0, 0, 1, 0);
+  const f3 o = xyz(ot), d = xyz(ds);
+  const uint32_t skip = __float_as_uint(ds.w);
+  const RayPre pre = ray_pre(d);
+  const f3 inv = mk(safe_rcp(d.x), safe_rcp(d.y), safe_rcp(d.z));
+  const float kGrow = 1.0f + 2.0f * 3.5762786865234375e-07f;  // 1 + 2*gamma_3
+  float t = live ? ot.w : 0.0f, bu = 0.0f, bv = 0.0f;
+  uint32_t best = 0xFFFFFFFFu, best_leaf = 0u;
+  bool done = !live;
+  for (uint32_t g = 0; g < n_runs; ++g) {
+    const float4 lo = box[2 * g], hi = box[2 * g + 1];
+    const float tx0 = (lo.x - o.x) * inv.x, tx1 = (hi.x - o.x) * inv.x;
+    const float ty0 = (lo.y - o.y) * inv.y, ty1 = (hi.y - o.y) * inv.y;
+    const float tz0 = (lo.z - o.z) * inv.z, tz1 = (hi.z - o.z) * inv.z;
+    const float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
+    const float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), t));
+    const bool in_box = !done && tn <= tf * kGrow;
+    if (!__any(in_box)) continue;  // wave-uniform
+    const uint32_t first = __float_as_uint(lo.w), end = __float_as_uint(hi.w);
+    for (uint32_t tid = first; tid < end; ++tid) {
+      const uint32_t leaf = S.orig_info[tid].x;
+      if (leaf == 0xFFFFFFFFu) continue;  // uniform
+      const float4 *tp = S.tris + 3 * (size_t)leaf;
+      float tt = t, uu, vv;
+      if (in_box && tid != skip && tri_hit(pre, o, d, xyz(tp[0]), xyz(tp[1]), xyz(tp[2]), tt, uu, vv)) {
+        t = tt;
+        best = tid;
+        best_leaf = leaf;
+        bu = uu;
+        bv = vv;
+        if (ANY) done = true;
+      }
+    }
+    if (ANY && !__any(!done)) break;  // every lane occluded (or idle)
+  }
+  if (!live) return;
+  if (ANY) {
+    occluded[i] = best != 0xFFFFFFFFu ? 1 : 0;
+  } else {
+    const float tr = best == 0xFFFFFFFFu ? 1e30f : refine_t(S.tris + 3 * (size_t)best_leaf, o, d);
+    hits[i] = make_float4(tr, __uint_as_float(best), bu, bv);
+  }
+}
+
 __global__ void k_zero_u32(uint32_t *p, uint32_t n) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0u;
@@ -444,6 +526,28 @@ hipError_t launch_spheres(const float *spheres, uint32_t ns, const float4 *rays,
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_spheres, dim3((n + 255) / 256), dim3(256), 0, st,
                      reinterpret_cast<const float4 *>(spheres), ns, rays, n, hits);
+  return hipGetLastError();
+}
+
+hipError_t launch_dummy_boxes(const chk_scene_s *s, uint32_t n_runs, uint32_t run_len,
+                              float4 *box, hipStream_t st) {
+  if (n_runs == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_dummy_boxes, dim3((n_runs + 255) / 256), dim3(256), 0, st, s->d, n_runs,
+                     run_len, box);
+  return hipGetLastError();
+}
+
+hipError_t launch_dummy_trace(const chk_scene_s *s, bool any, const float4 *box, uint32_t n_runs,
+                              const float4 *rays, uint32_t n, float4 *hits, uint8_t *occluded,
+                              hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  const dim3 g((n + 255) / 256), b(256);
+  if (any)
+    hipLaunchKernelGGL(k_dummy_trace<true>, g, b, 0, st, s->d, box, n_runs, rays, n, nullptr,
+                       occluded);
+  else
+    hipLaunchKernelGGL(k_dummy_trace<false>, g, b, 0, st, s->d, box, n_runs, rays, n, hits,
+                       nullptr);
   return hipGetLastError();
 }
 

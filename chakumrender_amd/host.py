@@ -486,7 +486,7 @@ def render_batch(ctx: abi.Context, items, tuning=None, stream=None):
 
 
 def batch_autotune(ctx: abi.Context, items, stream=None):
-    """chk_batch_autotune: the fastest tuning (dict) and its batch time at spp / 4."""
+    """chk_batch_autotune: the fastest tuning (dict) and its batch time."""
     arr, images = _requests(items)
     T = abi.chk_batch_tuning()
     ms = C.c_double(0)

@@ -158,6 +158,18 @@ chk_status chk_scene_destroy(chk_scene_t scene);
  * second trace: rays of ordinary scenes overflow it. */
 chk_status chk_scene_set_trace_stack(chk_scene_t scene, uint32_t entries);
 chk_status chk_scene_get_info(chk_scene_t scene, chk_scene_info *info);
+/* Accelerator of chk_trace_closest and chk_trace_any: CHK_ACCEL_BVH (the
+ * default; BvhAccel, bvhAccel.h:71-74) or CHK_ACCEL_DUMMY, the reference's
+ * DummyAccel (dummyAccel.h:11-20, dummyAccel.cpp:10-59): the triangles in
+ * their original order in floor(sqrt(n)) runs with one box each, every
+ * triangle of every box the ray crosses tested in order, the first of
+ * equally near ones kept.  Hits and ids are those of the BVH (the reference's
+ * DummyAccel reports the run index as the id, dummyAccel.cpp:49; here it is
+ * the triangle's, and the run is id / floor(sqrt(n)), at most the last run).
+ * The renders always traverse the BVH. */
+#define CHK_ACCEL_BVH 0u
+#define CHK_ACCEL_DUMMY 1u
+chk_status chk_scene_set_accel(chk_scene_t scene, uint32_t kind);
 
 /* Closest hit for n rays.  Replaces BvhAccel::intersectT
  * (bvhAccel.cpp:200-249) batched.  rays, hits: dev. */
@@ -353,10 +365,10 @@ typedef struct chk_batch_stats {
 chk_status chk_render_batch(chk_ctx_t ctx, const chk_request *reqs, uint32_t n,
                             const chk_batch_tuning *tuning, double *latency_ms,
                             chk_batch_stats *stats, void *stream);
-/* Autotuner: times every chunk_paths x trace_block x shade_block candidate on
- * the batch (spp / 4, scratch accumulators; the requests' own images are not
- * touched), then eager launches against graph replay for the fastest.
- * best_ms (may be NULL): its batch time at spp / 4. */
+/* Autotuner: times every chunk_paths (2^22 .. 2^25) x trace_block (64, 128)
+ * candidate on the batch (scratch accumulators; the requests' own images are
+ * not touched), then eager launches against graph replay for the fastest.
+ * best_ms (may be NULL): its batch time. */
 chk_status chk_batch_autotune(chk_ctx_t ctx, const chk_request *reqs, uint32_t n,
                               chk_batch_tuning *best, double *best_ms, void *stream);
 

@@ -83,9 +83,9 @@ def test_graph_replays_match_eager(ctx, scene_set):
 def test_autotune_choice_renders_identically(ctx, scene_set):
     items = batch_items(scene_set)
     best, ms = host.batch_autotune(ctx, items)
-    print(f"autotuned {best}, {ms:.3f} ms at spp/4")
-    assert best["chunk_paths"] in (1 << 21, 1 << 22, 1 << 23)
-    assert best["trace_block"] in (64, 128, 256) and best["shade_block"] in (128, 256, 512)
+    print(f"autotuned {best}, {ms:.3f} ms")
+    assert best["chunk_paths"] in (1 << 22, 1 << 23, 1 << 24, 1 << 25)
+    assert best["trace_block"] in (64, 128) and best["shade_block"] == 256
     ref = single(items)
     got, _, _ = host.render_batch(ctx, items, best)
     for (a, _), (ra, _) in zip(got, ref):

@@ -176,3 +176,40 @@ def test_spheres_match_reference_intersectS(golden):
     far = (G["id"] >= 0) & ok & (np.abs(h[:, 0] - G["t"]) > 1e-6 * G["t"])
     print(f"t beyond 1e-6 of the reference from direction rounding: {far.sum()}")
     assert far.mean() < 0.01
+
+
+@pytest.mark.parametrize("name", ["cornell", "ico"])
+def test_dummy_accel_equals_bvh_and_reference(ctx, golden, name):
+    """CHK_ACCEL_DUMMY (the reference's DummyAccel: sqrt(n) runs of triangles
+    in original order, each with its box) returns the BVH's hits: ids bit
+    for bit and the same FP64-refined t, against the reference's golden
+    closest hits as the BVH test does; occlusion equal to the BVH's."""
+    G = golden("bvh_closest.npz")
+    tris, rays = G[f"{name}_tris"], G[f"{name}_rays"]
+    sc = trace_scene(tris)
+    acc = host.BvhAccel(ctx, sc)
+    r = host.make_rays(rays[:, :3], rays[:, 3:6])
+    hb = acc.closest_raw(r)
+    acc.ds.set_accel(abi.ACCEL_DUMMY)
+    print(f"{name}: dummy accel over {tris.shape[0]} triangles, {rays.shape[0]} rays", flush=True)
+    hd = acc.closest_raw(r)
+    gid = hd[:, 1].view(np.int32).copy()
+    b = oracle.closest_brute(tris.astype(np.float64), rays)
+    check_closest(tris, rays, G[f"{name}_id"], G[f"{name}_t"], gid, hd[:, 0].astype(np.float64), b["t2"])
+    # equal to the BVH's hits outside the tie band (of two equally near
+    # triangles DummyAccel keeps the first in the original order)
+    band, _ = classify_closest(tris, rays, G[f"{name}_id"], G[f"{name}_t"], gid,
+                               hd[:, 0].astype(np.float64), b["t2"])
+    np.testing.assert_array_equal(gid[~band], hb[~band, 1].view(np.int32))
+    np.testing.assert_array_equal(hd[~band, 0], hb[~band, 0])
+    # occlusion: segments to the closest hit (shortened) and beyond it
+    t = np.where(hb[:, 0] < 1e29, hb[:, 0], 10.0).astype(np.float64)
+    for scale in (0.5, 2.0):
+        sh = host.make_rays(rays[:, :3], rays[:, 3:6], tmax=t * scale)
+        od = acc.occluded_raw(sh)
+        acc.ds.set_accel(abi.ACCEL_BVH)
+        ob = acc.occluded_raw(sh)
+        acc.ds.set_accel(abi.ACCEL_DUMMY)
+        np.testing.assert_array_equal(od, ob)
+    with pytest.raises(abi.ChkError, match="kind"):
+        acc.ds.set_accel(7)
