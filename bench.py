@@ -696,6 +696,8 @@ def main():
     abi.sync()
     ext_ms = shade_ms = shadow_ms = other_ms = 0.0
     launches = 0
+    shadow_launches = cnt["extend_launches"] - cnt["chunks"]  # no shadow launch after the last bounce
+    shadow_launches_timed = 0
     t_start = time.perf_counter()
     for _ in range(args.steps):
         _, st = host.render(ds, prof, image=image, download=False)
@@ -704,6 +706,7 @@ def main():
         shadow_ms += st["shadow_ms"]
         other_ms += st["other_ms"]
         launches += st["extend_launches"]
+        shadow_launches_timed += st["extend_launches"] - st["chunks"]
     abi.sync()
     elapsed = time.perf_counter() - t_start
 
@@ -720,9 +723,14 @@ def main():
     alg_bytes_launch = alg_bytes_total / cnt["extend_launches"]
     avg_launch_s = ext_ms / 1e3 / launches
     achieved = alg_bytes_launch / avg_launch_s / 1e9
-    traffic = None
+    traffic, meas = None, {}
     if args.traffic and Path(args.traffic).exists():
-        traffic = json.loads(Path(args.traffic).read_text()).get("bytes_per_launch")
+        meas = json.loads(Path(args.traffic).read_text())
+        traffic = meas.get("bytes_per_launch")
+    # any-hit: the same work model over the shadow rays
+    sh_bytes_launch = ((cnt["shadow_node_visits"] * NODE_BYTES + cnt["shadow_tri_tests"] * TRI_BYTES
+                        + cnt["rays_shadow"] * RAY_BYTES) / max(1, shadow_launches))
+    sh_launch_s = shadow_ms / 1e3 / max(1, shadow_launches_timed)
 
     line = {
         "metric": "Mpaths/s",
@@ -761,6 +769,30 @@ def main():
             "bytes_per_launch": int(alg_bytes_launch),
             "avg_launch_ms": round(avg_launch_s * 1e3, 4),
             "work_model": f"{NODE_BYTES} B/node visit + {TRI_BYTES} B/triangle test + {RAY_BYTES} B/ray",
+            "what_the_frac_is": "algorithmic node/triangle/ray bytes per second against the HBM peak; "
+                                "the caches serve almost all of them (measured below)",
+            "measured": None if not meas else {
+                "hbm_bytes_per_launch": traffic,
+                "hbm_gbs": round(traffic / avg_launch_s / 1e9, 1),
+                "hbm_frac": round(traffic / avg_launch_s / 1e9 / HBM_PEAK_GBS, 4),
+                "l1_hit_rate": meas.get("l1_hit_rate"), "l2_hit_rate": meas.get("l2_hit_rate"),
+                "valu_busy_pct": meas.get("valu_busy_pct"),
+                "valu_utilization_pct": meas.get("valu_utilization_pct"),
+                "waves_per_cu": meas.get("waves_per_cu"),
+                "bound_by": "divergent VALU issue and dependent-fetch latency with L1/L2-resident "
+                            "nodes (profiles/traversal_counters_icosphere100k_1080p.md)"},
+        },
+        "any_hit": {
+            "kernel": "k_trace<any>",
+            "rays_per_frame": cnt["rays_shadow"],
+            "node_visits_per_ray": round(cnt["shadow_node_visits"] / max(1, cnt["rays_shadow"]), 2),
+            "tri_tests_per_ray": round(cnt["shadow_tri_tests"] / max(1, cnt["rays_shadow"]), 2),
+            "closest_node_visits_per_ray": round(cnt["node_visits"] / max(1, cnt["rays_extend"]), 2),
+            "closest_tri_tests_per_ray": round(cnt["tri_tests"] / max(1, cnt["rays_extend"]), 2),
+            "avg_launch_ms": round(sh_launch_s * 1e3, 4),
+            "achieved_gbs": round(sh_bytes_launch / sh_launch_s / 1e9, 1) if sh_launch_s > 0 else None,
+            "grays_per_s": round(cnt["rays_shadow"] * args.steps / (shadow_ms / 1e3) / 1e9, 3)
+                           if shadow_ms > 0 else None,
         },
     }
 

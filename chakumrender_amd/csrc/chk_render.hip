@@ -17,7 +17,7 @@
 //   a path stays in one segment (one XCD under round-robin dispatch) for its
 //   whole life and appends never contend across XCDs.
 //     any hit over the shadow queue                (shader.cpp:76-97)
-//   k_accumulate: per-pixel FP64 sums in sample order (main.cpp:62,83)
+//   k_accumulate: per-pixel FP64 sums, G lanes per pixel (main.cpp:62,83)
 // Every path owns one FP32 radiance slot; queue order never changes a
 // path's result (its randomness is keyed by pixel, sample and depth).
 #include <chrono>
@@ -590,23 +590,48 @@ __global__ __launch_bounds__(256) void k_prop_in(RenderArgs A, uint32_t segcap,
   for (int k = 0; k < 7; ++k) o[k] = s[k];
 }
 
-// Per-pixel FP64 accumulation in sample order (main.cpp:62,83).
-__global__ void k_accumulate(RenderArgs A, const float4 *__restrict__ path_L,
-                             double *__restrict__ accum, double *__restrict__ sumsq) {
-  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= A.n_pix) return;
+// Per-pixel FP64 accumulation (main.cpp:62,83).  G lanes share a pixel (G
+// the largest power of two <= min(spp, 64)): lane j adds samples j, j + G,
+// ... in order, then the G partial sums are combined by a fixed butterfly,
+// so the result depends only on spp, and a wave reads its pixels' path
+// radiances as contiguous memory.
+__device__ inline uint32_t acc_lanes(uint32_t spp) {
+  uint32_t g = 1;
+  while (g < 64 && 2 * g <= spp) g *= 2;
+  return g;
+}
+__global__ __launch_bounds__(256) void k_accumulate(RenderArgs A, const float4 *__restrict__ path_L,
+                                                    double *__restrict__ accum,
+                                                    double *__restrict__ sumsq) {
+  const uint32_t G = acc_lanes(A.spp);
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t q = t / G, j = t % G;
+  const bool live = q < A.n_pix;
   double s0 = 0, s1 = 0, s2 = 0, q0 = 0, q1 = 0, q2 = 0;
-  const float4 *p = path_L + A.p0 + (size_t)q * A.spp;
-  for (uint32_t s = 0; s < A.spp; ++s) {
-    const float4 L = p[s];
-    const double a = L.x, b = L.y, c = L.z;
-    s0 += a;
-    s1 += b;
-    s2 += c;
-    q0 += a * a;
-    q1 += b * b;
-    q2 += c * c;
+  if (live) {
+    const float4 *p = path_L + A.p0 + (size_t)q * A.spp;
+    for (uint32_t s = j; s < A.spp; s += G) {
+      const float4 L = p[s];
+      const double a = L.x, b = L.y, c = L.z;
+      s0 += a;
+      s1 += b;
+      s2 += c;
+      q0 += a * a;
+      q1 += b * b;
+      q2 += c * c;
+    }
   }
+  // every lane of the wave takes part (G divides 64, so groups never
+  // straddle waves)
+  for (uint32_t o = G / 2; o > 0; o /= 2) {
+    s0 += __shfl_xor(s0, o);
+    s1 += __shfl_xor(s1, o);
+    s2 += __shfl_xor(s2, o);
+    q0 += __shfl_xor(q0, o);
+    q1 += __shfl_xor(q1, o);
+    q2 += __shfl_xor(q2, o);
+  }
+  if (!live || j != 0) return;
   const size_t o = 3 * (size_t)image_pixel(A, A.pix0 + q);
   accum[o] += s0;
   accum[o + 1] += s1;
@@ -616,6 +641,13 @@ __global__ void k_accumulate(RenderArgs A, const float4 *__restrict__ path_L,
     sumsq[o + 1] += q1;
     sumsq[o + 2] += q2;
   }
+}
+
+// Grid of k_accumulate for n_pix pixels of spp samples.
+inline uint32_t accumulate_blocks(uint32_t n_pix, uint32_t spp) {
+  uint32_t g = 1;
+  while (g < 64 && 2 * g <= spp) g *= 2;
+  return (uint32_t)(((uint64_t)n_pix * g + 255) / 256);
 }
 
 struct Profiler {
@@ -928,7 +960,8 @@ chk_status render_impl(chk_scene_s *sc, const chk_camera *cam, const chk_render_
       }
     }
     prof.begin(3);
-    hipLaunchKernelGGL(k_accumulate, dim3((A.n_pix + 255) / 256), dim3(256), 0, st, A, B.path_L,
+    hipLaunchKernelGGL(k_accumulate, dim3(accumulate_blocks(A.n_pix, A.spp)), dim3(256), 0, st, A,
+                       B.path_L,
                        accum, (P->flags & CHK_RENDER_SUMSQ) ? sumsq : nullptr);
     prof.end();
   }
@@ -1236,7 +1269,8 @@ chk_status batch_impl(chk_ctx_s *ctx, const chk_request *R, uint32_t n,
     for (uint32_t k = 0; k < C.n_seg; ++k) {
       const RenderArgs &a = seg_args[C.first_seg + k];
       const chk_request &q = R[segs[C.first_seg + k].req];
-      hipLaunchKernelGGL(k_accumulate, dim3((a.n_pix + 255) / 256), dim3(256), 0, s, a, B.path_L,
+      hipLaunchKernelGGL(k_accumulate, dim3(accumulate_blocks(a.n_pix, a.spp)), dim3(256), 0, s, a,
+                         B.path_L,
                          q.accum, (q.params.flags & CHK_RENDER_SUMSQ) ? q.sumsq : nullptr);
     }
     return hipGetLastError();

@@ -444,15 +444,16 @@ hipError_t launch_s(const chk_scene_s *s, int stack, const TraceShape &T, const 
 #undef CHK_LAUNCH
 }
 
-// The listed rays again with the tree's full stack (fixed grid: one block
-// per CU; an empty list costs one short launch).
+// The listed rays again with the tree's full stack (fixed grid: four
+// blocks per CU, 8 waves per CU within the LDS of the 64-entry stack; an
+// empty list costs one short launch).
 template <bool ANY, bool MULTI>
 hipError_t launch_retry(const chk_scene_s *s, int stack, const float4 *rays, float4 *hits,
                         uint8_t *occ, const float4 *sL, float4 *pL, bool leaf_ids,
                         unsigned long long *counters, const SlotScenes &M, hipStream_t st,
                         const uint32_t *ovf, uint32_t ovf_cap) {
   const DeviceScene S = s ? s->d : DeviceScene{};
-  const dim3 g(256), b(128);
+  const dim3 g(1024), b(128);
   if (stack <= 32)
     hipLaunchKernelGGL((k_trace_retry<ANY, 32, 128, MULTI>), g, b, 0, st, S, rays, hits, occ, sL,
                        pL, leaf_ids ? 1 : 0, counters, M, ovf, ovf_cap);
