@@ -176,33 +176,49 @@ LBuffers lcarve(void *base, uint32_t qcap) {
 
 // Block-wide compacting append into two queues: every thread of the block
 // calls it once; threads with pa (pb) get consecutive slots after the
-// block's base in queue a (b).  One atomic per queue per block.
+// block's base in queue a (b), ordered by key ka (kb) in 0..kBins-1 and, for
+// equal keys, by thread.  The keys are the rays' direction octants: the
+// waves of the next traversal then hold rays of one octant from a few
+// neighbouring paths, which walk the same nodes in the same order.  One
+// atomic per queue per block; the order is a function of the block's
+// contents only.
+constexpr uint32_t kBins = 8;
+__device__ inline uint32_t octant(float4 dir) {
+  return (dir.x < 0.0f ? 1u : 0u) | (dir.y < 0.0f ? 2u : 0u) | (dir.z < 0.0f ? 4u : 0u);
+}
 template <int NW>
-__device__ inline void block_append2(bool pa, bool pb, uint32_t *ca, uint32_t *cb, uint32_t &sa,
-                                     uint32_t &sb) {
-  __shared__ uint32_t cnt[2][NW];
+__device__ inline void block_append2(bool pa, bool pb, uint32_t ka, uint32_t kb, uint32_t *ca,
+                                     uint32_t *cb, uint32_t &sa, uint32_t &sb) {
+  __shared__ uint32_t cnt[2][kBins][NW];
   __shared__ uint32_t base[2];
   const uint32_t lane = __lane_id(), w = threadIdx.x >> 6;
-  const unsigned long long ma = __ballot(pa), mb = __ballot(pb);
-  if (lane == 0) {
-    cnt[0][w] = (uint32_t)__popcll(ma);
-    cnt[1][w] = (uint32_t)__popcll(mb);
+  const unsigned long long below = (1ull << lane) - 1ull;
+  unsigned long long mine_a = 0, mine_b = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < kBins; ++k) {
+    const unsigned long long ma = __ballot(pa && ka == k), mb = __ballot(pb && kb == k);
+    mine_a = ka == k ? ma : mine_a;
+    mine_b = kb == k ? mb : mine_b;
+    if (lane == 0) {
+      cnt[0][k][w] = (uint32_t)__popcll(ma);
+      cnt[1][k][w] = (uint32_t)__popcll(mb);
+    }
   }
   __syncthreads();
+  // exclusive scan over (bin, wave) in that order, then the block's base
   if (threadIdx.x < 2) {
     uint32_t tot = 0;
-    for (int k = 0; k < NW; ++k) tot += cnt[threadIdx.x][k];
+    for (uint32_t k = 0; k < kBins; ++k)
+      for (int v = 0; v < NW; ++v) {
+        const uint32_t c = cnt[threadIdx.x][k][v];
+        cnt[threadIdx.x][k][v] = tot;
+        tot += c;
+      }
     base[threadIdx.x] = tot ? atomicAdd(threadIdx.x ? cb : ca, tot) : 0u;
   }
   __syncthreads();
-  uint32_t oa = base[0], ob = base[1];
-  for (uint32_t k = 0; k < w; ++k) {
-    oa += cnt[0][k];
-    ob += cnt[1][k];
-  }
-  const unsigned long long below = (1ull << lane) - 1ull;
-  sa = oa + (uint32_t)__popcll(ma & below);
-  sb = ob + (uint32_t)__popcll(mb & below);
+  sa = base[0] + cnt[0][pa ? ka : 0][w] + (uint32_t)__popcll(mine_a & below);
+  sb = base[1] + cnt[1][pb ? kb : 0][w] + (uint32_t)__popcll(mine_b & below);
 }
 
 __global__ void k_reset(uint32_t *ctrl, unsigned long long *stats, int clear_stats) {
@@ -528,7 +544,7 @@ __global__ __launch_bounds__(BS) void k_shade(
     }
   }
   uint32_t slot, sslot;
-  block_append2<BS / 64>(emit_ray, emit_sh, ctrl + out_q + r * kSegStride,
+  block_append2<BS / 64>(emit_ray, emit_sh, octant(nr1), octant(sr1), ctrl + out_q + r * kSegStride,
                          ctrl + kQS + r * kSegStride, slot, sslot);
   const size_t seg0 = (size_t)r * segcap;
   if (emit_ray) {
@@ -590,14 +606,16 @@ __global__ __launch_bounds__(256) void k_prop_in(RenderArgs A, uint32_t segcap,
   for (int k = 0; k < 7; ++k) o[k] = s[k];
 }
 
-// Per-pixel FP64 accumulation (main.cpp:62,83).  G lanes share a pixel (G
-// the largest power of two <= min(spp, 64)): lane j adds samples j, j + G,
-// ... in order, then the G partial sums are combined by a fixed butterfly,
-// so the result depends only on spp, and a wave reads its pixels' path
-// radiances as contiguous memory.
-__device__ inline uint32_t acc_lanes(uint32_t spp) {
+// Per-pixel FP64 accumulation (main.cpp:62,83).  G lanes share a pixel
+// (acc_lanes: up to 8 below 4096 spp, where a chunk holds many pixels; 64
+// from 4096 spp, where it holds a few hundred): lane j adds samples j,
+// j + G, ... in order, then the G partial sums are combined by a fixed
+// butterfly, so the result depends only on spp, and G lanes read a pixel's
+// consecutive path radiances together.
+__host__ __device__ inline uint32_t acc_lanes(uint32_t spp) {
+  const uint32_t cap = spp >= 4096 ? 64u : 8u;
   uint32_t g = 1;
-  while (g < 64 && 2 * g <= spp) g *= 2;
+  while (g < cap && 2 * g <= spp) g *= 2;
   return g;
 }
 __global__ __launch_bounds__(256) void k_accumulate(RenderArgs A, const float4 *__restrict__ path_L,
@@ -610,6 +628,7 @@ __global__ __launch_bounds__(256) void k_accumulate(RenderArgs A, const float4 *
   double s0 = 0, s1 = 0, s2 = 0, q0 = 0, q1 = 0, q2 = 0;
   if (live) {
     const float4 *p = path_L + A.p0 + (size_t)q * A.spp;
+#pragma unroll 4
     for (uint32_t s = j; s < A.spp; s += G) {
       const float4 L = p[s];
       const double a = L.x, b = L.y, c = L.z;
@@ -645,9 +664,7 @@ __global__ __launch_bounds__(256) void k_accumulate(RenderArgs A, const float4 *
 
 // Grid of k_accumulate for n_pix pixels of spp samples.
 inline uint32_t accumulate_blocks(uint32_t n_pix, uint32_t spp) {
-  uint32_t g = 1;
-  while (g < 64 && 2 * g <= spp) g *= 2;
-  return (uint32_t)(((uint64_t)n_pix * g + 255) / 256);
+  return (uint32_t)(((uint64_t)n_pix * acc_lanes(spp) + 255) / 256);
 }
 
 struct Profiler {
