@@ -1014,11 +1014,13 @@ chk_status render_impl(chk_scene_s *sc, const chk_camera *cam, const chk_render_
 
 // ====================================================================== batching
 // Many requests (scenes, cameras, parameters) in one wavefront: requests are
-// ordered by cost (W * H * spp, shortest first) and packed into chunks of at
-// most chunk_paths paths; a chunk holds segments (pixel ranges) of several
-// requests.  Every queue slot carries its segment index, so the traversal
-// kernels find the ray's scene and the shading kernel the request's
-// arguments.  Per chunk the sequence is the single-request one; with
+// ordered by cost (W * H * spp, shortest first).  Requests below a quarter
+// of chunk_paths are packed into shared chunks of at most chunk_paths paths,
+// which hold segments (pixel ranges) of several requests: every queue slot
+// carries its segment index, so the traversal kernels find the ray's scene
+// and the shading kernel the request's arguments.  A larger request gets
+// chunks of its own and runs the single-scene kernels, as chk_render would.
+// Per chunk the sequence is the single-request one; with
 // use_graph each chunk's launches are captured into a hipGraph once and
 // replayed; the executable graphs are kept until a call with other
 // requests captures again or the buffers they name are reallocated.  Eager
@@ -1155,8 +1157,14 @@ chk_status batch_impl(chk_ctx_s *ctx, const chk_request *R, uint32_t n,
   uint64_t total_paths = 0;
   {
     BatchChunk cur{0, 0, 0};
+    const uint64_t own = T.chunk_paths / 4;
     for (uint32_t i : order) {
       const uint32_t npix = args[i].win_w * args[i].win_h, spp = args[i].spp;
+      const bool alone = (uint64_t)npix * spp >= own;
+      if (alone && cur.n_seg) {  // close the shared chunk
+        chunks.push_back(cur);
+        cur = BatchChunk{(uint32_t)segs.size(), 0, 0};
+      }
       uint32_t pix = 0;
       while (pix < npix) {
         const uint32_t room = (T.chunk_paths - cur.total) / spp;
@@ -1172,6 +1180,10 @@ chk_status batch_impl(chk_ctx_s *ctx, const chk_request *R, uint32_t n,
         pix += take;
         total_paths += (uint64_t)take * spp;
         last_chunk[i] = (uint32_t)chunks.size();
+      }
+      if (alone) {  // nothing shares the request's last chunk
+        chunks.push_back(cur);
+        cur = BatchChunk{(uint32_t)segs.size(), 0, 0};
       }
     }
     if (cur.n_seg) chunks.push_back(cur);

@@ -327,9 +327,10 @@ chk_status chk_render_shade(chk_scene_t scene, const chk_camera *camera,
  * No single reference counterpart: the reference renders one image per
  * process (main.cpp:62-94).  Many render requests -- each with its own
  * scene, camera, parameters (CHK_MIS_BALANCE) and accumulators -- run in one
- * wavefront: requests are ordered by cost (W * H * spp, shortest first) and
- * packed into chunks of tuning->chunk_paths paths; a chunk mixes requests
- * and scenes.  Each request's image equals its chk_render image exactly
+ * wavefront: requests are ordered by cost (W * H * spp, shortest first);
+ * those below a quarter of tuning->chunk_paths paths are packed into shared
+ * chunks that mix requests and scenes, larger ones get chunks of their own
+ * (single-scene kernels).  Each request's image equals its chk_render image exactly
  * (same keys, same per-path arithmetic, same per-pixel summation order). */
 typedef struct chk_request {
   chk_scene_t scene;
