@@ -129,6 +129,19 @@ def batch_leg(args, ctx):
     paths = runs[-1]["paths"]
     lat = np.array(lats[-1])
     dev_ms = float(np.mean([r["batch_ms"] for r in runs]))
+    # the same tuning through captured graphs (capture on the first call,
+    # replays after)
+    gtun = dict(best, use_graph=1)
+    gb = host.Batch(ctx, items)
+    gb.run(gtun)
+    abi.sync()
+    glats, gruns = [], []
+    for _ in range(args.batch_reps):
+        gl, gst = gb.run(gtun)
+        glats.append(gl)
+        gruns.append(gst)
+    glat = np.array(glats[-1])
+    del gb
     # the same requests one chk_render call each, in the batch's order
     # (shortest first), into their own accumulators
     imgs = [host.Image(it["camera"].width, it["camera"].height) for it in items]
@@ -158,6 +171,12 @@ def batch_leg(args, ctx):
         "latency_ms_p50": round(float(np.percentile(lat, 50)), 3),
         "latency_ms_p99": round(float(np.percentile(lat, 99)), 3),
         "tuning": best,
+        "graph": {"batch_ms": round(float(np.mean([r["batch_ms"] for r in gruns])), 3),
+                  "mpaths_per_s": round(paths / (float(np.mean([r["batch_ms"] for r in gruns])) / 1e3)
+                                        / 1e6, 2),
+                  "latency_ms_p50": round(float(np.percentile(glat, 50)), 3),
+                  "latency_ms_p99": round(float(np.percentile(glat, 99)), 3),
+                  "graph_cached": gruns[-1]["graph_cached"]},
         "autotune_s": round(tune_s, 2),
         "chunks": runs[-1]["chunks"],
         "segments": runs[-1]["segments"],

@@ -41,6 +41,12 @@ inline int mlp_param_offset_b(int layer) { return layer * (64 * 64 + 64) + 64 * 
 //   G[l]: backward A fragments, index ((it * OT + ot) * 64 + lane) * 4 + j
 //         = W[in = it*16 + lane%16][out = ot*16 + 4*(lane/16) + j]
 // Out-of-range outputs (>= 33) are zero.
+//   I[l]: inference A fragments for v_mfma_f32_16x16x32_bf16, index
+//         ((ot * 2 + kt) * 64 + lane) * 8 + j
+//         = W[in = perm_l(kt*32 + 8*(lane/16) + j)][out = ot*16 + lane%16]
+//         with the input orders of the inference kernel: perm_enc for l = 0
+//         (lane group g computes octave g of every input), perm_hid for
+//         l > 0 (the 16 activations a lane holds after a layer, in order).
 constexpr int kFragF0 = 0;
 constexpr int kFragLayerF = 16 * 64 * 4;  // 4096 bf16 per 64x64 layer
 constexpr int kFragLastF = 12 * 64 * 4;   // 3072 for the output layer
@@ -49,9 +55,28 @@ constexpr int kFragTotal = 3 * kFragLayerF + kFragLastF;  // per F or G set
 struct MlpDev {
   const uint16_t *F;  // kFragTotal bf16
   const uint16_t *G;  // kFragTotal bf16
+  const uint16_t *I;  // kFragTotal bf16
   const float *bias;  // b1 b2 b3 (64 each) b4 (33 -> 48 padded with 0): 240 floats
   uint32_t n_lights;
 };
+
+// Input orders of the inference fragments.  K position kp of a 32-wide K
+// tile pair: lane group g = (kp % 32) / 8 holds lane-local value
+// u = (kp / 32) * 8 + kp % 8 (0..15).
+//   perm_enc: u < 14: sin (u even) or cos (u odd) of 2^g pi s_(u/2), i.e.
+//   encoding feature 9 (u/2) + 1 + g or 9 (u/2) + 5 + g; u = 14, 15: feature
+//   9 e (the input s_e) for e = 2g + u - 14 < 7, else 63 (the constant 1).
+//   perm_hid: after a layer a lane holds rows 16 ot + 4 g + v of the four
+//   output tiles; u = 4 ot + v.
+__host__ __device__ constexpr int perm_enc(int kp) {
+  const int g = (kp % 32) / 8, u = (kp / 32) * 8 + kp % 8;
+  return u < 14 ? 9 * (u / 2) + ((u & 1) ? 5 : 1) + g
+                : (2 * g + u - 14 < 7 ? 9 * (2 * g + u - 14) : 63);
+}
+__host__ __device__ constexpr int perm_hid(int kp) {
+  const int g = (kp % 32) / 8, u = (kp / 32) * 8 + kp % 8;
+  return 16 * (u / 4) + 4 * g + u % 4;
+}
 
 // Per-vertex training record of the learned render (40 bytes), written by
 // the shading kernel at the vertex, completed by the shadow pass (occlusion

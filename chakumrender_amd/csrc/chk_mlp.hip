@@ -136,7 +136,8 @@ __device__ inline void proposal_head(const f4 out[3], int lane, uint32_t L, floa
   s += __shfl_xor(s, 16);
   s += __shfl_xor(s, 32);
   sum_e = s;
-  const float invL = 1.0f / (float)L, inv_s = 1.0f / s;
+  // v_rcp_f32 (1 ulp) for the two reciprocals and the logistic below
+  const float invL = __builtin_amdgcn_rcpf((float)L), inv_s = __builtin_amdgcn_rcpf(s);
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     const int o = (k >> 2) * 16 + 4 * g + (k & 3);
@@ -144,107 +145,135 @@ __device__ inline void proposal_head(const f4 out[3], int lane, uint32_t L, floa
   }
   // mixing logit: output 32 = tile 2, row 0 -> lanes of group 0, v = 0
   const float zc = __shfl(out[2][0], lane & 15);
-  mix = kPropEpsC + (1.0f - 2.0f * kPropEpsC) / (1.0f + __expf(-zc));
+  mix = kPropEpsC + (1.0f - 2.0f * kPropEpsC) * __builtin_amdgcn_rcpf(1.0f + __expf(-zc));
 }
 
 // ------------------------------------------------------------------ kernels
-// Inference over the items of R.  A block stages the forward fragments and
-// the biases in LDS once; each wave then takes 32 points (two groups of 16)
-// at a time from its segment (r = blockIdx % kSegs):
-//  * encoding: the 28 sincospi values of a point (7 inputs x 4 octaves) are
-//    spread over its two lanes (octaves h and h + 2, h = lane / 32) and
-//    written as bf16 into the wave's LDS table in B-operand order, so each
-//    lane then reads its four K tiles with four 8-byte loads.  One
-//    sincospif per octave keeps the values those of the training kernel's
-//    encoding (a double-angle step from the octave below would save a
-//    tenth of the time and move a few features by a bf16 step);
-//  * every A fragment read from LDS feeds both groups' MFMAs.
-constexpr int kEvalBlock = 512;
+// Inference over the items of R on v_mfma_f32_16x16x32_bf16.  A block
+// stages the inference fragments and the biases in LDS once; each wave then
+// takes NG groups of 16 points at a time from its segment
+// (r = blockIdx % kSegs).  A point's four lanes (lane % 16, groups
+// g = lane / 16) each hold 16 of its values in B-operand order:
+//  * encoding: lane group g evaluates octave g, sin and cos of
+//    2^g pi s_i for the 7 inputs (v_sin_f32 / v_cos_f32 take revolutions:
+//    2^(g-1) s_i), plus two of the 8 linear and constant features, and
+//    packs them to bf16 in registers (input order perm_enc);
+//  * after each layer a lane's D rows are the next layer's B operand as
+//    they stand (input order perm_hid), so activations never leave
+//    registers;
+//  * every A fragment read from LDS feeds the NG groups' MFMAs.
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef short sh8 __attribute__((ext_vector_type(8)));
+constexpr int kEvalBlock = 256;
 constexpr int kEvalWaves = kEvalBlock / 64;
+constexpr int kEvalGroups = 2;  // groups of 16 points per wave step
+
+__device__ inline f4 mfma32(bf8 a, bf8 b, f4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+typedef uint32_t u4v __attribute__((ext_vector_type(4)));
+__device__ inline bf8 pack8(f4 a, f4 b) {
+  return __builtin_bit_cast(bf8, (u4v{pack2(a[0], a[1]), pack2(a[2], a[3]), pack2(b[0], b[1]),
+                                      pack2(b[2], b[3])}));
+}
+__device__ inline bf8 relu8(bf8 v) {
+  return __builtin_bit_cast(bf8, __builtin_elementwise_max(__builtin_bit_cast(sh8, v), sh8{}));
+}
+
+template <int NG, int OT>
+__device__ inline void layer32(const bf8 *F, const float *b, const bf8 (&in)[NG][2], int lane,
+                               f4 (&acc)[NG][OT]) {
+  const int g = lane >> 4;
+#pragma unroll
+  for (int ot = 0; ot < OT; ++ot) {
+    const f4 bias = *reinterpret_cast<const f4 *>(b + ot * 16 + 4 * g);
+    const bf8 a0 = F[(ot * 2) * 64 + lane], a1 = F[(ot * 2 + 1) * 64 + lane];
+#pragma unroll
+    for (int n = 0; n < NG; ++n) acc[n][ot] = mfma32(a1, in[n][1], mfma32(a0, in[n][0], bias));
+  }
+}
+template <int NG>
+__device__ inline void hidden32(const bf8 *F, const float *b, const bf8 (&in)[NG][2], int lane,
+                                bf8 (&out)[NG][2]) {
+  f4 acc[NG][4];
+  layer32<NG, 4>(F, b, in, lane, acc);
+#pragma unroll
+  for (int n = 0; n < NG; ++n)
+#pragma unroll
+    for (int kt = 0; kt < 2; ++kt) out[n][kt] = relu8(pack8(acc[n][2 * kt], acc[n][2 * kt + 1]));
+}
+
+constexpr bool is_permutation(bool enc) {
+  bool seen[64] = {};
+  for (int kp = 0; kp < 64; ++kp) {
+    const int f = enc ? perm_enc(kp) : perm_hid(kp);
+    if (f < 0 || f > 63 || seen[f]) return false;
+    seen[f] = true;
+  }
+  return true;
+}
+static_assert(is_permutation(true) && is_permutation(false), "inference input orders");
 
 struct EvalLds {
-  s4 F[kFragTotal / 4];
+  bf8 I[kFragTotal / 8];
   __attribute__((aligned(16))) float bias[240];
-  s4 enc[kEvalWaves][2][4][64];  // [wave][group][K tile][lane]
 };
-static_assert(sizeof(EvalLds) <= 65536, "eval LDS fits one block's static limit");
 
-// Orders the wave's LDS writes before its reads of other lanes' entries (and
-// the reads of one iteration before the writes of the next).
-__device__ inline void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-}
-
-// Feature f of point pt (0..31) in the table: group pt / 16, K tile f / 16,
-// lane 16 ((f / 4) % 4) + pt % 16, element f % 4.  The index is the sum of
-// a point part and a feature part.
-__device__ constexpr uint32_t enc_pt(uint32_t pt) { return ((pt >> 4) * 256 + (pt & 15)) << 2; }
-__device__ constexpr uint32_t enc_f(uint32_t f) {
-  return (((f >> 4) * 64 + ((f >> 2) & 3) * 16) << 2) + (f & 3);
-}
-__device__ inline void enc_put(uint16_t *T, uint32_t ipt, uint32_t ifeat, float v) {
-  T[ipt + ifeat] = (uint16_t)bf16_bits(v);
-}
-
+template <int NG>
 __global__ __launch_bounds__(kEvalBlock) void k_mlp_eval(MlpDev W, const float *__restrict__ inputs,
                                                          SegRange R, float *__restrict__ probs,
                                                          float *__restrict__ mix_out,
                                                          float *__restrict__ logits) {
   __shared__ EvalLds S;
-  const s4 *gF = reinterpret_cast<const s4 *>(W.F);
-  for (uint32_t i = threadIdx.x; i < kFragTotal / 4; i += kEvalBlock) S.F[i] = gF[i];
+  const bf8 *gI = reinterpret_cast<const bf8 *>(W.I);
+  for (uint32_t i = threadIdx.x; i < kFragTotal / 8; i += kEvalBlock) S.I[i] = gI[i];
   for (uint32_t i = threadIdx.x; i < 240; i += kEvalBlock) S.bias[i] = W.bias[i];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  uint16_t *T = reinterpret_cast<uint16_t *>(&S.enc[wv][0][0][0]);
-  // feature 63 is the constant 1 (K tile 3, lane group 3, element 3)
-  if (lane < 32) T[enc_pt(lane) + enc_f(63)] = 0x3F80;
   __syncthreads();
-
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const uint32_t r = blockIdx.x % kSegs;
   const uint32_t k = R.count_of(r);
   const size_t seg0 = (size_t)r * R.segcap;
   const uint32_t wave = (blockIdx.x / kSegs) * kEvalWaves + wv;
   const uint32_t nw = (gridDim.x / kSegs) * kEvalWaves;
   const int g = lane >> 4;
-  const uint32_t pt = lane & 31, h = lane >> 5;
-  for (uint32_t g0 = wave * 32; g0 < k; g0 += nw * 32) {
-    wave_lds_sync();
-    {
-      const bool valid = g0 + pt < k;
-      const float *in = inputs + 7 * (seg0 + g0 + pt);
-      const uint32_t ipt = enc_pt(pt);
+  const float rev = 0.5f * (float)(1 << g);  // revolutions per unit input at octave g
+  for (uint32_t g0 = wave * 16 * NG; g0 < k; g0 += nw * 16 * NG) {
+    bf8 x[NG][2];
+#pragma unroll
+    for (int n = 0; n < NG; ++n) {
+      // a point past the end reads the last point's inputs (its results
+      // are not stored)
+      const uint32_t pl = min(g0 + 16 * n + (lane & 15), k - 1);
+      const float *in = inputs + 7 * (seg0 + pl);
+      float s[7];
+#pragma unroll
+      for (int i = 0; i < 7; ++i) s[i] = in[i];
+      float v[16];
 #pragma unroll
       for (int i = 0; i < 7; ++i) {
-        const float x = valid ? in[i] : 0.0f;
-        if (h == 0) enc_put(T, ipt, enc_f(9 * i), x);
-#pragma unroll
-        for (int o = 0; o < 2; ++o) {
-          // octave h + 2 o: sin(2^oct pi x) is feature 9 i + 1 + oct, cos 9 i + 5 + oct
-          const uint32_t oct = h + 2 * o;
-          float sn, cs;
-          sincospif((float)(1u << oct) * x, &sn, &cs);
-          const uint32_t fs = h ? enc_f(9 * i + 2 + 2 * o) : enc_f(9 * i + 1 + 2 * o);
-          const uint32_t fc = h ? enc_f(9 * i + 6 + 2 * o) : enc_f(9 * i + 5 + 2 * o);
-          enc_put(T, ipt, fs, sn);
-          enc_put(T, ipt, fc, cs);
-        }
+        const float a = s[i] * rev;
+        v[2 * i] = __builtin_amdgcn_sinf(a);
+        v[2 * i + 1] = __builtin_amdgcn_cosf(a);
       }
+      v[14] = g == 0 ? s[0] : g == 1 ? s[2] : g == 2 ? s[4] : s[6];
+      v[15] = g == 0 ? s[1] : g == 1 ? s[3] : g == 2 ? s[5] : 1.0f;
+      x[n][0] = pack8(f4{v[0], v[1], v[2], v[3]}, f4{v[4], v[5], v[6], v[7]});
+      x[n][1] = pack8(f4{v[8], v[9], v[10], v[11]}, f4{v[12], v[13], v[14], v[15]});
     }
-    wave_lds_sync();
-    s4 x[2][4], h1[2][4], h2[2][4];
+    // an offset the compiler cannot see through, so that the fragment reads
+    // stay in the loop instead of being hoisted into 120 registers
+    uint32_t z = 0;
+    asm volatile("" : "+s"(z));
+    const bf8 *I = S.I + z;
+    const float *bias = S.bias + z;
+    bf8 h1[NG][2], h2[NG][2];
+    hidden32<NG>(I, bias, x, lane, h1);
+    hidden32<NG>(I + kFragLayerF / 8, bias + 64, h1, lane, h2);
+    hidden32<NG>(I + 2 * kFragLayerF / 8, bias + 128, h2, lane, h1);
+    f4 out[NG][3];
+    layer32<NG, 3>(I + 3 * kFragLayerF / 8, bias + 192, h1, lane, out);
 #pragma unroll
-    for (int n = 0; n < 2; ++n)
-#pragma unroll
-      for (int kt = 0; kt < 4; ++kt) x[n][kt] = S.enc[wv][n][kt][lane];
-    hidden_n<2>(S.F, S.bias, x, lane, h1);
-    hidden_n<2>(S.F + kFragLayerF / 4, S.bias + 64, h1, lane, h2);
-    hidden_n<2>(S.F + 2 * kFragLayerF / 4, S.bias + 128, h2, lane, h1);
-    f4 out[2][3];
-    layer_acc<2, 3>(S.F + 3 * kFragLayerF / 4, S.bias + 192, h1, lane, out);
-#pragma unroll
-    for (int n = 0; n < 2; ++n) {
+    for (int n = 0; n < NG; ++n) {
       float q[8], mix, se, mx;
       proposal_head(out[n], lane, W.n_lights, q, mix, se, mx);
       const uint32_t pl = g0 + 16 * n + (lane & 15);
@@ -456,7 +485,8 @@ __global__ void k_adam(float *__restrict__ p, float *__restrict__ m, float *__re
 
 // Rebuild the bf16 fragments and the padded bias vector from the FP32 params.
 __global__ void k_mlp_pack(const float *__restrict__ P, uint16_t *__restrict__ F,
-                           uint16_t *__restrict__ G, float *__restrict__ bias) {
+                           uint16_t *__restrict__ G, uint16_t *__restrict__ I,
+                           float *__restrict__ bias) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx < 240) {
     const int l = idx < 192 ? idx / 64 : 3, o = idx < 192 ? idx % 64 : idx - 192;
@@ -484,6 +514,13 @@ __global__ void k_mlp_pack(const float *__restrict__ P, uint16_t *__restrict__ F
     const int it = t / OT, ot = t % OT;
     const int in = it * 16 + (lane & 15), out = ot * 16 + 4 * (lane >> 4) + j;
     G[idx] = out < O ? f2bf(W[in * O + out]) : 0;
+  }
+  {  // I: ((ot * 2 + kt) * 64 + lane) * 8 + j
+    const int j8 = r & 7, lane8 = (r >> 3) & 63, t8 = r >> 9;
+    const int ot = t8 / 2, kt = t8 % 2;
+    const int kp = kt * 32 + 8 * (lane8 >> 4) + j8;
+    const int in = l == 0 ? perm_enc(kp) : perm_hid(kp), out = ot * 16 + (lane8 & 15);
+    I[idx] = out < O ? f2bf(W[in * O + out]) : 0;
   }
 }
 
@@ -648,7 +685,7 @@ struct chk_proposal_s {
   uint32_t n_lights = 0;
   float *params = nullptr;  // FP32 master
   float *adam_m = nullptr, *adam_v = nullptr;
-  uint16_t *F = nullptr, *G = nullptr;
+  uint16_t *F = nullptr, *G = nullptr, *I = nullptr;
   float *bias = nullptr;
   uint64_t step = 0;
   // training scratch
@@ -665,6 +702,7 @@ MlpDev proposal_dev(const chk_proposal_s *p) {
   MlpDev W;
   W.F = p->F;
   W.G = p->G;
+  W.I = p->I;
   W.bias = p->bias;
   W.n_lights = p->n_lights;
   return W;
@@ -674,18 +712,20 @@ hipError_t launch_mlp_eval(const chk_proposal_s *p, const float *inputs, const S
                            float *probs, float *mix, float *logits, hipStream_t st) {
   // probs rows are 128 bytes, written as two 16-byte stores per lane
   if (((uintptr_t)probs & 15) != 0) return hipErrorInvalidValue;
-  const uint32_t groups = R.segcap / 32;  // per segment, 32 points per wave step
+  // per segment: wave steps of 16 kEvalGroups points; at most four blocks
+  // per CU over the whole grid
+  const uint32_t steps = (R.segcap + 16 * kEvalGroups - 1) / (16 * kEvalGroups);
   const uint32_t per_seg = std::max<uint32_t>(
-      1u, std::min<uint32_t>((groups + kEvalWaves - 1) / kEvalWaves,
-                             std::max<uint32_t>(1u, (uint32_t)p->ctx->n_cu / 4)));
-  hipLaunchKernelGGL(k_mlp_eval, dim3(kSegs * per_seg), dim3(kEvalBlock), 0, st, proposal_dev(p),
-                     inputs, R, probs, mix, logits);
+      1u, std::min<uint32_t>((steps + kEvalWaves - 1) / kEvalWaves,
+                             std::max<uint32_t>(1u, (uint32_t)p->ctx->n_cu * 4 / kSegs)));
+  hipLaunchKernelGGL(k_mlp_eval<kEvalGroups>, dim3(kSegs * per_seg), dim3(kEvalBlock), 0, st,
+                     proposal_dev(p), inputs, R, probs, mix, logits);
   return hipGetLastError();
 }
 
 static hipError_t pack(chk_proposal_s *p, hipStream_t st) {
   hipLaunchKernelGGL(k_mlp_pack, dim3((kFragTotal + 255) / 256), dim3(256), 0, st, p->params, p->F,
-                     p->G, p->bias);
+                     p->G, p->I, p->bias);
   return hipGetLastError();
 }
 
@@ -845,6 +885,7 @@ chk_status chk_proposal_create(chk_ctx_t ctx, uint32_t n_lights, uint64_t seed,
             hipMalloc(&p->adam_v, pb) == hipSuccess &&
             hipMalloc(&p->F, kFragTotal * 2) == hipSuccess &&
             hipMalloc(&p->G, kFragTotal * 2) == hipSuccess &&
+            hipMalloc(&p->I, kFragTotal * 2) == hipSuccess &&
             hipMalloc(&p->bias, 256 * 4) == hipSuccess;
   if (!ok) {
     chk_proposal_destroy(p);
@@ -879,7 +920,7 @@ chk_status chk_proposal_destroy(chk_proposal_t p) {
   if (!p) return CHK_OK;
   (void)hipSetDevice(p->ctx->device);
   for (void *q : {(void *)p->params, (void *)p->adam_m, (void *)p->adam_v, (void *)p->F,
-                  (void *)p->G, (void *)p->bias, p->scratch, p->train})
+                  (void *)p->G, (void *)p->I, (void *)p->bias, p->scratch, p->train})
     if (q) (void)hipFree(q);
   delete p;
   return CHK_OK;
