@@ -112,6 +112,8 @@ __device__ inline void hidden(const MlpDev &W, int l, const s4 in[4], int lane, 
 
 // Softmax over the first n_lights logits of each point (column), defensive
 // mixture, and the mixing weight.  Lanes l, l^16, l^32, l^48 share a point.
+// FULL: n_lights is 32 (no logit is masked).
+template <bool FULL>
 __device__ inline void proposal_head(const f4 out[3], int lane, uint32_t L, float q[8], float &mix,
                                      float &sum_e, float &mx) {
   const int g = lane >> 4;
@@ -122,7 +124,7 @@ __device__ inline void proposal_head(const f4 out[3], int lane, uint32_t L, floa
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int o = t * 16 + 4 * g + v;
-      e[t * 4 + v] = (uint32_t)o < L ? out[t][v] : -INFINITY;
+      e[t * 4 + v] = FULL || (uint32_t)o < L ? out[t][v] : -INFINITY;
       mx = fmaxf(mx, e[t * 4 + v]);
     }
   mx = fmaxf(mx, __shfl_xor(mx, 16));
@@ -130,18 +132,19 @@ __device__ inline void proposal_head(const f4 out[3], int lane, uint32_t L, floa
   float s = 0.0f;
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
-    e[k] = e[k] == -INFINITY ? 0.0f : __expf(e[k] - mx);
+    e[k] = __expf(e[k] - mx);  // 0 for a masked logit (mx is finite)
     s += e[k];
   }
   s += __shfl_xor(s, 16);
   s += __shfl_xor(s, 32);
   sum_e = s;
   // v_rcp_f32 (1 ulp) for the two reciprocals and the logistic below
-  const float invL = __builtin_amdgcn_rcpf((float)L), inv_s = __builtin_amdgcn_rcpf(s);
+  const float invL = FULL ? 1.0f / 32.0f : __builtin_amdgcn_rcpf((float)L);
+  const float inv_s = __builtin_amdgcn_rcpf(s);
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     const int o = (k >> 2) * 16 + 4 * g + (k & 3);
-    q[k] = (uint32_t)o < L ? (1.0f - kPropEpsQ) * (e[k] * inv_s) + kPropEpsQ * invL : 0.0f;
+    q[k] = FULL || (uint32_t)o < L ? (1.0f - kPropEpsQ) * (e[k] * inv_s) + kPropEpsQ * invL : 0.0f;
   }
   // mixing logit: output 32 = tile 2, row 0 -> lanes of group 0, v = 0
   const float zc = __shfl(out[2][0], lane & 15);
@@ -164,7 +167,7 @@ __device__ inline void proposal_head(const f4 out[3], int lane, uint32_t L, floa
 //  * every A fragment read from LDS feeds the NG groups' MFMAs.
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef short sh8 __attribute__((ext_vector_type(8)));
-constexpr int kEvalBlock = 256;
+constexpr int kEvalBlock = 512;
 constexpr int kEvalWaves = kEvalBlock / 64;
 constexpr int kEvalGroups = 2;  // groups of 16 points per wave step
 
@@ -214,12 +217,91 @@ constexpr bool is_permutation(bool enc) {
 }
 static_assert(is_permutation(true) && is_permutation(false), "inference input orders");
 
+// Orders the wave's LDS writes before its reads of other lanes' entries (and
+// the reads before the next writes).
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// Rows of q' of one group of 16 points, staged so that each store
+// instruction writes 1 KB of whole rows (pitch 36 floats: 16-byte aligned,
+// fewer bank conflicts than 32).
+constexpr int kStagePitch = 36;
 struct EvalLds {
   bf8 I[kFragTotal / 8];
   __attribute__((aligned(16))) float bias[240];
+  __attribute__((aligned(16))) float stage[kEvalWaves][16 * kStagePitch];
 };
 
+// Inputs of NG groups of 16 points from base.  A point past the end reads
+// the last point's inputs, so it computes that point's results bit for bit
+// and may store them again (the same values to the same addresses).
+// buf[n][7], buf[n][8]: inputs 2g and 2g + 1 of lane group g (input 6
+// twice for g = 3), loaded as such rather than selected from buf[n][0..6]
+// with a lane-dependent index, which would put the array in scratch memory.
 template <int NG>
+__device__ inline void eval_load(float (&buf)[NG][9], const float *__restrict__ inputs,
+                                 size_t seg0, uint32_t base, uint32_t k, int lane) {
+  const int g = lane >> 4;
+  const int e0 = 2 * g, e1 = min(2 * g + 1, 6);
+#pragma unroll
+  for (int n = 0; n < NG; ++n) {
+    const float *in = inputs + 7 * (seg0 + min(base + 16 * n + (lane & 15), k - 1));
+#pragma unroll
+    for (int i = 0; i < 7; ++i) buf[n][i] = in[i];
+    buf[n][7] = in[e0];
+    buf[n][8] = in[e1];
+  }
+}
+
+// Lane group g's 16 B-operand values of its point: sin and cos of
+// 2^g pi s_i (perm_enc order), inputs 2g and 2g + 1 (the constant 1 for
+// g = 3).
+template <int NG>
+__device__ inline void eval_encode(const float (&s)[NG][9], int lane, bf8 (&x)[NG][2]) {
+  const int g = lane >> 4;
+  const float rev = 0.5f * (float)(1 << g);  // revolutions per unit input at octave g
+#pragma unroll
+  for (int n = 0; n < NG; ++n) {
+    float v[16];
+#pragma unroll
+    for (int i = 0; i < 7; ++i) {
+      const float a = s[n][i] * rev;
+      v[2 * i] = __builtin_amdgcn_sinf(a);
+      v[2 * i + 1] = __builtin_amdgcn_cosf(a);
+    }
+    v[14] = s[n][7];
+    v[15] = g == 3 ? 1.0f : s[n][8];
+    x[n][0] = pack8(f4{v[0], v[1], v[2], v[3]}, f4{v[4], v[5], v[6], v[7]});
+    x[n][1] = pack8(f4{v[8], v[9], v[10], v[11]}, f4{v[12], v[13], v[14], v[15]});
+  }
+}
+
+// The four layers of NG groups.
+template <int NG>
+__device__ inline void eval_layers(const EvalLds &S, const bf8 (&x)[NG][2], int lane,
+                                   f4 (&out)[NG][3]) {
+  // an offset the compiler cannot see through, so that the fragment reads
+  // stay in the loop instead of being hoisted into 120 registers
+  uint32_t z = 0;
+  asm volatile("" : "+s"(z));
+  const bf8 *I = S.I + z;
+  const float *bias = S.bias + z;
+  bf8 h1[NG][2], h2[NG][2];
+  hidden32<NG>(I, bias, x, lane, h1);
+  hidden32<NG>(I + kFragLayerF / 8, bias + 64, h1, lane, h2);
+  hidden32<NG>(I + 2 * kFragLayerF / 8, bias + 128, h2, lane, h1);
+  layer32<NG, 3>(I + 3 * kFragLayerF / 8, bias + 192, h1, lane, out);
+}
+
+// Each wave runs wave steps of NG groups of 16 points, g0 += stride.  The
+// next step's inputs are loaded before this step's results are stored, and
+// no branch skips a store (points past the end store the last point's
+// results again): vector memory operations complete in order, so the wait
+// for those inputs is then a wait for the loads alone, not for the stores.
+template <int NG, bool FULL>
 __global__ __launch_bounds__(kEvalBlock) void k_mlp_eval(MlpDev W, const float *__restrict__ inputs,
                                                          SegRange R, float *__restrict__ probs,
                                                          float *__restrict__ mix_out,
@@ -229,71 +311,66 @@ __global__ __launch_bounds__(kEvalBlock) void k_mlp_eval(MlpDev W, const float *
   for (uint32_t i = threadIdx.x; i < kFragTotal / 8; i += kEvalBlock) S.I[i] = gI[i];
   for (uint32_t i = threadIdx.x; i < 240; i += kEvalBlock) S.bias[i] = W.bias[i];
   __syncthreads();
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, g = lane >> 4;
+  const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const uint32_t r = blockIdx.x % kSegs;
   const uint32_t k = R.count_of(r);
   const size_t seg0 = (size_t)r * R.segcap;
   const uint32_t wave = (blockIdx.x / kSegs) * kEvalWaves + wv;
   const uint32_t nw = (gridDim.x / kSegs) * kEvalWaves;
-  const int g = lane >> 4;
-  const float rev = 0.5f * (float)(1 << g);  // revolutions per unit input at octave g
-  for (uint32_t g0 = wave * 16 * NG; g0 < k; g0 += nw * 16 * NG) {
-    bf8 x[NG][2];
-#pragma unroll
-    for (int n = 0; n < NG; ++n) {
-      // a point past the end reads the last point's inputs (its results
-      // are not stored)
-      const uint32_t pl = min(g0 + 16 * n + (lane & 15), k - 1);
-      const float *in = inputs + 7 * (seg0 + pl);
-      float s[7];
-#pragma unroll
-      for (int i = 0; i < 7; ++i) s[i] = in[i];
-      float v[16];
-#pragma unroll
-      for (int i = 0; i < 7; ++i) {
-        const float a = s[i] * rev;
-        v[2 * i] = __builtin_amdgcn_sinf(a);
-        v[2 * i + 1] = __builtin_amdgcn_cosf(a);
-      }
-      v[14] = g == 0 ? s[0] : g == 1 ? s[2] : g == 2 ? s[4] : s[6];
-      v[15] = g == 0 ? s[1] : g == 1 ? s[3] : g == 2 ? s[5] : 1.0f;
-      x[n][0] = pack8(f4{v[0], v[1], v[2], v[3]}, f4{v[4], v[5], v[6], v[7]});
-      x[n][1] = pack8(f4{v[8], v[9], v[10], v[11]}, f4{v[12], v[13], v[14], v[15]});
-    }
-    // an offset the compiler cannot see through, so that the fragment reads
-    // stay in the loop instead of being hoisted into 120 registers
-    uint32_t z = 0;
-    asm volatile("" : "+s"(z));
-    const bf8 *I = S.I + z;
-    const float *bias = S.bias + z;
-    bf8 h1[NG][2], h2[NG][2];
-    hidden32<NG>(I, bias, x, lane, h1);
-    hidden32<NG>(I + kFragLayerF / 8, bias + 64, h1, lane, h2);
-    hidden32<NG>(I + 2 * kFragLayerF / 8, bias + 128, h2, lane, h1);
+  const uint32_t stride = nw * 16 * NG;
+  uint32_t g0 = wave * 16 * NG;
+  if (g0 >= k) return;
+  float s[NG][9];
+  bf8 x[NG][2];
+  eval_load<NG>(s, inputs, seg0, g0, k, lane);
+  eval_encode<NG>(s, lane, x);
+  for (;;) {
     f4 out[NG][3];
-    layer32<NG, 3>(I + 3 * kFragLayerF / 8, bias + 192, h1, lane, out);
+    eval_layers<NG>(S, x, lane, out);
+    float q[NG][8], mix[NG];
 #pragma unroll
     for (int n = 0; n < NG; ++n) {
-      float q[8], mix, se, mx;
-      proposal_head(out[n], lane, W.n_lights, q, mix, se, mx);
-      const uint32_t pl = g0 + 16 * n + (lane & 15);
-      if (pl < k) {
-        const size_t p = seg0 + pl;
-        float *pr = probs + 32 * p + 4 * g;
-        *reinterpret_cast<f4 *>(pr) = f4{q[0], q[1], q[2], q[3]};
-        *reinterpret_cast<f4 *>(pr + 16) = f4{q[4], q[5], q[6], q[7]};
-        if (g == 0) mix_out[p] = mix;
-        if (logits) {
+      float se, mx;
+      proposal_head<FULL>(out[n], lane, W.n_lights, q[n], mix[n], se, mx);
+    }
+    const uint32_t g1 = g0 + stride;
+    eval_load<NG>(s, inputs, seg0, g1, k, lane);
+    asm volatile("" ::: "memory");  // the loads above stay before the stores
+    float *st = S.stage[wv];
 #pragma unroll
-          for (int t = 0; t < 3; ++t)
+    for (int n = 0; n < NG; ++n) {
+      const size_t p = seg0 + min(g0 + 16 * n + (lane & 15), k - 1);
+      // lane (point p, group g) holds q'_(4g..4g+3) and q'_(16+4g..16+4g+3)
+      // of its row; after the exchange, lane l stores 16 bytes of row l / 8
+      // (then of row 8 + l / 8) at float 4 (l % 8): 1 KB of rows per store
+      float *w = st + kStagePitch * (lane & 15) + 4 * g;
+      *reinterpret_cast<f4 *>(w) = f4{q[n][0], q[n][1], q[n][2], q[n][3]};
+      *reinterpret_cast<f4 *>(w + 16) = f4{q[n][4], q[n][5], q[n][6], q[n][7]};
+      wave_lds_sync();
+      const f4 r0 = *reinterpret_cast<const f4 *>(st + kStagePitch * (lane >> 3) + 4 * (lane & 7));
+      const f4 r1 =
+          *reinterpret_cast<const f4 *>(st + kStagePitch * (8 + (lane >> 3)) + 4 * (lane & 7));
+      wave_lds_sync();
+      // a row past the end holds the last point's values: stored there again
+      const size_t d0 = seg0 + min(g0 + 16 * n + (lane >> 3), k - 1);
+      const size_t d1 = seg0 + min(g0 + 16 * n + 8 + (lane >> 3), k - 1);
+      *reinterpret_cast<f4 *>(probs + 32 * d0 + 4 * (lane & 7)) = r0;
+      *reinterpret_cast<f4 *>(probs + 32 * d1 + 4 * (lane & 7)) = r1;
+      mix_out[p] = mix[n];  // the four lanes of a point hold the same value
+      if (logits) {
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-              const int o = t * 16 + 4 * g + v;
-              if (o < kMlpOut) logits[33 * p + o] = out[n][t][v];
-            }
-        }
+        for (int t = 0; t < 3; ++t)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int o = t * 16 + 4 * g + v;
+            if (o < kMlpOut) logits[33 * p + o] = out[n][t][v];
+          }
       }
     }
+    if (g1 >= k) break;
+    g0 = g1;
+    eval_encode<NG>(s, lane, x);
   }
 }
 
@@ -712,14 +789,18 @@ hipError_t launch_mlp_eval(const chk_proposal_s *p, const float *inputs, const S
                            float *probs, float *mix, float *logits, hipStream_t st) {
   // probs rows are 128 bytes, written as two 16-byte stores per lane
   if (((uintptr_t)probs & 15) != 0) return hipErrorInvalidValue;
-  // per segment: wave steps of 16 kEvalGroups points; at most four blocks
-  // per CU over the whole grid
+  // per segment: wave steps of 16 kEvalGroups points; at most two blocks
+  // (16 waves) per CU over the whole grid
   const uint32_t steps = (R.segcap + 16 * kEvalGroups - 1) / (16 * kEvalGroups);
   const uint32_t per_seg = std::max<uint32_t>(
       1u, std::min<uint32_t>((steps + kEvalWaves - 1) / kEvalWaves,
-                             std::max<uint32_t>(1u, (uint32_t)p->ctx->n_cu * 4 / kSegs)));
-  hipLaunchKernelGGL(k_mlp_eval<kEvalGroups>, dim3(kSegs * per_seg), dim3(kEvalBlock), 0, st,
-                     proposal_dev(p), inputs, R, probs, mix, logits);
+                             std::max<uint32_t>(1u, (uint32_t)p->ctx->n_cu * 2 / kSegs)));
+  if (p->n_lights == (uint32_t)kMaxProposalLights)
+    hipLaunchKernelGGL((k_mlp_eval<kEvalGroups, true>), dim3(kSegs * per_seg), dim3(kEvalBlock), 0,
+                       st, proposal_dev(p), inputs, R, probs, mix, logits);
+  else
+    hipLaunchKernelGGL((k_mlp_eval<kEvalGroups, false>), dim3(kSegs * per_seg), dim3(kEvalBlock),
+                       0, st, proposal_dev(p), inputs, R, probs, mix, logits);
   return hipGetLastError();
 }
 
