@@ -355,8 +355,9 @@ __global__ __launch_bounds__(kEvalBlock) void k_mlp_eval(MlpDev W, const float *
       // a row past the end holds the last point's values: stored there again
       const size_t d0 = seg0 + min(g0 + 16 * n + (lane >> 3), k - 1);
       const size_t d1 = seg0 + min(g0 + 16 * n + 8 + (lane >> 3), k - 1);
-      *reinterpret_cast<f4 *>(probs + 32 * d0 + 4 * (lane & 7)) = r0;
-      *reinterpret_cast<f4 *>(probs + 32 * d1 + 4 * (lane & 7)) = r1;
+      // streaming stores: the rows are read by a later kernel, not again here
+      __builtin_nontemporal_store(r0, reinterpret_cast<f4 *>(probs + 32 * d0 + 4 * (lane & 7)));
+      __builtin_nontemporal_store(r1, reinterpret_cast<f4 *>(probs + 32 * d1 + 4 * (lane & 7)));
       mix_out[p] = mix[n];  // the four lanes of a point hold the same value
       if (logits) {
 #pragma unroll
