@@ -16,8 +16,8 @@
 // formulas lands up to 1e-2 away from the reference on random inputs, so
 // the kernel computes in the reference's precision (FP32 in, FP32 out).
 //
-// One shading point per lane; the light triangles are wave-uniform (scalar
-// loads); the two table rows come from L2.
+// One shading point per lane; the light triangles are staged in LDS in
+// FP64; the two table rows come from L2.
 #include "chk_internal.h"
 #include "chk_ltc_dev.h"
 
@@ -25,12 +25,35 @@ namespace chk {
 namespace {
 
 constexpr int kLtcBlock = 256;
+constexpr int kLtcWaves = kLtcBlock / 64;
+constexpr int kChunk = 32;  // light triangles per chunk
 
-// The LTC frame of triangle j seen from x: its vertices relative to x,
+// Light triangles of the current chunk in FP64 (vertices, radiance), and per
+// wave: the 64 points' LTC frames, the (triangle, point) pairs of the chunk
+// whose triangle lies above the point's horizon, and the results of one
+// window of 64 pairs.
+struct LtcLds {
+  double tri[kChunk][9];
+  double rad[kChunk][3];
+  struct Wave {
+    double frame[12][64];  // x, then the rows of Minv * R, component-major
+    uint16_t pair[kChunk * 64];  // triangle << 6 | lane, triangle-major
+    unsigned long long mask[kChunk];
+    uint32_t base[kChunk];
+    double res[64];
+  } w[kLtcWaves];
+};
+
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// The LTC frame of triangle t seen from x: its vertices relative to x,
 // multiplied by the rows of Minv * R.
-__device__ inline void ltc_vertices(const float *__restrict__ lt, uint32_t j, d3 x, d3 R0, d3 R1,
-                                    d3 R2, d3 &e1, d3 &e2, d3 &e3) {
-  const float *p = lt + 9 * (size_t)j;
+__device__ inline void ltc_vertices(const double *p, d3 x, d3 R0, d3 R1, d3 R2, d3 &e1, d3 &e2,
+                                    d3 &e3) {
   const d3 a = dk(p[0], p[1], p[2]) - x, b = dk(p[3], p[4], p[5]) - x,
            c = dk(p[6], p[7], p[8]) - x;
   e1 = dk(dot(R0, a), dot(R1, a), dot(R2, a));
@@ -38,14 +61,87 @@ __device__ inline void ltc_vertices(const float *__restrict__ lt, uint32_t j, d3
   e3 = dk(dot(R0, c), dot(R1, c), dot(R2, c));
 }
 
-// Light triangles are taken 32 at a time.  First pass: every lane
-// integrates the triangles entirely above its horizon (three edges, no
-// clipping; the value of tri64<true>) and marks those that straddle it.
-// Second pass: each lane walks its own marked triangles (the lanes of a wave
-// in step, each on its next one), clipping and integrating them (tri64's
-// clip branch); a lane with none left repeats a harmless evaluation.  The
-// clip code thus runs only as often as the lane with the most straddling
-// triangles needs it.  Triangles below the horizon contribute 0 in both.
+// |sum of the three edge terms| of a triangle above the horizon
+// (lightSources_CV.cpp:293-314 without clipping).
+__device__ inline double ltc_unclipped(d3 e1, d3 e2, d3 e3) {
+  const d3 c1 = cross(e1, e2), c2 = cross(e2, e3), c3 = cross(e3, e1);
+  return fabs(edge64(e1, e2, c1, c1.z) + edge64(e2, e3, c2, c2.z) + edge64(e3, e1, c3, c3.z));
+}
+
+// The same for a triangle straddling the horizon: clipTriangle
+// (lightSources_CV.cpp:129-219) to z >= 0, then the 4-point polygon.
+__device__ inline double ltc_clipped(d3 e1, d3 e2, d3 e3) {
+  const d3 zero = dk(0, 0, 0), zax = dk(0, 0, 1);
+  Clip4 cl;
+  cl.q0 = cl.q1 = cl.q2 = cl.q3 = zero;
+  cl.idx = 0;
+  const bool ok = clip64<true>(e1, e2, e3, zax, zero, e1.z, e2.z, e3.z, cl);
+  const d3 c1 = cross(cl.q0, cl.q1), c2 = cross(cl.q1, cl.q2), c3 = cross(cl.q2, cl.q3),
+           c4 = cross(cl.q3, cl.q0);
+  const double sum = edge64(cl.q0, cl.q1, c1, c1.z) + edge64(cl.q1, cl.q2, c2, c2.z) +
+                     edge64(cl.q2, cl.q3, c3, c3.z) + edge64(cl.q3, cl.q0, c4, c4.z);
+  return ok ? fabs(sum) : 0.0;
+}
+
+// The wave's (triangle, point) pairs of one kind (bits: this lane's
+// triangles of the chunk), listed triangle-major and evaluated 64 at a time,
+// each lane on the next pair whatever its point (the point's frame comes
+// from LDS); each lane then adds its own pairs' results in triangle order.
+template <bool CLIP>
+__device__ inline void ltc_pairs(const LtcLds &S, LtcLds::Wave &Wl, uint32_t bits, uint32_t jn,
+                                 uint32_t lane, double &s0, double &s1, double &s2) {
+  const unsigned long long below_me = (1ull << lane) - 1ull;
+  uint32_t np = 0;
+  for (uint32_t t = 0; t < jn; ++t) {
+    const bool mine = (bits >> t) & 1u;
+    const unsigned long long mt = __ballot(mine);
+    if (mine) Wl.pair[np + __popcll(mt & below_me)] = (uint16_t)((t << 6) | lane);
+    if (lane == 0) {
+      Wl.mask[t] = mt;
+      Wl.base[t] = np;
+    }
+    np += (uint32_t)__popcll(mt);
+  }
+  wave_lds_sync();
+  for (uint32_t w0 = 0; w0 < np; w0 += 64) {
+    {
+      const uint32_t e = min(w0 + lane, np - 1);
+      const uint32_t pr = Wl.pair[e], t = pr >> 6, P = pr & 63u;
+      const d3 xp = dk(Wl.frame[0][P], Wl.frame[1][P], Wl.frame[2][P]);
+      const d3 Q0 = dk(Wl.frame[3][P], Wl.frame[4][P], Wl.frame[5][P]);
+      const d3 Q1 = dk(Wl.frame[6][P], Wl.frame[7][P], Wl.frame[8][P]);
+      const d3 Q2 = dk(Wl.frame[9][P], Wl.frame[10][P], Wl.frame[11][P]);
+      d3 e1, e2, e3;
+      ltc_vertices(S.tri[t], xp, Q0, Q1, Q2, e1, e2, e3);
+      Wl.res[lane] = CLIP ? ltc_clipped(e1, e2, e3) : ltc_unclipped(e1, e2, e3);
+    }
+    wave_lds_sync();
+    // this lane's pairs in the window, in triangle order
+    while (bits != 0u) {
+      const uint32_t t = (uint32_t)__builtin_ctz(bits);
+      const uint32_t idx = Wl.base[t] + (uint32_t)__popcll(Wl.mask[t] & below_me);
+      if (idx >= w0 + 64) break;
+      const double sv = Wl.res[idx - w0];
+      s0 += S.rad[t][0] * sv;
+      s1 += S.rad[t][1] * sv;
+      s2 += S.rad[t][2] * sv;
+      bits &= bits - 1u;
+    }
+    wave_lds_sync();  // the window's results are read before the next is written
+  }
+}
+
+// Light triangles are taken kChunk at a time.  For the LTC part each lane
+// classifies the chunk's triangles against its horizon (z of the three
+// transformed vertices): above, straddling or below.  The wave then
+// evaluates its pairs with a triangle above (three edges), then those with
+// a triangle straddling (clipped, four edges), each as a compact list
+// (ltc_pairs), so no lane works on a triangle below its horizon and no lane
+// waits for the one with the most straddling triangles.  A lane's sums take
+// the terms of the triangles above in triangle order, then those of the
+// straddling ones, as a loop over all triangles adding 0 for those below
+// would.  The diffuse part (ks < 1) evaluates every triangle in each lane
+// (tri64<false>).
 __global__ __launch_bounds__(kLtcBlock) void k_ltc(const double *__restrict__ tab,
                                                    const float *__restrict__ X,
                                                    const float *__restrict__ N,
@@ -55,72 +151,67 @@ __global__ __launch_bounds__(kLtcBlock) void k_ltc(const double *__restrict__ ta
                                                    const float *__restrict__ lt,
                                                    const float *__restrict__ lrad, uint32_t nlt,
                                                    float *__restrict__ rgb) {
+  __shared__ LtcLds S;
+  const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  LtcLds::Wave &Wl = S.w[wv];
   const uint32_t i = blockIdx.x * kLtcBlock + threadIdx.x;
-  if (i >= k) return;
-  const size_t o = 3 * (size_t)i;
+  const bool valid = i < k;
+  const size_t o = 3 * (size_t)(valid ? i : 0);
   const d3 x = dk(X[o], X[o + 1], X[o + 2]), n = dk(N[o], N[o + 1], N[o + 2]),
            wo = dk(V[o], V[o + 1], V[o + 2]);
-  const double kss = ks[i];
+  const double kss = ks[valid ? i : 0];
   // M_GGX((float)acos(n.wo), (float)alpha) (ltc.h:18-61, materialTypes.cpp:54)
-  const LtcFit F = ltc_fit(tab, (float)acos(dot(n, wo)), (double)alpha[i]);
+  const LtcFit F = ltc_fit(tab, (float)acos(dot(n, wo)), (double)alpha[valid ? i : 0]);
   const double *m = F.mi;
   // frame of getBrdfApproxParam (materialTypes.cpp:59-68); rows of Minv * R
   const d3 T1 = normalize(wo - n * dot(n, wo));
   const d3 T2 = normalize(cross(n, T1));
   const d3 R0 = T1 * m[0] + n * m[1], R1 = T2 * m[2], R2 = T1 * m[3] + n * m[4];
+  {
+    const double fr[12] = {x.x, x.y, x.z, R0.x, R0.y, R0.z, R1.x, R1.y, R1.z, R2.x, R2.y, R2.z};
+#pragma unroll
+    for (int c = 0; c < 12; ++c) Wl.frame[c][lane] = fr[c];
+  }
   double sl0 = 0, sl1 = 0, sl2 = 0, sd0 = 0, sd1 = 0, sd2 = 0;
-  const d3 zero = dk(0, 0, 0), zax = dk(0, 0, 1);
-  const bool do_ltc = kss != 0.0, do_dif = kss != 1.0;
-  for (uint32_t j0 = 0; j0 < nlt; j0 += 32) {
-    const uint32_t jn = min(32u, nlt - j0);
-    uint32_t straddle = 0;
-    for (uint32_t jj = 0; jj < jn; ++jj) {
-      const uint32_t j = j0 + jj;
-      const double r0c = lrad[3 * j], r1c = lrad[3 * j + 1], r2c = lrad[3 * j + 2];
-      if (do_ltc) {
-        d3 e1, e2, e3;
-        ltc_vertices(lt, j, x, R0, R1, R2, e1, e2, e3);
-        const bool above = e1.z > 0 && e2.z > 0 && e3.z > 0;
-        const bool below = e1.z < 0 && e2.z < 0 && e3.z < 0;
-        straddle |= (!above && !below) ? 1u << jj : 0u;
-        const d3 c1 = cross(e1, e2), c2 = cross(e2, e3), c3 = cross(e3, e1);
-        const double sum = edge64(e1, e2, c1, c1.z) + edge64(e2, e3, c2, c2.z) +
-                           edge64(e3, e1, c3, c3.z);
-        const double sv = above ? fabs(sum) : 0.0;
-        sl0 += r0c * sv;
-        sl1 += r1c * sv;
-        sl2 += r2c * sv;
+  const bool do_ltc = valid && kss != 0.0, do_dif = valid && kss != 1.0;
+  for (uint32_t j0 = 0; j0 < nlt; j0 += kChunk) {
+    const uint32_t jn = min((uint32_t)kChunk, nlt - j0);
+    __syncthreads();  // the previous chunk's triangles are no longer read
+    for (uint32_t q = threadIdx.x; q < jn * 12; q += kLtcBlock) {
+      const uint32_t t = q / 12, c = q % 12;
+      if (c < 9)
+        S.tri[t][c] = lt[9 * (size_t)(j0 + t) + c];
+      else
+        S.rad[t][c - 9] = lrad[3 * (size_t)(j0 + t) + c - 9];
+    }
+    __syncthreads();
+    uint32_t above = 0, straddle = 0;
+    if (do_ltc) {
+      for (uint32_t t = 0; t < jn; ++t) {
+        const double *p = S.tri[t];
+        const double z1 = dot(R2, dk(p[0], p[1], p[2]) - x), z2 = dot(R2, dk(p[3], p[4], p[5]) - x),
+                     z3 = dot(R2, dk(p[6], p[7], p[8]) - x);
+        const bool a = z1 > 0 && z2 > 0 && z3 > 0, b = z1 < 0 && z2 < 0 && z3 < 0;
+        above |= a ? 1u << t : 0u;
+        straddle |= (!a && !b) ? 1u << t : 0u;
       }
-      if (do_dif) {
-        const float *p = lt + 9 * (size_t)j;
+    }
+    ltc_pairs<false>(S, Wl, above, jn, lane, sl0, sl1, sl2);
+    ltc_pairs<true>(S, Wl, straddle, jn, lane, sl0, sl1, sl2);
+    if (__any(do_dif)) {  // wave-uniform (tri64 votes across the wave)
+      for (uint32_t t = 0; t < jn; ++t) {
+        const double *p = S.tri[t];
         const d3 A = dk(p[0], p[1], p[2]), B = dk(p[3], p[4], p[5]), C = dk(p[6], p[7], p[8]);
         const double sv = tri64<false>(A, B, C, n, x);
-        sd0 += r0c * sv;
-        sd1 += r1c * sv;
-        sd2 += r2c * sv;
+        if (do_dif) {
+          sd0 += S.rad[t][0] * sv;
+          sd1 += S.rad[t][1] * sv;
+          sd2 += S.rad[t][2] * sv;
+        }
       }
     }
-    while (__any(straddle != 0)) {  // wave-uniform loop
-      const bool mine = straddle != 0;
-      const uint32_t jj = mine ? (uint32_t)__builtin_ctz(straddle) : 0u;
-      straddle &= straddle - 1u;
-      const uint32_t j = j0 + jj;
-      d3 e1, e2, e3;
-      ltc_vertices(lt, j, x, R0, R1, R2, e1, e2, e3);
-      Clip4 cl;
-      cl.q0 = cl.q1 = cl.q2 = cl.q3 = zero;
-      cl.idx = 0;
-      const bool ok = clip64<true>(e1, e2, e3, zax, zero, e1.z, e2.z, e3.z, cl);
-      const d3 c1 = cross(cl.q0, cl.q1), c2 = cross(cl.q1, cl.q2), c3 = cross(cl.q2, cl.q3),
-               c4 = cross(cl.q3, cl.q0);
-      const double sum = edge64(cl.q0, cl.q1, c1, c1.z) + edge64(cl.q1, cl.q2, c2, c2.z) +
-                         edge64(cl.q2, cl.q3, c3, c3.z) + edge64(cl.q3, cl.q0, c4, c4.z);
-      const double sv = (mine && ok) ? fabs(sum) : 0.0;
-      sl0 += lrad[3 * j] * sv;
-      sl1 += lrad[3 * j + 1] * sv;
-      sl2 += lrad[3 * j + 2] * sv;
-    }
   }
+  if (!valid) return;
   // Sum / (2 pi), then Analytic = LTC * amp * ks + diffuse * (1 - ks)
   const double wl = F.amp * kss / (2.0 * kPi64), wd = (1.0 - kss) / (2.0 * kPi64);
   rgb[o] = (float)(sl0 * wl + sd0 * wd);
