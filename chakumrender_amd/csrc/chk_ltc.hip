@@ -40,8 +40,9 @@ struct LtcLds {
     uint16_t pair[kChunk * 64];  // triangle << 6 | lane, triangle-major
     unsigned long long mask[kChunk];
     uint32_t base[kChunk];
-    double res[64];
+    double res[2][64];
   } w[kLtcWaves];
+  uint32_t quads;  // the chunk's triangles pair up into quads (ltc_quad_pairs)
 };
 
 __device__ inline void wave_lds_sync() {
@@ -113,7 +114,7 @@ __device__ inline void ltc_pairs(const LtcLds &S, LtcLds::Wave &Wl, uint32_t bit
       const d3 Q2 = dk(Wl.frame[9][P], Wl.frame[10][P], Wl.frame[11][P]);
       d3 e1, e2, e3;
       ltc_vertices(S.tri[t], xp, Q0, Q1, Q2, e1, e2, e3);
-      Wl.res[lane] = CLIP ? ltc_clipped(e1, e2, e3) : ltc_unclipped(e1, e2, e3);
+      Wl.res[0][lane] = CLIP ? ltc_clipped(e1, e2, e3) : ltc_unclipped(e1, e2, e3);
     }
     wave_lds_sync();
     // this lane's pairs in the window, in triangle order
@@ -121,11 +122,82 @@ __device__ inline void ltc_pairs(const LtcLds &S, LtcLds::Wave &Wl, uint32_t bit
       const uint32_t t = (uint32_t)__builtin_ctz(bits);
       const uint32_t idx = Wl.base[t] + (uint32_t)__popcll(Wl.mask[t] & below_me);
       if (idx >= w0 + 64) break;
-      const double sv = Wl.res[idx - w0];
+      const double sv = Wl.res[0][idx - w0];
       s0 += S.rad[t][0] * sv;
       s1 += S.rad[t][1] * sv;
       s2 += S.rad[t][2] * sv;
       bits &= bits - 1u;
+    }
+    wave_lds_sync();  // the window's results are read before the next is written
+  }
+}
+
+// The same for a chunk whose triangles pair up into quads (triangles 2q,
+// 2q + 1 = (a, b, c), (a, c, d), as MESHSOURCE quads are listed): one unit
+// per quad with either triangle above the horizon (which: bit 0 for the
+// first, bit 1 for the second).  A unit transforms the four corners once and
+// evaluates five edges, the diagonal's term serving both triangles (c -> a
+// for the first, its negation a -> c for the second); each triangle's sum
+// keeps its own absolute value, as in the reference.
+__device__ inline void ltc_quad_pairs(const LtcLds &S, LtcLds::Wave &Wl, uint32_t above,
+                                      uint32_t nq, uint32_t lane, double &s0, double &s1,
+                                      double &s2) {
+  const unsigned long long below_me = (1ull << lane) - 1ull;
+  uint32_t np = 0;
+  for (uint32_t q = 0; q < nq; ++q) {
+    const uint32_t which = (above >> (2 * q)) & 3u;
+    const unsigned long long mt = __ballot(which != 0u);
+    if (which) Wl.pair[np + __popcll(mt & below_me)] = (uint16_t)((q << 8) | (which << 6) | lane);
+    if (lane == 0) {
+      Wl.mask[q] = mt;
+      Wl.base[q] = np;
+    }
+    np += (uint32_t)__popcll(mt);
+  }
+  wave_lds_sync();
+  uint32_t units = 0;  // this lane's quads with a triangle above
+  for (uint32_t q = 0; q < nq; ++q) units |= ((above >> (2 * q)) & 3u) ? 1u << q : 0u;
+  for (uint32_t w0 = 0; w0 < np; w0 += 64) {
+    {
+      const uint32_t e = min(w0 + lane, np - 1);
+      const uint32_t pr = Wl.pair[e], q = pr >> 8, P = pr & 63u;
+      const d3 xp = dk(Wl.frame[0][P], Wl.frame[1][P], Wl.frame[2][P]);
+      const d3 Q0 = dk(Wl.frame[3][P], Wl.frame[4][P], Wl.frame[5][P]);
+      const d3 Q1 = dk(Wl.frame[6][P], Wl.frame[7][P], Wl.frame[8][P]);
+      const d3 Q2 = dk(Wl.frame[9][P], Wl.frame[10][P], Wl.frame[11][P]);
+      d3 ea, eb, ec, ed;
+      ltc_vertices(S.tri[2 * q], xp, Q0, Q1, Q2, ea, eb, ec);
+      {
+        const double *p2 = S.tri[2 * q + 1];
+        const d3 dd = dk(p2[6], p2[7], p2[8]) - xp;
+        ed = dk(dot(Q0, dd), dot(Q1, dd), dot(Q2, dd));
+      }
+      const d3 cab = cross(ea, eb), cbc = cross(eb, ec), cca = cross(ec, ea), ccd = cross(ec, ed),
+               cda = cross(ed, ea);
+      const double Eca = edge64(ec, ea, cca, cca.z);
+      Wl.res[0][lane] = fabs(edge64(ea, eb, cab, cab.z) + edge64(eb, ec, cbc, cbc.z) + Eca);
+      Wl.res[1][lane] = fabs(-Eca + edge64(ec, ed, ccd, ccd.z) + edge64(ed, ea, cda, cda.z));
+    }
+    wave_lds_sync();
+    // this lane's units in the window, in quad (so triangle) order
+    while (units != 0u) {
+      const uint32_t q = (uint32_t)__builtin_ctz(units);
+      const uint32_t idx = Wl.base[q] + (uint32_t)__popcll(Wl.mask[q] & below_me);
+      if (idx >= w0 + 64) break;
+      const uint32_t which = (above >> (2 * q)) & 3u;
+      if (which & 1u) {
+        const double sv = Wl.res[0][idx - w0];
+        s0 += S.rad[2 * q][0] * sv;
+        s1 += S.rad[2 * q][1] * sv;
+        s2 += S.rad[2 * q][2] * sv;
+      }
+      if (which & 2u) {
+        const double sv = Wl.res[1][idx - w0];
+        s0 += S.rad[2 * q + 1][0] * sv;
+        s1 += S.rad[2 * q + 1][1] * sv;
+        s2 += S.rad[2 * q + 1][2] * sv;
+      }
+      units &= units - 1u;
     }
     wave_lds_sync();  // the window's results are read before the next is written
   }
@@ -184,9 +256,32 @@ __global__ __launch_bounds__(kLtcBlock) void k_ltc(const double *__restrict__ ta
       else
         S.rad[t][c - 9] = lrad[3 * (size_t)(j0 + t) + c - 9];
     }
+    if (threadIdx.x == 0) S.quads = 1u;
     __syncthreads();
+    // quads: an even count, each odd triangle (a', c', d) repeating corners
+    // a and c of the triangle (a, b, c) before it
+    if (threadIdx.x < jn / 2 || (threadIdx.x == 0 && (jn & 1u))) {
+      const uint32_t q = threadIdx.x;
+      const double *u = S.tri[2 * q], *v = S.tri[(2 * q + 1) % kChunk];
+      const bool ok = (jn & 1u) == 0u && v[0] == u[0] && v[1] == u[1] && v[2] == u[2] &&
+                      v[3] == u[6] && v[4] == u[7] && v[5] == u[8];
+      if (!ok) S.quads = 0u;
+    }
+    __syncthreads();
+    const bool quads = S.quads != 0u;
     uint32_t above = 0, straddle = 0;
-    if (do_ltc) {
+    if (do_ltc && quads) {  // the shared corners a, c once per quad
+      for (uint32_t q = 0; q < jn / 2; ++q) {
+        const double *p = S.tri[2 * q], *p2 = S.tri[2 * q + 1];
+        const double za = dot(R2, dk(p[0], p[1], p[2]) - x), zb = dot(R2, dk(p[3], p[4], p[5]) - x),
+                     zc = dot(R2, dk(p[6], p[7], p[8]) - x),
+                     zd = dot(R2, dk(p2[6], p2[7], p2[8]) - x);
+        const bool a1 = za > 0 && zb > 0 && zc > 0, b1 = za < 0 && zb < 0 && zc < 0;
+        const bool a2 = za > 0 && zc > 0 && zd > 0, b2 = za < 0 && zc < 0 && zd < 0;
+        above |= (a1 ? 1u << (2 * q) : 0u) | (a2 ? 2u << (2 * q) : 0u);
+        straddle |= (!a1 && !b1 ? 1u << (2 * q) : 0u) | (!a2 && !b2 ? 2u << (2 * q) : 0u);
+      }
+    } else if (do_ltc) {
       for (uint32_t t = 0; t < jn; ++t) {
         const double *p = S.tri[t];
         const double z1 = dot(R2, dk(p[0], p[1], p[2]) - x), z2 = dot(R2, dk(p[3], p[4], p[5]) - x),
@@ -196,7 +291,10 @@ __global__ __launch_bounds__(kLtcBlock) void k_ltc(const double *__restrict__ ta
         straddle |= (!a && !b) ? 1u << t : 0u;
       }
     }
-    ltc_pairs<false>(S, Wl, above, jn, lane, sl0, sl1, sl2);
+    if (quads)  // block-uniform
+      ltc_quad_pairs(S, Wl, above, jn / 2, lane, sl0, sl1, sl2);
+    else
+      ltc_pairs<false>(S, Wl, above, jn, lane, sl0, sl1, sl2);
     ltc_pairs<true>(S, Wl, straddle, jn, lane, sl0, sl1, sl2);
     if (__any(do_dif)) {  // wave-uniform (tri64 votes across the wave)
       for (uint32_t t = 0; t < jn; ++t) {

@@ -162,6 +162,26 @@ def test_ltc_random_quads_match_oracle(ctx):
     assert e.max() < 2e-7
 
 
+@pytest.mark.parametrize("order", ["reversed", "odd"])
+def test_ltc_triangles_not_in_quads_match_oracle(ctx, order):
+    """The same random quads as triangles that do not pair up as (a, b, c),
+    (a, c, d) (their order reversed; or one triangle dropped, an odd count):
+    the per-triangle path, against the oracle on the same list."""
+    P = scenes.ltc_points(20000, seed=0x5EED0013, n_quads=16)
+    tris, rad = scenes.quads_to_tris(P["quads"], P["radiance"])
+    tris, rad = (tris[::-1], rad[::-1]) if order == "reversed" else (tris[:-1], rad[:-1])
+    tris, rad = np.ascontiguousarray(tris), np.ascontiguousarray(rad.astype(np.float32).astype(np.float64))
+    ks = np.array([0.0, 0.5, 1.0], np.float32)[np.arange(20000) % 3]
+    ins = [P["x"].astype(np.float32), P["n"].astype(np.float32), P["v"].astype(np.float32),
+           P["alpha"].astype(np.float32), ks]
+    o = oracle.ltc_shade(*ins[:4], ks.astype(np.float64), tris.astype(np.float32), rad)
+    ls = host.LightSource(ctx, tris, rad)
+    rgb = ls.getLightFromMeshSource_Analytic(*ins).astype(np.float64)
+    e = _ltc_err(rgb, o["rgb"])
+    print(f"{order}: max rel {e.max():.2e}, p99.9 {np.quantile(e, 0.999):.2e}")
+    assert e.max() < 2e-7
+
+
 def test_ltc_closed_box_integrates_to_one(ctx):
     """A closed box around the shading point covers the whole sphere: the
     cosine-domain integral of the upper hemisphere is 1 after clipping, so
