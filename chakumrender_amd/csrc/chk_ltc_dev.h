@@ -39,7 +39,7 @@ __device__ inline double rsq64(double x) {
 __device__ inline double atan2_upper(double y, double x) {
   const double ax = fabs(x);
   const double mx = fmax(y, ax), mn = fmin(y, ax);
-  const double t = mx > 0.0 ? mn * rcp64(mx) : 0.0;
+  const double t = mn * rcp64(fmax(mx, 1e-300));  // 0 for x = y = 0
   const double w = t * t;
   double p = 7.063142635888046e-05;
   p = fma(p, w, -0.0006774105399645076);
@@ -70,15 +70,15 @@ __device__ inline double atan2_upper(double y, double x) {
 // reference's acos(v1.v2), without its loss of precision near 0).  For
 // theta <= 0.001 the reference uses the limit cn/(|p1||p2|), which is
 // cn (theta/|c|) (sin(theta)/theta), here with the series 1 - theta^2/6
-// (1 - theta^2/20) (exact to 1e-22 there).  A zero edge (p1 = p2) gives 0.
+// + theta^4/120 (exact to 1e-22 there).  A zero edge (c = 0, so cn = 0)
+// gives 0: |c|^2 is floored at 1e-300, so every factor stays finite.
 __device__ inline double edge64(d3 p1, d3 p2, d3 c, double cn) {
   const double cc = dot(c, c);
-  const double rs = rsq64(cc > 0.0 ? cc : 1.0);
+  const double rs = rsq64(fmax(cc, 1e-300));
   const double th = atan2_upper(cc * rs, dot(p1, p2));
-  const double q = th * rs;
   const double t2 = th * th;
-  const double g = th > 0.001 ? q : q * fma(-t2 * (1.0 / 6.0), fma(-t2, 1.0 / 20.0, 1.0), 1.0);
-  return cc > 0.0 ? cn * g : 0.0;
+  const double sr = fma(t2, fma(t2, 1.0 / 120.0, -1.0 / 6.0), 1.0);
+  return cn * ((th * rs) * (th > 0.001 ? 1.0 : sr));
 }
 
 // n.v for the clip plane: the LTC part clips against z = 0 in its own
