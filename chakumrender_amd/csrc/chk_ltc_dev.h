@@ -19,7 +19,7 @@ constexpr double kPi64 = 3.14159265359;  // mathPrimitives.h:4
 // of both matrices are zero in the fit.
 constexpr int kLtcRec = 12;
 
-// 1/x and 1/sqrt(x) for positive normal x: the hardware approximation
+// 1/x (normal x of either sign) and 1/sqrt(x) (positive normal x): the hardware approximation
 // (v_rcp_f64 / v_rsq_f64, relative error below 2^-24 on gfx950) and one
 // Newton step (relative error below 5e-15, measured over 2^20 arguments
 // spanning 2^-30..2^31), cheaper than a divide or sqrt.
@@ -125,7 +125,7 @@ __device__ bool clip64(d3 A, d3 B, d3 C, d3 n, d3 P, double nA, double nB, doubl
                pC = ndot<ZPLANE>(n, P - C);
   {
     const bool cuts = nAB * nAB > eps;
-    const d3 I = dAB * (pA / (cuts ? nAB : 1.0)) + A;
+    const d3 I = dAB * (pA * rcp64(cuts ? nAB : 1.0)) + A;
     const bool hit = cuts && dot(I - A, B - I) > 0;
     dead = dead || (!cuts && pA * pA < eps);
     cnt += hit ? 1 : 0;
@@ -133,7 +133,7 @@ __device__ bool clip64(d3 A, d3 B, d3 C, d3 n, d3 P, double nA, double nB, doubl
   }
   {
     const bool cuts = nBC * nBC > eps;
-    const d3 I = dBC * (pB / (cuts ? nBC : 1.0)) + B;
+    const d3 I = dBC * (pB * rcp64(cuts ? nBC : 1.0)) + B;
     const bool hit = cuts && dot(I - B, C - I) > 0;
     dead = dead || (!cuts && pB * pB < eps);
     cnt += hit ? 1 : 0;
@@ -141,7 +141,7 @@ __device__ bool clip64(d3 A, d3 B, d3 C, d3 n, d3 P, double nA, double nB, doubl
   }
   {
     const bool cuts = nCA * nCA > eps;
-    const d3 I = dCA * (pC / (cuts ? nCA : 1.0)) + C;
+    const d3 I = dCA * (pC * rcp64(cuts ? nCA : 1.0)) + C;
     // the reference tests only the x components on this edge
     const bool hit = cuts && (I.x - C.x) * (A.x - I.x) > 0;
     dead = dead || (!cuts && pC * pC < eps);
