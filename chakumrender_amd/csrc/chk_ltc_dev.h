@@ -33,31 +33,32 @@ __device__ inline double rsq64(double x) {
 }
 
 // atan2(y, x) for y >= 0 (an angle in [0, pi]): t = min/max of (y, |x|) in
-// [0, 1], atan(t) = t P(t^2) with P the degree-16 Chebyshev fit of
-// atan(sqrt(w))/sqrt(w) on [0, 1] (relative error 7.7e-15 in FP64;
-// gen_atan_poly.py), then the octant.  Branch free.
+// [0, 1], atan(t) = t P(t^2) with P the degree-14 Chebyshev fit of
+// atan(sqrt(w))/sqrt(w) on [0, 1] (relative error 2.9e-13 in FP64;
+// gen_atan_poly.py 14), then the octant.  Branch free.  The edge terms of a
+// triangle seen nearly edge-on cancel by up to ~5 orders of magnitude, so
+// the fit keeps 13 digits (degree 12, at 1.1e-11, moved such points' FP32
+// results by 4e-7 relative).
 __device__ inline double atan2_upper(double y, double x) {
   const double ax = fabs(x);
   const double mx = fmax(y, ax), mn = fmin(y, ax);
   const double t = mn * rcp64(fmax(mx, 1e-300));  // 0 for x = y = 0
   const double w = t * t;
-  double p = 7.063142635888046e-05;
-  p = fma(p, w, -0.0006774105399645076);
-  p = fma(p, w, 0.0030659957028108006);
-  p = fma(p, w, -0.008792435542217635);
-  p = fma(p, w, 0.018198570485395787);
-  p = fma(p, w, -0.029589012194344372);
-  p = fma(p, w, 0.04051778977125066);
-  p = fma(p, w, -0.04977996675955047);
-  p = fma(p, w, 0.05794192316881845);
-  p = fma(p, w, -0.06646145790221254);
-  p = fma(p, w, 0.07688811662401294);
-  p = fma(p, w, -0.09090489647799033);
-  p = fma(p, w, 0.11111077584971772);
-  p = fma(p, w, -0.142857126460504);
-  p = fma(p, w, 0.19999999957481826);
-  p = fma(p, w, -0.3333333333289398);
-  p = fma(p, w, 0.9999999999999925);
+  double p = 0.0001704604046618824;
+  p = fma(p, w, -0.0014666686334116588);
+  p = fma(p, w, 0.005936376596452122);
+  p = fma(p, w, -0.01523249190907656);
+  p = fma(p, w, 0.028410612112699588);
+  p = fma(p, w, -0.042324616793255394);
+  p = fma(p, w, 0.054582394542429476);
+  p = fma(p, w, -0.06536228811849373);
+  p = fma(p, w, 0.07663177357754078);
+  p = fma(p, w, -0.09086360660727331);
+  p = fma(p, w, 0.11110640790649381);
+  p = fma(p, w, -0.14285684661254053);
+  p = fma(p, w, 0.1999999901342103);
+  p = fma(p, w, -0.3333333332025471);
+  p = fma(p, w, 0.99999999999971);
   double a = t * p;
   a = y > ax ? 1.5707963267948966 - a : a;
   return x < 0.0 ? 3.141592653589793 - a : a;
