@@ -162,6 +162,33 @@ def test_ltc_random_quads_match_oracle(ctx):
     assert e.max() < 2e-7
 
 
+def test_ltc_error_by_roughness(ctx):
+    """Relative error against the oracle per GGX roughness bin (200,000
+    random points, 16 random quads, ks = 1, the LTC part): every bin at the
+    FP32 rounding of the output."""
+    P = scenes.ltc_points(200000, seed=0x5EED0023, n_quads=16)
+    tris, rad = scenes.quads_to_tris(P["quads"], P["radiance"])
+    rad = rad.astype(np.float32).astype(np.float64)
+    ks = np.ones(200000, np.float32)
+    ins = [P["x"].astype(np.float32), P["n"].astype(np.float32), P["v"].astype(np.float32),
+           P["alpha"].astype(np.float32), ks]
+    o = oracle.ltc_shade(*ins[:4], ks.astype(np.float64), tris.astype(np.float32), rad)
+    ls = host.LightSource(ctx, tris, rad)
+    rgb = ls.getLightFromMeshSource_Analytic(*ins).astype(np.float64)
+    g = o["rgb"]
+    scale = np.abs(g).max(1)
+    err = np.abs(rgb - g).max(1) / np.maximum(scale, 1e-30)
+    lit = scale > 1e-6 * scale.max()
+    edges = np.array([0.02, 0.05, 0.1, 0.2, 0.3, 0.4, 0.5, 0.6, 0.7, 0.8, 0.9, 1.0])
+    b = np.clip(np.searchsorted(edges, P["alpha"], side="right") - 1, 0, len(edges) - 2)
+    print("alpha bin | lit points | max rel | p99 | median")
+    for i in range(len(edges) - 1):
+        e = err[lit & (b == i)]
+        print(f"[{edges[i]:.2f}, {edges[i + 1]:.2f}) | {e.size} | {e.max():.2e} | "
+              f"{np.quantile(e, 0.99):.2e} | {np.median(e):.2e}")
+        assert e.size > 1000 and e.max() < 2e-7, (i, e.max())
+
+
 @pytest.mark.parametrize("order", ["reversed", "odd"])
 def test_ltc_triangles_not_in_quads_match_oracle(ctx, order):
     """The same random quads as triangles that do not pair up as (a, b, c),
