@@ -215,7 +215,7 @@ def ltc_leg(args, ctx):
         abi.sync()
         return (time.perf_counter() - t0) / reps
 
-    trials = [trial() for _ in range(3)]
+    trials = [trial() for _ in range(5)]
     dt = min(trials)
     pairs = k * args.ltc_quads
     flop = 180.0 * pairs
@@ -223,7 +223,7 @@ def ltc_leg(args, ctx):
         "workload": f"{k} points x {args.ltc_quads} quads ({2 * args.ltc_quads} triangles), "
                     "GGX LTC part (ks = 1), FP64 arithmetic, FP32 in/out",
         "ms": round(dt * 1e3, 3),
-        "timing": "best of 3 trials of 5 launches (wall clock, synchronised)",
+        "timing": "best of 5 trials of 5 launches (wall clock, synchronised; the clock ramps over the first)",
         "trials_ms": [round(t * 1e3, 3) for t in trials],
         "gpairs_per_s": round(pairs / dt / 1e9, 3),
         "roofline": {"bound": "fp64 valu", "achieved": round(flop / dt / 1e12, 3),
@@ -264,14 +264,14 @@ def mlp_leg(args, ctx):
         abi.sync()
         return (time.perf_counter() - t0) / reps
 
-    trials = [trial() for _ in range(3)]
+    trials = [trial() for _ in range(5)]
     dt = min(trials)
     flop = 28800.0 * k
     return {
         "workload": f"{k} queries, 7 inputs -> 64-wide encoding -> 3 x 64 ReLU -> 33 outputs, "
                     "BF16 MFMA with FP32 accumulation, 32-light softmax + mixing sigmoid",
         "ms": round(dt * 1e3, 4),
-        "timing": "best of 3 trials of 10 launches (wall clock, synchronised)",
+        "timing": "best of 5 trials of 10 launches (wall clock, synchronised)",
         "trials_ms": [round(t * 1e3, 4) for t in trials],
         "gqueries_per_s": round(k / dt / 1e9, 3),
         "hbm_gbs": round(160.0 * k / dt / 1e9, 1),
