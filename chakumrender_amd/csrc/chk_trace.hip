@@ -22,6 +22,16 @@ __device__ inline float safe_rcp(float x) {
   return 1.0f / (fabsf(x) > 1e-20f ? x : copysignf(1e-20f, x));
 }
 
+// Traversal with postponed leaves (Aila and Laine, "Understanding the
+// Efficiency of Ray Traversal on GPUs", HPG 2009): a lane that reaches a leaf
+// keeps it aside and goes on down inner nodes while other lanes of its wave
+// still descend; the wave leaves the inner-node loop once every lane holds
+// a leaf (or has no node left) and then tests its leaves together.  Inner
+// nodes and triangle tests thus each run with more of the wave's lanes
+// active.  The leaves a ray tests, and so its closest hit, are those of the
+// one-node-per-step order; only the order among them can change (the first
+// of two exactly equally near triangles found wins, as in
+// BvhAccel::intersectT's strict test).
 template <bool ANY, bool COUNT, int STACK, int BS>
 __device__ inline void traverse(const DeviceScene &S, uint32_t *__restrict__ stk, f3 o, f3 d,
                                 float tmax, uint32_t skip, bool leaf_ids, float &t_out,
@@ -33,12 +43,26 @@ __device__ inline void traverse(const DeviceScene &S, uint32_t *__restrict__ stk
   float t = tmax;
   uint32_t best = 0xFFFFFFFFu, best_leaf = 0u;
   float bu = 0.0f, bv = 0.0f;
-  uint32_t entry = 0;  // root
+  uint32_t entry = 0;  // root; valid while live
+  bool live = true;
+  uint32_t pend = 0;   // postponed leaf (leaves carry kLeafBit, so 0 is none)
   int sp = 0;
   hit_any = false;
   const float kGrow = 1.0f + 2.0f * 3.5762786865234375e-07f;  // 1 + 2*gamma_3
+  auto pop = [&]() {
+    if (sp == 0) {
+      live = false;
+    } else {
+      --sp;
+      entry = stk[sp * BS];
+    }
+  };
   while (true) {
-    if (!(entry & kLeafBit)) {
+    // ---- inner nodes
+    while (true) {
+      const bool searching = live && !(entry & kLeafBit);
+      if (!__any(searching) || __all(pend != 0 || !searching)) break;  // wave-uniform
+      if (!searching) continue;
       if (COUNT) ++nv;
       const float4 *np = reinterpret_cast<const float4 *>(S.nodes + entry);
       const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
@@ -84,10 +108,25 @@ __device__ inline void traverse(const DeviceScene &S, uint32_t *__restrict__ stk
             overflow = 1;
           }
         }
-        continue;
+      } else {
+        pop();
       }
-    } else {
-      const uint32_t first = leaf_first(entry), cnt = leaf_count(entry);
+      if (live && (entry & kLeafBit) && pend == 0) {  // postpone the first leaf
+        pend = entry;
+        pop();
+      }
+    }
+    // ---- leaves: the postponed one, then any that the lane holds next
+    while (true) {
+      uint32_t lf = pend;
+      pend = 0;
+      if (lf == 0 && live && (entry & kLeafBit)) {
+        lf = entry;
+        pop();
+      }
+      if (!__any(lf != 0)) break;  // wave-uniform
+      if (lf == 0) continue;
+      const uint32_t first = leaf_first(lf), cnt = leaf_count(lf);
       for (uint32_t k = 0; k < cnt; ++k) {
         const uint32_t li = first + k;
         const float4 *tp = S.tris + 3 * (size_t)li;
@@ -106,11 +145,10 @@ __device__ inline void traverse(const DeviceScene &S, uint32_t *__restrict__ stk
           if (ANY) break;
         }
       }
-      if (ANY && hit_any) break;
+      if (ANY && hit_any) live = false;  // occluded: nothing more to test
     }
-    if (sp == 0) break;
-    --sp;
-    entry = stk[sp * BS];
+    if (!__any(live)) break;  // wave-uniform
+    if (!live) continue;
   }
   t_out = best == 0xFFFFFFFFu ? 1e30f : t;
   id_out = best;
