@@ -614,6 +614,10 @@ chk_status chk_scene_destroy(chk_scene_t sc) {
   if (!sc) return CHK_OK;
   (void)hipSetDevice(sc->ctx->device);
   if (sc->work) (void)hipFree(sc->work);
+  if (sc->work2) (void)hipFree(sc->work2);
+  if (sc->st2) (void)hipStreamDestroy(sc->st2);
+  if (sc->ev_fork) (void)hipEventDestroy(sc->ev_fork);
+  if (sc->ev_join) (void)hipEventDestroy(sc->ev_join);
   if (sc->lwork) (void)hipFree(sc->lwork);
   if (sc->dwork) (void)hipFree(sc->dwork);
   if (sc->env_mem) (void)hipFree(sc->env_mem);

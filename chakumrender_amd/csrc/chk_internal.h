@@ -127,6 +127,12 @@ struct chk_scene_s {
   size_t work_bytes = 0;
   void *lwork = nullptr;  // learned-proposal buffers of the render
   size_t lwork_bytes = 0;
+  // a second path-queue set and stream: chk_render runs every other chunk
+  // there, so one chunk's last, sparse bounces overlap the next one's first
+  void *work2 = nullptr;
+  size_t work2_bytes = 0;
+  hipStream_t st2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   void *dwork = nullptr;  // direct-lighting estimators' chunk buffers
   size_t dwork_bytes = 0;
   uint32_t n_mats = 0;

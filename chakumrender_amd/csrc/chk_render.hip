@@ -862,6 +862,31 @@ chk_status render_impl(chk_scene_s *sc, const chk_camera *cam, const chk_render_
   }
   Buffers B = carve(work, qcap, cap);
   CHK_HIP(launch_zero_u32(B.ovf, 2, st));  // overflow count, retries
+  // Plain renders of several chunks alternate two queue sets and streams:
+  // chunk c + 1 starts while chunk c finishes its last bounces, which hold
+  // few rays and leave most of the GPU idle.  Chunks hold disjoint pixels,
+  // so their accumulations never meet; the statistics are shared atomics.
+  const uint64_t n_chunks_total = (n_pix_total + pix_per_chunk - 1) / pix_per_chunk;
+  const bool dual = !learned && !ws && !count && n_chunks_total > 1;
+  Buffers B2 = B;
+  if (dual) {
+    if (sc->work2_bytes < need) {
+      if (sc->work2) (void)hipFree(sc->work2);
+      sc->work2 = nullptr;
+      sc->work2_bytes = 0;
+      if (hipMalloc(&sc->work2, need) != hipSuccess) {
+        set_error("chk_render: cannot allocate the second path-queue set");
+        return CHK_ENOMEM;
+      }
+      sc->work2_bytes = need;
+    }
+    if (!sc->st2) CHK_HIP(hipStreamCreateWithFlags(&sc->st2, hipStreamNonBlocking));
+    if (!sc->ev_fork) CHK_HIP(hipEventCreateWithFlags(&sc->ev_fork, hipEventDisableTiming));
+    if (!sc->ev_join) CHK_HIP(hipEventCreateWithFlags(&sc->ev_join, hipEventDisableTiming));
+    B2 = carve(sc->work2, qcap, cap);
+    B2.stats = B.stats;
+    CHK_HIP(launch_zero_u32(B2.ovf, 2, st));
+  }
   LBuffers LB{};
   if (learned) {
     const size_t lneed = lbuffers_bytes(qcap);
@@ -887,39 +912,48 @@ chk_status render_impl(chk_scene_s *sc, const chk_camera *cam, const chk_render_
 
   hipLaunchKernelGGL(k_reset, dim3((kCtrlWords + 255) / 256), dim3(256), 0, st, B.ctrl, B.stats,
                      1);
+  if (dual) {
+    CHK_HIP(hipEventRecord(sc->ev_fork, st));
+    CHK_HIP(hipStreamWaitEvent(sc->st2, sc->ev_fork, 0));
+  }
+  const hipStream_t st_main = st;
   for (uint64_t pix0 = 0; pix0 < n_pix_total; pix0 += pix_per_chunk) {
+    const bool second = dual && (chunks & 1u);
+    const hipStream_t st_c = second ? sc->st2 : st_main;
+    Buffers &Bc = second ? B2 : B;
+    prof.st = st_c;
     ++chunks;
     A.pix0 = (uint32_t)pix0;
     A.n_pix = (uint32_t)std::min<uint64_t>(pix_per_chunk, n_pix_total - pix0);
     const uint32_t n_paths = A.n_pix * A.spp;
     prof.begin(3);
-    hipLaunchKernelGGL(k_reset, dim3((kCtrlWords + 255) / 256), dim3(256), 0, st, B.ctrl, B.stats,
+    hipLaunchKernelGGL(k_reset, dim3((kCtrlWords + 255) / 256), dim3(256), 0, st_c, Bc.ctrl, Bc.stats,
                        0);
-    hipLaunchKernelGGL(k_queue_counts, dim3(1), dim3(64), 0, st, B.ctrl, n_paths);
-    hipLaunchKernelGGL(k_camera, dim3((n_paths + 255) / 256), dim3(256), 0, st, A, segcap,
-                       n_paths, B.q_ray[0], B.q_thr[0], B.q_pid[0], B.path_L, nullptr, 0u);
+    hipLaunchKernelGGL(k_queue_counts, dim3(1), dim3(64), 0, st_c, Bc.ctrl, n_paths);
+    hipLaunchKernelGGL(k_camera, dim3((n_paths + 255) / 256), dim3(256), 0, st_c, A, segcap,
+                       n_paths, Bc.q_ray[0], Bc.q_thr[0], Bc.q_pid[0], Bc.path_L, nullptr, 0u);
     prof.end();
     for (uint32_t depth = 0; depth <= P->max_depth; ++depth) {
       const uint32_t in = depth & 1u, out = in ^ 1u;
-      hipLaunchKernelGGL(k_reset_bounce, dim3(1), dim3(64), 0, st, B.ctrl, qbase(out));
+      hipLaunchKernelGGL(k_reset_bounce, dim3(1), dim3(64), 0, st_c, Bc.ctrl, qbase(out));
       prof.begin(0);
-      hipError_t e = launch_closest(sc, TraceShape{sc->stack_size, 128, B.ovf, qcap, sc->shallow_stack},
-                                    B.q_ray[in],
-                                    SegRange{B.ctrl + qbase(in), 0u, segcap}, B.hits, true, count,
-                                    B.stats + kNodeVisits, SlotScenes{}, st);
+      hipError_t e = launch_closest(sc, TraceShape{sc->stack_size, 128, Bc.ovf, qcap, sc->shallow_stack},
+                                    Bc.q_ray[in],
+                                    SegRange{Bc.ctrl + qbase(in), 0u, segcap}, Bc.hits, true, count,
+                                    Bc.stats + kNodeVisits, SlotScenes{}, st_c);
       prof.end();
       ++ext_launches;
       if (e != hipSuccess) {
         set_error(std::string("chk_render: closest-hit launch: ") + hipGetErrorString(e));
         return CHK_EDEVICE;
       }
-      const SegRange inR{B.ctrl + qbase(in), 0u, segcap};
+      const SegRange inR{Bc.ctrl + qbase(in), 0u, segcap};
       LearnedIO io{};
       if (learned) {
         prof.begin(4);
-        hipLaunchKernelGGL(k_prop_in, dim3(kSegs * (segcap / 256)), dim3(256), 0, st, A, segcap,
-                           B.q_ray[in], B.hits, B.ctrl, qbase(in), LB.inputs[in]);
-        e = launch_mlp_eval(prop, LB.inputs[in], inR, LB.probs[in], LB.mix[in], nullptr, st);
+        hipLaunchKernelGGL(k_prop_in, dim3(kSegs * (segcap / 256)), dim3(256), 0, st_c, A, segcap,
+                           Bc.q_ray[in], Bc.hits, Bc.ctrl, qbase(in), LB.inputs[in]);
+        e = launch_mlp_eval(prop, LB.inputs[in], inR, LB.probs[in], LB.mix[in], nullptr, st_c);
         prof.end();
         if (e != hipSuccess) {
           set_error(std::string("chk_render_learned: proposal launch: ") + hipGetErrorString(e));
@@ -938,14 +972,14 @@ chk_status render_impl(chk_scene_s *sc, const chk_camera *cam, const chk_render_
       prof.begin(1);
       if (learned)
         hipLaunchKernelGGL((k_shade<true, false, kShadeBlock>), dim3(sgrid), dim3(kShadeBlock), 0,
-                           st, A, depth, segcap, B.q_ray[in], B.q_thr[in], B.q_pid[in], B.hits,
-                           B.q_ray[out], B.q_thr[out], B.q_pid[out], B.s_ray, B.s_L, B.path_L,
-                           B.ctrl, qbase(in), qbase(out), io, BatchIO{});
+                           st_c, A, depth, segcap, Bc.q_ray[in], Bc.q_thr[in], Bc.q_pid[in], Bc.hits,
+                           Bc.q_ray[out], Bc.q_thr[out], Bc.q_pid[out], Bc.s_ray, Bc.s_L, Bc.path_L,
+                           Bc.ctrl, qbase(in), qbase(out), io, BatchIO{});
       else
         hipLaunchKernelGGL((k_shade<false, false, kShadeBlock>), dim3(sgrid), dim3(kShadeBlock), 0,
-                           st, A, depth, segcap, B.q_ray[in], B.q_thr[in], B.q_pid[in], B.hits,
-                           B.q_ray[out], B.q_thr[out], B.q_pid[out], B.s_ray, B.s_L, B.path_L,
-                           B.ctrl, qbase(in), qbase(out), io, BatchIO{});
+                           st_c, A, depth, segcap, Bc.q_ray[in], Bc.q_thr[in], Bc.q_pid[in], Bc.hits,
+                           Bc.q_ray[out], Bc.q_thr[out], Bc.q_pid[out], Bc.s_ray, Bc.s_L, Bc.path_L,
+                           Bc.ctrl, qbase(in), qbase(out), io, BatchIO{});
       prof.end();
       // the previous bounce's records are complete now (its shadow rays
       // were traced, this bounce found the emitters its BSDF rays hit)
@@ -954,10 +988,10 @@ chk_status render_impl(chk_scene_s *sc, const chk_camera *cam, const chk_render_
         const uint32_t stride = std::max(1u, cap / train->batch);
         for (uint32_t t = 0; t < train->steps_per_bounce; ++t) {
           chk_status ts = proposal_train_step(
-              prop, SegRange{B.ctrl + kQP, 0u, segcap}, LB.rec[out], LB.inputs[out],
+              prop, SegRange{Bc.ctrl + kQP, 0u, segcap}, LB.rec[out], LB.inputs[out],
               LB.probs[out], LB.mix[out], LB.s_occ, train->batch, stride,
               (uint32_t)(((uint64_t)t * stride) / train->steps_per_bounce), train->lr,
-              train->flags, st);
+              train->flags, st_c);
           if (ts != CHK_OK) return ts;
           ++train_steps;
         }
@@ -965,10 +999,10 @@ chk_status render_impl(chk_scene_s *sc, const chk_camera *cam, const chk_render_
       }
       if (depth < P->max_depth) {
         prof.begin(2);
-        e = launch_any(sc, TraceShape{sc->stack_size, 128, B.ovf, qcap, sc->shallow_stack},
-                       B.s_ray,
-                       SegRange{B.ctrl + kQS, 0u, segcap}, training ? LB.s_occ : nullptr, B.s_L,
-                       B.path_L, true, count, B.stats + kShNodeVisits, SlotScenes{}, st);
+        e = launch_any(sc, TraceShape{sc->stack_size, 128, Bc.ovf, qcap, sc->shallow_stack},
+                       Bc.s_ray,
+                       SegRange{Bc.ctrl + kQS, 0u, segcap}, training ? LB.s_occ : nullptr, Bc.s_L,
+                       Bc.path_L, true, count, Bc.stats + kShNodeVisits, SlotScenes{}, st_c);
         prof.end();
         if (e != hipSuccess) {
           set_error(std::string("chk_render: any-hit launch: ") + hipGetErrorString(e));
@@ -977,20 +1011,25 @@ chk_status render_impl(chk_scene_s *sc, const chk_camera *cam, const chk_render_
       }
     }
     prof.begin(3);
-    hipLaunchKernelGGL(k_accumulate, dim3(accumulate_blocks(A.n_pix, A.spp)), dim3(256), 0, st, A,
-                       B.path_L,
+    hipLaunchKernelGGL(k_accumulate, dim3(accumulate_blocks(A.n_pix, A.spp)), dim3(256), 0, st_c, A,
+                       Bc.path_L,
                        accum, (P->flags & CHK_RENDER_SUMSQ) ? sumsq : nullptr);
     prof.end();
   }
   CHK_HIP(hipGetLastError());
+  if (dual) {
+    CHK_HIP(hipEventRecord(sc->ev_join, sc->st2));
+    CHK_HIP(hipStreamWaitEvent(st, sc->ev_join, 0));
+  }
   if (stats) {
     std::memset(stats, 0, sizeof(*stats));
     unsigned long long h[kStatWords];
     CHK_HIP(hipMemcpyAsync(h, B.stats, sizeof(h), hipMemcpyDeviceToHost, st));
-    uint32_t retries = 0;
+    uint32_t retries = 0, retries2 = 0;
     CHK_HIP(hipMemcpyAsync(&retries, B.ovf + 1, 4, hipMemcpyDeviceToHost, st));
+    if (dual) CHK_HIP(hipMemcpyAsync(&retries2, B2.ovf + 1, 4, hipMemcpyDeviceToHost, st));
     CHK_HIP(hipStreamSynchronize(st));
-    stats->stack_retries = retries;
+    stats->stack_retries = retries + retries2;
     stats->paths = n_pix_total * P->spp;
     stats->rays_extend = h[kRaysExt];
     stats->rays_shadow = h[kRaysShadow];

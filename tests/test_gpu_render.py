@@ -213,6 +213,29 @@ def test_render_window_equals_whole_image(ctx, win):
     np.testing.assert_array_equal(got[1][0], full)
 
 
+def test_multi_chunk_render_equals_its_windows(ctx):
+    """A render of three chunks of 2^24 paths (its chunks alternate two
+    queue sets and streams) equals, bit for bit, the same image rendered as
+    three row windows of one chunk each, and is deterministic."""
+    W, H, spp = 512, 1536, 64  # 50,331,648 paths
+    sc = scenes.cornell(width=W, height=H, sphere_tris=2000, n_lights=3, light_rng=True)
+    ds = abi.DeviceScene(ctx, sc)
+    P = dict(spp=spp, max_depth=3, seed=0x5EED0051, flags=abi.RENDER_SUMSQ)
+    full, fsq, st = host.render(ds, P)
+    assert st["chunks"] == 3
+    again, _, _ = host.render(ds, P)
+    np.testing.assert_array_equal(again, full)
+    img = host.Image(W, H, sumsq=True)
+    rows = (1 << 24) // (W * spp)
+    for y0 in range(0, H, rows):
+        _, st1 = host.render(ds, dict(P, window=(0, y0, W, min(rows, H - y0))), image=img,
+                             download=False)
+        assert st1["chunks"] == 1
+    acc, sq = img.get()
+    np.testing.assert_array_equal(acc, full)
+    np.testing.assert_array_equal(sq, fsq)
+
+
 def test_render_window_rejected_outside_the_image(ctx):
     ds = abi.DeviceScene(ctx, scenes.cornell(width=16, height=8))
     for bad in [(0, 0, 17, 1), (15, 0, 2, 1), (0, 8, 1, 1), (0, 0, 0, 4)]:
