@@ -1076,8 +1076,10 @@ struct BatchChunk {
 };
 
 struct BatchState {
-  void *work = nullptr;
-  size_t work_bytes = 0;
+  void *work = nullptr, *work2 = nullptr;  // two path-queue sets (chunks alternate)
+  size_t work_bytes = 0, work2_bytes = 0;
+  hipStream_t st2 = nullptr;  // the stream of every other chunk
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   void *tab = nullptr;
   size_t tab_bytes = 0;
   std::vector<unsigned char> key;  // request records + tuning of the cached graphs
@@ -1091,7 +1093,11 @@ struct BatchState {
   ~BatchState() {
     drop_graphs();
     if (cap) (void)hipStreamDestroy(cap);
+    if (st2) (void)hipStreamDestroy(st2);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
     if (work) (void)hipFree(work);
+    if (work2) (void)hipFree(work2);
     if (tab) (void)hipFree(tab);
   }
 };
@@ -1254,16 +1260,37 @@ chk_status batch_impl(chk_ctx_s *ctx, const chk_request *R, uint32_t n,
   bool moved = false;
   chk_status cs = ensure(bs->work, bs->work_bytes, need, "the path queues", moved);
   if (cs != CHK_OK) return cs;
+  // chunks alternate two queue sets and streams (as in chk_render)
+  const bool dual = chunks.size() > 1;
+  if (dual) {
+    cs = ensure(bs->work2, bs->work2_bytes, need, "the second path-queue set", moved);
+    if (cs != CHK_OK) return cs;
+    if (!bs->st2) CHK_HIP(hipStreamCreateWithFlags(&bs->st2, hipStreamNonBlocking));
+    if (!bs->ev_fork) CHK_HIP(hipEventCreateWithFlags(&bs->ev_fork, hipEventDisableTiming));
+    if (!bs->ev_join) CHK_HIP(hipEventCreateWithFlags(&bs->ev_join, hipEventDisableTiming));
+  }
   const size_t tab_need = segs.size() * (sizeof(RenderArgs) + sizeof(DeviceScene)) + 512;
   cs = ensure(bs->tab, bs->tab_bytes, tab_need, "the request tables", moved);
   if (cs != CHK_OK) return cs;
   if (moved) bs->drop_graphs();
   const bool cached = T.use_graph && bs->key == key && !bs->execs.empty();
   if (T.use_graph && !cached) bs->drop_graphs();
-  Buffers B = carve(bs->work, qcap, cap);
-  uint32_t *segbuf = (uint32_t *)((char *)bs->work + ((buffers_bytes(qcap, cap) + 255) & ~(size_t)255));
-  uint32_t *q_seg[2] = {segbuf, segbuf + qcap};
-  uint32_t *s_seg = segbuf + 2 * (size_t)qcap;
+  // queue set k: path queues, counters and the slots' segment indices
+  struct QSet {
+    Buffers B;
+    uint32_t *q_seg[2];
+    uint32_t *s_seg;
+  } qs[2];
+  for (int k = 0; k < (dual ? 2 : 1); ++k) {
+    void *w = k ? bs->work2 : bs->work;
+    qs[k].B = carve(w, qcap, cap);
+    uint32_t *segbuf = (uint32_t *)((char *)w + ((buffers_bytes(qcap, cap) + 255) & ~(size_t)255));
+    qs[k].q_seg[0] = segbuf;
+    qs[k].q_seg[1] = segbuf + qcap;
+    qs[k].s_seg = segbuf + 2 * (size_t)qcap;
+  }
+  if (dual) qs[1].B.stats = qs[0].B.stats;  // one set of statistics counters
+  const Buffers &B0 = qs[0].B;
   std::vector<RenderArgs> seg_args(segs.size());
   std::vector<DeviceScene> seg_scene(segs.size());
   for (size_t k = 0; k < segs.size(); ++k) {
@@ -1282,10 +1309,14 @@ chk_status batch_impl(chk_ctx_s *ctx, const chk_request *R, uint32_t n,
                     hipMemcpyHostToDevice));
   CHK_HIP(hipMemcpy(d_scene, seg_scene.data(), segs.size() * sizeof(DeviceScene),
                     hipMemcpyHostToDevice));
-  const TraceShape shape{stack, (int)T.trace_block, B.ovf, qcap, 16};
   auto qbase = [](uint32_t q) { return q == 0 ? (uint32_t)kQ0 : (uint32_t)kQ1; };
-  // ---- the launches of one chunk
+  // ---- the launches of one chunk (queue set c % 2 when dual)
   auto emit_chunk = [&](uint32_t c, hipStream_t s) -> hipError_t {
+    const QSet &X = qs[dual ? (c & 1u) : 0u];
+    const Buffers &B = X.B;
+    uint32_t *const *q_seg = X.q_seg;
+    uint32_t *s_seg = X.s_seg;
+    const TraceShape shape{stack, (int)T.trace_block, B.ovf, qcap, 16};
     const BatchChunk &C = chunks[c];
     // a chunk of one request (a large request fills chunks alone) runs the
     // single-scene kernels: no per-slot scene and argument lookups
@@ -1293,7 +1324,7 @@ chk_status batch_impl(chk_ctx_s *ctx, const chk_request *R, uint32_t n,
     const chk_scene_s *ssc = single ? R[segs[C.first_seg].req].scene : nullptr;
     const TraceShape sshape{ssc ? ssc->stack_size : stack, (int)T.trace_block, B.ovf, qcap, 16};
     hipLaunchKernelGGL(k_reset, dim3((kCtrlWords + 255) / 256), dim3(256), 0, s, B.ctrl, B.stats,
-                       c == 0 ? 1 : 0);
+                       0);
     hipLaunchKernelGGL(k_queue_counts, dim3(1), dim3(64), 0, s, B.ctrl, C.total);
     for (uint32_t k = 0; k < C.n_seg; ++k) {
       const RenderArgs &a = seg_args[C.first_seg + k];
@@ -1374,19 +1405,32 @@ chk_status batch_impl(chk_ctx_s *ctx, const chk_request *R, uint32_t n,
     capture_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   }
-  // ---- run, one event after each chunk
-  std::vector<hipEvent_t> ev(chunks.size() + 1, nullptr);
+  // ---- run, one event after each chunk and one at the end
+  std::vector<hipEvent_t> ev(chunks.size() + 2, nullptr);
   for (auto &e : ev) CHK_HIP(hipEventCreate(&e));
   auto free_events = [&]() {
     for (auto &e : ev)
       if (e) (void)hipEventDestroy(e);
   };
+  // the statistics are cleared here, outside the captured chunks
+  hipLaunchKernelGGL(k_reset, dim3((kCtrlWords + 255) / 256), dim3(256), 0, st, B0.ctrl, B0.stats,
+                     1);
   hipError_t err = hipEventRecord(ev[0], st);
-  for (uint32_t c = 0; c < chunks.size() && err == hipSuccess; ++c) {
-    err = T.use_graph ? hipGraphLaunch(bs->execs[c], st) : emit_chunk(c, st);
-    if (err == hipSuccess) err = hipEventRecord(ev[c + 1], st);
+  if (err == hipSuccess && dual) {
+    err = hipEventRecord(bs->ev_fork, st);
+    if (err == hipSuccess) err = hipStreamWaitEvent(bs->st2, bs->ev_fork, 0);
   }
-  if (err == hipSuccess) err = hipEventSynchronize(ev[chunks.size()]);
+  for (uint32_t c = 0; c < chunks.size() && err == hipSuccess; ++c) {
+    const hipStream_t sc = dual && (c & 1u) ? bs->st2 : st;
+    err = T.use_graph ? hipGraphLaunch(bs->execs[c], sc) : emit_chunk(c, sc);
+    if (err == hipSuccess) err = hipEventRecord(ev[c + 1], sc);
+  }
+  if (err == hipSuccess && dual) {
+    err = hipEventRecord(bs->ev_join, bs->st2);
+    if (err == hipSuccess) err = hipStreamWaitEvent(st, bs->ev_join, 0);
+  }
+  if (err == hipSuccess) err = hipEventRecord(ev[chunks.size() + 1], st);
+  if (err == hipSuccess) err = hipEventSynchronize(ev[chunks.size() + 1]);
   if (err != hipSuccess) {
     free_events();
     set_error(std::string("chk_render_batch: ") + hipGetErrorString(err));
@@ -1400,7 +1444,7 @@ chk_status batch_impl(chk_ctx_s *ctx, const chk_request *R, uint32_t n,
     }
   }
   unsigned long long h[kStatWords];
-  err = hipMemcpy(h, B.stats, sizeof(h), hipMemcpyDeviceToHost);
+  err = hipMemcpy(h, B0.stats, sizeof(h), hipMemcpyDeviceToHost);
   if (err != hipSuccess) {
     free_events();
     set_error(std::string("chk_render_batch: statistics readback: ") + hipGetErrorString(err));
@@ -1413,7 +1457,7 @@ chk_status batch_impl(chk_ctx_s *ctx, const chk_request *R, uint32_t n,
   }
   if (stats) {
     float ms = 0.0f;
-    (void)hipEventElapsedTime(&ms, ev[0], ev[chunks.size()]);
+    (void)hipEventElapsedTime(&ms, ev[0], ev[chunks.size() + 1]);
     stats->paths = total_paths;
     stats->rays_extend = h[kRaysExt];
     stats->rays_shadow = h[kRaysShadow];

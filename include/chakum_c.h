@@ -330,8 +330,10 @@ chk_status chk_render_shade(chk_scene_t scene, const chk_camera *camera,
  * wavefront: requests are ordered by cost (W * H * spp, shortest first);
  * those below a quarter of tuning->chunk_paths paths are packed into shared
  * chunks that mix requests and scenes, larger ones get chunks of their own
- * (single-scene kernels).  Each request's image equals its chk_render image exactly
- * (same keys, same per-path arithmetic, same per-pixel summation order). */
+ * (single-scene kernels).  Chunks are launched in that order, alternating two
+ * queue sets and streams, so consecutive chunks overlap.  Each request's
+ * image equals its chk_render image exactly (same keys, same per-path
+ * arithmetic, same per-pixel summation order). */
 typedef struct chk_request {
   chk_scene_t scene;
   chk_camera camera;
