@@ -740,6 +740,12 @@ def main():
         shadow_launches_timed += st["extend_launches"] - st["chunks"]
     abi.sync()
     elapsed = time.perf_counter() - t_start
+    # after the timed steps: one frame with its chunks one after another, so
+    # each launch has the GPU to itself (the timed steps overlap consecutive
+    # chunks on two streams, and a launch then shares the GPU)
+    _, st1 = host.render(ds, dict(params, flags=abi.RENDER_PROFILE | abi.RENDER_ONE_STREAM),
+                         image=image, download=False)
+    abi.sync()
 
     paths = args.width * args.height * args.spp
     ms_per_step = elapsed * 1e3 / args.steps
@@ -788,7 +794,18 @@ def main():
         "kernel_ms_per_step": {"closest_hit": round(ext_ms / args.steps, 3),
                                "shade": round(shade_ms / args.steps, 3),
                                "any_hit": round(shadow_ms / args.steps, 3),
-                               "camera_accumulate": round(other_ms / args.steps, 3)},
+                               "camera_accumulate": round(other_ms / args.steps, 3),
+                               "note": "HIP-event spans of each phase on its chunk's stream; "
+                                       "consecutive chunks overlap on two streams, so the spans "
+                                       "add up to more than ms_per_step"},
+        "kernel_ms_one_stream": {"closest_hit": round(st1["extend_ms"], 3),
+                                 "shade": round(st1["shade_ms"], 3),
+                                 "any_hit": round(st1["shadow_ms"], 3),
+                                 "camera_accumulate": round(st1["other_ms"], 3),
+                                 "frame_ms": round(st1["extend_ms"] + st1["shade_ms"] +
+                                                   st1["shadow_ms"] + st1["other_ms"], 3),
+                                 "note": "one frame after the timed steps with its chunks one "
+                                         "after another (CHK_RENDER_ONE_STREAM)"},
         "roofline": {
             "kernel": "k_trace<closest> (chakumrender_amd/csrc/chk_trace.hip)",
             "bound": "hbm",
@@ -801,7 +818,15 @@ def main():
             "avg_launch_ms": round(avg_launch_s * 1e3, 4),
             "work_model": f"{NODE_BYTES} B/node visit + {TRI_BYTES} B/triangle test + {RAY_BYTES} B/ray",
             "what_the_frac_is": "algorithmic node/triangle/ray bytes per second against the HBM peak; "
-                                "the caches serve almost all of them (measured below)",
+                                "the caches serve almost all of them (measured below); the "
+                                "timed launches share the GPU with the other stream's chunk",
+            "one_stream": {
+                "avg_launch_ms": round(st1["extend_ms"] / st1["extend_launches"], 4),
+                "achieved": round(alg_bytes_launch / (st1["extend_ms"] / 1e3 / st1["extend_launches"])
+                                  / 1e9, 2),
+                "frac": round(alg_bytes_launch / (st1["extend_ms"] / 1e3 / st1["extend_launches"])
+                              / 1e9 / HBM_PEAK_GBS, 5),
+                "note": "the same kernel with the GPU to itself (CHK_RENDER_ONE_STREAM frame)"},
             "measured": None if not meas else {
                 "hbm_bytes_per_launch": traffic,
                 "hbm_gbs": round(traffic / avg_launch_s / 1e9, 1),

@@ -174,6 +174,7 @@ assert C.sizeof(chk_render_params) == 48
 RENDER_PHYSICAL_EMITTER = 0x1
 RENDER_SUMSQ = 0x2
 RENDER_PROFILE = 0x100
+RENDER_ONE_STREAM = 0x200
 JITTER_BOX, JITTER_REFERENCE = 0, 1
 MIS_BALANCE, MIS_LEARNED = 0, 1
 TRAIN_FREEZE_MIX, TRAIN_FREEZE_LIGHTS, TRAIN_LIGHTS_CHI2 = 1, 2, 4

@@ -615,6 +615,7 @@ chk_status chk_scene_destroy(chk_scene_t sc) {
   (void)hipSetDevice(sc->ctx->device);
   if (sc->work) (void)hipFree(sc->work);
   if (sc->work2) (void)hipFree(sc->work2);
+  if (sc->lwork2) (void)hipFree(sc->lwork2);
   if (sc->st2) (void)hipStreamDestroy(sc->st2);
   if (sc->ev_fork) (void)hipEventDestroy(sc->ev_fork);
   if (sc->ev_join) (void)hipEventDestroy(sc->ev_join);

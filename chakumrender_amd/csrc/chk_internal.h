@@ -129,8 +129,8 @@ struct chk_scene_s {
   size_t lwork_bytes = 0;
   // a second path-queue set and stream: chk_render runs every other chunk
   // there, so one chunk's last, sparse bounces overlap the next one's first
-  void *work2 = nullptr;
-  size_t work2_bytes = 0;
+  void *work2 = nullptr, *lwork2 = nullptr;
+  size_t work2_bytes = 0, lwork2_bytes = 0;
   hipStream_t st2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   void *dwork = nullptr;  // direct-lighting estimators' chunk buffers

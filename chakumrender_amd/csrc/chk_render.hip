@@ -867,7 +867,10 @@ chk_status render_impl(chk_scene_s *sc, const chk_camera *cam, const chk_render_
   // few rays and leave most of the GPU idle.  Chunks hold disjoint pixels,
   // so their accumulations never meet; the statistics are shared atomics.
   const uint64_t n_chunks_total = (n_pix_total + pix_per_chunk - 1) / pix_per_chunk;
-  const bool dual = !learned && !ws && !count && n_chunks_total > 1;
+  // (learned renders only when they do not train: training steps update the
+  // network between the bounces of one chunk)
+  const bool dual = (!learned || !training) && !ws && !count && n_chunks_total > 1 &&
+                    !(P->flags & CHK_RENDER_ONE_STREAM);
   Buffers B2 = B;
   if (dual) {
     if (sc->work2_bytes < need) {
@@ -902,6 +905,21 @@ chk_status render_impl(chk_scene_s *sc, const chk_camera *cam, const chk_render_
     }
     LB = lcarve(sc->lwork, qcap);
   }
+  LBuffers LB2 = LB;
+  if (learned && dual) {
+    const size_t lneed = lbuffers_bytes(qcap);
+    if (sc->lwork2_bytes < lneed) {
+      if (sc->lwork2) (void)hipFree(sc->lwork2);
+      sc->lwork2 = nullptr;
+      sc->lwork2_bytes = 0;
+      if (hipMalloc(&sc->lwork2, lneed) != hipSuccess) {
+        set_error("chk_render_learned: cannot allocate the second proposal-buffer set");
+        return CHK_ENOMEM;
+      }
+      sc->lwork2_bytes = lneed;
+    }
+    LB2 = lcarve(sc->lwork2, qcap);
+  }
 
   Profiler prof;
   prof.on = (P->flags & CHK_RENDER_PROFILE) != 0;
@@ -921,6 +939,7 @@ chk_status render_impl(chk_scene_s *sc, const chk_camera *cam, const chk_render_
     const bool second = dual && (chunks & 1u);
     const hipStream_t st_c = second ? sc->st2 : st_main;
     Buffers &Bc = second ? B2 : B;
+    const LBuffers &LBc = second ? LB2 : LB;
     prof.st = st_c;
     ++chunks;
     A.pix0 = (uint32_t)pix0;
@@ -952,22 +971,22 @@ chk_status render_impl(chk_scene_s *sc, const chk_camera *cam, const chk_render_
       if (learned) {
         prof.begin(4);
         hipLaunchKernelGGL(k_prop_in, dim3(kSegs * (segcap / 256)), dim3(256), 0, st_c, A, segcap,
-                           Bc.q_ray[in], Bc.hits, Bc.ctrl, qbase(in), LB.inputs[in]);
-        e = launch_mlp_eval(prop, LB.inputs[in], inR, LB.probs[in], LB.mix[in], nullptr, st_c);
+                           Bc.q_ray[in], Bc.hits, Bc.ctrl, qbase(in), LBc.inputs[in]);
+        e = launch_mlp_eval(prop, LBc.inputs[in], inR, LBc.probs[in], LBc.mix[in], nullptr, st_c);
         prof.end();
         if (e != hipSuccess) {
           set_error(std::string("chk_render_learned: proposal launch: ") + hipGetErrorString(e));
           return CHK_EDEVICE;
         }
-        io.probs = LB.probs[in];
-        io.mix = LB.mix[in];
-        io.probs_prev = LB.probs[out];
-        io.mix_prev = LB.mix[out];
-        io.in_prev = LB.prev[in];
-        io.out_prev = LB.prev[out];
+        io.probs = LBc.probs[in];
+        io.mix = LBc.mix[in];
+        io.probs_prev = LBc.probs[out];
+        io.mix_prev = LBc.mix[out];
+        io.in_prev = LBc.prev[in];
+        io.out_prev = LBc.prev[out];
         // records only feed the training steps of a training pass
-        io.rec = training ? LB.rec[in] : nullptr;
-        io.rec_prev = training ? LB.rec[out] : nullptr;
+        io.rec = training ? LBc.rec[in] : nullptr;
+        io.rec_prev = training ? LBc.rec[out] : nullptr;
       }
       prof.begin(1);
       if (learned)
@@ -988,8 +1007,8 @@ chk_status render_impl(chk_scene_s *sc, const chk_camera *cam, const chk_render_
         const uint32_t stride = std::max(1u, cap / train->batch);
         for (uint32_t t = 0; t < train->steps_per_bounce; ++t) {
           chk_status ts = proposal_train_step(
-              prop, SegRange{Bc.ctrl + kQP, 0u, segcap}, LB.rec[out], LB.inputs[out],
-              LB.probs[out], LB.mix[out], LB.s_occ, train->batch, stride,
+              prop, SegRange{Bc.ctrl + kQP, 0u, segcap}, LBc.rec[out], LBc.inputs[out],
+              LBc.probs[out], LBc.mix[out], LBc.s_occ, train->batch, stride,
               (uint32_t)(((uint64_t)t * stride) / train->steps_per_bounce), train->lr,
               train->flags, st_c);
           if (ts != CHK_OK) return ts;
@@ -1001,7 +1020,7 @@ chk_status render_impl(chk_scene_s *sc, const chk_camera *cam, const chk_render_
         prof.begin(2);
         e = launch_any(sc, TraceShape{sc->stack_size, 128, Bc.ovf, qcap, sc->shallow_stack},
                        Bc.s_ray,
-                       SegRange{Bc.ctrl + kQS, 0u, segcap}, training ? LB.s_occ : nullptr, Bc.s_L,
+                       SegRange{Bc.ctrl + kQS, 0u, segcap}, training ? LBc.s_occ : nullptr, Bc.s_L,
                        Bc.path_L, true, count, Bc.stats + kShNodeVisits, SlotScenes{}, st_c);
         prof.end();
         if (e != hipSuccess) {

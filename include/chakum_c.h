@@ -95,6 +95,11 @@ typedef struct chk_render_stats {
 
 /* Time every kernel launch with HIP events into chk_render_stats. */
 #define CHK_RENDER_PROFILE 0x100u
+/* Run a render's chunks one after another on the caller's stream.  By default
+ * a plain render (or a learned one that does not train) of several chunks
+ * alternates two queue sets and streams, so that consecutive chunks overlap;
+ * this flag gives each kernel launch the GPU to itself, for timing. */
+#define CHK_RENDER_ONE_STREAM 0x200u
 
 const char *chk_status_string(chk_status s);
 /* Detail message of the last failing call on this thread. */

@@ -99,6 +99,7 @@ def test_ctypes_layouts_equal_the_c_compilers(tmp_path):
 @pytest.mark.parametrize("flag,val", [("CHK_RENDER_PHYSICAL_EMITTER", abi.RENDER_PHYSICAL_EMITTER),
                                       ("CHK_RENDER_SUMSQ", abi.RENDER_SUMSQ),
                                       ("CHK_RENDER_PROFILE", abi.RENDER_PROFILE),
+                                      ("CHK_RENDER_ONE_STREAM", abi.RENDER_ONE_STREAM),
                                       ("CHK_DIRECT_VISIBILITY", abi.DIRECT_VISIBILITY),
                                       ("CHK_MIS_LIGHT_SAMPLES", abi.MIS_LIGHT_SAMPLES),
                                       ("CHK_MIS_BSDF_SAMPLES", abi.MIS_BSDF_SAMPLES)])

@@ -216,7 +216,8 @@ def test_render_window_equals_whole_image(ctx, win):
 def test_multi_chunk_render_equals_its_windows(ctx):
     """A render of three chunks of 2^24 paths (its chunks alternate two
     queue sets and streams) equals, bit for bit, the same image rendered as
-    three row windows of one chunk each, and is deterministic."""
+    three row windows of one chunk each and with the chunks on one stream,
+    and is deterministic."""
     W, H, spp = 512, 1536, 64  # 50,331,648 paths
     sc = scenes.cornell(width=W, height=H, sphere_tris=2000, n_lights=3, light_rng=True)
     ds = abi.DeviceScene(ctx, sc)
@@ -225,6 +226,8 @@ def test_multi_chunk_render_equals_its_windows(ctx):
     assert st["chunks"] == 3
     again, _, _ = host.render(ds, P)
     np.testing.assert_array_equal(again, full)
+    one, _, st2 = host.render(ds, dict(P, flags=P["flags"] | abi.RENDER_ONE_STREAM))
+    np.testing.assert_array_equal(one, full)
     img = host.Image(W, H, sumsq=True)
     rows = (1 << 24) // (W * spp)
     for y0 in range(0, H, rows):
