@@ -1078,12 +1078,15 @@ chk_status render_impl(chk_scene_s *sc, const chk_camera *cam, const chk_render_
 // carries its segment index, so the traversal kernels find the ray's scene
 // and the shading kernel the request's arguments.  A larger request gets
 // chunks of its own and runs the single-scene kernels, as chk_render would.
-// Per chunk the sequence is the single-request one; with
-// use_graph each chunk's launches are captured into a hipGraph once and
-// replayed; the executable graphs are kept until a call with other
-// requests captures again or the buffers they name are reallocated.  Eager
-// calls in between leave them valid: every call re-uploads its own tables
-// before it launches anything.
+// Per chunk the sequence is the single-request one.  With use_graph the
+// whole batch -- every chunk on its alternating stream, the fork and join
+// between the two streams, the statistics reset and the per-chunk time
+// stamps -- is captured into one hipGraph, and a call is one hipGraphLaunch;
+// the executable graph is kept until a call with other requests captures
+// again or the buffers it names are reallocated.  Eager calls in between
+// leave it valid: every call re-uploads its own tables before it launches
+// anything.  Per-request latencies come from device time stamps written
+// after each chunk (the same kernels in both modes).
 
 namespace {
 
@@ -1101,17 +1104,24 @@ struct BatchState {
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   void *tab = nullptr;
   size_t tab_bytes = 0;
-  std::vector<unsigned char> key;  // request records + tuning of the cached graphs
-  std::vector<hipGraphExec_t> execs;
-  hipStream_t cap = nullptr;
+  std::vector<unsigned char> key;  // request records + tuning of the cached graph
+  hipGraphExec_t exec = nullptr;   // the whole batch, captured
+  hipStream_t cap = nullptr, cap2 = nullptr;  // the capture's two streams
+  hipEvent_t cev_fork = nullptr, cev_join = nullptr;
+  hipEvent_t ev_start = nullptr, ev_end = nullptr;
   void drop_graphs() {
-    for (hipGraphExec_t e : execs) (void)hipGraphExecDestroy(e);
-    execs.clear();
+    if (exec) (void)hipGraphExecDestroy(exec);
+    exec = nullptr;
     key.clear();
   }
   ~BatchState() {
     drop_graphs();
     if (cap) (void)hipStreamDestroy(cap);
+    if (cap2) (void)hipStreamDestroy(cap2);
+    if (cev_fork) (void)hipEventDestroy(cev_fork);
+    if (cev_join) (void)hipEventDestroy(cev_join);
+    if (ev_start) (void)hipEventDestroy(ev_start);
+    if (ev_end) (void)hipEventDestroy(ev_end);
     if (st2) (void)hipStreamDestroy(st2);
     if (ev_fork) (void)hipEventDestroy(ev_fork);
     if (ev_join) (void)hipEventDestroy(ev_join);
@@ -1151,6 +1161,13 @@ void launch_shade_batch(uint32_t grid, hipStream_t st, uint32_t depth, uint32_t 
                        segcap, B.q_ray[in], B.q_thr[in], B.q_pid[in], B.hits, B.q_ray[out],
                        B.q_thr[out], B.q_pid[out], B.s_ray, B.s_L, B.path_L, B.ctrl, qin, qout,
                        LearnedIO{}, Q);
+}
+
+// The device's constant-rate clock after a chunk's last kernel (per-request
+// latency inside one graph launch, where no host event can be recorded
+// between chunks).
+__global__ void k_stamp(unsigned long long *p) {
+  if (threadIdx.x == 0) *p = wall_clock64();
 }
 
 }  // namespace
@@ -1288,11 +1305,12 @@ chk_status batch_impl(chk_ctx_s *ctx, const chk_request *R, uint32_t n,
     if (!bs->ev_fork) CHK_HIP(hipEventCreateWithFlags(&bs->ev_fork, hipEventDisableTiming));
     if (!bs->ev_join) CHK_HIP(hipEventCreateWithFlags(&bs->ev_join, hipEventDisableTiming));
   }
-  const size_t tab_need = segs.size() * (sizeof(RenderArgs) + sizeof(DeviceScene)) + 512;
+  const size_t tab_need =
+      segs.size() * (sizeof(RenderArgs) + sizeof(DeviceScene)) + (chunks.size() + 2) * 8 + 768;
   cs = ensure(bs->tab, bs->tab_bytes, tab_need, "the request tables", moved);
   if (cs != CHK_OK) return cs;
   if (moved) bs->drop_graphs();
-  const bool cached = T.use_graph && bs->key == key && !bs->execs.empty();
+  const bool cached = T.use_graph && bs->key == key && bs->exec != nullptr;
   if (T.use_graph && !cached) bs->drop_graphs();
   // queue set k: path queues, counters and the slots' segment indices
   struct QSet {
@@ -1323,6 +1341,10 @@ chk_status batch_impl(chk_ctx_s *ctx, const chk_request *R, uint32_t n,
   RenderArgs *d_args = (RenderArgs *)bs->tab;
   DeviceScene *d_scene =
       (DeviceScene *)((char *)bs->tab + ((segs.size() * sizeof(RenderArgs) + 255) & ~(size_t)255));
+  // time stamps: [0] at the start, [c + 1] after chunk c
+  unsigned long long *d_stamp =
+      (unsigned long long *)((char *)d_scene +
+                             ((segs.size() * sizeof(DeviceScene) + 255) & ~(size_t)255));
   CHK_HIP(hipStreamSynchronize(st));
   CHK_HIP(hipMemcpy(d_args, seg_args.data(), segs.size() * sizeof(RenderArgs),
                     hipMemcpyHostToDevice));
@@ -1393,90 +1415,105 @@ chk_status batch_impl(chk_ctx_s *ctx, const chk_request *R, uint32_t n,
     }
     return hipGetLastError();
   };
-  // ---- capture (graph mode, not cached)
+  // ---- the whole batch on streams a (chunks 0, 2, ...) and b (1, 3, ...):
+  // statistics reset (outside the chunks, which clear only their queues),
+  // fork, chunks with a time stamp after each, join
+  auto emit_all = [&](hipStream_t a, hipStream_t b, hipEvent_t fork,
+                      hipEvent_t join) -> hipError_t {
+    hipLaunchKernelGGL(k_reset, dim3((kCtrlWords + 255) / 256), dim3(256), 0, a, B0.ctrl,
+                       B0.stats, 1);
+    hipLaunchKernelGGL(k_stamp, dim3(1), dim3(64), 0, a, d_stamp);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && dual) {
+      e = hipEventRecord(fork, a);
+      if (e == hipSuccess) e = hipStreamWaitEvent(b, fork, 0);
+    }
+    for (uint32_t c = 0; c < chunks.size() && e == hipSuccess; ++c) {
+      const hipStream_t s = dual && (c & 1u) ? b : a;
+      e = emit_chunk(c, s);
+      if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_stamp, dim3(1), dim3(64), 0, s, d_stamp + c + 1);
+        e = hipGetLastError();
+      }
+    }
+    if (e == hipSuccess && dual) {
+      e = hipEventRecord(join, b);
+      if (e == hipSuccess) e = hipStreamWaitEvent(a, join, 0);
+    }
+    return e;
+  };
+  if (!bs->ev_start) CHK_HIP(hipEventCreate(&bs->ev_start));
+  if (!bs->ev_end) CHK_HIP(hipEventCreate(&bs->ev_end));
+  // ---- capture (graph mode, not cached): one graph for the whole batch
   double capture_ms = 0.0;
   if (T.use_graph && !cached) {
     if (!bs->cap) CHK_HIP(hipStreamCreateWithFlags(&bs->cap, hipStreamNonBlocking));
+    if (!bs->cap2) CHK_HIP(hipStreamCreateWithFlags(&bs->cap2, hipStreamNonBlocking));
+    if (!bs->cev_fork) CHK_HIP(hipEventCreateWithFlags(&bs->cev_fork, hipEventDisableTiming));
+    if (!bs->cev_join) CHK_HIP(hipEventCreateWithFlags(&bs->cev_join, hipEventDisableTiming));
     const auto t0 = std::chrono::steady_clock::now();
-    for (uint32_t c = 0; c < chunks.size(); ++c) {
-      CHK_HIP(hipStreamBeginCapture(bs->cap, hipStreamCaptureModeThreadLocal));
-      const hipError_t e = emit_chunk(c, bs->cap);
-      hipGraph_t g = nullptr;
-      const hipError_t e2 = hipStreamEndCapture(bs->cap, &g);
-      if (e != hipSuccess || e2 != hipSuccess) {
-        if (g) (void)hipGraphDestroy(g);
-        bs->drop_graphs();
-        set_error(std::string("chk_render_batch: graph capture: ") +
-                  hipGetErrorString(e != hipSuccess ? e : e2));
-        return CHK_EDEVICE;
-      }
-      hipGraphExec_t x = nullptr;
-      const hipError_t e3 = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(g);
-      if (e3 != hipSuccess) {
-        bs->drop_graphs();
-        set_error(std::string("chk_render_batch: graph instantiate: ") + hipGetErrorString(e3));
-        return CHK_EDEVICE;
-      }
-      bs->execs.push_back(x);
+    CHK_HIP(hipStreamBeginCapture(bs->cap, hipStreamCaptureModeThreadLocal));
+    const hipError_t e = emit_all(bs->cap, bs->cap2, bs->cev_fork, bs->cev_join);
+    hipGraph_t g = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(bs->cap, &g);
+    if (e != hipSuccess || e2 != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      bs->drop_graphs();
+      set_error(std::string("chk_render_batch: graph capture: ") +
+                hipGetErrorString(e != hipSuccess ? e : e2));
+      return CHK_EDEVICE;
     }
+    hipGraphExec_t x = nullptr;
+    const hipError_t e3 = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e3 != hipSuccess) {
+      bs->drop_graphs();
+      set_error(std::string("chk_render_batch: graph instantiate: ") + hipGetErrorString(e3));
+      return CHK_EDEVICE;
+    }
+    bs->exec = x;
     bs->key = key;
     capture_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   }
-  // ---- run, one event after each chunk and one at the end
-  std::vector<hipEvent_t> ev(chunks.size() + 2, nullptr);
-  for (auto &e : ev) CHK_HIP(hipEventCreate(&e));
-  auto free_events = [&]() {
-    for (auto &e : ev)
-      if (e) (void)hipEventDestroy(e);
-  };
-  // the statistics are cleared here, outside the captured chunks
-  hipLaunchKernelGGL(k_reset, dim3((kCtrlWords + 255) / 256), dim3(256), 0, st, B0.ctrl, B0.stats,
-                     1);
-  hipError_t err = hipEventRecord(ev[0], st);
-  if (err == hipSuccess && dual) {
-    err = hipEventRecord(bs->ev_fork, st);
-    if (err == hipSuccess) err = hipStreamWaitEvent(bs->st2, bs->ev_fork, 0);
-  }
-  for (uint32_t c = 0; c < chunks.size() && err == hipSuccess; ++c) {
-    const hipStream_t sc = dual && (c & 1u) ? bs->st2 : st;
-    err = T.use_graph ? hipGraphLaunch(bs->execs[c], sc) : emit_chunk(c, sc);
-    if (err == hipSuccess) err = hipEventRecord(ev[c + 1], sc);
-  }
-  if (err == hipSuccess && dual) {
-    err = hipEventRecord(bs->ev_join, bs->st2);
-    if (err == hipSuccess) err = hipStreamWaitEvent(st, bs->ev_join, 0);
-  }
-  if (err == hipSuccess) err = hipEventRecord(ev[chunks.size() + 1], st);
-  if (err == hipSuccess) err = hipEventSynchronize(ev[chunks.size() + 1]);
+  // ---- run: one graph launch, or the same launches eagerly
+  hipError_t err = hipEventRecord(bs->ev_start, st);
+  if (err == hipSuccess)
+    err = T.use_graph ? hipGraphLaunch(bs->exec, st)
+                      : emit_all(st, bs->st2, bs->ev_fork, bs->ev_join);
+  if (err == hipSuccess) err = hipEventRecord(bs->ev_end, st);
+  if (err == hipSuccess) err = hipEventSynchronize(bs->ev_end);
   if (err != hipSuccess) {
-    free_events();
     set_error(std::string("chk_render_batch: ") + hipGetErrorString(err));
     return CHK_EDEVICE;
   }
   if (latency_ms) {
-    for (uint32_t i = 0; i < n; ++i) {
-      float ms = 0.0f;
-      (void)hipEventElapsedTime(&ms, ev[0], ev[last_chunk[i] + 1]);
-      latency_ms[i] = ms;
+    std::vector<unsigned long long> stamp(chunks.size() + 1);
+    err = hipMemcpy(stamp.data(), d_stamp, stamp.size() * 8, hipMemcpyDeviceToHost);
+    int khz = 0;
+    if (err == hipSuccess) err = hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate,
+                                                       ctx->device);
+    if (err != hipSuccess || khz <= 0) {
+      set_error(std::string("chk_render_batch: time stamps: ") +
+                (err != hipSuccess ? hipGetErrorString(err) : "no wall clock rate"));
+      return CHK_EDEVICE;
     }
+    for (uint32_t i = 0; i < n; ++i)
+      latency_ms[i] = (double)(stamp[last_chunk[i] + 1] - stamp[0]) / (double)khz;
   }
   unsigned long long h[kStatWords];
   err = hipMemcpy(h, B0.stats, sizeof(h), hipMemcpyDeviceToHost);
   if (err != hipSuccess) {
-    free_events();
     set_error(std::string("chk_render_batch: statistics readback: ") + hipGetErrorString(err));
     return CHK_EDEVICE;
   }
   if (h[kOverflow] || h[kShOverflow]) {
-    free_events();
     set_error("chk_render_batch: traversal stack overflow");
     return CHK_EINTERNAL;
   }
   if (stats) {
     float ms = 0.0f;
-    (void)hipEventElapsedTime(&ms, ev[0], ev[chunks.size() + 1]);
+    (void)hipEventElapsedTime(&ms, bs->ev_start, bs->ev_end);
     stats->paths = total_paths;
     stats->rays_extend = h[kRaysExt];
     stats->rays_shadow = h[kRaysShadow];
@@ -1486,7 +1523,6 @@ chk_status batch_impl(chk_ctx_s *ctx, const chk_request *R, uint32_t n,
     stats->capture_ms = capture_ms;
     stats->graph_cached = cached ? 1u : 0u;
   }
-  free_events();
   return CHK_OK;
 }
 
@@ -1494,7 +1530,9 @@ chk_status batch_impl(chk_ctx_s *ctx, const chk_request *R, uint32_t n,
 // itself (the chunk size that suits a request mix depends on the mix at
 // its real spp), rendering into scratch accumulators, then eager against
 // graph replay for the fastest.  The first call of each chunk size grows
-// the buffers; each candidate is timed on its second call.
+// the buffers; each candidate is timed on its second call.  The graph is
+// kept unless it is more than 1 % slower than eager launches: a call is
+// then one hipGraphLaunch, and the host thread issues nothing else.
 chk_status batch_autotune(chk_ctx_s *ctx, const chk_request *R, uint32_t n,
                           chk_batch_tuning *best, double *best_ms, hipStream_t st) {
   if (n == 0) {
@@ -1537,7 +1575,7 @@ chk_status batch_autotune(chk_ctx_s *ctx, const chk_request *R, uint32_t n,
     g.use_graph = 1;
     chk_batch_stats bst;
     for (int rep = 0; rep < 3 && s == CHK_OK; ++rep) s = batch_impl(ctx, Q.data(), n, &g, nullptr, &bst, st);
-    if (s == CHK_OK && bst.batch_ms < bms) {
+    if (s == CHK_OK && bst.batch_ms <= 1.01 * bms) {
       bms = bst.batch_ms;
       bt = g;
     }

@@ -351,8 +351,10 @@ typedef struct chk_batch_tuning {
   uint32_t chunk_paths;     /* paths in flight per chunk, 2^16..2^26 (0: 2^24) */
   uint32_t trace_block;     /* traversal threads per block: 64, 128, 256 (0: 128) */
   uint32_t shade_block;     /* shading threads per block: 128, 256, 512 (0: 256) */
-  uint32_t use_graph;       /* capture each chunk into a hipGraph; the executable
-                               graphs are reused by the next identical call */
+  uint32_t use_graph;       /* capture the whole batch (both streams) into one
+                               hipGraph: a call is one hipGraphLaunch; the
+                               executable graph is reused by the next identical
+                               call */
 } chk_batch_tuning;
 
 typedef struct chk_batch_stats {
@@ -369,14 +371,16 @@ typedef struct chk_batch_stats {
 
 /* Render n requests.  tuning NULL: defaults.  latency_ms (host, n, may be
  * NULL): per request, device time from the batch start to the end of the
- * chunk that finished it.  Synchronises the stream. */
+ * chunk that finished it (device clock stamps written after each chunk, in
+ * both modes).  Synchronises the stream. */
 chk_status chk_render_batch(chk_ctx_t ctx, const chk_request *reqs, uint32_t n,
                             const chk_batch_tuning *tuning, double *latency_ms,
                             chk_batch_stats *stats, void *stream);
 /* Autotuner: times every chunk_paths (2^22 .. 2^25) x trace_block (64, 128)
  * candidate on the batch (scratch accumulators; the requests' own images are
- * not touched), then eager launches against graph replay for the fastest.
- * best_ms (may be NULL): its batch time. */
+ * not touched), then eager launches against graph replay for the fastest;
+ * the graph is chosen unless it is more than 1 % slower.  best_ms (may be
+ * NULL): its batch time. */
 chk_status chk_batch_autotune(chk_ctx_t ctx, const chk_request *reqs, uint32_t n,
                               chk_batch_tuning *best, double *best_ms, void *stream);
 

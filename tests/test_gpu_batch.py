@@ -65,9 +65,13 @@ def test_graph_replays_match_eager(ctx, scene_set):
     eager = host.Batch(ctx, items)
     graph = host.Batch(ctx, items)
     for rep in range(3):
-        _, se = eager.run(tun)
-        _, sg = graph.run(dict(tun, use_graph=1))
+        le, se = eager.run(tun)
+        lg, sg = graph.run(dict(tun, use_graph=1))
         assert sg["graph_cached"] == (1 if rep > 0 else 0)
+        assert sg["chunks"] > 2  # the one graph holds both streams' chunks
+        # latencies come from device clock stamps inside the graph
+        for lat, s in ((le, se), (lg, sg)):
+            assert min(lat) > 0 and max(lat) <= s["batch_ms"] + 1e-3, (lat, s["batch_ms"])
         # the program's own eager work between replays
         host.render(scene_set[1], dict(spp=2, max_depth=3, seed=99))
         for k, ((a, s), (b, t)) in enumerate(zip(eager.get(), graph.get())):
