@@ -222,11 +222,12 @@ __device__ inline void finish_ray(const DeviceScene &S, size_t i, float4 ot, flo
 }
 
 // One ray per thread: block b traces rays q * BS ... of segment
-// r = b % kSegs, q = b / kSegs; the grid covers the segments' capacity and
-// blocks past a segment's live count exit at once.  (A persistent grid whose
-// waves fetched batches from a per-segment cursor measured no faster once
-// chunks hold 2^24 paths: the idle blocks of a launch cost less than the
-// fetch loop.)
+// r = b % kSegs, q = b / kSegs, then q + bps, q + 2 bps, ... (bps blocks
+// per segment); blocks past a segment's live count exit at once.  (A
+// persistent grid whose waves fetched batches from a per-segment cursor
+// measured no faster once chunks hold 2^24 paths: the idle blocks of a
+// launch cost less than the fetch loop.  Capping bps at 2^19 threads per
+// segment against covering the whole capacity: within 1 %.)
 template <bool ANY, bool COUNT, int STACK, int BS, bool MULTI>
 __global__ __launch_bounds__(BS) void k_trace(DeviceScene S0, const float4 *__restrict__ rays,
                                               SegRange W, float4 *__restrict__ hits,
@@ -247,15 +248,18 @@ __global__ __launch_bounds__(BS) void k_trace(DeviceScene S0, const float4 *__re
     for (uint32_t k = 0; k < kSegs; ++k) tot += W.count_of(k);
     atomicAdd(&counters[3], tot);  // rays traced by this launch
   }
-  const uint32_t off = q * BS + threadIdx.x;
   const uint32_t cnt = W.count_of(r);
   if (q * BS >= cnt) return;  // whole block idle (uniform)
-  uint32_t nv = 0, nt = 0, overflow = 0;
-  if (off < cnt) {
+  // a grid of bps blocks per segment walks the segment's live rays
+  uint32_t nv = 0, nt = 0, lost = 0;  // lost: overflowing rays the list had no room for
+  const uint32_t nq = gridDim.x / kSegs;  // blocks per segment
+  for (uint32_t qq = q; qq * BS < cnt; qq += nq) {  // block-uniform
+    const uint32_t off = qq * BS + threadIdx.x;
+    if (off >= cnt) continue;
     const size_t i = (size_t)r * W.segcap + off;
     const float4 ot = rays[2 * i], ds = rays[2 * i + 1];
     float t, u, v;
-    uint32_t id, leaf;
+    uint32_t id, leaf, overflow = 0;
     bool any;
     trace_ray<ANY, COUNT, STACK, BS, MULTI>(S0, M, stk, i, ot, ds, leaf_ids, t, id, u, v, any, leaf,
                                             nv, nt, overflow);
@@ -270,13 +274,14 @@ __global__ __launch_bounds__(BS) void k_trace(DeviceScene S0, const float4 *__re
         later = true;
       }
     }
+    lost += overflow;
     if (!later)
       finish_ray<ANY, MULTI>(S0, i, ot, ds, t, id, u, v, any, leaf, hits, occluded, shadow_L,
                              path_L, leaf_ids);
   }
   // wave-uniform condition: every lane of the wave takes part in the shuffles
-  if (COUNT || __any(overflow != 0)) {
-    unsigned long long a = nv, b = nt, c = overflow;
+  if (COUNT || __any(lost != 0)) {
+    unsigned long long a = nv, b = nt, c = lost;
     for (int o = 32; o > 0; o >>= 1) {
       a += __shfl_xor(a, o);
       b += __shfl_xor(b, o);
@@ -450,7 +455,11 @@ hipError_t launch_t(const chk_scene_s *s, const TraceShape &T, const float4 *ray
                     const SegRange &W, float4 *hits, uint8_t *occ, const float4 *sL, float4 *pL,
                     bool leaf_ids, unsigned long long *counters, const SlotScenes &M,
                     hipStream_t st, uint32_t *ovf, uint32_t ovf_cap) {
-  const uint32_t grid = kSegs * (W.segcap / BS);
+  // at most 2^19 threads per segment (4,096 blocks of 128; about 16 blocks
+  // per CU of the segment's XCD, 8 times over): a sparse bounce dispatches
+  // no more blocks than that, a full one loops
+  const uint32_t bps = std::min(W.segcap / BS, (1u << 19) / BS);
+  const uint32_t grid = kSegs * bps;
   if (grid == 0) return hipSuccess;
   hipLaunchKernelGGL((k_trace<ANY, COUNT, STACK, BS, MULTI>), dim3(grid), dim3(BS), 0, st,
                      s ? s->d : DeviceScene{}, rays, W, hits, occ, sL, pL, leaf_ids ? 1 : 0,
