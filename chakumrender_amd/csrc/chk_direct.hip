@@ -58,6 +58,19 @@ __device__ inline d3 f4d(float4 v) { return dk(v.x, v.y, v.z); }
 // Triangle::intersect (geometryPrimitives.cpp:29-71) in FP64 with the
 // triangle's constants precomputed (LightTri64): the reference's operations
 // without recomputing the normal and the edge dot products per call.
+// A miss is decided without the divisions whose quotient only the miss
+// tests read, where the sign or size of the numerator already decides them
+// (the results are those of the reference's divide-then-compare):
+//  * t = tn / rn with rn < 0: t < 0 exactly when tn > 0 (|rn| <= 1 for a
+//    unit direction and unit normal, so a positive tn never underflows to
+//    -0 in the quotient);
+//  * gamma = gn / den with den < 0 (b1^2 - c1 b0, Cauchy-Schwarz): gamma < 0
+//    exactly when gn > 0, and gamma > 1 when gn < den (1 + 2^-40) (the
+//    quotient then exceeds 1 + 2^-41, which rounds above 1);
+//  * beta = bn / b0 with b0 > 0: beta < 0 exactly when bn < 0, beta > 1 when
+//    bn > b0 (1 + 2^-40).
+// Numerators within 1e-280 of zero take the divisions (no underflow
+// question), as do degenerate triangles (den >= 0).
 __device__ inline double tri_isect64p(const LightTri64 &T, d3 o, d3 d, d3 *N) {
   const d3 n = dk(T.n[0], T.n[1], T.n[2]);
   double rn = dot(d, n);
@@ -69,13 +82,21 @@ __device__ inline double tri_isect64p(const LightTri64 &T, d3 o, d3 d, d3 *N) {
   *N = Nf;
   if (rn * rn <= 1e-10) return kInf64;
   const d3 A = dk(T.a[0], T.a[1], T.a[2]);
-  const double t = dot(A - o, Nf) / rn;
+  const double tn = dot(A - o, Nf);
+  if (tn > 1e-280) return kInf64;  // t < 0
+  const double t = tn / rn;
   if (t < 0) return kInf64;
   const d3 v0 = (o + d * t) - A;
   const double k0 = dot(v0, dk(T.v1[0], T.v1[1], T.v1[2]));
   const double k1 = dot(v0, dk(T.v2[0], T.v2[1], T.v2[2]));
-  const double gamma = (k0 * T.b1 - k1 * T.b0) / T.den;
-  const double beta = (k0 - gamma * T.b1) / T.b0;  // c0 = v2.v1 = b1
+  const double gn = k0 * T.b1 - k1 * T.b0;
+  const bool fast = T.den < 0;
+  constexpr double kOver = 1.0 + 0x1p-40;
+  if (fast && (gn > 1e-280 || gn < T.den * kOver)) return kInf64;  // gamma < 0 or > 1
+  const double gamma = gn / T.den;
+  const double bn = k0 - gamma * T.b1;  // c0 = v2.v1 = b1
+  if (fast && (bn < -1e-280 || bn > T.b0 * kOver)) return kInf64;  // beta < 0 or > 1
+  const double beta = bn / T.b0;
   if (gamma + beta > 1) return kInf64;
   if (0 > gamma || gamma > 1) return kInf64;
   if (0 > beta || beta > 1) return kInf64;
@@ -273,12 +294,15 @@ __device__ inline bool prim_info(const DeviceScene &S, uint32_t o, uint32_t *mat
 
 // BasePrimitive::brdf with the LTC approximation when it is set
 // (materialTypes.cpp:9-11,76-87; geometryPrimitives.cpp:7-9).
+__device__ inline double brdf_apx_f(const MatT<double> &m, const PointRec &P, d3 wi, double f);
 __device__ inline double brdf_apx(const MatT<double> &m, const PointRec &P, d3 wi) {
-  if (!(P.flags & kApx)) {
-    double f, pdf;
-    brdf_pdf(m, P.n, P.wo_ref, P.wo, wi, f, pdf);
-    return f;
-  }
+  double f = 0.0, pdf;
+  if (!(P.flags & kApx)) brdf_pdf(m, P.n, P.wo_ref, P.wo, wi, f, pdf);
+  return brdf_apx_f(m, P, wi, f);
+}
+// The same with brdf(wi) given (returned when no approximation is set).
+__device__ inline double brdf_apx_f(const MatT<double> &m, const PointRec &P, d3 wi, double f) {
+  if (!(P.flags & kApx)) return f;
   const d3 Lo = normalize(dk(dot(P.R0, wi), dot(P.R1, wi), dot(P.R2, wi)));
   const d3 L_ = dk(P.m[0] * Lo.x + P.m[1] * Lo.z, P.m[2] * Lo.y, P.m[3] * Lo.x + P.m[4] * Lo.z);
   const double l = sqrt(dot(L_, L_));
@@ -412,8 +436,18 @@ __device__ inline void no_ray(float4 *__restrict__ ray, size_t i) {
 // the BSDF (60).  EXT = false: a scene with mesh lights only (the other
 // counts are 0), compiled without their code.
 template <bool EXT>
+__device__ inline double mis_weight_bp(const DeviceScene &S, d3 x, d3 s, double bsdf_pdf);
+template <bool EXT>
 __device__ inline double mis_weight(const DeviceScene &S, const MatT<double> &m, d3 x, d3 n,
                                     d3 wo_ref, d3 wo, d3 s) {
+  double f, pdf;
+  brdf_pdf(m, n, wo_ref, wo, s, f, pdf);
+  return mis_weight_bp<EXT>(S, x, s, pdf);
+}
+// The same with the BSDF's pdfEval of s given (the callers that evaluate
+// the BSDF at s anyway).
+template <bool EXT>
+__device__ inline double mis_weight_bp(const DeviceScene &S, d3 x, d3 s, double bsdf_pdf) {
   double denom = 0;
   if (EXT)
     for (uint32_t j = 0; j < S.n_sph_src; ++j) {  // SoAnISSP (mis.cpp:35-39)
@@ -436,9 +470,7 @@ __device__ inline double mis_weight(const DeviceScene &S, const MatT<double> &m,
   if (EXT && S.env.on)  // EnMpIS (mis.cpp:84-87): grey(L) / normalizationArea
     denom += (double)CHK_MIS_LIGHT_SAMPLES *
              (grey(env_pixel_radiance(S.env, env_pixel(S.env, s.x, s.y, s.z))) * (1.0 / S.env.area));
-  double f, pdf;
-  brdf_pdf(m, n, wo_ref, wo, s, f, pdf);
-  denom += (double)CHK_MIS_BSDF_SAMPLES * pdf;
+  denom += (double)CHK_MIS_BSDF_SAMPLES * bsdf_pdf;
   return 1.0 / (denom + 1e-20);
 }
 
@@ -520,7 +552,7 @@ __global__ __launch_bounds__(kDBlock) void k_dl_samples(DeviceScene S, const Poi
     double fa, pdf;
     brdf_pdf(m, R.n, R.wo_ref, R.wo, w, fa, pdf);
     // (visibility * brdfActual - brdfApproximate) * cosine * cosine1 / (d * d)
-    const double fx = brdf_apx(m, R, w);
+    const double fx = brdf_apx_f(m, R, w, fa);
     I.a[i] = (fa - fx) * cs * c1 / (d * d);
     I.b[i] = (0.0 * fa - fx) * cs * c1 / (d * d);
     if (EXT && S.n_spheres && sph_occluded(S, o, w, d)) {
@@ -638,15 +670,15 @@ __global__ __launch_bounds__(kDBlock) void k_dl_samples(DeviceScene S, const Poi
     no_ray(I.ray, i);
     return;
   }
-  const double wt = mis_weight<EXT>(S, m, R.x, R.n, R.wo_ref, R.wo, w);
   double fa, pdf;
   brdf_pdf(m, R.n, R.wo_ref, R.wo, w, fa, pdf);
+  const double wt = mis_weight_bp<EXT>(S, R.x, w, pdf);
   if (EST == CHK_DIRECT_MIS) {
     I.a[i] = fa * cs * wt;  // throughput
     I.b[i] = 0.0;
   } else {
     // (visibility * brdfActual - brdfApproximate) * cosine * weights[i]
-    const double fx = brdf_apx(m, R, w);
+    const double fx = brdf_apx_f(m, R, w, fa);
     I.a[i] = (fa - fx) * cs * wt;
     I.b[i] = (0.0 * fa - fx) * cs * wt;
   }
@@ -670,9 +702,12 @@ __global__ __launch_bounds__(kDBlock) void k_dl_samples(DeviceScene S, const Poi
 // in order, each term formed and added as the reference forms and adds it,
 // then G.  No item buffers and no shadow rays; every lane of a wave works on
 // the same slot, so the light / BSDF branches stay wave-uniform.  out as
-// k_dl_reduce.
+// k_dl_reduce.  Bounded to 3 waves per SIMD (168 VGPRs; 33 registers of
+// loop-invariant point data spill to scratch): 19.2 ms for the 1080p
+// shadeDirectOnly frame against 22.7 at 2 waves (205 VGPRs) and 19.0 at 4
+// (284 bytes of spill).
 template <bool EXT>
-__global__ __launch_bounds__(kDBlock) void k_dl_miscv_fused(DeviceScene S,
+__global__ __launch_bounds__(kDBlock, 3) void k_dl_miscv_fused(DeviceScene S,
                                                             const PointRec *__restrict__ pts,
                                                             uint32_t P, uint32_t n_slots,
                                                             uint32_t key0, uint32_t key1,
@@ -699,10 +734,10 @@ __global__ __launch_bounds__(kDBlock) void k_dl_miscv_fused(DeviceScene S,
         if (light_isect(S, j, o, w, &tri, &d, &N)) code = tri;
       }
       if (code == kNone) continue;
-      const double wt = mis_weight<EXT>(S, m, R.x, R.n, R.wo_ref, R.wo, w);
       double fa, pdf;
       brdf_pdf(m, R.n, R.wo_ref, R.wo, w, fa, pdf);
-      const double fx = brdf_apx(m, R, w);
+      const double wt = mis_weight_bp<EXT>(S, R.x, w, pdf);
+      const double fx = brdf_apx_f(m, R, w, fa);
       sum = sum + f4d(S.light_tris[code].rad) * ((1.0 * fa - fx) * cs * wt);
     }
     sum = sum + R.G;
@@ -715,6 +750,75 @@ __global__ __launch_bounds__(kDBlock) void k_dl_miscv_fused(DeviceScene S,
   out[3 * (size_t)p] = v.x;
   out[3 * (size_t)p + 1] = v.y;
   out[3 * (size_t)p + 2] = v.z;
+}
+
+// getLightFromMeshSource_Analytic_MISCV as shipped, without the shadow test
+// it ignores (lightSources_CV.cpp:399-450), in one pass per point and no
+// item buffers or shadow rays.  One wave per shading point: lane l takes
+// slots l, l + 64, l + 128, ..., forms each term as the reference forms it
+// and adds its own terms in slot order; the 64 lane sums are then combined
+// by a fixed butterfly (the same association in every lane) and G is added.
+// The point's record and material are wave-uniform (scalar loads).  A chunk
+// of P points is P waves here against P / 64 in k_dl_miscv_fused, which is
+// what a small chunk needs to fill the GPU; for large chunks the lane
+// kernel is faster (its slots, and so its branches, are wave-uniform; here
+// a wave's lanes mix light and BSDF slots, and its third pass holds 12 of
+// 140 slots).  The sum differs from the reference's sequential one by FP64
+// rounding only, and is the same on every run.  out as k_dl_reduce.
+constexpr int kMiscvWaves = 4;  // points (waves) per block
+constexpr uint32_t kMiscvWavePoints = 600000;  // chunks below this size use it
+template <bool EXT>
+__global__ __launch_bounds__(64 * kMiscvWaves, 3) void k_dl_miscv_wave(
+    DeviceScene S, const PointRec *__restrict__ pts, uint32_t P, uint32_t n_slots, uint32_t key0,
+    uint32_t key1, bool colour, double *__restrict__ out) {
+  const uint32_t p =
+      __builtin_amdgcn_readfirstlane(blockIdx.x * kMiscvWaves + (threadIdx.x >> 6));
+  const uint32_t lane = threadIdx.x & 63u;
+  if (p >= P) return;  // wave-uniform
+  const PointRec &R = pts[p];
+  d3 v = R.pre;
+  if (R.flags & kActive) {
+    const MatT<double> m = load_mat<double>(S.mats + R.mat);
+    const d3 o = R.x + R.n * 1e-8;  // rr.o (lightSources_CV.cpp:412)
+    const Slots L = mis_slots(S);
+    d3 sum = dk(0, 0, 0);
+    for (uint32_t slot = lane; slot < n_slots; slot += 64) {
+      const u4 r = philox(u4{R.ca, R.cb, slot, CHK_STREAM_DIRECT + R.cd}, key0, key1);
+      const d3 w = slot_dir<EXT>(S, R, m, L, slot, r);
+      const double cs = dot(w, R.n);
+      if (cs < 0) continue;
+      uint32_t code = kNone;
+      for (uint32_t j = 0; j < S.n_lights && code == kNone; ++j) {
+        uint32_t tri;
+        double d;
+        d3 N;
+        if (light_isect(S, j, o, w, &tri, &d, &N)) code = tri;
+      }
+      if (code == kNone) continue;
+      double fa, pdf;
+      brdf_pdf(m, R.n, R.wo_ref, R.wo, w, fa, pdf);
+      const double wt = mis_weight_bp<EXT>(S, R.x, w, pdf);
+      const double fx = brdf_apx_f(m, R, w, fa);
+      sum = sum + f4d(S.light_tris[code].rad) * ((1.0 * fa - fx) * cs * wt);
+    }
+#pragma unroll
+    for (int k = 1; k < 64; k <<= 1) {
+      sum.x += __shfl_xor(sum.x, k);
+      sum.y += __shfl_xor(sum.y, k);
+      sum.z += __shfl_xor(sum.z, k);
+    }
+    sum = sum + R.G;
+    v = sum;
+    if (colour) {
+      const chk_material &mt = S.mats[R.mat];
+      v = dk(mt.color[0] * sum.x, mt.color[1] * sum.y, mt.color[2] * sum.z);
+    }
+  }
+  if (lane == 0) {
+    out[3 * (size_t)p] = v.x;
+    out[3 * (size_t)p + 1] = v.y;
+    out[3 * (size_t)p + 2] = v.z;
+  }
 }
 
 // Per point: the reference's sums in its order.  out: k x 3 doubles (the
@@ -1340,7 +1444,7 @@ void launch_items(chk_scene_s *sc, const DWork &W, uint32_t P, uint32_t n_slots,
                        p.flags, p.seed_lo, p.seed_hi, W.I);
 }
 
-// MISCV as shipped runs fused (k_dl_miscv_fused): no per-item storage.
+// MISCV as shipped runs fused (k_dl_miscv_wave): no per-item storage.
 bool fused(const chk_direct_params &p) {
   return p.estimator == CHK_DIRECT_MISCV && !(p.flags & CHK_DIRECT_VISIBILITY);
 }
@@ -1360,13 +1464,28 @@ chk_status run_items(chk_scene_s *sc, const DWork &W, uint32_t P, const chk_dire
   const uint32_t n_slots = n_slots_of(sc, p);
   const uint32_t cvn = std::min(p.cv_samples, 1000u);
   if (fused(p)) {
-    const dim3 g((P + kDBlock - 1) / kDBlock), b(kDBlock);
-    if (sc->extras)
-      hipLaunchKernelGGL(k_dl_miscv_fused<true>, g, b, 0, st, sc->d, W.pts, P, n_slots, p.seed_lo,
-                         p.seed_hi, colour, W.val);
-    else
-      hipLaunchKernelGGL(k_dl_miscv_fused<false>, g, b, 0, st, sc->d, W.pts, P, n_slots,
-                         p.seed_lo, p.seed_hi, colour, W.val);
+    // Below 600,000 points a chunk cannot keep the point-per-lane kernel's
+    // waves resident long enough (its last wave of points runs alone): one
+    // wave per point instead (the path shaders' later vertices).  Measured
+    // on the 1080p direct leg: shadeDirectOnly 19.2 ms (wave kernel
+    // throughout: 29.4), shadeExplicit 38.9 ms (lane kernel throughout: 70.6).
+    if (P < kMiscvWavePoints) {
+      const dim3 g((P + kMiscvWaves - 1) / kMiscvWaves), b(64 * kMiscvWaves);
+      if (sc->extras)
+        hipLaunchKernelGGL((k_dl_miscv_wave<true>), g, b, 0, st, sc->d, W.pts, P, n_slots,
+                           p.seed_lo, p.seed_hi, colour, W.val);
+      else
+        hipLaunchKernelGGL((k_dl_miscv_wave<false>), g, b, 0, st, sc->d, W.pts, P, n_slots,
+                           p.seed_lo, p.seed_hi, colour, W.val);
+    } else {
+      const dim3 g((P + kDBlock - 1) / kDBlock), b(kDBlock);
+      if (sc->extras)
+        hipLaunchKernelGGL((k_dl_miscv_fused<true>), g, b, 0, st, sc->d, W.pts, P, n_slots,
+                           p.seed_lo, p.seed_hi, colour, W.val);
+      else
+        hipLaunchKernelGGL((k_dl_miscv_fused<false>), g, b, 0, st, sc->d, W.pts, P, n_slots,
+                           p.seed_lo, p.seed_hi, colour, W.val);
+    }
     CHK_HIP(hipGetLastError());
     return CHK_OK;
   }
