@@ -10,6 +10,7 @@
 
 #include "chk_device.h"
 #include "chk_internal.h"
+#include "chk_light64.h"
 
 namespace chk {
 
@@ -421,27 +422,7 @@ chk_status chk_scene_create_desc(chk_ctx_t ctx, const chk_scene_desc *desc, chk_
       T.c_leaf = make_float4(p[6], p[7], p[8], lf);
       T.rad = make_float4(mt.radiance[0], mt.radiance[1], mt.radiance[2], (float)bin_end);
       ltris.push_back(T);
-      {
-        LightTri64 Q;
-        double v1[3], v2[3];
-        for (int c = 0; c < 3; ++c) {
-          Q.a[c] = p[c];
-          v1[c] = Q.v1[c] = (double)p[3 + c] - p[c];
-          v2[c] = Q.v2[c] = (double)p[6 + c] - p[c];
-        }
-        // (C-A) x (B-A), normalised as Vec::norm: v * (1 / sqrt(v.v))
-        const double nx = v2[1] * v1[2] - v2[2] * v1[1], ny = v2[2] * v1[0] - v2[0] * v1[2],
-                     nz = v2[0] * v1[1] - v2[1] * v1[0];
-        const double inv = 1.0 / std::sqrt(nx * nx + ny * ny + nz * nz);
-        Q.n[0] = nx * inv;
-        Q.n[1] = ny * inv;
-        Q.n[2] = nz * inv;
-        Q.b0 = v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2];
-        Q.b1 = v1[0] * v2[0] + v1[1] * v2[1] + v1[2] * v2[2];
-        Q.c1 = v2[0] * v2[0] + v2[1] * v2[1] + v2[2] * v2[2];
-        Q.den = Q.b1 * Q.b1 - Q.c1 * Q.b0;
-        ltri64.push_back(Q);
-      }
+      ltri64.push_back(make_light64(p));
       ++R.count;
     }
     R.area = (float)area;

@@ -25,12 +25,12 @@
 
 #include "chk_env_dev.h"
 #include "chk_internal.h"
+#include "chk_light64.h"
 #include "chk_ltc_dev.h"
 
 namespace chk {
 namespace {
 
-constexpr double kInf64 = 1e20;  // mathPrimitives.h:5
 constexpr int kDBlock = 128;
 constexpr uint64_t kItemBudget = 1ull << 24;  // (point, slot) items per chunk
 constexpr uint32_t kNone = 0xFFFFFFFFu;
@@ -54,54 +54,6 @@ struct PointRec {
 
 __device__ inline double ud(uint32_t x) { return ((double)x + 0.5) * (1.0 / 4294967296.0); }
 __device__ inline d3 f4d(float4 v) { return dk(v.x, v.y, v.z); }
-
-// Triangle::intersect (geometryPrimitives.cpp:29-71) in FP64 with the
-// triangle's constants precomputed (LightTri64): the reference's operations
-// without recomputing the normal and the edge dot products per call.
-// A miss is decided without the divisions whose quotient only the miss
-// tests read, where the sign or size of the numerator already decides them
-// (the results are those of the reference's divide-then-compare):
-//  * t = tn / rn with rn < 0: t < 0 exactly when tn > 0 (|rn| <= 1 for a
-//    unit direction and unit normal, so a positive tn never underflows to
-//    -0 in the quotient);
-//  * gamma = gn / den with den < 0 (b1^2 - c1 b0, Cauchy-Schwarz): gamma < 0
-//    exactly when gn > 0, and gamma > 1 when gn < den (1 + 2^-40) (the
-//    quotient then exceeds 1 + 2^-41, which rounds above 1);
-//  * beta = bn / b0 with b0 > 0: beta < 0 exactly when bn < 0, beta > 1 when
-//    bn > b0 (1 + 2^-40).
-// Numerators within 1e-280 of zero take the divisions (no underflow
-// question), as do degenerate triangles (den >= 0).
-__device__ inline double tri_isect64p(const LightTri64 &T, d3 o, d3 d, d3 *N) {
-  const d3 n = dk(T.n[0], T.n[1], T.n[2]);
-  double rn = dot(d, n);
-  d3 Nf = n;
-  if (rn > 0) {
-    Nf = n * -1.0;
-    rn = -rn;
-  }
-  *N = Nf;
-  if (rn * rn <= 1e-10) return kInf64;
-  const d3 A = dk(T.a[0], T.a[1], T.a[2]);
-  const double tn = dot(A - o, Nf);
-  if (tn > 1e-280) return kInf64;  // t < 0
-  const double t = tn / rn;
-  if (t < 0) return kInf64;
-  const d3 v0 = (o + d * t) - A;
-  const double k0 = dot(v0, dk(T.v1[0], T.v1[1], T.v1[2]));
-  const double k1 = dot(v0, dk(T.v2[0], T.v2[1], T.v2[2]));
-  const double gn = k0 * T.b1 - k1 * T.b0;
-  const bool fast = T.den < 0;
-  constexpr double kOver = 1.0 + 0x1p-40;
-  if (fast && (gn > 1e-280 || gn < T.den * kOver)) return kInf64;  // gamma < 0 or > 1
-  const double gamma = gn / T.den;
-  const double bn = k0 - gamma * T.b1;  // c0 = v2.v1 = b1
-  if (fast && (bn < -1e-280 || bn > T.b0 * kOver)) return kInf64;  // beta < 0 or > 1
-  const double beta = bn / T.b0;
-  if (gamma + beta > 1) return kInf64;
-  if (0 > gamma || gamma > 1) return kInf64;
-  if (0 > beta || beta > 1) return kInf64;
-  return t;
-}
 
 // MeshLight::intersect (lightSources.cpp:830-847): brute force from the last
 // triangle, `(d = ...) && d < t`.  tri: index into the light-triangle table.
