@@ -202,7 +202,13 @@ struct TraceShape {
   uint32_t *ovf = nullptr;
   uint32_t ovf_cap = 0;
   int shallow = 16;  // the shorter stack: 16 or 32 (8 only for single-scene 128-thread launches)
+  uint32_t stream = 0;  // >0: ray-replacement traversal with this many blocks per segment
 };
+// Blocks per segment of the ray-replacement traversal in the renders: the
+// closest-hit rays after the camera rays and the shadow rays.  Camera rays
+// keep the one-ray-per-lane kernel: a wave's 64 samples of one pixel walk
+// the same nodes, and replacing finished rays would break that up.
+constexpr uint32_t kStreamBlocksClosest = 1024, kStreamBlocksAny = 768;
 // Words of an overflow list for n ray slots.
 inline size_t ovf_words(uint32_t n) { return 64 + (size_t)n; }
 // Per-ray scenes of a batched launch: ray i belongs to scene[slot_seg[i]]

@@ -956,7 +956,9 @@ chk_status render_impl(chk_scene_s *sc, const chk_camera *cam, const chk_render_
       const uint32_t in = depth & 1u, out = in ^ 1u;
       hipLaunchKernelGGL(k_reset_bounce, dim3(1), dim3(64), 0, st_c, Bc.ctrl, qbase(out));
       prof.begin(0);
-      hipError_t e = launch_closest(sc, TraceShape{sc->stack_size, 128, Bc.ovf, qcap, sc->shallow_stack},
+      hipError_t e = launch_closest(sc,
+                                    TraceShape{sc->stack_size, 128, Bc.ovf, qcap, sc->shallow_stack,
+                                               depth ? kStreamBlocksClosest : 0u},
                                     Bc.q_ray[in],
                                     SegRange{Bc.ctrl + qbase(in), 0u, segcap}, Bc.hits, true, count,
                                     Bc.stats + kNodeVisits, SlotScenes{}, st_c);
@@ -1018,7 +1020,9 @@ chk_status render_impl(chk_scene_s *sc, const chk_camera *cam, const chk_render_
       }
       if (depth < P->max_depth) {
         prof.begin(2);
-        e = launch_any(sc, TraceShape{sc->stack_size, 128, Bc.ovf, qcap, sc->shallow_stack},
+        e = launch_any(sc,
+                       TraceShape{sc->stack_size, 128, Bc.ovf, qcap, sc->shallow_stack,
+                                  kStreamBlocksAny},
                        Bc.s_ray,
                        SegRange{Bc.ctrl + kQS, 0u, segcap}, training ? LBc.s_occ : nullptr, Bc.s_L,
                        Bc.path_L, true, count, Bc.stats + kShNodeVisits, SlotScenes{}, st_c);
