@@ -1381,10 +1381,14 @@ chk_status batch_impl(chk_ctx_s *ctx, const chk_request *R, uint32_t n,
     for (uint32_t depth = 0; depth <= max_depth; ++depth) {
       const uint32_t in = depth & 1u, out = in ^ 1u;
       hipLaunchKernelGGL(k_reset_bounce, dim3(1), dim3(64), 0, s, B.ctrl, qbase(out));
+      // ray replacement after the camera rays and for shadow rays (as chk_render)
+      TraceShape cshape = single ? sshape : shape, ashape = single ? sshape : shape;
+      cshape.stream = depth ? kStreamBlocksClosest : 0u;
+      ashape.stream = kStreamBlocksAny;
       hipError_t e =
-          single ? launch_closest(ssc, sshape, B.q_ray[in], SegRange{B.ctrl + qbase(in), 0u, segcap},
+          single ? launch_closest(ssc, cshape, B.q_ray[in], SegRange{B.ctrl + qbase(in), 0u, segcap},
                                   B.hits, true, false, B.stats + kNodeVisits, SlotScenes{}, s)
-                 : launch_closest(nullptr, shape, B.q_ray[in],
+                 : launch_closest(nullptr, cshape, B.q_ray[in],
                                   SegRange{B.ctrl + qbase(in), 0u, segcap}, B.hits, true, false,
                                   B.stats + kNodeVisits,
                                   SlotScenes{d_scene + C.first_seg, q_seg[in]}, s);
@@ -1401,10 +1405,10 @@ chk_status batch_impl(chk_ctx_s *ctx, const chk_request *R, uint32_t n,
         launch_shade_batch<256>(sgrid, s, depth, segcap, B, in, out, qbase(in), qbase(out), Q,
                                 single);
       if (depth < max_depth) {
-        e = single ? launch_any(ssc, sshape, B.s_ray, SegRange{B.ctrl + kQS, 0u, segcap}, nullptr,
+        e = single ? launch_any(ssc, ashape, B.s_ray, SegRange{B.ctrl + kQS, 0u, segcap}, nullptr,
                                 B.s_L, B.path_L, true, false, B.stats + kShNodeVisits,
                                 SlotScenes{}, s)
-                   : launch_any(nullptr, shape, B.s_ray, SegRange{B.ctrl + kQS, 0u, segcap},
+                   : launch_any(nullptr, ashape, B.s_ray, SegRange{B.ctrl + kQS, 0u, segcap},
                                 nullptr, B.s_L, B.path_L, true, false, B.stats + kShNodeVisits,
                                 SlotScenes{d_scene + C.first_seg, s_seg}, s);
         if (e != hipSuccess) return e;

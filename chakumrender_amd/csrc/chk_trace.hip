@@ -308,14 +308,15 @@ __global__ __launch_bounds__(BS) void k_trace(DeviceScene S0, const float4 *__re
 // at 1,024 blocks per segment, shadow rays at 768): 2,085 -> 2,293 Mpaths/s;
 // for the coherent camera rays it is slower than k_trace (their waves hold
 // 64 samples of one pixel, which replacement scatters).
-template <bool ANY, int STACK, int BS>
+template <bool ANY, int STACK, int BS, bool MULTI>
 __global__ __launch_bounds__(BS) void k_trace_stream(DeviceScene S, const float4 *__restrict__ rays,
                                                      SegRange W, float4 *__restrict__ hits,
                                                      uint8_t *__restrict__ occluded,
                                                      const float4 *__restrict__ shadow_L,
                                                      float4 *__restrict__ path_L, int leaf_ids,
                                                      unsigned long long *__restrict__ counters,
-                                                     uint32_t *__restrict__ ovf, uint32_t ovf_cap) {
+                                                     SlotScenes M, uint32_t *__restrict__ ovf,
+                                                     uint32_t ovf_cap) {
   __shared__ uint32_t stack_mem[STACK * BS];
   __shared__ uint32_t cursor;
   uint32_t *stk = stack_mem + threadIdx.x;
@@ -343,8 +344,16 @@ __global__ __launch_bounds__(BS) void k_trace_stream(DeviceScene S, const float4
   uint32_t skip = 0, best = 0xFFFFFFFFu, best_leaf = 0, entry = 0, pend = 0, lost = 0;
   int sp = 0;
   bool live = false, hit_any = false, ovfl = false;
+  // the ray's scene (a batched launch: the scene of the ray's request)
+  const Node4 *nodes = S.nodes;
+  const float4 *tris = S.tris;
   auto start = [&]() {
     const size_t i = seg0 + idx;
+    if (MULTI) {
+      const DeviceScene *ps = M.scene + M.slot_seg[i];
+      nodes = ps->nodes;
+      tris = ps->tris;
+    }
     const float4 ot = rays[2 * i], ds = rays[2 * i + 1];
     o = xyz(ot);
     d = xyz(ds);
@@ -380,7 +389,7 @@ __global__ __launch_bounds__(BS) void k_trace_stream(DeviceScene S, const float4
           (__popcll(__ballot(searching && pend == 0)) < 4 && __any(pend != 0)))
         break;  // wave-uniform
       if (!searching) continue;
-      const float4 *np = reinterpret_cast<const float4 *>(S.nodes + entry);
+      const float4 *np = reinterpret_cast<const float4 *>(nodes + entry);
       const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
       const uint4 ch = *reinterpret_cast<const uint4 *>(np + 6);
       float key[4];
@@ -443,7 +452,7 @@ __global__ __launch_bounds__(BS) void k_trace_stream(DeviceScene S, const float4
       const uint32_t first = leaf_first(lf), lcnt = leaf_count(lf);
       for (uint32_t k = 0; k < lcnt; ++k) {
         const uint32_t li = first + k;
-        const float4 *tp = S.tris + 3 * (size_t)li;
+        const float4 *tp = tris + 3 * (size_t)li;
         const float4 a = tp[0], b = tp[1], c = tp[2];
         const uint32_t tid = leaf_ids ? li : __float_as_uint(a.w);
         if (tid == skip) continue;
@@ -474,7 +483,7 @@ __global__ __launch_bounds__(BS) void k_trace_stream(DeviceScene S, const float4
       if (ovfl && !later) ++lost;
       if (!later) {
         const float tr = best == 0xFFFFFFFFu ? 1e30f : t;
-        finish_ray<ANY, false>(S, i, make_float4(o.x, o.y, o.z, 0.0f),
+        finish_ray<ANY, MULTI>(S, i, make_float4(o.x, o.y, o.z, 0.0f),
                                make_float4(d.x, d.y, d.z, 0.0f), tr, best, bu, bv, hit_any,
                                best_leaf, hits, occluded, shadow_L, path_L, leaf_ids);
       }
@@ -736,17 +745,35 @@ hipError_t launch_kind(const chk_scene_s *s, const TraceShape &T, const float4 *
                                       sL, pL, leaf_ids, counters, M, st);
   int shallow = T.shallow;
   if (shallow == 8 && (multi || T.block != 128)) shallow = 16;
-  if (T.stream > 0 && !multi && T.ovf && T.stack > shallow && shallow == 16 && T.block == 128) {
+  if (T.stream > 0 && T.ovf && T.stack > 16 && shallow == 16 && (T.block == 64 || T.block == 128)) {
     // ray replacement over the 16-entry stack, then the overflow list again
-    // with the full stack (as below)
+    // with the full stack (as below); T.stream counts 128-thread blocks
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, T.ovf, 1u);
-    const uint32_t bps = std::min<uint32_t>(W.segcap / 128, T.stream);
-    hipLaunchKernelGGL((k_trace_stream<ANY, 16, 128>), dim3(kSegs * bps), dim3(128), 0, st, s->d,
-                       rays, W, hits, occ, sL, pL, leaf_ids ? 1 : 0, counters, T.ovf, T.ovf_cap);
+    const DeviceScene S = s ? s->d : DeviceScene{};
+    const uint32_t bps = std::min<uint32_t>(W.segcap / T.block, T.stream * (128 / T.block));
+    const dim3 g(kSegs * bps), b(T.block);
+    const int li = leaf_ids ? 1 : 0;
+    if (T.block == 64) {
+      if (multi)
+        hipLaunchKernelGGL((k_trace_stream<ANY, 16, 64, true>), g, b, 0, st, S, rays, W, hits, occ,
+                           sL, pL, li, counters, M, T.ovf, T.ovf_cap);
+      else
+        hipLaunchKernelGGL((k_trace_stream<ANY, 16, 64, false>), g, b, 0, st, S, rays, W, hits, occ,
+                           sL, pL, li, counters, M, T.ovf, T.ovf_cap);
+    } else {
+      if (multi)
+        hipLaunchKernelGGL((k_trace_stream<ANY, 16, 128, true>), g, b, 0, st, S, rays, W, hits,
+                           occ, sL, pL, li, counters, M, T.ovf, T.ovf_cap);
+      else
+        hipLaunchKernelGGL((k_trace_stream<ANY, 16, 128, false>), g, b, 0, st, S, rays, W, hits,
+                           occ, sL, pL, li, counters, M, T.ovf, T.ovf_cap);
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    return launch_retry<ANY, false>(s, T.stack, rays, hits, occ, sL, pL, leaf_ids, counters, M, st,
-                                    T.ovf, T.ovf_cap);
+    return multi ? launch_retry<ANY, true>(s, T.stack, rays, hits, occ, sL, pL, leaf_ids, counters,
+                                           M, st, T.ovf, T.ovf_cap)
+                 : launch_retry<ANY, false>(s, T.stack, rays, hits, occ, sL, pL, leaf_ids,
+                                            counters, M, st, T.ovf, T.ovf_cap);
   }
   if (T.ovf && T.stack > shallow) {
     // a stack shorter than the tree's worst case for every ray (more waves
