@@ -809,7 +809,9 @@ def main():
                                  "note": "one frame after the timed steps with its chunks one "
                                          "after another (CHK_RENDER_ONE_STREAM)"},
         "roofline": {
-            "kernel": "k_trace<closest> (chakumrender_amd/csrc/chk_trace.hip)",
+            "kernel": "closest-hit traversal: k_trace<closest> for the camera rays, "
+                      "k_trace_stream<closest> (ray replacement) for the later bounces "
+                      "(chakumrender_amd/csrc/chk_trace.hip)",
             "bound": "hbm",
             "achieved": round(achieved, 2),
             "peak": HBM_PEAK_GBS,
@@ -841,7 +843,7 @@ def main():
                             "nodes (profiles/traversal_counters_icosphere100k_1080p.md)"},
         },
         "any_hit": {
-            "kernel": "k_trace<any>",
+            "kernel": "k_trace_stream<any> (ray replacement)",
             "rays_per_frame": cnt["rays_shadow"],
             "node_visits_per_ray": round(cnt["shadow_node_visits"] / max(1, cnt["rays_shadow"]), 2),
             "tri_tests_per_ray": round(cnt["shadow_tri_tests"] / max(1, cnt["rays_shadow"]), 2),

@@ -171,6 +171,27 @@ def test_short_stack_retries_are_exact(ctx):
         ds.set_trace_stack(24)
 
 
+def test_ray_replacement_equals_one_ray_per_lane(ctx):
+    """The renders trace the closest hits after the camera rays, and the
+    shadow rays, with ray replacement (a lane takes the next ray of its
+    block's share when its ray finishes) over the 16-entry stack; with the
+    32-entry stack every ray is traced one per lane.  On the 100,839-triangle
+    icosphere (8 lights, depth 6) the images are equal bit for bit, and so
+    are the per-path ray counts."""
+    sc = scenes.icosphere_scene(freq=71, width=96, height=54)
+    ds = abi.DeviceScene(ctx, sc)
+    assert 16 < ds.info()["stack_bound"] <= 32
+    P = dict(spp=16, max_depth=6, seed=0x5EED0022, flags=abi.RENDER_SUMSQ)
+    a, sa, st = host.render(ds, P)
+    ds.set_trace_stack(32)
+    b, sb, st32 = host.render(ds, P)
+    ds.set_trace_stack(16)
+    print(f"{st['rays_extend']} + {st['rays_shadow']} rays, {st['stack_retries']} traced again")
+    assert (st["rays_extend"], st["rays_shadow"]) == (st32["rays_extend"], st32["rays_shadow"])
+    np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(sa, sb)
+
+
 def test_deep_tree_short_stack_equals_full_stack(ctx):
     """A tree whose worst-case stack (40) exceeds 32 entries is rendered
     with the 32-entry stack plus retries; the counted render traces every ray
