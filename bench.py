@@ -839,6 +839,13 @@ def main():
                 "valu_busy_pct": meas.get("valu_busy_pct"),
                 "valu_utilization_pct": meas.get("valu_utilization_pct"),
                 "waves_per_cu": meas.get("waves_per_cu"),
+                "valu_issue": None if not meas.get("valu_wave_insts_per_launch") else {
+                    "wave_insts_per_launch": meas["valu_wave_insts_per_launch"],
+                    "achieved_ginst_s": round(meas["valu_wave_insts_per_launch"] / avg_launch_s / 1e9, 1),
+                    "peak_ginst_s": meas["valu_issue_peak_ginst_s"],
+                    "frac": round(meas["valu_wave_insts_per_launch"] / avg_launch_s / 1e9
+                                  / meas["valu_issue_peak_ginst_s"], 3),
+                    "peak_note": meas.get("valu_issue_peak_note")},
                 "bound_by": "divergent VALU issue and dependent-fetch latency with L1/L2-resident "
                             "nodes (profiles/traversal_counters_icosphere100k_1080p.md)"},
         },
