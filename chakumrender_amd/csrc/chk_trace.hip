@@ -104,15 +104,17 @@ __device__ inline void traverse(const DeviceScene &S, uint32_t *__restrict__ stk
 #undef CHK_CX
       if (nh > 0) {
         entry = cid[0];
-        // push far-to-near so the nearer ones pop first
-        for (int c = nh - 1; c >= 1; --c) {
-          if (sp < STACK) {
-            stk[sp * BS] = cid[c];
-            ++sp;
-          } else {
-            overflow = 1;
-          }
+        // push the other nh - 1 far to near, so the nearer ones pop first:
+        // cid[1 + k] to slot sp + nh - 2 - k, without a loop (entries past
+        // the stack are dropped and the ray marked, as one push at a time)
+        const int np = nh - 1;
+        if (sp + np > STACK) overflow = 1;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          const int pos = sp + np - 1 - k;
+          if (k < np && pos < STACK) stk[pos * BS] = cid[1 + k];
         }
+        sp = min(sp + np, STACK);
       } else {
         pop();
       }
@@ -423,14 +425,14 @@ __global__ __launch_bounds__(BS) void k_trace_stream(DeviceScene S, const float4
 #undef CHK_CX
       if (nh > 0) {
         entry = cid[0];
-        for (int c = nh - 1; c >= 1; --c) {
-          if (sp < STACK) {
-            stk[sp * BS] = cid[c];
-            ++sp;
-          } else {
-            ovfl = true;
-          }
+        const int np = nh - 1;  // pushed far to near, as in traverse
+        if (sp + np > STACK) ovfl = true;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          const int pos = sp + np - 1 - k;
+          if (k < np && pos < STACK) stk[pos * BS] = cid[1 + k];
         }
+        sp = min(sp + np, STACK);
       } else {
         pop();
       }
