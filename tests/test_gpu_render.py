@@ -235,11 +235,11 @@ def test_render_window_equals_whole_image(ctx, win):
 
 
 def test_multi_chunk_render_equals_its_windows(ctx):
-    """A render of three chunks of 2^24 paths (its chunks alternate two
+    """A render of three chunks of 2^25 paths (its chunks alternate two
     queue sets and streams) equals, bit for bit, the same image rendered as
     three row windows of one chunk each and with the chunks on one stream,
     and is deterministic."""
-    W, H, spp = 512, 1536, 64  # 50,331,648 paths
+    W, H, spp = 512, 3072, 64  # 100,663,296 paths
     sc = scenes.cornell(width=W, height=H, sphere_tris=2000, n_lights=3, light_rng=True)
     ds = abi.DeviceScene(ctx, sc)
     P = dict(spp=spp, max_depth=3, seed=0x5EED0051, flags=abi.RENDER_SUMSQ)
@@ -250,7 +250,7 @@ def test_multi_chunk_render_equals_its_windows(ctx):
     one, _, st2 = host.render(ds, dict(P, flags=P["flags"] | abi.RENDER_ONE_STREAM))
     np.testing.assert_array_equal(one, full)
     img = host.Image(W, H, sumsq=True)
-    rows = (1 << 24) // (W * spp)
+    rows = (1 << 25) // (W * spp)
     for y0 in range(0, H, rows):
         _, st1 = host.render(ds, dict(P, window=(0, y0, W, min(rows, H - y0))), image=img,
                              download=False)
