@@ -71,7 +71,7 @@ __host__ __device__ inline v3<T> cross(v3<T> a, v3<T> b) {
 // render's error budget allows them)
 __device__ inline float rsq(float a) { return rsqrtf(a); }
 __device__ inline double rsq(double a) { return 1.0 / sqrt(a); }
-__device__ inline float sqr_t(float a) { return sqrtf(a); }
+__device__ inline float sqr_t(float a) { return __builtin_amdgcn_sqrtf(a); }
 __device__ inline double sqr_t(double a) { return sqrt(a); }
 __device__ inline float ex(float a) { return __expf(a); }
 __device__ inline double ex(double a) { return exp(a); }
@@ -89,6 +89,10 @@ __device__ inline float csign(float a, float b) { return copysignf(a, b); }
 __device__ inline double csign(double a, double b) { return copysign(a, b); }
 __device__ inline void sncs(float a, float *s, float *c) { __sincosf(a, s, c); }
 __device__ inline void sncs(double a, double *s, double *c) { sincos(a, s, c); }
+// a / b: FP32 through the hardware reciprocal (v_rcp_f32, 1 ulp) instead of
+// the correctly rounded division sequence; FP64 exact
+__device__ inline float dv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ inline double dv(double a, double b) { return a / b; }
 
 template <typename T>
 __device__ inline v3<T> normalize(v3<T> a) {
@@ -120,7 +124,7 @@ __device__ inline float u01(uint32_t x) { return (float)(x >> 8) * (1.0f / 16777
 template <typename T>
 __device__ inline void onb(v3<T> n, v3<T> &t, v3<T> &b) {
   const T sign = csign(T(1), n.z);
-  const T a = T(-1) / (sign + n.z);
+  const T a = dv(T(-1), sign + n.z);
   const T bb = n.x * n.y * a;
   t = mk3(T(1) + sign * n.x * n.x * a, sign * bb, -sign * n.x);
   b = mk3(bb, sign + n.y * n.y * a, -n.y);
@@ -163,12 +167,12 @@ __device__ inline T inv_pi_v() {
 // fresnel (materialTypes.cpp:156-178)
 template <typename T>
 __device__ inline T fresnel(const MatT<T> &m, T c) {
-  T g = m.iior / m.eior;
+  T g = dv(m.iior, m.eior);
   g = sqr_t(fm(g, g, fm(c, c, T(-1))));
   const T t1 = g - c, t2 = g + c;
   const T a = fm(c, t2, T(-1)), b = fm(c, t1, T(1));
-  const T q = t1 / t2;
-  return T(0.5) * q * q * (T(1) + (a * a) / (b * b));
+  const T q = dv(t1, t2);
+  return T(0.5) * q * q * (T(1) + dv(a * a, b * b));
 }
 
 // Microfacet terms from cos = h.n and sin^2 = |h x n|^2.
@@ -176,13 +180,13 @@ __device__ inline T fresnel(const MatT<T> &m, T c) {
 template <typename T>
 __device__ inline T ggx_D(T a2, T c, T s2) {
   const T den = fm(a2, c * c, s2);  // cos^2 (a2 + tan2)
-  return a2 / (pi_v<T>() * den * den);
+  return dv(a2, pi_v<T>() * den * den);
 }
 // beckmannDist (materialTypes.cpp:212-222): exp(-tan2/a2) / (pi cos^4 a2)
 template <typename T>
 __device__ inline T beck_D(T a2, T c, T s2) {
   const T c2 = c * c;
-  return ex(-s2 / (c2 * a2)) / (pi_v<T>() * c2 * c2 * a2);
+  return dv(ex(dv(-s2, c2 * a2)), pi_v<T>() * c2 * c2 * a2);
 }
 // Phong D (materialTypes.cpp:39,107): (e+2)/(2 pi) cos^e
 template <typename T>
@@ -199,12 +203,13 @@ __device__ inline T ggx_G1(T a2, T cv, T s2v, T hv) {
   const T c2 = cv * cv;
   if (s2v <= T(1e-15) * c2) return T(1);
   if (hv <= T(1e-15)) return T(0);
-  return T(2) / (T(1) + sqr_t(T(1) + a2 * s2v / c2));
+  return dv(T(2), T(1) + sqr_t(T(1) + dv(a2 * s2v, c2)));
 }
 // rational G1 of beckmannG1/phongG1 (materialTypes.cpp:224-258)
 template <typename T>
 __device__ inline T rational_G1(T a) {
-  if (a < T(1.6)) return (T(3.535) * a + T(2.181) * a * a) / (T(1) + T(2.276) * a + T(2.577) * a * a);
+  if (a < T(1.6))
+    return dv(T(3.535) * a + T(2.181) * a * a, T(1) + T(2.276) * a + T(2.577) * a * a);
   return T(1);
 }
 template <typename T>
@@ -213,7 +218,7 @@ __device__ inline T beck_G1(T alpha, T cv, T s2v, T hv) {
   if (c2 >= T(0.999999)) return T(1);
   if (hv <= T(1e-15)) return T(0);
   // a = 1 / (alpha tan) = |cv| / (alpha sqrt(s2v))
-  return rational_G1(fab(cv) / (alpha * sqr_t(s2v)));
+  return rational_G1(dv(fab(cv), alpha * sqr_t(s2v)));
 }
 template <typename T>
 __device__ inline T phong_G1(T e, T cv, T s2v, T hv) {
@@ -221,7 +226,7 @@ __device__ inline T phong_G1(T e, T cv, T s2v, T hv) {
   if (c2 >= T(0.999999)) return T(1);
   if (hv <= T(1e-15)) return T(0);
   // a = sqrt((0.5 e + 1) / tan^2)
-  return rational_G1(sqr_t((T(0.5) * e + T(1)) * c2 / s2v));
+  return rational_G1(sqr_t(dv((T(0.5) * e + T(1)) * c2, s2v)));
 }
 
 // |a x b|^2 for unit vectors: sin^2 of the angle between them.
@@ -268,8 +273,8 @@ __device__ inline void brdf_pdf(const MatT<T> &m, v3<T> n, v3<T> wo_ref, v3<T> w
       D = phong_D(m.alpha, ch, s2h);
       G = phong_G1(m.alpha, cwo, s2o, hwo) * phong_G1(m.alpha, cwi, s2i, hwi);
     }
-    f = ((m.ks * F * D * G) / (T(4) * cwi * cwo) + diffuse_f) * m.refl;
-    pdf = m.ks * D * ch / (T(4) * hwo) + diffuse_p;
+    f = (dv(m.ks * F * D * G, T(4) * cwi * cwo) + diffuse_f) * m.refl;
+    pdf = dv(m.ks * D * ch, T(4) * hwo) + diffuse_p;
     return;
   }
   f = T(0);
@@ -286,7 +291,7 @@ __device__ inline v3<T> brdf_sample(const MatT<T> &m, v3<T> n, v3<T> wo_ref, v3<
   if (ulobe < m.ks) {
     if (m.brdf == CHK_BRDF_CLASSIC_PHONG) {
       // getClassicPhongDirectionSamples (domainSampler.cpp:264-314)
-      const T ct = pw(T(1) - e1, T(1) / (m.pe + T(1)));
+      const T ct = pw(T(1) - e1, dv(T(1), m.pe + T(1)));
       const T st = sqr_t(fmx(T(0), T(1) - ct * ct));
       onb(wo_ref, t, b);
       return t * (st * cphi) + b * (st * sphi) + wo_ref * ct;
@@ -294,7 +299,7 @@ __device__ inline v3<T> brdf_sample(const MatT<T> &m, v3<T> n, v3<T> wo_ref, v3<
     T ct = T(1), st = T(0);
     if (m.brdf == CHK_BRDF_GGX) {
       // tan = alpha sqrt(e1 / (1 - e1)) (domainSampler.cpp:225)
-      const T t2 = m.alpha * m.alpha * e1 / (T(1) - e1);
+      const T t2 = dv(m.alpha * m.alpha * e1, T(1) - e1);
       ct = rsq(T(1) + t2);
       st = sqr_t(t2) * ct;
     } else if (m.brdf == CHK_BRDF_BECKMANN) {
@@ -303,7 +308,7 @@ __device__ inline v3<T> brdf_sample(const MatT<T> &m, v3<T> n, v3<T> wo_ref, v3<
       ct = rsq(T(1) + t2);
       st = sqr_t(t2) * ct;
     } else if (m.brdf == CHK_BRDF_PHONG) {
-      ct = pw(e1, T(1) / (m.alpha + T(2)));  // domainSampler.cpp:235
+      ct = pw(e1, dv(T(1), m.alpha + T(2)));  // domainSampler.cpp:235
       st = sqr_t(fmx(T(0), T(1) - ct * ct));
     }
     onb(n, t, b);

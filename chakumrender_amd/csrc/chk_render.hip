@@ -362,12 +362,12 @@ __device__ __forceinline__ void shade_vertex(const RenderArgs &A, uint32_t depth
         if (info.y != 0xFFFFFFFFu) {
           const float cl = fabsf(dot(d, Ng));
           const float pb = thr4.w;
-          const float pa = h.x * h.x / (S.lights[info.y].area * cl);  // mis.cpp:55-66
+          const float pa = dv(h.x * h.x, S.lights[info.y].area * cl);  // mis.cpp:55-66
           if (LEARNED) {
             const uint32_t prev = P.in_prev[i];
             const float c = P.mix_prev[prev];
             const float pl = P.probs_prev[32 * (size_t)prev + info.y] * pa;
-            w = (1.0f - c) * pb / (c * pl + (1.0f - c) * pb);
+            w = dv((1.0f - c) * pb, c * pl + (1.0f - c) * pb);
             if (P.rec_prev) {
               P.rec_prev[prev].le = lum(m.rad);
               P.rec_prev[prev].plb = pl;
@@ -375,7 +375,7 @@ __device__ __forceinline__ void shade_vertex(const RenderArgs &A, uint32_t depth
             }
           } else {
             const float pl = A.inv_nl * pa;
-            w = pb / (pb + pl);  // balance heuristic, N_light = N_bsdf = 1 (mis.cpp:209-220)
+            w = dv(pb, pb + pl);  // balance heuristic, N_light = N_bsdf = 1 (mis.cpp:209-220)
           }
         }
         Le = thr * m.rad * w;
@@ -441,20 +441,21 @@ __device__ __forceinline__ void shade_vertex(const RenderArgs &A, uint32_t depth
         // MeshLight::getSamples point (lightSources.cpp:812-824)
         const float e1 = sqrtf(u01(rr.y)), e2 = u01(rr.z);
         const f3 yp = la * (1.0f - e1) + lb * (e1 * (1.0f - e2)) + lc * (e2 * e1);
-        const f3 dv = yp - X;
-        const float d2 = dot(dv, dv);
+        const f3 dy = yp - X;
+        const float d2 = dot(dy, dy);
         const float dist = sqrtf(d2);
-        const f3 wi = dv * (1.0f / dist);
+        const f3 wi = dy * dv(1.0f, dist);
         const float cx = dot(wi, n);
         const f3 nl = cross(lc - la, lb - la);
         const float cl = fabsf(dot(wi, nl)) * rsqrtf(dot(nl, nl));
         if (cx > 0.0f && cl > 0.0f) {
           float f, pb;
           brdf_pdf(m, n, wo_ref, wo, wi, f, pb);
-          const float pl = qj * d2 / (L.area * cl);  // mis.cpp:55-66
+          const float pl = dv(qj * d2, L.area * cl);  // mis.cpp:55-66
           const f3 lrad = xyz(lt.rad);
           // c-weighted balance heuristic; c = 1/2 gives f cos / (p_l + p_b)
-          const f3 cc = thr * m.c * lrad * (f * cx * (2.0f * c) / (2.0f * c * pl + 2.0f * (1.0f - c) * pb));
+          const f3 cc =
+              thr * m.c * lrad * dv(f * cx * (2.0f * c), 2.0f * c * pl + 2.0f * (1.0f - c) * pb);
           if (LEARNED) {
             rec.light = j;
             rec.gl = f * cx * lum(m.c * lrad);
@@ -466,7 +467,7 @@ __device__ __forceinline__ void shade_vertex(const RenderArgs &A, uint32_t depth
             // reaches the light exactly at t = sdist (shader.cpp:82-90)
             const f3 sd = yp - org;
             const float sdist = sqrtf(dot(sd, sd));
-            const f3 sdir = sd * (1.0f / sdist);
+            const f3 sdir = sd * dv(1.0f, sdist);
             emit_sh = true;
             sr0 = make_float4(org.x, org.y, org.z, sdist - 1e-5f);
             sr1 = make_float4(sdir.x, sdir.y, sdir.z, __uint_as_float(leaf));
@@ -488,7 +489,7 @@ __device__ __forceinline__ void shade_vertex(const RenderArgs &A, uint32_t depth
           if (pb > 0.0f && isfinite(pb) && isfinite(f)) {
             // the weight 2(1-c) times the 1/2 of the emission weight at
             // the next vertex is 1 for the plain path
-            const f3 t2 = thr * m.c * (f * cs / pb);
+            const f3 t2 = thr * m.c * dv(f * cs, pb);
             if (LEARNED) {
               rec.fb = f * cs * lum(m.c);
               rec.pbb = pb;
