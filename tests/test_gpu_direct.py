@@ -67,6 +67,31 @@ def test_same_samples_as_oracle(env, name):
         assert abs(g.sum() - o.sum()) <= 0.01 * abs(o.sum())
 
 
+def test_miscv_lane_kernel_equals_wave_kernel(env):
+    """The shipped MISCV estimator runs chunks of 600,000 points or more with
+    a point per lane, smaller chunks with a wave per point.  700,000 points
+    in one call (one chunk: the lane kernel) and their first 100,000 alone
+    (the wave kernel) draw the same samples (the key is the point's index in
+    the call): the values agree to FP64 rounding (the two kernels add a
+    point's 140 terms in different orders), and the lane kernel's first
+    points equal the oracle's to the bar of the tests above."""
+    sc, _, SL, D = env
+    x, n, d, prim = scenes.shading_points(sc, 700_000, seed=12)
+    big = SL.estimate(abi.DIRECT_MISCV, x, n, d, prim, estimate=0)
+    k = 100_000
+    small = SL.estimate(abi.DIRECT_MISCV, x[:k], n[:k], d[:k], prim[:k], estimate=0)
+    assert np.isfinite(big).all() and np.abs(big).max() > 0
+    scale = np.maximum(np.abs(big[:k]), 1e-3 * np.abs(big[:k]).max())
+    rel = np.abs(big[:k] - small) / scale
+    print(f"lane vs wave kernel: max rel {rel.max():.2e}")
+    assert rel.max() <= 1e-10
+    P = host.direct_params(abi.DIRECT_MISCV)
+    P.estimate = 0
+    o = D.direct_light(P, x[:512], n[:512], d[:512], prim[:512])
+    orel = np.abs(big[:512] - o) / np.maximum(np.abs(o), 1e-6)
+    assert (orel > 1e-8).any(1).mean() <= 0.02
+
+
 @pytest.mark.parametrize("name,key,cv", [("mis", "mis", 0), ("analytic_cv16", "cv", 16)])
 def test_means_match_reference(env, G, name, key, cv):
     _, _, SL, _ = env
