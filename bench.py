@@ -23,7 +23,7 @@ from pathlib import Path
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
-NODE_BYTES = 128      # one 4-wide node (chk_bvh.h Node4)
+NODE_BYTES = 64       # one quantized 4-wide node (chk_bvh.h Node4Q)
 TRI_BYTES = 48        # one leaf triangle record (three float4)
 RAY_BYTES = 32 + 16   # chk_ray in + hit record out
 HBM_PEAK_GBS = 8000.0

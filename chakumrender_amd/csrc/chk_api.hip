@@ -433,7 +433,7 @@ chk_status chk_scene_create_desc(chk_ctx_t ctx, const chk_scene_desc *desc, chk_
   }
 
   // one device blob
-  const size_t b_nodes = align256(bvh.nodes.size() * sizeof(Node4));
+  const size_t b_nodes = align256(bvh.qnodes.size() * sizeof(Node4Q));
   const size_t b_tris = align256(std::max<size_t>(bvh.tris.size(), 1) * sizeof(TriRec));
   const size_t b_info = align256(info.size() * 4);
   const size_t b_mats = align256(std::max<uint32_t>(n_mats, 1) * sizeof(chk_material));
@@ -473,8 +473,8 @@ chk_status chk_scene_create_desc(chk_ctx_t ctx, const chk_scene_desc *desc, chk_
   sc->ctx = ctx;
   sc->blob = blob;
   sc->blob_bytes = total;
-  sc->d.nodes = (const Node4 *)p;
-  hipError_t e = hipMemcpy(p, bvh.nodes.data(), bvh.nodes.size() * sizeof(Node4),
+  sc->d.nodes = (const Node4Q *)p;
+  hipError_t e = hipMemcpy(p, bvh.qnodes.data(), bvh.qnodes.size() * sizeof(Node4Q),
                            hipMemcpyHostToDevice);
   p += b_nodes;
   sc->d.tris = (const float4 *)p;

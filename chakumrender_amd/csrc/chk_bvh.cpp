@@ -1,5 +1,6 @@
 // Host BVH build: binned SAH binary tree -> 4-wide collapse -> 128-byte SoA
-// nodes.  See chk_bvh.h.
+// nodes (host side) -> 64-byte quantized nodes (what the device traverses).
+// See chk_bvh.h.
 #include "chk_bvh.h"
 
 #include <algorithm>
@@ -227,6 +228,54 @@ struct Collapser {
   }
 };
 
+// One axis of a node: grid origin o (the lowest low plane), step s (a power
+// of two with every high plane within 255 steps) and the children's indices,
+// checked in quad precision so that o + q s never cuts into a box.
+void quantize_axis(const Node4 &nd, const float *lo, const float *hi, float &o_out, float &s_out,
+                   uint32_t &qlo, uint32_t &qhi) {
+  float o = INFINITY, top = -INFINITY;
+  for (int i = 0; i < 4; ++i) {
+    if (nd.child[i] == kEmptyChild) continue;
+    o = std::min(o, lo[i]);
+    top = std::max(top, hi[i]);
+  }
+  qlo = qhi = 0;
+  if (!(o <= top)) {  // no child: any grid (the slots are skipped by their id)
+    o_out = 0.0f;
+    s_out = 1.0f;
+    return;
+  }
+  const __float128 O = o;
+  const double extent = (double)((__float128)top - O);
+  int k = -126;
+  while (k < 127 && std::ldexp(255.0, k) < extent) ++k;
+  for (;; ++k) {
+    const double s = std::ldexp(1.0, k);
+    bool ok = true;
+    uint32_t ql = 0, qh = 0;
+    for (int i = 0; i < 4; ++i) {
+      if (nd.child[i] == kEmptyChild) continue;
+      long long a = (long long)std::floor((double)(((__float128)lo[i] - O) / s));
+      long long b = (long long)std::ceil((double)(((__float128)hi[i] - O) / s));
+      while (a > 0 && O + (__float128)a * s > (__float128)lo[i]) --a;
+      while (O + (__float128)b * s < (__float128)hi[i]) ++b;
+      if (a < 0 || b > 255) {
+        ok = false;
+        break;
+      }
+      ql |= (uint32_t)a << (8 * i);
+      qh |= (uint32_t)b << (8 * i);
+    }
+    if (ok) {
+      o_out = o;
+      s_out = (float)s;
+      qlo = ql;
+      qhi = qh;
+      return;
+    }
+  }
+}
+
 }  // namespace
 
 bool build_bvh4(const float *tris, uint32_t n, Bvh4 &out, std::string *msg) {
@@ -269,6 +318,11 @@ bool build_bvh4(const float *tris, uint32_t n, Bvh4 &out, std::string *msg) {
       nd.child[i] = kEmptyChild;
     }
     out.nodes.push_back(nd);
+    Node4Q q;
+    std::memset(&q, 0, sizeof(q));
+    q.sx = q.sy = q.sz = 1.0f;
+    for (int i = 0; i < 4; ++i) q.child[i] = kEmptyChild;
+    out.qnodes.assign(1, q);
     out.stats.n_nodes = 1;
     out.stats.max_depth = 1;
     return true;
@@ -341,6 +395,15 @@ bool build_bvh4(const float *tris, uint32_t n, Bvh4 &out, std::string *msg) {
     }
   }
   out.stats.sah_cost = cost + 1.0;
+  out.qnodes.resize(out.nodes.size());
+  for (size_t j = 0; j < out.nodes.size(); ++j) {
+    const Node4 &nd = out.nodes[j];
+    Node4Q &q = out.qnodes[j];
+    quantize_axis(nd, nd.lo_x, nd.hi_x, q.ox, q.sx, q.qlx, q.qhx);
+    quantize_axis(nd, nd.lo_y, nd.hi_y, q.oy, q.sy, q.qly, q.qhy);
+    quantize_axis(nd, nd.lo_z, nd.hi_z, q.oz, q.sz, q.qlz, q.qhz);
+    for (int i = 0; i < 4; ++i) q.child[i] = nd.child[i];
+  }
   return true;
 }
 

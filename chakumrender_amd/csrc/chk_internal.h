@@ -80,7 +80,7 @@ struct SphereRec {
 };
 
 struct DeviceScene {
-  const Node4 *nodes = nullptr;
+  const Node4Q *nodes = nullptr;
   const float4 *tris = nullptr;        // 3 float4 per leaf triangle
   const uint2 *leaf_info = nullptr;    // (material, light or 0xFFFFFFFF)
   const chk_material *mats = nullptr;

@@ -1,26 +1,24 @@
 // Closest-hit and any-hit BVH traversal for gfx950.
 //
 // Restates BvhAccel::intersectT (bvhAccel.cpp:200-249) and shadow()
-// (shader.cpp:76-97) over the 4-wide SoA tree of chk_bvh.cpp:
+// (shader.cpp:76-97) over the 4-wide tree of chk_bvh.cpp (64-byte nodes with
+// 8-bit quantized child boxes):
 //  * one ray per lane; the rays of a launch are split into kSegs segments
 //    and block b works on segment b % kSegs, so with round-robin dispatch
 //    each XCD (and its L2) serves one segment; no global atomics;
 //  * one ray per lane, the current node in VGPRs, pending nodes on a
 //    per-lane stack in LDS laid out [slot][lane] (bank-conflict free);
-//  * child boxes are tested with the robust slab test of Ize ("Robust BVH
-//    Ray Traversal", JCGT 2(2) 2013), children ordered by entry distance;
+//  * child boxes are tested conservatively, after Ize's robust slab test
+//    ("Robust BVH Ray Traversal", JCGT 2(2) 2013; see node_hits_q),
+//    children ordered by entry distance;
 //  * triangles use the watertight test (chk_device.h) with the reference's
 //    accept rules.
 #include "chk_device.h"
 #include "chk_internal.h"
+#include "chk_node_test.h"
 
 namespace chk {
 namespace {
-
-
-__device__ inline float safe_rcp(float x) {
-  return 1.0f / (fabsf(x) > 1e-20f ? x : copysignf(1e-20f, x));
-}
 
 // Traversal with postponed leaves (Aila and Laine, "Understanding the
 // Efficiency of Ray Traversal on GPUs", HPG 2009): a lane that reaches a leaf
@@ -48,7 +46,6 @@ __device__ inline void traverse(const DeviceScene &S, uint32_t *__restrict__ stk
   uint32_t pend = 0;   // postponed leaf (leaves carry kLeafBit, so 0 is none)
   int sp = 0;
   hit_any = false;
-  const float kGrow = 1.0f + 2.0f * 3.5762786865234375e-07f;  // 1 + 2*gamma_3
   auto pop = [&]() {
     if (sp == 0) {
       live = false;
@@ -69,26 +66,9 @@ __device__ inline void traverse(const DeviceScene &S, uint32_t *__restrict__ stk
         break;  // wave-uniform
       if (!searching) continue;
       if (COUNT) ++nv;
-      const float4 *np = reinterpret_cast<const float4 *>(S.nodes + entry);
-      const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
-      const uint4 ch = *reinterpret_cast<const uint4 *>(np + 6);
       float key[4];
-      uint32_t cid[4] = {ch.x, ch.y, ch.z, ch.w};
-      const float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, hxa[4] = {hx.x, hx.y, hx.z, hx.w};
-      const float lya[4] = {ly.x, ly.y, ly.z, ly.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w};
-      const float lza[4] = {lz.x, lz.y, lz.z, lz.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
-      int nh = 0;
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        float tx0 = (lxa[c] - o.x) * inv.x, tx1 = (hxa[c] - o.x) * inv.x;
-        float ty0 = (lya[c] - o.y) * inv.y, ty1 = (hya[c] - o.y) * inv.y;
-        float tz0 = (lza[c] - o.z) * inv.z, tz1 = (hza[c] - o.z) * inv.z;
-        float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
-        float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), t));
-        bool h = tn <= tf * kGrow;
-        key[c] = h ? tn : kInf;
-        nh += h ? 1 : 0;
-      }
+      uint32_t cid[4];
+      const int nh = node_hits_q(S.nodes, entry, o, inv, t, key, cid);
       // sort (key, cid) ascending: 5 compare-exchanges
 #define CHK_CX(a, b)                                      \
   {                                                       \
@@ -336,7 +316,6 @@ __global__ __launch_bounds__(BS) void k_trace_stream(DeviceScene S, const float4
   if (threadIdx.x == 0) cursor = lo + BS;
   __syncthreads();
   const size_t seg0 = (size_t)r * W.segcap;
-  const float kGrow = 1.0f + 2.0f * 3.5762786865234375e-07f;  // 1 + 2*gamma_3
   // this lane's ray and its traversal state
   uint32_t idx = lo + threadIdx.x;
   bool has = idx < hi;
@@ -347,7 +326,7 @@ __global__ __launch_bounds__(BS) void k_trace_stream(DeviceScene S, const float4
   int sp = 0;
   bool live = false, hit_any = false, ovfl = false;
   // the ray's scene (a batched launch: the scene of the ray's request)
-  const Node4 *nodes = S.nodes;
+  const Node4Q *nodes = S.nodes;
   const float4 *tris = S.tris;
   auto start = [&]() {
     const size_t i = seg0 + idx;
@@ -391,26 +370,9 @@ __global__ __launch_bounds__(BS) void k_trace_stream(DeviceScene S, const float4
           (__popcll(__ballot(searching && pend == 0)) < 4 && __any(pend != 0)))
         break;  // wave-uniform
       if (!searching) continue;
-      const float4 *np = reinterpret_cast<const float4 *>(nodes + entry);
-      const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
-      const uint4 ch = *reinterpret_cast<const uint4 *>(np + 6);
       float key[4];
-      uint32_t cid[4] = {ch.x, ch.y, ch.z, ch.w};
-      const float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, hxa[4] = {hx.x, hx.y, hx.z, hx.w};
-      const float lya[4] = {ly.x, ly.y, ly.z, ly.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w};
-      const float lza[4] = {lz.x, lz.y, lz.z, lz.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
-      int nh = 0;
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        float tx0 = (lxa[c] - o.x) * inv.x, tx1 = (hxa[c] - o.x) * inv.x;
-        float ty0 = (lya[c] - o.y) * inv.y, ty1 = (hya[c] - o.y) * inv.y;
-        float tz0 = (lza[c] - o.z) * inv.z, tz1 = (hza[c] - o.z) * inv.z;
-        float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
-        float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), t));
-        bool h = tn <= tf * kGrow;
-        key[c] = h ? tn : kInf;
-        nh += h ? 1 : 0;
-      }
+      uint32_t cid[4];
+      const int nh = node_hits_q(nodes, entry, o, inv, t, key, cid);
 #define CHK_CX(a, b)                                      \
   {                                                       \
     bool sw = key[b] < key[a];                            \

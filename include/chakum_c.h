@@ -62,7 +62,7 @@ typedef struct chk_hit {
 typedef struct chk_scene_info {
   uint32_t n_tris;          /* triangles passed in */
   uint32_t n_tris_bvh;      /* triangles in the BVH (zero-area ones are dropped) */
-  uint32_t n_nodes;         /* 128-byte 4-wide nodes */
+  uint32_t n_nodes;         /* 4-wide nodes (64 bytes each on the device) */
   uint32_t n_leaves;
   uint32_t max_depth;
   uint32_t stack_bound;     /* worst-case traversal stack entries */
