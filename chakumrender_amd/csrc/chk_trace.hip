@@ -290,15 +290,15 @@ __global__ __launch_bounds__(BS) void k_trace(DeviceScene S0, const float4 *__re
 // at 1,024 blocks per segment, shadow rays at 768): 2,085 -> 2,293 Mpaths/s;
 // for the coherent camera rays it is slower than k_trace (their waves hold
 // 64 samples of one pixel, which replacement scatters).
+// (The any-hit instantiation is bounded to at least 7 waves per SIMD: 71
+// VGPRs; unbounded the compiler takes 75, 6 waves per SIMD, and its launches
+// run about 5 % slower.)
 template <bool ANY, int STACK, int BS, bool MULTI>
-__global__ __launch_bounds__(BS) void k_trace_stream(DeviceScene S, const float4 *__restrict__ rays,
-                                                     SegRange W, float4 *__restrict__ hits,
-                                                     uint8_t *__restrict__ occluded,
-                                                     const float4 *__restrict__ shadow_L,
-                                                     float4 *__restrict__ path_L, int leaf_ids,
-                                                     unsigned long long *__restrict__ counters,
-                                                     SlotScenes M, uint32_t *__restrict__ ovf,
-                                                     uint32_t ovf_cap) {
+__global__ __launch_bounds__(BS, ANY ? 7 : 1) void k_trace_stream(
+    DeviceScene S, const float4 *__restrict__ rays, SegRange W, float4 *__restrict__ hits,
+    uint8_t *__restrict__ occluded, const float4 *__restrict__ shadow_L,
+    float4 *__restrict__ path_L, int leaf_ids, unsigned long long *__restrict__ counters,
+    SlotScenes M, uint32_t *__restrict__ ovf, uint32_t ovf_cap) {
   __shared__ uint32_t stack_mem[STACK * BS];
   __shared__ uint32_t cursor;
   uint32_t *stk = stack_mem + threadIdx.x;
