@@ -249,7 +249,8 @@ void quantize_axis(const Node4 &nd, const float *lo, const float *hi, float &o_o
   const double extent = (double)((__float128)top - O);
   int k = -126;
   while (k < 127 && std::ldexp(255.0, k) < extent) ++k;
-  for (;; ++k) {
+  // (finite boxes: extent < 2^129, so some k <= 121 always fits)
+  for (; k <= 127; ++k) {
     const double s = std::ldexp(1.0, k);
     bool ok = true;
     uint32_t ql = 0, qh = 0;
@@ -274,6 +275,10 @@ void quantize_axis(const Node4 &nd, const float *lo, const float *hi, float &o_o
       return;
     }
   }
+  o_out = o;  // not reached for finite boxes
+  s_out = std::ldexp(1.0f, 127);
+  qlo = 0;
+  qhi = 0xFFFFFFFFu;
 }
 
 }  // namespace
