@@ -266,6 +266,9 @@ def mlp_leg(args, ctx):
         abi.sync()
         return (time.perf_counter() - t0) / reps
 
+    t_w = time.perf_counter()  # untimed: the clock ramps up over the first 0.1 s
+    while time.perf_counter() - t_w < 0.1:
+        trial(20)
     trials = [trial() for _ in range(5)]
     dt = min(trials)
     flop = 28800.0 * k
@@ -273,7 +276,8 @@ def mlp_leg(args, ctx):
         "workload": f"{k} queries, 7 inputs -> 64-wide encoding -> 3 x 64 ReLU -> 33 outputs, "
                     "BF16 MFMA with FP32 accumulation, 32-light softmax + mixing sigmoid",
         "ms": round(dt * 1e3, 4),
-        "timing": "best of 5 trials of 10 launches (wall clock, synchronised)",
+        "timing": "best of 5 trials of 10 launches (wall clock, synchronised) after 0.1 s of "
+                  "untimed launches",
         "trials_ms": [round(t * 1e3, 4) for t in trials],
         "gqueries_per_s": round(k / dt / 1e9, 3),
         "hbm_gbs": round(160.0 * k / dt / 1e9, 1),
