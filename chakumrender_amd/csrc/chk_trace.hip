@@ -322,7 +322,7 @@ __global__ __launch_bounds__(BS, ANY ? 7 : 1) void k_trace_stream(
   f3 o = mk(0, 0, 0), d = mk(0, 0, 1), inv = mk(0, 0, 1);
   RayPre pre = ray_pre(d);
   float t = 0.0f, bu = 0.0f, bv = 0.0f;
-  uint32_t skip = 0, best = 0xFFFFFFFFu, best_leaf = 0, entry = 0, pend = 0, lost = 0;
+  uint32_t skip = 0, best = 0xFFFFFFFFu, best_leaf = 0, entry = 0, pend = 0, lost = 0, spid = 0;
   int sp = 0;
   bool live = false, hit_any = false, ovfl = false;
   // the ray's scene (a batched launch: the scene of the ray's request)
@@ -340,6 +340,9 @@ __global__ __launch_bounds__(BS, ANY ? 7 : 1) void k_trace_stream(
     d = xyz(ds);
     t = ot.w;
     skip = __float_as_uint(ds.w);
+    // an unoccluded shadow ray adds its contribution to its path: the path
+    // id is read now, so that the two loads of that add go out together
+    if (ANY && path_L) spid = reinterpret_cast<const uint32_t *>(shadow_L + i)[3];
     pre = ray_pre(d);
     inv = mk(safe_rcp(d.x), safe_rcp(d.y), safe_rcp(d.z));
     best = 0xFFFFFFFFu;
@@ -445,7 +448,17 @@ __global__ __launch_bounds__(BS, ANY ? 7 : 1) void k_trace_stream(
         }
       }
       if (ovfl && !later) ++lost;
-      if (!later) {
+      if (!later && ANY) {  // as finish_ray, with the path id already at hand
+        if (occluded) occluded[i] = hit_any ? 1 : 0;
+        if (path_L && !hit_any) {
+          const float4 c = shadow_L[i];
+          float4 L = path_L[spid];
+          L.x += c.x;
+          L.y += c.y;
+          L.z += c.z;
+          path_L[spid] = L;
+        }
+      } else if (!later) {
         const float tr = best == 0xFFFFFFFFu ? 1e30f : t;
         finish_ray<ANY, MULTI>(S, i, make_float4(o.x, o.y, o.z, 0.0f),
                                make_float4(d.x, d.y, d.z, 0.0f), tr, best, bu, bv, hit_any,
