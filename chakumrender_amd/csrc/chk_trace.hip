@@ -323,6 +323,7 @@ __global__ __launch_bounds__(BS, ANY ? 7 : 1) void k_trace_stream(
   RayPre pre = ray_pre(d);
   float t = 0.0f, bu = 0.0f, bv = 0.0f;
   uint32_t skip = 0, best = 0xFFFFFFFFu, best_leaf = 0, entry = 0, pend = 0, lost = 0, spid = 0;
+  f3 sc = mk(0, 0, 0);
   int sp = 0;
   bool live = false, hit_any = false, ovfl = false;
   // the ray's scene (a batched launch: the scene of the ray's request)
@@ -340,9 +341,14 @@ __global__ __launch_bounds__(BS, ANY ? 7 : 1) void k_trace_stream(
     d = xyz(ds);
     t = ot.w;
     skip = __float_as_uint(ds.w);
-    // an unoccluded shadow ray adds its contribution to its path: the path
-    // id is read now, so that the two loads of that add go out together
-    if (ANY && path_L) spid = reinterpret_cast<const uint32_t *>(shadow_L + i)[3];
+    // an unoccluded shadow ray adds its contribution to its path: the
+    // contribution and the path id are read now, so that the add waits for
+    // one load (the path's radiance) instead of two in a row
+    if (ANY && path_L) {
+      const float4 c = shadow_L[i];
+      sc = xyz(c);
+      spid = __float_as_uint(c.w);
+    }
     pre = ray_pre(d);
     inv = mk(safe_rcp(d.x), safe_rcp(d.y), safe_rcp(d.z));
     best = 0xFFFFFFFFu;
@@ -448,14 +454,13 @@ __global__ __launch_bounds__(BS, ANY ? 7 : 1) void k_trace_stream(
         }
       }
       if (ovfl && !later) ++lost;
-      if (!later && ANY) {  // as finish_ray, with the path id already at hand
+      if (!later && ANY) {  // as finish_ray, with the shadow record already at hand
         if (occluded) occluded[i] = hit_any ? 1 : 0;
         if (path_L && !hit_any) {
-          const float4 c = shadow_L[i];
           float4 L = path_L[spid];
-          L.x += c.x;
-          L.y += c.y;
-          L.z += c.z;
+          L.x += sc.x;
+          L.y += sc.y;
+          L.z += sc.z;
           path_L[spid] = L;
         }
       } else if (!later) {
